@@ -1,0 +1,231 @@
+"""ctypes loader for the CPU oracle (oracle/libpicoracle.so).
+
+Test infrastructure only: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg, always as the checker.  The product package never
+imports this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libpicoracle.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_ip32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lib = None
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = ctypes.CDLL(_LIB_PATH)
+    c_d, c_i32, c_i64, c_vp = ctypes.c_double, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
+    sig = {
+        "orc_jl_mod1f": (c_d, [c_d]),
+        "orc_jl_mod1i": (c_i64, [c_i64, c_i64]),
+        "orc_jl_round": (c_d, [c_d]),
+        "orc_exp": (c_d, [c_d]),
+        "orc_erf": (c_d, [c_d]),
+        "orc_tree_sum": (c_d, [_dp, c_i64]),
+        "orc_norm2": (c_d, [_dp, c_i64]),
+        "orc_isapprox": (ctypes.c_int, [_dp, _dp, c_i64, c_d, c_d]),
+        "orc_fxp_frac_bits": (ctypes.c_int, [c_i64, c_d]),
+        "orc_fxp_from_double": (None, [c_d, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+        "orc_fxp_to_double": (c_d, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int]),
+        "orc_twiddles": (None, [c_i32, _dp, _dp]),
+        "orc_fft": (None, [c_i32, _dp, _dp, c_i32]),
+        "orc_dft_bruteforce": (None, [c_i32, _dp, _dp, _dp, _dp, c_i32]),
+        "orc_fieldsolve": (None, [c_i32, _dp, _dp]),
+        "orc_ngp_cell": (c_i64, [c_d, c_i32]),
+        "orc_gauss_stencil": (None, [c_i32, c_i32, c_d, _ip64, _dp]),
+        "orc_deposit": (None, [c_i32, c_i32, c_i64, _dp, c_vp, c_d, c_i32, _dp]),
+        "orc_drift_half": (None, [c_i64, _dp, _dp, c_d]),
+        "orc_kick": (None, [c_i32, c_i32, c_i64, _dp, _dp, _dp, c_d]),
+        "orc_midpoint_position": (None, [c_i64, _dp, _dp, _dp, _dp, c_d]),
+        "orc_push_midpoint": (None, [c_i32, c_i32, c_i64, _dp, _dp, _dp, _dp, _dp, c_d]),
+        "orc_wrap": (None, [c_i64, _dp]),
+        "orc_diagnostics": (None, [c_i32, c_i64, _dp, _dp, c_d, _dp]),
+        "orc_quiet_start": (None, [c_i64, _dp, _dp]),
+        "orc_run": (ctypes.c_int, [c_i32, c_i32, c_i32, c_i64, c_d, c_d, c_d, c_d, c_i32,
+                                   _dp, _dp, _dp, c_i64, c_vp, c_vp, c_i32, c_i32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def jl_mod1f(x): return lib().orc_jl_mod1f(float(x))
+def jl_mod1i(i, n): return lib().orc_jl_mod1i(int(i), int(n))
+def jl_round(x): return lib().orc_jl_round(float(x))
+def exp(x): return lib().orc_exp(float(x))
+def erf(x): return lib().orc_erf(float(x))
+
+
+def erf_array(xs):
+    L = lib()
+    return np.array([L.orc_erf(float(v)) for v in np.asarray(xs).ravel()]).reshape(np.shape(xs))
+
+
+def exp_array(xs):
+    L = lib()
+    return np.array([L.orc_exp(float(v)) for v in np.asarray(xs).ravel()]).reshape(np.shape(xs))
+
+
+def tree_sum(x):
+    x = _f64(x)
+    return lib().orc_tree_sum(x, x.size)
+
+
+def norm2(x):
+    x = _f64(x)
+    return lib().orc_norm2(x, x.size)
+
+
+def isapprox(F, E, rtol, atol=0.0):
+    F, E = _f64(F), _f64(E)
+    return bool(lib().orc_isapprox(F, E, F.size, float(rtol), float(atol)))
+
+
+def fxp_frac_bits(P, w):
+    return lib().orc_fxp_frac_bits(int(P), float(w))
+
+
+def fxp_from_double(a, q):
+    lo, hi = ctypes.c_uint64(), ctypes.c_uint64()
+    lib().orc_fxp_from_double(float(a), int(q), ctypes.byref(lo), ctypes.byref(hi))
+    return (hi.value << 64) | lo.value
+
+
+def fxp_to_double(S, q):
+    return lib().orc_fxp_to_double(S & ((1 << 64) - 1), S >> 64, int(q))
+
+
+def twiddles(N):
+    re, im = np.zeros(N // 2), np.zeros(N // 2)
+    lib().orc_twiddles(N, re, im)
+    return re + 1j * im
+
+
+def fft(z, inverse=False):
+    z = np.asarray(z, dtype=np.complex128)
+    re, im = _f64(z.real).copy(), _f64(z.imag).copy()
+    lib().orc_fft(z.size, re, im, 1 if inverse else 0)
+    return re + 1j * im
+
+
+def dft_bruteforce(z, inverse=False):
+    z = np.asarray(z, dtype=np.complex128)
+    N = z.size
+    orr, oi = np.zeros(N), np.zeros(N)
+    lib().orc_dft_bruteforce(N, _f64(z.real).copy(), _f64(z.imag).copy(), orr, oi, 1 if inverse else 0)
+    return orr + 1j * oi
+
+
+def fieldsolve(rho):
+    rho = _f64(rho)
+    E = np.zeros_like(rho)
+    lib().orc_fieldsolve(rho.size, rho, E)
+    return E
+
+
+def ngp_cell(x, N):
+    return lib().orc_ngp_cell(float(x), int(N))
+
+
+def gauss_stencil(N, R, c):
+    cells = np.zeros(2 * R + 1, dtype=np.int64)
+    wt = np.zeros(2 * R + 1)
+    lib().orc_gauss_stencil(int(N), int(R), float(c), cells, wt)
+    return cells, wt
+
+
+def deposit(N, shape, x, y=None, w=1.0, mode=0):
+    x = _f64(x)
+    y = None if y is None else _f64(y)
+    rho = np.zeros(N)
+    lib().orc_deposit(int(N), int(shape), x.size, x, _ptr(y), float(w), int(mode), rho)
+    return rho
+
+
+def drift_half(x, v, dt):
+    x = _f64(x).copy()
+    lib().orc_drift_half(x.size, x, _f64(v), float(dt))
+    return x
+
+
+def kick(N, shape, v, x, E, dt):
+    v = _f64(v).copy()
+    lib().orc_kick(int(N), int(shape), v.size, v, _f64(x), _f64(E), float(dt))
+    return v
+
+
+def midpoint_position(X, V, v, dt):
+    X = _f64(X)
+    x = np.zeros_like(X)
+    lib().orc_midpoint_position(X.size, X, _f64(V), _f64(v), x, float(dt))
+    return x
+
+
+def push_midpoint(N, R, X, V, x, E, dt):
+    X = _f64(X)
+    v = np.zeros_like(X)
+    lib().orc_push_midpoint(int(N), int(R), X.size, X, _f64(V), _f64(x), v, _f64(E), float(dt))
+    return v
+
+
+def wrap(x):
+    x = _f64(x).copy()
+    lib().orc_wrap(x.size, x)
+    return x
+
+
+def diagnostics(E, v, W):
+    E, v = _f64(E), _f64(v)
+    out = np.zeros(4)
+    lib().orc_diagnostics(E.size, v.size, E, v, float(W), out)
+    return out
+
+
+def quiet_start(P):
+    x, v = np.zeros(P), np.zeros(P)
+    lib().orc_quiet_start(int(P), x, v)
+    return x, v
+
+
+def run(N, shape, scheme, W, w, dt, x, v, nsteps, rtol=1e-8, max_sweeps=10, E=None,
+        deposit_mode=0, nthreads=1):
+    """Run nsteps of the restated script loop.  Returns (x, v, E, diag, sweeps)."""
+    x = _f64(x).copy()
+    v = _f64(v).copy()
+    E = np.zeros(N) if E is None else _f64(E).copy()
+    diag = np.zeros((nsteps, 4))
+    sweeps = np.zeros(nsteps, dtype=np.int32)
+    rc = lib().orc_run(int(N), int(shape), int(scheme), x.size, float(W), float(w), float(dt),
+                       float(rtol), int(max_sweeps), x, v, E, int(nsteps),
+                       _ptr(diag), _ptr(sweeps), int(deposit_mode), int(nthreads))
+    if rc != 0:
+        raise ValueError(f"orc_run rejected its arguments (rc={rc})")
+    return x, v, E, diag, sweeps

@@ -1,0 +1,89 @@
+/*
+ * picoracle.h -- CPU FP64 restatement of the electrostatic PIC step of
+ * jwscook/ParticleInCellCodeGolf.jl.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle: it is loaded by the
+ * tests under tests/, by __graft_entry__.smoke() and by the cpu_baseline leg of
+ * bench.py, always as the checker and never as the thing measured or shipped.
+ * The product (libpicgolf.so) never links, loads or calls it.
+ *
+ * Parity status of each function (how it is pinned) is listed in picoracle.c.
+ */
+#ifndef PICORACLE_H
+#define PICORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Julia scalar semantics (src/NGPFourier.jl:2-3, src/GaussianFixedPointQuiet.jl:6,10) */
+double  orc_jl_mod1f(double x);                 /* Julia mod(x, 1) on Float64     */
+int64_t orc_jl_mod1i(int64_t i, int64_t n);     /* Julia mod1(i, n) on Int        */
+double  orc_jl_round(double x);                 /* Julia round(x): ties to even   */
+
+/* OpenLibm 0.8.1 (FreeBSD msun) erf / exp restatements */
+double orc_exp(double x);
+double orc_erf(double x);
+
+/* canonical reductions shared with the GPU path */
+double orc_tree_sum(const double* x, int64_t n);
+double orc_norm2(const double* x, int64_t n);
+int    orc_isapprox(const double* F, const double* E, int64_t n, double rtol, double atol);
+
+/* fixed-point exact per-cell accumulation */
+int    orc_fxp_frac_bits(int64_t P, double w);
+void   orc_fxp_from_double(double a, int q, uint64_t* lo, uint64_t* hi);
+double orc_fxp_to_double(uint64_t lo, uint64_t hi, int q);
+
+/* spectral solve (A5) */
+void orc_twiddles(int32_t N, double* re, double* im);
+void orc_fft(int32_t N, double* re, double* im, int32_t inverse);
+void orc_dft_bruteforce(int32_t N, const double* re, const double* im,
+                        double* out_re, double* out_im, int32_t inverse);
+void orc_fieldsolve(int32_t N, const double* rho, double* E);
+
+/* stencils (A2, A7) */
+int64_t orc_ngp_cell(double x, int32_t N);      /* 0-based cell of f(x)          */
+void    orc_gauss_stencil(int32_t N, int32_t R, double c, int64_t* cells, double* wt);
+
+/* deposits (A3, A8): mode 0 = exact per-cell sum (the semantics the GPU
+ * reproduces bitwise), mode 1 = the scripts' literal sequential order */
+void orc_deposit(int32_t N, int32_t shape, int64_t P, const double* x,
+                 const double* y, double w, int32_t mode, double* rho);
+
+/* particle updates (A4, A6, A10, A11) */
+void orc_drift_half(int64_t P, double* x, const double* v, double dt);
+void orc_kick(int32_t N, int32_t shape, int64_t P, double* v, const double* x,
+              const double* E, double dt);
+void orc_midpoint_position(int64_t P, const double* X, const double* V,
+                           const double* v, double* x, double dt);
+void orc_push_midpoint(int32_t N, int32_t R, int64_t P, const double* X,
+                       const double* V, const double* x, double* v,
+                       const double* E, double dt);
+void orc_wrap(int64_t P, double* x);
+
+/* diagnostics (A12) */
+void orc_diagnostics(int32_t N, int64_t P, const double* E, const double* v,
+                     double W, double* out4);
+
+/* initial conditions (A13) */
+void orc_quiet_start(int64_t P, double* x, double* v);
+
+/* whole runs.  shape: 0 = NGP, 6 / 7 = Gaussian half-width.  scheme: 0 =
+ * leapfrog (src/NGPFourier.jl, src/Gaussian.jl), 1 = implicit midpoint
+ * (src/GaussianFixedPoint.jl, src/GaussianFixedPointQuiet.jl).
+ * E (length N) carries the field between steps (the scripts start from zeros).
+ * diag: nsteps x 4 (may be NULL); sweeps: nsteps (may be NULL).
+ * deposit_mode as for orc_deposit; nthreads > 1 uses OpenMP (bitwise equal to
+ * the serial run because the per-cell sums are exact). */
+int orc_run(int32_t N, int32_t shape, int32_t scheme, int64_t P, double W,
+            double w, double dt, double rtol, int32_t max_sweeps,
+            double* x, double* v, double* E, int64_t nsteps,
+            double* diag, int32_t* sweeps, int32_t deposit_mode,
+            int32_t nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
