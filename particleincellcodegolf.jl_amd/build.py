@@ -1,0 +1,64 @@
+"""Builds libpicgolf.so (HIP kernels for gfx950 + the C ABI) in-tree.
+
+    python particleincellcodegolf.jl_amd/build.py
+
+The library lands in particleincellcodegolf.jl_amd/lib/.  All device code is
+compiled with -ffp-contract=off (the FP64 path must round exactly as the CPU
+restatement does) and with packed FP32 instructions disabled.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB_DIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIB_DIR, "libpicgolf.so")
+ARCH = os.environ.get("PIC_OFFLOAD_ARCH", "gfx950")
+
+SOURCES = ["kernels_1d.hip", "capi.cpp"]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose=False):
+    os.makedirs(LIB_DIR, exist_ok=True)
+    objs = []
+    common = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall",
+              "-Wno-unused-function", "-I", os.path.join(HERE, "..", "include")]
+    for src in SOURCES:
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(LIB_DIR, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc(), *common, "-c", path, "-o", obj]
+        # hipcc compiles .cpp as HIP too: every unit gets the same target flags
+        cmd[1:1] = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+        run(cmd, verbose)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs], verbose)
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+def run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    noise = "is not a recognized feature for this target"
+    err = "\n".join(l for l in (r.stderr or "").splitlines() if noise not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{err}")
+    if verbose and err.strip():
+        print(err, file=sys.stderr)
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,854 @@
+// capi.cpp -- host runtime behind include/pic.h: contexts, device buffers,
+// instance/tile tables, hipGraph capture of the step, and the operator calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pic.h"
+#include "pic_internal.h"
+
+using namespace pic;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct pic_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<pic_config> cfg;
+    int ninst = 0;
+    int R = 7, scheme = 1, precision = 0, max_sweeps = 10;
+    int maxN = 0;
+    int64_t Ptot = 0;
+    int numCU = 256;
+    std::vector<InstDesc> hinst;
+    std::vector<TileDesc> htiles;
+    std::vector<RedDesc> hreds;
+    std::vector<ChunkDesc> hchunks;
+    RunBuffers rb{};
+    std::vector<void*> allocs;
+    // growable scratch for the operators
+    DevBuf op_a, op_b, op_c, op_d, op_e, op_f, op_part, op_dpart, op_flag;
+    double* tw = nullptr;
+    hipGraphExec_t graph_many = nullptr;
+    hipGraphExec_t graph_one = nullptr;
+    int64_t graph_many_steps = 0;
+    int64_t steps_done = 0;
+    std::string err;
+};
+
+static thread_local std::string g_create_err;
+
+static void set_err(pic_ctx* c, const char* fmt, ...) {
+    if (!c) return;
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            set_err(ctx, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                    __LINE__);                                                              \
+            return e_ == hipErrorOutOfMemory ? PIC_ENOMEM : PIC_EHIP;                       \
+        }                                                                                   \
+    } while (0)
+
+static int ilog2i(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+
+// Fraction bits of the exact per-cell accumulator: P*w*2^q < 2^125.
+static int fxp_frac_bits(int64_t P, double w) {
+    double total = (double)P * std::fabs(w);
+    if (!(total > 0.0) || !std::isfinite(total)) return 64;
+    int e = std::ilogb(total);
+    int q = 124 - e;
+    return q > 1100 ? 1100 : q;
+}
+
+// W[m] = exp(-2 pi i m / 4096), m < 2048, from cos/sin on [0, pi/4] and exact
+// quarter-wave symmetries; (re, im) pairs.
+static void twiddles4096(std::vector<double>& out) {
+    const int N = kMaxN, h = N / 2, q = N / 4, e = N / 8;
+    const double twopi = 6.283185307179586;
+    out.assign(2 * h, 0.0);
+    for (int m = 0; m < h; ++m) {
+        double c, s;
+        if (m <= e) {
+            double th = twopi * (double)m / (double)N;
+            c = std::cos(th);
+            s = std::sin(th);
+        } else if (m <= q) {
+            double th = twopi * (double)(q - m) / (double)N;
+            c = std::sin(th);
+            s = std::cos(th);
+        } else {
+            int mm = h - m;
+            if (mm <= e) {
+                double th = twopi * (double)mm / (double)N;
+                c = -std::cos(th);
+                s = std::sin(th);
+            } else {
+                double th = twopi * (double)(q - mm) / (double)N;
+                c = -std::sin(th);
+                s = std::cos(th);
+            }
+        }
+        out[2 * m] = c;
+        out[2 * m + 1] = -s;
+    }
+}
+
+template <typename T>
+static int dalloc(pic_ctx* c, T** p, size_t count) {
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    void* q = nullptr;
+    HIPCHK(c, hipMalloc(&q, bytes));
+    c->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return PIC_OK;
+}
+
+static int grow(pic_ctx* c, DevBuf& b, size_t bytes) {
+    if (b.bytes >= bytes && b.p) return PIC_OK;
+    if (b.p) {
+        HIPCHK(c, hipFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    HIPCHK(c, hipMalloc(&b.p, std::max<size_t>(bytes, 64)));
+    b.bytes = std::max<size_t>(bytes, 64);
+    return PIC_OK;
+}
+
+static void drop_graphs(pic_ctx* c) {
+    if (c->graph_many) (void)hipGraphExecDestroy(c->graph_many);
+    if (c->graph_one) (void)hipGraphExecDestroy(c->graph_one);
+    c->graph_many = c->graph_one = nullptr;
+    c->graph_many_steps = 0;
+}
+
+static const char* validate(const pic_config& k) {
+    if (k.N < 8 || k.N > kMaxN || (k.N & (k.N - 1)) != 0) return "N must be a power of two in [8, 4096]";
+    if (k.P < 1 || k.P > (int64_t)1 << 31) return "P must be in [1, 2^31]";
+    if (k.shape != PIC_SHAPE_NGP && k.shape != PIC_SHAPE_GAUSS6 && k.shape != PIC_SHAPE_GAUSS7)
+        return "shape must be PIC_SHAPE_NGP, PIC_SHAPE_GAUSS6 or PIC_SHAPE_GAUSS7";
+    if (k.scheme != PIC_SCHEME_LEAPFROG && k.scheme != PIC_SCHEME_MIDPOINT) return "unknown scheme";
+    if (k.scheme == PIC_SCHEME_MIDPOINT && k.shape == PIC_SHAPE_NGP)
+        return "the midpoint scheme needs a Gaussian shape (as in the scripts)";
+    if (k.precision != PIC_PREC_FP64 && k.precision != PIC_PREC_MIXED) return "unknown precision";
+    if (k.precision == PIC_PREC_MIXED && k.scheme != PIC_SCHEME_MIDPOINT)
+        return "mixed precision is provided for the midpoint scheme";
+    if (!(k.w >= 0.0) || !std::isfinite(k.w)) return "w must be finite and >= 0";
+    if (!std::isfinite(k.dt) || !std::isfinite(k.W) || !(k.W > 0.0)) return "dt must be finite, W finite and > 0";
+    if (k.scheme == PIC_SCHEME_MIDPOINT && (k.max_sweeps < 1 || k.max_sweeps > 64))
+        return "max_sweeps must be in [1, 64]";
+    if (!(k.rtol >= 0.0)) return "rtol must be >= 0";
+    if (k.precision == PIC_PREC_MIXED && !mixed_available()) return "mixed precision kernels are not built";
+    return nullptr;
+}
+
+extern "C" {
+
+int pic_version(void) { return 1; }
+
+const char* pic_strerror(int code) {
+    switch (code) {
+        case PIC_OK: return "ok";
+        case PIC_EINVAL: return "invalid argument";
+        case PIC_EHIP: return "HIP runtime error";
+        case PIC_ENOMEM: return "out of device memory";
+        case PIC_ENONFINITE: return "non-finite charge or field";
+        case PIC_ESTATE: return "call out of order";
+        default: return "unknown error";
+    }
+}
+
+// pic_last_error(NULL) reports why the last pic_create / pic_create_batch on
+// this thread failed.
+const char* pic_last_error(const pic_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+static int build_tables(pic_ctx* c) {
+    const int n = c->ninst;
+    c->hinst.assign(n, InstDesc{});
+    c->htiles.clear();
+    c->hreds.clear();
+    c->hchunks.clear();
+    // tile size: every tile of every instance holds at most tileP particles,
+    // so the launch is balanced; enough tiles to cover the chip
+    const int lds_push = push_lds_bytes(c->maxN, c->precision);
+    int wg_per_cu = std::max(1, std::min(4, (160 * 1024) / std::max(lds_push, 1)));
+    const int64_t target = (int64_t)c->numCU * wg_per_cu * (c->precision == 0 ? 1 : 2);
+    int64_t tileP = (c->Ptot + target - 1) / target;
+    tileP = std::max<int64_t>(64, ((tileP + 63) / 64) * 64);
+    tileP = std::min<int64_t>(tileP, (int64_t)1 << 20);
+    int block = 1024;
+    if (c->precision == 0 && wg_per_cu >= 2) block = 512;
+    while (block > 64 && block / 2 >= tileP) block /= 2;
+    c->rb.block_push = block;
+    c->rb.lds_push = lds_push;
+    int64_t poff = 0, part0 = 0;
+    int32_t goff = 0;
+    for (int i = 0; i < n; ++i) {
+        const pic_config& k = c->cfg[i];
+        InstDesc& d = c->hinst[i];
+        d.N = k.N;
+        d.log2N = ilog2i(k.N);
+        d.nmask = k.N - 1;
+        d.R = k.shape;
+        d.P = k.P;
+        d.poff = poff;
+        d.goff = goff;
+        d.part0 = part0;
+        d.q = fxp_frac_bits(k.P, k.w);
+        d.max_sweeps = k.max_sweeps;
+        d.w = k.w;
+        d.dt = k.dt;
+        d.W = k.W;
+        d.rtol = k.rtol;
+        d.wq = k.w * 0x1p-30;
+        d.tile0 = (int32_t)c->htiles.size();
+        const int64_t nt = (k.P + tileP - 1) / tileP;
+        d.ntiles = (int32_t)nt;
+        for (int64_t t = 0; t < nt; ++t) {
+            TileDesc T{};
+            T.inst = i;
+            T.local = (int32_t)t;
+            T.start = t * tileP;
+            T.count = std::min<int64_t>(tileP, k.P - t * tileP);
+            T.part_off = part0 + t * k.N;
+            c->htiles.push_back(T);
+        }
+        part0 += nt * k.N;
+        for (int c0 = 0; c0 < k.N; c0 += 256) c->hreds.push_back(RedDesc{i, c0});
+        d.chunk0 = (int32_t)c->hchunks.size();
+        const int64_t nch = (k.P + kTreeChunk - 1) / kTreeChunk;
+        d.nchunks = (int32_t)nch;
+        for (int64_t ch = 0; ch < nch; ++ch) {
+            ChunkDesc C{};
+            C.inst = i;
+            C.start = ch * kTreeChunk;
+            C.count = std::min<int64_t>(kTreeChunk, k.P - ch * kTreeChunk);
+            c->hchunks.push_back(C);
+        }
+        poff += k.P;
+        goff += k.N;
+    }
+    RunBuffers& b = c->rb;
+    b.ninst = n;
+    b.ntiles = (int32_t)c->htiles.size();
+    b.nreds = (int32_t)c->hreds.size();
+    b.nchunks = (int32_t)c->hchunks.size();
+    b.maxN = c->maxN;
+    b.R = c->R;
+    b.scheme = c->scheme;
+    b.precision = c->precision;
+    b.max_sweeps = c->max_sweeps;
+    b.block_solve = c->maxN >= 2048 ? 1024 : 256;
+    int rc;
+    if ((rc = dalloc(c, &b.insts, n))) return rc;
+    if ((rc = dalloc(c, &b.st, n))) return rc;
+    if ((rc = dalloc(c, &b.tiles, b.ntiles))) return rc;
+    if ((rc = dalloc(c, &b.reds, b.nreds))) return rc;
+    if ((rc = dalloc(c, &b.chunks, b.nchunks))) return rc;
+    HIPCHK(c, hipMemcpy(b.insts, c->hinst.data(), sizeof(InstDesc) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b.tiles, c->htiles.data(), sizeof(TileDesc) * b.ntiles, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b.reds, c->hreds.data(), sizeof(RedDesc) * b.nreds, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b.chunks, c->hchunks.data(), sizeof(ChunkDesc) * b.nchunks, hipMemcpyHostToDevice));
+    std::vector<InstState> st(n);
+    for (auto& s : st) {
+        std::memset(&s, 0, sizeof s);
+        s.last_sweep = -1;
+    }
+    HIPCHK(c, hipMemcpy(b.st, st.data(), sizeof(InstState) * n, hipMemcpyHostToDevice));
+    const size_t P = (size_t)c->Ptot, G = (size_t)goff;
+    if (c->precision == PIC_PREC_FP64) {
+        for (int j = 0; j < 2; ++j) {
+            if ((rc = dalloc(c, &b.X[j], P))) return rc;
+            if ((rc = dalloc(c, &b.V[j], P))) return rc;
+            if (c->scheme == PIC_SCHEME_MIDPOINT && (rc = dalloc(c, &b.vit[j], P))) return rc;
+        }
+        if ((rc = dalloc(c, reinterpret_cast<unsigned __int128**>(&b.part), (size_t)part0))) return rc;
+    } else {
+        for (int j = 0; j < 2; ++j) {
+            if ((rc = dalloc(c, &b.Xq[j], P))) return rc;
+            if ((rc = dalloc(c, &b.Vh[j], P))) return rc;
+            if ((rc = dalloc(c, &b.Vl[j], P))) return rc;
+            if ((rc = dalloc(c, &b.vf[j], P))) return rc;
+        }
+        if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.part), (size_t)part0))) return rc;
+    }
+    if ((rc = dalloc(c, &b.rho, G))) return rc;
+    if ((rc = dalloc(c, &b.E[0], G))) return rc;
+    if ((rc = dalloc(c, &b.E[1], G))) return rc;
+    if ((rc = dalloc(c, &b.E32, G))) return rc;
+    if ((rc = dalloc(c, &b.dpart, 2 * (size_t)b.nchunks))) return rc;
+    HIPCHK(c, hipMemset(b.E[0], 0, sizeof(double) * G));
+    HIPCHK(c, hipMemset(b.E[1], 0, sizeof(double) * G));
+    HIPCHK(c, hipMemset(b.E32, 0, sizeof(float) * G));
+    b.tw = c->tw;
+    b.diag = nullptr;
+    b.sweeps = nullptr;
+    b.diag_cap = 0;
+    return PIC_OK;
+}
+
+int pic_create_batch(pic_ctx** out, const pic_config* cfgs, int32_t n) {
+    if (!out || !cfgs || n < 1) return PIC_EINVAL;
+    *out = nullptr;
+    pic_ctx* c = new pic_ctx();
+    c->cfg.assign(cfgs, cfgs + n);
+    c->ninst = n;
+    for (int i = 0; i < n; ++i) {
+        const char* why = validate(cfgs[i]);
+        if (!why && (cfgs[i].shape != cfgs[0].shape || cfgs[i].scheme != cfgs[0].scheme ||
+                     cfgs[i].precision != cfgs[0].precision || cfgs[i].max_sweeps != cfgs[0].max_sweeps ||
+                     cfgs[i].device != cfgs[0].device))
+            why = "all instances of a batch share shape, scheme, precision, max_sweeps and device";
+        if (why) {
+            g_create_err = std::string("instance ") + std::to_string(i) + ": " + why;
+            delete c;
+            return PIC_EINVAL;
+        }
+        c->maxN = std::max(c->maxN, cfgs[i].N);
+        c->Ptot += cfgs[i].P;
+    }
+    c->R = cfgs[0].shape;
+    c->scheme = cfgs[0].scheme;
+    c->precision = cfgs[0].precision;
+    c->max_sweeps = cfgs[0].scheme == PIC_SCHEME_MIDPOINT ? cfgs[0].max_sweeps : 1;
+    c->device = cfgs[0].device;
+    int rc = PIC_OK;
+    auto fail = [&](int code) {
+        g_create_err = c->err.empty() ? pic_strerror(code) : c->err;
+        pic_destroy(c);
+        return code;
+    };
+    if (hipSetDevice(c->device) != hipSuccess) return fail(PIC_EHIP);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) return fail(PIC_EHIP);
+    c->numCU = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    if (configure_kernels() != 0) {
+        set_err(c, "could not raise the dynamic LDS limit of the kernels");
+        return fail(PIC_EHIP);
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(PIC_EHIP);
+    std::vector<double> tw;
+    twiddles4096(tw);
+    if ((rc = dalloc(c, &c->tw, tw.size()))) return fail(rc);
+    if (hipMemcpy(c->tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(PIC_EHIP);
+    if ((rc = build_tables(c))) return fail(rc);
+    *out = c;
+    return PIC_OK;
+}
+
+int pic_create(pic_ctx** out, const pic_config* cfg) { return pic_create_batch(out, cfg, 1); }
+
+int pic_destroy(pic_ctx* c) {
+    if (!c) return PIC_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
+    for (void* p : c->allocs) (void)hipFree(p);
+    for (DevBuf* b : {&c->op_a, &c->op_b, &c->op_c, &c->op_d, &c->op_e, &c->op_f, &c->op_part,
+                      &c->op_dpart, &c->op_flag})
+        if (b->p) (void)hipFree(b->p);
+    if (c->rb.diag) (void)hipFree(c->rb.diag);
+    if (c->rb.sweeps) (void)hipFree(c->rb.sweeps);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PIC_OK;
+}
+
+static int read_states(pic_ctx* c, std::vector<InstState>& st) {
+    st.resize(c->ninst);
+    HIPCHK(c, hipMemcpyAsync(st.data(), c->rb.st, sizeof(InstState) * c->ninst, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+static int write_states(pic_ctx* c, const std::vector<InstState>& st) {
+    HIPCHK(c, hipMemcpyAsync(c->rb.st, st.data(), sizeof(InstState) * c->ninst, hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_upload(pic_ctx* c, int32_t inst, const double* x, const double* v, const double* E) {
+    if (!c) return PIC_EINVAL;
+    if (inst < 0 || inst >= c->ninst || !x || !v) {
+        set_err(c, "pic_upload: bad instance or null array");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    const InstDesc& d = c->hinst[inst];
+    const int par = (int)(c->steps_done & 1);
+    const size_t P = (size_t)d.P;
+    if (c->precision == PIC_PREC_FP64) {
+        HIPCHK(c, hipMemcpyAsync(c->rb.X[par] + d.poff, x, P * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->rb.V[par] + d.poff, v, P * 8, hipMemcpyHostToDevice, c->stream));
+    } else {
+        std::vector<uint32_t> xq(P);
+        std::vector<float> vh(P), vl(P);
+        for (size_t j = 0; j < P; ++j) {
+            double f = x[j] - std::floor(x[j]);
+            double s = std::nearbyint(f * 4294967296.0);
+            xq[j] = (uint32_t)(uint64_t)(int64_t)s;  // 2^32 wraps to 0
+            vh[j] = (float)v[j];
+            vl[j] = (float)(v[j] - (double)vh[j]);
+        }
+        HIPCHK(c, hipMemcpyAsync(c->rb.Xq[par] + d.poff, xq.data(), P * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->rb.Vh[par] + d.poff, vh.data(), P * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->rb.Vl[par] + d.poff, vl.data(), P * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    std::vector<double> e((size_t)d.N, 0.0);
+    if (E) std::memcpy(e.data(), E, sizeof(double) * d.N);
+    std::vector<float> e32((size_t)d.N);
+    for (int i = 0; i < d.N; ++i) e32[i] = (float)e[i];
+    HIPCHK(c, hipMemcpyAsync(c->rb.E[0] + d.goff, e.data(), sizeof(double) * d.N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->rb.E32 + d.goff, e32.data(), sizeof(float) * d.N, hipMemcpyHostToDevice, c->stream));
+    std::vector<InstState> st;
+    int rc = read_states(c, st);
+    if (rc) return rc;
+    st[inst].e_cur = 0;
+    st[inst].err = 0;
+    return write_states(c, st);
+}
+
+int pic_download(pic_ctx* c, int32_t inst, double* x, double* v, double* E) {
+    if (!c) return PIC_EINVAL;
+    if (inst < 0 || inst >= c->ninst) {
+        set_err(c, "pic_download: bad instance");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    const InstDesc& d = c->hinst[inst];
+    const int par = (int)(c->steps_done & 1);
+    const size_t P = (size_t)d.P;
+    if (c->precision == PIC_PREC_FP64) {
+        if (x) HIPCHK(c, hipMemcpyAsync(x, c->rb.X[par] + d.poff, P * 8, hipMemcpyDeviceToHost, c->stream));
+        if (v) HIPCHK(c, hipMemcpyAsync(v, c->rb.V[par] + d.poff, P * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<uint32_t> xq(P);
+        std::vector<float> vh(P), vl(P);
+        HIPCHK(c, hipMemcpyAsync(xq.data(), c->rb.Xq[par] + d.poff, P * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(vh.data(), c->rb.Vh[par] + d.poff, P * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(vl.data(), c->rb.Vl[par] + d.poff, P * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (size_t j = 0; j < P; ++j) {
+            if (x) x[j] = (double)xq[j] * 0x1p-32;
+            if (v) v[j] = (double)vh[j] + (double)vl[j];
+        }
+    }
+    if (E) {
+        std::vector<InstState> st;
+        int rc = read_states(c, st);
+        if (rc) return rc;
+        const double* src = c->rb.E[st[inst].e_cur] + d.goff;
+        HIPCHK(c, hipMemcpy(E, src, sizeof(double) * d.N, hipMemcpyDeviceToHost));
+    }
+    return PIC_OK;
+}
+
+// ---- run ------------------------------------------------------------------
+
+static void enqueue_step(pic_ctx* c, hipStream_t s) {
+    const RunBuffers& b = c->rb;
+    if (c->scheme == PIC_SCHEME_MIDPOINT) {
+        launch_mid_push0(b, s);
+        for (int k = 0; k < c->max_sweeps; ++k) {
+            launch_reduce(b, k, s);
+            launch_solve(b, k, s);
+            launch_mid_push(b, k, s);
+        }
+    } else {
+        launch_lf_first(b, s);
+        launch_reduce(b, 0, s);
+        launch_solve(b, 0, s);
+        launch_lf_second(b, s);
+    }
+    launch_diag(b, s);
+}
+
+static int capture(pic_ctx* c, int64_t steps, hipGraphExec_t* out) {
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int64_t i = 0; i < steps; ++i) enqueue_step(c, c->stream);
+    HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        set_err(c, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        return PIC_EHIP;
+    }
+    return PIC_OK;
+}
+
+static int ensure_diag(pic_ctx* c, int64_t nsteps) {
+    RunBuffers& b = c->rb;
+    if (b.diag_cap >= nsteps && b.diag) return PIC_OK;
+    drop_graphs(c);
+    if (b.diag) (void)hipFree(b.diag);
+    if (b.sweeps) (void)hipFree(b.sweeps);
+    b.diag = nullptr;
+    b.sweeps = nullptr;
+    const int64_t cap = std::max<int64_t>(nsteps, 64);
+    HIPCHK(c, hipMalloc(&b.diag, sizeof(double) * 4 * cap * c->ninst));
+    HIPCHK(c, hipMalloc(&b.sweeps, sizeof(int32_t) * cap * c->ninst));
+    b.diag_cap = cap;
+    return PIC_OK;
+}
+
+static int begin_run(pic_ctx* c, int64_t nsteps) {
+    int rc = ensure_diag(c, nsteps);
+    if (rc) return rc;
+    std::vector<InstState> st;
+    if ((rc = read_states(c, st))) return rc;
+    for (auto& s : st) s.run_base = s.step;
+    return write_states(c, st);
+}
+
+static int finish_run(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipGetLastError());
+    c->steps_done += nsteps;
+    std::vector<InstState> st;
+    int rc = read_states(c, st);
+    if (rc) return rc;
+    const int64_t cap = c->rb.diag_cap;
+    for (int i = 0; i < c->ninst; ++i) {
+        if (diag)
+            HIPCHK(c, hipMemcpy(diag + (size_t)i * nsteps * 4, c->rb.diag + (size_t)i * cap * 4,
+                                sizeof(double) * 4 * nsteps, hipMemcpyDeviceToHost));
+        if (sweeps)
+            HIPCHK(c, hipMemcpy(sweeps + (size_t)i * nsteps, c->rb.sweeps + (size_t)i * cap,
+                                sizeof(int32_t) * nsteps, hipMemcpyDeviceToHost));
+    }
+    for (int i = 0; i < c->ninst; ++i)
+        if (st[i].err & 1) {
+            set_err(c, "instance %d: non-finite charge or field", i);
+            return PIC_ENONFINITE;
+        }
+    return PIC_OK;
+}
+
+int pic_run(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps) {
+    if (!c) return PIC_EINVAL;
+    if (nsteps < 0) {
+        set_err(c, "pic_run: nsteps < 0");
+        return PIC_EINVAL;
+    }
+    if (nsteps == 0) return PIC_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = begin_run(c, nsteps);
+    if (rc) return rc;
+    const int64_t G = 8;
+    if (!c->graph_one && (rc = capture(c, 1, &c->graph_one))) return rc;
+    if (nsteps >= G && !c->graph_many) {
+        if ((rc = capture(c, G, &c->graph_many))) return rc;
+        c->graph_many_steps = G;
+    }
+    int64_t left = nsteps;
+    while (c->graph_many && left >= c->graph_many_steps) {
+        HIPCHK(c, hipGraphLaunch(c->graph_many, c->stream));
+        left -= c->graph_many_steps;
+    }
+    while (left > 0) {
+        HIPCHK(c, hipGraphLaunch(c->graph_one, c->stream));
+        --left;
+    }
+    return finish_run(c, nsteps, diag, sweeps);
+}
+
+int pic_run_profiled(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps, pic_kernel_stat* stats,
+                     int32_t* n_stats) {
+    if (!c) return PIC_EINVAL;
+    if (nsteps <= 0) {
+        set_err(c, "pic_run_profiled: nsteps must be > 0");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = begin_run(c, nsteps);
+    if (rc) return rc;
+    enum { K_PUSH0, K_PUSH, K_REDUCE, K_SOLVE, K_LF1, K_LF2, K_DIAG, K_N };
+    const char* names[K_N] = {"push_first", "push_sweep", "reduce", "solve", "leapfrog_first",
+                              "leapfrog_second", "diagnostics"};
+    double total[K_N] = {0};
+    int64_t count[K_N] = {0};
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    auto timed = [&](int kind, auto&& launch) -> int {
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+        launch();
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+        total[kind] += ms;
+        count[kind] += 1;
+        return PIC_OK;
+    };
+    const RunBuffers& b = c->rb;
+    std::vector<InstState> st;
+    for (int64_t t = 0; t < nsteps && rc == PIC_OK; ++t) {
+        if (c->scheme == PIC_SCHEME_MIDPOINT) {
+            if ((rc = timed(K_PUSH0, [&] { launch_mid_push0(b, c->stream); }))) break;
+            for (int k = 0; k < c->max_sweeps; ++k) {
+                if ((rc = timed(K_REDUCE, [&] { launch_reduce(b, k, c->stream); }))) break;
+                if ((rc = timed(K_SOLVE, [&] { launch_solve(b, k, c->stream); }))) break;
+                if ((rc = timed(K_PUSH, [&] { launch_mid_push(b, k, c->stream); }))) break;
+                if ((rc = read_states(c, st))) break;
+                bool all_done = true;
+                for (auto& s : st) all_done = all_done && s.last_sweep >= 0 && s.last_sweep <= k;
+                if (all_done) break;
+            }
+        } else {
+            if ((rc = timed(K_LF1, [&] { launch_lf_first(b, c->stream); }))) break;
+            if ((rc = timed(K_REDUCE, [&] { launch_reduce(b, 0, c->stream); }))) break;
+            if ((rc = timed(K_SOLVE, [&] { launch_solve(b, 0, c->stream); }))) break;
+            if ((rc = timed(K_LF2, [&] { launch_lf_second(b, c->stream); }))) break;
+        }
+        if (rc) break;
+        rc = timed(K_DIAG, [&] { launch_diag(b, c->stream); });
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (stats && n_stats) {
+        int m = 0;
+        for (int k = 0; k < K_N && m < *n_stats; ++k) {
+            if (!count[k]) continue;
+            std::memset(&stats[m], 0, sizeof(pic_kernel_stat));
+            std::snprintf(stats[m].name, sizeof stats[m].name, "%s", names[k]);
+            stats[m].launches = count[k];
+            stats[m].total_ms = total[k];
+            ++m;
+        }
+        *n_stats = m;
+    }
+    return finish_run(c, nsteps, diag, sweeps);
+}
+
+// ---- operators ------------------------------------------------------------
+
+static OpBuffers op_view(pic_ctx* c) {
+    OpBuffers o{};
+    o.N = c->cfg[0].N;
+    o.log2N = ilog2i(o.N);
+    o.tw = c->tw;
+    o.part = c->op_part.p;
+    o.dpart = (double*)c->op_dpart.p;
+    o.flag = (int32_t*)c->op_flag.p;
+    return o;
+}
+
+static int h2d(pic_ctx* c, DevBuf& b, const double* src, int64_t n) {
+    int rc = grow(c, b, sizeof(double) * (size_t)n);
+    if (rc) return rc;
+    if (src) HIPCHK(c, hipMemcpyAsync(b.p, src, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    return PIC_OK;
+}
+
+static int d2h(pic_ctx* c, double* dst, const DevBuf& b, int64_t n) {
+    HIPCHK(c, hipMemcpyAsync(dst, b.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipGetLastError());
+    return PIC_OK;
+}
+
+static int check_shape(pic_ctx* c, int32_t shape, bool allow_ngp) {
+    if (shape == PIC_SHAPE_GAUSS6 || shape == PIC_SHAPE_GAUSS7 || (allow_ngp && shape == PIC_SHAPE_NGP))
+        return PIC_OK;
+    set_err(c, "unsupported shape %d", shape);
+    return PIC_EINVAL;
+}
+
+int pic_deposit(pic_ctx* c, const double* x, const double* y, int64_t P, double w, int32_t shape,
+                double* rho) {
+    if (!c) return PIC_EINVAL;
+    if (!x || !rho || P < 1 || !(w >= 0.0) || !std::isfinite(w)) {
+        set_err(c, "pic_deposit: null array, P < 1 or bad w");
+        return PIC_EINVAL;
+    }
+    int rc = check_shape(c, shape, true);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int N = c->cfg[0].N;
+    if ((rc = h2d(c, c->op_a, x, P))) return rc;
+    if (y && (rc = h2d(c, c->op_b, y, P))) return rc;
+    if ((rc = h2d(c, c->op_c, nullptr, N))) return rc;
+    if ((rc = grow(c, c->op_part, (size_t)op_tiles(P) * N * 16))) return rc;
+    OpBuffers o = op_view(c);
+    launch_op_deposit(o, (const double*)c->op_a.p, y ? (const double*)c->op_b.p : nullptr, P, w, shape,
+                      fxp_frac_bits(P, w), (double*)c->op_c.p, c->stream);
+    return d2h(c, rho, c->op_c, N);
+}
+
+int pic_fieldsolve(pic_ctx* c, const double* rho, double* E) {
+    if (!c) return PIC_EINVAL;
+    if (!rho || !E) {
+        set_err(c, "pic_fieldsolve: null array");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    const int N = c->cfg[0].N;
+    int rc;
+    if ((rc = h2d(c, c->op_a, rho, N))) return rc;
+    if ((rc = h2d(c, c->op_c, nullptr, N))) return rc;
+    OpBuffers o = op_view(c);
+    launch_op_fieldsolve(o, (const double*)c->op_a.p, (double*)c->op_c.p, c->stream);
+    return d2h(c, E, c->op_c, N);
+}
+
+int pic_drift_half(pic_ctx* c, double* x, const double* v, int64_t P, double dt) {
+    if (!c) return PIC_EINVAL;
+    if (!x || !v || P < 1) {
+        set_err(c, "pic_drift_half: null array or P < 1");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = h2d(c, c->op_a, x, P))) return rc;
+    if ((rc = h2d(c, c->op_b, v, P))) return rc;
+    launch_op_drift_half((double*)c->op_a.p, (const double*)c->op_b.p, P, dt, c->stream);
+    return d2h(c, x, c->op_a, P);
+}
+
+int pic_push_leapfrog(pic_ctx* c, double* x, double* v, const double* E, int64_t P, double dt,
+                      int32_t shape) {
+    if (!c) return PIC_EINVAL;
+    if (!x || !v || !E || P < 1) {
+        set_err(c, "pic_push_leapfrog: null array or P < 1");
+        return PIC_EINVAL;
+    }
+    int rc = check_shape(c, shape, true);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int N = c->cfg[0].N;
+    if ((rc = h2d(c, c->op_a, x, P))) return rc;
+    if ((rc = h2d(c, c->op_b, v, P))) return rc;
+    if ((rc = h2d(c, c->op_c, E, N))) return rc;
+    OpBuffers o = op_view(c);
+    launch_op_push_leapfrog(o, (double*)c->op_a.p, (double*)c->op_b.p, (const double*)c->op_c.p, P, dt, shape,
+                            c->stream);
+    if ((rc = d2h(c, x, c->op_a, P))) return rc;
+    return d2h(c, v, c->op_b, P);
+}
+
+int pic_midpoint_position(pic_ctx* c, const double* X, const double* V, const double* v, double* x,
+                          int64_t P, double dt) {
+    if (!c) return PIC_EINVAL;
+    if (!X || !V || !v || !x || P < 1) {
+        set_err(c, "pic_midpoint_position: null array or P < 1");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = h2d(c, c->op_a, X, P))) return rc;
+    if ((rc = h2d(c, c->op_b, V, P))) return rc;
+    if ((rc = h2d(c, c->op_d, v, P))) return rc;
+    if ((rc = h2d(c, c->op_e, nullptr, P))) return rc;
+    launch_op_midpos((const double*)c->op_a.p, (const double*)c->op_b.p, (const double*)c->op_d.p,
+                     (double*)c->op_e.p, P, dt, c->stream);
+    return d2h(c, x, c->op_e, P);
+}
+
+int pic_push_midpoint(pic_ctx* c, const double* X, const double* V, const double* x, double* v,
+                      const double* E, int64_t P, double dt, int32_t shape) {
+    if (!c) return PIC_EINVAL;
+    if (!X || !V || !x || !v || !E || P < 1) {
+        set_err(c, "pic_push_midpoint: null array or P < 1");
+        return PIC_EINVAL;
+    }
+    int rc = check_shape(c, shape, false);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int N = c->cfg[0].N;
+    if ((rc = h2d(c, c->op_a, X, P))) return rc;
+    if ((rc = h2d(c, c->op_b, V, P))) return rc;
+    if ((rc = h2d(c, c->op_d, x, P))) return rc;
+    if ((rc = h2d(c, c->op_e, nullptr, P))) return rc;
+    if ((rc = h2d(c, c->op_c, E, N))) return rc;
+    OpBuffers o = op_view(c);
+    launch_op_push_midpoint(o, (const double*)c->op_a.p, (const double*)c->op_b.p, (const double*)c->op_d.p,
+                            (double*)c->op_e.p, (const double*)c->op_c.p, P, dt, shape, c->stream);
+    return d2h(c, v, c->op_e, P);
+}
+
+int pic_wrap(pic_ctx* c, double* x, int64_t P) {
+    if (!c) return PIC_EINVAL;
+    if (!x || P < 1) {
+        set_err(c, "pic_wrap: null array or P < 1");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = h2d(c, c->op_a, x, P))) return rc;
+    launch_op_wrap((double*)c->op_a.p, P, c->stream);
+    return d2h(c, x, c->op_a, P);
+}
+
+int pic_isapprox(pic_ctx* c, const double* F, const double* E, int32_t n, double rtol, double atol,
+                 int32_t* result) {
+    if (!c) return PIC_EINVAL;
+    if (!F || !E || !result || n < 1 || n > kTreeChunk) {
+        set_err(c, "pic_isapprox: null array or n outside [1, 4096]");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = h2d(c, c->op_a, F, n))) return rc;
+    if ((rc = h2d(c, c->op_b, E, n))) return rc;
+    if ((rc = grow(c, c->op_flag, 64))) return rc;
+    launch_op_isapprox((const double*)c->op_a.p, (const double*)c->op_b.p, n, rtol, atol,
+                       (int32_t*)c->op_flag.p, c->stream);
+    HIPCHK(c, hipMemcpyAsync(result, c->op_flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_diagnostics(pic_ctx* c, const double* E, const double* v, int64_t P, double W, double* out4) {
+    if (!c) return PIC_EINVAL;
+    if (!E || !v || !out4 || P < 1) {
+        set_err(c, "pic_diagnostics: null array or P < 1");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    const int N = c->cfg[0].N;
+    int rc;
+    if ((rc = h2d(c, c->op_a, E, N))) return rc;
+    if ((rc = h2d(c, c->op_b, v, P))) return rc;
+    if ((rc = h2d(c, c->op_c, nullptr, 4))) return rc;
+    if ((rc = grow(c, c->op_dpart, sizeof(double) * 2 * (size_t)((P + kTreeChunk - 1) / kTreeChunk)))) return rc;
+    OpBuffers o = op_view(c);
+    launch_op_diagnostics(o, (const double*)c->op_a.p, (const double*)c->op_b.p, P, W, (double*)c->op_c.p,
+                          c->stream);
+    return d2h(c, out4, c->op_c, 4);
+}
+
+}  // extern "C"

@@ -1,0 +1,800 @@
+// kernels_1d.hip -- FP64 gfx950 kernels of the 1D1V electrostatic PIC step
+// (src/NGPFourier.jl, src/Gaussian.jl, src/GaussianFixedPoint.jl,
+// src/GaussianFixedPointQuiet.jl).  Compiled with -ffp-contract=off: every
+// operation rounds as the CPU restatement does, so results are bitwise equal.
+//
+// Data layout in HBM: structure-of-arrays, one FP64 array per particle
+// quantity (X, V, Picard iterate v), instances packed back to back; grids
+// (rho, E) packed the same way.  Per sweep one push kernel streams the
+// particles once: it gathers the field at the sweep's midpoint, updates v,
+// and deposits the charge of the next midpoint into an LDS grid of exact
+// integer accumulators (no float atomics, so results never depend on
+// scheduling).  Per-tile partials are summed per cell by the reduce kernel;
+// one workgroup per instance runs the FFT pair and the convergence test in
+// LDS.
+#include "pic_device.h"
+#include "pic_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace pic {
+
+// ---------------------------------------------------------------------------
+// Stencil helpers (R = 6 or 7)
+// ---------------------------------------------------------------------------
+
+// S = sum(k -> E[k[1]]*k[2], d(c)) as a left fold over the 2R+1 cells
+// (src/GaussianFixedPointQuiet.jl:10); E is staged in LDS.
+template <int R>
+__device__ __forceinline__ double gather_gauss(const double* __restrict__ Eld, double c,
+                                               double Nd, int32_t nmask) {
+    const int64_t i0 = (int64_t)rint(c * Nd);
+    double prev = edge_half_erf(i0 - R, c, Nd);
+    double S = 0.0;
+#pragma unroll 1
+    for (int j = -R; j <= R; ++j) {
+        double next = edge_half_erf(i0 + j + 1, c, Nd);
+        double wt = next - prev;
+        prev = next;
+        double term = Eld[cell0(i0 + j, nmask)] * wt;
+        S = (j == -R) ? term : S + term;
+    }
+    return S;
+}
+
+// r[k[1]] += k[2]*w for the 2R+1 cells of d(c) (src/GaussianFixedPointQuiet.jl:7),
+// each product rounded to the 2^-q integer grid and added into LDS limbs.
+template <int R>
+__device__ __forceinline__ void deposit_gauss(u64* __restrict__ acc, int32_t N, double c,
+                                              double Nd, int32_t nmask, double w, int q) {
+    const int64_t i0 = (int64_t)rint(c * Nd);
+    double prev = edge_half_erf(i0 - R, c, Nd);
+#pragma unroll 1
+    for (int j = -R; j <= R; ++j) {
+        double next = edge_half_erf(i0 + j + 1, c, Nd);
+        double wt = next - prev;
+        prev = next;
+        double a = wt * w;
+        if (a != 0.0) lds_add_fxp(acc, N, cell0(i0 + j, nmask), fxp_from(a, q));
+    }
+}
+
+__device__ __forceinline__ void zero_lds(u64* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+// Write this block's LDS limbs as normalised u128 partials.
+__device__ __forceinline__ void flush_partials(const u64* __restrict__ acc, int32_t N,
+                                               u128* __restrict__ out) {
+    for (int c = threadIdx.x; c < N; c += blockDim.x)
+        out[c] = limbs_to_u128(acc[c], acc[N + c], acc[2 * N + c]);
+}
+
+__device__ __forceinline__ bool sweep_over(const InstState& S, int k) {
+    return S.last_sweep >= 0 && S.last_sweep < k;
+}
+
+// ---------------------------------------------------------------------------
+// Implicit-midpoint push (src/GaussianFixedPointQuiet.jl:8-10).
+// MODE 0: the first deposit of a step, v_0 = V, so x_0 = X + ((V+V)/2)*dt.
+// MODE 1: sweep k: gather E_k at c_k = (x_k+X)/2, v_{k+1} = V + S*dt; if
+//         sweep k ends the loop write the step's result (mod(x_k,1), v_{k+1}),
+//         else deposit the charge of sweep k+1 at c_{k+1}.
+// ---------------------------------------------------------------------------
+template <int R, int MODE>
+__global__ void __launch_bounds__(1024)
+k_mid_push64(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
+             const TileDesc* __restrict__ tiles, double* X0, double* X1, double* V0,
+             double* V1, double* vit0, double* vit1, const double* __restrict__ E0,
+             const double* __restrict__ E1, u128* __restrict__ part, int k) {
+    extern __shared__ u64 lds[];
+    const TileDesc T = tiles[blockIdx.x];
+    const InstDesc I = insts[T.inst];
+    InstState S = st[T.inst];
+    if (MODE == 0) {
+        if (T.local == 0 && threadIdx.x == 0) st[T.inst].last_sweep = -1;
+        S.last_sweep = -1;
+    }
+    if (MODE == 1 && sweep_over(S, k)) return;
+    const bool last = (MODE == 1) && (S.last_sweep == k);
+    const int32_t N = I.N;
+    double* Eld = reinterpret_cast<double*>(lds);
+    u64* acc = lds + N;
+    if (MODE == 1) {
+        const double* Ein = (S.e_cur ? E1 : E0) + I.goff;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) Eld[i] = Ein[i];
+    }
+    if (!last) zero_lds(acc, 3 * N);
+    __syncthreads();
+
+    const int par = S.step & 1;
+    const double* X = par ? X1 : X0;
+    const double* V = par ? V1 : V0;
+    double* Xo = par ? X0 : X1;
+    double* Vo = par ? V0 : V1;
+    const double* vin = (k & 1) ? vit1 : vit0;
+    double* vout = ((k + 1) & 1) ? vit1 : vit0;
+    const double Nd = (double)N, dt = I.dt, w = I.w;
+    const int32_t nmask = I.nmask;
+    const int q = I.q;
+    const int64_t base = I.poff + T.start;
+    for (int64_t p = threadIdx.x; p < T.count; p += blockDim.x) {
+        const int64_t g = base + p;
+        const double Xp = X[g], Vp = V[g];
+        double vn = Vp;
+        if (MODE == 1) {
+            const double vk = (k == 0) ? Vp : vin[g];
+            const double x = Xp + ((vk + Vp) / 2.0) * dt;
+            const double c = (x + Xp) / 2.0;
+            const double Sg = gather_gauss<R>(Eld, c, Nd, nmask);
+            vn = Vp + Sg * dt;
+            if (last) {
+                Xo[g] = jl_mod1f(x);
+                Vo[g] = vn;
+                continue;
+            }
+            vout[g] = vn;
+        }
+        const double x2 = Xp + ((vn + Vp) / 2.0) * dt;
+        const double c2 = (x2 + Xp) / 2.0;
+        deposit_gauss<R>(acc, N, c2, Nd, nmask, w, q);
+    }
+    if (last) return;
+    __syncthreads();
+    flush_partials(acc, N, part + T.part_off);
+}
+
+// ---------------------------------------------------------------------------
+// Leapfrog (src/NGPFourier.jl:4-6, src/Gaussian.jl:8-10).
+// first:  u(); deposit at x.        second: u(); v += gather(E at x)*dt.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ void __launch_bounds__(1024)
+k_lf_first64(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
+             const TileDesc* __restrict__ tiles, double* X0, double* X1,
+             const double* V0, const double* V1, u128* __restrict__ part) {
+    extern __shared__ u64 lds[];
+    const TileDesc T = tiles[blockIdx.x];
+    const InstDesc I = insts[T.inst];
+    const InstState S = st[T.inst];
+    const int32_t N = I.N;
+    u64* acc = lds;
+    zero_lds(acc, R == 0 ? N : 3 * N);
+    __syncthreads();
+    const int par = S.step & 1;
+    const double* X = par ? X1 : X0;
+    const double* V = par ? V1 : V0;
+    double* Xo = par ? X0 : X1;
+    const double Nd = (double)N, dt = I.dt;
+    const int64_t base = I.poff + T.start;
+    for (int64_t p = threadIdx.x; p < T.count; p += blockDim.x) {
+        const int64_t g = base + p;
+        const double x = jl_mod1f(X[g] + (V[g] / 2.0) * dt);
+        Xo[g] = x;
+        if (R == 0) {
+            atomicAdd(&acc[ngp_cell(x, Nd, I.nmask)], 1ull);
+        } else {
+            deposit_gauss<R>(acc, N, (x + x) / 2.0, Nd, I.nmask, I.w, I.q);
+        }
+    }
+    __syncthreads();
+    u128* out = part + T.part_off;
+    if (R == 0) {
+        const u128 one = fxp_from(I.w, I.q);
+        for (int c = threadIdx.x; c < N; c += blockDim.x) out[c] = (u128)acc[c] * one;
+    } else {
+        flush_partials(acc, N, out);
+    }
+}
+
+template <int R>
+__global__ void __launch_bounds__(1024)
+k_lf_second64(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
+              const TileDesc* __restrict__ tiles, double* X0, double* X1, const double* V0,
+              const double* V1, double* Vo0, double* Vo1, const double* __restrict__ E0) {
+    extern __shared__ u64 lds[];
+    const TileDesc T = tiles[blockIdx.x];
+    const InstDesc I = insts[T.inst];
+    const InstState S = st[T.inst];
+    const int32_t N = I.N;
+    double* Eld = reinterpret_cast<double*>(lds);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) Eld[i] = E0[I.goff + i];
+    __syncthreads();
+    const int par = S.step & 1;
+    const double* V = par ? V1 : V0;
+    double* Xn = par ? X0 : X1;  // written by the first half of this step
+    double* Vn = par ? Vo0 : Vo1;
+    const double Nd = (double)N, dt = I.dt;
+    const int64_t base = I.poff + T.start;
+    for (int64_t p = threadIdx.x; p < T.count; p += blockDim.x) {
+        const int64_t g = base + p;
+        const double v = V[g];
+        const double x = jl_mod1f(Xn[g] + (v / 2.0) * dt);
+        Xn[g] = x;
+        double Sg;
+        if (R == 0) Sg = Eld[ngp_cell(x, Nd, I.nmask)];
+        else Sg = gather_gauss<R>(Eld, x, Nd, I.nmask);
+        Vn[g] = v + Sg * dt;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-cell sum of the tiles' partials, rounded once to Float64.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_reduce64(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
+           const RedDesc* __restrict__ reds, const u128* __restrict__ part,
+           double* __restrict__ rho, int k) {
+    const RedDesc D = reds[blockIdx.x];
+    const InstDesc I = insts[D.inst];
+    const InstState S = st[D.inst];
+    if (sweep_over(S, k)) return;
+    const int c = D.c0 + threadIdx.x;
+    if (c >= I.N) return;
+    u128 s = 0;
+    const u128* p = part + I.part0 + c;
+    for (int t = 0; t < I.ntiles; ++t) s += p[(int64_t)t * I.N];
+    const double r = fxp_to(s, I.q);
+    rho[I.goff + c] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Spectral solve in LDS: E = real(ifft((xi = fft(rho) ./ ik; xi[1] *= 0; xi)))
+// with the radix-2 decimation-in-time transform of the CPU restatement.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t brev(uint32_t i, int bits) {
+    return __brev(i) >> (32 - bits);
+}
+
+__device__ void fft_stages(double* re, double* im, const double* twr, const double* twi,
+                           int N, int log2N, bool inverse) {
+    for (int s = 1; s <= log2N; ++s) {
+        const int h = 1 << (s - 1);
+        const int step = N >> s;
+        for (int b = threadIdx.x; b < N / 2; b += blockDim.x) {
+            const int grp = b >> (s - 1);
+            const int j = b & (h - 1);
+            const int i1 = (grp << s) + j;
+            const int i2 = i1 + h;
+            const double wr = twr[j * step];
+            const double wi = inverse ? -twi[j * step] : twi[j * step];
+            const double ar = re[i2], ai = im[i2];
+            const double tr = ar * wr - ai * wi;
+            const double ti = ar * wi + ai * wr;
+            const double ur = re[i1], ui = im[i1];
+            re[i1] = ur + tr;
+            im[i1] = ui + ti;
+            re[i2] = ur - tr;
+            im[i2] = ui - ti;
+        }
+        __syncthreads();
+    }
+}
+
+// On entry re/im hold rho in bit-reversed order; on exit re holds E.
+__device__ void solve_lds(double* re, double* im, double* twr, double* twi,
+                          const double* __restrict__ tw, int N, int log2N) {
+    const int stride = kMaxN / N;
+    for (int m = threadIdx.x; m < N / 2; m += blockDim.x) {
+        twr[m] = tw[2 * m * stride];
+        twi[m] = tw[2 * m * stride + 1];
+    }
+    __syncthreads();
+    fft_stages(re, im, twr, twi, N, log2N, false);
+    // xi = rho^ ./ ik with ik = 2 pi i [1, 1:N/2, -N/2+1:-1]; Julia's complex
+    // division by (0, k) is (Im * (1/k), -(Re * (1/k))); xi[1] *= 0.
+    constexpr int kMaxPer = kMaxN / 256;
+    double xr[kMaxPer], xi[kMaxPer];
+    int cnt = 0;
+    for (int m = threadIdx.x; m < N; m += blockDim.x, ++cnt) {
+        const int mp = m == 0 ? 1 : (m <= N / 2 ? m : m - N);
+        const double kk = kTwoPi * (double)mp;
+        const double t = 1.0 / kk;
+        double a = im[m] * t;
+        double b = -(re[m] * t);
+        if (m == 0) { a = a * 0.0; b = b * 0.0; }
+        xr[cnt] = a;
+        xi[cnt] = b;
+    }
+    __syncthreads();
+    cnt = 0;
+    for (int m = threadIdx.x; m < N; m += blockDim.x, ++cnt) {
+        const uint32_t r = brev((uint32_t)m, log2N);
+        re[r] = xr[cnt];
+        im[r] = xi[cnt];
+    }
+    __syncthreads();
+    fft_stages(re, im, twr, twi, N, log2N, true);
+    const double scale = 1.0 / (double)N;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) re[j] = re[j] * scale;
+    __syncthreads();
+}
+
+// isapprox(F, E; rtol, atol) on arrays (src/GaussianFixedPointQuiet.jl:8), as
+// restated on the CPU.  All threads of the block call it.
+__device__ bool block_isapprox(double* red, const double* F, const double* E, int n,
+                               double rtol, double atol) {
+    const double dn = block_norm2(red, n, [&](int64_t i) { return F[i] - E[i]; });
+    if (isfinite(dn)) {
+        const double nf = block_norm2(red, n, [&](int64_t i) { return F[i]; });
+        const double ne = block_norm2(red, n, [&](int64_t i) { return E[i]; });
+        double mx = nf > ne ? nf : ne;
+        if (nf != nf || ne != ne) mx = __longlong_as_double(0x7ff8000000000000LL);
+        const double tol = rtol * mx;
+        double lim = atol > tol ? atol : tol;
+        if (tol != tol) lim = tol;
+        return dn <= lim;
+    }
+    __shared__ int ok;
+    if (threadIdx.x == 0) ok = 1;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double a = F[i], b = E[i];
+        if (a == b) continue;
+        bool good = isfinite(a) && isfinite(b);
+        if (good) {
+            const double ma = fabs(a) > fabs(b) ? fabs(a) : fabs(b);
+            const double t = rtol * ma;
+            good = fabs(a - b) <= (atol > t ? atol : t);
+        }
+        if (!good) ok = 0;
+    }
+    __syncthreads();
+    const bool r = ok != 0;
+    __syncthreads();
+    return r;
+}
+
+// One block per instance.  TEST: the midpoint's convergence test between the
+// previous field (e_cur) and the new one -- the test the scripts run at the
+// top of sweep k+1 -- ending the loop on success or at the last sweep.
+template <bool TEST>
+__global__ void __launch_bounds__(1024)
+k_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
+        const double* __restrict__ rho, double* E0, double* E1, float* E32,
+        const double* __restrict__ tw, int k) {
+    extern __shared__ double sm[];
+    const InstDesc I = insts[blockIdx.x];
+    InstState S = st[blockIdx.x];
+    if (TEST && sweep_over(S, k)) return;
+    const int N = I.N, log2N = I.log2N;
+    double* re = sm;
+    double* im = sm + N;
+    double* twr = sm + 2 * N;
+    double* twi = sm + 2 * N + N / 2;
+    double* red = sm + 3 * N;
+    __shared__ int nonfinite;
+    if (threadIdx.x == 0) nonfinite = 0;
+    __syncthreads();
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        const double r = rho[I.goff + j];
+        if (!isfinite(r)) nonfinite = 1;
+        const uint32_t b = brev((uint32_t)j, log2N);
+        re[b] = r;
+        im[b] = 0.0;
+    }
+    __syncthreads();
+    solve_lds(re, im, twr, twi, tw, N, log2N);
+    const int cur = TEST ? S.e_cur : 0;
+    const int nxt = TEST ? 1 - cur : 0;
+    double* Eo = (nxt ? E1 : E0) + I.goff;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        const double e = re[j];
+        if (!isfinite(e)) nonfinite = 1;
+        Eo[j] = e;
+        E32[I.goff + j] = (float)e;
+    }
+    bool conv = false;
+    if (TEST) {
+        const double* F = (cur ? E1 : E0) + I.goff;
+        conv = block_isapprox(red, F, re, N, I.rtol, 0.0);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (nonfinite) S.err |= 1;
+        if (TEST) {
+            S.e_cur = nxt;
+            if (conv || k == I.max_sweeps - 1) S.last_sweep = k;
+        } else {
+            S.e_cur = 0;
+            S.last_sweep = 0;
+        }
+        st[blockIdx.x] = S;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics (src/GaussianFixedPointQuiet.jl:11,13):
+// D[1:2] = (sum(E.^2)/N, sum(v.^2)*W/P) ./ 2; D[3:4] = (D1 + D2, sum(v/P));
+// D[1:3] .*= 2/W.  Sums use the canonical tree.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_diag_chunks64(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
+                const ChunkDesc* __restrict__ chunks, const double* V0, const double* V1,
+                double* __restrict__ dpart) {
+    __shared__ double red[kTreeLanes];
+    const ChunkDesc C = chunks[blockIdx.x];
+    const InstDesc I = insts[C.inst];
+    const InstState S = st[C.inst];
+    // the step's result sits in the buffer of the next parity
+    const double* V = ((S.step + 1) & 1) ? V1 : V0;
+    const double* v = V + I.poff + C.start;
+    const double Pd = (double)I.P;
+    const double s2 = block_chunk_sum(red, C.count, [&](int64_t i) { const double a = v[i]; return a * a; });
+    const double s1 = block_chunk_sum(red, C.count, [&](int64_t i) { return v[i] / Pd; });
+    if (threadIdx.x == 0) {
+        dpart[2 * blockIdx.x] = s2;
+        dpart[2 * blockIdx.x + 1] = s1;
+    }
+}
+
+// Canonical tree over n values spread with stride 2 (in place in dpart).
+__device__ double tree_over_partials(double* red, double* vals, int64_t n) {
+    while (n > kTreeChunk) {
+        const int64_t C = (n + kTreeChunk - 1) / kTreeChunk;
+        for (int64_t c = 0; c < C; ++c) {
+            const int64_t m = n - c * kTreeChunk < kTreeChunk ? n - c * kTreeChunk : kTreeChunk;
+            const double s = block_chunk_sum(red, m, [&](int64_t i) { return vals[2 * (c * kTreeChunk + i)]; });
+            if (threadIdx.x == 0) vals[2 * c] = s;
+            __syncthreads();
+        }
+        n = C;
+    }
+    return block_chunk_sum(red, n, [&](int64_t i) { return vals[2 * i]; });
+}
+
+__global__ void __launch_bounds__(256)
+k_diag_final(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
+             double* __restrict__ dpart, const double* E0, const double* E1,
+             double* __restrict__ diag, int32_t* __restrict__ sweeps, int64_t cap,
+             int scheme) {
+    __shared__ double red[kTreeLanes];
+    const InstDesc I = insts[blockIdx.x];
+    InstState S = st[blockIdx.x];
+    const double* E = (S.e_cur ? E1 : E0) + I.goff;
+    const double se = block_chunk_sum(red, I.N, [&](int64_t i) { return E[i] * E[i]; });
+    double* dp = dpart + 2 * (int64_t)I.chunk0;
+    const double sv = tree_over_partials(red, dp, I.nchunks);
+    const double sp = tree_over_partials(red, dp + 1, I.nchunks);
+    if (threadIdx.x == 0) {
+        const double d1 = (se / (double)I.N) / 2.0;
+        const double d2 = ((sv * I.W) / (double)I.P) / 2.0;
+        const double d3 = d1 + d2;
+        const double s = 2.0 / I.W;
+        const int64_t row = (int64_t)(S.step - S.run_base);
+        if (row >= 0 && row < cap) {
+            double* o = diag + ((int64_t)blockIdx.x * cap + row) * 4;
+            o[0] = d1 * s;
+            o[1] = d2 * s;
+            o[2] = d3 * s;
+            o[3] = sp;
+            sweeps[(int64_t)blockIdx.x * cap + row] = scheme == 1 ? S.last_sweep + 1 : 1;
+        }
+        S.step += 1;
+        st[blockIdx.x] = S;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers for the run
+// ---------------------------------------------------------------------------
+int push_lds_bytes(int N, int precision) {
+    if (precision == 0) return N * 8 * 4;  // E + three limb arrays
+    return N * 4 + N * 8;                   // FP32 E + u64 accumulators
+}
+
+void launch_mid_push0(const RunBuffers& b, hipStream_t s) {
+    const dim3 g(b.ntiles), t(b.block_push);
+    u128* part = (u128*)b.part;
+    if (b.R == 6)
+        k_mid_push64<6, 0><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
+                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, 0);
+    else
+        k_mid_push64<7, 0><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
+                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, 0);
+}
+
+void launch_mid_push(const RunBuffers& b, int k, hipStream_t s) {
+    const dim3 g(b.ntiles), t(b.block_push);
+    u128* part = (u128*)b.part;
+    if (b.R == 6)
+        k_mid_push64<6, 1><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
+                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, k);
+    else
+        k_mid_push64<7, 1><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
+                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, k);
+}
+
+void launch_reduce(const RunBuffers& b, int k, hipStream_t s) {
+    k_reduce64<<<b.nreds, 256, 0, s>>>(b.insts, b.st, b.reds, (const u128*)b.part, b.rho, k);
+}
+
+static int solve_lds_bytes(int N) { return (3 * N + kTreeLanes) * 8; }
+
+void launch_solve(const RunBuffers& b, int k, hipStream_t s) {
+    const int lds = solve_lds_bytes(b.maxN);
+    if (b.scheme == 1)
+        k_solve<true><<<b.ninst, b.block_solve, lds, s>>>(b.insts, b.st, b.rho, b.E[0], b.E[1], b.E32, b.tw, k);
+    else
+        k_solve<false><<<b.ninst, b.block_solve, lds, s>>>(b.insts, b.st, b.rho, b.E[0], b.E[1], b.E32, b.tw, k);
+}
+
+void launch_lf_first(const RunBuffers& b, hipStream_t s) {
+    const dim3 g(b.ntiles), t(b.block_push);
+    u128* part = (u128*)b.part;
+    if (b.R == 0)
+        k_lf_first64<0><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1], part);
+    else if (b.R == 6)
+        k_lf_first64<6><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1], part);
+    else
+        k_lf_first64<7><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1], part);
+}
+
+void launch_lf_second(const RunBuffers& b, hipStream_t s) {
+    const dim3 g(b.ntiles), t(b.block_push);
+    if (b.R == 0)
+        k_lf_second64<0><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
+                                                 b.V[0], b.V[1], b.E[0]);
+    else if (b.R == 6)
+        k_lf_second64<6><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
+                                                 b.V[0], b.V[1], b.E[0]);
+    else
+        k_lf_second64<7><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
+                                                 b.V[0], b.V[1], b.E[0]);
+}
+
+void launch_diag(const RunBuffers& b, hipStream_t s) {
+    if (b.precision == 0) {
+        k_diag_chunks64<<<b.nchunks, 256, 0, s>>>(b.insts, b.st, b.chunks, b.V[0], b.V[1], b.dpart);
+        k_diag_final<<<b.ninst, 256, 0, s>>>(b.insts, b.st, b.dpart, b.E[0], b.E[1], b.diag, b.sweeps,
+                                             b.diag_cap, b.scheme);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Operators: the scripts' closures one at a time on contiguous arrays.
+// ---------------------------------------------------------------------------
+constexpr int kOpTileP = 16384;
+
+int op_tiles(int64_t P) {
+    int64_t t = (P + kOpTileP - 1) / kOpTileP;
+    return (int)(t < 1 ? 1 : t);
+}
+
+template <int R>
+__global__ void __launch_bounds__(256)
+k_op_deposit(const double* __restrict__ x, const double* __restrict__ y, int64_t P, double w,
+             int q, int N, u128* __restrict__ part) {
+    extern __shared__ u64 lds[];
+    zero_lds(lds, R == 0 ? N : 3 * N);
+    __syncthreads();
+    const double Nd = (double)N;
+    const int32_t nmask = N - 1;
+    const int64_t start = (int64_t)blockIdx.x * kOpTileP;
+    const int64_t end = start + kOpTileP < P ? start + kOpTileP : P;
+    for (int64_t j = start + threadIdx.x; j < end; j += blockDim.x) {
+        const double xj = x[j];
+        if (R == 0) {
+            atomicAdd(&lds[ngp_cell(xj, Nd, nmask)], 1ull);
+        } else {
+            const double c = y ? (xj + y[j]) / 2.0 : (xj + xj) / 2.0;
+            deposit_gauss<R>(lds, N, c, Nd, nmask, w, q);
+        }
+    }
+    __syncthreads();
+    u128* out = part + (int64_t)blockIdx.x * N;
+    if (R == 0) {
+        const u128 one = fxp_from(w, q);
+        for (int c = threadIdx.x; c < N; c += blockDim.x) out[c] = (u128)lds[c] * one;
+    } else {
+        flush_partials(lds, N, out);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_op_reduce(const u128* __restrict__ part, int ntiles, int N, int q, double* __restrict__ rho) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    u128 s = 0;
+    for (int t = 0; t < ntiles; ++t) s += part[(int64_t)t * N + c];
+    rho[c] = fxp_to(s, q);
+}
+
+__global__ void __launch_bounds__(1024)
+k_op_fieldsolve(const double* __restrict__ rho, double* __restrict__ E, const double* __restrict__ tw,
+                int N, int log2N) {
+    extern __shared__ double sm[];
+    double* re = sm;
+    double* im = sm + N;
+    double* twr = sm + 2 * N;
+    double* twi = sm + 2 * N + N / 2;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        const uint32_t b = brev((uint32_t)j, log2N);
+        re[b] = rho[j];
+        im[b] = 0.0;
+    }
+    __syncthreads();
+    solve_lds(re, im, twr, twi, tw, N, log2N);
+    for (int j = threadIdx.x; j < N; j += blockDim.x) E[j] = re[j];
+}
+
+__global__ void k_op_drift_half(double* __restrict__ x, const double* __restrict__ v, int64_t P, double dt) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < P) x[j] = jl_mod1f(x[j] + (v[j] / 2.0) * dt);
+}
+
+template <int R>
+__global__ void __launch_bounds__(256)
+k_op_push_leapfrog(double* __restrict__ x, double* __restrict__ v, const double* __restrict__ E,
+                   int64_t P, double dt, int N) {
+    extern __shared__ double Eld[];
+    for (int i = threadIdx.x; i < N; i += blockDim.x) Eld[i] = E[i];
+    __syncthreads();
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P) return;
+    const double Nd = (double)N;
+    const double vj = v[j];
+    const double xj = jl_mod1f(x[j] + (vj / 2.0) * dt);
+    x[j] = xj;
+    const double Sg = R == 0 ? Eld[ngp_cell(xj, Nd, N - 1)] : gather_gauss<R>(Eld, xj, Nd, N - 1);
+    v[j] = vj + Sg * dt;
+}
+
+__global__ void k_op_midpos(const double* __restrict__ X, const double* __restrict__ V,
+                            const double* __restrict__ v, double* __restrict__ x, int64_t P, double dt) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < P) x[j] = X[j] + ((v[j] + V[j]) / 2.0) * dt;
+}
+
+template <int R>
+__global__ void __launch_bounds__(256)
+k_op_push_midpoint(const double* __restrict__ X, const double* __restrict__ V,
+                   const double* __restrict__ x, double* __restrict__ v,
+                   const double* __restrict__ E, int64_t P, double dt, int N) {
+    extern __shared__ double Eld[];
+    for (int i = threadIdx.x; i < N; i += blockDim.x) Eld[i] = E[i];
+    __syncthreads();
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P) return;
+    const double Sg = gather_gauss<R>(Eld, (x[j] + X[j]) / 2.0, (double)N, N - 1);
+    v[j] = V[j] + Sg * dt;
+}
+
+__global__ void k_op_wrap(double* __restrict__ x, int64_t P) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < P) x[j] = jl_mod1f(x[j]);
+}
+
+__global__ void __launch_bounds__(256)
+k_op_isapprox(const double* __restrict__ F, const double* __restrict__ E, int n, double rtol,
+              double atol, int32_t* out) {
+    __shared__ double red[kTreeLanes];
+    const bool r = block_isapprox(red, F, E, n, rtol, atol);
+    if (threadIdx.x == 0) *out = r ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(256)
+k_op_diag_chunks(const double* __restrict__ v, int64_t P, double* __restrict__ dpart) {
+    __shared__ double red[kTreeLanes];
+    const int64_t start = (int64_t)blockIdx.x * kTreeChunk;
+    const int64_t n = P - start < kTreeChunk ? P - start : kTreeChunk;
+    const double Pd = (double)P;
+    const double* vv = v + start;
+    const double s2 = block_chunk_sum(red, n, [&](int64_t i) { const double a = vv[i]; return a * a; });
+    const double s1 = block_chunk_sum(red, n, [&](int64_t i) { return vv[i] / Pd; });
+    if (threadIdx.x == 0) {
+        dpart[2 * blockIdx.x] = s2;
+        dpart[2 * blockIdx.x + 1] = s1;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_op_diag_final(const double* __restrict__ E, int N, double* __restrict__ dpart, int64_t nchunks,
+                int64_t P, double W, double* __restrict__ out) {
+    __shared__ double red[kTreeLanes];
+    const double se = block_chunk_sum(red, N, [&](int64_t i) { return E[i] * E[i]; });
+    const double sv = tree_over_partials(red, dpart, nchunks);
+    const double sp = tree_over_partials(red, dpart + 1, nchunks);
+    if (threadIdx.x == 0) {
+        const double d1 = (se / (double)N) / 2.0;
+        const double d2 = ((sv * W) / (double)P) / 2.0;
+        const double d3 = d1 + d2;
+        const double s = 2.0 / W;
+        out[0] = d1 * s;
+        out[1] = d2 * s;
+        out[2] = d3 * s;
+        out[3] = sp;
+    }
+}
+
+static inline int blocks_for(int64_t P, int t) { return (int)((P + t - 1) / t); }
+
+void launch_op_deposit(const OpBuffers& o, const double* x, const double* y, int64_t P, double w,
+                       int R, int q, double* rho, hipStream_t s) {
+    const int nt = op_tiles(P);
+    const int lds = (R == 0 ? 1 : 3) * o.N * 8;
+    u128* part = (u128*)o.part;
+    if (R == 0) k_op_deposit<0><<<nt, 256, lds, s>>>(x, y, P, w, q, o.N, part);
+    else if (R == 6) k_op_deposit<6><<<nt, 256, lds, s>>>(x, y, P, w, q, o.N, part);
+    else k_op_deposit<7><<<nt, 256, lds, s>>>(x, y, P, w, q, o.N, part);
+    k_op_reduce<<<(o.N + 255) / 256, 256, 0, s>>>(part, nt, o.N, q, rho);
+}
+
+void launch_op_fieldsolve(const OpBuffers& o, const double* rho, double* E, hipStream_t s) {
+    const int t = o.N >= 2048 ? 1024 : 256;
+    k_op_fieldsolve<<<1, t, solve_lds_bytes(o.N), s>>>(rho, E, o.tw, o.N, o.log2N);
+}
+
+void launch_op_drift_half(double* x, const double* v, int64_t P, double dt, hipStream_t s) {
+    k_op_drift_half<<<blocks_for(P, 256), 256, 0, s>>>(x, v, P, dt);
+}
+
+void launch_op_push_leapfrog(const OpBuffers& o, double* x, double* v, const double* E, int64_t P,
+                             double dt, int R, hipStream_t s) {
+    const int lds = o.N * 8;
+    if (R == 0) k_op_push_leapfrog<0><<<blocks_for(P, 256), 256, lds, s>>>(x, v, E, P, dt, o.N);
+    else if (R == 6) k_op_push_leapfrog<6><<<blocks_for(P, 256), 256, lds, s>>>(x, v, E, P, dt, o.N);
+    else k_op_push_leapfrog<7><<<blocks_for(P, 256), 256, lds, s>>>(x, v, E, P, dt, o.N);
+}
+
+void launch_op_midpos(const double* X, const double* V, const double* v, double* x, int64_t P,
+                      double dt, hipStream_t s) {
+    k_op_midpos<<<blocks_for(P, 256), 256, 0, s>>>(X, V, v, x, P, dt);
+}
+
+void launch_op_push_midpoint(const OpBuffers& o, const double* X, const double* V, const double* x,
+                             double* v, const double* E, int64_t P, double dt, int R, hipStream_t s) {
+    const int lds = o.N * 8;
+    if (R == 6) k_op_push_midpoint<6><<<blocks_for(P, 256), 256, lds, s>>>(X, V, x, v, E, P, dt, o.N);
+    else k_op_push_midpoint<7><<<blocks_for(P, 256), 256, lds, s>>>(X, V, x, v, E, P, dt, o.N);
+}
+
+void launch_op_wrap(double* x, int64_t P, hipStream_t s) {
+    k_op_wrap<<<blocks_for(P, 256), 256, 0, s>>>(x, P);
+}
+
+void launch_op_isapprox(const double* F, const double* E, int32_t n, double rtol, double atol,
+                        int32_t* out, hipStream_t s) {
+    k_op_isapprox<<<1, 256, 0, s>>>(F, E, n, rtol, atol, out);
+}
+
+void launch_op_diagnostics(const OpBuffers& o, const double* E, const double* v, int64_t P, double W,
+                           double* out4, hipStream_t s) {
+    const int64_t nch = (P + kTreeChunk - 1) / kTreeChunk;
+    k_op_diag_chunks<<<(int)nch, 256, 0, s>>>(v, P, o.dpart);
+    k_op_diag_final<<<1, 256, 0, s>>>(E, o.N, o.dpart, nch, P, W, out4);
+}
+
+bool mixed_available() { return false; }
+
+template <typename K>
+static hipError_t allow_lds(K kernel, int bytes) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+int configure_kernels() {
+    const int big = 160 * 1024;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t r) { if (r != hipSuccess) e = r; };
+    chk(allow_lds(k_mid_push64<6, 0>, big));
+    chk(allow_lds(k_mid_push64<7, 0>, big));
+    chk(allow_lds(k_mid_push64<6, 1>, big));
+    chk(allow_lds(k_mid_push64<7, 1>, big));
+    chk(allow_lds(k_lf_first64<0>, big));
+    chk(allow_lds(k_lf_first64<6>, big));
+    chk(allow_lds(k_lf_first64<7>, big));
+    chk(allow_lds(k_lf_second64<0>, big));
+    chk(allow_lds(k_lf_second64<6>, big));
+    chk(allow_lds(k_lf_second64<7>, big));
+    chk(allow_lds(k_solve<true>, big));
+    chk(allow_lds(k_solve<false>, big));
+    chk(allow_lds(k_op_deposit<0>, big));
+    chk(allow_lds(k_op_deposit<6>, big));
+    chk(allow_lds(k_op_deposit<7>, big));
+    chk(allow_lds(k_op_fieldsolve, big));
+    (void)hipGetLastError();
+    return e == hipSuccess ? 0 : 1;
+}
+
+}  // namespace pic

@@ -1,0 +1,125 @@
+// pic_internal.h -- descriptors shared by the host runtime and the kernels,
+// and the host-side launchers the kernels file exports.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pic {
+
+constexpr int kMaxN = 4096;       // largest grid (one LDS-resident transform)
+constexpr int kTreeLanes = 256;   // canonical reduction: lanes per chunk
+constexpr int kTreeChunk = 4096;  // canonical reduction: elements per chunk
+
+// One simulation instance (a batch holds many).  Device-resident, read-only
+// during a run.
+struct InstDesc {
+    int32_t N, log2N, nmask, R;  // cells, log2(N), N-1, stencil half-width (0: NGP)
+    int64_t P;                   // particles
+    int64_t poff;                // first particle in the packed particle arrays
+    int64_t part0;               // first u128 partial of this instance
+    int32_t goff;                // first cell in the packed grid arrays
+    int32_t tile0, ntiles;       // tiles of this instance in the tile list
+    int32_t chunk0, nchunks;     // diagnostics chunks (4096 particles each)
+    int32_t q;                   // fixed-point fraction bits for the FP64 deposit
+    int32_t max_sweeps;
+    int32_t pad0;
+    double w, dt, W, rtol;
+    double wq;                   // mixed deposit: w * 2^-30 (charge per weight quantum)
+};
+
+// Device-mutable per-instance loop state.
+struct InstState {
+    int32_t e_cur;       // E buffer (0/1) holding the latest field
+    int32_t last_sweep;  // -1 while sweeping; k once solve k ended the Picard loop
+    int32_t step;        // steps completed since upload
+    int32_t err;         // bit 0: non-finite charge/field
+    int32_t run_base;    // step at the start of the current pic_run
+    int32_t pad[3];
+};
+
+struct TileDesc {
+    int32_t inst;
+    int32_t local;       // tile index within the instance
+    int64_t start;       // first particle (instance-relative)
+    int64_t count;
+    int64_t part_off;    // first u128 partial of this tile
+};
+
+struct RedDesc {         // one block of <= 256 cells of one instance
+    int32_t inst, c0;
+};
+
+struct ChunkDesc {       // one diagnostics chunk
+    int32_t inst, pad;
+    int64_t start, count;
+};
+
+struct RunBuffers {
+    InstDesc* insts;
+    InstState* st;
+    TileDesc* tiles;
+    RedDesc* reds;
+    ChunkDesc* chunks;
+    int32_t ninst, ntiles, nreds, nchunks;
+    int32_t maxN, R, scheme, precision, max_sweeps;
+    int32_t block_push, block_solve, lds_push;
+    // FP64 particle state, ping-pong by step parity, and the Picard iterate
+    double* X[2];
+    double* V[2];
+    double* vit[2];
+    // mixed particle state: 32-bit fixed-point positions (2^-32 of the box),
+    // FP32 velocity as a compensated hi + lo pair, FP32 Picard iterate
+    uint32_t* Xq[2];
+    float* Vh[2];
+    float* Vl[2];
+    float* vf[2];
+    void* part;        // per-tile charge partials: u128 (FP64) or u64 (mixed)
+    double* rho;       // packed charge, sum of N over instances
+    double* E[2];      // packed field, ping-pong (InstState::e_cur)
+    float* E32;        // FP32 copy of the latest field for the mixed gather
+    const double* tw;  // twiddles of the 4096-point transform, (re, im) pairs
+    double* dpart;     // diagnostics partials, 2 per chunk
+    double* diag;      // [ninst][diag_cap][4]
+    int32_t* sweeps;   // [ninst][diag_cap]
+    int64_t diag_cap;
+};
+
+// kernels_1d.hip
+void launch_mid_push0(const RunBuffers& b, hipStream_t s);
+void launch_mid_push(const RunBuffers& b, int k, hipStream_t s);
+void launch_reduce(const RunBuffers& b, int k, hipStream_t s);
+void launch_solve(const RunBuffers& b, int k, hipStream_t s);
+void launch_lf_first(const RunBuffers& b, hipStream_t s);
+void launch_lf_second(const RunBuffers& b, hipStream_t s);
+void launch_diag(const RunBuffers& b, hipStream_t s);
+int push_lds_bytes(int N, int precision);
+int configure_kernels();      // raise the dynamic LDS limit; 0 on success
+bool mixed_available();
+
+// operators (single grid of N cells, contiguous particle arrays)
+struct OpBuffers {
+    int32_t N, log2N;
+    const double* tw;
+    void* part;          // >= ntiles * N u128
+    double* dpart;       // >= 2 * ceil(P/4096)
+    int32_t* flag;
+};
+void launch_op_deposit(const OpBuffers& o, const double* x, const double* y, int64_t P,
+                       double w, int R, int q, double* rho, hipStream_t s);
+void launch_op_fieldsolve(const OpBuffers& o, const double* rho, double* E, hipStream_t s);
+void launch_op_drift_half(double* x, const double* v, int64_t P, double dt, hipStream_t s);
+void launch_op_push_leapfrog(const OpBuffers& o, double* x, double* v, const double* E,
+                             int64_t P, double dt, int R, hipStream_t s);
+void launch_op_midpos(const double* X, const double* V, const double* v, double* x,
+                      int64_t P, double dt, hipStream_t s);
+void launch_op_push_midpoint(const OpBuffers& o, const double* X, const double* V,
+                             const double* x, double* v, const double* E, int64_t P,
+                             double dt, int R, hipStream_t s);
+void launch_op_wrap(double* x, int64_t P, hipStream_t s);
+void launch_op_isapprox(const double* F, const double* E, int32_t n, double rtol,
+                        double atol, int32_t* out, hipStream_t s);
+void launch_op_diagnostics(const OpBuffers& o, const double* E, const double* v, int64_t P,
+                           double W, double* out4, hipStream_t s);
+int op_tiles(int64_t P);
+
+}  // namespace pic

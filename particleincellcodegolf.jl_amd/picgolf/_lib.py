@@ -1,0 +1,112 @@
+"""ctypes binding of libpicgolf.so (include/pic.h).
+
+The library is the product: HIP kernels for gfx950 behind a C ABI.  Loading
+fails loudly when it is missing -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.environ.get("PICGOLF_LIB", os.path.join(_PKG, "lib", "libpicgolf.so"))
+
+PIC_OK = 0
+PIC_EINVAL = -1
+PIC_EHIP = -2
+PIC_ENOMEM = -3
+PIC_ENONFINITE = -4
+PIC_ESTATE = -5
+
+SHAPE_NGP, SHAPE_GAUSS6, SHAPE_GAUSS7 = 0, 6, 7
+LEAPFROG, MIDPOINT = 0, 1
+FP64, MIXED = 0, 1
+
+# every symbol include/pic.h declares
+EXPORTS = [
+    "pic_create", "pic_create_batch", "pic_destroy", "pic_last_error", "pic_strerror",
+    "pic_version", "pic_upload", "pic_download", "pic_run", "pic_run_profiled",
+    "pic_deposit", "pic_fieldsolve", "pic_drift_half", "pic_push_leapfrog",
+    "pic_midpoint_position", "pic_push_midpoint", "pic_wrap", "pic_isapprox",
+    "pic_diagnostics",
+]
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+class PicError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code
+
+
+class pic_config(ctypes.Structure):
+    _fields_ = [
+        ("N", ctypes.c_int32), ("shape", ctypes.c_int32), ("scheme", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("P", ctypes.c_int64), ("W", ctypes.c_double),
+        ("w", ctypes.c_double), ("dt", ctypes.c_double), ("rtol", ctypes.c_double),
+        ("max_sweeps", ctypes.c_int32), ("device", ctypes.c_int32),
+    ]
+
+
+class pic_kernel_stat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_int64), ("total_ms", ctypes.c_double)]
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def load():
+    """Load libpicgolf.so (raises LibraryMissing if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(f"{LIB_PATH} is missing: build it with "
+                             "`python particleincellcodegolf.jl_amd/build.py`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, d = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    ip = ctypes.POINTER(ctypes.c_int32)
+    sig = {
+        "pic_create": [ctypes.POINTER(vp), ctypes.POINTER(pic_config)],
+        "pic_create_batch": [ctypes.POINTER(vp), ctypes.POINTER(pic_config), i32],
+        "pic_destroy": [vp],
+        "pic_upload": [vp, i32, vp, vp, vp],
+        "pic_download": [vp, i32, vp, vp, vp],
+        "pic_run": [vp, i64, vp, vp],
+        "pic_run_profiled": [vp, i64, vp, vp, ctypes.POINTER(pic_kernel_stat), ip],
+        "pic_deposit": [vp, vp, vp, i64, d, i32, vp],
+        "pic_fieldsolve": [vp, vp, vp],
+        "pic_drift_half": [vp, vp, vp, i64, d],
+        "pic_push_leapfrog": [vp, vp, vp, vp, i64, d, i32],
+        "pic_midpoint_position": [vp, vp, vp, vp, vp, i64, d],
+        "pic_push_midpoint": [vp, vp, vp, vp, vp, vp, i64, d, i32],
+        "pic_wrap": [vp, vp, i64],
+        "pic_isapprox": [vp, vp, vp, i32, d, d, ip],
+        "pic_diagnostics": [vp, vp, vp, i64, d, vp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.pic_last_error.argtypes = [vp]
+    L.pic_last_error.restype = ctypes.c_char_p
+    L.pic_strerror.argtypes = [ctypes.c_int]
+    L.pic_strerror.restype = ctypes.c_char_p
+    L.pic_version.argtypes = []
+    L.pic_version.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def ptr(a):
+    if a is None:
+        return None
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)

@@ -1,0 +1,84 @@
+"""The five benchmark workloads with their initial conditions.
+
+Initial particle states are generated once on the host (numpy PCG64 streams
+from a seed) so the GPU run and the CPU restatement start from identical bits.
+
+  ngp_two_stream     N=128, P=8192, NGP leapfrog, dt=1/4N, 1024 steps, W=200
+                     (src/NGPFourier.jl:1-2; seeded uniform x, v=-1/+1 by half)
+  quiet_two_stream   N=64, P=2048, Gaussian +-7 midpoint, dt=1/6N, 8192 steps,
+                     W=32pi^2/3, rtol=4eps, bit-reversal quiet start
+                     (src/GaussianFixedPointQuiet.jl:1-6)
+  scaled_two_stream  N=4096, P=2^24, Gaussian +-7 midpoint, dt=1/6N, 2000 steps,
+                     W=400, rtol=1e-8 (src/GaussianFixedPoint.jl:4), seeded
+                     uniform x, v=-1/+1 by half plus N(0, 0.05^2)
+  ensemble           512 instances: N in {32..1024}, 16-256 particles per cell,
+                     W in [100, 400], beams +-u with u in [0.5, 1.5]
+"""
+import numpy as np
+
+from .core import Config
+
+EPS = float(np.finfo(np.float64).eps)
+
+
+def quiet_start(P):
+    """x = (bitreverse.(0:P-1) .+ 2.0^63) / 2.0^64 over signed Int64;
+    v = -1 for the first half, +1 after (src/GaussianFixedPointQuiet.jl:2-3)."""
+    i = np.arange(P, dtype=np.uint64)
+    r = np.zeros(P, dtype=np.uint64)
+    for b in range(64):
+        r |= ((i >> np.uint64(b)) & np.uint64(1)) << np.uint64(63 - b)
+    x = (r.view(np.int64).astype(np.float64) + 2.0 ** 63) / 2.0 ** 64
+    v = np.where(np.arange(1, P + 1) > P / 2, 1.0, -1.0)
+    return x, v
+
+
+def beams(P, u=1.0):
+    """v = collect(1:P .> P/2) .* 2 .- 1 scaled by the beam speed u."""
+    return np.where(np.arange(1, P + 1) > P / 2, u, -u).astype(np.float64)
+
+
+def ngp_two_stream(seed=0, P=None):
+    N = 128
+    P = 64 * N if P is None else P
+    W = 200.0
+    cfg = Config(N=N, P=P, W=W, w=W / P * N, dt=1 / (4 * N), shape="ngp", scheme="leapfrog")
+    rng = np.random.default_rng(seed)
+    return cfg, rng.random(P), beams(P), 1024
+
+
+def quiet_two_stream(precision="fp64"):
+    N = 64
+    P = 32 * N
+    W = 32 * np.pi ** 2 / 3
+    cfg = Config(N=N, P=P, W=W, w=W / P * N, dt=1 / (6 * N), shape="gauss7", scheme="midpoint",
+                 precision=precision, rtol=4 * EPS, max_sweeps=10)
+    x, v = quiet_start(P)
+    return cfg, x, v, 2 ** 13
+
+
+def scaled_two_stream(precision="mixed", seed=0, P=2 ** 24, N=4096):
+    W = 400.0
+    cfg = Config(N=N, P=P, W=W, w=W / P * N, dt=1 / (6 * N), shape="gauss7", scheme="midpoint",
+                 precision=precision, rtol=1e-8, max_sweeps=10)
+    rng = np.random.default_rng(seed)
+    x = rng.random(P)
+    v = beams(P) + 0.05 * rng.standard_normal(P)
+    return cfg, x, v, 2000
+
+
+def ensemble(n=512, precision="mixed", seed=0):
+    """Independent two-stream instances with uneven N, P, W and beam speed."""
+    rng = np.random.default_rng(seed)
+    cfgs, states = [], []
+    for k in range(n):
+        N = int(2 ** rng.integers(5, 11))        # 32 .. 1024
+        ppc = int(rng.integers(16, 257))
+        P = N * ppc
+        W = float(rng.uniform(100.0, 400.0))
+        u = float(rng.uniform(0.5, 1.5))
+        cfgs.append(Config(N=N, P=P, W=W, w=W / P * N, dt=1 / (6 * N), shape="gauss7", scheme="midpoint",
+                           precision=precision, rtol=1e-8, max_sweeps=10))
+        r = np.random.default_rng(seed * 100003 + k + 1)
+        states.append((r.random(P), beams(P, u)))
+    return cfgs, states, 1000

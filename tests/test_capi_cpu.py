@@ -1,0 +1,74 @@
+"""CPU-only checks of the product library: it loads, exports every symbol
+include/pic.h declares, and rejects bad configurations before touching a GPU."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "pic.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*|void)\s+(pic_\w+)\s*\(", txt, re.M)))
+
+
+def test_header_and_binding_agree():
+    import picgolf
+    assert sorted(picgolf.EXPORTS) == header_functions()
+
+
+def test_library_exports_every_symbol():
+    import picgolf
+    L = picgolf.load()
+    for name in header_functions():
+        assert hasattr(L, name), name
+    assert L.pic_version() >= 1
+    assert L.pic_strerror(-4) == b"non-finite charge or field"
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("N", 100, "power of two"),
+    ("N", 8192, "power of two"),
+    ("P", 0, "P must"),
+    ("shape", 3, "shape"),
+    ("scheme", 5, "scheme"),
+    ("precision", 7, "precision"),
+    ("w", -1.0, "w must"),
+    ("max_sweeps", 0, "max_sweeps"),
+])
+def test_create_rejects_bad_config_with_message(field, value, msg):
+    import picgolf
+    from picgolf.core import Config
+    cfg = Config(N=64, P=2048, W=100.0, dt=1 / 384)
+    setattr(cfg, field, value)
+    with pytest.raises(picgolf.PicError) as e:
+        picgolf.Context(cfg)
+    assert e.value.code == -1 and msg in str(e.value)
+
+
+def test_midpoint_needs_gaussian_shape():
+    import picgolf
+    from picgolf.core import Config
+    with pytest.raises(picgolf.PicError, match="Gaussian"):
+        picgolf.Context(Config(N=64, P=64, W=100.0, dt=1e-3, shape="ngp", scheme="midpoint"))
+
+
+def test_quiet_start_matches_oracle(orc):
+    from picgolf.configs import quiet_start
+    x, v = quiet_start(2048)
+    xo, vo = orc.quiet_start(2048)
+    assert np.array_equal(x, xo) and np.array_equal(v, vo)
+
+
+def test_presets_follow_the_scripts():
+    from picgolf import configs
+    cfg, x, v, T = configs.ngp_two_stream()
+    assert (cfg.N, cfg.P, cfg.dt, cfg.W, cfg.w, T) == (128, 8192, 1 / 512, 200.0, 3.125, 1024)
+    cfg, x, v, T = configs.quiet_two_stream()
+    assert (cfg.N, cfg.P, cfg.dt, T) == (64, 2048, 1 / 384, 8192)
+    assert cfg.rtol == 4 * np.finfo(float).eps
+    cfg, x, v, T = configs.scaled_two_stream(P=4096)
+    assert (cfg.N, cfg.dt, cfg.W, T) == (4096, 1 / 24576, 400.0, 2000)
+    assert abs(np.std(v[:2048] + 1) - 0.05) < 0.005
