@@ -773,8 +773,11 @@ static hipError_t allow_lds(K kernel, int bytes) {
                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
+// Largest dynamic LDS any launch here asks for: the FP64 push at N = 4096
+// (E + three limb arrays, 32 bytes per cell).  A refusal is not fatal: a
+// launch that really lacks LDS fails with its own error.
 int configure_kernels() {
-    const int big = 160 * 1024;
+    const int big = 32 * kMaxN;
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t r) { if (r != hipSuccess) e = r; };
     chk(allow_lds(k_mid_push64<6, 0>, big));
@@ -794,7 +797,8 @@ int configure_kernels() {
     chk(allow_lds(k_op_deposit<7>, big));
     chk(allow_lds(k_op_fieldsolve, big));
     (void)hipGetLastError();
-    return e == hipSuccess ? 0 : 1;
+    (void)e;
+    return 0;
 }
 
 }  // namespace pic
