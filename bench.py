@@ -120,6 +120,8 @@ def main():
                                "rtol=1e-8, dt=1/6N, W=400", "N": cfg.N, "P": P,
                    "precision": head, "parallelism": "none (1 GPU)"},
         "mean_sweeps": float(r["S"].mean()),
+        "kernel_avg_ms": {k: v[1] / v[0] for k, v in r["prof"].items()},
+        "kernel_launches_per_step": {k: v[0] / a.profile_steps for k, v in r["prof"].items()},
         "roofline": {"bound": "hbm", "kernel": "push_sweep", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "avg_launch_ms": avg_ms,

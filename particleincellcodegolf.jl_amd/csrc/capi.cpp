@@ -198,7 +198,9 @@ static int build_tables(pic_ctx* c) {
     // so the launch is balanced; enough tiles to cover the chip
     const int lds_push = push_lds_bytes(c->maxN, c->precision);
     int wg_per_cu = std::max(1, std::min(4, (160 * 1024) / std::max(lds_push, 1)));
-    const int64_t target = (int64_t)c->numCU * wg_per_cu * (c->precision == 0 ? 1 : 2);
+    // FP64: as many tiles as fit at once (LDS-bound); mixed: two 1024-thread
+    // blocks per CU, so the per-tile partials (N u64 each) stay small
+    const int64_t target = (int64_t)c->numCU * (c->precision == 0 ? wg_per_cu : 2);
     int64_t tileP = (c->Ptot + target - 1) / target;
     tileP = std::max<int64_t>(64, ((tileP + 63) / 64) * 64);
     tileP = std::min<int64_t>(tileP, (int64_t)1 << 20);
@@ -207,7 +209,7 @@ static int build_tables(pic_ctx* c) {
     while (block > 64 && block / 2 >= tileP) block /= 2;
     c->rb.block_push = block;
     c->rb.lds_push = lds_push;
-    int64_t poff = 0, part0 = 0;
+    int64_t poff = 0, part0 = 0, red0 = 0;
     int32_t goff = 0;
     for (int i = 0; i < n; ++i) {
         const pic_config& k = c->cfg[i];
@@ -240,7 +242,15 @@ static int build_tables(pic_ctx* c) {
             c->htiles.push_back(T);
         }
         part0 += nt * k.N;
-        for (int c0 = 0; c0 < k.N; c0 += 256) c->hreds.push_back(RedDesc{i, c0});
+        // stage-1 reduction: at most 16 groups of tiles per instance
+        int tpg = (int)((nt + 15) / 16);
+        int G = (int)((nt + tpg - 1) / tpg);
+        d.tpg = tpg;
+        d.rgroups = G;
+        d.red0 = red0;
+        red0 += (int64_t)G * k.N;
+        for (int g = 0; g < G; ++g)
+            for (int c0 = 0; c0 < k.N; c0 += 256) c->hreds.push_back(RedDesc{i, c0, g, 0});
         d.chunk0 = (int32_t)c->hchunks.size();
         const int64_t nch = (k.P + kTreeChunk - 1) / kTreeChunk;
         d.nchunks = (int32_t)nch;
@@ -264,7 +274,7 @@ static int build_tables(pic_ctx* c) {
     b.scheme = c->scheme;
     b.precision = c->precision;
     b.max_sweeps = c->max_sweeps;
-    b.block_solve = c->maxN >= 2048 ? 1024 : 256;
+    b.block_solve = 256;
     int rc;
     if ((rc = dalloc(c, &b.insts, n))) return rc;
     if ((rc = dalloc(c, &b.st, n))) return rc;
@@ -289,6 +299,7 @@ static int build_tables(pic_ctx* c) {
             if (c->scheme == PIC_SCHEME_MIDPOINT && (rc = dalloc(c, &b.vit[j], P))) return rc;
         }
         if ((rc = dalloc(c, reinterpret_cast<unsigned __int128**>(&b.part), (size_t)part0))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<unsigned __int128**>(&b.part2), (size_t)red0))) return rc;
     } else {
         for (int j = 0; j < 2; ++j) {
             if ((rc = dalloc(c, &b.Xq[j], P))) return rc;
@@ -297,6 +308,7 @@ static int build_tables(pic_ctx* c) {
             if ((rc = dalloc(c, &b.vf[j], P))) return rc;
         }
         if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.part), (size_t)part0))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.part2), (size_t)red0))) return rc;
     }
     if ((rc = dalloc(c, &b.rho, G))) return rc;
     if ((rc = dalloc(c, &b.E[0], G))) return rc;

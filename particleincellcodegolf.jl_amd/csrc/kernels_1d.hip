@@ -219,59 +219,104 @@ k_lf_second64(const InstDesc* __restrict__ insts, const InstState* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Per-cell sum of the tiles' partials, rounded once to Float64.
+// Charge reduction, stage 1: each block sums one group of tiles' partials for
+// 256 cells (integer sums, so the grouping never changes the result); the
+// solve kernel adds the group sums and rounds once to Float64.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_reduce64(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
+k_reduce64(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
            const RedDesc* __restrict__ reds, const u128* __restrict__ part,
-           double* __restrict__ rho, int k) {
+           u128* __restrict__ part2, int k) {
     const RedDesc D = reds[blockIdx.x];
     const InstDesc I = insts[D.inst];
     const InstState S = st[D.inst];
     if (sweep_over(S, k)) return;
     const int c = D.c0 + threadIdx.x;
     if (c >= I.N) return;
+    const int t0 = D.g * I.tpg;
+    const int t1 = t0 + I.tpg < I.ntiles ? t0 + I.tpg : I.ntiles;
     u128 s = 0;
     const u128* p = part + I.part0 + c;
-    for (int t = 0; t < I.ntiles; ++t) s += p[(int64_t)t * I.N];
-    const double r = fxp_to(s, I.q);
-    rho[I.goff + c] = r;
+    for (int t = t0; t < t1; ++t) s += p[(int64_t)t * I.N];
+    part2[I.red0 + (int64_t)D.g * I.N + c] = s;
 }
 
 // ---------------------------------------------------------------------------
 // Spectral solve in LDS: E = real(ifft((xi = fft(rho) ./ ik; xi[1] *= 0; xi)))
-// with the radix-2 decimation-in-time transform of the CPU restatement.
+// with the radix-2 decimation-in-time transform of the CPU restatement.  The
+// stages are executed four at a time: a group of 16 elements spaced 2^s0
+// apart is closed under stages s0+1..s0+4, so one thread loads it into
+// registers and applies exactly the radix-2 butterflies (same operands, same
+// twiddles, same order per element) before writing back -- three LDS passes
+// per 4096-point transform instead of twelve.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t brev(uint32_t i, int bits) {
     return __brev(i) >> (32 - bits);
 }
+// LDS padding (one slot per 16 doubles) against bank conflicts
+__device__ __forceinline__ int padi(int i) { return i + (i >> 4); }
 
-__device__ void fft_stages(double* re, double* im, const double* twr, const double* twi,
-                           int N, int log2N, bool inverse) {
-    for (int s = 1; s <= log2N; ++s) {
-        const int h = 1 << (s - 1);
-        const int step = N >> s;
-        for (int b = threadIdx.x; b < N / 2; b += blockDim.x) {
-            const int grp = b >> (s - 1);
-            const int j = b & (h - 1);
-            const int i1 = (grp << s) + j;
-            const int i2 = i1 + h;
-            const double wr = twr[j * step];
-            const double wi = inverse ? -twi[j * step] : twi[j * step];
-            const double ar = re[i2], ai = im[i2];
-            const double tr = ar * wr - ai * wi;
-            const double ti = ar * wi + ai * wr;
-            const double ur = re[i1], ui = im[i1];
-            re[i1] = ur + tr;
-            im[i1] = ui + ti;
-            re[i2] = ur - tr;
-            im[i2] = ui - ti;
+template <int M>
+__device__ __forceinline__ void fft_pass(double* re, double* im, const double* twr, const double* twi,
+                                         int N, int log2N, int s0, bool inverse) {
+    constexpr int G = 1 << M;
+    const int groups = N >> M;
+    for (int gi = threadIdx.x; gi < groups; gi += blockDim.x) {
+        const int lo = gi & ((1 << s0) - 1);
+        const int hi = gi >> s0;
+        const int base = (hi << (s0 + M)) + lo;
+        double ar[G], ai[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const int id = padi(base + (j << s0));
+            ar[j] = re[id];
+            ai[j] = im[id];
         }
-        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            const int tsh = log2N - (s0 + t + 1);
+#pragma unroll
+            for (int j1 = 0; j1 < G; ++j1) {
+                if (j1 & (1 << t)) continue;
+                const int j2 = j1 | (1 << t);
+                const int ti = (((j1 & ((1 << t) - 1)) << s0) + lo) << tsh;
+                const double wr = twr[ti];
+                const double wi = inverse ? -twi[ti] : twi[ti];
+                const double br = ar[j2], bi = ai[j2];
+                const double tr = br * wr - bi * wi;
+                const double tq = br * wi + bi * wr;
+                const double ur = ar[j1], ui = ai[j1];
+                ar[j1] = ur + tr;
+                ai[j1] = ui + tq;
+                ar[j2] = ur - tr;
+                ai[j2] = ui - tq;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const int id = padi(base + (j << s0));
+            re[id] = ar[j];
+            im[id] = ai[j];
+        }
+    }
+    __syncthreads();
+}
+
+__device__ void fft_lds(double* re, double* im, const double* twr, const double* twi, int N, int log2N,
+                        bool inverse) {
+    for (int s0 = 0; s0 < log2N; s0 += 4) {
+        const int m = log2N - s0 < 4 ? log2N - s0 : 4;
+        if (m == 4) fft_pass<4>(re, im, twr, twi, N, log2N, s0, inverse);
+        else if (m == 3) fft_pass<3>(re, im, twr, twi, N, log2N, s0, inverse);
+        else if (m == 2) fft_pass<2>(re, im, twr, twi, N, log2N, s0, inverse);
+        else fft_pass<1>(re, im, twr, twi, N, log2N, s0, inverse);
     }
 }
 
-// On entry re/im hold rho in bit-reversed order; on exit re holds E.
+// LDS doubles used by the solve: padded re/im, twiddles, reduction scratch
+__host__ __device__ constexpr int solve_lds_doubles(int N) { return 2 * (N + N / 16) + N + kTreeLanes + 8; }
+
+// On entry re/im hold rho in bit-reversed order; on exit re holds E (padded).
 __device__ void solve_lds(double* re, double* im, double* twr, double* twi,
                           const double* __restrict__ tw, int N, int log2N) {
     const int stride = kMaxN / N;
@@ -280,18 +325,18 @@ __device__ void solve_lds(double* re, double* im, double* twr, double* twi,
         twi[m] = tw[2 * m * stride + 1];
     }
     __syncthreads();
-    fft_stages(re, im, twr, twi, N, log2N, false);
+    fft_lds(re, im, twr, twi, N, log2N, false);
     // xi = rho^ ./ ik with ik = 2 pi i [1, 1:N/2, -N/2+1:-1]; Julia's complex
     // division by (0, k) is (Im * (1/k), -(Re * (1/k))); xi[1] *= 0.
-    constexpr int kMaxPer = kMaxN / 256;
+    constexpr int kMaxPer = kMaxN / kTreeLanes;
     double xr[kMaxPer], xi[kMaxPer];
     int cnt = 0;
     for (int m = threadIdx.x; m < N; m += blockDim.x, ++cnt) {
         const int mp = m == 0 ? 1 : (m <= N / 2 ? m : m - N);
         const double kk = kTwoPi * (double)mp;
         const double t = 1.0 / kk;
-        double a = im[m] * t;
-        double b = -(re[m] * t);
+        double a = im[padi(m)] * t;
+        double b = -(re[padi(m)] * t);
         if (m == 0) { a = a * 0.0; b = b * 0.0; }
         xr[cnt] = a;
         xi[cnt] = b;
@@ -299,25 +344,25 @@ __device__ void solve_lds(double* re, double* im, double* twr, double* twi,
     __syncthreads();
     cnt = 0;
     for (int m = threadIdx.x; m < N; m += blockDim.x, ++cnt) {
-        const uint32_t r = brev((uint32_t)m, log2N);
+        const int r = padi((int)brev((uint32_t)m, log2N));
         re[r] = xr[cnt];
         im[r] = xi[cnt];
     }
     __syncthreads();
-    fft_stages(re, im, twr, twi, N, log2N, true);
+    fft_lds(re, im, twr, twi, N, log2N, true);
     const double scale = 1.0 / (double)N;
-    for (int j = threadIdx.x; j < N; j += blockDim.x) re[j] = re[j] * scale;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) re[padi(j)] = re[padi(j)] * scale;
     __syncthreads();
 }
 
 // isapprox(F, E; rtol, atol) on arrays (src/GaussianFixedPointQuiet.jl:8), as
 // restated on the CPU.  All threads of the block call it.
-__device__ bool block_isapprox(double* red, const double* F, const double* E, int n,
-                               double rtol, double atol) {
-    const double dn = block_norm2(red, n, [&](int64_t i) { return F[i] - E[i]; });
+template <typename FF, typename FE>
+__device__ bool block_isapprox(double* red, FF&& F, FE&& E, int n, double rtol, double atol) {
+    const double dn = block_norm2(red, n, [&](int64_t i) { return F(i) - E(i); });
     if (isfinite(dn)) {
-        const double nf = block_norm2(red, n, [&](int64_t i) { return F[i]; });
-        const double ne = block_norm2(red, n, [&](int64_t i) { return E[i]; });
+        const double nf = block_norm2(red, n, F);
+        const double ne = block_norm2(red, n, E);
         double mx = nf > ne ? nf : ne;
         if (nf != nf || ne != ne) mx = __longlong_as_double(0x7ff8000000000000LL);
         const double tol = rtol * mx;
@@ -325,51 +370,56 @@ __device__ bool block_isapprox(double* red, const double* F, const double* E, in
         if (tol != tol) lim = tol;
         return dn <= lim;
     }
-    __shared__ int ok;
-    if (threadIdx.x == 0) ok = 1;
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const double a = F[i], b = E[i];
-        if (a == b) continue;
-        bool good = isfinite(a) && isfinite(b);
-        if (good) {
-            const double ma = fabs(a) > fabs(b) ? fabs(a) : fabs(b);
-            const double t = rtol * ma;
-            good = fabs(a - b) <= (atol > t ? atol : t);
+    double bad = 0.0;
+    if (threadIdx.x < kTreeLanes)
+        for (int i = threadIdx.x; i < n; i += kTreeLanes) {
+            const double a = F(i), b = E(i);
+            if (a == b) continue;
+            bool good = isfinite(a) && isfinite(b);
+            if (good) {
+                const double ma = fabs(a) > fabs(b) ? fabs(a) : fabs(b);
+                const double t = rtol * ma;
+                good = fabs(a - b) <= (atol > t ? atol : t);
+            }
+            if (!good) bad = 1.0;
         }
-        if (!good) ok = 0;
-    }
-    __syncthreads();
-    const bool r = ok != 0;
-    __syncthreads();
-    return r;
+    return tree256<true>(red, bad) == 0.0;
 }
 
-// One block per instance.  TEST: the midpoint's convergence test between the
-// previous field (e_cur) and the new one -- the test the scripts run at the
-// top of sweep k+1 -- ending the loop on success or at the last sweep.
+// One block of 256 threads per instance.  Adds the charge group sums, solves,
+// and (TEST) runs the midpoint's convergence test between the previous field
+// (e_cur) and the new one -- the test the scripts run at the top of sweep
+// k+1 -- ending the loop on success or at the last sweep.
 template <bool TEST>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(256)
 k_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
-        const double* __restrict__ rho, double* E0, double* E1, float* E32,
+        const void* __restrict__ part2, int precision, double* E0, double* E1, float* E32,
         const double* __restrict__ tw, int k) {
     extern __shared__ double sm[];
     const InstDesc I = insts[blockIdx.x];
     InstState S = st[blockIdx.x];
     if (TEST && sweep_over(S, k)) return;
     const int N = I.N, log2N = I.log2N;
+    const int NP = N + N / 16;
     double* re = sm;
-    double* im = sm + N;
-    double* twr = sm + 2 * N;
-    double* twi = sm + 2 * N + N / 2;
-    double* red = sm + 3 * N;
-    __shared__ int nonfinite;
-    if (threadIdx.x == 0) nonfinite = 0;
-    __syncthreads();
+    double* im = sm + NP;
+    double* twr = sm + 2 * NP;
+    double* twi = twr + N / 2;
+    double* red = twi + N / 2;
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
-        const double r = rho[I.goff + j];
-        if (!isfinite(r)) nonfinite = 1;
-        const uint32_t b = brev((uint32_t)j, log2N);
+        double r;
+        if (precision == 0) {
+            const u128* p = (const u128*)part2 + I.red0 + j;
+            u128 s = 0;
+            for (int g = 0; g < I.rgroups; ++g) s += p[(int64_t)g * N];
+            r = fxp_to(s, I.q);
+        } else {
+            const u64* p = (const u64*)part2 + I.red0 + j;
+            u64 s = 0;
+            for (int g = 0; g < I.rgroups; ++g) s += p[(int64_t)g * N];
+            r = (double)s * I.wq;
+        }
+        const int b = padi((int)brev((uint32_t)j, log2N));
         re[b] = r;
         im[b] = 0.0;
     }
@@ -379,19 +429,19 @@ k_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     const int nxt = TEST ? 1 - cur : 0;
     double* Eo = (nxt ? E1 : E0) + I.goff;
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
-        const double e = re[j];
-        if (!isfinite(e)) nonfinite = 1;
+        const double e = re[padi(j)];
         Eo[j] = e;
         E32[I.goff + j] = (float)e;
     }
+    const double emax = block_maxabs(red, N, [&](int64_t i) { return re[padi((int)i)]; });
     bool conv = false;
     if (TEST) {
         const double* F = (cur ? E1 : E0) + I.goff;
-        conv = block_isapprox(red, F, re, N, I.rtol, 0.0);
+        conv = block_isapprox(red, [&](int64_t i) { return F[i]; }, [&](int64_t i) { return re[padi((int)i)]; },
+                              N, I.rtol, 0.0);
     }
-    __syncthreads();
     if (threadIdx.x == 0) {
-        if (nonfinite) S.err |= 1;
+        if (!isfinite(emax)) S.err |= 1;
         if (TEST) {
             S.e_cur = nxt;
             if (conv || k == I.max_sweeps - 1) S.last_sweep = k;
@@ -412,7 +462,7 @@ __global__ void __launch_bounds__(256)
 k_diag_chunks64(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
                 const ChunkDesc* __restrict__ chunks, const double* V0, const double* V1,
                 double* __restrict__ dpart) {
-    __shared__ double red[kTreeLanes];
+    __shared__ double red[kTreeLanes + 1];
     const ChunkDesc C = chunks[blockIdx.x];
     const InstDesc I = insts[C.inst];
     const InstState S = st[C.inst];
@@ -448,7 +498,7 @@ k_diag_final(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
              double* __restrict__ dpart, const double* E0, const double* E1,
              double* __restrict__ diag, int32_t* __restrict__ sweeps, int64_t cap,
              int scheme) {
-    __shared__ double red[kTreeLanes];
+    __shared__ double red[kTreeLanes + 1];
     const InstDesc I = insts[blockIdx.x];
     InstState S = st[blockIdx.x];
     const double* E = (S.e_cur ? E1 : E0) + I.goff;
@@ -484,6 +534,7 @@ int push_lds_bytes(int N, int precision) {
 }
 
 void launch_mid_push0(const RunBuffers& b, hipStream_t s) {
+    if (b.precision == 1) return launch_mid_push32(b, 0, 0, s);
     const dim3 g(b.ntiles), t(b.block_push);
     u128* part = (u128*)b.part;
     if (b.R == 6)
@@ -495,6 +546,7 @@ void launch_mid_push0(const RunBuffers& b, hipStream_t s) {
 }
 
 void launch_mid_push(const RunBuffers& b, int k, hipStream_t s) {
+    if (b.precision == 1) return launch_mid_push32(b, 1, k, s);
     const dim3 g(b.ntiles), t(b.block_push);
     u128* part = (u128*)b.part;
     if (b.R == 6)
@@ -506,17 +558,20 @@ void launch_mid_push(const RunBuffers& b, int k, hipStream_t s) {
 }
 
 void launch_reduce(const RunBuffers& b, int k, hipStream_t s) {
-    k_reduce64<<<b.nreds, 256, 0, s>>>(b.insts, b.st, b.reds, (const u128*)b.part, b.rho, k);
+    if (b.precision == 1) return launch_reduce32(b, k, s);
+    k_reduce64<<<b.nreds, 256, 0, s>>>(b.insts, b.st, b.reds, (const u128*)b.part, (u128*)b.part2, k);
 }
 
-static int solve_lds_bytes(int N) { return (3 * N + kTreeLanes) * 8; }
+static int solve_lds_bytes(int N) { return solve_lds_doubles(N) * 8; }
 
 void launch_solve(const RunBuffers& b, int k, hipStream_t s) {
     const int lds = solve_lds_bytes(b.maxN);
     if (b.scheme == 1)
-        k_solve<true><<<b.ninst, b.block_solve, lds, s>>>(b.insts, b.st, b.rho, b.E[0], b.E[1], b.E32, b.tw, k);
+        k_solve<true><<<b.ninst, 256, lds, s>>>(b.insts, b.st, b.part2, b.precision, b.E[0], b.E[1], b.E32,
+                                                b.tw, k);
     else
-        k_solve<false><<<b.ninst, b.block_solve, lds, s>>>(b.insts, b.st, b.rho, b.E[0], b.E[1], b.E32, b.tw, k);
+        k_solve<false><<<b.ninst, 256, lds, s>>>(b.insts, b.st, b.part2, b.precision, b.E[0], b.E[1], b.E32,
+                                                 b.tw, k);
 }
 
 void launch_lf_first(const RunBuffers& b, hipStream_t s) {
@@ -544,11 +599,12 @@ void launch_lf_second(const RunBuffers& b, hipStream_t s) {
 }
 
 void launch_diag(const RunBuffers& b, hipStream_t s) {
-    if (b.precision == 0) {
+    if (b.precision == 0)
         k_diag_chunks64<<<b.nchunks, 256, 0, s>>>(b.insts, b.st, b.chunks, b.V[0], b.V[1], b.dpart);
-        k_diag_final<<<b.ninst, 256, 0, s>>>(b.insts, b.st, b.dpart, b.E[0], b.E[1], b.diag, b.sweeps,
-                                             b.diag_cap, b.scheme);
-    }
+    else
+        launch_diag_chunks32(b, s);
+    k_diag_final<<<b.ninst, 256, 0, s>>>(b.insts, b.st, b.dpart, b.E[0], b.E[1], b.diag, b.sweeps,
+                                         b.diag_cap, b.scheme);
 }
 
 // ---------------------------------------------------------------------------
@@ -600,22 +656,23 @@ k_op_reduce(const u128* __restrict__ part, int ntiles, int N, int q, double* __r
     rho[c] = fxp_to(s, q);
 }
 
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(256)
 k_op_fieldsolve(const double* __restrict__ rho, double* __restrict__ E, const double* __restrict__ tw,
                 int N, int log2N) {
     extern __shared__ double sm[];
+    const int NP = N + N / 16;
     double* re = sm;
-    double* im = sm + N;
-    double* twr = sm + 2 * N;
-    double* twi = sm + 2 * N + N / 2;
+    double* im = sm + NP;
+    double* twr = sm + 2 * NP;
+    double* twi = twr + N / 2;
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
-        const uint32_t b = brev((uint32_t)j, log2N);
+        const int b = padi((int)brev((uint32_t)j, log2N));
         re[b] = rho[j];
         im[b] = 0.0;
     }
     __syncthreads();
     solve_lds(re, im, twr, twi, tw, N, log2N);
-    for (int j = threadIdx.x; j < N; j += blockDim.x) E[j] = re[j];
+    for (int j = threadIdx.x; j < N; j += blockDim.x) E[j] = re[padi(j)];
 }
 
 __global__ void k_op_drift_half(double* __restrict__ x, const double* __restrict__ v, int64_t P, double dt) {
@@ -668,14 +725,15 @@ __global__ void k_op_wrap(double* __restrict__ x, int64_t P) {
 __global__ void __launch_bounds__(256)
 k_op_isapprox(const double* __restrict__ F, const double* __restrict__ E, int n, double rtol,
               double atol, int32_t* out) {
-    __shared__ double red[kTreeLanes];
-    const bool r = block_isapprox(red, F, E, n, rtol, atol);
+    __shared__ double red[kTreeLanes + 1];
+    const bool r = block_isapprox(red, [&](int64_t i) { return F[i]; }, [&](int64_t i) { return E[i]; }, n, rtol,
+                                  atol);
     if (threadIdx.x == 0) *out = r ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(256)
 k_op_diag_chunks(const double* __restrict__ v, int64_t P, double* __restrict__ dpart) {
-    __shared__ double red[kTreeLanes];
+    __shared__ double red[kTreeLanes + 1];
     const int64_t start = (int64_t)blockIdx.x * kTreeChunk;
     const int64_t n = P - start < kTreeChunk ? P - start : kTreeChunk;
     const double Pd = (double)P;
@@ -691,7 +749,7 @@ k_op_diag_chunks(const double* __restrict__ v, int64_t P, double* __restrict__ d
 __global__ void __launch_bounds__(256)
 k_op_diag_final(const double* __restrict__ E, int N, double* __restrict__ dpart, int64_t nchunks,
                 int64_t P, double W, double* __restrict__ out) {
-    __shared__ double red[kTreeLanes];
+    __shared__ double red[kTreeLanes + 1];
     const double se = block_chunk_sum(red, N, [&](int64_t i) { return E[i] * E[i]; });
     const double sv = tree_over_partials(red, dpart, nchunks);
     const double sp = tree_over_partials(red, dpart + 1, nchunks);
@@ -721,8 +779,7 @@ void launch_op_deposit(const OpBuffers& o, const double* x, const double* y, int
 }
 
 void launch_op_fieldsolve(const OpBuffers& o, const double* rho, double* E, hipStream_t s) {
-    const int t = o.N >= 2048 ? 1024 : 256;
-    k_op_fieldsolve<<<1, t, solve_lds_bytes(o.N), s>>>(rho, E, o.tw, o.N, o.log2N);
+    k_op_fieldsolve<<<1, 256, solve_lds_bytes(o.N), s>>>(rho, E, o.tw, o.N, o.log2N);
 }
 
 void launch_op_drift_half(double* x, const double* v, int64_t P, double dt, hipStream_t s) {
@@ -765,7 +822,7 @@ void launch_op_diagnostics(const OpBuffers& o, const double* E, const double* v,
     k_op_diag_final<<<1, 256, 0, s>>>(E, o.N, o.dpart, nch, P, W, out4);
 }
 
-bool mixed_available() { return false; }
+bool mixed_available() { return true; }
 
 template <typename K>
 static hipError_t allow_lds(K kernel, int bytes) {
@@ -798,7 +855,7 @@ int configure_kernels() {
     chk(allow_lds(k_op_fieldsolve, big));
     (void)hipGetLastError();
     (void)e;
-    return 0;
+    return configure_kernels_mixed();
 }
 
 }  // namespace pic

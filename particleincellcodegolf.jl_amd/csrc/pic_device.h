@@ -247,31 +247,46 @@ __device__ __forceinline__ u128 limbs_to_u128(u64 l0, u64 l1, u64 l2) {
 // ---------------------------------------------------------------------------
 // Canonical reduction tree (defined in the CPU restatement): within a chunk of
 // at most 4096 values, lane t < 256 sums elements t, t+256, ... in order from
-// +0.0, then a halving tree a[t] += a[t+h] for h = 128..1.  `red` is LDS of
-// 256 doubles; all threads of the block must call it.
+// +0.0, then a halving tree a[t] += a[t+h] for h = 128..1.  The h = 128, 64
+// steps go through LDS, h = 32..1 through wave shuffles (lane t adds lane
+// t+h: the same operands as the LDS form).  Requires blockDim.x >= 256; all
+// threads of the block must call.  `red` is LDS of >= 256 doubles.
 // ---------------------------------------------------------------------------
-template <typename F>
-__device__ __forceinline__ double block_chunk_sum(double* red, int64_t n, F&& elem) {
-    const int t = threadIdx.x;
-    if (t < kTreeLanes) {
-        double s = 0.0;
-        for (int64_t i = t; i < n; i += kTreeLanes) s = s + elem(i);
-        red[t] = s;
-    }
-    __syncthreads();
-    for (int h = kTreeLanes / 2; h > 0; h >>= 1) {
-        if (t < h) red[t] = red[t] + red[t + h];
-        __syncthreads();
-    }
-    double r = red[0];
-    __syncthreads();
-    return r;
-}
-
 __device__ __forceinline__ double nanmax(double a, double b) {
     if (a != a) return a;
     if (b != b) return b;
     return a > b ? a : b;
+}
+
+template <bool MAX>
+__device__ __forceinline__ double tree256(double* red, double v) {
+    const int t = threadIdx.x;
+    if (t < kTreeLanes) red[t] = v;
+    __syncthreads();
+    if (t < 128) red[t] = MAX ? nanmax(red[t], red[t + 128]) : red[t] + red[t + 128];
+    __syncthreads();
+    if (t < 64) {
+        double r = MAX ? nanmax(red[t], red[t + 64]) : red[t] + red[t + 64];
+#pragma unroll
+        for (int h = 32; h > 0; h >>= 1) {
+            const double o = __shfl_down(r, h, 64);
+            r = MAX ? nanmax(r, o) : r + o;
+        }
+        if (t == 0) red[kTreeLanes] = r;
+    }
+    __syncthreads();
+    const double r = red[kTreeLanes];
+    __syncthreads();
+    return r;
+}
+
+template <typename F>
+__device__ __forceinline__ double block_chunk_sum(double* red, int64_t n, F&& elem) {
+    const int t = threadIdx.x;
+    double s = 0.0;
+    if (t < kTreeLanes)
+        for (int64_t i = t; i < n; i += kTreeLanes) s = s + elem(i);
+    return tree256<false>(red, s);
 }
 
 // NaN-propagating max of |x| over n values (order-independent).
@@ -279,20 +294,9 @@ template <typename F>
 __device__ __forceinline__ double block_maxabs(double* red, int64_t n, F&& elem) {
     const int t = threadIdx.x;
     double m = 0.0;
-    for (int64_t i = t; i < n; i += blockDim.x) m = nanmax(m, fabs(elem(i)));
-    if (t < kTreeLanes) red[t] = m;
-    __syncthreads();
-    for (int g = kTreeLanes; g < (int)blockDim.x; g += kTreeLanes) {
-        if (t >= g && t < g + kTreeLanes) red[t - g] = nanmax(red[t - g], m);
-        __syncthreads();
-    }
-    for (int h = kTreeLanes / 2; h > 0; h >>= 1) {
-        if (t < h) red[t] = nanmax(red[t], red[t + h]);
-        __syncthreads();
-    }
-    double r = red[0];
-    __syncthreads();
-    return r;
+    if (t < kTreeLanes)
+        for (int64_t i = t; i < n; i += kTreeLanes) m = nanmax(m, fabs(elem(i)));
+    return tree256<true>(red, m);
 }
 
 // LinearAlgebra.norm (generic_norm2) with the canonical sum, n <= 4096.

@@ -22,7 +22,9 @@ struct InstDesc {
     int32_t chunk0, nchunks;     // diagnostics chunks (4096 particles each)
     int32_t q;                   // fixed-point fraction bits for the FP64 deposit
     int32_t max_sweeps;
+    int32_t rgroups, tpg;        // charge reduction: groups of tpg tiles each
     int32_t pad0;
+    int64_t red0;                // first group partial of this instance
     double w, dt, W, rtol;
     double wq;                   // mixed deposit: w * 2^-30 (charge per weight quantum)
 };
@@ -45,8 +47,8 @@ struct TileDesc {
     int64_t part_off;    // first u128 partial of this tile
 };
 
-struct RedDesc {         // one block of <= 256 cells of one instance
-    int32_t inst, c0;
+struct RedDesc {         // one block of <= 256 cells of one tile group of one instance
+    int32_t inst, c0, g, pad;
 };
 
 struct ChunkDesc {       // one diagnostics chunk
@@ -74,6 +76,7 @@ struct RunBuffers {
     float* Vl[2];
     float* vf[2];
     void* part;        // per-tile charge partials: u128 (FP64) or u64 (mixed)
+    void* part2;       // per-tile-group sums of `part`, same element type
     double* rho;       // packed charge, sum of N over instances
     double* E[2];      // packed field, ping-pong (InstState::e_cur)
     float* E32;        // FP32 copy of the latest field for the mixed gather
@@ -95,6 +98,12 @@ void launch_diag(const RunBuffers& b, hipStream_t s);
 int push_lds_bytes(int N, int precision);
 int configure_kernels();      // raise the dynamic LDS limit; 0 on success
 bool mixed_available();
+
+// kernels_mixed.hip
+void launch_mid_push32(const RunBuffers& b, int mode, int k, hipStream_t s);
+void launch_reduce32(const RunBuffers& b, int k, hipStream_t s);
+void launch_diag_chunks32(const RunBuffers& b, hipStream_t s);
+int configure_kernels_mixed();
 
 // operators (single grid of N cells, contiguous particle arrays)
 struct OpBuffers {
