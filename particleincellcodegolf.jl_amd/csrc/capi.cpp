@@ -35,7 +35,6 @@ struct pic_ctx {
     int numCU = 256;
     std::vector<InstDesc> hinst;
     std::vector<TileDesc> htiles;
-    std::vector<RedDesc> hreds;
     std::vector<ChunkDesc> hchunks;
     RunBuffers rb{};
     std::vector<void*> allocs;
@@ -119,6 +118,25 @@ static void twiddles4096(std::vector<double>& out) {
     }
 }
 
+// Rearranges the 4096-point table per FFT pass of kPassStages stages (see
+// twp_base in kernels_1d.hip): entry (2^s0 - 1) + ((2^t - 1 + jlo) << s0) + lo
+// of the pass starting at stage s0+1 holds W[(jlo*2^s0 + lo) << (11 - s0 - t)].
+static void twiddle_passes(const std::vector<double>& tw, std::vector<double>& out) {
+    const int L = 12;  // log2 of the 4096-point table's transform
+    out.assign(2 * ((1 << L) - 1), 0.0);
+    for (int s0 = 0; s0 < L; s0 += kPassStages) {
+        const int M = std::min(kPassStages, L - s0);
+        for (int t = 0; t < M; ++t)
+            for (int jlo = 0; jlo < (1 << t); ++jlo)
+                for (int lo = 0; lo < (1 << s0); ++lo) {
+                    const int idx = ((jlo << s0) + lo) << (11 - s0 - t);
+                    const int e = ((1 << s0) - 1) + (((1 << t) - 1 + jlo) << s0) + lo;
+                    out[2 * e] = tw[2 * idx];
+                    out[2 * e + 1] = tw[2 * idx + 1];
+                }
+    }
+}
+
 template <typename T>
 static int dalloc(pic_ctx* c, T** p, size_t count) {
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
@@ -192,7 +210,6 @@ static int build_tables(pic_ctx* c) {
     const int n = c->ninst;
     c->hinst.assign(n, InstDesc{});
     c->htiles.clear();
-    c->hreds.clear();
     c->hchunks.clear();
     // tile size: every tile of every instance holds at most tileP particles,
     // so the launch is balanced; enough tiles to cover the chip
@@ -209,7 +226,7 @@ static int build_tables(pic_ctx* c) {
     while (block > 64 && block / 2 >= tileP) block /= 2;
     c->rb.block_push = block;
     c->rb.lds_push = lds_push;
-    int64_t poff = 0, part0 = 0, red0 = 0;
+    int64_t poff = 0, red0 = 0;
     int32_t goff = 0;
     for (int i = 0; i < n; ++i) {
         const pic_config& k = c->cfg[i];
@@ -221,7 +238,6 @@ static int build_tables(pic_ctx* c) {
         d.P = k.P;
         d.poff = poff;
         d.goff = goff;
-        d.part0 = part0;
         d.q = fxp_frac_bits(k.P, k.w);
         d.max_sweeps = k.max_sweeps;
         d.w = k.w;
@@ -238,19 +254,10 @@ static int build_tables(pic_ctx* c) {
             T.local = (int32_t)t;
             T.start = t * tileP;
             T.count = std::min<int64_t>(tileP, k.P - t * tileP);
-            T.part_off = part0 + t * k.N;
             c->htiles.push_back(T);
         }
-        part0 += nt * k.N;
-        // stage-1 reduction: at most 16 groups of tiles per instance
-        int tpg = (int)((nt + 15) / 16);
-        int G = (int)((nt + tpg - 1) / tpg);
-        d.tpg = tpg;
-        d.rgroups = G;
         d.red0 = red0;
-        red0 += (int64_t)G * k.N;
-        for (int g = 0; g < G; ++g)
-            for (int c0 = 0; c0 < k.N; c0 += 256) c->hreds.push_back(RedDesc{i, c0, g, 0});
+        red0 += k.N;
         d.chunk0 = (int32_t)c->hchunks.size();
         const int64_t nch = (k.P + kTreeChunk - 1) / kTreeChunk;
         d.nchunks = (int32_t)nch;
@@ -267,7 +274,6 @@ static int build_tables(pic_ctx* c) {
     RunBuffers& b = c->rb;
     b.ninst = n;
     b.ntiles = (int32_t)c->htiles.size();
-    b.nreds = (int32_t)c->hreds.size();
     b.nchunks = (int32_t)c->hchunks.size();
     b.maxN = c->maxN;
     b.R = c->R;
@@ -279,11 +285,9 @@ static int build_tables(pic_ctx* c) {
     if ((rc = dalloc(c, &b.insts, n))) return rc;
     if ((rc = dalloc(c, &b.st, n))) return rc;
     if ((rc = dalloc(c, &b.tiles, b.ntiles))) return rc;
-    if ((rc = dalloc(c, &b.reds, b.nreds))) return rc;
     if ((rc = dalloc(c, &b.chunks, b.nchunks))) return rc;
     HIPCHK(c, hipMemcpy(b.insts, c->hinst.data(), sizeof(InstDesc) * n, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(b.tiles, c->htiles.data(), sizeof(TileDesc) * b.ntiles, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(b.reds, c->hreds.data(), sizeof(RedDesc) * b.nreds, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(b.chunks, c->hchunks.data(), sizeof(ChunkDesc) * b.nchunks, hipMemcpyHostToDevice));
     std::vector<InstState> st(n);
     for (auto& s : st) {
@@ -298,8 +302,8 @@ static int build_tables(pic_ctx* c) {
             if ((rc = dalloc(c, &b.V[j], P))) return rc;
             if (c->scheme == PIC_SCHEME_MIDPOINT && (rc = dalloc(c, &b.vit[j], P))) return rc;
         }
-        if ((rc = dalloc(c, reinterpret_cast<unsigned __int128**>(&b.part), (size_t)part0))) return rc;
-        if ((rc = dalloc(c, reinterpret_cast<unsigned __int128**>(&b.part2), (size_t)red0))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.racc), 3 * (size_t)red0))) return rc;
+        HIPCHK(c, hipMemset(b.racc, 0, 3 * 8 * (size_t)red0));
     } else {
         for (int j = 0; j < 2; ++j) {
             if ((rc = dalloc(c, &b.Xq[j], P))) return rc;
@@ -307,8 +311,8 @@ static int build_tables(pic_ctx* c) {
             if ((rc = dalloc(c, &b.Vl[j], P))) return rc;
             if ((rc = dalloc(c, &b.vf[j], P))) return rc;
         }
-        if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.part), (size_t)part0))) return rc;
-        if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.part2), (size_t)red0))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.racc), (size_t)red0))) return rc;
+        HIPCHK(c, hipMemset(b.racc, 0, 8 * (size_t)red0));
     }
     if ((rc = dalloc(c, &b.rho, G))) return rc;
     if ((rc = dalloc(c, &b.E[0], G))) return rc;
@@ -365,10 +369,11 @@ int pic_create_batch(pic_ctx** out, const pic_config* cfgs, int32_t n) {
         return fail(PIC_EHIP);
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(PIC_EHIP);
-    std::vector<double> tw;
+    std::vector<double> tw, twp;
     twiddles4096(tw);
-    if ((rc = dalloc(c, &c->tw, tw.size()))) return fail(rc);
-    if (hipMemcpy(c->tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice) != hipSuccess)
+    twiddle_passes(tw, twp);
+    if ((rc = dalloc(c, &c->tw, twp.size()))) return fail(rc);
+    if (hipMemcpy(c->tw, twp.data(), sizeof(double) * twp.size(), hipMemcpyHostToDevice) != hipSuccess)
         return fail(PIC_EHIP);
     if ((rc = build_tables(c))) return fail(rc);
     *out = c;
@@ -493,13 +498,11 @@ static void enqueue_step(pic_ctx* c, hipStream_t s) {
     if (c->scheme == PIC_SCHEME_MIDPOINT) {
         launch_mid_push0(b, s);
         for (int k = 0; k < c->max_sweeps; ++k) {
-            launch_reduce(b, k, s);
             launch_solve(b, k, s);
             launch_mid_push(b, k, s);
         }
     } else {
         launch_lf_first(b, s);
-        launch_reduce(b, 0, s);
         launch_solve(b, 0, s);
         launch_lf_second(b, s);
     }
@@ -606,9 +609,9 @@ int pic_run_profiled(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps, 
     HIPCHK(c, hipSetDevice(c->device));
     int rc = begin_run(c, nsteps);
     if (rc) return rc;
-    enum { K_PUSH0, K_PUSH, K_REDUCE, K_SOLVE, K_LF1, K_LF2, K_DIAG, K_N };
-    const char* names[K_N] = {"push_first", "push_sweep", "reduce", "solve", "leapfrog_first",
-                              "leapfrog_second", "diagnostics"};
+    enum { K_PUSH0, K_PUSH, K_SOLVE, K_LF1, K_LF2, K_DIAG, K_N };
+    const char* names[K_N] = {"push_first", "push_sweep", "solve", "leapfrog_first", "leapfrog_second",
+                              "diagnostics"};
     double total[K_N] = {0};
     int64_t count[K_N] = {0};
     hipEvent_t e0, e1;
@@ -631,7 +634,6 @@ int pic_run_profiled(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps, 
         if (c->scheme == PIC_SCHEME_MIDPOINT) {
             if ((rc = timed(K_PUSH0, [&] { launch_mid_push0(b, c->stream); }))) break;
             for (int k = 0; k < c->max_sweeps; ++k) {
-                if ((rc = timed(K_REDUCE, [&] { launch_reduce(b, k, c->stream); }))) break;
                 if ((rc = timed(K_SOLVE, [&] { launch_solve(b, k, c->stream); }))) break;
                 if ((rc = timed(K_PUSH, [&] { launch_mid_push(b, k, c->stream); }))) break;
                 if ((rc = read_states(c, st))) break;
@@ -641,7 +643,6 @@ int pic_run_profiled(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps, 
             }
         } else {
             if ((rc = timed(K_LF1, [&] { launch_lf_first(b, c->stream); }))) break;
-            if ((rc = timed(K_REDUCE, [&] { launch_reduce(b, 0, c->stream); }))) break;
             if ((rc = timed(K_SOLVE, [&] { launch_solve(b, 0, c->stream); }))) break;
             if ((rc = timed(K_LF2, [&] { launch_lf_second(b, c->stream); }))) break;
         }

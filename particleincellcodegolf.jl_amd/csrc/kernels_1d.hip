@@ -63,11 +63,19 @@ __device__ __forceinline__ void zero_lds(u64* p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
-// Write this block's LDS limbs as normalised u128 partials.
-__device__ __forceinline__ void flush_partials(const u64* __restrict__ acc, int32_t N,
-                                               u128* __restrict__ out) {
+// Adds this block's LDS limbs into the instance's per-cell accumulator: the
+// block total of a cell is normalised to 128 bits and re-split into 43-bit
+// limbs, so device-scope u64 atomics from every block never carry.
+__device__ __forceinline__ void flush_charge64(u128 v, u64* __restrict__ a) {
+    const u64 l0 = (u64)v & kLimbMask, l1 = (u64)(v >> 43) & kLimbMask, l2 = (u64)(v >> 86);
+    if (l0) atomicAdd(a, l0);
+    if (l1) atomicAdd(a + 1, l1);
+    if (l2) atomicAdd(a + 2, l2);
+}
+
+__device__ __forceinline__ void flush_limbs(const u64* __restrict__ acc, int32_t N, u64* __restrict__ racc) {
     for (int c = threadIdx.x; c < N; c += blockDim.x)
-        out[c] = limbs_to_u128(acc[c], acc[N + c], acc[2 * N + c]);
+        flush_charge64(limbs_to_u128(acc[c], acc[N + c], acc[2 * N + c]), racc + 3 * c);
 }
 
 __device__ __forceinline__ bool sweep_over(const InstState& S, int k) {
@@ -86,7 +94,7 @@ __global__ void __launch_bounds__(1024)
 k_mid_push64(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
              const TileDesc* __restrict__ tiles, double* X0, double* X1, double* V0,
              double* V1, double* vit0, double* vit1, const double* __restrict__ E0,
-             const double* __restrict__ E1, u128* __restrict__ part, int k) {
+             const double* __restrict__ E1, u64* __restrict__ racc, int k) {
     extern __shared__ u64 lds[];
     const TileDesc T = tiles[blockIdx.x];
     const InstDesc I = insts[T.inst];
@@ -141,7 +149,7 @@ k_mid_push64(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     }
     if (last) return;
     __syncthreads();
-    flush_partials(acc, N, part + T.part_off);
+    flush_limbs(acc, N, racc + 3 * I.red0);
 }
 
 // ---------------------------------------------------------------------------
@@ -152,7 +160,7 @@ template <int R>
 __global__ void __launch_bounds__(1024)
 k_lf_first64(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
              const TileDesc* __restrict__ tiles, double* X0, double* X1,
-             const double* V0, const double* V1, u128* __restrict__ part) {
+             const double* V0, const double* V1, u64* __restrict__ racc) {
     extern __shared__ u64 lds[];
     const TileDesc T = tiles[blockIdx.x];
     const InstDesc I = insts[T.inst];
@@ -178,12 +186,13 @@ k_lf_first64(const InstDesc* __restrict__ insts, const InstState* __restrict__ s
         }
     }
     __syncthreads();
-    u128* out = part + T.part_off;
+    u64* out = racc + 3 * I.red0;
     if (R == 0) {
         const u128 one = fxp_from(I.w, I.q);
-        for (int c = threadIdx.x; c < N; c += blockDim.x) out[c] = (u128)acc[c] * one;
+        for (int c = threadIdx.x; c < N; c += blockDim.x)
+            if (acc[c]) flush_charge64((u128)acc[c] * one, out + 3 * c);
     } else {
-        flush_partials(acc, N, out);
+        flush_limbs(acc, N, out);
     }
 }
 
@@ -219,29 +228,6 @@ k_lf_second64(const InstDesc* __restrict__ insts, const InstState* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Charge reduction, stage 1: each block sums one group of tiles' partials for
-// 256 cells (integer sums, so the grouping never changes the result); the
-// solve kernel adds the group sums and rounds once to Float64.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-k_reduce64(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
-           const RedDesc* __restrict__ reds, const u128* __restrict__ part,
-           u128* __restrict__ part2, int k) {
-    const RedDesc D = reds[blockIdx.x];
-    const InstDesc I = insts[D.inst];
-    const InstState S = st[D.inst];
-    if (sweep_over(S, k)) return;
-    const int c = D.c0 + threadIdx.x;
-    if (c >= I.N) return;
-    const int t0 = D.g * I.tpg;
-    const int t1 = t0 + I.tpg < I.ntiles ? t0 + I.tpg : I.ntiles;
-    u128 s = 0;
-    const u128* p = part + I.part0 + c;
-    for (int t = t0; t < t1; ++t) s += p[(int64_t)t * I.N];
-    part2[I.red0 + (int64_t)D.g * I.N + c] = s;
-}
-
-// ---------------------------------------------------------------------------
 // Spectral solve in LDS: E = real(ifft((xi = fft(rho) ./ ik; xi[1] *= 0; xi)))
 // with the radix-2 decimation-in-time transform of the CPU restatement.  The
 // stages are executed four at a time: a group of 16 elements spaced 2^s0
@@ -256,11 +242,18 @@ __device__ __forceinline__ uint32_t brev(uint32_t i, int bits) {
 // LDS padding (one slot per 16 doubles) against bank conflicts
 __device__ __forceinline__ int padi(int i) { return i + (i >> 4); }
 
+// Per-pass twiddle table (built on the host from the 4096-point table): the
+// butterfly of local stage t in the pass starting at stage s0+1 uses
+// W4096[(jlo*2^s0 + lo) << (11 - s0 - t)] for any N, jlo < 2^t, lo < 2^s0.
+// Stored lane-contiguous: twp[(2^s0 - 1) + ((2^t - 1 + jlo) << s0) + lo].
+__host__ __device__ constexpr int twp_base(int s0) { return (1 << s0) - 1; }
+
 template <int M>
-__device__ __forceinline__ void fft_pass(double* re, double* im, const double* twr, const double* twi,
-                                         int N, int log2N, int s0, bool inverse) {
+__device__ __forceinline__ void fft_pass(double* re, double* im, const double2* __restrict__ twp,
+                                         int N, int s0, bool inverse) {
     constexpr int G = 1 << M;
     const int groups = N >> M;
+    const double2* tb = twp + twp_base(s0);
     for (int gi = threadIdx.x; gi < groups; gi += blockDim.x) {
         const int lo = gi & ((1 << s0) - 1);
         const int hi = gi >> s0;
@@ -274,14 +267,13 @@ __device__ __forceinline__ void fft_pass(double* re, double* im, const double* t
         }
 #pragma unroll
         for (int t = 0; t < M; ++t) {
-            const int tsh = log2N - (s0 + t + 1);
 #pragma unroll
             for (int j1 = 0; j1 < G; ++j1) {
                 if (j1 & (1 << t)) continue;
                 const int j2 = j1 | (1 << t);
-                const int ti = (((j1 & ((1 << t) - 1)) << s0) + lo) << tsh;
-                const double wr = twr[ti];
-                const double wi = inverse ? -twi[ti] : twi[ti];
+                const double2 w = tb[(((1 << t) - 1 + (j1 & ((1 << t) - 1))) << s0) + lo];
+                const double wr = w.x;
+                const double wi = inverse ? -w.y : w.y;
                 const double br = ar[j2], bi = ai[j2];
                 const double tr = br * wr - bi * wi;
                 const double tq = br * wi + bi * wr;
@@ -302,67 +294,105 @@ __device__ __forceinline__ void fft_pass(double* re, double* im, const double* t
     __syncthreads();
 }
 
-__device__ void fft_lds(double* re, double* im, const double* twr, const double* twi, int N, int log2N,
+__device__ void fft_lds(double* re, double* im, const double2* __restrict__ twp, int N, int log2N,
                         bool inverse) {
-    for (int s0 = 0; s0 < log2N; s0 += 4) {
-        const int m = log2N - s0 < 4 ? log2N - s0 : 4;
-        if (m == 4) fft_pass<4>(re, im, twr, twi, N, log2N, s0, inverse);
-        else if (m == 3) fft_pass<3>(re, im, twr, twi, N, log2N, s0, inverse);
-        else if (m == 2) fft_pass<2>(re, im, twr, twi, N, log2N, s0, inverse);
-        else fft_pass<1>(re, im, twr, twi, N, log2N, s0, inverse);
+    for (int s0 = 0; s0 < log2N; s0 += kPassStages) {
+        const int m = log2N - s0 < kPassStages ? log2N - s0 : kPassStages;
+        if (m == 3) fft_pass<3>(re, im, twp, N, s0, inverse);
+        else if (m == 2) fft_pass<2>(re, im, twp, N, s0, inverse);
+        else fft_pass<1>(re, im, twp, N, s0, inverse);
     }
 }
 
-// LDS doubles used by the solve: padded re/im, twiddles, reduction scratch
-__host__ __device__ constexpr int solve_lds_doubles(int N) { return 2 * (N + N / 16) + N + kTreeLanes + 8; }
+// LDS doubles used by the solve: padded re/im, previous field, reduction scratch
+__host__ __device__ constexpr int solve_lds_doubles(int N) { return 2 * (N + N / 16) + N + 4 * kTreeLanes + 8; }
 
 // On entry re/im hold rho in bit-reversed order; on exit re holds E (padded).
-__device__ void solve_lds(double* re, double* im, double* twr, double* twi,
-                          const double* __restrict__ tw, int N, int log2N) {
-    const int stride = kMaxN / N;
-    for (int m = threadIdx.x; m < N / 2; m += blockDim.x) {
-        twr[m] = tw[2 * m * stride];
-        twi[m] = tw[2 * m * stride + 1];
-    }
-    __syncthreads();
-    fft_lds(re, im, twr, twi, N, log2N, false);
+// Requires blockDim.x == kSolveThreads.
+__device__ void solve_lds(double* re, double* im, const double2* __restrict__ twp, int N, int log2N) {
+    fft_lds(re, im, twp, N, log2N, false);
     // xi = rho^ ./ ik with ik = 2 pi i [1, 1:N/2, -N/2+1:-1]; Julia's complex
     // division by (0, k) is (Im * (1/k), -(Re * (1/k))); xi[1] *= 0.
-    constexpr int kMaxPer = kMaxN / kTreeLanes;
-    double xr[kMaxPer], xi[kMaxPer];
-    int cnt = 0;
-    for (int m = threadIdx.x; m < N; m += blockDim.x, ++cnt) {
-        const int mp = m == 0 ? 1 : (m <= N / 2 ? m : m - N);
-        const double kk = kTwoPi * (double)mp;
-        const double t = 1.0 / kk;
-        double a = im[padi(m)] * t;
-        double b = -(re[padi(m)] * t);
-        if (m == 0) { a = a * 0.0; b = b * 0.0; }
-        xr[cnt] = a;
-        xi[cnt] = b;
+    constexpr int kPer = kMaxN / kSolveThreads;
+    double xr[kPer], xi[kPer];
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+        const int m = threadIdx.x + c * kSolveThreads;
+        if (m < N) {
+            const int mp = m == 0 ? 1 : (m <= N / 2 ? m : m - N);
+            const double kk = kTwoPi * (double)mp;
+            const double t = 1.0 / kk;
+            double a = im[padi(m)] * t;
+            double b = -(re[padi(m)] * t);
+            if (m == 0) { a = a * 0.0; b = b * 0.0; }
+            xr[c] = a;
+            xi[c] = b;
+        }
     }
     __syncthreads();
-    cnt = 0;
-    for (int m = threadIdx.x; m < N; m += blockDim.x, ++cnt) {
-        const int r = padi((int)brev((uint32_t)m, log2N));
-        re[r] = xr[cnt];
-        im[r] = xi[cnt];
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+        const int m = threadIdx.x + c * kSolveThreads;
+        if (m < N) {
+            const int r = padi((int)brev((uint32_t)m, log2N));
+            re[r] = xr[c];
+            im[r] = xi[c];
+        }
     }
     __syncthreads();
-    fft_lds(re, im, twr, twi, N, log2N, true);
+    fft_lds(re, im, twp, N, log2N, true);
     const double scale = 1.0 / (double)N;
     for (int j = threadIdx.x; j < N; j += blockDim.x) re[padi(j)] = re[padi(j)] * scale;
     __syncthreads();
 }
 
+// Julia's norm of three vectors (d = F - E, F, E) from their max-abs values:
+// m == 0 or Inf -> m; n*m^2 finite and m^2 != 0 -> sqrt(sum x^2); otherwise
+// m*sqrt(sum (|x|/m)^2).  Sums use the canonical tree.
+template <typename FD, typename FF, typename FE>
+__device__ void block_norms3(double* red, int n, FD&& d, FF&& F, FE&& E, double& nd, double& nf, double& ne,
+                             double& me) {
+    const int t = threadIdx.x;
+    double md = 0.0, mf = 0.0;
+    me = 0.0;
+    if (t < kTreeLanes)
+        for (int i = t; i < n; i += kTreeLanes) {
+            md = nanmax(md, fabs(d(i)));
+            mf = nanmax(mf, fabs(F(i)));
+            me = nanmax(me, fabs(E(i)));
+        }
+    tree256x3<true>(red, md, mf, me);
+    const double m[3] = {md, mf, me};
+    bool direct[3], scaled[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        direct[q] = m[q] == 0.0 || isinf(m[q]);
+        scaled[q] = !direct[q] && !(isfinite((double)n * m[q] * m[q]) && m[q] * m[q] != 0.0);
+    }
+    double sd = 0.0, sf = 0.0, se = 0.0;
+    if (t < kTreeLanes)
+        for (int i = t; i < n; i += kTreeLanes) {
+            double x = d(i), y = F(i), z = E(i);
+            if (scaled[0]) x = fabs(x) / m[0];
+            if (scaled[1]) y = fabs(y) / m[1];
+            if (scaled[2]) z = fabs(z) / m[2];
+            sd = sd + x * x;
+            sf = sf + y * y;
+            se = se + z * z;
+        }
+    tree256x3<false>(red, sd, sf, se);
+    nd = direct[0] ? m[0] : (scaled[0] ? m[0] * sqrt(sd) : sqrt(sd));
+    nf = direct[1] ? m[1] : (scaled[1] ? m[1] * sqrt(sf) : sqrt(sf));
+    ne = direct[2] ? m[2] : (scaled[2] ? m[2] * sqrt(se) : sqrt(se));
+}
+
 // isapprox(F, E; rtol, atol) on arrays (src/GaussianFixedPointQuiet.jl:8), as
-// restated on the CPU.  All threads of the block call it.
+// restated on the CPU.  All threads of the block call it; me returns max|E|.
 template <typename FF, typename FE>
-__device__ bool block_isapprox(double* red, FF&& F, FE&& E, int n, double rtol, double atol) {
-    const double dn = block_norm2(red, n, [&](int64_t i) { return F(i) - E(i); });
+__device__ bool block_isapprox(double* red, FF&& F, FE&& E, int n, double rtol, double atol, double& me) {
+    double dn, nf, ne;
+    block_norms3(red, n, [&](int i) { return F(i) - E(i); }, F, E, dn, nf, ne, me);
     if (isfinite(dn)) {
-        const double nf = block_norm2(red, n, F);
-        const double ne = block_norm2(red, n, E);
         double mx = nf > ne ? nf : ne;
         if (nf != nf || ne != ne) mx = __longlong_as_double(0x7ff8000000000000LL);
         const double tol = rtol * mx;
@@ -386,15 +416,15 @@ __device__ bool block_isapprox(double* red, FF&& F, FE&& E, int n, double rtol, 
     return tree256<true>(red, bad) == 0.0;
 }
 
-// One block of 256 threads per instance.  Adds the charge group sums, solves,
+// One block of kSolveThreads threads per instance.  Reads the charge, solves,
 // and (TEST) runs the midpoint's convergence test between the previous field
 // (e_cur) and the new one -- the test the scripts run at the top of sweep
 // k+1 -- ending the loop on success or at the last sweep.
 template <bool TEST>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kSolveThreads)
 k_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
-        const void* __restrict__ part2, int precision, double* E0, double* E1, float* E32,
-        const double* __restrict__ tw, int k) {
+        void* __restrict__ racc, int precision, double* E0, double* E1, float* E32,
+        const double2* __restrict__ twp, int k) {
     extern __shared__ double sm[];
     const InstDesc I = insts[blockIdx.x];
     InstState S = st[blockIdx.x];
@@ -403,42 +433,66 @@ k_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     const int NP = N + N / 16;
     double* re = sm;
     double* im = sm + NP;
-    double* twr = sm + 2 * NP;
-    double* twi = twr + N / 2;
-    double* red = twi + N / 2;
-    for (int j = threadIdx.x; j < N; j += blockDim.x) {
-        double r;
-        if (precision == 0) {
-            const u128* p = (const u128*)part2 + I.red0 + j;
-            u128 s = 0;
-            for (int g = 0; g < I.rgroups; ++g) s += p[(int64_t)g * N];
-            r = fxp_to(s, I.q);
-        } else {
-            const u64* p = (const u64*)part2 + I.red0 + j;
-            u64 s = 0;
-            for (int g = 0; g < I.rgroups; ++g) s += p[(int64_t)g * N];
-            r = (double)s * I.wq;
-        }
-        const int b = padi((int)brev((uint32_t)j, log2N));
-        re[b] = r;
-        im[b] = 0.0;
-    }
-    __syncthreads();
-    solve_lds(re, im, twr, twi, tw, N, log2N);
+    double* Fl = sm + 2 * NP;
+    double* red = Fl + N;
     const int cur = TEST ? S.e_cur : 0;
     const int nxt = TEST ? 1 - cur : 0;
+    const double* F = (cur ? E1 : E0) + I.goff;
+    constexpr int kPer = kMaxN / kSolveThreads;
+    // batched loads: all of this thread's cells are in flight together
+    u64 a0[kPer], a1[kPer], a2[kPer];
+    double fv[kPer];
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+        const int j = threadIdx.x + c * kSolveThreads;
+        if (j < N) {
+            if (precision == 0) {
+                const u64* a = (const u64*)racc + 3 * (I.red0 + j);
+                a0[c] = a[0];
+                a1[c] = a[1];
+                a2[c] = a[2];
+            } else {
+                a0[c] = ((const u64*)racc)[I.red0 + j];
+            }
+            if (TEST) fv[c] = F[j];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+        const int j = threadIdx.x + c * kSolveThreads;
+        if (j < N) {
+            double r;
+            if (precision == 0) {
+                u64* a = (u64*)racc + 3 * (I.red0 + j);
+                a[0] = 0;
+                a[1] = 0;
+                a[2] = 0;
+                r = fxp_to(limbs_to_u128(a0[c], a1[c], a2[c]), I.q);
+            } else {
+                ((u64*)racc)[I.red0 + j] = 0;
+                r = (double)a0[c] * I.wq;
+            }
+            const int b = padi((int)brev((uint32_t)j, log2N));
+            re[b] = r;
+            im[b] = 0.0;
+            if (TEST) Fl[j] = fv[c];
+        }
+    }
+    __syncthreads();
+    solve_lds(re, im, twp, N, log2N);
     double* Eo = (nxt ? E1 : E0) + I.goff;
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
         const double e = re[padi(j)];
         Eo[j] = e;
         E32[I.goff + j] = (float)e;
     }
-    const double emax = block_maxabs(red, N, [&](int64_t i) { return re[padi((int)i)]; });
     bool conv = false;
+    double emax;
     if (TEST) {
-        const double* F = (cur ? E1 : E0) + I.goff;
-        conv = block_isapprox(red, [&](int64_t i) { return F[i]; }, [&](int64_t i) { return re[padi((int)i)]; },
-                              N, I.rtol, 0.0);
+        conv = block_isapprox(red, [&](int i) { return Fl[i]; }, [&](int i) { return re[padi(i)]; }, N, I.rtol,
+                              0.0, emax);
+    } else {
+        emax = block_maxabs(red, N, [&](int64_t i) { return re[padi((int)i)]; });
     }
     if (threadIdx.x == 0) {
         if (!isfinite(emax)) S.err |= 1;
@@ -462,16 +516,14 @@ __global__ void __launch_bounds__(256)
 k_diag_chunks64(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
                 const ChunkDesc* __restrict__ chunks, const double* V0, const double* V1,
                 double* __restrict__ dpart) {
-    __shared__ double red[kTreeLanes + 1];
+    __shared__ double red[4 * kTreeLanes + 8];
     const ChunkDesc C = chunks[blockIdx.x];
     const InstDesc I = insts[C.inst];
     const InstState S = st[C.inst];
     // the step's result sits in the buffer of the next parity
-    const double* V = ((S.step + 1) & 1) ? V1 : V0;
-    const double* v = V + I.poff + C.start;
-    const double Pd = (double)I.P;
-    const double s2 = block_chunk_sum(red, C.count, [&](int64_t i) { const double a = v[i]; return a * a; });
-    const double s1 = block_chunk_sum(red, C.count, [&](int64_t i) { return v[i] / Pd; });
+    const double* v = (((S.step + 1) & 1) ? V1 : V0) + I.poff + C.start;
+    double s2, s1;
+    chunk_sums_v(red, C.count, (double)I.P, [&](int64_t i) { return v[i]; }, s2, s1);
     if (threadIdx.x == 0) {
         dpart[2 * blockIdx.x] = s2;
         dpart[2 * blockIdx.x + 1] = s1;
@@ -493,19 +545,54 @@ __device__ double tree_over_partials(double* red, double* vals, int64_t n) {
     return block_chunk_sum(red, n, [&](int64_t i) { return vals[2 * i]; });
 }
 
+// sum(E.^2) over N <= 4096 cells and the canonical tree over the chunk
+// partials of v^2 and v/P; the common case (<= 4096 chunks) in one pass.
+__device__ void diag_sums(double* red, const double* E, int N, double* dp, int64_t nchunks, double& se,
+                          double& sv, double& sp) {
+    if (nchunks > kTreeChunk) {
+        se = block_chunk_sum(red, N, [&](int64_t i) { return E[i] * E[i]; });
+        sv = tree_over_partials(red, dp, nchunks);
+        sp = tree_over_partials(red, dp + 1, nchunks);
+        return;
+    }
+    double a = 0.0, b = 0.0, c = 0.0;
+    if (threadIdx.x < kTreeLanes) {
+        constexpr int K = kTreeChunk / kTreeLanes;
+        double e[K], x[K], y[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const int64_t j = threadIdx.x + (int64_t)i * kTreeLanes;
+            e[i] = j < N ? E[j] : 0.0;
+            x[i] = j < nchunks ? dp[2 * j] : 0.0;
+            y[i] = j < nchunks ? dp[2 * j + 1] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const int64_t j = threadIdx.x + (int64_t)i * kTreeLanes;
+            if (j < N) a = a + e[i] * e[i];
+            if (j < nchunks) {
+                b = b + x[i];
+                c = c + y[i];
+            }
+        }
+    }
+    tree256x3<false>(red, a, b, c);
+    se = a;
+    sv = b;
+    sp = c;
+}
+
 __global__ void __launch_bounds__(256)
 k_diag_final(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
              double* __restrict__ dpart, const double* E0, const double* E1,
              double* __restrict__ diag, int32_t* __restrict__ sweeps, int64_t cap,
              int scheme) {
-    __shared__ double red[kTreeLanes + 1];
+    __shared__ double red[4 * kTreeLanes + 8];
     const InstDesc I = insts[blockIdx.x];
     InstState S = st[blockIdx.x];
     const double* E = (S.e_cur ? E1 : E0) + I.goff;
-    const double se = block_chunk_sum(red, I.N, [&](int64_t i) { return E[i] * E[i]; });
-    double* dp = dpart + 2 * (int64_t)I.chunk0;
-    const double sv = tree_over_partials(red, dp, I.nchunks);
-    const double sp = tree_over_partials(red, dp + 1, I.nchunks);
+    double se, sv, sp;
+    diag_sums(red, E, I.N, dpart + 2 * (int64_t)I.chunk0, I.nchunks, se, sv, sp);
     if (threadIdx.x == 0) {
         const double d1 = (se / (double)I.N) / 2.0;
         const double d2 = ((sv * I.W) / (double)I.P) / 2.0;
@@ -530,13 +617,13 @@ k_diag_final(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
 // ---------------------------------------------------------------------------
 int push_lds_bytes(int N, int precision) {
     if (precision == 0) return N * 8 * 4;  // E + three limb arrays
-    return N * 4 + N * 8;                   // FP32 E + u64 accumulators
+    return 12 * (N + 16);                   // FP32 E + u64 accumulators, with halos
 }
 
 void launch_mid_push0(const RunBuffers& b, hipStream_t s) {
     if (b.precision == 1) return launch_mid_push32(b, 0, 0, s);
     const dim3 g(b.ntiles), t(b.block_push);
-    u128* part = (u128*)b.part;
+    u64* part = (u64*)b.racc;
     if (b.R == 6)
         k_mid_push64<6, 0><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
                                                    b.vit[0], b.vit[1], b.E[0], b.E[1], part, 0);
@@ -548,7 +635,7 @@ void launch_mid_push0(const RunBuffers& b, hipStream_t s) {
 void launch_mid_push(const RunBuffers& b, int k, hipStream_t s) {
     if (b.precision == 1) return launch_mid_push32(b, 1, k, s);
     const dim3 g(b.ntiles), t(b.block_push);
-    u128* part = (u128*)b.part;
+    u64* part = (u64*)b.racc;
     if (b.R == 6)
         k_mid_push64<6, 1><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
                                                    b.vit[0], b.vit[1], b.E[0], b.E[1], part, k);
@@ -557,26 +644,21 @@ void launch_mid_push(const RunBuffers& b, int k, hipStream_t s) {
                                                    b.vit[0], b.vit[1], b.E[0], b.E[1], part, k);
 }
 
-void launch_reduce(const RunBuffers& b, int k, hipStream_t s) {
-    if (b.precision == 1) return launch_reduce32(b, k, s);
-    k_reduce64<<<b.nreds, 256, 0, s>>>(b.insts, b.st, b.reds, (const u128*)b.part, (u128*)b.part2, k);
-}
-
 static int solve_lds_bytes(int N) { return solve_lds_doubles(N) * 8; }
 
 void launch_solve(const RunBuffers& b, int k, hipStream_t s) {
     const int lds = solve_lds_bytes(b.maxN);
     if (b.scheme == 1)
-        k_solve<true><<<b.ninst, 256, lds, s>>>(b.insts, b.st, b.part2, b.precision, b.E[0], b.E[1], b.E32,
-                                                b.tw, k);
+        k_solve<true><<<b.ninst, kSolveThreads, lds, s>>>(b.insts, b.st, b.racc, b.precision, b.E[0], b.E[1], b.E32,
+                                                (const double2*)b.tw, k);
     else
-        k_solve<false><<<b.ninst, 256, lds, s>>>(b.insts, b.st, b.part2, b.precision, b.E[0], b.E[1], b.E32,
-                                                 b.tw, k);
+        k_solve<false><<<b.ninst, kSolveThreads, lds, s>>>(b.insts, b.st, b.racc, b.precision, b.E[0], b.E[1], b.E32,
+                                                 (const double2*)b.tw, k);
 }
 
 void launch_lf_first(const RunBuffers& b, hipStream_t s) {
     const dim3 g(b.ntiles), t(b.block_push);
-    u128* part = (u128*)b.part;
+    u64* part = (u64*)b.racc;
     if (b.R == 0)
         k_lf_first64<0><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1], part);
     else if (b.R == 6)
@@ -643,7 +725,8 @@ k_op_deposit(const double* __restrict__ x, const double* __restrict__ y, int64_t
         const u128 one = fxp_from(w, q);
         for (int c = threadIdx.x; c < N; c += blockDim.x) out[c] = (u128)lds[c] * one;
     } else {
-        flush_partials(lds, N, out);
+        for (int c = threadIdx.x; c < N; c += blockDim.x)
+            out[c] = limbs_to_u128(lds[c], lds[N + c], lds[2 * N + c]);
     }
 }
 
@@ -656,22 +739,20 @@ k_op_reduce(const u128* __restrict__ part, int ntiles, int N, int q, double* __r
     rho[c] = fxp_to(s, q);
 }
 
-__global__ void __launch_bounds__(256)
-k_op_fieldsolve(const double* __restrict__ rho, double* __restrict__ E, const double* __restrict__ tw,
+__global__ void __launch_bounds__(kSolveThreads)
+k_op_fieldsolve(const double* __restrict__ rho, double* __restrict__ E, const double2* __restrict__ twp,
                 int N, int log2N) {
     extern __shared__ double sm[];
     const int NP = N + N / 16;
     double* re = sm;
     double* im = sm + NP;
-    double* twr = sm + 2 * NP;
-    double* twi = twr + N / 2;
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
         const int b = padi((int)brev((uint32_t)j, log2N));
         re[b] = rho[j];
         im[b] = 0.0;
     }
     __syncthreads();
-    solve_lds(re, im, twr, twi, tw, N, log2N);
+    solve_lds(re, im, twp, N, log2N);
     for (int j = threadIdx.x; j < N; j += blockDim.x) E[j] = re[padi(j)];
 }
 
@@ -725,21 +806,20 @@ __global__ void k_op_wrap(double* __restrict__ x, int64_t P) {
 __global__ void __launch_bounds__(256)
 k_op_isapprox(const double* __restrict__ F, const double* __restrict__ E, int n, double rtol,
               double atol, int32_t* out) {
-    __shared__ double red[kTreeLanes + 1];
-    const bool r = block_isapprox(red, [&](int64_t i) { return F[i]; }, [&](int64_t i) { return E[i]; }, n, rtol,
-                                  atol);
+    __shared__ double red[4 * kTreeLanes + 8];
+    double me;
+    const bool r = block_isapprox(red, [&](int i) { return F[i]; }, [&](int i) { return E[i]; }, n, rtol, atol, me);
     if (threadIdx.x == 0) *out = r ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(256)
 k_op_diag_chunks(const double* __restrict__ v, int64_t P, double* __restrict__ dpart) {
-    __shared__ double red[kTreeLanes + 1];
+    __shared__ double red[4 * kTreeLanes + 8];
     const int64_t start = (int64_t)blockIdx.x * kTreeChunk;
     const int64_t n = P - start < kTreeChunk ? P - start : kTreeChunk;
-    const double Pd = (double)P;
     const double* vv = v + start;
-    const double s2 = block_chunk_sum(red, n, [&](int64_t i) { const double a = vv[i]; return a * a; });
-    const double s1 = block_chunk_sum(red, n, [&](int64_t i) { return vv[i] / Pd; });
+    double s2, s1;
+    chunk_sums_v(red, n, (double)P, [&](int64_t i) { return vv[i]; }, s2, s1);
     if (threadIdx.x == 0) {
         dpart[2 * blockIdx.x] = s2;
         dpart[2 * blockIdx.x + 1] = s1;
@@ -749,10 +829,9 @@ k_op_diag_chunks(const double* __restrict__ v, int64_t P, double* __restrict__ d
 __global__ void __launch_bounds__(256)
 k_op_diag_final(const double* __restrict__ E, int N, double* __restrict__ dpart, int64_t nchunks,
                 int64_t P, double W, double* __restrict__ out) {
-    __shared__ double red[kTreeLanes + 1];
-    const double se = block_chunk_sum(red, N, [&](int64_t i) { return E[i] * E[i]; });
-    const double sv = tree_over_partials(red, dpart, nchunks);
-    const double sp = tree_over_partials(red, dpart + 1, nchunks);
+    __shared__ double red[4 * kTreeLanes + 8];
+    double se, sv, sp;
+    diag_sums(red, E, N, dpart, nchunks, se, sv, sp);
     if (threadIdx.x == 0) {
         const double d1 = (se / (double)N) / 2.0;
         const double d2 = ((sv * W) / (double)P) / 2.0;
@@ -779,7 +858,7 @@ void launch_op_deposit(const OpBuffers& o, const double* x, const double* y, int
 }
 
 void launch_op_fieldsolve(const OpBuffers& o, const double* rho, double* E, hipStream_t s) {
-    k_op_fieldsolve<<<1, 256, solve_lds_bytes(o.N), s>>>(rho, E, o.tw, o.N, o.log2N);
+    k_op_fieldsolve<<<1, kSolveThreads, solve_lds_bytes(o.N), s>>>(rho, E, (const double2*)o.tw, o.N, o.log2N);
 }
 
 void launch_op_drift_half(double* x, const double* v, int64_t P, double dt, hipStream_t s) {

@@ -16,28 +16,35 @@
 
 namespace pic {
 
-constexpr float kQuantum = 1073741824.0f;  // 2^30 weight quanta per particle
+// The +-5 window of a centre in cell i0 covers array cells i0-6 .. i0+4 (Julia
+// index i <-> array i-1, i0 in 0..N).  LDS copies of E and of the charge
+// accumulator carry a halo: slot k holds cell (k - kHalo) mod N, so a window
+// is kWin consecutive slots starting at i0 with no wrap-around arithmetic.
+constexpr int kHalo = kStencil32 + 1;
+constexpr int kWin = 2 * kStencil32 + 1;
+__host__ __device__ constexpr int pad_cells(int N) { return N + 2 * kHalo + 4; }
 
 // Stencil of a fixed-point centre: i0 = nearest cell centre (Julia index,
-// N == 0), d = offset of the centre from it in cells, in [-1/2, 1/2).
+// N == 0) and the kWin weights, in 2^-30 quanta of a particle's charge, of the
+// cells i0-6 .. i0+4 (array index = Julia index - 1).
 struct Stencil32 {
     int32_t i0;
-    float w[2 * kStencil32 + 1];
+    float w[kWin];
 };
 
 __device__ __forceinline__ void stencil32(uint32_t cq, int shift, Stencil32& s) {
     const uint64_t t = (uint64_t)cq + (1ull << (shift - 1));
     const uint32_t i0 = (uint32_t)(t >> shift);
     const int32_t dint = (int32_t)(cq - (i0 << shift));
-    const float d = ldexpf((float)dint, -shift);
+    const float d = ldexpf((float)dint, -shift);  // centre - cell centre, in cells
     const float u = d * d;
     s.i0 = (int32_t)i0;
-    s.w[kStencil32] = wA0(u);
-#define PIC_PAIR(J)                                \
-    {                                              \
-        const float a = wA##J(u), b = d * wB##J(u); \
-        s.w[kStencil32 + J] = a + b;               \
-        s.w[kStencil32 - J] = a - b;               \
+    s.w[kStencil32] = qA0(u);
+#define PIC_PAIR(J)                                  \
+    {                                                \
+        const float a = qA##J(u), b = d * qB##J(u);  \
+        s.w[kStencil32 + J] = a + b;                 \
+        s.w[kStencil32 - J] = a - b;                 \
     }
     PIC_PAIR(1)
     PIC_PAIR(2)
@@ -51,12 +58,33 @@ __device__ __forceinline__ uint32_t advance(uint32_t xq, float v_sum, float scal
     return xq + (uint32_t)(int32_t)rintf(v_sum * scale);
 }
 
+// Deposit quanta of window slot J, rounded half up; the |j| >= 4 tails are
+// clamped at zero (fit noise can dip below zero there; nearer cells are never
+// below 1e4 quanta).
+template <int J>
+__device__ __forceinline__ u64 quanta(float wq) {
+    constexpr int d = J - kStencil32;
+    const float r = wq + 0.5f;
+    return (u64)__float2uint_rz((d >= 4 || d <= -4) ? fmaxf(r, 0.0f) : r);
+}
+
+// Adds this block's halo-padded LDS charge into the instance's per-cell
+// accumulator (device-scope integer atomics; cell c collects every slot
+// k == c + kHalo (mod N)).
+__device__ __forceinline__ void flush_charge32(const u64* acc, int N, int NPAD, u64* __restrict__ racc) {
+    for (int c = threadIdx.x; c < N; c += blockDim.x) {
+        u64 v = 0;
+        for (int kk = (c + kHalo) % N; kk < NPAD; kk += N) v += acc[kk];
+        if (v) atomicAdd(racc + c, v);
+    }
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(1024)
 k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
              const TileDesc* __restrict__ tiles, uint32_t* Xq0, uint32_t* Xq1, float* Vh0,
              float* Vh1, float* Vl0, float* Vl1, float* vf0, float* vf1,
-             const float* __restrict__ E32, u64* __restrict__ part, int k) {
+             const float* __restrict__ E32, u64* __restrict__ racc, int k) {
     extern __shared__ u64 lds[];
     const TileDesc T = tiles[blockIdx.x];
     const InstDesc I = insts[T.inst];
@@ -68,12 +96,16 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     if (MODE == 1 && S.last_sweep >= 0 && S.last_sweep < k) return;
     const bool last = (MODE == 1) && (S.last_sweep == k);
     const int32_t N = I.N, nmask = I.nmask;
+    const int NPAD = pad_cells(N);
+    // E*dt in units of 2^30 so that quanta weights give the kick directly
+    const float escale = (float)(I.dt * 0x1p-30);
     u64* acc = lds;
-    float* Eld = reinterpret_cast<float*>(lds + N);
+    float* Eld = reinterpret_cast<float*>(lds + NPAD);
     if (MODE == 1)
-        for (int i = threadIdx.x; i < N; i += blockDim.x) Eld[i] = E32[I.goff + i];
+        for (int i = threadIdx.x; i < NPAD; i += blockDim.x)
+            Eld[i] = E32[I.goff + ((i - kHalo) & nmask)] * escale;
     if (!last)
-        for (int i = threadIdx.x; i < N; i += blockDim.x) acc[i] = 0;
+        for (int i = threadIdx.x; i < NPAD; i += blockDim.x) acc[i] = 0;
     __syncthreads();
 
     const int par = S.step & 1;
@@ -86,7 +118,6 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     const float* vin = (k & 1) ? vf1 : vf0;
     float* vout = ((k + 1) & 1) ? vf1 : vf0;
     const int shift = 32 - I.log2N;
-    const float dtf = (float)I.dt;
     const float cscale = (float)(I.dt * 1073741824.0);  // (v + V)/4*dt in 2^-32 units
     const float xscale = (float)(I.dt * 2147483648.0);  // (v + V)/2*dt in 2^-32 units
     const int64_t base = I.poff + T.start;
@@ -96,14 +127,15 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
         const float vh = Vh[g];
         float vn = vh;
         Stencil32 s;
+        uint32_t cq = 0;
         if (MODE == 1) {
             const float vk = (k == 0) ? vh : vin[g];
-            stencil32(advance(xq, vk + vh, cscale), shift, s);
-            float acc_e = 0.0f;
+            cq = advance(xq, vk + vh, cscale);
+            stencil32(cq, shift, s);
+            const float* e = Eld + s.i0;  // slot of array cell i0-6
+            float kick = 0.0f;
 #pragma unroll
-            for (int j = -kStencil32; j <= kStencil32; ++j)
-                acc_e = fmaf(Eld[(s.i0 + j - 1) & nmask], s.w[j + kStencil32], acc_e);
-            const float kick = acc_e * dtf;
+            for (int j = 0; j < kWin; ++j) kick = fmaf(e[j], s.w[j], kick);
             vn = vh + kick;
             if (last) {
                 // V_new = (V_hi + V_lo) + kick, renormalised (two-sum)
@@ -119,56 +151,33 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
             }
             vout[g] = vn;
         }
-        stencil32(advance(xq, vn + vh, cscale), shift, s);
-#pragma unroll
-        for (int j = -kStencil32; j <= kStencil32; ++j) {
-            const uint32_t qv = (uint32_t)rintf(fmaxf(s.w[j + kStencil32], 0.0f) * kQuantum);
-            if (qv) atomicAdd(&acc[(s.i0 + j - 1) & nmask], (u64)qv);
-        }
+        const uint32_t cq2 = advance(xq, vn + vh, cscale);
+        // once the Picard iterate settles the next centre repeats bit for bit
+        if (MODE == 0 || cq2 != cq) stencil32(cq2, shift, s);
+        u64* a = acc + s.i0;
+#define PIC_DEP(J) atomicAdd(&a[J], quanta<J>(s.w[J]));
+        PIC_DEP(0) PIC_DEP(1) PIC_DEP(2) PIC_DEP(3) PIC_DEP(4) PIC_DEP(5)
+        PIC_DEP(6) PIC_DEP(7) PIC_DEP(8) PIC_DEP(9) PIC_DEP(10)
+#undef PIC_DEP
     }
     if (last) return;
     __syncthreads();
-    u64* out = part + T.part_off;
-    for (int c = threadIdx.x; c < N; c += blockDim.x) out[c] = acc[c];
-}
-
-// Charge reduction, stage 1 (see k_reduce64): u64 sums of weight quanta.
-__global__ void __launch_bounds__(256)
-k_reduce32(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
-           const RedDesc* __restrict__ reds, const u64* __restrict__ part, u64* __restrict__ part2, int k) {
-    const RedDesc D = reds[blockIdx.x];
-    const InstDesc I = insts[D.inst];
-    const InstState S = st[D.inst];
-    if (S.last_sweep >= 0 && S.last_sweep < k) return;
-    const int c = D.c0 + threadIdx.x;
-    if (c >= I.N) return;
-    const int t0 = D.g * I.tpg;
-    const int t1 = t0 + I.tpg < I.ntiles ? t0 + I.tpg : I.ntiles;
-    u64 s = 0;
-    const u64* p = part + I.part0 + c;
-    for (int t = t0; t < t1; ++t) s += p[(int64_t)t * I.N];
-    part2[I.red0 + (int64_t)D.g * I.N + c] = s;
+    flush_charge32(acc, N, NPAD, racc + I.red0);
 }
 
 __global__ void __launch_bounds__(256)
 k_diag_chunks32(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
                 const ChunkDesc* __restrict__ chunks, const float* Vh0, const float* Vh1,
                 const float* Vl0, const float* Vl1, double* __restrict__ dpart) {
-    __shared__ double red[kTreeLanes + 1];
+    __shared__ double red[4 * kTreeLanes + 8];
     const ChunkDesc C = chunks[blockIdx.x];
     const InstDesc I = insts[C.inst];
     const InstState S = st[C.inst];
     const int nxt = (S.step + 1) & 1;
     const float* vh = (nxt ? Vh1 : Vh0) + I.poff + C.start;
     const float* vl = (nxt ? Vl1 : Vl0) + I.poff + C.start;
-    const double Pd = (double)I.P;
-    const double s2 = block_chunk_sum(red, C.count, [&](int64_t i) {
-        const double a = (double)vh[i] + (double)vl[i];
-        return a * a;
-    });
-    const double s1 = block_chunk_sum(red, C.count, [&](int64_t i) {
-        return ((double)vh[i] + (double)vl[i]) / Pd;
-    });
+    double s2, s1;
+    chunk_sums_v(red, C.count, (double)I.P, [&](int64_t i) { return (double)vh[i] + (double)vl[i]; }, s2, s1);
     if (threadIdx.x == 0) {
         dpart[2 * blockIdx.x] = s2;
         dpart[2 * blockIdx.x + 1] = s1;
@@ -177,7 +186,7 @@ k_diag_chunks32(const InstDesc* __restrict__ insts, const InstState* __restrict_
 
 void launch_mid_push32(const RunBuffers& b, int mode, int k, hipStream_t s) {
     const dim3 g(b.ntiles), t(b.block_push);
-    u64* part = (u64*)b.part;
+    u64* part = (u64*)b.racc;
     if (mode == 0)
         k_mid_push32<0><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.Xq[0], b.Xq[1], b.Vh[0], b.Vh[1],
                                                  b.Vl[0], b.Vl[1], b.vf[0], b.vf[1], b.E32, part, 0);
@@ -186,17 +195,13 @@ void launch_mid_push32(const RunBuffers& b, int mode, int k, hipStream_t s) {
                                                  b.Vl[0], b.Vl[1], b.vf[0], b.vf[1], b.E32, part, k);
 }
 
-void launch_reduce32(const RunBuffers& b, int k, hipStream_t s) {
-    k_reduce32<<<b.nreds, 256, 0, s>>>(b.insts, b.st, b.reds, (const u64*)b.part, (u64*)b.part2, k);
-}
-
 void launch_diag_chunks32(const RunBuffers& b, hipStream_t s) {
     k_diag_chunks32<<<b.nchunks, 256, 0, s>>>(b.insts, b.st, b.chunks, b.Vh[0], b.Vh[1], b.Vl[0], b.Vl[1],
                                               b.dpart);
 }
 
 int configure_kernels_mixed() {
-    const int bytes = 12 * kMaxN;
+    const int bytes = 12 * pad_cells(kMaxN);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_mid_push32<0>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_mid_push32<1>),

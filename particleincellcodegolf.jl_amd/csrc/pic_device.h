@@ -312,4 +312,68 @@ __device__ __forceinline__ double block_norm2(double* red, int64_t n, F&& elem) 
     return m * sqrt(s);
 }
 
+// Three canonical trees at once (sum or NaN-propagating max); red holds
+// >= 3*256 + 3 doubles.  Same operands per tree as tree256.
+template <bool MAX>
+__device__ __forceinline__ void tree256x3(double* red, double& a, double& b, double& c) {
+    const int t = threadIdx.x;
+    auto op = [](double x, double y) { return MAX ? nanmax(x, y) : x + y; };
+    if (t < kTreeLanes) {
+        red[t] = a;
+        red[kTreeLanes + t] = b;
+        red[2 * kTreeLanes + t] = c;
+    }
+    __syncthreads();
+    if (t < 128) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) red[q * kTreeLanes + t] = op(red[q * kTreeLanes + t], red[q * kTreeLanes + t + 128]);
+    }
+    __syncthreads();
+    if (t < 64) {
+        double r[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) r[q] = op(red[q * kTreeLanes + t], red[q * kTreeLanes + t + 64]);
+#pragma unroll
+        for (int h = 32; h > 0; h >>= 1)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) r[q] = op(r[q], __shfl_down(r[q], h, 64));
+        if (t == 0) {
+            red[3 * kTreeLanes] = r[0];
+            red[3 * kTreeLanes + 1] = r[1];
+            red[3 * kTreeLanes + 2] = r[2];
+        }
+    }
+    __syncthreads();
+    a = red[3 * kTreeLanes];
+    b = red[3 * kTreeLanes + 1];
+    c = red[3 * kTreeLanes + 2];
+    __syncthreads();
+}
+
+// Canonical chunk sums of v^2 and v/P over one 4096-particle chunk, one pass.
+template <typename FV>
+__device__ __forceinline__ void chunk_sums_v(double* red, int64_t n, double Pd, FV&& vel, double& s2,
+                                             double& s1) {
+    double a = 0.0, b = 0.0, z = 0.0;
+    if (threadIdx.x < kTreeLanes) {
+        double v[kTreeChunk / kTreeLanes];
+#pragma unroll
+        for (int i = 0; i < kTreeChunk / kTreeLanes; ++i) {
+            const int64_t e = threadIdx.x + (int64_t)i * kTreeLanes;
+            v[i] = e < n ? vel(e) : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < kTreeChunk / kTreeLanes; ++i) {
+            const int64_t e = threadIdx.x + (int64_t)i * kTreeLanes;
+            if (e < n) {
+                a = a + v[i] * v[i];
+                b = b + v[i] / Pd;
+            }
+        }
+    }
+    tree256x3<false>(red, a, b, z);
+    s2 = a;
+    s1 = b;
+}
+
 }  // namespace pic

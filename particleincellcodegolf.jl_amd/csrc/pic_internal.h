@@ -9,6 +9,8 @@ namespace pic {
 constexpr int kMaxN = 4096;       // largest grid (one LDS-resident transform)
 constexpr int kTreeLanes = 256;   // canonical reduction: lanes per chunk
 constexpr int kTreeChunk = 4096;  // canonical reduction: elements per chunk
+constexpr int kSolveThreads = 512;  // threads of the one-block-per-instance solve
+constexpr int kPassStages = 3;      // radix-2 FFT stages applied per LDS pass
 
 // One simulation instance (a batch holds many).  Device-resident, read-only
 // during a run.
@@ -16,15 +18,12 @@ struct InstDesc {
     int32_t N, log2N, nmask, R;  // cells, log2(N), N-1, stencil half-width (0: NGP)
     int64_t P;                   // particles
     int64_t poff;                // first particle in the packed particle arrays
-    int64_t part0;               // first u128 partial of this instance
     int32_t goff;                // first cell in the packed grid arrays
     int32_t tile0, ntiles;       // tiles of this instance in the tile list
     int32_t chunk0, nchunks;     // diagnostics chunks (4096 particles each)
     int32_t q;                   // fixed-point fraction bits for the FP64 deposit
     int32_t max_sweeps;
-    int32_t rgroups, tpg;        // charge reduction: groups of tpg tiles each
-    int32_t pad0;
-    int64_t red0;                // first group partial of this instance
+    int64_t red0;                // first cell of this instance in the charge accumulator
     double w, dt, W, rtol;
     double wq;                   // mixed deposit: w * 2^-30 (charge per weight quantum)
 };
@@ -44,11 +43,6 @@ struct TileDesc {
     int32_t local;       // tile index within the instance
     int64_t start;       // first particle (instance-relative)
     int64_t count;
-    int64_t part_off;    // first u128 partial of this tile
-};
-
-struct RedDesc {         // one block of <= 256 cells of one tile group of one instance
-    int32_t inst, c0, g, pad;
 };
 
 struct ChunkDesc {       // one diagnostics chunk
@@ -60,9 +54,8 @@ struct RunBuffers {
     InstDesc* insts;
     InstState* st;
     TileDesc* tiles;
-    RedDesc* reds;
     ChunkDesc* chunks;
-    int32_t ninst, ntiles, nreds, nchunks;
+    int32_t ninst, ntiles, nchunks;
     int32_t maxN, R, scheme, precision, max_sweeps;
     int32_t block_push, block_solve, lds_push;
     // FP64 particle state, ping-pong by step parity, and the Picard iterate
@@ -75,8 +68,7 @@ struct RunBuffers {
     float* Vh[2];
     float* Vl[2];
     float* vf[2];
-    void* part;        // per-tile charge partials: u128 (FP64) or u64 (mixed)
-    void* part2;       // per-tile-group sums of `part`, same element type
+    void* racc;        // per-cell charge accumulator: 3 u64 limbs (FP64) or u64 quanta (mixed)
     double* rho;       // packed charge, sum of N over instances
     double* E[2];      // packed field, ping-pong (InstState::e_cur)
     float* E32;        // FP32 copy of the latest field for the mixed gather
@@ -90,7 +82,6 @@ struct RunBuffers {
 // kernels_1d.hip
 void launch_mid_push0(const RunBuffers& b, hipStream_t s);
 void launch_mid_push(const RunBuffers& b, int k, hipStream_t s);
-void launch_reduce(const RunBuffers& b, int k, hipStream_t s);
 void launch_solve(const RunBuffers& b, int k, hipStream_t s);
 void launch_lf_first(const RunBuffers& b, hipStream_t s);
 void launch_lf_second(const RunBuffers& b, hipStream_t s);
@@ -101,7 +92,6 @@ bool mixed_available();
 
 // kernels_mixed.hip
 void launch_mid_push32(const RunBuffers& b, int mode, int k, hipStream_t s);
-void launch_reduce32(const RunBuffers& b, int k, hipStream_t s);
 void launch_diag_chunks32(const RunBuffers& b, hipStream_t s);
 int configure_kernels_mixed();
 
