@@ -1,0 +1,263 @@
+"""
+    PICGolf
+
+Drop-in operators for the electrostatic PIC scripts of
+jwscook/ParticleInCellCodeGolf.jl, computed on an AMD MI355X by
+`libpicgolf.so` (HIP kernels for gfx950 behind the C ABI in `include/pic.h`).
+
+The scripts' closures map onto these functions:
+
+| script expression                                   | PICGolf                          |
+|-----------------------------------------------------|----------------------------------|
+| `ρ(x, y)` (src/GaussianFixedPointQuiet.jl:7)         | `deposit!(r, ctx, x, y)`         |
+| `n[f(j)] += w` (src/NGPFourier.jl:5)                 | `deposit!(n, ctx, x; shape=:ngp)`|
+| `real(ifft(fft(ρ)./ik))` (src/GaussianFixedPointQuiet.jl:9) | `fieldsolve!(E, ctx, ρ)` |
+| `u()` (src/NGPFourier.jl:2)                          | `drift!(ctx, x, v, dt)`          |
+| `u(); v += E[f.(x)]*dt` (src/NGPFourier.jl:6)        | `push!(ctx, x, v, E, dt)`        |
+| `x .= X .+ (v .+ V)/2*dt` (src/GaussianFixedPointQuiet.jl:9) | `midpoint_position!(x, ctx, X, V, v, dt)` |
+| `v[j] = V[j] + sum(...)*dt` (src/GaussianFixedPointQuiet.jl:10) | `push!(ctx, X, V, x, v, E, dt)` |
+| `x .= mod.(x, 1)` (src/GaussianFixedPointQuiet.jl:10) | `wrap!(ctx, x)`                  |
+| `isapprox(F, E, rtol=l)` (src/GaussianFixedPointQuiet.jl:8) | `isapprox(ctx, F, E; rtol)` |
+| `D[t,1:4]` (src/GaussianFixedPointQuiet.jl:11,13)    | `diagnostics(ctx, E, v)`         |
+| the whole `@gif for t ∈ 1:T` loop                    | `upload!`, `run!`, `download`    |
+
+Every call copies its arrays to the device and back and is synchronous; a
+nonzero status from the library becomes an `error` carrying the library's
+message.  The FP64 operators are bit-identical to the CPU restatement shipped
+with the library (oracle/), which follows the scripts expression by
+expression.
+"""
+module PICGolf
+
+export Context, PicConfig, deposit!, fieldsolve!, drift!, midpoint_position!, wrap!,
+       diagnostics, upload!, download, run!, quiet_start
+
+const libpicgolf = get(ENV, "PICGOLF_LIB", normpath(joinpath(@__DIR__, "..", "lib", "libpicgolf.so")))
+
+const SHAPES = Dict(:ngp => Int32(0), :gauss6 => Int32(6), :gauss7 => Int32(7))
+const SCHEMES = Dict(:leapfrog => Int32(0), :midpoint => Int32(1))
+const PRECISIONS = Dict(:fp64 => Int32(0), :mixed => Int32(1))
+
+"Mirror of `struct pic_config` in include/pic.h."
+struct PicConfig
+    N::Int32
+    shape::Int32
+    scheme::Int32
+    precision::Int32
+    P::Int64
+    W::Float64
+    w::Float64
+    dt::Float64
+    rtol::Float64
+    max_sweeps::Int32
+    device::Int32
+end
+
+"""
+    PicConfig(; N, P, W, dt, w=W/P*N, shape=:gauss7, scheme=:midpoint,
+              precision=:fp64, rtol=1e-8, max_sweeps=10, device=0)
+
+Keyword arguments are the scripts' literals (src/GaussianFixedPointQuiet.jl:1-5).
+"""
+PicConfig(; N, P, W, dt, w=W / P * N, shape=:gauss7, scheme=:midpoint, precision=:fp64,
+          rtol=1e-8, max_sweeps=10, device=0) =
+    PicConfig(Int32(N), SHAPES[shape], SCHEMES[scheme], PRECISIONS[precision], Int64(P),
+              Float64(W), Float64(w), Float64(dt), Float64(rtol), Int32(max_sweeps), Int32(device))
+
+mutable struct Context
+    ptr::Ptr{Cvoid}
+    cfgs::Vector{PicConfig}
+end
+
+lasterror(p::Ptr{Cvoid}) = unsafe_string(ccall((:pic_last_error, libpicgolf), Cstring, (Ptr{Cvoid},), p))
+
+function check(ctx::Context, rc::Cint, what::AbstractString)
+    rc == 0 || error(what, ": ", lasterror(ctx.ptr), " (code ", rc, ")")
+    nothing
+end
+
+"""
+    Context(cfg::PicConfig) / Context(cfgs::Vector{PicConfig})
+
+One device context holding one instance or a batch of independent instances.
+"""
+function Context(cfgs::Vector{PicConfig})
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:pic_create_batch, libpicgolf), Cint, (Ref{Ptr{Cvoid}}, Ptr{PicConfig}, Int32),
+               h, cfgs, length(cfgs))
+    rc == 0 || error("pic_create_batch: ", lasterror(C_NULL), " (code ", rc, ")")
+    ctx = Context(h[], cfgs)
+    finalizer(destroy!, ctx)
+    ctx
+end
+Context(cfg::PicConfig) = Context([cfg])
+
+function destroy!(ctx::Context)
+    if ctx.ptr != C_NULL
+        ccall((:pic_destroy, libpicgolf), Cint, (Ptr{Cvoid},), ctx.ptr)
+        ctx.ptr = C_NULL
+    end
+    nothing
+end
+
+Ncells(ctx::Context) = Int(ctx.cfgs[1].N)
+
+"""
+    deposit!(ρ, ctx, x, y=x; w, shape)
+
+`r .*= 0; for l ∈ d.((x.+y)./2); for k ∈ l; r[k[1]] += k[2]*w; end; end`
+(src/GaussianFixedPointQuiet.jl:7), or with `shape=:ngp` the histogram
+`n[f(j)] += w` of src/NGPFourier.jl:5.  Per-cell sums are exact (integer
+fixed point) and rounded once.
+"""
+function deposit!(ρ::Vector{Float64}, ctx::Context, x::Vector{Float64}, y::Union{Vector{Float64},Nothing}=nothing;
+                  w::Real=ctx.cfgs[1].w, shape::Union{Symbol,Integer}=Int(ctx.cfgs[1].shape))
+    length(ρ) == Ncells(ctx) || throw(DimensionMismatch("ρ must have N = $(Ncells(ctx)) cells"))
+    y === nothing || length(y) == length(x) || throw(DimensionMismatch("x and y differ in length"))
+    s = shape isa Symbol ? SHAPES[shape] : Int32(shape)
+    GC.@preserve ρ x y begin
+        yp = y === nothing || y === x ? C_NULL : pointer(y)
+        rc = ccall((:pic_deposit, libpicgolf), Cint,
+                   (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Int32, Ptr{Float64}),
+                   ctx.ptr, x, yp, length(x), w, s, ρ)
+    end
+    check(ctx, rc, "pic_deposit")
+    ρ
+end
+
+"""
+    fieldsolve!(E, ctx, ρ)
+
+`E .= real.(ifft((ξ .= fft(ρ)./ik; ξ[1] *= 0; ξ)))` (src/GaussianFixedPointQuiet.jl:9).
+"""
+function fieldsolve!(E::Vector{Float64}, ctx::Context, ρ::Vector{Float64})
+    length(E) == length(ρ) == Ncells(ctx) || throw(DimensionMismatch("E and ρ must have N cells"))
+    rc = GC.@preserve E ρ ccall((:pic_fieldsolve, libpicgolf), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                                ctx.ptr, ρ, E)
+    check(ctx, rc, "pic_fieldsolve")
+    E
+end
+
+"`u() = (x .= mod.(x .+ v/2*dt, 1))` (src/NGPFourier.jl:2)."
+function drift!(ctx::Context, x::Vector{Float64}, v::Vector{Float64}, dt::Real)
+    rc = GC.@preserve x v ccall((:pic_drift_half, libpicgolf), Cint,
+                                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Float64), ctx.ptr, x, v, length(x), dt)
+    check(ctx, rc, "pic_drift_half")
+    x
+end
+
+"""
+    push!(ctx, x, v, E, dt; shape)
+
+Leapfrog second half: `u(); v += E[f.(x)]*dt` (src/NGPFourier.jl:6) for NGP,
+`u(); v[j] += sum(k->E[k[1]]*k[2], d(x[j]))*dt` (src/Gaussian.jl:9-10) for the
+Gaussian shape.
+"""
+function Base.push!(ctx::Context, x::Vector{Float64}, v::Vector{Float64}, E::Vector{Float64}, dt::Real;
+                    shape::Union{Symbol,Integer}=Int(ctx.cfgs[1].shape))
+    s = shape isa Symbol ? SHAPES[shape] : Int32(shape)
+    rc = GC.@preserve x v E ccall((:pic_push_leapfrog, libpicgolf), Cint,
+                                  (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Int32),
+                                  ctx.ptr, x, v, E, length(x), dt, s)
+    check(ctx, rc, "pic_push_leapfrog")
+    (x, v)
+end
+
+"""
+    push!(ctx, X, V, x, v, E, dt; shape)
+
+Midpoint gather-kick: `v[j] = V[j] + sum(k->E[k[1]]*k[2], d((x[j]+X[j])/2))*dt`
+(src/GaussianFixedPointQuiet.jl:10).
+"""
+function Base.push!(ctx::Context, X::Vector{Float64}, V::Vector{Float64}, x::Vector{Float64},
+                    v::Vector{Float64}, E::Vector{Float64}, dt::Real;
+                    shape::Union{Symbol,Integer}=Int(ctx.cfgs[1].shape))
+    s = shape isa Symbol ? SHAPES[shape] : Int32(shape)
+    rc = GC.@preserve X V x v E ccall((:pic_push_midpoint, libpicgolf), Cint,
+                                      (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                                       Ptr{Float64}, Int64, Float64, Int32),
+                                      ctx.ptr, X, V, x, v, E, length(X), dt, s)
+    check(ctx, rc, "pic_push_midpoint")
+    v
+end
+
+"`x .= X .+ (v .+ V)/2*dt` (src/GaussianFixedPointQuiet.jl:9)."
+function midpoint_position!(x::Vector{Float64}, ctx::Context, X::Vector{Float64}, V::Vector{Float64},
+                            v::Vector{Float64}, dt::Real)
+    rc = GC.@preserve x X V v ccall((:pic_midpoint_position, libpicgolf), Cint,
+                                    (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Float64),
+                                    ctx.ptr, X, V, v, x, length(X), dt)
+    check(ctx, rc, "pic_midpoint_position")
+    x
+end
+
+"`x .= mod.(x, 1)` (src/GaussianFixedPointQuiet.jl:10)."
+function wrap!(ctx::Context, x::Vector{Float64})
+    rc = GC.@preserve x ccall((:pic_wrap, libpicgolf), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64), ctx.ptr, x, length(x))
+    check(ctx, rc, "pic_wrap")
+    x
+end
+
+"`isapprox(F, E, rtol=rtol, atol=atol)` (src/GaussianFixedPointQuiet.jl:8), computed on the GPU."
+function Base.isapprox(ctx::Context, F::Vector{Float64}, E::Vector{Float64}; rtol::Real, atol::Real=0)
+    out = Ref{Int32}(0)
+    rc = GC.@preserve F E ccall((:pic_isapprox, libpicgolf), Cint,
+                                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int32, Float64, Float64, Ref{Int32}),
+                                ctx.ptr, F, E, length(F), rtol, atol, out)
+    check(ctx, rc, "pic_isapprox")
+    out[] != 0
+end
+
+"`D[t,1:4]` of src/GaussianFixedPointQuiet.jl:11,13 for one state."
+function diagnostics(ctx::Context, E::Vector{Float64}, v::Vector{Float64}; W::Real=ctx.cfgs[1].W)
+    out = zeros(4)
+    rc = GC.@preserve E v out ccall((:pic_diagnostics, libpicgolf), Cint,
+                                    (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Ptr{Float64}),
+                                    ctx.ptr, E, v, length(v), W, out)
+    check(ctx, rc, "pic_diagnostics")
+    out
+end
+
+"Upload (x, v) and the field carried between steps (zeros as in the scripts) for instance `inst` (1-based)."
+function upload!(ctx::Context, x::Vector{Float64}, v::Vector{Float64}, E::Union{Vector{Float64},Nothing}=nothing;
+                 inst::Integer=1)
+    GC.@preserve x v E begin
+        Ep = E === nothing ? C_NULL : pointer(E)
+        rc = ccall((:pic_upload, libpicgolf), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                   ctx.ptr, inst - 1, x, v, Ep)
+    end
+    check(ctx, rc, "pic_upload")
+    ctx
+end
+
+"Download (x, v, E) of instance `inst` (1-based)."
+function download(ctx::Context; inst::Integer=1)
+    c = ctx.cfgs[inst]
+    x, v, E = zeros(c.P), zeros(c.P), zeros(c.N)
+    rc = GC.@preserve x v E ccall((:pic_download, libpicgolf), Cint,
+                                  (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                                  ctx.ptr, inst - 1, x, v, E)
+    check(ctx, rc, "pic_download")
+    (x, v, E)
+end
+
+"""
+    run!(ctx, nsteps) -> (D, sweeps)
+
+Advance every instance `nsteps` steps on the device.  `D[t, :, i]` is the
+script's `D[t, 1:4]` for instance i; `sweeps[t, i]` the Picard sweeps taken.
+"""
+function run!(ctx::Context, nsteps::Integer)
+    n = length(ctx.cfgs)
+    D = zeros(4, nsteps, n)          # C row-major [inst][step][4]
+    S = zeros(Int32, nsteps, n)
+    rc = GC.@preserve D S ccall((:pic_run, libpicgolf), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Int32}),
+                                ctx.ptr, nsteps, D, S)
+    check(ctx, rc, "pic_run")
+    (permutedims(D, (2, 1, 3)), S)
+end
+
+"`x = (bitreverse.(0:P-1) .+ 2.0^63)/2.0^64`, `v = ∓1` by half (src/GaussianFixedPointQuiet.jl:2-3)."
+quiet_start(P::Integer) = ((bitreverse.(0:P-1) .+ 2.0^63) / 2.0^64, collect(1:P .> P / 2) .* 2 .- 1.0)
+
+end # module
