@@ -33,7 +33,140 @@ def parse():
     ap.add_argument("--particles", type=int, default=2 ** 24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of known size (HBM counter calibration)")
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="print a per-kernel table from a rocprofv3 --kernel-trace csv directory")
+    ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
+                    help="per-kernel HBM bytes per dispatch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
+                         "csv directories, scaled by the same counters of --copy-calibration")
     return ap.parse_args()
+
+
+# Dispatches shorter than this did no particle work: the sweep loop runs on the
+# device, and launches past the converged sweep return at their first test.
+PREDICATED_OFF_US = 8.0
+COPY_BYTES = 1 << 30
+
+
+def _short(name):
+    import re
+    return re.sub(r"\(.*", "", name).replace("void ", "")
+
+
+def _csv_rows(d, suffix):
+    import csv
+    import glob
+    rows = []
+    for f in sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True)):
+        with open(f) as fh:
+            rows += list(csv.DictReader(fh))
+    if not rows:
+        sys.exit(f"no *{suffix} under {d}")
+    return rows
+
+
+def summarize_trace(d):
+    """Per-kernel durations from a kernel trace, split into dispatches that did
+    work and the predicated-off ones."""
+    by = {}
+    for r in _csv_rows(d, "kernel_trace.csv"):
+        us = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        by.setdefault(_short(r["Kernel_Name"]), []).append(us)
+    print(f"dispatches >= {PREDICATED_OFF_US} us are 'live' (did particle or field work); shorter ones are "
+          f"predicated off (or tiny by nature, e.g. k_diag_final)")
+    print(f"{'kernel':34s} {'calls':>6s} {'avg_us':>9s} | {'live':>5s} {'live_avg_us':>11s} {'min':>8s} {'max':>8s}"
+          f" | {'off':>4s} {'off_avg_us':>10s}")
+    for k, ds in sorted(by.items(), key=lambda kv: -sum(kv[1])):
+        live = [x for x in ds if x >= PREDICATED_OFF_US]
+        off = [x for x in ds if x < PREDICATED_OFF_US]
+        la = sum(live) / len(live) if live else 0.0
+        oa = sum(off) / len(off) if off else 0.0
+        print(f"{k[:34]:34s} {len(ds):6d} {sum(ds) / len(ds):9.2f} | {len(live):5d} {la:11.2f} "
+              f"{min(live or [0]):8.2f} {max(live or [0]):8.2f} | {len(off):4d} {oa:10.2f}")
+
+
+def _pmc(d, counter):
+    per, name = {}, {}
+    for r in _csv_rows(d, "counter_collection.csv"):
+        if r["Counter_Name"] != counter:
+            continue
+        key = r["Dispatch_Id"]
+        per[key] = per.get(key, 0.0) + float(r["Counter_Value"])
+        name[key] = _short(r["Kernel_Name"])
+    by = {}
+    for key, v in per.items():
+        by.setdefault(name[key], []).append(v)
+    return by
+
+
+def summarize_pmc(fetch_dir, write_dir, copy_fetch, copy_write):
+    """HBM bytes per dispatch: FETCH_SIZE and WRITE_SIZE (KiB) scaled so that
+    the copy kernel of known size reads and writes exactly COPY_BYTES."""
+    cfs = [v for k, l in _pmc(copy_fetch, "FETCH_SIZE").items() for v in l]
+    cws = [v for k, l in _pmc(copy_write, "WRITE_SIZE").items() for v in l]
+    # the copy is the dispatch with the largest fetch (the fill before it reads nothing)
+    cf, cw = max(cfs), max(cws)
+    kf, kw = COPY_BYTES / 1024 / cf, COPY_BYTES / 1024 / cw
+    print(f"copy calibration ({COPY_BYTES} B read + written): FETCH_SIZE {cf:.1f} KiB -> factor {kf:.4f}; "
+          f"WRITE_SIZE {cw:.1f} KiB -> factor {kw:.4f}")
+    fe, wr = _pmc(fetch_dir, "FETCH_SIZE"), _pmc(write_dir, "WRITE_SIZE")
+    print(f"{'kernel':34s} {'disp':>5s} {'live':>5s} {'fetch_MB/live':>14s} {'write_MB/live':>14s} {'total_MB/live':>14s}")
+    for k in sorted(fe, key=lambda k: -sum(fe[k])):
+        f = sorted(fe[k])
+        w = sorted(wr.get(k, [0.0]))
+        # live dispatches move at least 1 MiB; predicated-off ones read descriptors only
+        fl = [x for x in f if x * kf * 1024 > (1 << 20)]
+        wl = w[len(w) - len(fl):] if fl else []
+        if not fl:
+            print(f"{k[:34]:34s} {len(f):5d} {0:5d} {'-':>14s} {'-':>14s} {'-':>14s}")
+            continue
+        fb = sum(fl) / len(fl) * kf * 1024 / 1e6
+        wb = sum(wl) / max(1, len(wl)) * kw * 1024 / 1e6
+        print(f"{k[:34]:34s} {len(f):5d} {len(fl):5d} {fb:14.2f} {wb:14.2f} {fb + wb:14.2f}")
+
+
+def copy_calibration(nbytes=1 << 30, reps=4):
+    """A plain streaming copy of known size on the GPU: dst = -src over nbytes of
+    FP32, reps times.  Reads and writes nbytes per launch; profiling its
+    FETCH_SIZE / WRITE_SIZE gives the scale factor for the HBM traffic counters."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                                           "launches": reps}}))
+
+
+def profiled_traffic(kernel, prec):
+    """HBM bytes per launch of `kernel` as last measured with rocprofv3
+    FETCH_SIZE + WRITE_SIZE (copy-calibrated), from the summaries in profiles/."""
+    import glob
+    import re
+    key = re.escape(f"traffic_bytes_per_launch[{kernel},{prec}]")
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(f) as fh:
+            for line in fh:
+                m = re.match(key + r"\s*=\s*([0-9.eE+]+)", line.strip())
+                if m:
+                    return float(m.group(1))
+    return None
+
+
+def sweep_bytes(S, prec):
+    """Algorithmic HBM bytes of one step's S live push_sweep launches, per particle.
+    Sweep k < S-1 reads X, V (and the iterate v_k for k > 0) and writes v_{k+1};
+    the last sweep reads X, V, v_k (mixed: also the low velocity word) and writes
+    X, V (mixed: V as hi and lo words)."""
+    if prec == "mixed":
+        first, mid, last0, last = 12, 16, 24, 28
+    else:
+        first, mid, last0, last = 24, 32, 32, 40
+    if S <= 1:
+        return last0
+    return first + mid * (S - 2) + last
 
 
 def timed_run(cfg, x, v, steps, warmup, profile_steps):
@@ -77,6 +210,17 @@ def cpu_baseline(cfg, x, v, budget_s=15.0):
 
 def main():
     a = parse()
+    if a.profile_steps < 1:
+        sys.exit("--profile-steps must be >= 1 (the roofline uses the profiled push launches)")
+    if a.copy_calibration:
+        copy_calibration()
+        return
+    if a.summarize_trace:
+        summarize_trace(a.summarize_trace)
+        return
+    if a.summarize_pmc:
+        summarize_pmc(*a.summarize_pmc)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU sharding is a design "
               f"note, not built", file=sys.stderr)
@@ -97,11 +241,11 @@ def main():
     r = results[head]
     cfg = base[0]
     P = cfg.P
-    s = 4 if head == "mixed" else 8
     # dominant kernel: the per-sweep push (gather + next deposit)
     launches, ms = r["prof"]["push_sweep"]
     avg_ms = ms / launches
-    bytes_per_launch = 4 * s * P        # read X, V, v_k and write v_{k+1}
+    S = launches / a.profile_steps
+    bytes_per_launch = P * sweep_bytes(round(S), head) / S
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     out = {
         "metric": "particle-pushes/s on 1 MI355X, mixed vs FP64 plain, at <=1e-6 field-energy error",
@@ -124,7 +268,7 @@ def main():
         "kernel_launches_per_step": {k: v[0] / a.profile_steps for k, v in r["prof"].items()},
         "roofline": {"bound": "hbm", "kernel": "push_sweep", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "avg_launch_ms": avg_ms,
+                     "traffic": profiled_traffic("push_sweep", head), "avg_launch_ms": avg_ms,
                      "bytes_per_launch": bytes_per_launch},
     }
     if "fp64" in results and "mixed" in results:
