@@ -60,6 +60,21 @@ def lib():
         "orc_quiet_start": (None, [c_i64, _dp, _dp]),
         "orc_run": (ctypes.c_int, [c_i32, c_i32, c_i32, c_i64, c_d, c_d, c_d, c_d, c_i32,
                                    _dp, _dp, _dp, c_i64, c_vp, c_vp, c_i32, c_i32]),
+        "orc2d_unimod": (c_d, [c_d, c_d]),
+        "orc2d_unimodi": (c_i64, [c_i64, c_i64]),
+        "orc2d_area_weights": (None, [c_d, c_i32, c_i32, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32),
+                                      ctypes.POINTER(c_d), ctypes.POINTER(c_d)]),
+        "orc2d_gather": (None, [c_i32, c_i32, _dp, _dp, c_d, c_d, c_i32, ctypes.POINTER(c_d),
+                                ctypes.POINTER(c_d)]),
+        "orc2d_boris": (None, [ctypes.POINTER(c_d), ctypes.POINTER(c_d), ctypes.POINTER(c_d), c_d, c_d, c_d,
+                               c_d]),
+        "orc2d_push": (None, [c_i32, c_i32, c_i64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, c_d, c_d, c_i32, c_i32]),
+        "orc2d_deposit": (None, [c_i32, c_i32, c_i64, _dp, _dp, c_d, c_i32, c_i32, _dp]),
+        "orc2d_fft": (None, [c_i32, c_i32, _dp, _dp, c_i32]),
+        "orc2d_fieldsolve": (None, [c_i32, c_i32, _dp, _dp, _dp, c_vp]),
+        "orc2d_diagnostics": (None, [c_i32, c_i32, c_i64, _dp, _dp, _dp, _dp, c_d, _dp]),
+        "orc2d_run": (ctypes.c_int, [c_i32, c_i32, c_i64, c_d, c_d, c_d, c_i32, c_i32, _dp, _dp, _dp, _dp,
+                                     _dp, _dp, _dp, c_i64, c_vp, c_i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -229,3 +244,85 @@ def run(N, shape, scheme, W, w, dt, x, v, nsteps, rtol=1e-8, max_sweeps=10, E=No
     if rc != 0:
         raise ValueError(f"orc_run rejected its arguments (rc={rc})")
     return x, v, E, diag, sweeps
+
+
+# ---- electrostatic 2D3V (src/Electrostatic2D3V.jl) -------------------------
+# Grids are (NY, NX) numpy arrays in C order, i.e. Julia's column-major NX x NY
+# with x fastest; flat index i + NX*j.
+
+def area_weights(z, NZ, textbook=0):
+    c0, c1, w0, w1 = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
+    lib().orc2d_area_weights(float(z), int(NZ), int(textbook), ctypes.byref(c0), ctypes.byref(c1),
+                             ctypes.byref(w0), ctypes.byref(w1))
+    return (c0.value, w0.value), (c1.value, w1.value)
+
+
+def gather2d(Ex, Ey, x, y, textbook=0):
+    Ex, Ey = _f64(Ex), _f64(Ey)
+    NY, NX = Ex.shape
+    ex, ey = ctypes.c_double(), ctypes.c_double()
+    lib().orc2d_gather(NX, NY, Ex.ravel(), Ey.ravel(), float(x), float(y), int(textbook),
+                       ctypes.byref(ex), ctypes.byref(ey))
+    return ex.value, ey.value
+
+
+def boris(vx, vy, vz, Ex, Ey, dt, B0):
+    a, b, c = ctypes.c_double(vx), ctypes.c_double(vy), ctypes.c_double(vz)
+    lib().orc2d_boris(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), float(Ex), float(Ey), float(dt),
+                      float(B0))
+    return a.value, b.value, c.value
+
+
+def push2d(x, y, vx, vy, vz, Ex, Ey, dt, B0, textbook=0, nthreads=1):
+    """Gather, Boris and move every particle; returns new (x, y, vx, vy, vz)."""
+    x, y, vx, vy, vz = (_f64(a).copy() for a in (x, y, vx, vy, vz))
+    Ex, Ey = _f64(Ex), _f64(Ey)
+    NY, NX = Ex.shape
+    lib().orc2d_push(NX, NY, x.size, x, y, vx, vy, vz, Ex.ravel(), Ey.ravel(), float(dt), float(B0),
+                     int(textbook), int(nthreads))
+    return x, y, vx, vy, vz
+
+
+def deposit2d(NX, NY, x, y, w, textbook=0, nthreads=1):
+    x, y = _f64(x), _f64(y)
+    rho = np.zeros((NY, NX))
+    lib().orc2d_deposit(int(NX), int(NY), x.size, x, y, float(w), int(textbook), int(nthreads), rho.ravel())
+    return rho
+
+
+def fft2d(re, im, inverse=0):
+    re, im = _f64(re).copy(), _f64(im).copy()
+    NY, NX = re.shape
+    lib().orc2d_fft(NX, NY, re.ravel(), im.ravel(), int(inverse))
+    return re, im
+
+
+def fieldsolve2d(rho, want_phi=False):
+    rho = _f64(rho)
+    NY, NX = rho.shape
+    Ex, Ey = np.zeros((NY, NX)), np.zeros((NY, NX))
+    phi = np.zeros((NY, NX)) if want_phi else None
+    lib().orc2d_fieldsolve(NX, NY, rho.ravel(), Ex.ravel(), Ey.ravel(), _ptr(phi))
+    return (Ex, Ey, phi) if want_phi else (Ex, Ey)
+
+
+def diagnostics2d(Ex, Ey, vx, vy, w):
+    Ex, Ey, vx, vy = _f64(Ex), _f64(Ey), _f64(vx), _f64(vy)
+    NY, NX = Ex.shape
+    out = np.zeros(5)
+    lib().orc2d_diagnostics(NX, NY, vx.size, Ex.ravel(), Ey.ravel(), vx, vy, float(w), out)
+    return out
+
+
+def run2d(NX, NY, dt, B0, w, x, y, vx, vy, vz, nsteps, ns=2, textbook=0, Ex=None, Ey=None, nthreads=1):
+    """nsteps of the restated 2D3V loop.  Returns (x, y, vx, vy, vz, Ex, Ey, K)
+    with K of shape (nsteps // ns, 5)."""
+    x, y, vx, vy, vz = (_f64(a).copy() for a in (x, y, vx, vy, vz))
+    Ex = np.zeros((NY, NX)) if Ex is None else _f64(Ex).copy()
+    Ey = np.zeros((NY, NX)) if Ey is None else _f64(Ey).copy()
+    K = np.zeros((nsteps // ns, 5))
+    rc = lib().orc2d_run(int(NX), int(NY), x.size, float(dt), float(B0), float(w), int(textbook), int(ns),
+                         x, y, vx, vy, vz, Ex.ravel(), Ey.ravel(), int(nsteps), _ptr(K), int(nthreads))
+    if rc != 0:
+        raise ValueError(f"orc2d_run rejected its arguments (rc={rc})")
+    return x, y, vx, vy, vz, Ex, Ey, K

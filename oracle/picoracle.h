@@ -83,6 +83,30 @@ int orc_run(int32_t N, int32_t shape, int32_t scheme, int64_t P, double W,
             double* diag, int32_t* sweeps, int32_t deposit_mode,
             int32_t nthreads);
 
+/* electrostatic 2D3V (src/Electrostatic2D3V.jl), pic2d.c.  Grids are
+ * column-major NX x NY (x fastest); cells are returned 0-based.  textbook = 0
+ * is the reference's i / i+1 area pairing, 1 the i-1 / i variant. */
+double  orc2d_unimod(double x, double n);
+int64_t orc2d_unimodi(int64_t x, int64_t n);
+void orc2d_area_weights(double z, int32_t NZ, int32_t textbook, int32_t* c0, int32_t* c1,
+                        double* w0, double* w1);
+void orc2d_gather(int32_t NX, int32_t NY, const double* Ex, const double* Ey, double x, double y,
+                  int32_t textbook, double* ex, double* ey);
+void orc2d_boris(double* vx, double* vy, double* vz, double Ex, double Ey, double dt, double B0);
+void orc2d_push(int32_t NX, int32_t NY, int64_t P, double* x, double* y, double* vx, double* vy,
+                double* vz, const double* Ex, const double* Ey, double dt, double B0,
+                int32_t textbook, int32_t nthreads);
+void orc2d_deposit(int32_t NX, int32_t NY, int64_t P, const double* x, const double* y, double w,
+                   int32_t textbook, int32_t nthreads, double* rho);
+void orc2d_fft(int32_t NX, int32_t NY, double* re, double* im, int32_t inverse);
+void orc2d_fieldsolve(int32_t NX, int32_t NY, const double* rho, double* Ex, double* Ey,
+                      double* phi_out);
+void orc2d_diagnostics(int32_t NX, int32_t NY, int64_t P, const double* Ex, const double* Ey,
+                       const double* vx, const double* vy, double w, double* K5);
+int  orc2d_run(int32_t NX, int32_t NY, int64_t P, double dt, double B0, double w,
+               int32_t textbook, int32_t ns, double* x, double* y, double* vx, double* vy,
+               double* vz, double* Ex, double* Ey, int64_t nsteps, double* K, int32_t nthreads);
+
 #ifdef __cplusplus
 }
 #endif
