@@ -137,6 +137,15 @@ static void twiddle_passes(const std::vector<double>& tw, std::vector<double>& o
     }
 }
 
+// The per-pass table for transforms of any power-of-two length <= 4096 (used
+// by the 1D solve and by the 2D row/column transforms).
+std::vector<double> pic::pass_twiddle_table() {
+    std::vector<double> tw, twp;
+    twiddles4096(tw);
+    twiddle_passes(tw, twp);
+    return twp;
+}
+
 template <typename T>
 static int dalloc(pic_ctx* c, T** p, size_t count) {
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);

@@ -236,11 +236,6 @@ k_lf_second64(const InstDesc* __restrict__ insts, const InstState* __restrict__ 
 // twiddles, same order per element) before writing back -- three LDS passes
 // per 4096-point transform instead of twelve.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t brev(uint32_t i, int bits) {
-    return __brev(i) >> (32 - bits);
-}
-// LDS padding (one slot per 16 doubles) against bank conflicts
-__device__ __forceinline__ int padi(int i) { return i + (i >> 4); }
 
 // Per-pass twiddle table (built on the host from the 4096-point table): the
 // butterfly of local stage t in the pass starting at stage s0+1 uses
@@ -265,25 +260,7 @@ __device__ __forceinline__ void fft_pass(double* re, double* im, const double2* 
             ar[j] = re[id];
             ai[j] = im[id];
         }
-#pragma unroll
-        for (int t = 0; t < M; ++t) {
-#pragma unroll
-            for (int j1 = 0; j1 < G; ++j1) {
-                if (j1 & (1 << t)) continue;
-                const int j2 = j1 | (1 << t);
-                const double2 w = tb[(((1 << t) - 1 + (j1 & ((1 << t) - 1))) << s0) + lo];
-                const double wr = w.x;
-                const double wi = inverse ? -w.y : w.y;
-                const double br = ar[j2], bi = ai[j2];
-                const double tr = br * wr - bi * wi;
-                const double tq = br * wi + bi * wr;
-                const double ur = ar[j1], ui = ai[j1];
-                ar[j1] = ur + tr;
-                ai[j1] = ui + tq;
-                ar[j2] = ur - tr;
-                ai[j2] = ui - tq;
-            }
-        }
+        radix2_group<M>(ar, ai, tb, lo, s0, inverse);
 #pragma unroll
         for (int j = 0; j < G; ++j) {
             const int id = padi(base + (j << s0));

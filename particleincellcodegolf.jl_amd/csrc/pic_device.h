@@ -245,6 +245,45 @@ __device__ __forceinline__ u128 limbs_to_u128(u64 l0, u64 l1, u64 l2) {
 }
 
 // ---------------------------------------------------------------------------
+// Radix-2 decimation-in-time butterflies of M consecutive stages (s0+1 ..
+// s0+M) on a group of 2^M elements spaced 2^s0 apart, held in registers:
+// t = b*w (tr = br*wr - bi*wi, ti = br*wi + bi*wr), b = a - t, a = a + t,
+// exactly as the stage-by-stage transform of the CPU restatement.  tb is the
+// per-pass twiddle table at this pass (entry ((2^t - 1 + jlo) << s0) + lo);
+// the inverse transform uses the conjugate twiddles.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t brev(uint32_t i, int bits) {
+    return __brev(i) >> (32 - bits);
+}
+// LDS padding (one slot per 16 doubles) against bank conflicts
+__device__ __forceinline__ int padi(int i) { return i + (i >> 4); }
+
+template <int M>
+__device__ __forceinline__ void radix2_group(double* ar, double* ai, const double2* __restrict__ tb, int lo,
+                                             int s0, bool inverse) {
+    constexpr int G = 1 << M;
+#pragma unroll
+    for (int t = 0; t < M; ++t) {
+#pragma unroll
+        for (int j1 = 0; j1 < G; ++j1) {
+            if (j1 & (1 << t)) continue;
+            const int j2 = j1 | (1 << t);
+            const double2 w = tb[(((1 << t) - 1 + (j1 & ((1 << t) - 1))) << s0) + lo];
+            const double wr = w.x;
+            const double wi = inverse ? -w.y : w.y;
+            const double br = ar[j2], bi = ai[j2];
+            const double tr = br * wr - bi * wi;
+            const double tq = br * wi + bi * wr;
+            const double ur = ar[j1], ui = ai[j1];
+            ar[j1] = ur + tr;
+            ai[j1] = ui + tq;
+            ar[j2] = ur - tr;
+            ai[j2] = ui - tq;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Canonical reduction tree (defined in the CPU restatement): within a chunk of
 // at most 4096 values, lane t < 256 sums elements t, t+256, ... in order from
 // +0.0, then a halving tree a[t] += a[t+h] for h = 128..1.  The h = 128, 64

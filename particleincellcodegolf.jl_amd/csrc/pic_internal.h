@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace pic {
 
 constexpr int kMaxN = 4096;       // largest grid (one LDS-resident transform)
@@ -11,6 +13,10 @@ constexpr int kTreeLanes = 256;   // canonical reduction: lanes per chunk
 constexpr int kTreeChunk = 4096;  // canonical reduction: elements per chunk
 constexpr int kSolveThreads = 512;  // threads of the one-block-per-instance solve
 constexpr int kPassStages = 3;      // radix-2 FFT stages applied per LDS pass
+
+// Per-pass twiddle table (double2 pairs) of the radix-2 transforms, built on
+// the host from W[m] = exp(-2 pi i m/4096).
+std::vector<double> pass_twiddle_table();
 
 // One simulation instance (a batch holds many).  Device-resident, read-only
 // during a run.

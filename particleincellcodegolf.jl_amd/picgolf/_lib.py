@@ -31,6 +31,14 @@ EXPORTS = [
     "pic_diagnostics",
 ]
 
+# every symbol include/pic2d.h declares
+EXPORTS_2D = [
+    "pic2d_create", "pic2d_destroy", "pic2d_last_error", "pic2d_upload", "pic2d_download",
+    "pic2d_run", "pic2d_deposit", "pic2d_fieldsolve", "pic2d_gather", "pic2d_boris", "pic2d_push",
+    "pic2d_diagnostics",
+]
+PAIR_REFERENCE, PAIR_TEXTBOOK = 0, 1
+
 
 class LibraryMissing(RuntimeError):
     pass
@@ -48,6 +56,14 @@ class pic_config(ctypes.Structure):
         ("precision", ctypes.c_int32), ("P", ctypes.c_int64), ("W", ctypes.c_double),
         ("w", ctypes.c_double), ("dt", ctypes.c_double), ("rtol", ctypes.c_double),
         ("max_sweeps", ctypes.c_int32), ("device", ctypes.c_int32),
+    ]
+
+
+class pic2d_config(ctypes.Structure):
+    _fields_ = [
+        ("NX", ctypes.c_int32), ("NY", ctypes.c_int32), ("P", ctypes.c_int64),
+        ("precision", ctypes.c_int32), ("pairing", ctypes.c_int32), ("dt", ctypes.c_double),
+        ("w", ctypes.c_double), ("B0", ctypes.c_double), ("ns", ctypes.c_int32), ("device", ctypes.c_int32),
     ]
 
 
@@ -88,12 +104,27 @@ def load():
         "pic_isapprox": [vp, vp, vp, i32, d, d, ip],
         "pic_diagnostics": [vp, vp, vp, i64, d, vp],
     }
+    sig.update({
+        "pic2d_create": [ctypes.POINTER(vp), ctypes.POINTER(pic2d_config)],
+        "pic2d_destroy": [vp],
+        "pic2d_upload": [vp, vp, vp, vp, vp, vp, vp, vp],
+        "pic2d_download": [vp, vp, vp, vp, vp, vp, vp, vp],
+        "pic2d_run": [vp, i64, vp],
+        "pic2d_deposit": [vp, vp, vp, i64, d, vp],
+        "pic2d_fieldsolve": [vp, vp, vp, vp, vp],
+        "pic2d_gather": [vp, vp, vp, vp, vp, i64, vp, vp],
+        "pic2d_boris": [vp, vp, vp, vp, vp, vp, i64, d, d],
+        "pic2d_push": [vp, vp, vp, vp, vp, vp, i64, vp, vp, d, d],
+        "pic2d_diagnostics": [vp, vp, vp, vp, vp, i64, d, vp],
+    })
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = ctypes.c_int
     L.pic_last_error.argtypes = [vp]
     L.pic_last_error.restype = ctypes.c_char_p
+    L.pic2d_last_error.argtypes = [vp]
+    L.pic2d_last_error.restype = ctypes.c_char_p
     L.pic_strerror.argtypes = [ctypes.c_int]
     L.pic_strerror.restype = ctypes.c_char_p
     L.pic_version.argtypes = []

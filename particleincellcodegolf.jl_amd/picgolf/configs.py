@@ -13,10 +13,16 @@ from a seed) so the GPU run and the CPU restatement start from identical bits.
                      uniform x, v=-1/+1 by half plus N(0, 0.05^2)
   ensemble           512 instances: N in {32..1024}, 16-256 particles per cell,
                      W in [100, 400], beams +-u with u in [0.5, 1.5]
+  es2d3v             256 x 256 grid, 64 particles per cell, electrostatic 2D3V
+                     with B = B0 x (src/Electrostatic2D3V.jl): seeded uniform
+                     positions in (0, 1], vx = -1/+1 by half + N(0, 0.1^2),
+                     vy, vz ~ N(0, 0.1^2); dt = 1/1024, 500 steps, n0 = 4 pi^2,
+                     B0 = sqrt(n0)/4, w = n0/P/(dx dy)
 """
 import numpy as np
 
 from .core import Config
+from .core2d import Config2D
 
 EPS = float(np.finfo(np.float64).eps)
 
@@ -82,3 +88,20 @@ def ensemble(n=512, precision="mixed", seed=0):
         r = np.random.default_rng(seed * 100003 + k + 1)
         states.append((r.random(P), beams(P, u)))
     return cfgs, states, 1000
+
+
+def es2d3v(precision="mixed", seed=0, NX=256, NY=256, ppc=64, pairing="reference"):
+    """The electrostatic 2D3V workload.  Positions are drawn in (0, 1] (the
+    script's unimod convention); n0 = 4 pi^2 and B0 = sqrt(n0)/4 as in
+    src/Electrostatic2D3V.jl:24."""
+    P = NX * NY * ppc
+    n0 = 4 * np.pi ** 2
+    cfg = Config2D(NX=NX, NY=NY, P=P, dt=1 / 1024, w=n0 / P * NX * NY, B0=np.sqrt(n0) / 4, ns=2,
+                   precision=precision, pairing=pairing)
+    rng = np.random.default_rng(seed)
+    x = 1.0 - rng.random(P)          # (0, 1]
+    y = 1.0 - rng.random(P)
+    vx = beams(P) + 0.1 * rng.standard_normal(P)
+    vy = 0.1 * rng.standard_normal(P)
+    vz = 0.1 * rng.standard_normal(P)
+    return cfg, (x, y, vx, vy, vz), 500

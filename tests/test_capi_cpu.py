@@ -9,9 +9,9 @@ import pytest
 from conftest import ROOT
 
 
-def header_functions():
-    txt = open(os.path.join(ROOT, "include", "pic.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*|void)\s+(pic_\w+)\s*\(", txt, re.M)))
+def header_functions(name="pic.h", prefix="pic_"):
+    txt = open(os.path.join(ROOT, "include", name)).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*|void)\s+(" + prefix + r"\w+)\s*\(", txt, re.M)))
 
 
 def test_header_and_binding_agree():
@@ -72,3 +72,46 @@ def test_presets_follow_the_scripts():
     cfg, x, v, T = configs.scaled_two_stream(P=4096)
     assert (cfg.N, cfg.dt, cfg.W, T) == (4096, 1 / 24576, 400.0, 2000)
     assert abs(np.std(v[:2048] + 1) - 0.05) < 0.005
+
+
+def test_header2d_and_binding_agree():
+    import picgolf
+    assert sorted(picgolf.EXPORTS_2D) == header_functions("pic2d.h", "pic2d_")
+
+
+def test_library_exports_every_2d_symbol():
+    import picgolf
+    L = picgolf.load()
+    for name in header_functions("pic2d.h", "pic2d_"):
+        assert hasattr(L, name), name
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("NX", 100, "powers of two"),
+    ("NY", 2048, "powers of two"),
+    ("P", 0, "P must"),
+    ("P", 2 ** 22 + 1, "P must"),
+    ("precision", 7, "precision"),
+    ("pairing", 3, "pairing"),
+    ("dt", 0.0, "dt and w"),
+    ("w", float("nan"), "dt and w"),
+    ("ns", 0, "ns must"),
+])
+def test_create2d_rejects_bad_config_with_message(field, value, msg):
+    import picgolf
+    from picgolf.core2d import Config2D
+    cfg = Config2D(NX=32, NY=16, P=4096, dt=1 / 256, w=1.0, B0=1.0)
+    setattr(cfg, field, value)
+    with pytest.raises(picgolf.PicError) as e:
+        picgolf.Context2D(cfg)
+    assert e.value.code == -1 and msg in str(e.value)
+
+
+def test_es2d3v_preset_follows_the_script():
+    from picgolf import configs
+    cfg, (x, y, vx, vy, vz), T = configs.es2d3v(NX=32, NY=32, ppc=16)
+    n0 = 4 * np.pi ** 2
+    assert (cfg.NX, cfg.NY, cfg.P, cfg.dt, cfg.ns, T) == (32, 32, 32 * 32 * 16, 1 / 1024, 2, 500)
+    assert cfg.B0 == np.sqrt(n0) / 4 and cfg.w == n0 / cfg.P * 32 * 32
+    assert np.all((x > 0) & (x <= 1)) and np.all((y > 0) & (y <= 1))
+    assert abs(np.std(vx[: cfg.P // 2] + 1) - 0.1) < 0.01 and abs(np.std(vz) - 0.1) < 0.01

@@ -1,0 +1,203 @@
+"""GPU parity of the 2D3V path (src/Electrostatic2D3V.jl) through the C ABI
+in include/pic2d.h: every FP64 operator and whole FP64 runs bit for bit
+against the CPU restatement; the mixed path against the FP64 path with its
+error stated; plasma physics on the mixed path."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N0 = 4 * np.pi ** 2
+
+
+@pytest.fixture(scope="module")
+def pg():
+    import picgolf
+    picgolf.load()
+    return picgolf
+
+
+def same(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+def ctx(pg, NX, NY, P, pairing="reference", precision="fp64", dt=1 / 1024, ns=2):
+    from picgolf.core2d import Config2D
+    return pg.Context2D(Config2D(NX=NX, NY=NY, P=P, dt=dt, w=N0 / P * NX * NY, B0=np.sqrt(N0) / 4, ns=ns,
+                                 precision=precision, pairing=pairing))
+
+
+def state(P, seed):
+    rng = np.random.default_rng(seed)
+    x, y = 1.0 - rng.random(P), 1.0 - rng.random(P)
+    vx = np.where(np.arange(P) < P // 2, -1.0, 1.0) + 0.1 * rng.standard_normal(P)
+    return x, y, vx, 0.1 * rng.standard_normal(P), 0.1 * rng.standard_normal(P)
+
+
+@pytest.mark.parametrize("pairing", ["reference", "textbook"])
+@pytest.mark.parametrize("NX,NY", [(8, 8), (32, 16), (16, 64)])
+def test_gather_boris_push_bitwise(pg, orc, pairing, NX, NY):
+    P = 3000
+    x, y, vx, vy, vz = state(P, NX * NY)
+    x[:3] = [1.0, 0.5, 1.0 / NX]            # z == 1 and exact cell edges
+    rng = np.random.default_rng(1)
+    Ex, Ey = rng.standard_normal((NY, NX)), rng.standard_normal((NY, NX))
+    tb = 1 if pairing == "textbook" else 0
+    with ctx(pg, NX, NY, P, pairing) as c:
+        ex, ey = c.gather(Ex, Ey, x, y)
+        ref = np.array([orc.gather2d(Ex, Ey, x[p], y[p], tb) for p in range(P)])
+        assert same(ex, ref[:, 0]) and same(ey, ref[:, 1])
+        a, b, d = vx.copy(), vy.copy(), vz.copy()
+        c.boris(a, b, d, ex, ey, dt=0.01, B0=2.0)
+        ref = np.array([orc.boris(vx[p], vy[p], vz[p], ex[p], ey[p], 0.01, 2.0) for p in range(P)])
+        assert same(a, ref[:, 0]) and same(b, ref[:, 1]) and same(d, ref[:, 2])
+        got = [arr.copy() for arr in (x, y, vx, vy, vz)]
+        c.push(*got, Ex, Ey, dt=1 / 64, B0=np.pi / 2)
+        ref = orc.push2d(x, y, vx, vy, vz, Ex, Ey, 1 / 64, np.pi / 2, textbook=tb)
+        for g, r in zip(got, ref):
+            assert same(g, r)
+
+
+@pytest.mark.parametrize("pairing", ["reference", "textbook"])
+@pytest.mark.parametrize("NX,NY,P", [(8, 8, 1), (32, 16, 5000), (256, 256, 200000)])
+def test_deposit_bitwise(pg, orc, pairing, NX, NY, P):
+    x, y, *_ = state(P, P)
+    w = 0.37 * NX * NY / P
+    tb = 1 if pairing == "textbook" else 0
+    with ctx(pg, NX, NY, P, pairing) as c:
+        got = c.deposit(x, y, w=w)
+    assert same(got, orc.deposit2d(NX, NY, x, y, w, textbook=tb, nthreads=8))
+
+
+@pytest.mark.parametrize("NX,NY", [(8, 8), (16, 32), (256, 256), (1024, 8), (8, 1024), (512, 128)])
+def test_fieldsolve_bitwise(pg, orc, NX, NY):
+    rng = np.random.default_rng(NX + 7 * NY)
+    rho = 3.0 + rng.standard_normal((NY, NX))
+    with ctx(pg, NX, NY, 64) as c:
+        Ex, Ey, phi = c.fieldsolve(rho, want_phi=True)
+        Ex2, Ey2 = c.fieldsolve(rho)
+    rEx, rEy, rphi = orc.fieldsolve2d(rho, want_phi=True)
+    assert same(Ex, rEx) and same(Ey, rEy) and same(phi, rphi)
+    assert same(Ex2, rEx) and same(Ey2, rEy)
+
+
+def test_fieldsolve_nonfinite_is_reported(pg):
+    rho = np.ones((16, 16))
+    rho[3, 4] = np.inf
+    with ctx(pg, 16, 16, 64) as c:
+        with pytest.raises(pg.PicError) as e:
+            c.fieldsolve(rho)
+    assert e.value.code == -4
+
+
+@pytest.mark.parametrize("P", [1, 4095, 4097, 70000])
+def test_diagnostics_bitwise(pg, orc, P):
+    rng = np.random.default_rng(P)
+    Ex, Ey = rng.standard_normal((64, 128)), rng.standard_normal((64, 128))
+    vx, vy = rng.standard_normal(P), rng.standard_normal(P)
+    with ctx(pg, 128, 64, max(P, 1)) as c:
+        got = c.diagnostics(Ex, Ey, vx, vy, w=0.3)
+    assert same(got, orc.diagnostics2d(Ex, Ey, vx, vy, 0.3))
+
+
+@pytest.mark.parametrize("pairing", ["reference", "textbook"])
+def test_fp64_run_bitwise(pg, orc, pairing):
+    NX, NY, P, T = 32, 16, 32 * 16 * 16, 40
+    x, y, vx, vy, vz = state(P, 5)
+    tb = 1 if pairing == "textbook" else 0
+    with ctx(pg, NX, NY, P, pairing, dt=1 / 128) as c:
+        c.upload(x, y, vx, vy, vz)
+        K1 = c.run(T // 2 + 1)          # odd split: diagnostics rows follow the step count
+        K2 = c.run(T - (T // 2 + 1))
+        got = c.download()
+    ref = orc.run2d(NX, NY, 1 / 128, np.sqrt(N0) / 4, N0 / P * NX * NY, x, y, vx, vy, vz, T, ns=2,
+                    textbook=tb, nthreads=8)
+    for g, r in zip(got, ref[:7]):
+        assert same(g, r)
+    assert same(np.vstack([K1, K2]), ref[7])
+
+
+def test_fp64_run_bitwise_full_size(pg, orc, capsys):
+    # the 256 x 256, 64 particles-per-cell workload, a few steps
+    from picgolf import configs
+    cfg, st, _ = configs.es2d3v(precision="fp64")
+    T = 4
+    with pg.Context2D(cfg) as c:
+        c.upload(*st)
+        K = c.run(T)
+        got = c.download()
+    ref = orc.run2d(cfg.NX, cfg.NY, cfg.dt, cfg.B0, cfg.w, *st, T, ns=2, nthreads=16)
+    for g, r in zip(got, ref[:7]):
+        assert same(g, r)
+    assert same(K, ref[7])
+
+
+# Field-energy tolerance of the mixed path against FP64 over 200 steps.  The
+# reference pairing's shape jumps by two cells when a particle crosses a cell
+# edge, so a position difference of ~1e-7 cell (FP32 velocity, 2^-20 weight
+# quanta) moves a whole particle's charge for the few particles that sit that
+# close to an edge; the textbook (continuous) shape has no such events.
+K1_TOL = {"reference": 1e-3, "textbook": 1e-5}
+
+
+@pytest.mark.parametrize("pairing", ["reference", "textbook"])
+def test_mixed_against_fp64(pg, pairing):
+    """Field energy, kinetic energy and momentum of the mixed path against the
+    FP64 path over 200 steps of the full 256 x 256 workload."""
+    from picgolf import configs
+    T = 200
+    out = {}
+    for prec in ("fp64", "mixed"):
+        cfg, st, _ = configs.es2d3v(precision=prec, pairing=pairing)
+        with pg.Context2D(cfg) as c:
+            c.upload(*st)
+            out[prec] = (c.run(T), c.download())
+    Kf, Km = out["fp64"][0], out["mixed"][0]
+    rel = np.abs(Km - Kf) / np.abs(Kf)
+    print(f"2D3V {pairing} mixed vs FP64 over {T} steps: K1 max rel err {rel[:, 0].max():.3e}, "
+          f"K2 {rel[:, 1].max():.3e}, |dK4| {np.abs(Km[:, 3] - Kf[:, 3]).max():.3e}")
+    assert rel[:, 0].max() < K1_TOL[pairing]     # field energy
+    assert rel[:, 1].max() < 1e-7                # kinetic energy
+    # momentum along B: conserved by both to roundoff of the velocity format
+    assert np.abs(Km[:, 3] - Km[0, 3]).max() < 1e-7
+    assert np.abs(Kf[:, 3] - Kf[0, 3]).max() < 1e-13
+
+
+def test_mixed_plasma_frequencies(pg):
+    # cold lattice, textbook pairing: omega_p along B, upper hybrid across it
+    from test_oracle_2d import cold_lattice, oscillation_omega
+    NX = NY = 64
+    B0 = np.sqrt(N0) / 4
+    for axis, expect in ((0, np.sqrt(N0)), (1, np.sqrt(N0 + B0 ** 2))):
+        x, y = cold_lattice(NX, NY, 2, 1e-4, axis)
+        P = x.size
+        z = np.zeros(P)
+        with ctx(pg, NX, NY, P, "textbook", "mixed", dt=1 / 400, ns=1) as c:
+            c.upload(np.where(x > 1, x - 1, x), np.where(y > 1, y - 1, y), z, z, z)
+            K = c.run(600)
+        assert oscillation_omega(K[:, 0], 1 / 400) == pytest.approx(expect, rel=0.005)
+
+
+def test_mixed_upload_download_round_trip(pg):
+    P = 10000
+    x, y, vx, vy, vz = state(P, 9)
+    x[0], y[0] = 1.0, 1.0
+    with ctx(pg, 64, 32, P, precision="mixed") as c:
+        c.upload(x, y, vx, vy, vz)
+        gx, gy, gvx, gvy, gvz, Ex, Ey = c.download()
+    assert np.max(np.abs(gx - x)) <= 2 ** -33 and np.max(np.abs(gy - y)) <= 2 ** -33
+    assert gx[0] == 1.0 and gy[0] == 1.0
+    assert np.all((gx > 0) & (gx <= 1))
+    assert np.max(np.abs(gvx - vx) / np.abs(vx)) < 2 ** -24
+    assert not Ex.any() and not Ey.any()
+
+
+def test_upload_rejects_positions_outside_unit_box(pg):
+    P = 16
+    x, y, vx, vy, vz = state(P, 3)
+    x[5] = 0.0
+    with ctx(pg, 16, 16, P) as c:
+        with pytest.raises(pg.PicError) as e:
+            c.upload(x, y, vx, vy, vz)
+    assert e.value.code == -1 and "outside" in str(e.value)
