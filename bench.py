@@ -32,6 +32,8 @@ def parse():
     ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"])
     ap.add_argument("--particles", type=int, default=2 ** 24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the ensemble and 2D3V workloads reported beside the headline")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (HBM counter calibration)")
@@ -184,6 +186,64 @@ def timed_run(cfg, x, v, steps, warmup, profile_steps):
     return (t1 - t0), D[0], S[0], prof
 
 
+def bench_ensemble(steps=10, warmup=2, n=512):
+    """The batched ensemble: n independent two-stream instances with uneven N,
+    P, W and beam speed in one context, mixed and FP64.  Field-energy error of
+    the mixed run against the FP64 run over the timed steps (linear phase)."""
+    import numpy as np
+    import picgolf
+    from picgolf import configs
+    res = {}
+    for prec in ("mixed", "fp64"):
+        cfgs, states, _ = configs.ensemble(n, precision=prec)
+        with picgolf.Context(cfgs) as c:
+            for i, (x, v) in enumerate(states):
+                c.upload(x, v, inst=i)
+            c.run(warmup)
+            t0 = time.perf_counter()
+            D, S = c.run(steps)
+            secs = time.perf_counter() - t0
+        Ptot = sum(k.P for k in cfgs)
+        res[prec] = dict(pushes=Ptot * steps / secs, ms=secs / steps * 1e3, D=D, S=S, Ptot=Ptot)
+    f, m = res["fp64"], res["mixed"]
+    rel = np.abs(m["D"][:, :, 0] - f["D"][:, :, 0]) / np.abs(f["D"][:, :, 0])
+    return {"workload": f"ensemble of {n} two-stream instances, N in 32..1024, 16-256 particles per cell, "
+                        f"Gaussian +-7 midpoint, rtol=1e-8 (sum P = {m['Ptot']})",
+            "mixed_pushes_per_s": m["pushes"], "fp64_pushes_per_s": f["pushes"],
+            "mixed_over_fp64": m["pushes"] / f["pushes"], "mixed_ms_per_step": m["ms"],
+            "fp64_ms_per_step": f["ms"], "mean_sweeps_mixed": float(m["S"].mean()),
+            "field_energy_rel_err_mixed_vs_fp64": float(rel.max()), "steps": steps, "warmup": warmup}
+
+
+def bench_2d(steps=500):
+    """The electrostatic 2D3V workload (256 x 256, 64 particles per cell), the
+    whole 500-step run timed, mixed and FP64."""
+    import numpy as np
+    import picgolf
+    from picgolf import configs
+    res = {}
+    for prec in ("mixed", "fp64"):
+        cfg, st, _ = configs.es2d3v(precision=prec)
+        with picgolf.Context2D(cfg) as c:
+            c.upload(*st)
+            c.run(2)                 # compile-free warm start: graphs captured, caches warm
+            c.upload(*st)
+            t0 = time.perf_counter()
+            K = c.run(steps)
+            secs = time.perf_counter() - t0
+        res[prec] = dict(pushes=cfg.P * steps / secs, ms=secs / steps * 1e3, K=K, P=cfg.P)
+    f, m = res["fp64"], res["mixed"]
+    rel = np.abs(m["K"][:, 0] - f["K"][:, 0]) / np.abs(f["K"][:, 0])
+    return {"workload": "electrostatic 2D3V 256x256, 64 particles per cell (4194304), B0 = pi/2 along x, "
+                        "dt=1/1024, reference area pairing",
+            "mixed_pushes_per_s": m["pushes"], "fp64_pushes_per_s": f["pushes"],
+            "mixed_over_fp64": m["pushes"] / f["pushes"], "mixed_ms_per_step": m["ms"],
+            "fp64_ms_per_step": f["ms"], "steps": steps,
+            "field_energy_rel_err_mixed_vs_fp64_max": float(rel.max()),
+            "field_energy_rel_err_mixed_vs_fp64_first_100_steps": float(rel[:50].max()),
+            "momentum_x_drift_mixed": float(np.abs(m["K"][:, 3] - m["K"][0, 3]).max())}
+
+
 def cpu_baseline(cfg, x, v, budget_s=15.0):
     """The CPU restatement (FP64) timed on this host on a bounded sample: the
     same N and stencil with the first n particles, steps until ~budget_s."""
@@ -278,6 +338,9 @@ def main():
         d1f, d1m = f["D"][:, 0], m["D"][:, 0]
         out["field_energy_rel_err_mixed_vs_fp64"] = float(np.max(np.abs(d1m - d1f) / np.abs(d1f)))
         out["momentum_drift_mixed"] = float(np.max(np.abs(m["D"][:, 3] - m["D"][0, 3])))
+    if not a.no_extra:
+        out["workloads"] = {"ensemble_512": bench_ensemble(), "es2d3v_256": bench_2d()}
+        print(f"[bench] extra workloads: {json.dumps(out['workloads'])}", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(*base)
     print(json.dumps(out))
