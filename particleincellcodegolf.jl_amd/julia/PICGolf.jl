@@ -30,7 +30,8 @@ expression.
 module PICGolf
 
 export Context, PicConfig, deposit!, fieldsolve!, drift!, midpoint_position!, wrap!,
-       diagnostics, upload!, download, run!, quiet_start
+       diagnostics, upload!, download, run!, quiet_start,
+       Pic2DConfig, Context2D, gather, boris!
 
 const libpicgolf = get(ENV, "PICGOLF_LIB", normpath(joinpath(@__DIR__, "..", "lib", "libpicgolf.so")))
 
@@ -259,5 +260,190 @@ end
 
 "`x = (bitreverse.(0:P-1) .+ 2.0^63)/2.0^64`, `v = ∓1` by half (src/GaussianFixedPointQuiet.jl:2-3)."
 quiet_start(P::Integer) = ((bitreverse.(0:P-1) .+ 2.0^63) / 2.0^64, collect(1:P .> P / 2) .* 2 .- 1.0)
+
+# ---------------------------------------------------------------------------
+# Electrostatic 2D3V (src/Electrostatic2D3V.jl), C ABI in include/pic2d.h.
+# Grids are Matrix{Float64}(undef, NX, NY): the script's own layout.
+#
+# | script expression                                   | PICGolf                               |
+# |-----------------------------------------------------|---------------------------------------|
+# | `eval(Ex, Ey, x[i], y[i])` (:94-103)                 | `gather(ctx, Ex, Ey, x, y)`           |
+# | `boris(vx, vy, vz, Ex, Ey, dt)` (:32-41)             | `boris!(ctx, vx, vy, vz, ex, ey, dt)` |
+# | particle loop without the deposit (:128-134)         | `push!(ctx, x, y, vx, vy, vz, Ex, Ey)`|
+# | `depositcharge!` + `sum(ns, dims=3)` (:105-109,139-141) | `deposit!(ρ, ctx, x, y)`          |
+# | field invert + field solve (:142-157)                | `fieldsolve!(Ex, Ey, ctx, ρ)`         |
+# | `K[ti, 1:5]` (:166-170)                              | `diagnostics(ctx, Ex, Ey, vx, vy)`    |
+# | the `for t in 1:T` loop (:120-176)                   | `upload!`, `run!`, `download`         |
+# ---------------------------------------------------------------------------
+
+const PAIRINGS = Dict(:reference => Int32(0), :textbook => Int32(1))
+
+"Mirror of `struct pic2d_config` in include/pic2d.h."
+struct Pic2DConfig
+    NX::Int32
+    NY::Int32
+    P::Int64
+    precision::Int32
+    pairing::Int32
+    dt::Float64
+    w::Float64
+    B0::Float64
+    ns::Int32
+    device::Int32
+end
+
+"""
+    Pic2DConfig(; NX, NY, P, dt, w, B0, ns=2, precision=:fp64, pairing=:reference, device=0)
+
+`w = n0/P/(Δx*Δy)`, `B0` along x and `NS` as in src/Electrostatic2D3V.jl:23-25.
+"""
+Pic2DConfig(; NX, NY, P, dt, w, B0, ns=2, precision=:fp64, pairing=:reference, device=0) =
+    Pic2DConfig(Int32(NX), Int32(NY), Int64(P), PRECISIONS[precision], PAIRINGS[pairing], Float64(dt),
+                Float64(w), Float64(B0), Int32(ns), Int32(device))
+
+mutable struct Context2D
+    ptr::Ptr{Cvoid}
+    cfg::Pic2DConfig
+    steps::Int64
+end
+
+lasterror2d(p::Ptr{Cvoid}) = unsafe_string(ccall((:pic2d_last_error, libpicgolf), Cstring, (Ptr{Cvoid},), p))
+
+function check(ctx::Context2D, rc::Cint, what::AbstractString)
+    rc == 0 || error(what, ": ", lasterror2d(ctx.ptr), " (code ", rc, ")")
+    nothing
+end
+
+function Context2D(cfg::Pic2DConfig)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:pic2d_create, libpicgolf), Cint, (Ref{Ptr{Cvoid}}, Ref{Pic2DConfig}), h, cfg)
+    rc == 0 || error("pic2d_create: ", lasterror2d(C_NULL), " (code ", rc, ")")
+    ctx = Context2D(h[], cfg, 0)
+    finalizer(destroy!, ctx)
+    ctx
+end
+
+function destroy!(ctx::Context2D)
+    if ctx.ptr != C_NULL
+        ccall((:pic2d_destroy, libpicgolf), Cint, (Ptr{Cvoid},), ctx.ptr)
+        ctx.ptr = C_NULL
+    end
+    nothing
+end
+
+grid(ctx::Context2D) = zeros(ctx.cfg.NX, ctx.cfg.NY)
+
+"`eval(Ex, Ey, x[p], y[p])` for every particle (src/Electrostatic2D3V.jl:94-103)."
+function gather(ctx::Context2D, Ex::Matrix{Float64}, Ey::Matrix{Float64}, x::Vector{Float64}, y::Vector{Float64})
+    ex, ey = similar(x), similar(x)
+    rc = GC.@preserve Ex Ey x y ex ey ccall((:pic2d_gather, libpicgolf), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}),
+        ctx.ptr, Ex, Ey, x, y, length(x), ex, ey)
+    check(ctx, rc, "pic2d_gather")
+    (ex, ey)
+end
+
+"`vx, vy, vz = boris(vx, vy, vz, ex, ey, dt)` in place (src/Electrostatic2D3V.jl:32-41)."
+function boris!(ctx::Context2D, vx::Vector{Float64}, vy::Vector{Float64}, vz::Vector{Float64},
+                ex::Vector{Float64}, ey::Vector{Float64}, dt::Real=ctx.cfg.dt; B0::Real=ctx.cfg.B0)
+    rc = GC.@preserve vx vy vz ex ey ccall((:pic2d_boris, libpicgolf), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Float64),
+        ctx.ptr, vx, vy, vz, ex, ey, length(vx), dt, B0)
+    check(ctx, rc, "pic2d_boris")
+    (vx, vy, vz)
+end
+
+"Gather, Boris and `x = unimod(x + vx*dt, 1)` in place (src/Electrostatic2D3V.jl:128-134)."
+function Base.push!(ctx::Context2D, x::Vector{Float64}, y::Vector{Float64}, vx::Vector{Float64},
+                    vy::Vector{Float64}, vz::Vector{Float64}, Ex::Matrix{Float64}, Ey::Matrix{Float64};
+                    dt::Real=ctx.cfg.dt, B0::Real=ctx.cfg.B0)
+    rc = GC.@preserve x y vx vy vz Ex Ey ccall((:pic2d_push, libpicgolf), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64,
+         Ptr{Float64}, Ptr{Float64}, Float64, Float64),
+        ctx.ptr, x, y, vx, vy, vz, length(x), Ex, Ey, dt, B0)
+    check(ctx, rc, "pic2d_push")
+    (x, y, vx, vy, vz)
+end
+
+"`depositcharge!` of every particle, summed over threads (src/Electrostatic2D3V.jl:105-109,139-141)."
+function deposit!(ρ::Matrix{Float64}, ctx::Context2D, x::Vector{Float64}, y::Vector{Float64}; w::Real=ctx.cfg.w)
+    size(ρ) == (ctx.cfg.NX, ctx.cfg.NY) || throw(DimensionMismatch("ρ must be NX x NY"))
+    rc = GC.@preserve ρ x y ccall((:pic2d_deposit, libpicgolf), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Ptr{Float64}), ctx.ptr, x, y, length(x), w, ρ)
+    check(ctx, rc, "pic2d_deposit")
+    ρ
+end
+
+"""
+    fieldsolve!(Ex, Ey, ctx, ρ; phi=nothing)
+
+`pfft*phi; phi[1,1] = 0; Ex = phi.*minvkk.*kx; Ey = ...; pifft*Ex; pifft*Ey`
+(src/Electrostatic2D3V.jl:142-157), real parts out; `phi` receives
+`real(pifft*phi)` as stored in `phis` (:173).
+"""
+function fieldsolve!(Ex::Matrix{Float64}, Ey::Matrix{Float64}, ctx::Context2D, ρ::Matrix{Float64};
+                     phi::Union{Matrix{Float64},Nothing}=nothing)
+    GC.@preserve Ex Ey ρ phi begin
+        pp = phi === nothing ? C_NULL : pointer(phi)
+        rc = ccall((:pic2d_fieldsolve, libpicgolf), Cint,
+                   (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}), ctx.ptr, ρ, Ex, Ey, pp)
+    end
+    check(ctx, rc, "pic2d_fieldsolve")
+    (Ex, Ey)
+end
+
+"`K[ti, 1:5]` of src/Electrostatic2D3V.jl:166-170 for one state."
+function diagnostics(ctx::Context2D, Ex::Matrix{Float64}, Ey::Matrix{Float64}, vx::Vector{Float64},
+                     vy::Vector{Float64}; w::Real=ctx.cfg.w)
+    K = zeros(5)
+    rc = GC.@preserve Ex Ey vx vy K ccall((:pic2d_diagnostics, libpicgolf), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Ptr{Float64}),
+        ctx.ptr, Ex, Ey, vx, vy, length(vx), w, K)
+    check(ctx, rc, "pic2d_diagnostics")
+    K
+end
+
+"Upload the particles (x, y in (0, 1]) and the field the next step gathers from (zeros by default)."
+function upload!(ctx::Context2D, x, y, vx, vy, vz; Ex::Union{Matrix{Float64},Nothing}=nothing,
+                 Ey::Union{Matrix{Float64},Nothing}=nothing)
+    GC.@preserve x y vx vy vz Ex Ey begin
+        ep = Ex === nothing ? C_NULL : pointer(Ex)
+        fp = Ey === nothing ? C_NULL : pointer(Ey)
+        rc = ccall((:pic2d_upload, libpicgolf), Cint,
+                   (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                    Ptr{Float64}, Ptr{Float64}), ctx.ptr, x, y, vx, vy, vz, ep, fp)
+    end
+    check(ctx, rc, "pic2d_upload")
+    ctx.steps = 0
+    ctx
+end
+
+"Download (x, y, vx, vy, vz, Ex, Ey)."
+function download(ctx::Context2D)
+    P = ctx.cfg.P
+    out = (zeros(P), zeros(P), zeros(P), zeros(P), zeros(P), grid(ctx), grid(ctx))
+    rc = GC.@preserve out ccall((:pic2d_download, libpicgolf), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+         Ptr{Float64}), ctx.ptr, out...)
+    check(ctx, rc, "pic2d_download")
+    out
+end
+
+"""
+    run!(ctx::Context2D, nsteps) -> K
+
+Advance `nsteps` steps; `K` holds one row `K[ti, 1:5]` per step t with
+`t % NS == 0` (src/Electrostatic2D3V.jl:164-169).
+"""
+function run!(ctx::Context2D, nsteps::Integer)
+    ns = ctx.cfg.ns
+    rows = div(ctx.steps + nsteps, ns) - div(ctx.steps, ns)
+    K = zeros(5, rows)
+    rc = GC.@preserve K ccall((:pic2d_run, libpicgolf), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}),
+                              ctx.ptr, nsteps, rows > 0 ? pointer(K) : C_NULL)
+    check(ctx, rc, "pic2d_run")
+    ctx.steps += nsteps
+    permutedims(K)
+end
 
 end # module
