@@ -44,6 +44,8 @@ struct pic_ctx {
     hipGraphExec_t graph_many = nullptr;
     hipGraphExec_t graph_one = nullptr;
     int64_t graph_many_steps = 0;
+    int graph_slots = 0;       // sweep slots per step in the captured graphs
+    int sweeps_seen = 0;       // most sweeps any step has needed (0: none run yet)
     int64_t steps_done = 0;
     std::string err;
 };
@@ -327,7 +329,9 @@ static int build_tables(pic_ctx* c) {
     if ((rc = dalloc(c, &b.E[0], G))) return rc;
     if ((rc = dalloc(c, &b.E[1], G))) return rc;
     if ((rc = dalloc(c, &b.E32, G))) return rc;
-    if ((rc = dalloc(c, &b.dpart, 2 * (size_t)b.nchunks))) return rc;
+    if ((rc = dalloc(c, &b.dpart, 2 * (size_t)std::max(b.nchunks, b.ntiles)))) return rc;
+    if ((rc = dalloc(c, &b.halt, 1))) return rc;
+    HIPCHK(c, hipMemset(b.halt, 0, sizeof(int32_t)));
     HIPCHK(c, hipMemset(b.E[0], 0, sizeof(double) * G));
     HIPCHK(c, hipMemset(b.E[1], 0, sizeof(double) * G));
     HIPCHK(c, hipMemset(b.E32, 0, sizeof(float) * G));
@@ -502,26 +506,40 @@ int pic_download(pic_ctx* c, int32_t inst, double* x, double* v, double* E) {
 
 // ---- run ------------------------------------------------------------------
 
-static void enqueue_step(pic_ctx* c, hipStream_t s) {
+// One step with `slots` sweep slots (midpoint; slots <= max_sweeps).  With
+// fewer slots than max_sweeps the last slot's solve halts the graph when an
+// instance needs more sweeps (RunBuffers::halt).
+static void enqueue_step(pic_ctx* c, hipStream_t s, int slots) {
     const RunBuffers& b = c->rb;
     if (c->scheme == PIC_SCHEME_MIDPOINT) {
         launch_mid_push0(b, s);
-        for (int k = 0; k < c->max_sweeps; ++k) {
-            launch_solve(b, k, s);
-            launch_mid_push(b, k, s);
+        for (int k = 0; k < slots; ++k) {
+            const bool last = k == slots - 1;
+            launch_solve(b, k, last && slots < c->max_sweeps, s);
+            launch_mid_push(b, k, last, s);
         }
     } else {
         launch_lf_first(b, s);
-        launch_solve(b, 0, s);
+        launch_solve(b, 0, false, s);
         launch_lf_second(b, s);
     }
     launch_diag(b, s);
 }
 
-static int capture(pic_ctx* c, int64_t steps, hipGraphExec_t* out) {
+// The rest of a step that halted after sweep slot k0 - 1.
+static void enqueue_tail(pic_ctx* c, hipStream_t s, int k0) {
+    const RunBuffers& b = c->rb;
+    for (int k = k0; k < c->max_sweeps; ++k) {
+        launch_solve(b, k, false, s);
+        launch_mid_push(b, k, false, s);
+    }
+    launch_diag(b, s);
+}
+
+static int capture(pic_ctx* c, int64_t steps, int slots, hipGraphExec_t* out) {
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int64_t i = 0; i < steps; ++i) enqueue_step(c, c->stream);
+    for (int64_t i = 0; i < steps; ++i) enqueue_step(c, c->stream, slots);
     HIPCHK(c, hipStreamEndCapture(c->stream, &g));
     hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -580,6 +598,24 @@ static int finish_run(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps)
     return PIC_OK;
 }
 
+// Sweep slots per step for the next graphs: every slot the loop may need
+// until a step has been observed, then the most sweeps seen so far.
+static int choose_slots(const pic_ctx* c) {
+    if (c->scheme != PIC_SCHEME_MIDPOINT || c->sweeps_seen == 0) return c->max_sweeps;
+    return std::min(c->max_sweeps, std::max(1, c->sweeps_seen));
+}
+
+static int note_sweeps(pic_ctx* c, int64_t row0, int64_t row1) {
+    if (c->scheme != PIC_SCHEME_MIDPOINT || row1 <= row0) return PIC_OK;
+    std::vector<int32_t> h((size_t)(row1 - row0));
+    for (int i = 0; i < c->ninst; ++i) {
+        HIPCHK(c, hipMemcpy(h.data(), c->rb.sweeps + (size_t)i * c->rb.diag_cap + row0, sizeof(int32_t) * h.size(),
+                            hipMemcpyDeviceToHost));
+        for (int32_t v : h) c->sweeps_seen = std::max(c->sweeps_seen, (int)v);
+    }
+    return PIC_OK;
+}
+
 int pic_run(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps) {
     if (!c) return PIC_EINVAL;
     if (nsteps < 0) {
@@ -591,19 +627,51 @@ int pic_run(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps) {
     int rc = begin_run(c, nsteps);
     if (rc) return rc;
     const int64_t G = 8;
-    if (!c->graph_one && (rc = capture(c, 1, &c->graph_one))) return rc;
-    if (nsteps >= G && !c->graph_many) {
-        if ((rc = capture(c, G, &c->graph_many))) return rc;
-        c->graph_many_steps = G;
-    }
-    int64_t left = nsteps;
-    while (c->graph_many && left >= c->graph_many_steps) {
-        HIPCHK(c, hipGraphLaunch(c->graph_many, c->stream));
-        left -= c->graph_many_steps;
-    }
-    while (left > 0) {
-        HIPCHK(c, hipGraphLaunch(c->graph_one, c->stream));
-        --left;
+    int64_t done = 0;
+    while (done < nsteps) {
+        const int slots = choose_slots(c);
+        if (slots != c->graph_slots) {
+            drop_graphs(c);
+            c->graph_slots = slots;
+        }
+        // until a step has been observed, run a few with every slot in place
+        const int64_t chunk = c->sweeps_seen == 0 ? std::min<int64_t>(nsteps - done, G) : nsteps - done;
+        if (!c->graph_one && (rc = capture(c, 1, slots, &c->graph_one))) return rc;
+        if (chunk >= G && !c->graph_many) {
+            if ((rc = capture(c, G, slots, &c->graph_many))) return rc;
+            c->graph_many_steps = G;
+        }
+        int64_t left = chunk;
+        while (c->graph_many && left >= c->graph_many_steps) {
+            HIPCHK(c, hipGraphLaunch(c->graph_many, c->stream));
+            left -= c->graph_many_steps;
+        }
+        while (left > 0) {
+            HIPCHK(c, hipGraphLaunch(c->graph_one, c->stream));
+            --left;
+        }
+        int32_t halt = 0;
+        HIPCHK(c, hipMemcpyAsync(&halt, c->rb.halt, sizeof halt, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipGetLastError());
+        std::vector<InstState> st;
+        if ((rc = read_states(c, st))) return rc;
+        int64_t completed = (int64_t)st[0].step - st[0].run_base;
+        if (halt) {
+            // a step needed more sweeps than the graph holds: finish it here
+            const int32_t zero = 0;
+            HIPCHK(c, hipMemcpyAsync(c->rb.halt, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+            enqueue_tail(c, c->stream, slots);
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipGetLastError());
+            completed += 1;
+        }
+        if ((rc = note_sweeps(c, done, completed))) return rc;
+        if (completed <= done || completed > nsteps) {
+            set_err(c, "pic_run: step bookkeeping lost (%lld of %lld)", (long long)completed, (long long)nsteps);
+            return PIC_ESTATE;
+        }
+        done = completed;
     }
     return finish_run(c, nsteps, diag, sweeps);
 }
@@ -643,8 +711,8 @@ int pic_run_profiled(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps, 
         if (c->scheme == PIC_SCHEME_MIDPOINT) {
             if ((rc = timed(K_PUSH0, [&] { launch_mid_push0(b, c->stream); }))) break;
             for (int k = 0; k < c->max_sweeps; ++k) {
-                if ((rc = timed(K_SOLVE, [&] { launch_solve(b, k, c->stream); }))) break;
-                if ((rc = timed(K_PUSH, [&] { launch_mid_push(b, k, c->stream); }))) break;
+                if ((rc = timed(K_SOLVE, [&] { launch_solve(b, k, false, c->stream); }))) break;
+                if ((rc = timed(K_PUSH, [&] { launch_mid_push(b, k, false, c->stream); }))) break;
                 if ((rc = read_states(c, st))) break;
                 bool all_done = true;
                 for (auto& s : st) all_done = all_done && s.last_sweep >= 0 && s.last_sweep <= k;
@@ -652,7 +720,7 @@ int pic_run_profiled(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps, 
             }
         } else {
             if ((rc = timed(K_LF1, [&] { launch_lf_first(b, c->stream); }))) break;
-            if ((rc = timed(K_SOLVE, [&] { launch_solve(b, 0, c->stream); }))) break;
+            if ((rc = timed(K_SOLVE, [&] { launch_solve(b, 0, false, c->stream); }))) break;
             if ((rc = timed(K_LF2, [&] { launch_lf_second(b, c->stream); }))) break;
         }
         if (rc) break;

@@ -82,6 +82,13 @@ __device__ __forceinline__ bool sweep_over(const InstState& S, int k) {
     return S.last_sweep >= 0 && S.last_sweep < k;
 }
 
+// *halt: 0 = running, 1 = a solve in the graph's last sweep slot left an
+// instance unconverged (that slot's push still runs), 2 = the graph is halted.
+__device__ __forceinline__ bool halted(const int32_t* __restrict__ halt, bool last_slot) {
+    const int32_t h = *halt;
+    return last_slot ? h >= 2 : h >= 1;
+}
+
 // ---------------------------------------------------------------------------
 // Implicit-midpoint push (src/GaussianFixedPointQuiet.jl:8-10).
 // MODE 0: the first deposit of a step, v_0 = V, so x_0 = X + ((V+V)/2)*dt.
@@ -94,8 +101,10 @@ __global__ void __launch_bounds__(1024)
 k_mid_push64(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
              const TileDesc* __restrict__ tiles, double* X0, double* X1, double* V0,
              double* V1, double* vit0, double* vit1, const double* __restrict__ E0,
-             const double* __restrict__ E1, u64* __restrict__ racc, int k) {
+             const double* __restrict__ E1, u64* __restrict__ racc, int k, const int32_t* __restrict__ halt,
+             int last_slot) {
     extern __shared__ u64 lds[];
+    if (halted(halt, MODE == 1 && last_slot)) return;
     const TileDesc T = tiles[blockIdx.x];
     const InstDesc I = insts[T.inst];
     InstState S = st[T.inst];
@@ -401,8 +410,9 @@ template <bool TEST>
 __global__ void __launch_bounds__(kSolveThreads)
 k_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
         void* __restrict__ racc, int precision, double* E0, double* E1, float* E32,
-        const double2* __restrict__ twp, int k) {
+        const double2* __restrict__ twp, int k, int32_t* __restrict__ halt, int stall) {
     extern __shared__ double sm[];
+    if (halted(halt, false)) return;
     const InstDesc I = insts[blockIdx.x];
     InstState S = st[blockIdx.x];
     if (TEST && sweep_over(S, k)) return;
@@ -476,6 +486,7 @@ k_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
         if (TEST) {
             S.e_cur = nxt;
             if (conv || k == I.max_sweeps - 1) S.last_sweep = k;
+            else if (stall) *halt = 1;
         } else {
             S.e_cur = 0;
             S.last_sweep = 0;
@@ -492,8 +503,9 @@ k_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
 __global__ void __launch_bounds__(256)
 k_diag_chunks64(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
                 const ChunkDesc* __restrict__ chunks, const double* V0, const double* V1,
-                double* __restrict__ dpart) {
+                double* __restrict__ dpart, const int32_t* __restrict__ halt) {
     __shared__ double red[4 * kTreeLanes + 8];
+    if (halted(halt, false)) return;
     const ChunkDesc C = chunks[blockIdx.x];
     const InstDesc I = insts[C.inst];
     const InstState S = st[C.inst];
@@ -563,13 +575,23 @@ __global__ void __launch_bounds__(256)
 k_diag_final(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
              double* __restrict__ dpart, const double* E0, const double* E1,
              double* __restrict__ diag, int32_t* __restrict__ sweeps, int64_t cap,
-             int scheme) {
+             int scheme, int32_t* __restrict__ halt, int tile_partials) {
     __shared__ double red[4 * kTreeLanes + 8];
+    if (*halt) {
+        // the step did not finish: no diagnostics; from here on the graph is halted
+        if (blockIdx.x == 0 && threadIdx.x == 0) *halt = 2;
+        return;
+    }
     const InstDesc I = insts[blockIdx.x];
     InstState S = st[blockIdx.x];
     const double* E = (S.e_cur ? E1 : E0) + I.goff;
     double se, sv, sp;
-    diag_sums(red, E, I.N, dpart + 2 * (int64_t)I.chunk0, I.nchunks, se, sv, sp);
+    // FP64: canonical partials per 4096-particle chunk; mixed: one partial pair
+    // per tile, written by the final sweep of the push
+    if (tile_partials)
+        diag_sums(red, E, I.N, dpart + 2 * (int64_t)I.tile0, I.ntiles, se, sv, sp);
+    else
+        diag_sums(red, E, I.N, dpart + 2 * (int64_t)I.chunk0, I.nchunks, se, sv, sp);
     if (threadIdx.x == 0) {
         const double d1 = (se / (double)I.N) / 2.0;
         const double d2 = ((sv * I.W) / (double)I.P) / 2.0;
@@ -598,39 +620,40 @@ int push_lds_bytes(int N, int precision) {
 }
 
 void launch_mid_push0(const RunBuffers& b, hipStream_t s) {
-    if (b.precision == 1) return launch_mid_push32(b, 0, 0, s);
+    if (b.precision == 1) return launch_mid_push32(b, 0, 0, false, s);
     const dim3 g(b.ntiles), t(b.block_push);
     u64* part = (u64*)b.racc;
     if (b.R == 6)
         k_mid_push64<6, 0><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
-                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, 0);
+                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, 0, b.halt, 0);
     else
         k_mid_push64<7, 0><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
-                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, 0);
+                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, 0, b.halt, 0);
 }
 
-void launch_mid_push(const RunBuffers& b, int k, hipStream_t s) {
-    if (b.precision == 1) return launch_mid_push32(b, 1, k, s);
+void launch_mid_push(const RunBuffers& b, int k, bool last_slot, hipStream_t s) {
+    if (b.precision == 1) return launch_mid_push32(b, 1, k, last_slot, s);
+    const int ls = last_slot ? 1 : 0;
     const dim3 g(b.ntiles), t(b.block_push);
     u64* part = (u64*)b.racc;
     if (b.R == 6)
         k_mid_push64<6, 1><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
-                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, k);
+                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, k, b.halt, ls);
     else
         k_mid_push64<7, 1><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
-                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, k);
+                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, k, b.halt, ls);
 }
 
 static int solve_lds_bytes(int N) { return solve_lds_doubles(N) * 8; }
 
-void launch_solve(const RunBuffers& b, int k, hipStream_t s) {
+void launch_solve(const RunBuffers& b, int k, bool stall, hipStream_t s) {
     const int lds = solve_lds_bytes(b.maxN);
     if (b.scheme == 1)
         k_solve<true><<<b.ninst, kSolveThreads, lds, s>>>(b.insts, b.st, b.racc, b.precision, b.E[0], b.E[1], b.E32,
-                                                (const double2*)b.tw, k);
+                                                (const double2*)b.tw, k, b.halt, stall ? 1 : 0);
     else
         k_solve<false><<<b.ninst, kSolveThreads, lds, s>>>(b.insts, b.st, b.racc, b.precision, b.E[0], b.E[1], b.E32,
-                                                 (const double2*)b.tw, k);
+                                                 (const double2*)b.tw, k, b.halt, 0);
 }
 
 void launch_lf_first(const RunBuffers& b, hipStream_t s) {
@@ -658,12 +681,14 @@ void launch_lf_second(const RunBuffers& b, hipStream_t s) {
 }
 
 void launch_diag(const RunBuffers& b, hipStream_t s) {
+    // the mixed midpoint push sums the kinetic terms in its final sweep
+    const bool fused = b.precision == 1 && b.scheme == 1;
     if (b.precision == 0)
-        k_diag_chunks64<<<b.nchunks, 256, 0, s>>>(b.insts, b.st, b.chunks, b.V[0], b.V[1], b.dpart);
-    else
+        k_diag_chunks64<<<b.nchunks, 256, 0, s>>>(b.insts, b.st, b.chunks, b.V[0], b.V[1], b.dpart, b.halt);
+    else if (!fused)
         launch_diag_chunks32(b, s);
     k_diag_final<<<b.ninst, 256, 0, s>>>(b.insts, b.st, b.dpart, b.E[0], b.E[1], b.diag, b.sweeps,
-                                         b.diag_cap, b.scheme);
+                                         b.diag_cap, b.scheme, b.halt, fused ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------

@@ -84,8 +84,13 @@ __global__ void __launch_bounds__(1024)
 k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
              const TileDesc* __restrict__ tiles, uint32_t* Xq0, uint32_t* Xq1, float* Vh0,
              float* Vh1, float* Vl0, float* Vl1, float* vf0, float* vf1,
-             const float* __restrict__ E32, u64* __restrict__ racc, int k) {
+             const float* __restrict__ E32, u64* __restrict__ racc, int k, const int32_t* __restrict__ halt,
+             int last_slot, double* __restrict__ dpart) {
     extern __shared__ u64 lds[];
+    {
+        const int32_t h = *halt;  // see RunBuffers::halt
+        if (h >= ((MODE == 1 && last_slot) ? 2 : 1)) return;
+    }
     const TileDesc T = tiles[blockIdx.x];
     const InstDesc I = insts[T.inst];
     InstState S = st[T.inst];
@@ -121,6 +126,8 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     const float cscale = (float)(I.dt * 1073741824.0);  // (v + V)/4*dt in 2^-32 units
     const float xscale = (float)(I.dt * 2147483648.0);  // (v + V)/2*dt in 2^-32 units
     const int64_t base = I.poff + T.start;
+    const double Pd = (double)I.P;
+    double d2 = 0.0, d1 = 0.0;  // final sweep: sum(v^2), sum(v/P) over this thread's particles
     for (int64_t p = threadIdx.x; p < T.count; p += blockDim.x) {
         const int64_t g = base + p;
         const uint32_t xq = Xq[g];
@@ -144,9 +151,13 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
                 const float err = (vh - (sum - bb)) + (kick - bb);
                 const float t = Vl[g] + err;
                 const float hi = sum + t;
+                const float lo = t - (hi - sum);
                 Xo[g] = advance(xq, vk + vh, xscale);
                 Vho[g] = hi;
-                Vlo[g] = t - (hi - sum);
+                Vlo[g] = lo;
+                const double vd = (double)hi + (double)lo;
+                d2 = d2 + vd * vd;
+                d1 = d1 + vd / Pd;
                 continue;
             }
             vout[g] = vn;
@@ -160,7 +171,33 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
         PIC_DEP(6) PIC_DEP(7) PIC_DEP(8) PIC_DEP(9) PIC_DEP(10)
 #undef PIC_DEP
     }
-    if (last) return;
+    if (last) {
+        // the step's kinetic diagnostics, one partial pair per tile (fixed
+        // order: lanes by shuffle tree, then waves in order)
+#pragma unroll
+        for (int h = 32; h > 0; h >>= 1) {
+            d2 = d2 + __shfl_down(d2, h, 64);
+            d1 = d1 + __shfl_down(d1, h, 64);
+        }
+        double* red = reinterpret_cast<double*>(lds);
+        __syncthreads();
+        const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            red[2 * wv] = d2;
+            red[2 * wv + 1] = d1;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double a = 0.0, b = 0.0;
+            for (int w = 0; w < nw; ++w) {
+                a = a + red[2 * w];
+                b = b + red[2 * w + 1];
+            }
+            dpart[2 * blockIdx.x] = a;
+            dpart[2 * blockIdx.x + 1] = b;
+        }
+        return;
+    }
     __syncthreads();
     flush_charge32(acc, N, NPAD, racc + I.red0);
 }
@@ -168,8 +205,9 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
 __global__ void __launch_bounds__(256)
 k_diag_chunks32(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
                 const ChunkDesc* __restrict__ chunks, const float* Vh0, const float* Vh1,
-                const float* Vl0, const float* Vl1, double* __restrict__ dpart) {
+                const float* Vl0, const float* Vl1, double* __restrict__ dpart, const int32_t* __restrict__ halt) {
     __shared__ double red[4 * kTreeLanes + 8];
+    if (*halt) return;
     const ChunkDesc C = chunks[blockIdx.x];
     const InstDesc I = insts[C.inst];
     const InstState S = st[C.inst];
@@ -184,20 +222,23 @@ k_diag_chunks32(const InstDesc* __restrict__ insts, const InstState* __restrict_
     }
 }
 
-void launch_mid_push32(const RunBuffers& b, int mode, int k, hipStream_t s) {
+void launch_mid_push32(const RunBuffers& b, int mode, int k, bool last_slot, hipStream_t s) {
+    const int ls = last_slot ? 1 : 0;
     const dim3 g(b.ntiles), t(b.block_push);
     u64* part = (u64*)b.racc;
     if (mode == 0)
         k_mid_push32<0><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.Xq[0], b.Xq[1], b.Vh[0], b.Vh[1],
-                                                 b.Vl[0], b.Vl[1], b.vf[0], b.vf[1], b.E32, part, 0);
+                                                 b.Vl[0], b.Vl[1], b.vf[0], b.vf[1], b.E32, part, 0, b.halt, 0,
+                                                 b.dpart);
     else
         k_mid_push32<1><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.Xq[0], b.Xq[1], b.Vh[0], b.Vh[1],
-                                                 b.Vl[0], b.Vl[1], b.vf[0], b.vf[1], b.E32, part, k);
+                                                 b.Vl[0], b.Vl[1], b.vf[0], b.vf[1], b.E32, part, k, b.halt, ls,
+                                                 b.dpart);
 }
 
 void launch_diag_chunks32(const RunBuffers& b, hipStream_t s) {
     k_diag_chunks32<<<b.nchunks, 256, 0, s>>>(b.insts, b.st, b.chunks, b.Vh[0], b.Vh[1], b.Vl[0], b.Vl[1],
-                                              b.dpart);
+                                              b.dpart, b.halt);
 }
 
 int configure_kernels_mixed() {

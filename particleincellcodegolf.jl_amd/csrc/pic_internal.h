@@ -83,12 +83,20 @@ struct RunBuffers {
     double* diag;      // [ninst][diag_cap][4]
     int32_t* sweeps;   // [ninst][diag_cap]
     int64_t diag_cap;
+    // Speculative sweep count.  A step graph may hold fewer sweep slots than
+    // max_sweeps; a solve in the last slot that finds an instance unconverged
+    // sets *halt = 1, the push of that slot still runs, diagnostics turn it
+    // into 2 and every later kernel returns at once.  The host then finishes
+    // the step with the remaining sweeps and resumes.
+    int32_t* halt;
 };
 
 // kernels_1d.hip
 void launch_mid_push0(const RunBuffers& b, hipStream_t s);
-void launch_mid_push(const RunBuffers& b, int k, hipStream_t s);
-void launch_solve(const RunBuffers& b, int k, hipStream_t s);
+// last_slot: this is the graph's last sweep slot (it runs while *halt == 1)
+void launch_mid_push(const RunBuffers& b, int k, bool last_slot, hipStream_t s);
+// stall: an instance still unconverged after this solve halts the graph
+void launch_solve(const RunBuffers& b, int k, bool stall, hipStream_t s);
 void launch_lf_first(const RunBuffers& b, hipStream_t s);
 void launch_lf_second(const RunBuffers& b, hipStream_t s);
 void launch_diag(const RunBuffers& b, hipStream_t s);
@@ -97,7 +105,7 @@ int configure_kernels();      // raise the dynamic LDS limit; 0 on success
 bool mixed_available();
 
 // kernels_mixed.hip
-void launch_mid_push32(const RunBuffers& b, int mode, int k, hipStream_t s);
+void launch_mid_push32(const RunBuffers& b, int mode, int k, bool last_slot, hipStream_t s);
 void launch_diag_chunks32(const RunBuffers& b, hipStream_t s);
 int configure_kernels_mixed();
 

@@ -87,3 +87,29 @@ def test_mixed_quiet_two_stream_growth_rate(pg):
     print("growth slopes (decades/time): mixed %.4f fp64 %.4f analytic %.4f" % (*slopes, slope_pred))
     assert slopes[0] == pytest.approx(slopes[1], rel=0.01)
     assert slopes[1] == pytest.approx(slope_pred, rel=0.02)
+
+
+def test_mixed_speculative_slots_match_full_slots():
+    """The mixed path through the halt-and-finish route equals a context that
+    runs every sweep slot (bit for bit)."""
+    import picgolf
+    from picgolf import configs
+    cfg, x, v, _ = configs.quiet_two_stream(precision="mixed")
+    N = cfg.N
+    j = np.arange(N)
+    E0 = 40.0 * np.sin(2 * np.pi * j / N)
+    rng = np.random.default_rng(12)
+    v2 = v + 0.5 * rng.standard_normal(v.size)
+    outs = []
+    for warm in (False, True):
+        with picgolf.Context(cfg) as c:
+            if warm:
+                c.upload(x, v)
+                c.run(20)                   # learns a small slot count
+            c.upload(x, v2, E0)
+            D, S = c.run(12)
+            outs.append((D, S, c.download()))
+    (Da, Sa, A), (Db, Sb, B) = outs
+    assert np.array_equal(Sa, Sb) and np.array_equal(Da, Db)
+    for a, b in zip(A, B):
+        assert np.array_equal(a, b)

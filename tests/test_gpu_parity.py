@@ -203,3 +203,28 @@ def test_full_size_scaled_fp64_properties(pg):
     assert np.all(np.isfinite(D))
     p0 = np.sum(v / cfg.P)
     assert abs(D[-1, 3] - p0) < 1e-13
+
+
+def test_speculative_sweep_slots_recover_bitwise(pg, orc):
+    """After a first run the step graphs hold only as many sweep slots as the
+    loop has needed so far.  A later state that needs more sweeps halts the
+    graph and finishes the step on the host side: results stay bit-identical
+    to the oracle and the sweep counts are the oracle's."""
+    from picgolf import configs
+    cfg, x, v, _ = configs.quiet_two_stream()
+    N = cfg.N
+    with pg.Context(cfg) as c:
+        c.upload(x, v)
+        _, S1 = c.run(20)
+        # a strong field and hot particles: more Picard sweeps per step
+        j = np.arange(N)
+        E0 = 40.0 * np.sin(2 * np.pi * j / N) + 15.0 * np.cos(6 * np.pi * j / N)
+        rng = np.random.default_rng(11)
+        v2 = v + 0.5 * rng.standard_normal(v.size)
+        c.upload(x, v2, E0)
+        D, S2 = c.run(12)
+        xg, vg, Eg = c.download()
+    xo, vo, Eo, Do, So = orc.run(N, 7, 1, cfg.W, cfg.w, cfg.dt, x, v2, 12, rtol=cfg.rtol, E=E0)
+    assert S2.max() > S1.max()                      # the halt-and-finish path ran
+    assert np.array_equal(S2[0], So)
+    assert same(D[0], Do) and same(xg, xo) and same(vg, vo) and same(Eg, Eo)
