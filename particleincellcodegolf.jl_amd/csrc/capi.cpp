@@ -45,6 +45,7 @@ struct pic_ctx {
     hipGraphExec_t graph_one = nullptr;
     int64_t graph_many_steps = 0;
     int graph_slots = 0;       // sweep slots per step in the captured graphs
+    bool primed = false;       // the accumulator holds the next step's first charge
     int sweeps_seen = 0;       // most sweeps any step has needed (0: none run yet)
     int64_t steps_done = 0;
     std::string err;
@@ -324,6 +325,8 @@ static int build_tables(pic_ctx* c) {
         }
         if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.racc), (size_t)red0))) return rc;
         HIPCHK(c, hipMemset(b.racc, 0, 8 * (size_t)red0));
+        b.racc_cells = red0;
+        b.prime_next = c->scheme == PIC_SCHEME_MIDPOINT ? 1 : 0;
     }
     if ((rc = dalloc(c, &b.rho, G))) return rc;
     if ((rc = dalloc(c, &b.E[0], G))) return rc;
@@ -465,6 +468,7 @@ int pic_upload(pic_ctx* c, int32_t inst, const double* x, const double* v, const
     if (rc) return rc;
     st[inst].e_cur = 0;
     st[inst].err = 0;
+    c->primed = false;  // the fused first deposit no longer matches the state
     return write_states(c, st);
 }
 
@@ -512,7 +516,7 @@ int pic_download(pic_ctx* c, int32_t inst, double* x, double* v, double* E) {
 static void enqueue_step(pic_ctx* c, hipStream_t s, int slots) {
     const RunBuffers& b = c->rb;
     if (c->scheme == PIC_SCHEME_MIDPOINT) {
-        launch_mid_push0(b, s);
+        if (!b.prime_next) launch_mid_push0(b, s);
         for (int k = 0; k < slots; ++k) {
             const bool last = k == slots - 1;
             launch_solve(b, k, last && slots < c->max_sweeps, s);
@@ -598,6 +602,19 @@ static int finish_run(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps)
     return PIC_OK;
 }
 
+// With the first deposit fused into the final sweep, a run that does not
+// follow a completed step (new context, or after an upload) starts with the
+// first deposit of its own.
+static int prime(pic_ctx* c) {
+    if (!c->rb.prime_next || c->primed) return PIC_OK;
+    const size_t bytes = sizeof(unsigned long long) * (size_t)c->rb.racc_cells;
+    HIPCHK(c, hipMemsetAsync(c->rb.racc, 0, bytes, c->stream));
+    launch_mid_push0(c->rb, c->stream);
+    HIPCHK(c, hipGetLastError());
+    c->primed = true;
+    return PIC_OK;
+}
+
 // Sweep slots per step for the next graphs: every slot the loop may need
 // until a step has been observed, then the most sweeps seen so far.
 static int choose_slots(const pic_ctx* c) {
@@ -626,6 +643,7 @@ int pic_run(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps) {
     HIPCHK(c, hipSetDevice(c->device));
     int rc = begin_run(c, nsteps);
     if (rc) return rc;
+    if ((rc = prime(c))) return rc;
     const int64_t G = 8;
     int64_t done = 0;
     while (done < nsteps) {
@@ -686,6 +704,7 @@ int pic_run_profiled(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps, 
     HIPCHK(c, hipSetDevice(c->device));
     int rc = begin_run(c, nsteps);
     if (rc) return rc;
+    if ((rc = prime(c))) return rc;
     enum { K_PUSH0, K_PUSH, K_SOLVE, K_LF1, K_LF2, K_DIAG, K_N };
     const char* names[K_N] = {"push_first", "push_sweep", "solve", "leapfrog_first", "leapfrog_second",
                               "diagnostics"};
@@ -709,7 +728,7 @@ int pic_run_profiled(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps, 
     std::vector<InstState> st;
     for (int64_t t = 0; t < nsteps && rc == PIC_OK; ++t) {
         if (c->scheme == PIC_SCHEME_MIDPOINT) {
-            if ((rc = timed(K_PUSH0, [&] { launch_mid_push0(b, c->stream); }))) break;
+            if (!b.prime_next && (rc = timed(K_PUSH0, [&] { launch_mid_push0(b, c->stream); }))) break;
             for (int k = 0; k < c->max_sweeps; ++k) {
                 if ((rc = timed(K_SOLVE, [&] { launch_solve(b, k, false, c->stream); }))) break;
                 if ((rc = timed(K_PUSH, [&] { launch_mid_push(b, k, false, c->stream); }))) break;

@@ -280,7 +280,7 @@ __device__ __forceinline__ void fft_pass(double* re, double* im, const double2* 
     __syncthreads();
 }
 
-__device__ void fft_lds(double* re, double* im, const double2* __restrict__ twp, int N, int log2N,
+__device__ __forceinline__ void fft_lds(double* re, double* im, const double2* __restrict__ twp, int N, int log2N,
                         bool inverse) {
     for (int s0 = 0; s0 < log2N; s0 += kPassStages) {
         const int m = log2N - s0 < kPassStages ? log2N - s0 : kPassStages;
@@ -295,7 +295,7 @@ __host__ __device__ constexpr int solve_lds_doubles(int N) { return 2 * (N + N /
 
 // On entry re/im hold rho in bit-reversed order; on exit re holds E (padded).
 // Requires blockDim.x == kSolveThreads.
-__device__ void solve_lds(double* re, double* im, const double2* __restrict__ twp, int N, int log2N) {
+__device__ __forceinline__ void solve_lds(double* re, double* im, const double2* __restrict__ twp, int N, int log2N) {
     fft_lds(re, im, twp, N, log2N, false);
     // xi = rho^ ./ ik with ik = 2 pi i [1, 1:N/2, -N/2+1:-1]; Julia's complex
     // division by (0, k) is (Im * (1/k), -(Re * (1/k))); xi[1] *= 0.
@@ -616,7 +616,7 @@ k_diag_final(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
 // ---------------------------------------------------------------------------
 int push_lds_bytes(int N, int precision) {
     if (precision == 0) return N * 8 * 4;  // E + three limb arrays
-    return 12 * (N + 16);                   // FP32 E + u64 accumulators, with halos
+    return push32_lds_bytes(N);             // FP32 E + u64 accumulators with halos, partials
 }
 
 void launch_mid_push0(const RunBuffers& b, hipStream_t s) {

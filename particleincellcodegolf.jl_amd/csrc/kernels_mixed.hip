@@ -23,6 +23,9 @@ namespace pic {
 constexpr int kHalo = kStencil32 + 1;
 constexpr int kWin = 2 * kStencil32 + 1;
 __host__ __device__ constexpr int pad_cells(int N) { return N + 2 * kHalo + 4; }
+// LDS of the push: u64 accumulator and FP32 E over the padded cells, then 32
+// doubles for the block's diagnostics partials
+int push32_lds_bytes(int N) { return 12 * pad_cells(N) + 8 * 4 + 256; }
 
 // Stencil of a fixed-point centre: i0 = nearest cell centre (Julia index,
 // N == 0) and the kWin weights, in 2^-30 quanta of a particle's charge, of the
@@ -79,13 +82,22 @@ __device__ __forceinline__ void flush_charge32(const u64* acc, int N, int NPAD, 
     }
 }
 
+// Adds a stencil's quanta to the LDS accumulator window starting at a.
+#define PIC_DEP(J) atomicAdd(&a[J], quanta<J>(st.w[J]));
+#define PIC_DEPOSIT(a, st_)                                                     \
+    {                                                                           \
+        const Stencil32& st = st_;                                              \
+        PIC_DEP(0) PIC_DEP(1) PIC_DEP(2) PIC_DEP(3) PIC_DEP(4) PIC_DEP(5)      \
+        PIC_DEP(6) PIC_DEP(7) PIC_DEP(8) PIC_DEP(9) PIC_DEP(10)                \
+    }
+
 template <int MODE>
 __global__ void __launch_bounds__(1024)
 k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
              const TileDesc* __restrict__ tiles, uint32_t* Xq0, uint32_t* Xq1, float* Vh0,
              float* Vh1, float* Vl0, float* Vl1, float* vf0, float* vf1,
              const float* __restrict__ E32, u64* __restrict__ racc, int k, const int32_t* __restrict__ halt,
-             int last_slot, double* __restrict__ dpart) {
+             int last_slot, double* __restrict__ dpart, int prime) {
     extern __shared__ u64 lds[];
     {
         const int32_t h = *halt;  // see RunBuffers::halt
@@ -109,7 +121,11 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     if (MODE == 1)
         for (int i = threadIdx.x; i < NPAD; i += blockDim.x)
             Eld[i] = E32[I.goff + ((i - kHalo) & nmask)] * escale;
-    if (!last)
+    // prime: the final sweep also deposits the next step's first charge
+    // (centre x + V*dt/2 with v_0 = V), which a first-deposit launch would
+    // otherwise compute from the state it writes
+    const bool deposit = !last || prime;
+    if (deposit)
         for (int i = threadIdx.x; i < NPAD; i += blockDim.x) acc[i] = 0;
     __syncthreads();
 
@@ -132,9 +148,9 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
         const int64_t g = base + p;
         const uint32_t xq = Xq[g];
         const float vh = Vh[g];
-        float vn = vh;
         Stencil32 s;
         uint32_t cq = 0;
+        uint32_t cdep;  // centre of this particle's deposit
         if (MODE == 1) {
             const float vk = (k == 0) ? vh : vin[g];
             cq = advance(xq, vk + vh, cscale);
@@ -143,7 +159,6 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
             float kick = 0.0f;
 #pragma unroll
             for (int j = 0; j < kWin; ++j) kick = fmaf(e[j], s.w[j], kick);
-            vn = vh + kick;
             if (last) {
                 // V_new = (V_hi + V_lo) + kick, renormalised (two-sum)
                 const float sum = vh + kick;
@@ -152,24 +167,27 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
                 const float t = Vl[g] + err;
                 const float hi = sum + t;
                 const float lo = t - (hi - sum);
-                Xo[g] = advance(xq, vk + vh, xscale);
+                const uint32_t xn = advance(xq, vk + vh, xscale);
+                Xo[g] = xn;
                 Vho[g] = hi;
                 Vlo[g] = lo;
                 const double vd = (double)hi + (double)lo;
                 d2 = d2 + vd * vd;
                 d1 = d1 + vd / Pd;
-                continue;
+                if (!prime) continue;
+                cdep = advance(xn, hi + hi, cscale);  // next step's first centre, v_0 = V
+            } else {
+                const float vn = vh + kick;
+                vout[g] = vn;
+                cdep = advance(xq, vn + vh, cscale);
             }
-            vout[g] = vn;
+        } else {
+            cdep = advance(xq, vh + vh, cscale);
         }
-        const uint32_t cq2 = advance(xq, vn + vh, cscale);
         // once the Picard iterate settles the next centre repeats bit for bit
-        if (MODE == 0 || cq2 != cq) stencil32(cq2, shift, s);
+        if (MODE == 0 || cdep != cq) stencil32(cdep, shift, s);
         u64* a = acc + s.i0;
-#define PIC_DEP(J) atomicAdd(&a[J], quanta<J>(s.w[J]));
-        PIC_DEP(0) PIC_DEP(1) PIC_DEP(2) PIC_DEP(3) PIC_DEP(4) PIC_DEP(5)
-        PIC_DEP(6) PIC_DEP(7) PIC_DEP(8) PIC_DEP(9) PIC_DEP(10)
-#undef PIC_DEP
+        PIC_DEPOSIT(a, s)
     }
     if (last) {
         // the step's kinetic diagnostics, one partial pair per tile (fixed
@@ -179,7 +197,7 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
             d2 = d2 + __shfl_down(d2, h, 64);
             d1 = d1 + __shfl_down(d1, h, 64);
         }
-        double* red = reinterpret_cast<double*>(lds);
+        double* red = reinterpret_cast<double*>(lds + NPAD + (NPAD + 1) / 2);
         __syncthreads();
         const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
         if ((threadIdx.x & 63) == 0) {
@@ -196,7 +214,7 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
             dpart[2 * blockIdx.x] = a;
             dpart[2 * blockIdx.x + 1] = b;
         }
-        return;
+        if (!prime) return;
     }
     __syncthreads();
     flush_charge32(acc, N, NPAD, racc + I.red0);
@@ -229,11 +247,11 @@ void launch_mid_push32(const RunBuffers& b, int mode, int k, bool last_slot, hip
     if (mode == 0)
         k_mid_push32<0><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.Xq[0], b.Xq[1], b.Vh[0], b.Vh[1],
                                                  b.Vl[0], b.Vl[1], b.vf[0], b.vf[1], b.E32, part, 0, b.halt, 0,
-                                                 b.dpart);
+                                                 b.dpart, 0);
     else
         k_mid_push32<1><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.Xq[0], b.Xq[1], b.Vh[0], b.Vh[1],
                                                  b.Vl[0], b.Vl[1], b.vf[0], b.vf[1], b.E32, part, k, b.halt, ls,
-                                                 b.dpart);
+                                                 b.dpart, b.prime_next);
 }
 
 void launch_diag_chunks32(const RunBuffers& b, hipStream_t s) {
@@ -242,7 +260,7 @@ void launch_diag_chunks32(const RunBuffers& b, hipStream_t s) {
 }
 
 int configure_kernels_mixed() {
-    const int bytes = 12 * pad_cells(kMaxN);
+    const int bytes = push32_lds_bytes(kMaxN);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_mid_push32<0>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_mid_push32<1>),

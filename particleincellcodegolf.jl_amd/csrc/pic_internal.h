@@ -89,6 +89,10 @@ struct RunBuffers {
     // into 2 and every later kernel returns at once.  The host then finishes
     // the step with the remaining sweeps and resumes.
     int32_t* halt;
+    // mixed midpoint: the final sweep also deposits the next step's first
+    // charge, so steps after the first carry no first-deposit launch
+    int32_t prime_next;
+    int64_t racc_cells;  // u64 words of the mixed accumulator
 };
 
 // kernels_1d.hip
@@ -106,6 +110,7 @@ bool mixed_available();
 
 // kernels_mixed.hip
 void launch_mid_push32(const RunBuffers& b, int mode, int k, bool last_slot, hipStream_t s);
+int push32_lds_bytes(int N);
 void launch_diag_chunks32(const RunBuffers& b, hipStream_t s);
 int configure_kernels_mixed();
 

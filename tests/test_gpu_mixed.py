@@ -113,3 +113,28 @@ def test_mixed_speculative_slots_match_full_slots():
     assert np.array_equal(Sa, Sb) and np.array_equal(Da, Db)
     for a, b in zip(A, B):
         assert np.array_equal(a, b)
+
+
+def test_fused_first_deposit_matches_a_fresh_start():
+    """The final sweep deposits the next step's first charge; a context that
+    starts from the downloaded state (and so deposits it in a launch of its
+    own) continues bit for bit."""
+    import picgolf
+    from picgolf import configs
+    cfg, x, v, _ = configs.scaled_two_stream(precision="mixed", P=2 ** 18, N=1024)
+    with picgolf.Context(cfg) as c:
+        c.upload(x, v)
+        D, S = c.run(20)
+        A = c.download()
+    with picgolf.Context(cfg) as c:
+        c.upload(x, v)
+        D1, S1 = c.run(10)
+        xm, vm, Em = c.download()
+    with picgolf.Context(cfg) as c:
+        c.upload(xm, vm, Em)
+        D2, S2 = c.run(10)
+        B = c.download()
+    assert np.array_equal(S[0], np.concatenate([S1[0], S2[0]]))
+    assert np.array_equal(D[0, :, 0], np.concatenate([D1[0, :, 0], D2[0, :, 0]]))
+    for a, b in zip(A, B):
+        assert np.array_equal(a, b)
