@@ -28,7 +28,7 @@ __host__ __device__ constexpr int pad_cells(int N) { return N + 2 * kHalo + 4; }
 int push32_lds_bytes(int N) { return 12 * pad_cells(N) + 8 * 4 + 256; }
 
 // Stencil of a fixed-point centre: i0 = nearest cell centre (Julia index,
-// N == 0) and the kWin weights, in 2^-30 quanta of a particle's charge, of the
+// N == 0) and the kWin weights, in 2^-30 units of a particle's charge, of the
 // cells i0-6 .. i0+4 (array index = Julia index - 1).
 struct Stencil32 {
     int32_t i0;
@@ -61,14 +61,27 @@ __device__ __forceinline__ uint32_t advance(uint32_t xq, float v_sum, float scal
     return xq + (uint32_t)(int32_t)rintf(v_sum * scale);
 }
 
-// Deposit quanta of window slot J, rounded half up; the |j| >= 4 tails are
-// clamped at zero (fit noise can dip below zero there; nearer cells are never
-// below 1e4 quanta).
+// Deposit quanta of window slot J: the weight in units of 2^-30 of a
+// particle's charge, rounded half up; the |j| >= 4 tails are clamped at zero
+// (fit noise can dip below zero there; nearer cells are never below 1e4
+// quanta).
 template <int J>
-__device__ __forceinline__ u64 quanta(float wq) {
+__device__ __forceinline__ uint32_t quanta(float w30) {
     constexpr int d = J - kStencil32;
-    const float r = wq + 0.5f;
-    return (u64)__float2uint_rz((d >= 4 || d <= -4) ? fmaxf(r, 0.0f) : r);
+    const float r = w30 + 0.5f;
+    return __float2uint_rz((d >= 4 || d <= -4) ? fmaxf(r, 0.0f) : r);
+}
+
+// Deposit of one particle's stencil: 11 exact integer adds into the block's
+// LDS accumulator.  Called by every lane of a wave (dep marks lanes with a
+// stencil to deposit).
+__device__ __forceinline__ void lane_deposit(u64* __restrict__ acc, const Stencil32& s, bool dep) {
+    if (!dep) return;
+    u64* a = acc + s.i0;
+#define PIC_DEP(J) atomicAdd(&a[J], (u64)quanta<J>(s.w[J]));
+    PIC_DEP(0) PIC_DEP(1) PIC_DEP(2) PIC_DEP(3) PIC_DEP(4) PIC_DEP(5) PIC_DEP(6) PIC_DEP(7) PIC_DEP(8) PIC_DEP(9)
+    PIC_DEP(10)
+#undef PIC_DEP
 }
 
 // Adds this block's halo-padded LDS charge into the instance's per-cell
@@ -81,15 +94,6 @@ __device__ __forceinline__ void flush_charge32(const u64* acc, int N, int NPAD, 
         if (v) atomicAdd(racc + c, v);
     }
 }
-
-// Adds a stencil's quanta to the LDS accumulator window starting at a.
-#define PIC_DEP(J) atomicAdd(&a[J], quanta<J>(st.w[J]));
-#define PIC_DEPOSIT(a, st_)                                                     \
-    {                                                                           \
-        const Stencil32& st = st_;                                              \
-        PIC_DEP(0) PIC_DEP(1) PIC_DEP(2) PIC_DEP(3) PIC_DEP(4) PIC_DEP(5)      \
-        PIC_DEP(6) PIC_DEP(7) PIC_DEP(8) PIC_DEP(9) PIC_DEP(10)                \
-    }
 
 template <int MODE>
 __global__ void __launch_bounds__(1024)
@@ -144,50 +148,59 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     const int64_t base = I.poff + T.start;
     const double Pd = (double)I.P;
     double d2 = 0.0, d1 = 0.0;  // final sweep: sum(v^2), sum(v/P) over this thread's particles
-    for (int64_t p = threadIdx.x; p < T.count; p += blockDim.x) {
-        const int64_t g = base + p;
-        const uint32_t xq = Xq[g];
-        const float vh = Vh[g];
+    // every lane of a wave runs the same iterations (the deposit reduces
+    // across the wave); lanes past the tile's end carry no particle
+    const int64_t nit = (T.count + blockDim.x - 1) / blockDim.x;
+    for (int64_t it = 0; it < nit; ++it) {
+        const int64_t p = threadIdx.x + it * blockDim.x;
+        const bool valid = p < T.count;
+        const int64_t g = base + (valid ? p : 0);
         Stencil32 s;
-        uint32_t cq = 0;
-        uint32_t cdep;  // centre of this particle's deposit
-        if (MODE == 1) {
-            const float vk = (k == 0) ? vh : vin[g];
-            cq = advance(xq, vk + vh, cscale);
-            stencil32(cq, shift, s);
-            const float* e = Eld + s.i0;  // slot of array cell i0-6
-            float kick = 0.0f;
+        s.i0 = 0;
+        bool dep = false;
+        if (valid) {
+            const uint32_t xq = Xq[g];
+            const float vh = Vh[g];
+            uint32_t cq = 0;
+            uint32_t cdep;  // centre of this particle's deposit
+            dep = true;
+            if (MODE == 1) {
+                const float vk = (k == 0) ? vh : vin[g];
+                cq = advance(xq, vk + vh, cscale);
+                stencil32(cq, shift, s);
+                const float* e = Eld + s.i0;  // slot of array cell i0-6
+                float kick = 0.0f;
 #pragma unroll
-            for (int j = 0; j < kWin; ++j) kick = fmaf(e[j], s.w[j], kick);
-            if (last) {
-                // V_new = (V_hi + V_lo) + kick, renormalised (two-sum)
-                const float sum = vh + kick;
-                const float bb = sum - vh;
-                const float err = (vh - (sum - bb)) + (kick - bb);
-                const float t = Vl[g] + err;
-                const float hi = sum + t;
-                const float lo = t - (hi - sum);
-                const uint32_t xn = advance(xq, vk + vh, xscale);
-                Xo[g] = xn;
-                Vho[g] = hi;
-                Vlo[g] = lo;
-                const double vd = (double)hi + (double)lo;
-                d2 = d2 + vd * vd;
-                d1 = d1 + vd / Pd;
-                if (!prime) continue;
-                cdep = advance(xn, hi + hi, cscale);  // next step's first centre, v_0 = V
+                for (int j = 0; j < kWin; ++j) kick = fmaf(e[j], s.w[j], kick);
+                if (last) {
+                    // V_new = (V_hi + V_lo) + kick, renormalised (two-sum)
+                    const float sum = vh + kick;
+                    const float bb = sum - vh;
+                    const float err = (vh - (sum - bb)) + (kick - bb);
+                    const float t = Vl[g] + err;
+                    const float hi = sum + t;
+                    const float lo = t - (hi - sum);
+                    const uint32_t xn = advance(xq, vk + vh, xscale);
+                    Xo[g] = xn;
+                    Vho[g] = hi;
+                    Vlo[g] = lo;
+                    const double vd = (double)hi + (double)lo;
+                    d2 = d2 + vd * vd;
+                    d1 = d1 + vd / Pd;
+                    dep = prime != 0;
+                    cdep = advance(xn, hi + hi, cscale);  // next step's first centre, v_0 = V
+                } else {
+                    const float vn = vh + kick;
+                    vout[g] = vn;
+                    cdep = advance(xq, vn + vh, cscale);
+                }
             } else {
-                const float vn = vh + kick;
-                vout[g] = vn;
-                cdep = advance(xq, vn + vh, cscale);
+                cdep = advance(xq, vh + vh, cscale);
             }
-        } else {
-            cdep = advance(xq, vh + vh, cscale);
+            // once the Picard iterate settles the next centre repeats bit for bit
+            if (dep && (MODE == 0 || cdep != cq)) stencil32(cdep, shift, s);
         }
-        // once the Picard iterate settles the next centre repeats bit for bit
-        if (MODE == 0 || cdep != cq) stencil32(cdep, shift, s);
-        u64* a = acc + s.i0;
-        PIC_DEPOSIT(a, s)
+        if (deposit) lane_deposit(acc, s, dep);
     }
     if (last) {
         // the step's kinetic diagnostics, one partial pair per tile (fixed
