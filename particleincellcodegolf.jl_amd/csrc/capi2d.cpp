@@ -24,8 +24,13 @@ struct pic2d_ctx {
     double* tw = nullptr;
     bool uploaded = false;
     int64_t step = 0;                // steps since upload
-    hipGraphExec_t g_plain = nullptr;
-    hipGraphExec_t g_diag = nullptr;
+    // step graphs per buffer parity of the mixed particle arrays
+    hipGraphExec_t g_plain[2] = {nullptr, nullptr};
+    hipGraphExec_t g_diag[2] = {nullptr, nullptr};
+    int par = 0;                     // mixed: which buffer set holds the particles
+    int64_t since_sort = 0;          // mixed: steps since the last tile sort
+    uint32_t *xq[2] = {}, *yq[2] = {}, *pid[2] = {};
+    float *vx[2] = {}, *vy[2] = {}, *vz[2] = {};
     // operator scratch (grown on demand)
     void* op[8] = {};
     size_t op_bytes[8] = {};
@@ -92,9 +97,38 @@ static int scratch(pic2d_ctx* c, int slot, size_t bytes, void** out) {
 }
 
 static void free_graphs(pic2d_ctx* c) {
-    if (c->g_plain) (void)hipGraphExecDestroy(c->g_plain);
-    if (c->g_diag) (void)hipGraphExecDestroy(c->g_diag);
-    c->g_plain = c->g_diag = nullptr;
+    for (int k = 0; k < 2; ++k) {
+        if (c->g_plain[k]) (void)hipGraphExecDestroy(c->g_plain[k]);
+        if (c->g_diag[k]) (void)hipGraphExecDestroy(c->g_diag[k]);
+        c->g_plain[k] = c->g_diag[k] = nullptr;
+    }
+}
+
+// Points the mixed particle arrays of the launch descriptors at buffer set par.
+static void set_par(pic2d_ctx* c, int par) {
+    Buffers2D& b = c->b;
+    c->par = par;
+    b.Xq = c->xq[par];
+    b.Yq = c->yq[par];
+    b.VXf = c->vx[par];
+    b.VYf = c->vy[par];
+    b.VZf = c->vz[par];
+    b.pid = c->pid[par];
+    b.Xq_alt = c->xq[1 - par];
+    b.Yq_alt = c->yq[1 - par];
+    b.VXf_alt = c->vx[1 - par];
+    b.VYf_alt = c->vy[1 - par];
+    b.VZf_alt = c->vz[1 - par];
+    b.pid_alt = c->pid[1 - par];
+}
+
+// Regroups the mixed particles by tile (into the other buffer set).
+static int tile_sort(pic2d_ctx* c) {
+    launch2d_tile_sort(c->b, c->stream);
+    HIPCHK(c, hipGetLastError());
+    set_par(c, 1 - c->par);
+    c->since_sort = 0;
+    return PIC_OK;
 }
 
 static void release(pic2d_ctx* c) {
@@ -207,9 +241,24 @@ extern "C" int pic2d_create(pic2d_ctx** out, const pic2d_config* cfg) {
     }
     int rc;
     if (cfg->precision == PIC_PREC_MIXED) {
-        if ((rc = dalloc(c, &b.Xq, P)) || (rc = dalloc(c, &b.Yq, P)) || (rc = dalloc(c, &b.VXf, P)) ||
-            (rc = dalloc(c, &b.VYf, P)) || (rc = dalloc(c, &b.VZf, P)) || (rc = dalloc(c, &b.racc, G)))
+        for (int k = 0; k < 2; ++k)
+            if ((rc = dalloc(c, &c->xq[k], P)) || (rc = dalloc(c, &c->yq[k], P)) || (rc = dalloc(c, &c->vx[k], P)) ||
+                (rc = dalloc(c, &c->vy[k], P)) || (rc = dalloc(c, &c->vz[k], P)) || (rc = dalloc(c, &c->pid[k], P)))
+                return fail(rc);
+        if ((rc = dalloc(c, &b.racc, G))) return fail(rc);
+        set_par(c, 0);
+        Tiles2D& T = b.T;
+        T.TX = std::min(kTile2D, NX);
+        T.TY = std::min(kTile2D, NY);
+        T.ntx = NX / T.TX;
+        T.nty = NY / T.TY;
+        T.ntiles = T.ntx * T.nty;
+        const int64_t per_tile = (P + T.ntiles - 1) / T.ntiles;
+        T.splits = (int)std::max<int64_t>(1, std::min<int64_t>(64, (per_tile + kTileBlockP - 1) / kTileBlockP));
+        if ((rc = dalloc(c, &T.count, T.ntiles)) || (rc = dalloc(c, &T.start, T.ntiles + 1)) ||
+            (rc = dalloc(c, &T.cursor, T.ntiles)))
             return fail(rc);
+        if (hipMemset(T.count, 0, sizeof(int32_t) * T.ntiles) != hipSuccess) return fail(PIC_EHIP);
     } else {
         if ((rc = dalloc(c, &b.X, P)) || (rc = dalloc(c, &b.Y, P)) || (rc = dalloc(c, &b.VX, P)) ||
             (rc = dalloc(c, &b.VY, P)) || (rc = dalloc(c, &b.VZ, P)) || (rc = dalloc(c, &b.racc, 3 * G)))
@@ -219,7 +268,7 @@ extern "C" int pic2d_create(pic2d_ctx** out, const pic2d_config* cfg) {
         (rc = dalloc(c, &b.T3, G)) || (rc = dalloc(c, &b.Ex, G)) || (rc = dalloc(c, &b.Ey, G)) ||
         (rc = dalloc(c, &b.phi, G)) || (rc = dalloc(c, &b.E32, G)) ||
         (rc = dalloc(c, &b.gpart, (G + kTreeChunk - 1) / kTreeChunk)) ||
-        (rc = dalloc(c, &b.ppart, 3 * ((P + kTreeChunk - 1) / kTreeChunk))) || (rc = dalloc(c, &b.drow, 1)) ||
+        (rc = dalloc(c, &b.ppart, 6 * ((P + kTreeChunk - 1) / kTreeChunk))) || (rc = dalloc(c, &b.drow, 1)) ||
         (rc = dalloc(c, &b.flag, 1)))
         return fail(rc);
     const std::vector<double> twp = pass_twiddle_table();
@@ -277,6 +326,8 @@ extern "C" int pic2d_upload(pic2d_ctx* c, const double* x, const double* y, cons
             for (int64_t p = 0; p < P; ++p) f[p] = (float)vs[k][p];
             HIPCHK(c, hipMemcpy(ds[k], f.data(), 4 * P, hipMemcpyHostToDevice));
         }
+        for (int64_t p = 0; p < P; ++p) q[p] = (uint32_t)p;
+        HIPCHK(c, hipMemcpy(b.pid, q.data(), 4 * P, hipMemcpyHostToDevice));
     } else {
         HIPCHK(c, hipMemcpy(b.X, x, 8 * P, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(b.Y, y, 8 * P, hipMemcpyHostToDevice));
@@ -293,6 +344,11 @@ extern "C" int pic2d_upload(pic2d_ctx* c, const double* x, const double* y, cons
     HIPCHK(c, hipMemcpy(b.Ey, ey.data(), 8 * G, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(b.E32, e32.data(), sizeof(float2) * G, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemset(b.flag, 0, sizeof(int32_t)));
+    if (c->cfg.precision == PIC_PREC_MIXED) {
+        int rc = tile_sort(c);
+        if (rc) return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     c->step = 0;
     c->uploaded = true;
     return PIC_OK;
@@ -310,11 +366,13 @@ extern "C" int pic2d_download(pic2d_ctx* c, double* x, double* y, double* vx, do
     const int64_t P = c->cfg.P, G = (int64_t)c->cfg.NX * c->cfg.NY;
     Buffers2D& b = c->b;
     if (c->cfg.precision == PIC_PREC_MIXED) {
-        std::vector<uint32_t> q(P);
+        // held particles are grouped by tile; pid gives each one's caller index
+        std::vector<uint32_t> q(P), id(P);
         std::vector<float> f(P);
+        HIPCHK(c, hipMemcpy(id.data(), b.pid, 4 * P, hipMemcpyDeviceToHost));
         auto tx = [&](uint32_t* src, double* dst) -> int {
             HIPCHK(c, hipMemcpy(q.data(), src, 4 * P, hipMemcpyDeviceToHost));
-            for (int64_t p = 0; p < P; ++p) dst[p] = q[p] ? std::ldexp((double)q[p], -32) : 1.0;
+            for (int64_t p = 0; p < P; ++p) dst[id[p]] = q[p] ? std::ldexp((double)q[p], -32) : 1.0;
             return PIC_OK;
         };
         int rc;
@@ -325,7 +383,7 @@ extern "C" int pic2d_download(pic2d_ctx* c, double* x, double* y, double* vx, do
         for (int k = 0; k < 3; ++k) {
             if (!ds[k]) continue;
             HIPCHK(c, hipMemcpy(f.data(), ss[k], 4 * P, hipMemcpyDeviceToHost));
-            for (int64_t p = 0; p < P; ++p) ds[k][p] = (double)f[p];
+            for (int64_t p = 0; p < P; ++p) ds[k][id[p]] = (double)f[p];
         }
     } else {
         if (x) HIPCHK(c, hipMemcpy(x, b.X, 8 * P, hipMemcpyDeviceToHost));
@@ -384,12 +442,18 @@ extern "C" int pic2d_run(pic2d_ctx* c, int64_t nsteps, double* K) {
         b.diag_cap = rows;
         free_graphs(c);  // the graphs hold the old pointer
     }
-    if (!c->g_plain && (rc = capture(c, false, &c->g_plain))) return rc;
-    if (!c->g_diag && (rc = capture(c, true, &c->g_diag))) return rc;
     const int64_t zero = 0;
     HIPCHK(c, hipMemcpyAsync(b.drow, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
-    for (int64_t t = t0 + 1; t <= t1; ++t)
-        HIPCHK(c, hipGraphLaunch(t % ns == 0 ? c->g_diag : c->g_plain, c->stream));
+    const bool mixed = c->cfg.precision == PIC_PREC_MIXED;
+    const int64_t every = c->cfg.sort_every > 0 ? c->cfg.sort_every : kSortEvery2D;
+    for (int64_t t = t0 + 1; t <= t1; ++t) {
+        if (mixed && c->since_sort >= every && (rc = tile_sort(c))) return rc;
+        const int k = c->par;
+        if (!c->g_plain[k] && (rc = capture(c, false, &c->g_plain[k]))) return rc;
+        if (!c->g_diag[k] && (rc = capture(c, true, &c->g_diag[k]))) return rc;
+        HIPCHK(c, hipGraphLaunch(t % ns == 0 ? c->g_diag[k] : c->g_plain[k], c->stream));
+        ++c->since_sort;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->step = t1;
     if (K && rows > 0) HIPCHK(c, hipMemcpy(K, b.diag, sizeof(double) * 5 * rows, hipMemcpyDeviceToHost));

@@ -194,23 +194,71 @@ k2d_boris64(double* __restrict__ VX, double* __restrict__ VY, double* __restrict
     }
 }
 
-// Mixed particle loop: FP32 gather with exact quanta weights, FP32 Boris,
-// fixed-point move (x += rint(vx*dt*2^32) mod 2^32), exact quanta deposit.
-__global__ void __launch_bounds__(kPush2DThreads)
-k2d_push32(Desc2D D, uint32_t* __restrict__ Xq, uint32_t* __restrict__ Yq, float* __restrict__ VX,
-           float* __restrict__ VY, float* __restrict__ VZ, const float2* __restrict__ E32, u64* __restrict__ racc,
-           float dt_2, float t1, float tscale, float xstep) {
+// ---------------------------------------------------------------------------
+// Mixed particle loop: FP32 gather with exact quanta weights, FP32 Boris about
+// x, fixed-point move (x += rint(vx*dt*2^32) mod 2^32), exact quanta deposit.
+// Particles are held by tile.  A block handles part of one tile's
+// particles: it stages the field of the tile and kHalo2D cells around it in
+// LDS, gathers and deposits there, and adds the staged charge to the grid
+// accumulator at the end.  A particle whose cells fall outside the staged
+// window (it drifted further than the halo since the last sort) reads and adds
+// through global memory instead, so every result is the same as without
+// tiles: the charge sums are exact integers, each particle's update depends
+// only on its own state and the field.
+// ---------------------------------------------------------------------------
+struct TileWin {
+    int32_t ox, oy;  // global cell of window slot (0, 0) (may be negative: periodic)
+    int32_t wx, wy;  // window size in cells
+};
+
+__device__ __forceinline__ int32_t win_rel(int32_t c, int32_t o, int32_t n) {
+    int32_t r = c - o;
+    r += (r < 0) ? n : 0;
+    r -= (r >= n) ? n : 0;
+    return r;
+}
+
+__global__ void __launch_bounds__(kTileThreads)
+k2d_push32t(Desc2D D, Tiles2D T, uint32_t* __restrict__ Xq, uint32_t* __restrict__ Yq, float* __restrict__ VX,
+            float* __restrict__ VY, float* __restrict__ VZ, const float2* __restrict__ E32, u64* __restrict__ racc,
+            float dt_2, float t1, float tscale, float xstep) {
+    __shared__ float2 ew[(kTile2D + 2 * kHalo2D) * (kTile2D + 2 * kHalo2D)];
+    __shared__ u64 aw[(kTile2D + 2 * kHalo2D) * (kTile2D + 2 * kHalo2D)];
     constexpr float kQ = 1.0f / (float)(1u << kQuanta2D);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < D.P; p += stride) {
+    const int tile = blockIdx.x / T.splits, part = blockIdx.x % T.splits;
+    const int tx = tile % T.ntx, ty = tile / T.ntx;
+    const int wx = T.TX + 2 * kHalo2D, wy = T.TY + 2 * kHalo2D, nw = wx * wy;
+    const int ox = tx * T.TX - kHalo2D, oy = ty * T.TY - kHalo2D;
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) {
+        const int gx = (ox + i % wx + D.NX) % D.NX, gy = (oy + i / wx + D.NY) % D.NY;
+        ew[i] = E32[gx + D.NX * gy];
+        aw[i] = 0;
+    }
+    __syncthreads();
+    const int64_t a0 = T.start[tile], a1 = T.start[tile + 1];
+    const int64_t len = a1 - a0;
+    const int64_t p0 = a0 + len * part / T.splits, p1 = a0 + len * (part + 1) / T.splits;
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
         uint32_t xq = Xq[p], yq = Yq[p];
         float vx = VX[p], vy = VY[p], vz = VZ[p];
         {
             const AreaQ ax = areaq(xq, D.NX, D.sx, D.pairing), ay = areaq(yq, D.NY, D.sy, D.pairing);
             const float wx0 = (float)ax.q0 * kQ, wx1 = (float)ax.q1 * kQ;
             const float wy0 = (float)ay.q0 * kQ, wy1 = (float)ay.q1 * kQ;
-            const float2 e00 = E32[ax.c0 + D.NX * ay.c0], e10 = E32[ax.c1 + D.NX * ay.c0];
-            const float2 e01 = E32[ax.c0 + D.NX * ay.c1], e11 = E32[ax.c1 + D.NX * ay.c1];
+            const int32_t rx0 = win_rel(ax.c0, ox, D.NX), rx1 = win_rel(ax.c1, ox, D.NX);
+            const int32_t ry0 = win_rel(ay.c0, oy, D.NY), ry1 = win_rel(ay.c1, oy, D.NY);
+            float2 e00, e10, e01, e11;
+            if (rx0 < wx && rx1 < wx && ry0 < wy && ry1 < wy) {
+                e00 = ew[rx0 + wx * ry0];
+                e10 = ew[rx1 + wx * ry0];
+                e01 = ew[rx0 + wx * ry1];
+                e11 = ew[rx1 + wx * ry1];
+            } else {
+                e00 = E32[ax.c0 + D.NX * ay.c0];
+                e10 = E32[ax.c1 + D.NX * ay.c0];
+                e01 = E32[ax.c0 + D.NX * ay.c1];
+                e11 = E32[ax.c1 + D.NX * ay.c1];
+            }
             float f1 = 0.0f, f2 = 0.0f;
             float wxy = wx0 * wy0;
             f1 = fmaf(e00.x, wxy, f1);
@@ -224,7 +272,6 @@ k2d_push32(Desc2D D, uint32_t* __restrict__ Xq, uint32_t* __restrict__ Yq, float
             wxy = wx1 * wy1;
             f1 = fmaf(e11.x, wxy, f1);
             f2 = fmaf(e11.y, wxy, f2);
-            // Boris about x
             const float e1 = f1 * dt_2, e2 = f2 * dt_2;
             const float m1 = vx + e1, m2 = vy + e2, m3 = vz;
             const float u2 = m2 + m3 * t1, u3 = m3 - m2 * t1;
@@ -242,13 +289,197 @@ k2d_push32(Desc2D D, uint32_t* __restrict__ Xq, uint32_t* __restrict__ Yq, float
         VY[p] = vy;
         VZ[p] = vz;
         const AreaQ bx = areaq(xq, D.NX, D.sx, D.pairing), by = areaq(yq, D.NY, D.sy, D.pairing);
-        const u64 a0 = (u64)by.q0, a1 = (u64)by.q1;
-        const int64_t r0 = (int64_t)D.NX * by.c0, r1 = (int64_t)D.NX * by.c1;
-        u64 v;
-        if ((v = (u64)bx.q0 * a0)) atomicAdd(racc + r0 + bx.c0, v);
-        if ((v = (u64)bx.q1 * a0)) atomicAdd(racc + r0 + bx.c1, v);
-        if ((v = (u64)bx.q0 * a1)) atomicAdd(racc + r1 + bx.c0, v);
-        if ((v = (u64)bx.q1 * a1)) atomicAdd(racc + r1 + bx.c1, v);
+        const u64 q00 = (u64)bx.q0 * by.q0, q10 = (u64)bx.q1 * by.q0, q01 = (u64)bx.q0 * by.q1,
+                  q11 = (u64)bx.q1 * by.q1;
+        const int32_t rx0 = win_rel(bx.c0, ox, D.NX), rx1 = win_rel(bx.c1, ox, D.NX);
+        const int32_t ry0 = win_rel(by.c0, oy, D.NY), ry1 = win_rel(by.c1, oy, D.NY);
+        if (rx0 < wx && rx1 < wx && ry0 < wy && ry1 < wy) {
+            if (q00) atomicAdd(&aw[rx0 + wx * ry0], q00);
+            if (q10) atomicAdd(&aw[rx1 + wx * ry0], q10);
+            if (q01) atomicAdd(&aw[rx0 + wx * ry1], q01);
+            if (q11) atomicAdd(&aw[rx1 + wx * ry1], q11);
+        } else {
+            const int64_t r0 = (int64_t)D.NX * by.c0, r1 = (int64_t)D.NX * by.c1;
+            if (q00) atomicAdd(racc + r0 + bx.c0, q00);
+            if (q10) atomicAdd(racc + r0 + bx.c1, q10);
+            if (q01) atomicAdd(racc + r1 + bx.c0, q01);
+            if (q11) atomicAdd(racc + r1 + bx.c1, q11);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) {
+        const u64 v = aw[i];
+        if (v) {
+            const int gx = (ox + i % wx + D.NX) % D.NX, gy = (oy + i / wx + D.NY) % D.NY;
+            atomicAdd(racc + gx + (int64_t)D.NX * gy, v);
+        }
+    }
+}
+
+// Tile of a held particle: the tile holding its cell.
+__device__ __forceinline__ int32_t tile_of(const Desc2D& D, const Tiles2D& T, uint32_t xq, uint32_t yq) {
+    const int32_t cx = (int32_t)(xq >> D.sx), cy = (int32_t)(yq >> D.sy);
+    return (cx / T.TX) + T.ntx * (cy / T.TY);
+}
+
+// Sort, pass 1: particles per tile (block histogram in LDS, then one global
+// add per tile).
+__global__ void __launch_bounds__(256)
+k2d_tile_count(Desc2D D, Tiles2D T, const uint32_t* __restrict__ Xq, const uint32_t* __restrict__ Yq) {
+    extern __shared__ int32_t hist[];
+    for (int i = threadIdx.x; i < T.ntiles; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < D.P; p += stride)
+        atomicAdd(&hist[tile_of(D, T, Xq[p], Yq[p])], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < T.ntiles; i += blockDim.x)
+        if (hist[i]) atomicAdd(&T.count[i], hist[i]);
+}
+
+// Sort, pass 2 (one block): exclusive scan of the counts into start and the
+// cursors; counts cleared for the next sort.
+__global__ void __launch_bounds__(1024) k2d_tile_scan(Tiles2D T) {
+    __shared__ int32_t part[1024];
+    const int per = (T.ntiles + blockDim.x - 1) / blockDim.x;
+    const int i0 = threadIdx.x * per;
+    int32_t s = 0;
+    for (int i = i0; i < i0 + per && i < T.ntiles; ++i) s += T.count[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t run = 0;
+        for (int k = 0; k < (int)blockDim.x; ++k) {
+            const int32_t v = part[k];
+            part[k] = run;
+            run += v;
+        }
+        T.start[T.ntiles] = run;
+    }
+    __syncthreads();
+    int32_t run = part[threadIdx.x];
+    for (int i = i0; i < i0 + per && i < T.ntiles; ++i) {
+        T.start[i] = run;
+        T.cursor[i] = run;
+        run += T.count[i];
+        T.count[i] = 0;
+    }
+}
+
+// Sort, pass 3: move every particle to a free slot of its tile's range in the
+// other buffers.  A block reserves one contiguous run per tile it holds, then
+// hands out slots from LDS; the order inside a tile is arbitrary (results do
+// not depend on it).
+__global__ void __launch_bounds__(256)
+k2d_tile_scatter(Desc2D D, Tiles2D T, const uint32_t* __restrict__ Xq, const uint32_t* __restrict__ Yq,
+                 const float* __restrict__ VX, const float* __restrict__ VY, const float* __restrict__ VZ,
+                 const uint32_t* __restrict__ pid, uint32_t* __restrict__ Xo, uint32_t* __restrict__ Yo,
+                 float* __restrict__ VXo, float* __restrict__ VYo, float* __restrict__ VZo,
+                 uint32_t* __restrict__ pido, int64_t per_block) {
+    extern __shared__ int32_t hist[];
+    for (int i = threadIdx.x; i < T.ntiles; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * per_block;
+    const int64_t b1 = b0 + per_block < D.P ? b0 + per_block : D.P;
+    for (int64_t p = b0 + threadIdx.x; p < b1; p += blockDim.x) atomicAdd(&hist[tile_of(D, T, Xq[p], Yq[p])], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < T.ntiles; i += blockDim.x)
+        if (hist[i]) hist[i] = atomicAdd(&T.cursor[i], hist[i]);
+    __syncthreads();
+    for (int64_t p = b0 + threadIdx.x; p < b1; p += blockDim.x) {
+        const uint32_t xq = Xq[p], yq = Yq[p];
+        const int32_t d = atomicAdd(&hist[tile_of(D, T, xq, yq)], 1);
+        Xo[d] = xq;
+        Yo[d] = yq;
+        VXo[d] = VX[p];
+        VYo[d] = VY[p];
+        VZo[d] = VZ[p];
+        pido[d] = pid[p];
+    }
+}
+
+__device__ double tree_strided(double* red, double* vals, int stride, int64_t n);
+
+// Mixed kinetic diagnostics, independent of the particle order: per chunk of
+// 4096 held particles the exact integer sums of (vx^2 + vy^2) on a 2^-50 grid
+// and of vx, vy on a 2^-40 grid.
+typedef __int128 i128;
+
+__device__ __forceinline__ int64_t fix_round(double a, int q) {
+    return (int64_t)rint(ldexp(a, q));
+}
+
+__global__ void __launch_bounds__(256)
+k2d_diag_parts32x(const float* __restrict__ VX, const float* __restrict__ VY, int64_t P, i128* __restrict__ ppart) {
+    __shared__ i128 red[3 * 256];
+    const int64_t s = (int64_t)blockIdx.x * kTreeChunk;
+    const int64_t n = P - s < kTreeChunk ? P - s : kTreeChunk;
+    i128 a = 0, b = 0, c = 0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const double vx = (double)VX[s + i], vy = (double)VY[s + i];
+        a += (i128)fix_round(vx * vx + vy * vy, 50);
+        b += (i128)fix_round(vx, 40);
+        c += (i128)fix_round(vy, 40);
+    }
+    red[threadIdx.x] = a;
+    red[256 + threadIdx.x] = b;
+    red[512 + threadIdx.x] = c;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            red[threadIdx.x] += red[threadIdx.x + h];
+            red[256 + threadIdx.x] += red[256 + threadIdx.x + h];
+            red[512 + threadIdx.x] += red[512 + threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ppart[3 * blockIdx.x] = red[0];
+        ppart[3 * blockIdx.x + 1] = red[256];
+        ppart[3 * blockIdx.x + 2] = red[512];
+    }
+}
+
+__device__ __forceinline__ double i128_to_double(i128 v, int q) {
+    return v < 0 ? -fxp_to((u128)(-v), q) : fxp_to((u128)v, q);
+}
+
+__global__ void __launch_bounds__(256)
+k2d_diag_final32x(double* __restrict__ gpart, int64_t ng, const i128* __restrict__ ppart, int64_t np, int64_t G,
+                  int64_t P, double w, double* __restrict__ out, int64_t* __restrict__ drow, int64_t cap) {
+    __shared__ double redd[kTreeLanes + 8];
+    __shared__ i128 red[3 * 256];
+    const double k1 = tree_strided(redd, gpart, 1, ng) / (double)G;
+    i128 a = 0, b = 0, c = 0;
+    for (int64_t i = threadIdx.x; i < np; i += blockDim.x) {
+        a += ppart[3 * i];
+        b += ppart[3 * i + 1];
+        c += ppart[3 * i + 2];
+    }
+    red[threadIdx.x] = a;
+    red[256 + threadIdx.x] = b;
+    red[512 + threadIdx.x] = c;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            red[threadIdx.x] += red[threadIdx.x + h];
+            red[256 + threadIdx.x] += red[256 + threadIdx.x + h];
+            red[512 + threadIdx.x] += red[512 + threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int64_t row = drow ? *drow : 0;
+        if (row < cap) {
+            double* o = out + 5 * row;
+            const double k2 = i128_to_double(red[0], 50) * w;
+            o[0] = k1;
+            o[1] = k2;
+            o[2] = k1 + k2;
+            o[3] = i128_to_double(red[256], 40) / (double)P;
+            o[4] = i128_to_double(red[512], 40) / (double)P;
+        }
+        if (drow) *drow = row + 1;
     }
 }
 
@@ -537,13 +768,23 @@ void launch2d_push(const Buffers2D& b, hipStream_t s) {
     if (b.precision == 1) {
         const double t1 = b.B0 * b.dt / 2.0;
         const double tscale = 2.0 / (1.0 + t1 * t1);
-        k2d_push32<<<b.blocks_push, kPush2DThreads, 0, s>>>(b.D, b.Xq, b.Yq, b.VXf, b.VYf, b.VZf, b.E32, b.racc,
-                                                          (float)(b.dt / 2.0), (float)t1, (float)tscale,
-                                                          (float)(b.dt * 4294967296.0));
+        k2d_push32t<<<b.T.ntiles * b.T.splits, kTileThreads, 0, s>>>(
+            b.D, b.T, b.Xq, b.Yq, b.VXf, b.VYf, b.VZf, b.E32, b.racc, (float)(b.dt / 2.0), (float)t1,
+            (float)tscale, (float)(b.dt * 4294967296.0));
     } else {
         k2d_push64<0><<<b.blocks_push, kPush2DThreads, 0, s>>>(b.D, b.X, b.Y, b.VX, b.VY, b.VZ, b.Ex, b.Ey, b.racc,
                                                              b.dt, b.B0);
     }
+}
+
+void launch2d_tile_sort(const Buffers2D& b, hipStream_t s) {
+    const size_t lds = sizeof(int32_t) * (size_t)b.T.ntiles;
+    const int64_t per = 16384;
+    const int nb = (int)((b.D.P + per - 1) / per);
+    k2d_tile_count<<<nb, 256, lds, s>>>(b.D, b.T, b.Xq, b.Yq);
+    k2d_tile_scan<<<1, 1024, 0, s>>>(b.T);
+    k2d_tile_scatter<<<nb, 256, lds, s>>>(b.D, b.T, b.Xq, b.Yq, b.VXf, b.VYf, b.VZf, b.pid, b.Xq_alt, b.Yq_alt,
+                                          b.VXf_alt, b.VYf_alt, b.VZf_alt, b.pid_alt, per);
 }
 
 void launch2d_solve(const Buffers2D& b, int src, const double* rho_in, bool want_phi, hipStream_t s) {
@@ -574,10 +815,14 @@ void launch2d_diag(const Buffers2D& b, hipStream_t s) {
     const int64_t G = (int64_t)b.D.NX * b.D.NY, P = b.D.P;
     const int64_t ng = (G + kTreeChunk - 1) / kTreeChunk, np = (P + kTreeChunk - 1) / kTreeChunk;
     k2d_diag_grid<<<(unsigned)ng, 256, 0, s>>>(b.Ex, b.Ey, G, b.gpart);
-    if (b.precision == 1)
-        k2d_diag_parts<float><<<(unsigned)np, 256, 0, s>>>(b.VXf, b.VYf, P, b.D.w, b.ppart);
-    else
-        k2d_diag_parts<double><<<(unsigned)np, 256, 0, s>>>(b.VX, b.VY, P, b.D.w, b.ppart);
+    if (b.precision == 1) {
+        // exact sums: the held particle order changes with every tile sort
+        i128* pp = reinterpret_cast<i128*>(b.ppart);
+        k2d_diag_parts32x<<<(unsigned)np, 256, 0, s>>>(b.VXf, b.VYf, P, pp);
+        k2d_diag_final32x<<<1, 256, 0, s>>>(b.gpart, ng, pp, np, G, P, b.D.w, b.diag, b.drow, b.diag_cap);
+        return;
+    }
+    k2d_diag_parts<double><<<(unsigned)np, 256, 0, s>>>(b.VX, b.VY, P, b.D.w, b.ppart);
     k2d_diag_final<<<1, 256, 0, s>>>(b.gpart, ng, b.ppart, np, G, P, b.diag, b.drow, b.diag_cap);
 }
 

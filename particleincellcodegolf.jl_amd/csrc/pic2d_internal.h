@@ -13,6 +13,20 @@ constexpr int kLimb2D = 42;         // FP64 exact deposit: 3 limbs of 42 bits (P
 constexpr int kQuanta2D = 20;       // mixed: area weight quanta per axis, 2^-20 of a cell
 constexpr int64_t kMaxP2D64 = int64_t(1) << 22;
 constexpr int64_t kMaxP2D32 = int64_t(1) << 24;
+// mixed path: particles held in tiles of kTile2D x kTile2D cells; a push
+// block stages its tile's field and charge with kHalo2D cells around it
+constexpr int kTile2D = 16;
+constexpr int kHalo2D = 4;
+constexpr int kTileBlockP = 4096;   // particles per push block (target)
+constexpr int kTileThreads = 256;
+constexpr int kSortEvery2D = 8;     // default steps between tile sorts
+
+struct Tiles2D {
+    int32_t TX, TY, ntx, nty, ntiles, splits;
+    int32_t* count;   // [ntiles] particles per tile (sort scratch, zero between sorts)
+    int32_t* start;   // [ntiles + 1] first held particle of each tile
+    int32_t* cursor;  // [ntiles] scatter cursors
+};
 
 struct Desc2D {
     int32_t NX, NY, lx, ly;  // grid and log2 sizes
@@ -34,6 +48,12 @@ struct Buffers2D {
     double *X, *Y, *VX, *VY, *VZ;
     uint32_t *Xq, *Yq;
     float *VXf, *VYf, *VZf;
+    uint32_t* pid;           // mixed: caller's index of each held particle
+    // mixed: the other buffer of each particle array (tile sorts move the
+    // particles there)
+    uint32_t *Xq_alt, *Yq_alt, *pid_alt;
+    float *VXf_alt, *VYf_alt, *VZf_alt;
+    Tiles2D T;
     unsigned long long* racc;  // FP64: 3 limbs x NX*NY; mixed: NX*NY quanta
     double2 *S, *T1, *T2, *T3;
     double *Ex, *Ey, *phi;
@@ -69,6 +89,8 @@ inline size_t cols_lds_bytes(int NX, int NY) {
 
 // one time step: push + deposit, then the field solve
 void launch2d_push(const Buffers2D& b, hipStream_t s);
+// mixed: regroup the held particles by tile into the _alt buffers
+void launch2d_tile_sort(const Buffers2D& b, hipStream_t s);
 // src: 0 = FP64 limbs, 1 = mixed quanta, 2 = rho array (b.phi reused as input)
 void launch2d_solve(const Buffers2D& b, int src, const double* rho_in, bool want_phi, hipStream_t s);
 // diagnostics from b.Ex/b.Ey and the particle velocities, appended at *drow

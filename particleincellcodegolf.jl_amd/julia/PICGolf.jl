@@ -290,16 +290,17 @@ struct Pic2DConfig
     B0::Float64
     ns::Int32
     device::Int32
+    sort_every::Int32
 end
 
 """
-    Pic2DConfig(; NX, NY, P, dt, w, B0, ns=2, precision=:fp64, pairing=:reference, device=0)
+    Pic2DConfig(; NX, NY, P, dt, w, B0, ns=2, precision=:fp64, pairing=:reference, device=0, sort_every=0)
 
 `w = n0/P/(Δx*Δy)`, `B0` along x and `NS` as in src/Electrostatic2D3V.jl:23-25.
 """
-Pic2DConfig(; NX, NY, P, dt, w, B0, ns=2, precision=:fp64, pairing=:reference, device=0) =
+Pic2DConfig(; NX, NY, P, dt, w, B0, ns=2, precision=:fp64, pairing=:reference, device=0, sort_every=0) =
     Pic2DConfig(Int32(NX), Int32(NY), Int64(P), PRECISIONS[precision], PAIRINGS[pairing], Float64(dt),
-                Float64(w), Float64(B0), Int32(ns), Int32(device))
+                Float64(w), Float64(B0), Int32(ns), Int32(device), Int32(sort_every))
 
 mutable struct Context2D
     ptr::Ptr{Cvoid}

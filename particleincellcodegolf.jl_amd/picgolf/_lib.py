@@ -64,6 +64,7 @@ class pic2d_config(ctypes.Structure):
         ("NX", ctypes.c_int32), ("NY", ctypes.c_int32), ("P", ctypes.c_int64),
         ("precision", ctypes.c_int32), ("pairing", ctypes.c_int32), ("dt", ctypes.c_double),
         ("w", ctypes.c_double), ("B0", ctypes.c_double), ("ns", ctypes.c_int32), ("device", ctypes.c_int32),
+        ("sort_every", ctypes.c_int32),
     ]
 
 

@@ -39,6 +39,7 @@ class Config2D:
     precision: str = "fp64"
     pairing: str = "reference"
     device: int = 0
+    sort_every: int = 0
 
     def to_c(self):
         c = L.pic2d_config()
@@ -46,7 +47,7 @@ class Config2D:
         c.precision = PRECISIONS[self.precision] if isinstance(self.precision, str) else int(self.precision)
         c.pairing = PAIRINGS[self.pairing] if isinstance(self.pairing, str) else int(self.pairing)
         c.dt, c.w, c.B0 = float(self.dt), float(self.w), float(self.B0)
-        c.ns, c.device = int(self.ns), int(self.device)
+        c.ns, c.device, c.sort_every = int(self.ns), int(self.device), int(self.sort_every)
         return c
 
 
