@@ -28,13 +28,15 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"])
     ap.add_argument("--particles", type=int, default=2 ** 24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the ensemble and 2D3V workloads reported beside the headline")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default particle tiling instead of autotuning it before the timed steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (HBM counter calibration)")
     ap.add_argument("--summarize-trace", metavar="DIR",
@@ -171,10 +173,11 @@ def sweep_bytes(S, prec):
     return first + mid * (S - 2) + last
 
 
-def timed_run(cfg, x, v, steps, warmup, profile_steps):
+def timed_run(cfg, x, v, steps, warmup, profile_steps, autotune=True):
     import picgolf
     with picgolf.Context(cfg) as c:
         c.upload(x, v)
+        tune = c.autotune(2) if autotune else None
         if warmup:
             c.run(warmup)
         t0 = time.perf_counter()
@@ -183,10 +186,10 @@ def timed_run(cfg, x, v, steps, warmup, profile_steps):
         prof = None
         if profile_steps:
             _, _, prof = c.run_profiled(profile_steps)
-    return (t1 - t0), D[0], S[0], prof
+    return (t1 - t0), D[0], S[0], prof, tune
 
 
-def bench_ensemble(steps=10, warmup=2, n=512):
+def bench_ensemble(steps=10, warmup=10, n=512, autotune=True):
     """The batched ensemble: n independent two-stream instances with uneven N,
     P, W and beam speed in one context, mixed and FP64.  Field-energy error of
     the mixed run against the FP64 run over the timed steps (linear phase)."""
@@ -199,12 +202,13 @@ def bench_ensemble(steps=10, warmup=2, n=512):
         with picgolf.Context(cfgs) as c:
             for i, (x, v) in enumerate(states):
                 c.upload(x, v, inst=i)
+            tune = c.autotune(2) if autotune else None
             c.run(warmup)
             t0 = time.perf_counter()
             D, S = c.run(steps)
             secs = time.perf_counter() - t0
         Ptot = sum(k.P for k in cfgs)
-        res[prec] = dict(pushes=Ptot * steps / secs, ms=secs / steps * 1e3, D=D, S=S, Ptot=Ptot)
+        res[prec] = dict(pushes=Ptot * steps / secs, ms=secs / steps * 1e3, D=D, S=S, Ptot=Ptot, tune=tune)
     f, m = res["fp64"], res["mixed"]
     rel = np.abs(m["D"][:, :, 0] - f["D"][:, :, 0]) / np.abs(f["D"][:, :, 0])
     return {"workload": f"ensemble of {n} two-stream instances, N in 32..1024, 16-256 particles per cell, "
@@ -212,7 +216,8 @@ def bench_ensemble(steps=10, warmup=2, n=512):
             "mixed_pushes_per_s": m["pushes"], "fp64_pushes_per_s": f["pushes"],
             "mixed_over_fp64": m["pushes"] / f["pushes"], "mixed_ms_per_step": m["ms"],
             "fp64_ms_per_step": f["ms"], "mean_sweeps_mixed": float(m["S"].mean()),
-            "field_energy_rel_err_mixed_vs_fp64": float(rel.max()), "steps": steps, "warmup": warmup}
+            "field_energy_rel_err_mixed_vs_fp64": float(rel.max()), "steps": steps, "warmup": warmup,
+            "tiling": {"mixed": m["tune"], "fp64": f["tune"]}}
 
 
 def bench_2d(steps=500):
@@ -293,8 +298,8 @@ def main():
     for prec in precs:
         cfg, x, v, _ = configs.scaled_two_stream(precision=prec, P=a.particles)
         base = (cfg, x, v)
-        secs, D, S, prof = timed_run(cfg, x, v, a.steps, a.warmup, a.profile_steps)
-        results[prec] = dict(secs=secs, D=D, S=S, prof=prof, pushes=cfg.P * a.steps / secs)
+        secs, D, S, prof, tune = timed_run(cfg, x, v, a.steps, a.warmup, a.profile_steps, not a.no_autotune)
+        results[prec] = dict(secs=secs, D=D, S=S, prof=prof, pushes=cfg.P * a.steps / secs, tune=tune)
         print(f"[bench] {prec}: {a.steps} steps in {secs:.3f}s, {results[prec]['pushes']:.4g} pushes/s, "
               f"mean sweeps {S.mean():.2f}", file=sys.stderr, flush=True)
     head = "mixed" if "mixed" in results else "fp64"
@@ -324,6 +329,7 @@ def main():
                                "rtol=1e-8, dt=1/6N, W=400", "N": cfg.N, "P": P,
                    "precision": head, "parallelism": "none (1 GPU)"},
         "mean_sweeps": float(r["S"].mean()),
+        "tiling": {k: v["tune"] for k, v in results.items()},
         "kernel_avg_ms": {k: v[1] / v[0] for k, v in r["prof"].items()},
         "kernel_launches_per_step": {k: v[0] / a.profile_steps for k, v in r["prof"].items()},
         "roofline": {"bound": "hbm", "kernel": "push_sweep", "achieved": achieved,
@@ -339,7 +345,7 @@ def main():
         out["field_energy_rel_err_mixed_vs_fp64"] = float(np.max(np.abs(d1m - d1f) / np.abs(d1f)))
         out["momentum_drift_mixed"] = float(np.max(np.abs(m["D"][:, 3] - m["D"][0, 3])))
     if not a.no_extra:
-        out["workloads"] = {"ensemble_512": bench_ensemble(), "es2d3v_256": bench_2d()}
+        out["workloads"] = {"ensemble_512": bench_ensemble(autotune=not a.no_autotune), "es2d3v_256": bench_2d()}
         print(f"[bench] extra workloads: {json.dumps(out['workloads'])}", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(*base)

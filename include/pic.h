@@ -102,6 +102,33 @@ int pic_run(pic_ctx* ctx, int64_t nsteps, double* diag, int32_t* sweeps);
 int pic_run_profiled(pic_ctx* ctx, int64_t nsteps, double* diag, int32_t* sweeps,
                      pic_kernel_stat* stats, int32_t* n_stats);
 
+/* ---- particle tiling of the push launches ------------------------------ */
+/* The reference has no counterpart: its particle loops are serial
+ * (src/GaussianFixedPointQuiet.jl:10).  A push launch runs one workgroup of
+ * `block` threads per tile of at most tile_particles particles of one
+ * instance.  The tiling changes how fast a run is, never its results: state,
+ * field, diagnostics and sweep counts are the same bit for bit under any
+ * tiling. */
+typedef struct pic_tuning {
+    int64_t tile_particles;    /* particles per tile chosen                        */
+    int32_t block;             /* threads per push workgroup                       */
+    int32_t n_tiles;           /* tiles over all instances                         */
+    int32_t candidates;        /* tilings timed                                    */
+    double ms_per_step;        /* trial time per step of the chosen tiling         */
+    double entry_ms_per_step;  /* trial time per step of the tiling on entry       */
+} pic_tuning;
+/* tile_particles >= 1 (rounded up to a multiple of 64, at most 2^20), block a
+ * power of two in [64, 1024] (halved while it exceeds twice the tile). */
+int pic_set_tiling(pic_ctx* ctx, int64_t tile_particles, int32_t block);
+int pic_get_tiling(const pic_ctx* ctx, int64_t* tile_particles, int32_t* block, int32_t* n_tiles);
+/* Autotuner: times trial_steps steps (1..64, after one warm-up step) of the
+ * context's current state under each candidate tiling -- 1, 2, 3, 4, 6, 8, 12
+ * or 16 tiles per compute unit, blocks of 256, 512 or 1024 threads -- and
+ * keeps the fastest (the entry tiling unless a candidate beats it).  The state
+ * is restored after every trial, so the call leaves the simulation where it
+ * was.  out may be NULL. */
+int pic_autotune(pic_ctx* ctx, int32_t trial_steps, pic_tuning* out);
+
 /* ---- operators (FP64, bitwise equal to the CPU restatement) ---------- */
 /* rho = r(x, y): r .*= 0; for l in d.((x.+y)./2); for k in l; r[k[1]] += k[2]*w
  * (src/GaussianFixedPointQuiet.jl:7; y = NULL means y = x); NGP:

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -16,6 +17,8 @@
 using namespace pic;
 
 namespace {
+
+constexpr int64_t kGraphSteps = 8;  // steps in the long step graph
 
 struct DevBuf {
     void* p = nullptr;
@@ -41,13 +44,17 @@ struct pic_ctx {
     // growable scratch for the operators
     DevBuf op_a, op_b, op_c, op_d, op_e, op_f, op_part, op_dpart, op_flag;
     double* tw = nullptr;
-    hipGraphExec_t graph_many = nullptr;
-    hipGraphExec_t graph_one = nullptr;
-    int64_t graph_many_steps = 0;
-    int graph_slots = 0;       // sweep slots per step in the captured graphs
+    // captured step graphs by sweep slots per step: one step, and kGraphSteps steps
+    struct StepGraphs {
+        hipGraphExec_t one = nullptr, many = nullptr;
+    };
+    std::map<int, StepGraphs> graphs;
     bool primed = false;       // the accumulator holds the next step's first charge
     int sweeps_seen = 0;       // most sweeps any step has needed (0: none run yet)
     int64_t steps_done = 0;
+    // particle tiling of the push launches (pic_set_tiling / pic_autotune)
+    int64_t tileP = 0;
+    DevBuf tiles_buf, dpart_buf, snap_buf;
     std::string err;
 };
 
@@ -121,32 +128,25 @@ static void twiddles4096(std::vector<double>& out) {
     }
 }
 
-// Rearranges the 4096-point table per FFT pass of kPassStages stages (see
-// twp_base in kernels_1d.hip): entry (2^s0 - 1) + ((2^t - 1 + jlo) << s0) + lo
-// of the pass starting at stage s0+1 holds W[(jlo*2^s0 + lo) << (11 - s0 - t)].
-static void twiddle_passes(const std::vector<double>& tw, std::vector<double>& out) {
-    const int L = 12;  // log2 of the 4096-point table's transform
-    out.assign(2 * ((1 << L) - 1), 0.0);
-    for (int s0 = 0; s0 < L; s0 += kPassStages) {
-        const int M = std::min(kPassStages, L - s0);
-        for (int t = 0; t < M; ++t)
-            for (int jlo = 0; jlo < (1 << t); ++jlo)
-                for (int lo = 0; lo < (1 << s0); ++lo) {
-                    const int idx = ((jlo << s0) + lo) << (11 - s0 - t);
-                    const int e = ((1 << s0) - 1) + (((1 << t) - 1 + jlo) << s0) + lo;
-                    out[2 * e] = tw[2 * idx];
-                    out[2 * e + 1] = tw[2 * idx + 1];
-                }
-    }
-}
-
-// The per-pass table for transforms of any power-of-two length <= 4096 (used
-// by the 1D solve and by the 2D row/column transforms).
+// The per-stage table of the radix-2 transforms plus the division factors of
+// the 1D solve.  Stage s (1..12) uses W_{2^s}^k = W[k << (12 - s)] for
+// k < 2^(s-1), stored at entry 2^(s-1) - 1 + k (a pass of stages s0+1..s0+M
+// reads lanes lo of entries 2^(s0+t) - 1 + jlo*2^s0 + lo, contiguous across
+// the lanes of a wave).  From double kXiOffset on: 1/(2 pi k) for
+// k = 1..kMaxN/2, the factor of mode k' = +-k in xi = rho^ / (2 pi i k').
 std::vector<double> pic::pass_twiddle_table() {
-    std::vector<double> tw, twp;
+    std::vector<double> tw;
     twiddles4096(tw);
-    twiddle_passes(tw, twp);
-    return twp;
+    const int L = 12;  // log2 of the 4096-point table's transform
+    std::vector<double> out(kXiOffset + kMaxN / 2 + 1, 0.0);
+    for (int s = 1; s <= L; ++s)
+        for (int k = 0; k < (1 << (s - 1)); ++k) {
+            const int e = (1 << (s - 1)) - 1 + k;
+            out[2 * e] = tw[2 * (k << (L - s))];
+            out[2 * e + 1] = tw[2 * (k << (L - s)) + 1];
+        }
+    for (int k = 1; k <= kMaxN / 2; ++k) out[kXiOffset + k] = 1.0 / (kTwoPi * (double)k);
+    return out;
 }
 
 template <typename T>
@@ -172,10 +172,11 @@ static int grow(pic_ctx* c, DevBuf& b, size_t bytes) {
 }
 
 static void drop_graphs(pic_ctx* c) {
-    if (c->graph_many) (void)hipGraphExecDestroy(c->graph_many);
-    if (c->graph_one) (void)hipGraphExecDestroy(c->graph_one);
-    c->graph_many = c->graph_one = nullptr;
-    c->graph_many_steps = 0;
+    for (auto& e : c->graphs) {
+        if (e.second.many) (void)hipGraphExecDestroy(e.second.many);
+        if (e.second.one) (void)hipGraphExecDestroy(e.second.one);
+    }
+    c->graphs.clear();
 }
 
 static const char* validate(const pic_config& k) {
@@ -218,26 +219,63 @@ const char* pic_strerror(int code) {
 // this thread failed.
 const char* pic_last_error(const pic_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
 
+// Default tiling: every tile of every instance holds at most tileP particles,
+// so the launch is balanced, with enough tiles to cover the chip.  FP64: as
+// many tiles as fit at once (LDS-bound); mixed: two 1024-thread blocks per CU,
+// so the per-tile partials (N u64 each) stay small.
+static void default_tiling(const pic_ctx* c, int64_t* tileP, int* block) {
+    const int lds_push = push_lds_bytes(c->maxN, c->precision);
+    const int wg_per_cu = std::max(1, std::min(4, (160 * 1024) / std::max(lds_push, 1)));
+    const int64_t target = (int64_t)c->numCU * (c->precision == 0 ? wg_per_cu : 2);
+    *tileP = (c->Ptot + target - 1) / target;
+    *block = (c->precision == 0 && wg_per_cu >= 2) ? 512 : 1024;
+}
+
+// Lays the particles of every instance out in tiles of at most tileP (rounded
+// up to a multiple of 64, within [64, 2^20]) run by blocks of `block` threads
+// (halved while a block would be twice the tile).  Only the launch layout
+// changes: the run's results do not depend on it.
+static int layout_tiles(pic_ctx* c, int64_t tileP, int block) {
+    tileP = std::max<int64_t>(64, ((tileP + 63) / 64) * 64);
+    tileP = std::min<int64_t>(tileP, (int64_t)1 << 20);
+    while (block > 64 && block / 2 >= tileP) block /= 2;
+    drop_graphs(c);
+    c->tileP = tileP;
+    c->htiles.clear();
+    for (int i = 0; i < c->ninst; ++i) {
+        InstDesc& d = c->hinst[i];
+        d.tile0 = (int32_t)c->htiles.size();
+        const int64_t nt = (d.P + tileP - 1) / tileP;
+        d.ntiles = (int32_t)nt;
+        for (int64_t t = 0; t < nt; ++t) {
+            TileDesc T{};
+            T.inst = i;
+            T.local = (int32_t)t;
+            T.start = t * tileP;
+            T.count = std::min<int64_t>(tileP, d.P - t * tileP);
+            c->htiles.push_back(T);
+        }
+    }
+    RunBuffers& b = c->rb;
+    b.block_push = block;
+    b.ntiles = (int32_t)c->htiles.size();
+    int rc;
+    if ((rc = grow(c, c->tiles_buf, sizeof(TileDesc) * (size_t)b.ntiles))) return rc;
+    // FP64: a double pair per diagnostics chunk; mixed midpoint: a 128-bit pair per tile
+    if ((rc = grow(c, c->dpart_buf, sizeof(double) * 4 * (size_t)std::max(b.nchunks, b.ntiles)))) return rc;
+    b.tiles = (TileDesc*)c->tiles_buf.p;
+    b.dpart = (double*)c->dpart_buf.p;
+    HIPCHK(c, hipMemcpy(b.insts, c->hinst.data(), sizeof(InstDesc) * c->ninst, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b.tiles, c->htiles.data(), sizeof(TileDesc) * b.ntiles, hipMemcpyHostToDevice));
+    return PIC_OK;
+}
+
 static int build_tables(pic_ctx* c) {
     const int n = c->ninst;
     c->hinst.assign(n, InstDesc{});
     c->htiles.clear();
     c->hchunks.clear();
-    // tile size: every tile of every instance holds at most tileP particles,
-    // so the launch is balanced; enough tiles to cover the chip
-    const int lds_push = push_lds_bytes(c->maxN, c->precision);
-    int wg_per_cu = std::max(1, std::min(4, (160 * 1024) / std::max(lds_push, 1)));
-    // FP64: as many tiles as fit at once (LDS-bound); mixed: two 1024-thread
-    // blocks per CU, so the per-tile partials (N u64 each) stay small
-    const int64_t target = (int64_t)c->numCU * (c->precision == 0 ? wg_per_cu : 2);
-    int64_t tileP = (c->Ptot + target - 1) / target;
-    tileP = std::max<int64_t>(64, ((tileP + 63) / 64) * 64);
-    tileP = std::min<int64_t>(tileP, (int64_t)1 << 20);
-    int block = 1024;
-    if (c->precision == 0 && wg_per_cu >= 2) block = 512;
-    while (block > 64 && block / 2 >= tileP) block /= 2;
-    c->rb.block_push = block;
-    c->rb.lds_push = lds_push;
+    c->rb.lds_push = push_lds_bytes(c->maxN, c->precision);
     int64_t poff = 0, red0 = 0;
     int32_t goff = 0;
     for (int i = 0; i < n; ++i) {
@@ -257,17 +295,6 @@ static int build_tables(pic_ctx* c) {
         d.W = k.W;
         d.rtol = k.rtol;
         d.wq = k.w * 0x1p-30;
-        d.tile0 = (int32_t)c->htiles.size();
-        const int64_t nt = (k.P + tileP - 1) / tileP;
-        d.ntiles = (int32_t)nt;
-        for (int64_t t = 0; t < nt; ++t) {
-            TileDesc T{};
-            T.inst = i;
-            T.local = (int32_t)t;
-            T.start = t * tileP;
-            T.count = std::min<int64_t>(tileP, k.P - t * tileP);
-            c->htiles.push_back(T);
-        }
         d.red0 = red0;
         red0 += k.N;
         d.chunk0 = (int32_t)c->hchunks.size();
@@ -285,7 +312,6 @@ static int build_tables(pic_ctx* c) {
     }
     RunBuffers& b = c->rb;
     b.ninst = n;
-    b.ntiles = (int32_t)c->htiles.size();
     b.nchunks = (int32_t)c->hchunks.size();
     b.maxN = c->maxN;
     b.R = c->R;
@@ -296,10 +322,11 @@ static int build_tables(pic_ctx* c) {
     int rc;
     if ((rc = dalloc(c, &b.insts, n))) return rc;
     if ((rc = dalloc(c, &b.st, n))) return rc;
-    if ((rc = dalloc(c, &b.tiles, b.ntiles))) return rc;
     if ((rc = dalloc(c, &b.chunks, b.nchunks))) return rc;
-    HIPCHK(c, hipMemcpy(b.insts, c->hinst.data(), sizeof(InstDesc) * n, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(b.tiles, c->htiles.data(), sizeof(TileDesc) * b.ntiles, hipMemcpyHostToDevice));
+    int64_t tileP;
+    int block;
+    default_tiling(c, &tileP, &block);
+    if ((rc = layout_tiles(c, tileP, block))) return rc;
     HIPCHK(c, hipMemcpy(b.chunks, c->hchunks.data(), sizeof(ChunkDesc) * b.nchunks, hipMemcpyHostToDevice));
     std::vector<InstState> st(n);
     for (auto& s : st) {
@@ -332,7 +359,6 @@ static int build_tables(pic_ctx* c) {
     if ((rc = dalloc(c, &b.E[0], G))) return rc;
     if ((rc = dalloc(c, &b.E[1], G))) return rc;
     if ((rc = dalloc(c, &b.E32, G))) return rc;
-    if ((rc = dalloc(c, &b.dpart, 2 * (size_t)std::max(b.nchunks, b.ntiles)))) return rc;
     if ((rc = dalloc(c, &b.halt, 1))) return rc;
     HIPCHK(c, hipMemset(b.halt, 0, sizeof(int32_t)));
     HIPCHK(c, hipMemset(b.E[0], 0, sizeof(double) * G));
@@ -385,9 +411,7 @@ int pic_create_batch(pic_ctx** out, const pic_config* cfgs, int32_t n) {
         return fail(PIC_EHIP);
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(PIC_EHIP);
-    std::vector<double> tw, twp;
-    twiddles4096(tw);
-    twiddle_passes(tw, twp);
+    const std::vector<double> twp = pass_twiddle_table();
     if ((rc = dalloc(c, &c->tw, twp.size()))) return fail(rc);
     if (hipMemcpy(c->tw, twp.data(), sizeof(double) * twp.size(), hipMemcpyHostToDevice) != hipSuccess)
         return fail(PIC_EHIP);
@@ -405,7 +429,7 @@ int pic_destroy(pic_ctx* c) {
     drop_graphs(c);
     for (void* p : c->allocs) (void)hipFree(p);
     for (DevBuf* b : {&c->op_a, &c->op_b, &c->op_c, &c->op_d, &c->op_e, &c->op_f, &c->op_part,
-                      &c->op_dpart, &c->op_flag})
+                      &c->op_dpart, &c->op_flag, &c->tiles_buf, &c->dpart_buf, &c->snap_buf})
         if (b->p) (void)hipFree(b->p);
     if (c->rb.diag) (void)hipFree(c->rb.diag);
     if (c->rb.sweeps) (void)hipFree(c->rb.sweeps);
@@ -569,6 +593,23 @@ static int ensure_diag(pic_ctx* c, int64_t nsteps) {
     return PIC_OK;
 }
 
+// Copies the leading dst_row bytes of every instance's row (src_row bytes
+// apart on the device) into consecutive host rows: one transfer of the whole
+// buffer (strided device-to-host copies are slow), then a host-side gather.
+static int copy_rows(pic_ctx* c, void* dst, const void* src, size_t dst_row, size_t src_row) {
+    const size_t ni = (size_t)c->ninst;
+    if (dst_row == src_row) {
+        HIPCHK(c, hipMemcpyAsync(dst, src, ni * src_row, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return PIC_OK;
+    }
+    std::vector<char> h(ni * src_row);
+    HIPCHK(c, hipMemcpyAsync(h.data(), src, h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < ni; ++i) std::memcpy((char*)dst + i * dst_row, h.data() + i * src_row, dst_row);
+    return PIC_OK;
+}
+
 static int begin_run(pic_ctx* c, int64_t nsteps) {
     int rc = ensure_diag(c, nsteps);
     if (rc) return rc;
@@ -585,15 +626,9 @@ static int finish_run(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps)
     std::vector<InstState> st;
     int rc = read_states(c, st);
     if (rc) return rc;
-    const int64_t cap = c->rb.diag_cap;
-    for (int i = 0; i < c->ninst; ++i) {
-        if (diag)
-            HIPCHK(c, hipMemcpy(diag + (size_t)i * nsteps * 4, c->rb.diag + (size_t)i * cap * 4,
-                                sizeof(double) * 4 * nsteps, hipMemcpyDeviceToHost));
-        if (sweeps)
-            HIPCHK(c, hipMemcpy(sweeps + (size_t)i * nsteps, c->rb.sweeps + (size_t)i * cap,
-                                sizeof(int32_t) * nsteps, hipMemcpyDeviceToHost));
-    }
+    const size_t cap = (size_t)c->rb.diag_cap, n = (size_t)nsteps;
+    if (diag && (rc = copy_rows(c, diag, c->rb.diag, n * 4 * sizeof(double), cap * 4 * sizeof(double)))) return rc;
+    if (sweeps && (rc = copy_rows(c, sweeps, c->rb.sweeps, n * sizeof(int32_t), cap * sizeof(int32_t)))) return rc;
     for (int i = 0; i < c->ninst; ++i)
         if (st[i].err & 1) {
             set_err(c, "instance %d: non-finite charge or field", i);
@@ -624,12 +659,12 @@ static int choose_slots(const pic_ctx* c) {
 
 static int note_sweeps(pic_ctx* c, int64_t row0, int64_t row1) {
     if (c->scheme != PIC_SCHEME_MIDPOINT || row1 <= row0) return PIC_OK;
-    std::vector<int32_t> h((size_t)(row1 - row0));
-    for (int i = 0; i < c->ninst; ++i) {
-        HIPCHK(c, hipMemcpy(h.data(), c->rb.sweeps + (size_t)i * c->rb.diag_cap + row0, sizeof(int32_t) * h.size(),
-                            hipMemcpyDeviceToHost));
-        for (int32_t v : h) c->sweeps_seen = std::max(c->sweeps_seen, (int)v);
-    }
+    const size_t n = (size_t)row1, cap = (size_t)c->rb.diag_cap;
+    std::vector<int32_t> h(n * (size_t)c->ninst);
+    int rc = copy_rows(c, h.data(), c->rb.sweeps, n * sizeof(int32_t), cap * sizeof(int32_t));
+    if (rc) return rc;
+    for (int i = 0; i < c->ninst; ++i)
+        for (size_t r = (size_t)row0; r < n; ++r) c->sweeps_seen = std::max(c->sweeps_seen, (int)h[i * n + r]);
     return PIC_OK;
 }
 
@@ -644,28 +679,22 @@ int pic_run(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps) {
     int rc = begin_run(c, nsteps);
     if (rc) return rc;
     if ((rc = prime(c))) return rc;
-    const int64_t G = 8;
+    const int64_t G = kGraphSteps;
     int64_t done = 0;
     while (done < nsteps) {
         const int slots = choose_slots(c);
-        if (slots != c->graph_slots) {
-            drop_graphs(c);
-            c->graph_slots = slots;
-        }
-        // until a step has been observed, run a few with every slot in place
-        const int64_t chunk = c->sweeps_seen == 0 ? std::min<int64_t>(nsteps - done, G) : nsteps - done;
-        if (!c->graph_one && (rc = capture(c, 1, slots, &c->graph_one))) return rc;
-        if (chunk >= G && !c->graph_many) {
-            if ((rc = capture(c, G, slots, &c->graph_many))) return rc;
-            c->graph_many_steps = G;
-        }
+        // until a step has been observed, run one with every slot in place
+        const int64_t chunk = c->sweeps_seen == 0 ? 1 : nsteps - done;
+        pic_ctx::StepGraphs& g = c->graphs[slots];
+        if (!g.one && (rc = capture(c, 1, slots, &g.one))) return rc;
+        if (chunk >= G && !g.many && (rc = capture(c, G, slots, &g.many))) return rc;
         int64_t left = chunk;
-        while (c->graph_many && left >= c->graph_many_steps) {
-            HIPCHK(c, hipGraphLaunch(c->graph_many, c->stream));
-            left -= c->graph_many_steps;
+        while (g.many && left >= G) {
+            HIPCHK(c, hipGraphLaunch(g.many, c->stream));
+            left -= G;
         }
         while (left > 0) {
-            HIPCHK(c, hipGraphLaunch(c->graph_one, c->stream));
+            HIPCHK(c, hipGraphLaunch(g.one, c->stream));
             --left;
         }
         int32_t halt = 0;
@@ -761,6 +790,159 @@ int pic_run_profiled(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps, 
         *n_stats = m;
     }
     return finish_run(c, nsteps, diag, sweeps);
+}
+
+// ---- tiling and the autotuner ----------------------------------------------
+
+// Device buffers a run step changes: particle state (both parities), field,
+// charge accumulator, loop state.  The autotuner restores them after each trial.
+static std::vector<std::pair<void*, size_t>> run_state(pic_ctx* c) {
+    RunBuffers& b = c->rb;
+    const size_t P = (size_t)c->Ptot, G = (size_t)(c->hinst.back().goff + c->hinst.back().N);
+    std::vector<std::pair<void*, size_t>> v;
+    for (int j = 0; j < 2; ++j) {
+        if (c->precision == PIC_PREC_FP64) {
+            v.push_back({b.X[j], P * 8});
+            v.push_back({b.V[j], P * 8});
+            if (b.vit[j]) v.push_back({b.vit[j], P * 8});
+        } else {
+            v.push_back({b.Xq[j], P * 4});
+            v.push_back({b.Vh[j], P * 4});
+            v.push_back({b.Vl[j], P * 4});
+            v.push_back({b.vf[j], P * 4});
+        }
+        v.push_back({b.E[j], G * 8});
+    }
+    v.push_back({b.E32, G * 4});
+    v.push_back({b.racc, 8 * (size_t)(c->hinst.back().red0 + c->hinst.back().N) * (c->precision == 0 ? 3 : 1)});
+    v.push_back({b.st, sizeof(InstState) * (size_t)c->ninst});
+    v.push_back({b.halt, sizeof(int32_t)});
+    return v;
+}
+
+static int copy_run_state(pic_ctx* c, bool save) {
+    size_t off = 0;
+    for (auto& e : run_state(c)) {
+        char* snap = (char*)c->snap_buf.p + off;
+        if (save) HIPCHK(c, hipMemcpyAsync(snap, e.first, e.second, hipMemcpyDeviceToDevice, c->stream));
+        else HIPCHK(c, hipMemcpyAsync(e.first, snap, e.second, hipMemcpyDeviceToDevice, c->stream));
+        off += (e.second + 255) & ~(size_t)255;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_set_tiling(pic_ctx* c, int64_t tile_particles, int32_t block) {
+    if (!c) return PIC_EINVAL;
+    if (tile_particles < 1 || block < 64 || block > 1024 || (block & (block - 1)) != 0) {
+        set_err(c, "pic_set_tiling: tile_particles must be >= 1 and block a power of two in [64, 1024]");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return layout_tiles(c, tile_particles, block);
+}
+
+int pic_get_tiling(const pic_ctx* c, int64_t* tile_particles, int32_t* block, int32_t* n_tiles) {
+    if (!c) return PIC_EINVAL;
+    if (tile_particles) *tile_particles = c->tileP;
+    if (block) *block = c->rb.block_push;
+    if (n_tiles) *n_tiles = c->rb.ntiles;
+    return PIC_OK;
+}
+
+// Times trial steps (eager launches, every sweep slot in place) of the
+// current state under the current tiling; the state is restored by the caller.
+static int time_trial(pic_ctx* c, int steps, double* ms_per_step) {
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    const bool primed = c->primed;
+    int rc = prime(c);
+    if (rc == PIC_OK) {
+        enqueue_step(c, c->stream, c->max_sweeps);  // warm-up
+        (void)hipEventRecord(e0, c->stream);
+        for (int i = 0; i < steps; ++i) enqueue_step(c, c->stream, c->max_sweeps);
+        (void)hipEventRecord(e1, c->stream);
+        hipError_t e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+            set_err(c, "pic_autotune: trial failed: %s", hipGetErrorString(e));
+            rc = PIC_EHIP;
+        }
+        *ms_per_step = ms / steps;
+    }
+    c->primed = primed;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int pic_autotune(pic_ctx* c, int32_t trial_steps, pic_tuning* out) {
+    if (!c) return PIC_EINVAL;
+    if (trial_steps < 1 || trial_steps > 64) {
+        set_err(c, "pic_autotune: trial_steps must be in [1, 64]");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = begin_run(c, trial_steps + 1);
+    if (rc) return rc;
+    size_t bytes = 0;
+    for (auto& e : run_state(c)) bytes += (e.second + 255) & ~(size_t)255;
+    if ((rc = grow(c, c->snap_buf, bytes))) return rc;
+    if ((rc = copy_run_state(c, true))) return rc;
+    const int64_t tile0 = c->tileP;
+    const int block0 = c->rb.block_push;
+    // candidates: the particles per workgroup for 1..16 tiles per CU, and
+    // blocks of 256, 512 and 1024 threads
+    struct Cand { int64_t tileP; int block; double ms; };
+    std::vector<Cand> cands;
+    const int mult[] = {1, 2, 3, 4, 6, 8, 12, 16};
+    for (int m : mult)
+        for (int block : {256, 512, 1024}) {
+            int64_t t = (c->Ptot + (int64_t)m * c->numCU - 1) / ((int64_t)m * c->numCU);
+            t = std::min<int64_t>(std::max<int64_t>(64, ((t + 63) / 64) * 64), (int64_t)1 << 20);
+            int bl = block;
+            while (bl > 64 && bl / 2 >= t) bl /= 2;
+            bool dup = false;
+            for (auto& q : cands) dup |= q.tileP == t && q.block == bl;
+            if (!dup) cands.push_back({t, bl, 0.0});
+        }
+    double best = 0.0, ms0 = 0.0;
+    int ibest = -1;
+    for (size_t i = 0; i <= cands.size() && rc == PIC_OK; ++i) {
+        // i == cands.size(): the tiling the context had on entry
+        const int64_t t = i < cands.size() ? cands[i].tileP : tile0;
+        const int bl = i < cands.size() ? cands[i].block : block0;
+        double ms = 0.0;
+        if ((rc = layout_tiles(c, t, bl))) break;
+        if ((rc = time_trial(c, trial_steps, &ms))) break;
+        rc = copy_run_state(c, false);
+        if (i < cands.size()) cands[i].ms = ms;
+        else ms0 = ms;
+        if (i < cands.size() && (ibest < 0 || ms < best)) {
+            best = ms;
+            ibest = (int)i;
+        }
+    }
+    if (rc) {
+        (void)copy_run_state(c, false);
+        (void)layout_tiles(c, tile0, block0);
+        return rc;
+    }
+    // keep the entry tiling unless a candidate beats it
+    const bool keep = ms0 <= best;
+    if ((rc = layout_tiles(c, keep ? tile0 : cands[ibest].tileP, keep ? block0 : cands[ibest].block))) return rc;
+    if (out) {
+        out->tile_particles = c->tileP;
+        out->block = c->rb.block_push;
+        out->n_tiles = c->rb.ntiles;
+        out->candidates = (int32_t)cands.size();
+        out->ms_per_step = keep ? ms0 : best;
+        out->entry_ms_per_step = ms0;
+    }
+    return PIC_OK;
 }
 
 // ---- operators ------------------------------------------------------------

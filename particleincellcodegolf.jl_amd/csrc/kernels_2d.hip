@@ -403,12 +403,6 @@ __device__ double tree_strided(double* red, double* vals, int stride, int64_t n)
 // Mixed kinetic diagnostics, independent of the particle order: per chunk of
 // 4096 held particles the exact integer sums of (vx^2 + vy^2) on a 2^-50 grid
 // and of vx, vy on a 2^-40 grid.
-typedef __int128 i128;
-
-__device__ __forceinline__ int64_t fix_round(double a, int q) {
-    return (int64_t)rint(ldexp(a, q));
-}
-
 __global__ void __launch_bounds__(256)
 k2d_diag_parts32x(const float* __restrict__ VX, const float* __restrict__ VY, int64_t P, i128* __restrict__ ppart) {
     __shared__ i128 red[3 * 256];
@@ -417,9 +411,9 @@ k2d_diag_parts32x(const float* __restrict__ VX, const float* __restrict__ VY, in
     i128 a = 0, b = 0, c = 0;
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
         const double vx = (double)VX[s + i], vy = (double)VY[s + i];
-        a += (i128)fix_round(vx * vx + vy * vy, 50);
-        b += (i128)fix_round(vx, 40);
-        c += (i128)fix_round(vy, 40);
+        a += fix_round128(vx * vx + vy * vy, 50);
+        b += fix_round128(vx, 40);
+        c += fix_round128(vy, 40);
     }
     red[threadIdx.x] = a;
     red[256 + threadIdx.x] = b;
@@ -438,10 +432,6 @@ k2d_diag_parts32x(const float* __restrict__ VX, const float* __restrict__ VY, in
         ppart[3 * blockIdx.x + 1] = red[256];
         ppart[3 * blockIdx.x + 2] = red[512];
     }
-}
-
-__device__ __forceinline__ double i128_to_double(i128 v, int q) {
-    return v < 0 ? -fxp_to((u128)(-v), q) : fxp_to((u128)v, q);
 }
 
 __global__ void __launch_bounds__(256)

@@ -25,7 +25,7 @@ constexpr int kWin = 2 * kStencil32 + 1;
 __host__ __device__ constexpr int pad_cells(int N) { return N + 2 * kHalo + 4; }
 // LDS of the push: u64 accumulator and FP32 E over the padded cells, then 32
 // doubles for the block's diagnostics partials
-int push32_lds_bytes(int N) { return 12 * pad_cells(N) + 8 * 4 + 256; }
+int push32_lds_bytes(int N) { return 12 * pad_cells(N) + 8 * 4 + 512; }
 
 // Stencil of a fixed-point centre: i0 = nearest cell centre (Julia index,
 // N == 0) and the kWin weights, in 2^-30 units of a particle's charge, of the
@@ -146,8 +146,9 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     const float cscale = (float)(I.dt * 1073741824.0);  // (v + V)/4*dt in 2^-32 units
     const float xscale = (float)(I.dt * 2147483648.0);  // (v + V)/2*dt in 2^-32 units
     const int64_t base = I.poff + T.start;
-    const double Pd = (double)I.P;
-    double d2 = 0.0, d1 = 0.0;  // final sweep: sum(v^2), sum(v/P) over this thread's particles
+    // final sweep: exact sums of v^2 (2^-kV2Bits grid) and v (2^-kV1Bits grid)
+    // over this thread's particles
+    i128 q2 = 0, q1 = 0;
     // every lane of a wave runs the same iterations (the deposit reduces
     // across the wave); lanes past the tile's end carry no particle
     const int64_t nit = (T.count + blockDim.x - 1) / blockDim.x;
@@ -184,9 +185,9 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
                     Xo[g] = xn;
                     Vho[g] = hi;
                     Vlo[g] = lo;
-                    const double vd = (double)hi + (double)lo;
-                    d2 = d2 + vd * vd;
-                    d1 = d1 + vd / Pd;
+                    const double vd = (double)hi + (double)lo;  // exact
+                    q2 += fix_round128(vd * vd, kV2Bits);
+                    q1 += fix_round128(vd, kV1Bits);
                     dep = prime != 0;
                     cdep = advance(xn, hi + hi, cscale);  // next step's first centre, v_0 = V
                 } else {
@@ -203,29 +204,30 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
         if (deposit) lane_deposit(acc, s, dep);
     }
     if (last) {
-        // the step's kinetic diagnostics, one partial pair per tile (fixed
-        // order: lanes by shuffle tree, then waves in order)
-#pragma unroll
-        for (int h = 32; h > 0; h >>= 1) {
-            d2 = d2 + __shfl_down(d2, h, 64);
-            d1 = d1 + __shfl_down(d1, h, 64);
-        }
-        double* red = reinterpret_cast<double*>(lds + NPAD + (NPAD + 1) / 2);
+        // the step's kinetic diagnostics: one exact partial pair per tile
+        q2 = wave_sum_i128(q2);
+        q1 = wave_sum_i128(q1);
+        u64* red = lds + NPAD + (NPAD + 1) / 2;  // 4 words per wave
         __syncthreads();
         const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
         if ((threadIdx.x & 63) == 0) {
-            red[2 * wv] = d2;
-            red[2 * wv + 1] = d1;
+            red[4 * wv] = (u64)q2;
+            red[4 * wv + 1] = (u64)((u128)q2 >> 64);
+            red[4 * wv + 2] = (u64)q1;
+            red[4 * wv + 3] = (u64)((u128)q1 >> 64);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            double a = 0.0, b = 0.0;
+            i128 a = 0, b = 0;
             for (int w = 0; w < nw; ++w) {
-                a = a + red[2 * w];
-                b = b + red[2 * w + 1];
+                a += (i128)(((u128)red[4 * w + 1] << 64) | red[4 * w]);
+                b += (i128)(((u128)red[4 * w + 3] << 64) | red[4 * w + 2]);
             }
-            dpart[2 * blockIdx.x] = a;
-            dpart[2 * blockIdx.x + 1] = b;
+            u64* o = reinterpret_cast<u64*>(dpart) + 4 * (int64_t)blockIdx.x;
+            o[0] = (u64)a;
+            o[1] = (u64)((u128)a >> 64);
+            o[2] = (u64)b;
+            o[3] = (u64)((u128)b >> 64);
         }
         if (!prime) return;
     }

@@ -16,8 +16,6 @@ typedef unsigned long long u64;
 
 namespace pic {
 
-constexpr double kTwoPi = 6.283185307179586;  // Julia 2pi = 2*Float64(pi)
-
 __device__ __forceinline__ uint32_t hiword(double d) {
     return (uint32_t)((u64)__double_as_longlong(d) >> 32);
 }
@@ -281,6 +279,41 @@ __device__ __forceinline__ void radix2_group(double* ar, double* ai, const doubl
             ai[j2] = ui - tq;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Order-independent sums of the mixed paths: each value rounded once onto a
+// fixed-point grid of 2^-q and summed exactly in 128-bit integers, so a sum
+// does not depend on how particles are split over tiles, threads or sorts.
+// ---------------------------------------------------------------------------
+typedef __int128 i128;
+
+// rint(a * 2^q) as a 128-bit integer; exact below 2^126 in magnitude, which
+// saturates.  Non-finite values give 0 (a non-finite field is reported by the
+// solve's check).
+__device__ __forceinline__ i128 fix_round128(double a, int q) {
+    const double t = rint(ldexp(a, q));
+    if (fabs(t) < 0x1p62) return (i128)(long long)t;
+    if (t != t || isinf(t)) return 0;
+    if (!(fabs(t) < 0x1p126)) return t > 0 ? ((i128)1 << 126) : -((i128)1 << 126);
+    const double hi = floor(t * 0x1p-64);  // exact: t is an integer beyond 2^53
+    const double lo = t - hi * 0x1p64;     // exact, in [0, 2^64)
+    return (i128)(long long)hi * ((i128)1 << 64) + (i128)(u64)lo;
+}
+
+__device__ __forceinline__ double i128_to_double(i128 v, int q) {
+    return v < 0 ? -fxp_to((u128)(-v), q) : fxp_to((u128)v, q);
+}
+
+// Sum over a wave (lane 0 receives it).
+__device__ __forceinline__ i128 wave_sum_i128(i128 v) {
+#pragma unroll
+    for (int h = 32; h > 0; h >>= 1) {
+        const u64 lo = __shfl_down((u64)v, h, 64);
+        const u64 hi = __shfl_down((u64)((u128)v >> 64), h, 64);
+        v += (i128)(((u128)hi << 64) | lo);
+    }
+    return v;
 }
 
 // ---------------------------------------------------------------------------

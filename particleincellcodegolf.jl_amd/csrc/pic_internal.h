@@ -11,11 +11,16 @@ namespace pic {
 constexpr int kMaxN = 4096;       // largest grid (one LDS-resident transform)
 constexpr int kTreeLanes = 256;   // canonical reduction: lanes per chunk
 constexpr int kTreeChunk = 4096;  // canonical reduction: elements per chunk
+constexpr int kRedDoubles = 6 * kTreeLanes + 8;  // LDS scratch of the norm trees
+constexpr int kV2Bits = 50;  // mixed kinetic sums: grid of v^2 is 2^-50
+constexpr int kV1Bits = 40;  // and of v, 2^-40
 constexpr int kSolveThreads = 512;  // threads of the one-block-per-instance solve
 constexpr int kPassStages = 3;      // radix-2 FFT stages applied per LDS pass
+constexpr int kXiOffset = 8192;     // doubles before the 1/(2 pi k) factors in the twiddle table
+constexpr double kTwoPi = 6.283185307179586;  // Julia 2pi = 2*Float64(pi)
 
-// Per-pass twiddle table (double2 pairs) of the radix-2 transforms, built on
-// the host from W[m] = exp(-2 pi i m/4096).
+// Twiddle table (double2 pairs per radix-2 stage, from W[m] = exp(-2 pi i m/4096))
+// followed by the solve's 1/(2 pi k) factors at double kXiOffset.
 std::vector<double> pass_twiddle_table();
 
 // One simulation instance (a batch holds many).  Device-resident, read-only

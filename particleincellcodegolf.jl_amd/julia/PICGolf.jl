@@ -30,7 +30,7 @@ expression.
 module PICGolf
 
 export Context, PicConfig, deposit!, fieldsolve!, drift!, midpoint_position!, wrap!,
-       diagnostics, upload!, download, run!, quiet_start,
+       diagnostics, upload!, download, run!, quiet_start, autotune!, set_tiling!, tiling, PicTuning,
        Pic2DConfig, Context2D, gather, boris!
 
 const libpicgolf = get(ENV, "PICGOLF_LIB", normpath(joinpath(@__DIR__, "..", "lib", "libpicgolf.so")))
@@ -256,6 +256,46 @@ function run!(ctx::Context, nsteps::Integer)
                                 ctx.ptr, nsteps, D, S)
     check(ctx, rc, "pic_run")
     (permutedims(D, (2, 1, 3)), S)
+end
+
+"Mirror of `struct pic_tuning` in include/pic.h."
+struct PicTuning
+    tile_particles::Int64
+    block::Int32
+    n_tiles::Int32
+    candidates::Int32
+    ms_per_step::Float64
+    entry_ms_per_step::Float64
+end
+
+"""
+    autotune!(ctx; trial_steps=2) -> PicTuning
+
+Times `trial_steps` steps of the current state under each candidate particle
+tiling of the push launches, keeps the fastest and restores the state.  Runs
+give the same results under any tiling.
+"""
+function autotune!(ctx::Context; trial_steps::Integer=2)
+    r = Ref{PicTuning}()
+    rc = ccall((:pic_autotune, libpicgolf), Cint, (Ptr{Cvoid}, Int32, Ref{PicTuning}), ctx.ptr, trial_steps, r)
+    check(ctx, rc, "pic_autotune")
+    r[]
+end
+
+"`set_tiling!(ctx, tile_particles, block)`: the push launch layout (results do not depend on it)."
+function set_tiling!(ctx::Context, tile_particles::Integer, block::Integer=1024)
+    rc = ccall((:pic_set_tiling, libpicgolf), Cint, (Ptr{Cvoid}, Int64, Int32), ctx.ptr, tile_particles, block)
+    check(ctx, rc, "pic_set_tiling")
+    tiling(ctx)
+end
+
+"`tiling(ctx) -> (tile_particles, block, n_tiles)`."
+function tiling(ctx::Context)
+    t, b, n = Ref{Int64}(0), Ref{Int32}(0), Ref{Int32}(0)
+    rc = ccall((:pic_get_tiling, libpicgolf), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int32}, Ref{Int32}),
+               ctx.ptr, t, b, n)
+    check(ctx, rc, "pic_get_tiling")
+    (t[], b[], n[])
 end
 
 "`x = (bitreverse.(0:P-1) .+ 2.0^63)/2.0^64`, `v = ∓1` by half (src/GaussianFixedPointQuiet.jl:2-3)."

@@ -28,7 +28,7 @@ EXPORTS = [
     "pic_version", "pic_upload", "pic_download", "pic_run", "pic_run_profiled",
     "pic_deposit", "pic_fieldsolve", "pic_drift_half", "pic_push_leapfrog",
     "pic_midpoint_position", "pic_push_midpoint", "pic_wrap", "pic_isapprox",
-    "pic_diagnostics",
+    "pic_diagnostics", "pic_set_tiling", "pic_get_tiling", "pic_autotune",
 ]
 
 # every symbol include/pic2d.h declares
@@ -68,6 +68,13 @@ class pic2d_config(ctypes.Structure):
     ]
 
 
+class pic_tuning(ctypes.Structure):
+    _fields_ = [
+        ("tile_particles", ctypes.c_int64), ("block", ctypes.c_int32), ("n_tiles", ctypes.c_int32),
+        ("candidates", ctypes.c_int32), ("ms_per_step", ctypes.c_double), ("entry_ms_per_step", ctypes.c_double),
+    ]
+
+
 class pic_kernel_stat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_int64), ("total_ms", ctypes.c_double)]
 
@@ -104,6 +111,9 @@ def load():
         "pic_wrap": [vp, vp, i64],
         "pic_isapprox": [vp, vp, vp, i32, d, d, ip],
         "pic_diagnostics": [vp, vp, vp, i64, d, vp],
+        "pic_set_tiling": [vp, i64, i32],
+        "pic_get_tiling": [vp, ctypes.POINTER(i64), ip, ip],
+        "pic_autotune": [vp, i32, ctypes.POINTER(pic_tuning)],
     }
     sig.update({
         "pic2d_create": [ctypes.POINTER(vp), ctypes.POINTER(pic2d_config)],

@@ -203,6 +203,28 @@ class Context:
         self._check(self._lib.pic_run(self._h, int(nsteps), L.ptr(diag), L.ptr(sweeps)), "pic_run")
         return diag, sweeps
 
+    # -- particle tiling of the push launches ---------------------------------
+    def tiling(self):
+        """(particles per tile, threads per push workgroup, tiles)."""
+        t, b, n = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        self._check(self._lib.pic_get_tiling(self._h, ctypes.byref(t), ctypes.byref(b), ctypes.byref(n)),
+                    "pic_get_tiling")
+        return t.value, b.value, n.value
+
+    def set_tiling(self, tile_particles, block=1024):
+        self._check(self._lib.pic_set_tiling(self._h, int(tile_particles), int(block)), "pic_set_tiling")
+        return self.tiling()
+
+    def autotune(self, trial_steps=2):
+        """Times trial_steps steps of the current state under each candidate
+        tiling, keeps the fastest and restores the state; returns the result
+        as a dict."""
+        r = L.pic_tuning()
+        self._check(self._lib.pic_autotune(self._h, int(trial_steps), ctypes.byref(r)), "pic_autotune")
+        return {"tile_particles": r.tile_particles, "block": r.block, "n_tiles": r.n_tiles,
+                "candidates": r.candidates, "ms_per_step": r.ms_per_step,
+                "entry_ms_per_step": r.entry_ms_per_step}
+
     def run_profiled(self, nsteps):
         n = len(self.configs)
         diag = np.zeros((n, nsteps, 4))

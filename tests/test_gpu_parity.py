@@ -51,7 +51,7 @@ def test_deposit_ngp_bitwise_and_ties(pg, orc):
     assert same(got, orc.deposit(N, 0, x, w=3.125))
 
 
-@pytest.mark.parametrize("N", [8, 64, 128, 1024, 4096])
+@pytest.mark.parametrize("N", [8, 16, 32, 64, 128, 256, 1024, 2048, 4096])
 def test_fieldsolve_bitwise(pg, orc, N):
     rng = np.random.default_rng(N)
     rho = 100 + rng.standard_normal(N)
