@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the ensemble and 2D3V workloads reported beside the headline")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--workload", default="scaled", choices=["scaled", "ensemble", "es2d3v"],
+                    help="scaled: the headline line (default); ensemble / es2d3v: only that workload, "
+                         "--steps timed steps, one JSON line (for profiling)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default particle tiling instead of autotuning it before the timed steps")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -290,6 +293,12 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU sharding is a design "
               f"note, not built", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "ensemble":
+        print(json.dumps(bench_ensemble(steps=a.steps, warmup=a.warmup, autotune=not a.no_autotune)))
+        return
+    if a.workload == "es2d3v":
+        print(json.dumps(bench_2d(steps=a.steps)))
+        return
     import numpy as np
     from picgolf import configs
     precs = ["mixed", "fp64"] if a.precision == "both" else [a.precision]
