@@ -60,6 +60,11 @@ def lib():
         "orc_quiet_start": (None, [c_i64, _dp, _dp]),
         "orc_run": (ctypes.c_int, [c_i32, c_i32, c_i32, c_i64, c_d, c_d, c_d, c_d, c_i32,
                                    _dp, _dp, _dp, c_i64, c_vp, c_vp, c_i32, c_i32]),
+        "orc_area_stencil": (None, [c_i32, c_d, _ip64, _dp]),
+        "orc_gather_at": (c_d, [c_i32, c_i32, _dp, c_d]),
+        "orc_deposit_par": (None, [c_i32, c_i32, c_i64, _dp, c_vp, c_d, _dp, c_i32]),
+        "orc_run_simpson": (ctypes.c_int, [c_i32, c_i32, c_i64, c_d, c_d, c_d, c_d, c_i32, _dp, _dp, _dp,
+                                           c_i64, c_vp, c_vp, c_i32]),
         "orc2d_unimod": (c_d, [c_d, c_d]),
         "orc2d_unimodi": (c_i64, [c_i64, c_i64]),
         "orc2d_area_weights": (None, [c_d, c_i32, c_i32, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32),
@@ -244,6 +249,38 @@ def run(N, shape, scheme, W, w, dt, x, v, nsteps, rtol=1e-8, max_sweeps=10, E=No
     if rc != 0:
         raise ValueError(f"orc_run rejected its arguments (rc={rc})")
     return x, v, E, diag, sweeps
+
+
+# ---- Simpson-1/3 fixed point (src/*Simpson13.jl) ----------------------------
+SHAPE_AREA = 1
+
+
+def area_stencil(N, c):
+    """d(y) of src/AreaFixedPointQuietSimpson13.jl:5: ((cell, weight), (cell, weight)), cells 0-based."""
+    cells = np.zeros(2, dtype=np.int64)
+    wt = np.zeros(2)
+    lib().orc_area_stencil(int(N), float(c), cells, wt)
+    return (int(cells[0]), float(wt[0])), (int(cells[1]), float(wt[1]))
+
+
+def gather_at(N, shape, E, c):
+    """sum(k -> E[k[1]]*k[2], d(c)) as a left fold; shape 1 (area), 6 or 7 (Gaussian)."""
+    return lib().orc_gather_at(int(N), int(shape), _f64(E), float(c))
+
+
+def run_simpson(N, shape, W, w, dt, x, v, nsteps, rtol, max_sweeps=10, E3=None, nthreads=1):
+    """nsteps of the Simpson-1/3 loop.  E3 is the (3, N) field matrix (rows
+    E[1,:], E[2,:], E[3,:]).  Returns (x, v, E3, diag, sweeps)."""
+    x = _f64(x).copy()
+    v = _f64(v).copy()
+    Ecm = np.zeros(3 * N) if E3 is None else np.ascontiguousarray(_f64(E3).T).ravel().copy()
+    diag = np.zeros((nsteps, 4))
+    sweeps = np.zeros(nsteps, dtype=np.int32)
+    rc = lib().orc_run_simpson(int(N), int(shape), x.size, float(W), float(w), float(dt), float(rtol),
+                               int(max_sweeps), x, v, Ecm, int(nsteps), _ptr(diag), _ptr(sweeps), int(nthreads))
+    if rc != 0:
+        raise ValueError(f"orc_run_simpson rejected its arguments (rc={rc})")
+    return x, v, Ecm.reshape(N, 3).T.copy(), diag, sweeps
 
 
 # ---- electrostatic 2D3V (src/Electrostatic2D3V.jl) -------------------------

@@ -546,6 +546,44 @@ void orc_gauss_stencil(int32_t N, int32_t R, double c, int64_t* cells, double* w
     }
 }
 
+/* d(y) = (i = Int(mod1(ceil(y*N), N)); o = ceil(y*N) - y*N;
+ *         ((i, 1-o), (mod1(i-1, N), o)))
+ * (src/AreaFixedPointQuietSimpson13.jl:5): linear weights onto the grid points
+ * i/N and (i-1)/N around y.  ceil(y*N) is an integer-valued Float64, so the
+ * Float64 mod1 of the script is the integer mod1 here.  Cells 0-based. */
+void orc_area_stencil(int32_t N, double c, int64_t* cells, double* wt) {
+    const double yN = c * (double)N;
+    const double cy = ceil(yN);
+    const int64_t i = orc_jl_mod1i((int64_t)cy, N);
+    const double o = cy - yN;
+    cells[0] = i - 1;
+    wt[0] = 1.0 - o;
+    cells[1] = orc_jl_mod1i(i - 1, N) - 1;
+    wt[1] = o;
+}
+
+/* The stencil of a shape (6 / 7: Gaussian half-width, ORC_SHAPE_AREA: linear
+ * area weights); returns the number of cells. */
+static int32_t stencil_of(int32_t N, int32_t shape, double c, int64_t* cells, double* wt) {
+    if (shape == ORC_SHAPE_AREA) {
+        orc_area_stencil(N, c, cells, wt);
+        return 2;
+    }
+    orc_gauss_stencil(N, shape, c, cells, wt);
+    return 2 * shape + 1;
+}
+
+/* sum(k -> E[k[1]]*k[2], d(c)): a left fold over the stencil in order
+ * (src/GaussianFixedPointQuiet.jl:10, src/AreaFixedPointQuietSimpson13.jl:11). */
+double orc_gather_at(int32_t N, int32_t shape, const double* E, double c) {
+    int64_t cells[2 * 16 + 1];
+    double wt[2 * 16 + 1];
+    const int32_t n = stencil_of(N, shape, c, cells, wt);
+    double s = E[cells[0]] * wt[0];
+    for (int32_t k = 1; k < n; ++k) s = s + E[cells[k]] * wt[k];
+    return s;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Deposits                                                                  */
 /* ------------------------------------------------------------------------ */
@@ -568,8 +606,8 @@ void orc_deposit(int32_t N, int32_t shape, int64_t P, const double* x,
         double wt[2 * MAXR + 1];
         for (int64_t j = 0; j < P; ++j) {
             double c = y ? (x[j] + y[j]) / 2.0 : (x[j] + x[j]) / 2.0;
-            orc_gauss_stencil(N, shape, c, cells, wt);
-            for (int32_t k = 0; k < 2 * shape + 1; ++k) rho[cells[k]] += wt[k] * w;
+            const int32_t n = stencil_of(N, shape, c, cells, wt);
+            for (int32_t k = 0; k < n; ++k) rho[cells[k]] += wt[k] * w;
         }
         return;
     }
@@ -583,8 +621,8 @@ void orc_deposit(int32_t N, int32_t shape, int64_t P, const double* x,
         double wt[2 * MAXR + 1];
         for (int64_t j = 0; j < P; ++j) {
             double c = y ? (x[j] + y[j]) / 2.0 : (x[j] + x[j]) / 2.0;
-            orc_gauss_stencil(N, shape, c, cells, wt);
-            for (int32_t k = 0; k < 2 * shape + 1; ++k) acc[cells[k]] += fxp_from(wt[k] * w, q);
+            const int32_t n = stencil_of(N, shape, c, cells, wt);
+            for (int32_t k = 0; k < n; ++k) acc[cells[k]] += fxp_from(wt[k] * w, q);
         }
     }
     for (int32_t i = 0; i < N; ++i) rho[i] = fxp_to(acc[i], q);
@@ -688,8 +726,8 @@ void orc_quiet_start(int64_t P, double* x, double* v) {
 
 /* Parallel exact deposit: per-thread u128 grids summed afterwards (integer
  * sums, so bitwise equal to the serial deposit). */
-static void deposit_exact_par(int32_t N, int32_t shape, int64_t P, const double* x,
-                              const double* y, double w, double* rho, int nthreads) {
+void orc_deposit_par(int32_t N, int32_t shape, int64_t P, const double* x,
+                     const double* y, double w, double* rho, int32_t nthreads) {
 #ifdef _OPENMP
     if (nthreads > 1) {
         int q = orc_fxp_frac_bits(P, w);
@@ -706,8 +744,8 @@ static void deposit_exact_par(int32_t N, int32_t shape, int64_t P, const double*
                     my[orc_ngp_cell(x[j], N)] += fxp_from(w, q);
                 } else {
                     double c = y ? (x[j] + y[j]) / 2.0 : (x[j] + x[j]) / 2.0;
-                    orc_gauss_stencil(N, shape, c, cells, wt);
-                    for (int32_t k = 0; k < 2 * shape + 1; ++k) my[cells[k]] += fxp_from(wt[k] * w, q);
+                    const int32_t n = stencil_of(N, shape, c, cells, wt);
+                    for (int32_t k = 0; k < n; ++k) my[cells[k]] += fxp_from(wt[k] * w, q);
                 }
             }
         }
@@ -756,7 +794,7 @@ int orc_run(int32_t N, int32_t shape, int32_t scheme, int64_t P, double W,
     if (scheme == 0) {
         for (int64_t t = 0; t < nsteps; ++t) {
             orc_drift_half(P, x, v, dt);
-            if (deposit_mode == 0) deposit_exact_par(N, shape, P, x, NULL, w, rho, nthreads);
+            if (deposit_mode == 0) orc_deposit_par(N, shape, P, x, NULL, w, rho, nthreads);
             else orc_deposit(N, shape, P, x, NULL, w, 1, rho);
             orc_fieldsolve(N, rho, E);
             orc_drift_half(P, x, v, dt);
@@ -779,7 +817,7 @@ int orc_run(int32_t N, int32_t shape, int32_t scheme, int64_t P, double W,
             if (orc_isapprox(F, E, N, rtol, 0.0)) break;
             memcpy(F, E, sizeof(double) * (size_t)N);
             orc_midpoint_position(P, X, V, v, x, dt);
-            if (deposit_mode == 0) deposit_exact_par(N, shape, P, x, X, w, rho, nthreads);
+            if (deposit_mode == 0) orc_deposit_par(N, shape, P, x, X, w, rho, nthreads);
             else orc_deposit(N, shape, P, x, X, w, 1, rho);
             orc_fieldsolve(N, rho, E);
             push_midpoint_par(N, shape, P, X, V, x, v, E, dt, nthreads);

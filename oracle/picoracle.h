@@ -47,10 +47,21 @@ void orc_fieldsolve(int32_t N, const double* rho, double* E);
 int64_t orc_ngp_cell(double x, int32_t N);      /* 0-based cell of f(x)          */
 void    orc_gauss_stencil(int32_t N, int32_t R, double c, int64_t* cells, double* wt);
 
+/* linear area weights of src/AreaFixedPointQuietSimpson13.jl:5 (shape code
+ * ORC_SHAPE_AREA wherever a shape is taken) */
+#define ORC_SHAPE_AREA 1
+void    orc_area_stencil(int32_t N, double c, int64_t* cells, double* wt);
+/* sum(k -> E[k[1]]*k[2], d(c)) for shape 1 / 6 / 7 (left fold) */
+double  orc_gather_at(int32_t N, int32_t shape, const double* E, double c);
+
 /* deposits (A3, A8): mode 0 = exact per-cell sum (the semantics the GPU
- * reproduces bitwise), mode 1 = the scripts' literal sequential order */
+ * reproduces bitwise), mode 1 = the scripts' literal sequential order.
+ * shape: 0 = NGP, 1 = area, 6 / 7 = Gaussian.  orc_deposit_par is mode 0 on
+ * nthreads OpenMP threads (bitwise equal). */
 void orc_deposit(int32_t N, int32_t shape, int64_t P, const double* x,
                  const double* y, double w, int32_t mode, double* rho);
+void orc_deposit_par(int32_t N, int32_t shape, int64_t P, const double* x,
+                     const double* y, double w, double* rho, int32_t nthreads);
 
 /* particle updates (A4, A6, A10, A11) */
 void orc_drift_half(int64_t P, double* x, const double* v, double dt);
@@ -82,6 +93,14 @@ int orc_run(int32_t N, int32_t shape, int32_t scheme, int64_t P, double W,
             double* x, double* v, double* E, int64_t nsteps,
             double* diag, int32_t* sweeps, int32_t deposit_mode,
             int32_t nthreads);
+
+/* Simpson-1/3 fixed point (src/GaussianFixedPointQuietSimpson13.jl:8-18,
+ * src/AreaFixedPointQuietSimpson13.jl:7-17), simpson.c.  shape: 1 = area,
+ * 6 / 7 = Gaussian.  E3 is the script's 3 x N field matrix in column-major
+ * order (E[r + 3*i] is row r+1, cell i+1), carried between steps. */
+int orc_run_simpson(int32_t N, int32_t shape, int64_t P, double W, double w, double dt,
+                    double rtol, int32_t max_sweeps, double* x, double* v, double* E3,
+                    int64_t nsteps, double* diag, int32_t* sweeps, int32_t nthreads);
 
 /* electrostatic 2D3V (src/Electrostatic2D3V.jl), pic2d.c.  Grids are
  * column-major NX x NY (x fastest); cells are returned 0-based.  textbook = 0
