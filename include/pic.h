@@ -37,6 +37,7 @@ extern "C" {
  * 6 (src/GaussianFixedPoint.jl:5, src/Gaussian.jl:6) or 7
  * (src/GaussianFixedPointQuiet.jl:6) */
 #define PIC_SHAPE_NGP 0
+#define PIC_SHAPE_AREA 1   /* linear area weights, src/AreaFixedPointQuietSimpson13.jl:5 (Simpson only) */
 #define PIC_SHAPE_GAUSS6 6
 #define PIC_SHAPE_GAUSS7 7
 
@@ -45,6 +46,7 @@ extern "C" {
  * (src/GaussianFixedPoint.jl:7-9, src/GaussianFixedPointQuiet.jl:8-10) */
 #define PIC_SCHEME_LEAPFROG 0
 #define PIC_SCHEME_MIDPOINT 1
+#define PIC_SCHEME_SIMPSON 2  /* Simpson-1/3 fixed point, src/GaussianFixedPointQuietSimpson13.jl:8-17 (FP64, N <= 1024) */
 
 /* precision of pic_run: FP64 throughout (bitwise equal to the CPU
  * restatement), or MIXED: FP32 particle push and shape weights with exact

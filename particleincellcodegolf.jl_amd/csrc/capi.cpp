@@ -182,17 +182,25 @@ static void drop_graphs(pic_ctx* c) {
 static const char* validate(const pic_config& k) {
     if (k.N < 8 || k.N > kMaxN || (k.N & (k.N - 1)) != 0) return "N must be a power of two in [8, 4096]";
     if (k.P < 1 || k.P > (int64_t)1 << 31) return "P must be in [1, 2^31]";
-    if (k.shape != PIC_SHAPE_NGP && k.shape != PIC_SHAPE_GAUSS6 && k.shape != PIC_SHAPE_GAUSS7)
-        return "shape must be PIC_SHAPE_NGP, PIC_SHAPE_GAUSS6 or PIC_SHAPE_GAUSS7";
-    if (k.scheme != PIC_SCHEME_LEAPFROG && k.scheme != PIC_SCHEME_MIDPOINT) return "unknown scheme";
-    if (k.scheme == PIC_SCHEME_MIDPOINT && k.shape == PIC_SHAPE_NGP)
+    if (k.shape != PIC_SHAPE_NGP && k.shape != PIC_SHAPE_AREA && k.shape != PIC_SHAPE_GAUSS6 &&
+        k.shape != PIC_SHAPE_GAUSS7)
+        return "shape must be PIC_SHAPE_NGP, PIC_SHAPE_AREA, PIC_SHAPE_GAUSS6 or PIC_SHAPE_GAUSS7";
+    if (k.scheme != PIC_SCHEME_LEAPFROG && k.scheme != PIC_SCHEME_MIDPOINT && k.scheme != PIC_SCHEME_SIMPSON)
+        return "unknown scheme";
+    if (k.scheme == PIC_SCHEME_MIDPOINT && k.shape != PIC_SHAPE_GAUSS6 && k.shape != PIC_SHAPE_GAUSS7)
         return "the midpoint scheme needs a Gaussian shape (as in the scripts)";
+    if (k.scheme == PIC_SCHEME_LEAPFROG && k.shape == PIC_SHAPE_AREA)
+        return "the area shape is provided for the Simpson scheme";
+    if (k.scheme == PIC_SCHEME_SIMPSON && k.shape == PIC_SHAPE_NGP)
+        return "the Simpson scheme needs the area or a Gaussian shape (as in the scripts)";
+    if (k.scheme == PIC_SCHEME_SIMPSON && k.N > 1024)
+        return "the Simpson scheme is provided for N <= 1024 (its 3 x N convergence test is one reduction chunk)";
     if (k.precision != PIC_PREC_FP64 && k.precision != PIC_PREC_MIXED) return "unknown precision";
     if (k.precision == PIC_PREC_MIXED && k.scheme != PIC_SCHEME_MIDPOINT)
         return "mixed precision is provided for the midpoint scheme";
     if (!(k.w >= 0.0) || !std::isfinite(k.w)) return "w must be finite and >= 0";
     if (!std::isfinite(k.dt) || !std::isfinite(k.W) || !(k.W > 0.0)) return "dt must be finite, W finite and > 0";
-    if (k.scheme == PIC_SCHEME_MIDPOINT && (k.max_sweeps < 1 || k.max_sweeps > 64))
+    if (k.scheme != PIC_SCHEME_LEAPFROG && (k.max_sweeps < 1 || k.max_sweeps > 64))
         return "max_sweeps must be in [1, 64]";
     if (!(k.rtol >= 0.0)) return "rtol must be >= 0";
     if (k.precision == PIC_PREC_MIXED && !mixed_available()) return "mixed precision kernels are not built";
@@ -224,7 +232,8 @@ const char* pic_last_error(const pic_ctx* ctx) { return ctx ? ctx->err.c_str() :
 // many tiles as fit at once (LDS-bound); mixed: two 1024-thread blocks per CU,
 // so the per-tile partials (N u64 each) stay small.
 static void default_tiling(const pic_ctx* c, int64_t* tileP, int* block) {
-    const int lds_push = push_lds_bytes(c->maxN, c->precision);
+    const int lds_push = c->scheme == PIC_SCHEME_SIMPSON ? simpson_push_lds_bytes(c->maxN)
+                                                          : push_lds_bytes(c->maxN, c->precision);
     const int wg_per_cu = std::max(1, std::min(4, (160 * 1024) / std::max(lds_push, 1)));
     const int64_t target = (int64_t)c->numCU * (c->precision == 0 ? wg_per_cu : 2);
     *tileP = (c->Ptot + target - 1) / target;
@@ -339,7 +348,7 @@ static int build_tables(pic_ctx* c) {
         for (int j = 0; j < 2; ++j) {
             if ((rc = dalloc(c, &b.X[j], P))) return rc;
             if ((rc = dalloc(c, &b.V[j], P))) return rc;
-            if (c->scheme == PIC_SCHEME_MIDPOINT && (rc = dalloc(c, &b.vit[j], P))) return rc;
+            if (c->scheme != PIC_SCHEME_LEAPFROG && (rc = dalloc(c, &b.vit[j], P))) return rc;
         }
         if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.racc), 3 * (size_t)red0))) return rc;
         HIPCHK(c, hipMemset(b.racc, 0, 3 * 8 * (size_t)red0));
@@ -354,6 +363,19 @@ static int build_tables(pic_ctx* c) {
         HIPCHK(c, hipMemset(b.racc, 0, 8 * (size_t)red0));
         b.racc_cells = red0;
         b.prime_next = c->scheme == PIC_SCHEME_MIDPOINT ? 1 : 0;
+    }
+    if (c->scheme == PIC_SCHEME_SIMPSON) {
+        if ((rc = dalloc(c, &b.Es1, G))) return rc;
+        if ((rc = dalloc(c, &b.Es2[0], G))) return rc;
+        if ((rc = dalloc(c, &b.Es2[1], G))) return rc;
+        if ((rc = dalloc(c, &b.Vg, P))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.racc2), 3 * (size_t)red0))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.racc3), 3 * (size_t)red0))) return rc;
+        HIPCHK(c, hipMemset(b.racc2, 0, 3 * 8 * (size_t)red0));
+        HIPCHK(c, hipMemset(b.racc3, 0, 3 * 8 * (size_t)red0));
+        HIPCHK(c, hipMemset(b.Es1, 0, sizeof(double) * G));
+        HIPCHK(c, hipMemset(b.Es2[0], 0, sizeof(double) * G));
+        HIPCHK(c, hipMemset(b.Es2[1], 0, sizeof(double) * G));
     }
     if ((rc = dalloc(c, &b.rho, G))) return rc;
     if ((rc = dalloc(c, &b.E[0], G))) return rc;
@@ -394,7 +416,7 @@ int pic_create_batch(pic_ctx** out, const pic_config* cfgs, int32_t n) {
     c->R = cfgs[0].shape;
     c->scheme = cfgs[0].scheme;
     c->precision = cfgs[0].precision;
-    c->max_sweeps = cfgs[0].scheme == PIC_SCHEME_MIDPOINT ? cfgs[0].max_sweeps : 1;
+    c->max_sweeps = cfgs[0].scheme != PIC_SCHEME_LEAPFROG ? cfgs[0].max_sweeps : 1;
     c->device = cfgs[0].device;
     int rc = PIC_OK;
     auto fail = [&](int code) {
@@ -481,8 +503,14 @@ int pic_upload(pic_ctx* c, int32_t inst, const double* x, const double* v, const
         HIPCHK(c, hipMemcpyAsync(c->rb.Vl[par] + d.poff, vl.data(), P * 4, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
-    std::vector<double> e((size_t)d.N, 0.0);
+    std::vector<double> e((size_t)d.N, 0.0), zero((size_t)d.N, 0.0);
     if (E) std::memcpy(e.data(), E, sizeof(double) * d.N);
+    if (c->scheme == PIC_SCHEME_SIMPSON) {
+        // E is the script's E[end,:]; E[1,:] and E[2,:] start at zero
+        for (double* g : {c->rb.Es1, c->rb.Es2[0], c->rb.Es2[1]})
+            HIPCHK(c, hipMemcpyAsync(g + d.goff, zero.data(), sizeof(double) * d.N, hipMemcpyHostToDevice,
+                                     c->stream));
+    }
     std::vector<float> e32((size_t)d.N);
     for (int i = 0; i < d.N; ++i) e32[i] = (float)e[i];
     HIPCHK(c, hipMemcpyAsync(c->rb.E[0] + d.goff, e.data(), sizeof(double) * d.N, hipMemcpyHostToDevice, c->stream));
@@ -546,6 +574,13 @@ static void enqueue_step(pic_ctx* c, hipStream_t s, int slots) {
             launch_solve(b, k, last && slots < c->max_sweeps, s);
             launch_mid_push(b, k, last, s);
         }
+    } else if (c->scheme == PIC_SCHEME_SIMPSON) {
+        launch_simp_first(b, s);
+        for (int k = 0; k < slots; ++k) {
+            const bool last = k == slots - 1;
+            launch_simp_solve(b, k, last && slots < c->max_sweeps, s);
+            launch_simp_push(b, k, last, s);
+        }
     } else {
         launch_lf_first(b, s);
         launch_solve(b, 0, false, s);
@@ -558,8 +593,13 @@ static void enqueue_step(pic_ctx* c, hipStream_t s, int slots) {
 static void enqueue_tail(pic_ctx* c, hipStream_t s, int k0) {
     const RunBuffers& b = c->rb;
     for (int k = k0; k < c->max_sweeps; ++k) {
-        launch_solve(b, k, false, s);
-        launch_mid_push(b, k, false, s);
+        if (c->scheme == PIC_SCHEME_SIMPSON) {
+            launch_simp_solve(b, k, false, s);
+            launch_simp_push(b, k, false, s);
+        } else {
+            launch_solve(b, k, false, s);
+            launch_mid_push(b, k, false, s);
+        }
     }
     launch_diag(b, s);
 }
@@ -653,12 +693,12 @@ static int prime(pic_ctx* c) {
 // Sweep slots per step for the next graphs: every slot the loop may need
 // until a step has been observed, then the most sweeps seen so far.
 static int choose_slots(const pic_ctx* c) {
-    if (c->scheme != PIC_SCHEME_MIDPOINT || c->sweeps_seen == 0) return c->max_sweeps;
+    if (c->scheme == PIC_SCHEME_LEAPFROG || c->sweeps_seen == 0) return c->max_sweeps;
     return std::min(c->max_sweeps, std::max(1, c->sweeps_seen));
 }
 
 static int note_sweeps(pic_ctx* c, int64_t row0, int64_t row1) {
-    if (c->scheme != PIC_SCHEME_MIDPOINT || row1 <= row0) return PIC_OK;
+    if (c->scheme == PIC_SCHEME_LEAPFROG || row1 <= row0) return PIC_OK;
     const size_t n = (size_t)row1, cap = (size_t)c->rb.diag_cap;
     std::vector<int32_t> h(n * (size_t)c->ninst);
     int rc = copy_rows(c, h.data(), c->rb.sweeps, n * sizeof(int32_t), cap * sizeof(int32_t));
@@ -766,6 +806,16 @@ int pic_run_profiled(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps, 
                 for (auto& s : st) all_done = all_done && s.last_sweep >= 0 && s.last_sweep <= k;
                 if (all_done) break;
             }
+        } else if (c->scheme == PIC_SCHEME_SIMPSON) {
+            if ((rc = timed(K_PUSH0, [&] { launch_simp_first(b, c->stream); }))) break;
+            for (int k = 0; k < c->max_sweeps; ++k) {
+                if ((rc = timed(K_SOLVE, [&] { launch_simp_solve(b, k, false, c->stream); }))) break;
+                if ((rc = timed(K_PUSH, [&] { launch_simp_push(b, k, false, c->stream); }))) break;
+                if ((rc = read_states(c, st))) break;
+                bool all_done = true;
+                for (auto& s : st) all_done = all_done && s.last_sweep >= 0 && s.last_sweep <= k;
+                if (all_done) break;
+            }
         } else {
             if ((rc = timed(K_LF1, [&] { launch_lf_first(b, c->stream); }))) break;
             if ((rc = timed(K_SOLVE, [&] { launch_solve(b, 0, false, c->stream); }))) break;
@@ -814,7 +864,16 @@ static std::vector<std::pair<void*, size_t>> run_state(pic_ctx* c) {
         v.push_back({b.E[j], G * 8});
     }
     v.push_back({b.E32, G * 4});
-    v.push_back({b.racc, 8 * (size_t)(c->hinst.back().red0 + c->hinst.back().N) * (c->precision == 0 ? 3 : 1)});
+    const size_t cells = (size_t)(c->hinst.back().red0 + c->hinst.back().N);
+    v.push_back({b.racc, 8 * cells * (c->precision == 0 ? 3 : 1)});
+    if (c->scheme == PIC_SCHEME_SIMPSON) {
+        v.push_back({b.Es1, G * 8});
+        v.push_back({b.Es2[0], G * 8});
+        v.push_back({b.Es2[1], G * 8});
+        v.push_back({b.Vg, P * 8});
+        v.push_back({b.racc2, 24 * cells});
+        v.push_back({b.racc3, 24 * cells});
+    }
     v.push_back({b.st, sizeof(InstState) * (size_t)c->ninst});
     v.push_back({b.halt, sizeof(int32_t)});
     return v;

@@ -802,11 +802,293 @@ k_diag_final(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
             o[1] = d2 * s;
             o[2] = d3 * s;
             o[3] = sp;
-            sweeps[(int64_t)blockIdx.x * cap + row] = scheme == 1 ? S.last_sweep + 1 : 1;
+            sweeps[(int64_t)blockIdx.x * cap + row] = scheme != 0 ? S.last_sweep + 1 : 1;
         }
         S.step += 1;
         st[blockIdx.x] = S;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Simpson-1/3 fixed point (src/GaussianFixedPointQuietSimpson13.jl:8-17,
+// src/AreaFixedPointQuietSimpson13.jl:7-16).  Per step E[1,:] =
+// solve(rho(X, X)); per sweep k, with x_k = X + ((v_k + V)/2)*dt,
+//   v_{k+1} = ((V + (g1*dt)/6) + (g2*(4*dt))/6) + (g3*dt)/6,
+// g1 = gather(E[1,:], (X+X)/2), g2 = gather(E[2,:], (X+x_k)/2) with
+// E[2,:] = solve(rho(X, x_k)), g3 = gather(E[3,:], (x_k+x_k)/2) with
+// E[3,:] = solve(rho(x_k, x_k)); the convergence test compares the 3 x N
+// field matrix of consecutive sweeps.  SH is the shape: 1 = the linear area
+// weights of src/AreaFixedPointQuietSimpson13.jl:5, 6 / 7 = Gaussian.
+// The push of sweep k computes v_{k+1} and, unless the loop ends there,
+// deposits both charges of sweep k+1, like the midpoint push.
+// ---------------------------------------------------------------------------
+
+// d(y) = (i = Int(mod1(ceil(y*N), N)); o = ceil(y*N) - y*N;
+//         ((i, 1-o), (mod1(i-1, N), o)))  (src/AreaFixedPointQuietSimpson13.jl:5)
+__device__ __forceinline__ void area_weights(double c, double Nd, int32_t nmask, int32_t& c0, double& w0,
+                                             int32_t& c1, double& w1) {
+    const double yN = c * Nd;
+    const double cy = ceil(yN);
+    const int64_t i = (int64_t)cy;
+    const double o = cy - yN;
+    c0 = cell0(i, nmask);
+    w0 = 1.0 - o;
+    c1 = cell0(i - 1, nmask);
+    w1 = o;
+}
+
+template <int SH>
+__device__ __forceinline__ double gather_shape(const double* __restrict__ Eld, double c, double Nd,
+                                               int32_t nmask) {
+    if constexpr (SH == 1) {
+        int32_t c0, c1;
+        double w0, w1;
+        area_weights(c, Nd, nmask, c0, w0, c1, w1);
+        const double s = Eld[c0] * w0;
+        return s + Eld[c1] * w1;
+    } else {
+        return gather_gauss<SH>(Eld, c, Nd, nmask);
+    }
+}
+
+template <int SH>
+__device__ __forceinline__ void deposit_shape(u64* __restrict__ acc, int32_t N, double c, double Nd,
+                                              int32_t nmask, double w, int q) {
+    if constexpr (SH == 1) {
+        int32_t c0, c1;
+        double w0, w1;
+        area_weights(c, Nd, nmask, c0, w0, c1, w1);
+        const double a0 = w0 * w, a1 = w1 * w;
+        if (a0 != 0.0) lds_add_fxp(acc, N, c0, fxp_from(a0, q));
+        if (a1 != 0.0) lds_add_fxp(acc, N, c1, fxp_from(a1, q));
+    } else {
+        deposit_gauss<SH>(acc, N, c, Nd, nmask, w, q);
+    }
+}
+
+// Reads and clears an instance's three-limb charge, into re/im in bit-reversed order.
+__device__ __forceinline__ void take_charge64(u64* __restrict__ a, double* re, double* im, int N, int log2N, int q) {
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        u64* p = a + 3 * j;
+        const u64 l0 = p[0], l1 = p[1], l2 = p[2];
+        p[0] = 0;
+        p[1] = 0;
+        p[2] = 0;
+        const int b = padi((int)brev((uint32_t)j, log2N));
+        re[b] = fxp_to(limbs_to_u128(l0, l1, l2), q);
+        im[b] = 0.0;
+    }
+}
+
+// rho(X, X) of the step into racc (the new loop starts: last_sweep = -1).
+template <int SH>
+__global__ void __launch_bounds__(1024)
+k_simp_first(const InstDesc* __restrict__ insts, InstState* __restrict__ st, const TileDesc* __restrict__ tiles,
+             const double* X0, const double* X1, u64* __restrict__ racc, const int32_t* __restrict__ halt) {
+    extern __shared__ u64 lds[];
+    if (halted(halt, false)) return;
+    const TileDesc T = tiles[blockIdx.x];
+    const InstDesc I = insts[T.inst];
+    const InstState S = st[T.inst];
+    if (T.local == 0 && threadIdx.x == 0) st[T.inst].last_sweep = -1;
+    const int32_t N = I.N;
+    u64* acc = lds;
+    zero_lds(acc, 3 * N);
+    __syncthreads();
+    const double* X = (S.step & 1) ? X1 : X0;
+    const double Nd = (double)N;
+    const int64_t base = I.poff + T.start;
+    for (int64_t p = threadIdx.x; p < T.count; p += blockDim.x) {
+        const double Xp = X[base + p];
+        deposit_shape<SH>(acc, N, (Xp + Xp) / 2.0, Nd, I.nmask, I.w, I.q);
+    }
+    __syncthreads();
+    flush_limbs(acc, N, racc + 3 * I.red0);
+}
+
+// E[1,:] = solve(rho(X, X)): one block per instance.
+__global__ void __launch_bounds__(kSolveThreads)
+k_simp_solve1(const InstDesc* __restrict__ insts, InstState* __restrict__ st, u64* __restrict__ racc,
+              double* __restrict__ Es1, const double2* __restrict__ twp, const int32_t* __restrict__ halt) {
+    extern __shared__ double sm[];
+    if (halted(halt, false)) return;
+    const InstDesc I = insts[blockIdx.x];
+    const int N = I.N, log2N = I.log2N, NP = N + N / 16;
+    double* re = sm;
+    double* im = sm + NP;
+    double* red = sm + 2 * NP + N;
+    SolveRegs R;
+    solve_prefetch(R, twp, N, log2N);
+    take_charge64(racc + 3 * I.red0, re, im, N, log2N, I.q);
+    __syncthreads();
+    solve_lds(re, im, R, twp, N, log2N);
+    for (int j = threadIdx.x; j < N; j += blockDim.x) Es1[I.goff + j] = re[padi(j)];
+    const double emax = block_maxabs(red, N, [&](int64_t i) { return re[padi((int)i)]; });
+    if (threadIdx.x == 0 && !isfinite(emax)) st[blockIdx.x].err |= 1;
+}
+
+// Per particle: Vg = V + (g1*dt)/6 and, with v_0 = V, the charges rho(X, x_0)
+// and rho(x_0, x_0) of sweep 0.
+template <int SH>
+__global__ void __launch_bounds__(1024)
+k_simp_start(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
+             const TileDesc* __restrict__ tiles, const double* X0, const double* X1, const double* V0,
+             const double* V1, double* __restrict__ Vg, const double* __restrict__ Es1, u64* __restrict__ racc2,
+             u64* __restrict__ racc3, const int32_t* __restrict__ halt) {
+    extern __shared__ u64 lds[];
+    if (halted(halt, false)) return;
+    const TileDesc T = tiles[blockIdx.x];
+    const InstDesc I = insts[T.inst];
+    const InstState S = st[T.inst];
+    const int32_t N = I.N;
+    double* E1ld = reinterpret_cast<double*>(lds);
+    u64* acc2 = lds + N;
+    u64* acc3 = lds + 4 * N;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) E1ld[i] = Es1[I.goff + i];
+    zero_lds(acc2, 6 * N);
+    __syncthreads();
+    const int par = S.step & 1;
+    const double* X = par ? X1 : X0;
+    const double* V = par ? V1 : V0;
+    const double Nd = (double)N, dt = I.dt;
+    const int64_t base = I.poff + T.start;
+    for (int64_t p = threadIdx.x; p < T.count; p += blockDim.x) {
+        const int64_t g = base + p;
+        const double Xp = X[g], Vp = V[g];
+        const double g1 = gather_shape<SH>(E1ld, (Xp + Xp) / 2.0, Nd, I.nmask);
+        Vg[g] = Vp + (g1 * dt) / 6.0;
+        const double x0 = Xp + ((Vp + Vp) / 2.0) * dt;
+        deposit_shape<SH>(acc2, N, (Xp + x0) / 2.0, Nd, I.nmask, I.w, I.q);
+        deposit_shape<SH>(acc3, N, (x0 + x0) / 2.0, Nd, I.nmask, I.w, I.q);
+    }
+    __syncthreads();
+    flush_limbs(acc2, N, racc2 + 3 * I.red0);
+    flush_limbs(acc3, N, racc3 + 3 * I.red0);
+}
+
+// Sweep k: E[2,:] and E[3,:] from the two charges, then the test the script
+// runs at the top of sweep k+1, isapprox(F, E) over the 3 x N matrices in
+// column-major order (F: the matrix at the start of sweep k).  One block per
+// instance; requires 3N <= 4096 (one canonical chunk).
+__global__ void __launch_bounds__(kSolveThreads)
+k_simp_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st, u64* __restrict__ racc2,
+             u64* __restrict__ racc3, const double* __restrict__ Es1, double* Es2a, double* Es2b, double* E3a,
+             double* E3b, const double2* __restrict__ twp, int k, int32_t* __restrict__ halt, int stall) {
+    extern __shared__ double sm[];
+    if (halted(halt, false)) return;
+    const InstDesc I = insts[blockIdx.x];
+    InstState S = st[blockIdx.x];
+    if (sweep_over(S, k)) return;
+    const int N = I.N, log2N = I.log2N, NP = N + N / 16;
+    double* re = sm;
+    double* im = sm + NP;
+    double* e2n = sm + 2 * NP;
+    double* red = e2n + N;
+    const int cur = S.e_cur, nxt = 1 - cur;
+    const double* E1 = Es1 + I.goff;
+    const double* E2c = (cur ? Es2b : Es2a) + I.goff;
+    const double* E3c = (cur ? E3b : E3a) + I.goff;
+    double* E2o = (nxt ? Es2b : Es2a) + I.goff;
+    double* E3o = (nxt ? E3b : E3a) + I.goff;
+    SolveRegs R;
+    solve_prefetch(R, twp, N, log2N);
+    take_charge64(racc2 + 3 * I.red0, re, im, N, log2N, I.q);
+    __syncthreads();
+    solve_lds(re, im, R, twp, N, log2N);
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        const double e = re[padi(j)];
+        E2o[j] = e;
+        e2n[j] = e;
+    }
+    __syncthreads();
+    solve_prefetch(R, twp, N, log2N);  // the first solve reloaded inverse twiddles
+    take_charge64(racc3 + 3 * I.red0, re, im, N, log2N, I.q);
+    __syncthreads();
+    solve_lds(re, im, R, twp, N, log2N);
+    for (int j = threadIdx.x; j < N; j += blockDim.x) E3o[j] = re[padi(j)];
+    double emax;
+    const bool conv = block_isapprox(
+        red,
+        [&](int e) {
+            const int r = e % 3, i = e / 3;
+            return r == 0 ? E1[i] : (r == 1 ? E2c[i] : E3c[i]);
+        },
+        [&](int e) {
+            const int r = e % 3, i = e / 3;
+            return r == 0 ? E1[i] : (r == 1 ? e2n[i] : re[padi(i)]);
+        },
+        3 * N, I.rtol, 0.0, emax);
+    if (threadIdx.x == 0) {
+        if (!isfinite(emax)) S.err |= 1;
+        S.e_cur = nxt;
+        if (conv || k == I.max_sweeps - 1) S.last_sweep = k;
+        else if (stall) *halt = 1;
+        st[blockIdx.x] = S;
+    }
+}
+
+// Sweep k: v_{k+1} from g2 and g3 at x_k; the final sweep writes
+// (mod(x_k, 1), v_{k+1}), any other deposits rho(X, x_{k+1}) and rho(x_{k+1}, x_{k+1}).
+template <int SH>
+__global__ void __launch_bounds__(1024)
+k_simp_push(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
+            const TileDesc* __restrict__ tiles, double* X0, double* X1, double* V0, double* V1, double* vit0,
+            double* vit1, const double* __restrict__ Vg, const double* Es2a, const double* Es2b,
+            const double* E3a, const double* E3b, u64* __restrict__ racc2, u64* __restrict__ racc3, int k,
+            const int32_t* __restrict__ halt, int last_slot) {
+    extern __shared__ u64 lds[];
+    if (halted(halt, last_slot)) return;
+    const TileDesc T = tiles[blockIdx.x];
+    const InstDesc I = insts[T.inst];
+    const InstState S = st[T.inst];
+    if (sweep_over(S, k)) return;
+    const bool last = S.last_sweep == k;
+    const int32_t N = I.N;
+    double* E2ld = reinterpret_cast<double*>(lds);
+    double* E3ld = E2ld + N;
+    u64* acc2 = lds + 2 * N;
+    u64* acc3 = lds + 5 * N;
+    const double* E2in = (S.e_cur ? Es2b : Es2a) + I.goff;
+    const double* E3in = (S.e_cur ? E3b : E3a) + I.goff;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        E2ld[i] = E2in[i];
+        E3ld[i] = E3in[i];
+    }
+    if (!last) zero_lds(acc2, 6 * N);
+    __syncthreads();
+    const int par = S.step & 1;
+    const double* X = par ? X1 : X0;
+    const double* V = par ? V1 : V0;
+    double* Xo = par ? X0 : X1;
+    double* Vo = par ? V0 : V1;
+    const double* vin = (k & 1) ? vit1 : vit0;
+    double* vout = ((k + 1) & 1) ? vit1 : vit0;
+    const double Nd = (double)N, dt = I.dt;
+    const int32_t nmask = I.nmask;
+    const int64_t base = I.poff + T.start;
+    for (int64_t p = threadIdx.x; p < T.count; p += blockDim.x) {
+        const int64_t g = base + p;
+        const double Xp = X[g], Vp = V[g];
+        const double vk = (k == 0) ? Vp : vin[g];
+        const double x = Xp + ((vk + Vp) / 2.0) * dt;
+        const double g2 = gather_shape<SH>(E2ld, (Xp + x) / 2.0, Nd, nmask);
+        const double g3 = gather_shape<SH>(E3ld, (x + x) / 2.0, Nd, nmask);
+        double vn = Vg[g] + (g2 * (4.0 * dt)) / 6.0;
+        vn = vn + (g3 * dt) / 6.0;
+        if (last) {
+            Xo[g] = jl_mod1f(x);
+            Vo[g] = vn;
+            continue;
+        }
+        vout[g] = vn;
+        const double x2 = Xp + ((vn + Vp) / 2.0) * dt;
+        deposit_shape<SH>(acc2, N, (Xp + x2) / 2.0, Nd, nmask, I.w, I.q);
+        deposit_shape<SH>(acc3, N, (x2 + x2) / 2.0, Nd, nmask, I.w, I.q);
+    }
+    if (last) return;
+    __syncthreads();
+    flush_limbs(acc2, N, racc2 + 3 * I.red0);
+    flush_limbs(acc3, N, racc3 + 3 * I.red0);
 }
 
 // ---------------------------------------------------------------------------
@@ -887,6 +1169,52 @@ void launch_diag(const RunBuffers& b, hipStream_t s) {
         launch_diag_chunks32(b, s);
     k_diag_final<<<b.ninst, 256, 0, s>>>(b.insts, b.st, b.dpart, b.E[0], b.E[1], b.diag, b.sweeps,
                                          b.diag_cap, b.scheme, b.halt, fused ? 1 : 0);
+}
+
+int simpson_push_lds_bytes(int N) { return 64 * N; }  // two fields + two 3-limb charges
+
+void launch_simp_first(const RunBuffers& b, hipStream_t s) {
+    const dim3 g(b.ntiles), t(b.block_push);
+    const int la = 24 * b.maxN, lb = 56 * b.maxN;
+    u64* r1 = (u64*)b.racc;
+    u64* r2 = (u64*)b.racc2;
+    u64* r3 = (u64*)b.racc3;
+    if (b.R == 1) k_simp_first<1><<<g, t, la, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], r1, b.halt);
+    else if (b.R == 6) k_simp_first<6><<<g, t, la, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], r1, b.halt);
+    else k_simp_first<7><<<g, t, la, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], r1, b.halt);
+    k_simp_solve1<<<b.ninst, kSolveThreads, solve_lds_bytes(b.maxN), s>>>(b.insts, b.st, r1, b.Es1,
+                                                                         (const double2*)b.tw, b.halt);
+    if (b.R == 1)
+        k_simp_start<1><<<g, t, lb, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1], b.Vg, b.Es1, r2,
+                                         r3, b.halt);
+    else if (b.R == 6)
+        k_simp_start<6><<<g, t, lb, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1], b.Vg, b.Es1, r2,
+                                         r3, b.halt);
+    else
+        k_simp_start<7><<<g, t, lb, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1], b.Vg, b.Es1, r2,
+                                         r3, b.halt);
+}
+
+void launch_simp_solve(const RunBuffers& b, int k, bool stall, hipStream_t s) {
+    k_simp_solve<<<b.ninst, kSolveThreads, solve_lds_bytes(b.maxN), s>>>(
+        b.insts, b.st, (u64*)b.racc2, (u64*)b.racc3, b.Es1, b.Es2[0], b.Es2[1], b.E[0], b.E[1],
+        (const double2*)b.tw, k, b.halt, stall ? 1 : 0);
+}
+
+void launch_simp_push(const RunBuffers& b, int k, bool last_slot, hipStream_t s) {
+    const dim3 g(b.ntiles), t(b.block_push);
+    const int lds = simpson_push_lds_bytes(b.maxN), ls = last_slot ? 1 : 0;
+    u64* r2 = (u64*)b.racc2;
+    u64* r3 = (u64*)b.racc3;
+    if (b.R == 1)
+        k_simp_push<1><<<g, t, lds, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1], b.vit[0],
+                                         b.vit[1], b.Vg, b.Es2[0], b.Es2[1], b.E[0], b.E[1], r2, r3, k, b.halt, ls);
+    else if (b.R == 6)
+        k_simp_push<6><<<g, t, lds, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1], b.vit[0],
+                                         b.vit[1], b.Vg, b.Es2[0], b.Es2[1], b.E[0], b.E[1], r2, r3, k, b.halt, ls);
+    else
+        k_simp_push<7><<<g, t, lds, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1], b.vit[0],
+                                         b.vit[1], b.Vg, b.Es2[0], b.Es2[1], b.E[0], b.E[1], r2, r3, k, b.halt, ls);
 }
 
 // ---------------------------------------------------------------------------
@@ -1128,6 +1456,17 @@ int configure_kernels() {
     chk(allow_lds(k_lf_second64<0>, big));
     chk(allow_lds(k_lf_second64<6>, big));
     chk(allow_lds(k_lf_second64<7>, big));
+    chk(allow_lds(k_simp_first<1>, big));
+    chk(allow_lds(k_simp_first<6>, big));
+    chk(allow_lds(k_simp_first<7>, big));
+    chk(allow_lds(k_simp_start<1>, big));
+    chk(allow_lds(k_simp_start<6>, big));
+    chk(allow_lds(k_simp_start<7>, big));
+    chk(allow_lds(k_simp_push<1>, big));
+    chk(allow_lds(k_simp_push<6>, big));
+    chk(allow_lds(k_simp_push<7>, big));
+    chk(allow_lds(k_simp_solve1, big));
+    chk(allow_lds(k_simp_solve, big));
     chk(allow_lds(k_solve<true>, big));
     chk(allow_lds(k_solve<false>, big));
     chk(allow_lds(k_op_deposit<0>, big));

@@ -98,6 +98,14 @@ struct RunBuffers {
     // charge, so steps after the first carry no first-deposit launch
     int32_t prime_next;
     int64_t racc_cells;  // u64 words of the mixed accumulator
+    // Simpson-1/3 (FP64): E[0..1] hold E[3,:] (ping-pong by e_cur), Es1 is
+    // E[1,:], Es2[0..1] E[2,:]; racc collects rho(X, X), racc2 rho(X, x) and
+    // racc3 rho(x, x); Vg[p] = V + (gather(E[1,:], X)*dt)/6 of the step.
+    double* Es1;
+    double* Es2[2];
+    void* racc2;
+    void* racc3;
+    double* Vg;
 };
 
 // kernels_1d.hip
@@ -106,6 +114,12 @@ void launch_mid_push0(const RunBuffers& b, hipStream_t s);
 void launch_mid_push(const RunBuffers& b, int k, bool last_slot, hipStream_t s);
 // stall: an instance still unconverged after this solve halts the graph
 void launch_solve(const RunBuffers& b, int k, bool stall, hipStream_t s);
+// Simpson-1/3: the step's prologue (rho(X,X), E[1,:], Vg and the charges of
+// sweep 0), then per sweep slot a solve (both fields and the test) and a push
+int simpson_push_lds_bytes(int N);
+void launch_simp_first(const RunBuffers& b, hipStream_t s);
+void launch_simp_solve(const RunBuffers& b, int k, bool stall, hipStream_t s);
+void launch_simp_push(const RunBuffers& b, int k, bool last_slot, hipStream_t s);
 void launch_lf_first(const RunBuffers& b, hipStream_t s);
 void launch_lf_second(const RunBuffers& b, hipStream_t s);
 void launch_diag(const RunBuffers& b, hipStream_t s);

@@ -18,8 +18,8 @@ PIC_ENOMEM = -3
 PIC_ENONFINITE = -4
 PIC_ESTATE = -5
 
-SHAPE_NGP, SHAPE_GAUSS6, SHAPE_GAUSS7 = 0, 6, 7
-LEAPFROG, MIDPOINT = 0, 1
+SHAPE_NGP, SHAPE_AREA, SHAPE_GAUSS6, SHAPE_GAUSS7 = 0, 1, 6, 7
+LEAPFROG, MIDPOINT, SIMPSON = 0, 1, 2
 FP64, MIXED = 0, 1
 
 # every symbol include/pic.h declares

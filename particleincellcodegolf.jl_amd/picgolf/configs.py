@@ -8,6 +8,8 @@ from a seed) so the GPU run and the CPU restatement start from identical bits.
   quiet_two_stream   N=64, P=2048, Gaussian +-7 midpoint, dt=1/6N, 8192 steps,
                      W=32pi^2/3, rtol=4eps, bit-reversal quiet start
                      (src/GaussianFixedPointQuiet.jl:1-6)
+  quiet_simpson      the same with the Simpson-1/3 fixed point, Gaussian or
+                     area shape (src/*FixedPointQuietSimpson13.jl:1-5)
   scaled_two_stream  N=4096, P=2^24, Gaussian +-7 midpoint, dt=1/6N, 2000 steps,
                      W=400, rtol=1e-8 (src/GaussianFixedPoint.jl:4), seeded
                      uniform x, v=-1/+1 by half plus N(0, 0.05^2)
@@ -59,6 +61,21 @@ def quiet_two_stream(precision="fp64"):
     W = 32 * np.pi ** 2 / 3
     cfg = Config(N=N, P=P, W=W, w=W / P * N, dt=1 / (6 * N), shape="gauss7", scheme="midpoint",
                  precision=precision, rtol=4 * EPS, max_sweeps=10)
+    x, v = quiet_start(P)
+    return cfg, x, v, 2 ** 13
+
+
+def quiet_simpson(shape="gauss7"):
+    """The Simpson-1/3 fixed-point scripts: N=64, P=2048, dt=1/6N, 8192 steps,
+    W=32pi^2/3, bit-reversal quiet start; the Gaussian +-7 shape with
+    rtol=4eps (src/GaussianFixedPointQuietSimpson13.jl:1-5) or the linear area
+    weights with rtol=1e-14 (src/AreaFixedPointQuietSimpson13.jl:1-5)."""
+    N = 64
+    P = 32 * N
+    W = 32 * np.pi ** 2 / 3
+    rtol = 1e-14 if shape == "area" else 4 * EPS
+    cfg = Config(N=N, P=P, W=W, w=W / P * N, dt=1 / (6 * N), shape=shape, scheme="simpson",
+                 precision="fp64", rtol=rtol, max_sweeps=10)
     x, v = quiet_start(P)
     return cfg, x, v, 2 ** 13
 

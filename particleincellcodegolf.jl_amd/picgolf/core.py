@@ -24,8 +24,8 @@ import numpy as np
 
 from . import _lib as L
 
-SHAPES = {"ngp": L.SHAPE_NGP, "gauss6": L.SHAPE_GAUSS6, "gauss7": L.SHAPE_GAUSS7}
-SCHEMES = {"leapfrog": L.LEAPFROG, "midpoint": L.MIDPOINT}
+SHAPES = {"ngp": L.SHAPE_NGP, "area": L.SHAPE_AREA, "gauss6": L.SHAPE_GAUSS6, "gauss7": L.SHAPE_GAUSS7}
+SCHEMES = {"leapfrog": L.LEAPFROG, "midpoint": L.MIDPOINT, "simpson": L.SIMPSON}
 PRECISIONS = {"fp64": L.FP64, "mixed": L.MIXED}
 
 

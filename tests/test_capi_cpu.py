@@ -48,6 +48,23 @@ def test_create_rejects_bad_config_with_message(field, value, msg):
     assert e.value.code == -1 and msg in str(e.value)
 
 
+@pytest.mark.parametrize("kw,msg", [
+    (dict(scheme="simpson", shape="ngp"), "Simpson scheme needs"),
+    (dict(scheme="simpson", N=2048), "N <= 1024"),
+    (dict(scheme="simpson", precision="mixed"), "mixed precision"),
+    (dict(scheme="midpoint", shape="area"), "Gaussian"),
+    (dict(scheme="leapfrog", shape="area"), "Simpson"),
+])
+def test_simpson_and_area_combinations_are_checked(kw, msg):
+    import picgolf
+    from picgolf.core import Config
+    base = dict(N=64, P=64, W=100.0, dt=1e-3, shape="gauss7", scheme="simpson")
+    base.update(kw)
+    with pytest.raises(picgolf.PicError) as e:
+        picgolf.Context(Config(**base))
+    assert e.value.code == -1 and msg in str(e.value)
+
+
 def test_midpoint_needs_gaussian_shape():
     import picgolf
     from picgolf.core import Config
