@@ -134,7 +134,8 @@ int pic_autotune(pic_ctx* ctx, int32_t trial_steps, pic_tuning* out);
 /* ---- operators (FP64, bitwise equal to the CPU restatement) ---------- */
 /* rho = r(x, y): r .*= 0; for l in d.((x.+y)./2); for k in l; r[k[1]] += k[2]*w
  * (src/GaussianFixedPointQuiet.jl:7; y = NULL means y = x); NGP:
- * n[f(j)] += w (src/NGPFourier.jl:5).  rho has the context's N cells. */
+ * n[f(j)] += w (src/NGPFourier.jl:5); PIC_SHAPE_AREA: the linear weights of
+ * src/AreaFixedPointQuietSimpson13.jl:5-6.  rho has the context's N cells. */
 int pic_deposit(pic_ctx* ctx, const double* x, const double* y, int64_t P,
                 double w, int32_t shape, double* rho);
 /* E = real(ifft((xi = fft(rho) ./ ik; xi[1] *= 0; xi)))
@@ -154,6 +155,12 @@ int pic_midpoint_position(pic_ctx* ctx, const double* X, const double* V,
 int pic_push_midpoint(pic_ctx* ctx, const double* X, const double* V,
                       const double* x, double* v, const double* E, int64_t P,
                       double dt, int32_t shape);
+/* v_out[j] = v_in[j] + sum(k->E[k[1]]*k[2], d((a[j]+b[j])/2))*coef/6: a stage
+ * kick of the Simpson-1/3 sweep (src/GaussianFixedPointQuietSimpson13.jl:12,14,16,
+ * src/AreaFixedPointQuietSimpson13.jl:11,13,15) with (a, b, coef) = (X, X, dt),
+ * (X, x, 4dt) and (x, x, dt).  shape: PIC_SHAPE_AREA, GAUSS6 or GAUSS7. */
+int pic_simpson_kick(pic_ctx* ctx, const double* a, const double* b, const double* v_in, double* v_out,
+                     const double* E, int64_t P, double coef, int32_t shape);
 /* x .= mod.(x, 1)   (src/GaussianFixedPointQuiet.jl:10) */
 int pic_wrap(pic_ctx* ctx, double* x, int64_t P);
 /* isapprox(F, E, rtol=rtol, atol=atol)   (src/GaussianFixedPointQuiet.jl:8) */

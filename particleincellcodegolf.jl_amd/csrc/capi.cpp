@@ -1031,8 +1031,9 @@ static int d2h(pic_ctx* c, double* dst, const DevBuf& b, int64_t n) {
     return PIC_OK;
 }
 
-static int check_shape(pic_ctx* c, int32_t shape, bool allow_ngp) {
-    if (shape == PIC_SHAPE_GAUSS6 || shape == PIC_SHAPE_GAUSS7 || (allow_ngp && shape == PIC_SHAPE_NGP))
+static int check_shape(pic_ctx* c, int32_t shape, bool allow_ngp, bool allow_area = false) {
+    if (shape == PIC_SHAPE_GAUSS6 || shape == PIC_SHAPE_GAUSS7 || (allow_ngp && shape == PIC_SHAPE_NGP) ||
+        (allow_area && shape == PIC_SHAPE_AREA))
         return PIC_OK;
     set_err(c, "unsupported shape %d", shape);
     return PIC_EINVAL;
@@ -1045,7 +1046,7 @@ int pic_deposit(pic_ctx* c, const double* x, const double* y, int64_t P, double 
         set_err(c, "pic_deposit: null array, P < 1 or bad w");
         return PIC_EINVAL;
     }
-    int rc = check_shape(c, shape, true);
+    int rc = check_shape(c, shape, true, true);
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
     const int N = c->cfg[0].N;
@@ -1148,6 +1149,28 @@ int pic_push_midpoint(pic_ctx* c, const double* X, const double* V, const double
     launch_op_push_midpoint(o, (const double*)c->op_a.p, (const double*)c->op_b.p, (const double*)c->op_d.p,
                             (double*)c->op_e.p, (const double*)c->op_c.p, P, dt, shape, c->stream);
     return d2h(c, v, c->op_e, P);
+}
+
+int pic_simpson_kick(pic_ctx* c, const double* a, const double* b, const double* v_in, double* v_out,
+                     const double* E, int64_t P, double coef, int32_t shape) {
+    if (!c) return PIC_EINVAL;
+    if (!a || !b || !v_in || !v_out || !E || P < 1) {
+        set_err(c, "pic_simpson_kick: null array or P < 1");
+        return PIC_EINVAL;
+    }
+    int rc = check_shape(c, shape, false, true);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int N = c->cfg[0].N;
+    if ((rc = h2d(c, c->op_a, a, P))) return rc;
+    if ((rc = h2d(c, c->op_b, b, P))) return rc;
+    if ((rc = h2d(c, c->op_d, v_in, P))) return rc;
+    if ((rc = h2d(c, c->op_e, nullptr, P))) return rc;
+    if ((rc = h2d(c, c->op_c, E, N))) return rc;
+    OpBuffers o = op_view(c);
+    launch_op_stage_kick(o, (const double*)c->op_a.p, (const double*)c->op_b.p, (const double*)c->op_d.p,
+                         (double*)c->op_e.p, (const double*)c->op_c.p, P, coef, shape, c->stream);
+    return d2h(c, v_out, c->op_e, P);
 }
 
 int pic_wrap(pic_ctx* c, double* x, int64_t P) {

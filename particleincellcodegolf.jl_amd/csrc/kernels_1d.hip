@@ -1244,7 +1244,7 @@ k_op_deposit(const double* __restrict__ x, const double* __restrict__ y, int64_t
             atomicAdd(&lds[ngp_cell(xj, Nd, nmask)], 1ull);
         } else {
             const double c = y ? (xj + y[j]) / 2.0 : (xj + xj) / 2.0;
-            deposit_gauss<R>(lds, N, c, Nd, nmask, w, q);
+            deposit_shape<R>(lds, N, c, Nd, nmask, w, q);
         }
     }
     __syncthreads();
@@ -1328,6 +1328,22 @@ k_op_push_midpoint(const double* __restrict__ X, const double* __restrict__ V,
     v[j] = V[j] + Sg * dt;
 }
 
+// v_out[j] = v_in[j] + sum(k->E[k[1]]*k[2], d((a[j]+b[j])/2))*coef/6: one
+// stage kick of the Simpson-1/3 sweep (src/GaussianFixedPointQuietSimpson13.jl:12,14,16
+// with (a, b, coef) = (X, X, dt), (X, x, 4dt), (x, x, dt)).
+template <int SH>
+__global__ void __launch_bounds__(256)
+k_op_stage_kick(const double* __restrict__ a, const double* __restrict__ b, const double* __restrict__ vin,
+                double* __restrict__ vout, const double* __restrict__ E, int64_t P, double coef, int N) {
+    extern __shared__ double Eld[];
+    for (int i = threadIdx.x; i < N; i += blockDim.x) Eld[i] = E[i];
+    __syncthreads();
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P) return;
+    const double g = gather_shape<SH>(Eld, (a[j] + b[j]) / 2.0, (double)N, N - 1);
+    vout[j] = vin[j] + (g * coef) / 6.0;
+}
+
 __global__ void k_op_wrap(double* __restrict__ x, int64_t P) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j < P) x[j] = jl_mod1f(x[j]);
@@ -1382,6 +1398,7 @@ void launch_op_deposit(const OpBuffers& o, const double* x, const double* y, int
     const int lds = (R == 0 ? 1 : 3) * o.N * 8;
     u128* part = (u128*)o.part;
     if (R == 0) k_op_deposit<0><<<nt, 256, lds, s>>>(x, y, P, w, q, o.N, part);
+    else if (R == 1) k_op_deposit<1><<<nt, 256, lds, s>>>(x, y, P, w, q, o.N, part);
     else if (R == 6) k_op_deposit<6><<<nt, 256, lds, s>>>(x, y, P, w, q, o.N, part);
     else k_op_deposit<7><<<nt, 256, lds, s>>>(x, y, P, w, q, o.N, part);
     k_op_reduce<<<(o.N + 255) / 256, 256, 0, s>>>(part, nt, o.N, q, rho);
@@ -1413,6 +1430,14 @@ void launch_op_push_midpoint(const OpBuffers& o, const double* X, const double* 
     const int lds = o.N * 8;
     if (R == 6) k_op_push_midpoint<6><<<blocks_for(P, 256), 256, lds, s>>>(X, V, x, v, E, P, dt, o.N);
     else k_op_push_midpoint<7><<<blocks_for(P, 256), 256, lds, s>>>(X, V, x, v, E, P, dt, o.N);
+}
+
+void launch_op_stage_kick(const OpBuffers& o, const double* a, const double* b, const double* vin, double* vout,
+                          const double* E, int64_t P, double coef, int shape, hipStream_t s) {
+    const int lds = o.N * 8, g = blocks_for(P, 256);
+    if (shape == 1) k_op_stage_kick<1><<<g, 256, lds, s>>>(a, b, vin, vout, E, P, coef, o.N);
+    else if (shape == 6) k_op_stage_kick<6><<<g, 256, lds, s>>>(a, b, vin, vout, E, P, coef, o.N);
+    else k_op_stage_kick<7><<<g, 256, lds, s>>>(a, b, vin, vout, E, P, coef, o.N);
 }
 
 void launch_op_wrap(double* x, int64_t P, hipStream_t s) {
@@ -1470,7 +1495,11 @@ int configure_kernels() {
     chk(allow_lds(k_solve<true>, big));
     chk(allow_lds(k_solve<false>, big));
     chk(allow_lds(k_op_deposit<0>, big));
+    chk(allow_lds(k_op_deposit<1>, big));
     chk(allow_lds(k_op_deposit<6>, big));
+    chk(allow_lds(k_op_stage_kick<1>, big));
+    chk(allow_lds(k_op_stage_kick<6>, big));
+    chk(allow_lds(k_op_stage_kick<7>, big));
     chk(allow_lds(k_op_deposit<7>, big));
     chk(allow_lds(k_op_fieldsolve, big));
     (void)hipGetLastError();

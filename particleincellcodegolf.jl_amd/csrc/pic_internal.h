@@ -152,6 +152,8 @@ void launch_op_midpos(const double* X, const double* V, const double* v, double*
 void launch_op_push_midpoint(const OpBuffers& o, const double* X, const double* V,
                              const double* x, double* v, const double* E, int64_t P,
                              double dt, int R, hipStream_t s);
+void launch_op_stage_kick(const OpBuffers& o, const double* a, const double* b, const double* vin, double* vout,
+                          const double* E, int64_t P, double coef, int shape, hipStream_t s);
 void launch_op_wrap(double* x, int64_t P, hipStream_t s);
 void launch_op_isapprox(const double* F, const double* E, int32_t n, double rtol,
                         double atol, int32_t* out, hipStream_t s);

@@ -20,6 +20,12 @@ The scripts' closures map onto these functions:
 | `isapprox(F, E, rtol=l)` (src/GaussianFixedPointQuiet.jl:8) | `isapprox(ctx, F, E; rtol)` |
 | `D[t,1:4]` (src/GaussianFixedPointQuiet.jl:11,13)    | `diagnostics(ctx, E, v)`         |
 | the whole `@gif for t ∈ 1:T` loop                    | `upload!`, `run!`, `download`    |
+| `v[j]=V[j]+sum(...)*dt/6` etc. (src/GaussianFixedPointQuietSimpson13.jl:12,14,16) | `simpson_kick!(v, ctx, a, b, v_in, E, coef)` |
+
+The Simpson-1/3 scripts (src/GaussianFixedPointQuietSimpson13.jl,
+src/AreaFixedPointQuietSimpson13.jl) run whole with
+`PicConfig(...; scheme=:simpson, shape=:gauss7 or :area)`; `download` returns
+their `E[end,:]`.
 
 Every call copies its arrays to the device and back and is synchronous; a
 nonzero status from the library becomes an `error` carrying the library's
@@ -31,12 +37,13 @@ module PICGolf
 
 export Context, PicConfig, deposit!, fieldsolve!, drift!, midpoint_position!, wrap!,
        diagnostics, upload!, download, run!, quiet_start, autotune!, set_tiling!, tiling, PicTuning,
+       simpson_kick!,
        Pic2DConfig, Context2D, gather, boris!
 
 const libpicgolf = get(ENV, "PICGOLF_LIB", normpath(joinpath(@__DIR__, "..", "lib", "libpicgolf.so")))
 
-const SHAPES = Dict(:ngp => Int32(0), :gauss6 => Int32(6), :gauss7 => Int32(7))
-const SCHEMES = Dict(:leapfrog => Int32(0), :midpoint => Int32(1))
+const SHAPES = Dict(:ngp => Int32(0), :area => Int32(1), :gauss6 => Int32(6), :gauss7 => Int32(7))
+const SCHEMES = Dict(:leapfrog => Int32(0), :midpoint => Int32(1), :simpson => Int32(2))
 const PRECISIONS = Dict(:fp64 => Int32(0), :mixed => Int32(1))
 
 "Mirror of `struct pic_config` in include/pic.h."
@@ -180,6 +187,27 @@ function Base.push!(ctx::Context, X::Vector{Float64}, V::Vector{Float64}, x::Vec
                                       ctx.ptr, X, V, x, v, E, length(X), dt, s)
     check(ctx, rc, "pic_push_midpoint")
     v
+end
+
+"""
+    simpson_kick!(v_out, ctx, a, b, v_in, E, coef; shape)
+
+One stage kick of the Simpson-1/3 sweep:
+`v_out[j] = v_in[j] + sum(k->E[k[1]]*k[2], d((a[j]+b[j])/2))*coef/6`
+(src/GaussianFixedPointQuietSimpson13.jl:12,14,16 with `(a, b, coef)` =
+`(X, X, dt)`, `(X, x, 4dt)`, `(x, x, dt)`; src/AreaFixedPointQuietSimpson13.jl:11,13,15
+with `shape=:area`).
+"""
+function simpson_kick!(v_out::Vector{Float64}, ctx::Context, a::Vector{Float64}, b::Vector{Float64},
+                       v_in::Vector{Float64}, E::Vector{Float64}, coef::Real;
+                       shape::Union{Symbol,Integer}=Int(ctx.cfgs[1].shape))
+    s = shape isa Symbol ? SHAPES[shape] : Int32(shape)
+    rc = GC.@preserve v_out a b v_in E ccall((:pic_simpson_kick, libpicgolf), Cint,
+                                             (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                                              Ptr{Float64}, Int64, Float64, Int32),
+                                             ctx.ptr, a, b, v_in, v_out, E, length(a), coef, s)
+    check(ctx, rc, "pic_simpson_kick")
+    v_out
 end
 
 "`x .= X .+ (v .+ V)/2*dt` (src/GaussianFixedPointQuiet.jl:9)."

@@ -27,7 +27,7 @@ EXPORTS = [
     "pic_create", "pic_create_batch", "pic_destroy", "pic_last_error", "pic_strerror",
     "pic_version", "pic_upload", "pic_download", "pic_run", "pic_run_profiled",
     "pic_deposit", "pic_fieldsolve", "pic_drift_half", "pic_push_leapfrog",
-    "pic_midpoint_position", "pic_push_midpoint", "pic_wrap", "pic_isapprox",
+    "pic_midpoint_position", "pic_push_midpoint", "pic_simpson_kick", "pic_wrap", "pic_isapprox",
     "pic_diagnostics", "pic_set_tiling", "pic_get_tiling", "pic_autotune",
 ]
 
@@ -108,6 +108,7 @@ def load():
         "pic_push_leapfrog": [vp, vp, vp, vp, i64, d, i32],
         "pic_midpoint_position": [vp, vp, vp, vp, vp, i64, d],
         "pic_push_midpoint": [vp, vp, vp, vp, vp, vp, i64, d, i32],
+        "pic_simpson_kick": [vp, vp, vp, vp, vp, vp, i64, d, i32],
         "pic_wrap": [vp, vp, i64],
         "pic_isapprox": [vp, vp, vp, i32, d, d, ip],
         "pic_diagnostics": [vp, vp, vp, i64, d, vp],

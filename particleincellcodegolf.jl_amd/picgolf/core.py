@@ -160,6 +160,17 @@ class Context:
         v[...] = vv
         return v
 
+    def simpson_kick(self, a, b, v_in, E, coef, shape=None):
+        """v_in + sum(k->E[k[1]]*k[2], d((a+b)/2))*coef/6, one stage kick of the
+        Simpson-1/3 sweep (src/GaussianFixedPointQuietSimpson13.jl:12,14,16 with
+        (a, b, coef) = (X, X, dt), (X, x, 4dt), (x, x, dt)); returns v_out."""
+        a, b, v_in, E = L.f64(a), L.f64(b), L.f64(v_in), L.f64(E)
+        out = np.zeros_like(a)
+        shape = _code(self.configs[0].shape if shape is None else shape, SHAPES, "shape")
+        self._check(self._lib.pic_simpson_kick(self._h, L.ptr(a), L.ptr(b), L.ptr(v_in), L.ptr(out), L.ptr(E),
+                                               a.size, float(coef), shape), "pic_simpson_kick")
+        return out
+
     def wrap(self, x):
         xx = L.f64(x).copy()
         self._check(self._lib.pic_wrap(self._h, L.ptr(xx), xx.size), "pic_wrap")

@@ -103,3 +103,25 @@ def test_quiet_simpson_growth_rate_on_device(pg, shape):
           f"mean sweeps {S.mean():.2f}")
     assert slope == pytest.approx(2 * gamma_decades(cfg.W), rel=0.01)
     assert np.max(np.abs(D[:, 3])) < 1e-14
+
+
+@pytest.mark.parametrize("shape", ["area", "gauss7", "gauss6"])
+def test_simpson_operators_bitwise(pg, orc, shape):
+    """The script's closures one at a time: rho(a, b) with the area or Gaussian
+    shape and the three stage kicks of a sweep."""
+    from picgolf.core import Config
+    code = SHAPE_CODE[shape]
+    N, P = 64, 5000
+    rng = np.random.default_rng(code)
+    X = rng.random(P)
+    x = X + 0.02 * rng.standard_normal(P)          # unwrapped, as inside the sweeps
+    V, v = rng.standard_normal(P), rng.standard_normal(P)
+    E = rng.standard_normal(N)
+    w, dt = 0.37, 1 / 384
+    with pg.Context(Config(N=N, P=P, W=100.0, dt=dt, shape=shape, scheme="simpson")) as c:
+        for a, b in ((X, X), (X, x), (x, x)):
+            assert same(c.deposit(a, b, w=w, shape=shape), orc.deposit(N, code, a, b, w=w))
+        for a, b, vin, coef in ((X, X, V, dt), (X, x, v, 4 * dt), (x, x, v, dt)):
+            got = c.simpson_kick(a, b, vin, E, coef)
+            ref = np.array([vin[j] + (orc.gather_at(N, code, E, (a[j] + b[j]) / 2) * coef) / 6 for j in range(P)])
+            assert same(got, ref)
