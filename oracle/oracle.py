@@ -65,6 +65,10 @@ def lib():
         "orc_deposit_par": (None, [c_i32, c_i32, c_i64, _dp, c_vp, c_d, _dp, c_i32]),
         "orc_run_simpson": (ctypes.c_int, [c_i32, c_i32, c_i64, c_d, c_d, c_d, c_d, c_i32, _dp, _dp, _dp,
                                            c_i64, c_vp, c_vp, c_i32]),
+        "orc_boris12": (None, [ctypes.POINTER(c_d), ctypes.POINTER(c_d), c_d, c_d, c_d, c_d]),
+        "orc_deposit_1d2v": (None, [c_i32, c_i32, c_i32, c_i64, _dp, c_d, _dp, _dp]),
+        "orc_run_1d2v": (ctypes.c_int, [c_i32, c_i32, c_i32, c_i64, _dp, _dp, _dp, c_d, c_d, c_d, c_d, c_i32,
+                                        _dp, _dp, _dp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i32]),
         "orc2d_unimod": (c_d, [c_d, c_d]),
         "orc2d_unimodi": (c_i64, [c_i64, c_i64]),
         "orc2d_area_weights": (None, [c_d, c_i32, c_i32, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32),
@@ -281,6 +285,48 @@ def run_simpson(N, shape, W, w, dt, x, v, nsteps, rtol, max_sweeps=10, E3=None, 
     if rc != 0:
         raise ValueError(f"orc_run_simpson rejected its arguments (rc={rc})")
     return x, v, Ecm.reshape(N, 3).T.copy(), diag, sweeps
+
+
+# ---- magnetised 1D2V (src/NGP1D2V.jl, src/NGP1D2V2S.jl) ----------------------
+
+def boris12(vx, vy, E, B, dt, q_m=1.0):
+    a, b = ctypes.c_double(vx), ctypes.c_double(vy)
+    lib().orc_boris12(ctypes.byref(a), ctypes.byref(b), float(E), float(B), float(dt), float(q_m))
+    return a.value, b.value
+
+
+def deposit_1d2v(N, R, q, w, xs):
+    """rho() over the species position arrays xs (charges q), exact."""
+    ns = len(xs)
+    x = np.concatenate([_f64(a) for a in xs])
+    rho = np.zeros(N)
+    lib().orc_deposit_1d2v(int(N), int(R), ns, x.size // ns, _f64(np.asarray(q, dtype=np.float64)), float(w),
+                           x, rho)
+    return rho
+
+
+def run_1d2v(N, R, q, q_m, mass, w, n0, B0, dt, bin, xs, vxs, vys, nsteps, t0=0, nbins=None, Es=None, D=None,
+             nthreads=1):
+    """nsteps of the 1D2V loop for the species arrays xs, vxs, vys (lists of
+    equal length P).  Returns (xs, vxs, vys, Es (nbins, N), D (nbins, 5)), Es
+    and D before the scripts' division by bin."""
+    ns = len(xs)
+    P = _f64(xs[0]).size
+    x = np.concatenate([_f64(a) for a in xs])
+    vx = np.concatenate([_f64(a) for a in vxs])
+    vy = np.concatenate([_f64(a) for a in vys])
+    if nbins is None:
+        nbins = (t0 + nsteps + bin - 1) // bin
+    Es = np.zeros((nbins, N)) if Es is None else np.ascontiguousarray(_f64(Es)).copy()
+    D = np.zeros((nbins, 5)) if D is None else np.ascontiguousarray(_f64(D)).copy()
+    arr = lambda a: _f64(np.asarray(a, dtype=np.float64))
+    rc = lib().orc_run_1d2v(int(N), int(R), ns, P, arr(q), arr(q_m), arr(mass), float(w), float(n0), float(B0),
+                            float(dt), int(bin), x, vx, vy, int(t0), int(nsteps), _ptr(Es), int(nbins), _ptr(D),
+                            int(nthreads))
+    if rc != 0:
+        raise ValueError(f"orc_run_1d2v rejected its arguments (rc={rc})")
+    split = lambda a: [a[s * P:(s + 1) * P].copy() for s in range(ns)]
+    return split(x), split(vx), split(vy), Es, D
 
 
 # ---- electrostatic 2D3V (src/Electrostatic2D3V.jl) -------------------------

@@ -102,6 +102,21 @@ int orc_run_simpson(int32_t N, int32_t shape, int64_t P, double W, double w, dou
                     double rtol, int32_t max_sweeps, double* x, double* v, double* E3,
                     int64_t nsteps, double* diag, int32_t* sweeps, int32_t nthreads);
 
+/* magnetised 1D2V, one or two species (src/NGP1D2V.jl, src/NGP1D2V2S.jl),
+ * pic1d2v.c.  Species s particle j is at s*P + j.  q, q_m, mass: per species
+ * charge, charge-to-mass ratio and kinetic weight in the diagnostics
+ * (1D2V: 1, 1, 1; 1D2V2S: (-1, 1), (-1, 1/M), (1, M)).  t0: steps already run;
+ * Es (N x nbins, N fastest) accumulates E into bin ceil(t/bin); D (nbins x 5)
+ * receives the script's row at every t with t % bin == 0 (both before the
+ * scripts' final division by bin). */
+void orc_boris12(double* vx, double* vy, double E, double B, double dt, double q_m);
+void orc_deposit_1d2v(int32_t N, int32_t R, int32_t nspecies, int64_t P, const double* q, double w,
+                      const double* x, double* rho);
+int  orc_run_1d2v(int32_t N, int32_t R, int32_t nspecies, int64_t P, const double* q, const double* q_m,
+                  const double* mass, double w, double n0, double B0, double dt, int32_t bin,
+                  double* x, double* vx, double* vy, int64_t t0, int64_t nsteps, double* Es,
+                  int64_t nbins, double* D, int32_t nthreads);
+
 /* electrostatic 2D3V (src/Electrostatic2D3V.jl), pic2d.c.  Grids are
  * column-major NX x NY (x fastest); cells are returned 0-based.  textbook = 0
  * is the reference's i / i+1 area pairing, 1 the i-1 / i variant. */
