@@ -39,6 +39,12 @@ EXPORTS_2D = [
 ]
 PAIR_REFERENCE, PAIR_TEXTBOOK = 0, 1
 
+# every symbol include/pic1d2v.h declares
+EXPORTS_1D2V = [
+    "pic1d2v_create", "pic1d2v_destroy", "pic1d2v_last_error", "pic1d2v_upload", "pic1d2v_download",
+    "pic1d2v_run", "pic1d2v_fields", "pic1d2v_deposit", "pic1d2v_gather", "pic1d2v_boris",
+]
+
 
 class LibraryMissing(RuntimeError):
     pass
@@ -65,6 +71,15 @@ class pic2d_config(ctypes.Structure):
         ("precision", ctypes.c_int32), ("pairing", ctypes.c_int32), ("dt", ctypes.c_double),
         ("w", ctypes.c_double), ("B0", ctypes.c_double), ("ns", ctypes.c_int32), ("device", ctypes.c_int32),
         ("sort_every", ctypes.c_int32),
+    ]
+
+
+class pic1d2v_config(ctypes.Structure):
+    _fields_ = [
+        ("N", ctypes.c_int32), ("shape", ctypes.c_int32), ("nspecies", ctypes.c_int32), ("bin", ctypes.c_int32),
+        ("P", ctypes.c_int64), ("q", ctypes.c_double * 2), ("q_m", ctypes.c_double * 2),
+        ("mass", ctypes.c_double * 2), ("w", ctypes.c_double), ("n0", ctypes.c_double), ("B0", ctypes.c_double),
+        ("dt", ctypes.c_double), ("device", ctypes.c_int32),
     ]
 
 
@@ -129,6 +144,17 @@ def load():
         "pic2d_push": [vp, vp, vp, vp, vp, vp, i64, vp, vp, d, d],
         "pic2d_diagnostics": [vp, vp, vp, vp, vp, i64, d, vp],
     })
+    sig.update({
+        "pic1d2v_create": [ctypes.POINTER(vp), ctypes.POINTER(pic1d2v_config)],
+        "pic1d2v_destroy": [vp],
+        "pic1d2v_upload": [vp, i32, vp, vp, vp],
+        "pic1d2v_download": [vp, i32, vp, vp, vp],
+        "pic1d2v_run": [vp, i64, vp],
+        "pic1d2v_fields": [vp, vp, i64],
+        "pic1d2v_deposit": [vp, vp, vp],
+        "pic1d2v_gather": [vp, vp, vp, i64, vp],
+        "pic1d2v_boris": [vp, vp, vp, vp, i64, d, d, d],
+    })
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -137,6 +163,8 @@ def load():
     L.pic_last_error.restype = ctypes.c_char_p
     L.pic2d_last_error.argtypes = [vp]
     L.pic2d_last_error.restype = ctypes.c_char_p
+    L.pic1d2v_last_error.argtypes = [vp]
+    L.pic1d2v_last_error.restype = ctypes.c_char_p
     L.pic_strerror.argtypes = [ctypes.c_int]
     L.pic_strerror.restype = ctypes.c_char_p
     L.pic_version.argtypes = []

@@ -15,6 +15,9 @@ from a seed) so the GPU run and the CPU restatement start from identical bits.
                      uniform x, v=-1/+1 by half plus N(0, 0.05^2)
   ensemble           512 instances: N in {32..1024}, 16-256 particles per cell,
                      W in [100, 400], beams +-u with u in [0.5, 1.5]
+  bernstein          magnetised 1D2V, one species (N=512, P=15N) or two
+                     (N=256, P=8N each, M=8), Boris push with the Gaussian
+                     +-7 stencil (src/NGP1D2V.jl, src/NGP1D2V2S.jl)
   es2d3v             256 x 256 grid, 64 particles per cell, electrostatic 2D3V
                      with B = B0 x (src/Electrostatic2D3V.jl): seeded uniform
                      positions in (0, 1], vx = -1/+1 by half + N(0, 0.1^2),
@@ -78,6 +81,50 @@ def quiet_simpson(shape="gauss7"):
                  precision="fp64", rtol=rtol, max_sweeps=10)
     x, v = quiet_start(P)
     return cfg, x, v, 2 ** 13
+
+
+def bernstein(species=1, seed=0):
+    """The magnetised 1D2V scripts (Bernstein waves).  species=1:
+    src/NGP1D2V.jl:22-32 -- N=512, P=15N, T=2^14 in bins of T/TO = 16 steps,
+    n0 = 4pi^2, vth = sqrt(n0)/N/4, dt = 1/N/6vth (6vth binds first), B0 = sqrt(n0)/16,
+    w = n0/P.  species=2: src/NGP1D2V2S.jl:13-21 -- N=256, P=8N per species,
+    T=2^16 in bins of 32 steps, M=8, vth = sqrt(n0)/N/8, dt = 1/N/16vth,
+    B0 = sqrt(n0)/8, w = n0/2P, charges -1 and +1, q/m -1 and 1/M; positions
+    from the bit-reversal quiet start (the second species shifted by pi).
+    Velocities are vth*erfinv(U) as in the scripts (U uniform in [0, 1), so
+    every component is >= 0); the uniforms come from a seeded numpy stream
+    (species=1) or the scripts' additive recurrences (species=2)."""
+    from scipy.special import erfinv
+    from .core1d2v import Config1D2V
+    n0 = 4 * np.pi ** 2
+    if species == 1:
+        N = 512
+        P = 15 * N
+        T, TO = 2 ** 14, 2 ** 14 // 16
+        vth = np.sqrt(n0) / N / 4
+        cfg = Config1D2V(N=N, P=P, bin=T // TO, w=n0 / P, n0=n0, B0=np.sqrt(n0) / 16, dt=1 / N / (6 * vth))
+        rng = np.random.default_rng(seed)
+        x, rvx, rvy = rng.random(P), rng.random(P), rng.random(P)
+        return cfg, [(x, vth * erfinv(rvx), vth * erfinv(rvy))], T
+    N = 256
+    P = 8 * N
+    T, TO, M = 2 ** 16, 2 ** 16 // 32, 8
+    vth = np.sqrt(n0) / N / 8
+    cfg = Config1D2V(N=N, P=P, bin=T // TO, w=n0 / (2 * P), n0=n0, B0=np.sqrt(n0) / 8, dt=1 / N / (16 * vth),
+                     q=(-1.0, 1.0), q_m=(-1.0, 1 / M), mass=(1.0, float(M)))
+    xq, _ = quiet_start(P)
+    x1 = xq - np.floor(xq)
+    x2 = (xq + np.pi) - np.floor(xq + np.pi)
+    # R(alpha, P) = rem.(n*alpha, 1), n = 1..P, alpha = 1/root of x^(d+1) = x + 1
+    def golden(d):
+        r = 1.5
+        for _ in range(200):
+            r = r - (r ** (d + 1) - r - 1) / ((d + 1) * r ** d - 1)
+        return 1 / r
+    n = np.arange(1, P + 1)
+    vx1 = vth * erfinv(np.remainder(n * golden(1), 1.0))
+    vy1 = vth * erfinv(np.remainder(n * golden(2), 1.0))
+    return cfg, [(x1, vx1, vy1), (x2, vx1 / np.sqrt(M), vy1 / np.sqrt(M))], T
 
 
 def scaled_two_stream(precision="mixed", seed=0, P=2 ** 24, N=4096):

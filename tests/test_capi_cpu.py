@@ -132,3 +132,42 @@ def test_es2d3v_preset_follows_the_script():
     assert cfg.B0 == np.sqrt(n0) / 4 and cfg.w == n0 / cfg.P * 32 * 32
     assert np.all((x > 0) & (x <= 1)) and np.all((y > 0) & (y <= 1))
     assert abs(np.std(vx[: cfg.P // 2] + 1) - 0.1) < 0.01 and abs(np.std(vz) - 0.1) < 0.01
+
+
+def test_header1d2v_and_binding_agree():
+    import picgolf
+    assert sorted(picgolf.EXPORTS_1D2V) == header_functions("pic1d2v.h", "pic1d2v_")
+
+
+def test_library_exports_every_1d2v_symbol():
+    import picgolf
+    L = picgolf.load()
+    for name in header_functions("pic1d2v.h", "pic1d2v_"):
+        assert hasattr(L, name), name
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("N", 100, "power of two"),
+    ("shape", 1, "shape"),
+    ("bin", 0, "bin"),
+    ("P", 0, "P must"),
+    ("n0", 0.0, "n0"),
+    ("q", (1.0, 2.0, 3.0), "nspecies"),
+])
+def test_create1d2v_rejects_bad_config_with_message(field, value, msg):
+    import ctypes
+    import picgolf
+    from picgolf.core1d2v import Config1D2V
+    cfg = Config1D2V(N=64, P=640, bin=4, w=0.1, n0=39.0, B0=1.0, dt=0.01)
+    if field == "q":
+        c = cfg.to_c()
+        c.nspecies = 3
+        h = ctypes.c_void_p()
+        L = picgolf.load()
+        assert L.pic1d2v_create(ctypes.byref(h), ctypes.byref(c)) == -1
+        assert msg in L.pic1d2v_last_error(None).decode()
+        return
+    setattr(cfg, field, value)
+    with pytest.raises(picgolf.PicError) as e:
+        picgolf.Context1D2V(cfg)
+    assert e.value.code == -1 and msg in str(e.value)
