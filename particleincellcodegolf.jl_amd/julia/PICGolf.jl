@@ -37,7 +37,7 @@ module PICGolf
 
 export Context, PicConfig, deposit!, fieldsolve!, drift!, midpoint_position!, wrap!,
        diagnostics, upload!, download, run!, quiet_start, autotune!, set_tiling!, tiling, PicTuning,
-       simpson_kick!,
+       simpson_kick!, Pic1D2VConfig, Context1D2V, fields,
        Pic2DConfig, Context2D, gather, boris!
 
 const libpicgolf = get(ENV, "PICGOLF_LIB", normpath(joinpath(@__DIR__, "..", "lib", "libpicgolf.so")))
@@ -513,6 +513,158 @@ function run!(ctx::Context2D, nsteps::Integer)
     check(ctx, rc, "pic2d_run")
     ctx.steps += nsteps
     permutedims(K)
+end
+
+# ---------------------------------------------------------------------------
+# Magnetised 1D2V (src/NGP1D2V.jl, src/NGP1D2V2S.jl), C ABI in include/pic1d2v.h.
+#
+# | script expression                                      | PICGolf                                  |
+# |--------------------------------------------------------|------------------------------------------|
+# | `rho()` / `rho(x)` (NGP1D2V2S.jl:28-29, NGP1D2V.jl:38)  | `deposit(ctx, xs)`                        |
+# | `sum(k->E[k[1]]*k[2], d(x[j]))` (:35, :42)              | `gather(ctx, E, x)`                       |
+# | `boris(vx, vy, E, B, dt, q_m)` (:5-11)                   | `boris!(ctx, vx, vy, Ej; q_m)`            |
+# | the `for t in 1:T` loop with `Es` and `D` (:31-57)       | `upload!`, `run!`, `fields`, `download`   |
+# ---------------------------------------------------------------------------
+
+"Mirror of `struct pic1d2v_config` in include/pic1d2v.h."
+struct Pic1D2VConfig
+    N::Int32
+    shape::Int32
+    nspecies::Int32
+    bin::Int32
+    P::Int64
+    q::NTuple{2,Float64}
+    q_m::NTuple{2,Float64}
+    mass::NTuple{2,Float64}
+    w::Float64
+    n0::Float64
+    B0::Float64
+    dt::Float64
+    device::Int32
+end
+
+"""
+    Pic1D2VConfig(; N, P, bin, w, n0, B0, dt, q=(1.0,), q_m=(1.0,), mass=(1.0,), shape=:gauss7, device=0)
+
+One species (src/NGP1D2V.jl: `q = q_m = mass = (1.0,)`, `w = n0/P`) or two
+(src/NGP1D2V2S.jl: `q = (-1.0, 1.0)`, `q_m = (-1.0, 1/M)`, `mass = (1.0, M)`,
+`w = n0/2P`); `bin = T÷TO`.
+"""
+function Pic1D2VConfig(; N, P, bin, w, n0, B0, dt, q=(1.0,), q_m=(1.0,), mass=(1.0,), shape=:gauss7, device=0)
+    ns = length(q)
+    pad(t) = (Float64(t[1]), ns == 2 ? Float64(t[2]) : 0.0)
+    Pic1D2VConfig(Int32(N), SHAPES[shape], Int32(ns), Int32(bin), Int64(P), pad(q), pad(q_m), pad(mass),
+                  Float64(w), Float64(n0), Float64(B0), Float64(dt), Int32(device))
+end
+
+mutable struct Context1D2V
+    ptr::Ptr{Cvoid}
+    cfg::Pic1D2VConfig
+    steps::Int64
+end
+
+function check(ctx::Context1D2V, rc::Cint, what::AbstractString)
+    rc == 0 || error("$what: ", unsafe_string(ccall((:pic1d2v_last_error, libpicgolf), Cstring, (Ptr{Cvoid},), ctx.ptr)))
+    nothing
+end
+
+function Context1D2V(cfg::Pic1D2VConfig)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:pic1d2v_create, libpicgolf), Cint, (Ref{Ptr{Cvoid}}, Ref{Pic1D2VConfig}), h, cfg)
+    rc == 0 || error("pic1d2v_create: ", unsafe_string(ccall((:pic1d2v_last_error, libpicgolf), Cstring,
+                                                             (Ptr{Cvoid},), C_NULL)))
+    ctx = Context1D2V(h[], cfg, 0)
+    finalizer(destroy!, ctx)
+    ctx
+end
+
+function destroy!(ctx::Context1D2V)
+    if ctx.ptr != C_NULL
+        ccall((:pic1d2v_destroy, libpicgolf), Cint, (Ptr{Cvoid},), ctx.ptr)
+        ctx.ptr = C_NULL
+    end
+    nothing
+end
+
+"`upload!(ctx, species, x, vx, vy)` (species 1 or 2); restarts the step count and the bins."
+function upload!(ctx::Context1D2V, species::Integer, x::Vector{Float64}, vx::Vector{Float64}, vy::Vector{Float64})
+    rc = GC.@preserve x vx vy ccall((:pic1d2v_upload, libpicgolf), Cint,
+                                    (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                                    ctx.ptr, species - 1, x, vx, vy)
+    check(ctx, rc, "pic1d2v_upload")
+    ctx.steps = 0
+    ctx
+end
+
+"`download(ctx, species) -> (x, vx, vy)`."
+function download(ctx::Context1D2V, species::Integer)
+    P = ctx.cfg.P
+    x, vx, vy = zeros(P), zeros(P), zeros(P)
+    rc = GC.@preserve x vx vy ccall((:pic1d2v_download, libpicgolf), Cint,
+                                    (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                                    ctx.ptr, species - 1, x, vx, vy)
+    check(ctx, rc, "pic1d2v_download")
+    (x, vx, vy)
+end
+
+"""
+    run!(ctx::Context1D2V, nsteps) -> D
+
+Advance `nsteps` steps; `D` holds the rows `D[ti, 1:5]` of the bins completed
+in this call, divided by the bin length as the scripts' `D ./= T÷TO` does.
+"""
+function run!(ctx::Context1D2V, nsteps::Integer)
+    b = ctx.cfg.bin
+    rows = div(ctx.steps + nsteps, b) - div(ctx.steps, b)
+    D = zeros(5, rows)
+    rc = GC.@preserve D ccall((:pic1d2v_run, libpicgolf), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}),
+                              ctx.ptr, nsteps, rows > 0 ? pointer(D) : C_NULL)
+    check(ctx, rc, "pic1d2v_run")
+    ctx.steps += nsteps
+    permutedims(D) ./ b
+end
+
+"`fields(ctx) -> Es`: the scripts' `Es` (N x bins so far, `./= T÷TO`)."
+function fields(ctx::Context1D2V)
+    b = ctx.cfg.bin
+    nb = cld(ctx.steps, b)
+    Es = zeros(ctx.cfg.N, nb)
+    if nb > 0
+        rc = GC.@preserve Es ccall((:pic1d2v_fields, libpicgolf), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64),
+                                   ctx.ptr, Es, nb)
+        check(ctx, rc, "pic1d2v_fields")
+    end
+    Es ./ b
+end
+
+"`deposit(ctx, xs) -> r`: `rho()` over the species' positions (src/NGP1D2V2S.jl:28-29)."
+function deposit(ctx::Context1D2V, xs::Vector{Vector{Float64}})
+    x = reduce(vcat, xs)
+    r = zeros(ctx.cfg.N)
+    rc = GC.@preserve x r ccall((:pic1d2v_deposit, libpicgolf), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                                ctx.ptr, x, r)
+    check(ctx, rc, "pic1d2v_deposit")
+    r
+end
+
+"`gather(ctx, E, x) -> Ej`: `sum(k->E[k[1]]*k[2], d(x[j]))` for every j."
+function gather(ctx::Context1D2V, E::Vector{Float64}, x::Vector{Float64})
+    Ej = zeros(length(x))
+    rc = GC.@preserve E x Ej ccall((:pic1d2v_gather, libpicgolf), Cint,
+                                   (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}),
+                                   ctx.ptr, E, x, length(x), Ej)
+    check(ctx, rc, "pic1d2v_gather")
+    Ej
+end
+
+"`boris!(ctx, vx, vy, Ej; B, dt, q_m)`: `vx[j], vy[j] = boris(vx[j], vy[j], Ej[j], B, dt, q_m)` in place."
+function boris!(ctx::Context1D2V, vx::Vector{Float64}, vy::Vector{Float64}, Ej::Vector{Float64};
+                B::Real=ctx.cfg.B0, dt::Real=ctx.cfg.dt, q_m::Real=1.0)
+    rc = GC.@preserve vx vy Ej ccall((:pic1d2v_boris, libpicgolf), Cint,
+                                     (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Float64,
+                                      Float64, Float64), ctx.ptr, vx, vy, Ej, length(vx), B, dt, q_m)
+    check(ctx, rc, "pic1d2v_boris")
+    (vx, vy)
 end
 
 end # module
