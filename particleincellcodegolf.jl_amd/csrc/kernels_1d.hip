@@ -179,7 +179,7 @@ k_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     InstState S = st[blockIdx.x];
     if (TEST && sweep_over(S, k)) return;
     const int N = I.N, log2N = I.log2N;
-    const int NP = N + N / 16;
+    const int NP = solve_np(N);
     double* re = sm;
     double* im = sm + NP;
     double* Fl = sm + 2 * NP;
@@ -187,10 +187,9 @@ k_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     const int cur = TEST ? S.e_cur : 0;
     const int nxt = TEST ? 1 - cur : 0;
     const double* F = (cur ? E1 : E0) + I.goff;
-    SolveRegs R;
-    solve_prefetch(R, twp, N, log2N);
     constexpr int kPer = kMaxN / kSolveThreads;
-    // batched loads: all of this thread's cells are in flight together
+    // batched loads: all of this thread's cells are in flight together, ahead
+    // of the transform's table loads (which are only needed later)
     u64 a0[kPer], a1[kPer], a2[kPer];
     double fv[kPer];
 #pragma unroll
@@ -208,21 +207,13 @@ k_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
             if (TEST) fv[c] = F[j];
         }
     }
+    SolveRegs R;
+    solve_prefetch(R, twp, N, log2N);
 #pragma unroll
     for (int c = 0; c < kPer; ++c) {
         const int j = threadIdx.x + c * kSolveThreads;
         if (j < N) {
-            double r;
-            if (precision == 0) {
-                u64* a = (u64*)racc + 3 * (I.red0 + j);
-                a[0] = 0;
-                a[1] = 0;
-                a[2] = 0;
-                r = fxp_to(limbs_to_u128(a0[c], a1[c], a2[c]), I.q);
-            } else {
-                ((u64*)racc)[I.red0 + j] = 0;
-                r = (double)a0[c] * I.wq;
-            }
+            const double r = precision == 0 ? fxp_to(limbs_to_u128(a0[c], a1[c], a2[c]), I.q) : (double)a0[c] * I.wq;
             const int b = padi((int)brev((uint32_t)j, log2N));
             re[b] = r;
             im[b] = 0.0;
@@ -231,19 +222,29 @@ k_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     }
     __syncthreads();
     solve_lds(re, im, R, twp, N, log2N);
-    double* Eo = (nxt ? E1 : E0) + I.goff;
-    for (int j = threadIdx.x; j < N; j += blockDim.x) {
-        const double e = re[padi(j)];
-        Eo[j] = e;
-        E32[I.goff + j] = (float)e;
-    }
     bool conv = false;
     double emax;
     if (TEST) {
-        conv = block_isapprox(red, [&](int i) { return Fl[i]; }, [&](int i) { return re[padi(i)]; }, N, I.rtol,
+        conv = block_isapprox(red, [&](int i) { return Fl[i]; }, [&](int i) { return solve_at(re, i); }, N, I.rtol,
                               0.0, emax);
     } else {
-        emax = block_maxabs(red, N, [&](int64_t i) { return re[padi((int)i)]; });
+        emax = block_maxabs(red, N, [&](int64_t i) { return solve_at(re, (int)i); });
+    }
+    // global stores last: every barrier above waits for a thread's outstanding
+    // stores, and nothing in this kernel reads these buffers back
+    double* Eo = (nxt ? E1 : E0) + I.goff;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        const double e = solve_at(re, j);
+        Eo[j] = e;
+        E32[I.goff + j] = (float)e;
+        if (precision == 0) {
+            u64* a = (u64*)racc + 3 * (I.red0 + j);
+            a[0] = 0;
+            a[1] = 0;
+            a[2] = 0;
+        } else {
+            ((u64*)racc)[I.red0 + j] = 0;
+        }
     }
     if (threadIdx.x == 0) {
         if (!isfinite(emax)) S.err |= 1;
@@ -449,7 +450,7 @@ k_simp_solve1(const InstDesc* __restrict__ insts, InstState* __restrict__ st, u6
     extern __shared__ double sm[];
     if (halted(halt, false)) return;
     const InstDesc I = insts[blockIdx.x];
-    const int N = I.N, log2N = I.log2N, NP = N + N / 16;
+    const int N = I.N, log2N = I.log2N, NP = solve_np(N);
     double* re = sm;
     double* im = sm + NP;
     double* red = sm + 2 * NP + N;
@@ -458,8 +459,8 @@ k_simp_solve1(const InstDesc* __restrict__ insts, InstState* __restrict__ st, u6
     take_charge64(racc + 3 * I.red0, re, im, N, log2N, I.q);
     __syncthreads();
     solve_lds(re, im, R, twp, N, log2N);
-    for (int j = threadIdx.x; j < N; j += blockDim.x) Es1[I.goff + j] = re[padi(j)];
-    const double emax = block_maxabs(red, N, [&](int64_t i) { return re[padi((int)i)]; });
+    for (int j = threadIdx.x; j < N; j += blockDim.x) Es1[I.goff + j] = solve_at(re, j);
+    const double emax = block_maxabs(red, N, [&](int64_t i) { return solve_at(re, (int)i); });
     if (threadIdx.x == 0 && !isfinite(emax)) st[blockIdx.x].err |= 1;
 }
 
@@ -515,7 +516,7 @@ k_simp_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st, u64
     const InstDesc I = insts[blockIdx.x];
     InstState S = st[blockIdx.x];
     if (sweep_over(S, k)) return;
-    const int N = I.N, log2N = I.log2N, NP = N + N / 16;
+    const int N = I.N, log2N = I.log2N, NP = solve_np(N);
     double* re = sm;
     double* im = sm + NP;
     double* e2n = sm + 2 * NP;
@@ -532,7 +533,7 @@ k_simp_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st, u64
     __syncthreads();
     solve_lds(re, im, R, twp, N, log2N);
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
-        const double e = re[padi(j)];
+        const double e = solve_at(re, j);
         E2o[j] = e;
         e2n[j] = e;
     }
@@ -541,7 +542,7 @@ k_simp_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st, u64
     take_charge64(racc3 + 3 * I.red0, re, im, N, log2N, I.q);
     __syncthreads();
     solve_lds(re, im, R, twp, N, log2N);
-    for (int j = threadIdx.x; j < N; j += blockDim.x) E3o[j] = re[padi(j)];
+    for (int j = threadIdx.x; j < N; j += blockDim.x) E3o[j] = solve_at(re, j);
     double emax;
     const bool conv = block_isapprox(
         red,
@@ -551,7 +552,7 @@ k_simp_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st, u64
         },
         [&](int e) {
             const int r = e % 3, i = e / 3;
-            return r == 0 ? E1[i] : (r == 1 ? e2n[i] : re[padi(i)]);
+            return r == 0 ? E1[i] : (r == 1 ? e2n[i] : solve_at(re, i));
         },
         3 * N, I.rtol, 0.0, emax);
     if (threadIdx.x == 0) {
@@ -807,7 +808,7 @@ __global__ void __launch_bounds__(kSolveThreads)
 k_op_fieldsolve(const double* __restrict__ rho, double* __restrict__ E, const double2* __restrict__ twp,
                 int N, int log2N) {
     extern __shared__ double sm[];
-    const int NP = N + N / 16;
+    const int NP = solve_np(N);
     double* re = sm;
     double* im = sm + NP;
     SolveRegs R;
@@ -819,7 +820,7 @@ k_op_fieldsolve(const double* __restrict__ rho, double* __restrict__ E, const do
     }
     __syncthreads();
     solve_lds(re, im, R, twp, N, log2N);
-    for (int j = threadIdx.x; j < N; j += blockDim.x) E[j] = re[padi(j)];
+    for (int j = threadIdx.x; j < N; j += blockDim.x) E[j] = solve_at(re, j);
 }
 
 __global__ void k_op_drift_half(double* __restrict__ x, const double* __restrict__ v, int64_t P, double dt) {

@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(kSolveThreads)
 k12_solve(u64* __restrict__ racc, double* __restrict__ E, double* __restrict__ Es, int64_t cap_bins,
           State12* __restrict__ st, const double2* __restrict__ twp, int N, int log2N, int qb, int bin) {
     extern __shared__ double sm[];
-    const int NP = N + N / 16;
+    const int NP = solve_np(N);
     double* re = sm;
     double* im = sm + NP;
     double* red = sm + 2 * NP + N;
@@ -99,15 +99,15 @@ k12_solve(u64* __restrict__ racc, double* __restrict__ E, double* __restrict__ E
     const int64_t t = st->step + 1;
     const int64_t bi = (t + bin - 1) / bin - 1;
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
-        const double e = re[padi(j)];
+        const double e = solve_at(re, j);
         E[j] = e;
         if (bi < cap_bins) Es[bi * N + j] = Es[bi * N + j] + e;
     }
     const double se = block_chunk_sum(red, N, [&](int64_t i) {
-        const double e = re[padi((int)i)];
+        const double e = solve_at(re, (int)i);
         return e * e;
     });
-    const double emax = block_maxabs(red, N, [&](int64_t i) { return re[padi((int)i)]; });
+    const double emax = block_maxabs(red, N, [&](int64_t i) { return solve_at(re, (int)i); });
     if (threadIdx.x == 0) {
         st->se = se;
         if (!isfinite(emax)) st->err |= 1;
@@ -273,7 +273,7 @@ void launch12_deposit(const Buffers12& b, hipStream_t s) {
 }
 
 void launch12_solve(const Buffers12& b, hipStream_t s) {
-    const int lds = (2 * (b.N + b.N / 16) + b.N + kRedDoubles) * 8;
+    const int lds = solve_lds_doubles(b.N) * 8;
     k12_solve<<<1, kSolveThreads, lds, s>>>((u64*)b.racc, b.E, b.Es, b.cap_bins, b.st, b.tw, b.N, b.log2N,
                                             b.qbits, b.bin);
 }

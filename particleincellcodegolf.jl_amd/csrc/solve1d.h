@@ -66,10 +66,20 @@ __device__ __forceinline__ void group_butterflies(double* ar, double* ai, const 
     }
 }
 
+// LDS layouts of the transform: the forward phase pads one double every 16
+// (padi); from the fused pass on, the inverse phase also pads one every 64
+// (padj), so that the fused pass's bit-reversed group stores (stride 64
+// doubles across a wave) spread over the banks like the strided passes do.
+__device__ __forceinline__ int padj(int i) { return i + (i >> 4) + (i >> 6); }
+// Doubles per padded re/im array (either layout).
+__host__ __device__ constexpr int solve_np(int N) { return N + N / 16 + N / 64; }
+// Element j of the solve's result (in the inverse layout).
+__device__ __forceinline__ double solve_at(const double* re, int j) { return re[padj(j)]; }
+
 // One LDS pass of M stages from stage s0+1; thread g owns group g (every pass
-// has N >> M <= 512 = kSolveThreads groups).  With scale_out the pass writes
-// only the real parts, times `scale`.
-template <int M>
+// has N >> M <= 512 = kSolveThreads groups).  INV: the inverse-phase layout.
+// With scale_out the pass writes only the real parts, times `scale`.
+template <int M, bool INV>
 __device__ __forceinline__ void lds_pass(double* re, double* im, const double2 (&w)[7], int N, int s0,
                                          bool inverse, bool scale_out, double scale) {
     constexpr int G = 1 << M;
@@ -80,14 +90,14 @@ __device__ __forceinline__ void lds_pass(double* re, double* im, const double2 (
         double ar[G], ai[G];
 #pragma unroll
         for (int j = 0; j < G; ++j) {
-            const int id = padi(base + (j << s0));
+            const int id = INV ? padj(base + (j << s0)) : padi(base + (j << s0));
             ar[j] = re[id];
             ai[j] = im[id];
         }
         group_butterflies<M>(ar, ai, w, inverse);
 #pragma unroll
         for (int j = 0; j < G; ++j) {
-            const int id = padi(base + (j << s0));
+            const int id = INV ? padj(base + (j << s0)) : padi(base + (j << s0));
             if (scale_out) {
                 re[id] = ar[j] * scale;
             } else {
@@ -99,11 +109,12 @@ __device__ __forceinline__ void lds_pass(double* re, double* im, const double2 (
     __syncthreads();
 }
 
+template <bool INV>
 __device__ __forceinline__ void lds_pass_m(int M, double* re, double* im, const double2 (&w)[7], int N, int s0,
                                            bool inverse, bool scale_out, double scale) {
-    if (M == 3) lds_pass<3>(re, im, w, N, s0, inverse, scale_out, scale);
-    else if (M == 2) lds_pass<2>(re, im, w, N, s0, inverse, scale_out, scale);
-    else lds_pass<1>(re, im, w, N, s0, inverse, scale_out, scale);
+    if (M == 3) lds_pass<3, INV>(re, im, w, N, s0, inverse, scale_out, scale);
+    else if (M == 2) lds_pass<2, INV>(re, im, w, N, s0, inverse, scale_out, scale);
+    else lds_pass<1, INV>(re, im, w, N, s0, inverse, scale_out, scale);
 }
 
 // Pass plans for L = log2 N >= 3, r = L % 3, np = ceil(L / 3) passes each:
@@ -142,9 +153,10 @@ __device__ __forceinline__ void solve_prefetch(SolveRegs& R, const double2* __re
 }
 
 // LDS doubles used by the solve: padded re/im, previous field, reduction scratch
-__host__ __device__ constexpr int solve_lds_doubles(int N) { return 2 * (N + N / 16) + N + kRedDoubles; }
+__host__ __device__ constexpr int solve_lds_doubles(int N) { return 2 * solve_np(N) + N + kRedDoubles; }
 
-// On entry re/im hold rho in bit-reversed order; on exit re holds E (padded).
+// On entry re/im hold rho in bit-reversed order (padi layout, arrays of
+// solve_np(N) doubles); on exit solve_at(re, j) is E[j].
 // Requires blockDim.x == kSolveThreads and R from solve_prefetch.
 __device__ __forceinline__ void solve_lds(double* re, double* im, SolveRegs& R, const double2* __restrict__ tw,
                                           int N, int L) {
@@ -152,7 +164,8 @@ __device__ __forceinline__ void solve_lds(double* re, double* im, SolveRegs& R, 
     const double scale = 1.0 / (double)N;
 #pragma unroll
     for (int p = 0; p < 3; ++p)
-        if (p < np - 1) lds_pass_m((p == 0 && r) ? r : 3, re, im, R.w[p], N, fwd_s0(p, r), false, false, 0.0);
+        if (p < np - 1)
+            lds_pass_m<false>((p == 0 && r) ? r : 3, re, im, R.w[p], N, fwd_s0(p, r), false, false, 0.0);
     // the forward transform's last pass, xi = rho^ ./ ik, the inverse's first pass
     const int g = threadIdx.x, s = L - 3;
     const bool own = g < (N >> 3);
@@ -200,7 +213,7 @@ __device__ __forceinline__ void solve_lds(double* re, double* im, SolveRegs& R, 
         const int base = (s ? (int)brev((uint32_t)g, s) : 0) << 3;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const int id = padi(base + q);
+            const int id = padj(base + q);
             if (np == 1) {
                 re[id] = br[q] * scale;
             } else {
@@ -214,7 +227,7 @@ __device__ __forceinline__ void solve_lds(double* re, double* im, SolveRegs& R, 
     for (int q = 1; q < 4; ++q)
         if (q < np) {
             const bool last = q == np - 1;
-            lds_pass_m(last && r ? r : 3, re, im, R.w[q], N, 3 * q, true, last, scale);
+            lds_pass_m<true>(last && r ? r : 3, re, im, R.w[q], N, 3 * q, true, last, scale);
         }
 }
 
@@ -268,22 +281,38 @@ __device__ __forceinline__ void tree256_max3_sum3(double* red, double (&m)[3], d
 // Julia's norm of three vectors (d = F - E, F, E) from their max-abs values:
 // m == 0 or Inf -> m; n*m^2 finite and m^2 != 0 -> sqrt(sum x^2); otherwise
 // m*sqrt(sum (|x|/m)^2).  Sums use the canonical tree.  One pass forms the
-// maxima and the plain sums together; a second pass runs only for a vector
-// that needs the scaled sum.
-template <typename FD, typename FF, typename FE>
-__device__ void block_norms3(double* red, int n, FD&& d, FF&& F, FE&& E, double& nd, double& nf, double& ne,
-                             double& me) {
+// maxima and the plain sums together (for n <= 4096 with all of a lane's
+// loads issued before its sums); a second pass runs only for a vector that
+// needs the scaled sum.
+template <typename FF, typename FE>
+__device__ void block_norms3(double* red, int n, FF&& F, FE&& E, double& nd, double& nf, double& ne, double& me) {
     const int t = threadIdx.x;
     double m[3] = {0.0, 0.0, 0.0}, s[3] = {0.0, 0.0, 0.0};
-    if (t < kTreeLanes)
-        for (int i = t; i < n; i += kTreeLanes) {
-            const double x[3] = {d(i), F(i), E(i)};
+    auto add = [&](double f, double e) {
+        const double x[3] = {f - e, f, e};
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                m[q] = nanmax(m[q], fabs(x[q]));
-                s[q] = s[q] + x[q] * x[q];
-            }
+        for (int q = 0; q < 3; ++q) {
+            m[q] = nanmax(m[q], fabs(x[q]));
+            s[q] = s[q] + x[q] * x[q];
         }
+    };
+    if (t < kTreeLanes) {
+        if (n <= kTreeChunk) {
+            constexpr int K = kTreeChunk / kTreeLanes;
+            double xf[K], xe[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int i = t + k * kTreeLanes;
+                xf[k] = i < n ? F(i) : 0.0;
+                xe[k] = i < n ? E(i) : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (t + k * kTreeLanes < n) add(xf[k], xe[k]);
+        } else {
+            for (int i = t; i < n; i += kTreeLanes) add(F(i), E(i));
+        }
+    }
     tree256_max3_sum3(red, m, s);
     bool direct[3], scaled[3];
 #pragma unroll
@@ -295,7 +324,8 @@ __device__ void block_norms3(double* red, int n, FD&& d, FF&& F, FE&& E, double&
         double z[3] = {0.0, 0.0, 0.0};
         if (t < kTreeLanes)
             for (int i = t; i < n; i += kTreeLanes) {
-                const double x[3] = {d(i), F(i), E(i)};
+                const double f = F(i), e = E(i);
+                const double x[3] = {f - e, f, e};
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
                     const double y = fabs(x[q]) / m[q];
@@ -318,7 +348,7 @@ __device__ void block_norms3(double* red, int n, FD&& d, FF&& F, FE&& E, double&
 template <typename FF, typename FE>
 __device__ bool block_isapprox(double* red, FF&& F, FE&& E, int n, double rtol, double atol, double& me) {
     double dn, nf, ne;
-    block_norms3(red, n, [&](int i) { return F(i) - E(i); }, F, E, dn, nf, ne, me);
+    block_norms3(red, n, F, E, dn, nf, ne, me);
     if (isfinite(dn)) {
         double mx = nf > ne ? nf : ne;
         if (nf != nf || ne != ne) mx = __longlong_as_double(0x7ff8000000000000LL);
