@@ -84,6 +84,24 @@ def lib():
         "orc2d_diagnostics": (None, [c_i32, c_i32, c_i64, _dp, _dp, _dp, _dp, c_d, _dp]),
         "orc2d_run": (ctypes.c_int, [c_i32, c_i32, c_i64, c_d, c_d, c_d, c_i32, c_i32, _dp, _dp, _dp, _dp,
                                      _dp, _dp, _dp, c_i64, c_vp, c_i32]),
+        "orc2d_fieldsolve_k": (ctypes.c_int, [c_i32, c_i32, c_d, c_d, _dp, _dp, _dp, c_vp]),
+        "orc23_unimod": (c_d, [c_d, c_d]),
+        "orc23_bspline": (ctypes.c_int, [c_i32, c_d, _dp]),
+        "orc23_weights": (ctypes.c_int, [c_i32, c_d, c_d, _ip64, _dp]),
+        "orc23_gather": (None, [c_i32, c_i32, c_d, c_d, c_i32, _dp, _dp, c_d, c_d, ctypes.POINTER(c_d),
+                                ctypes.POINTER(c_d)]),
+        "orc23_boris": (None, [c_i32, _dp, c_d, c_d, c_d, c_d, ctypes.POINTER(c_d), ctypes.POINTER(c_d),
+                               ctypes.POINTER(c_d)]),
+        "orc23_frac_bits": (ctypes.c_int, [c_i32, _ip64, _dp]),
+        "orc23_deposit": (None, [c_i32, c_i32, c_d, c_d, c_i32, c_i64, _dp, _dp, c_d, _dp]),
+        "orc23_fieldsolve": (ctypes.c_int, [c_i32, c_i32, c_d, c_d, _dp, _dp, _dp, c_vp]),
+        "orc23_order_key": (ctypes.c_uint64, [c_d]),
+        "orc23_ceil_index": (c_i64, [c_d]),
+        "orc23_sort": (None, [c_i64, _dp, _dp, _dp, _dp, _dp, c_d, c_d, _ip64]),
+        "orc23_diagnostics": (None, [c_i32, c_i32, c_i32, _dp, _dp, c_i32, _ip64, _dp, _dp, _dp, _dp]),
+        "orc23_run": (ctypes.c_int, [c_i32, c_i32, c_d, c_d, c_d, _dp, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32,
+                                     _ip32, _ip64, _dp, _dp, _dp, _dp, _dp, _dp, c_i64, c_i64, c_vp, c_vp, c_vp,
+                                     c_vp, c_i64, c_i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -409,3 +427,133 @@ def run2d(NX, NY, dt, B0, w, x, y, vx, vy, vz, nsteps, ns=2, textbook=0, Ex=None
     if rc != 0:
         raise ValueError(f"orc2d_run rejected its arguments (rc={rc})")
     return x, y, vx, vy, vz, Ex, Ey, K
+
+
+# ---- structured 2D3V module, electrostatic path (src/PIC2D3V.jl) ------------
+# Grids are (NY, NX) numpy arrays in C order (Julia's NX x NY, x fastest).
+# Particles of one species are a (5, P) array of rows x, y, vx, vy, vz (the
+# transpose of the reference's 5 x P xyv matrix as numpy sees it).
+
+SHAPES23 = {"ngp": 0, "area": 1, **{f"bspline{n}": 2 + n for n in range(6)}}
+
+
+def _shape23(shape):
+    return SHAPES23[shape] if isinstance(shape, str) else int(shape)
+
+
+def unimod23(x, n):
+    return lib().orc23_unimod(float(x), float(n))
+
+
+def bspline(n, x):
+    w = np.zeros(6)
+    m = lib().orc23_bspline(int(n), float(x), w)
+    return tuple(w[:m])
+
+
+def weights23(shape, z, NZ_Lz):
+    """(Julia cell indices, weights) of depositindicesfractions."""
+    c, w = np.zeros(6, dtype=np.int64), np.zeros(6)
+    m = lib().orc23_weights(_shape23(shape), float(z), float(NZ_Lz), c, w)
+    return list(c[:m]), list(w[:m])
+
+
+def gather23(Ex, Ey, x, y, shape, Lx=1.0, Ly=1.0):
+    Ex, Ey = _f64(Ex), _f64(Ey)
+    NY, NX = Ex.shape
+    ex, ey = ctypes.c_double(), ctypes.c_double()
+    lib().orc23_gather(NX, NY, NX / Lx, NY / Ly, _shape23(shape), Ex.ravel(), Ey.ravel(), float(x), float(y),
+                       ctypes.byref(ex), ctypes.byref(ey))
+    return ex.value, ey.value
+
+
+def boris23(vx, vy, vz, Ex, Ey, B, dt, q_m, variant=0):
+    a, b, c = ctypes.c_double(vx), ctypes.c_double(vy), ctypes.c_double(vz)
+    lib().orc23_boris(int(variant), _f64(B), float(dt), float(q_m), float(Ex), float(Ey), ctypes.byref(a),
+                      ctypes.byref(b), ctypes.byref(c))
+    return a.value, b.value, c.value
+
+
+def frac_bits23(P, w):
+    P, w = np.ascontiguousarray(P, dtype=np.int64), _f64(w)
+    return lib().orc23_frac_bits(P.size, P, w)
+
+
+def deposit23(NX, NY, x, y, w, shape, Lx=1.0, Ly=1.0):
+    x, y = _f64(x), _f64(y)
+    rho = np.zeros((NY, NX))
+    lib().orc23_deposit(int(NX), int(NY), NX / Lx, NY / Ly, _shape23(shape), x.size, x, y, float(w), rho.ravel())
+    return rho
+
+
+def fieldsolve23(rho, Lx=1.0, Ly=1.0):
+    """(Ex, Ey, phi, nonfinite) of the electrostatic solve in an Lx x Ly box."""
+    rho = _f64(rho)
+    NY, NX = rho.shape
+    Ex, Ey, phi = np.zeros((NY, NX)), np.zeros((NY, NX)), np.zeros((NY, NX))
+    bad = lib().orc23_fieldsolve(NX, NY, float(Lx), float(Ly), rho.ravel(), Ex.ravel(), Ey.ravel(), _ptr(phi))
+    return Ex, Ey, phi, bool(bad)
+
+
+def order_key(v):
+    return lib().orc23_order_key(float(v))
+
+
+def sort23(xyv, dx, dy):
+    """sort!(s, dx, dy): returns (sorted (5, P) copy, 0-based permutation)."""
+    a = _f64(xyv).copy()
+    P = a.shape[1]
+    perm = np.zeros(P, dtype=np.int64)
+    rows = [np.ascontiguousarray(a[k]) for k in range(5)]
+    lib().orc23_sort(P, *rows, float(dx), float(dy), perm)
+    return np.vstack(rows), perm
+
+
+def diagnostics23(Eacc_x, Eacc_y, buffer, species):
+    """Row of 8 (kinetic, field, momentum x3, characteristic momentum x3);
+    species: list of dicts with mass, weight, xyv ((5, P) array)."""
+    Ex, Ey = _f64(Eacc_x), _f64(Eacc_y)
+    NY, NX = Ex.shape
+    P = np.array([s["xyv"].shape[1] for s in species], dtype=np.int64)
+    state = np.concatenate([_f64(s["xyv"]).ravel() for s in species])
+    row = np.zeros(8)
+    lib().orc23_diagnostics(NX, NY, int(buffer), Ex.ravel(), Ey.ravel(), len(species), P,
+                            _f64([s["mass"] for s in species]), _f64([s["weight"] for s in species]), state, row)
+    return row
+
+
+def run23(NX, NY, Lx, Ly, dt, B, species, nsteps, t0=0, buffer=3, ntskip=1, ngskip=1, accumulate=1, boris=0,
+          Eacc=None, nbins=None, bins=None, nthreads=1):
+    """nsteps of loop! + diagnose! of the electrostatic PIC2D3V module.
+
+    species: list of dicts with shape, charge, mass, weight and xyv ((5, P)).
+    Eacc: (Ex, Ey) gathered field carried between calls (zeros at the start).
+    bins: (Exs, Eys, phis) arrays of shape (nbins, NY/ngskip, NX/ngskip),
+    accumulated in place (zeros when None).
+    Returns (species with new xyv, Eacc, rows (n, 8), bins, nonfinite)."""
+    P = np.array([s["xyv"].shape[1] for s in species], dtype=np.int64)
+    state = np.concatenate([_f64(s["xyv"]).ravel() for s in species])
+    Ex = np.zeros((NY, NX)) if Eacc is None else _f64(Eacc[0]).copy()
+    Ey = np.zeros((NY, NX)) if Eacc is None else _f64(Eacc[1]).copy()
+    t1 = t0 + nsteps
+    nrows = -(-t1 // ntskip) - (-(-t0 // ntskip))
+    rows = np.zeros((max(nrows, 0), 8))
+    if nbins is None:
+        nbins = -(-t1 // ntskip)
+    if bins is None:
+        bins = tuple(np.zeros((nbins, NY // ngskip, NX // ngskip)) for _ in range(3))
+    bins = tuple(_f64(b).copy() for b in bins)
+    bad = lib().orc23_run(int(NX), int(NY), float(Lx), float(Ly), float(dt), _f64(B), int(buffer), int(ntskip),
+                          int(ngskip), int(accumulate), int(boris), len(species),
+                          np.array([_shape23(s["shape"]) for s in species], dtype=np.int32), P,
+                          _f64([s["charge"] for s in species]), _f64([s["mass"] for s in species]),
+                          _f64([s["weight"] for s in species]), state, Ex.ravel(), Ey.ravel(), int(t0),
+                          int(nsteps), _ptr(rows), _ptr(bins[0]), _ptr(bins[1]), _ptr(bins[2]), int(nbins),
+                          int(nthreads))
+    out, o = [], 0
+    for s, n in zip(species, P):
+        d = dict(s)
+        d["xyv"] = state[o:o + 5 * n].reshape(5, n).copy()
+        out.append(d)
+        o += 5 * n
+    return out, (Ex, Ey), rows, bins, bool(bad)

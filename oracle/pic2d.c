@@ -198,7 +198,16 @@ void orc2d_fft(int32_t NX, int32_t NY, double* re, double* im, int32_t inverse) 
  * mean. */
 void orc2d_fieldsolve(int32_t NX, int32_t NY, const double* rho, double* Ex, double* Ey,
                       double* phi_out) {
+    (void)orc2d_fieldsolve_k(NX, NY, kTwoPi2d, kTwoPi2d, rho, Ex, Ey, phi_out);
+}
+
+/* The same solve with wavenumber scales kxs, kys (kx = kxs*m): 2pi/Lx and
+ * 2pi/Ly in an Lx x Ly box (src/PIC2D3V.jl:252-253).  Returns 1 if a tmp was
+ * not finite (the @assert of src/Electrostatic2D3V.jl:148), else 0. */
+int orc2d_fieldsolve_k(int32_t NX, int32_t NY, double kxs, double kys, const double* rho, double* Ex,
+                       double* Ey, double* phi_out) {
     const int64_t G = (int64_t)NX * NY;
+    int bad = 0;
     double* pr = (double*)malloc(sizeof(double) * (size_t)G);
     double* pi = (double*)malloc(sizeof(double) * (size_t)G);
     double* xr = (double*)malloc(sizeof(double) * (size_t)G);
@@ -210,9 +219,9 @@ void orc2d_fieldsolve(int32_t NX, int32_t NY, const double* rho, double* Ex, dou
     pr[0] = 0.0;
     pi[0] = 0.0;
     for (int32_t j = 0; j < NY; ++j) {
-        const double ky = kTwoPi2d * (double)(j < NY / 2 ? j : j - NY);
+        const double ky = kys * (double)(j < NY / 2 ? j : j - NY);
         for (int32_t i = 0; i < NX; ++i) {
-            const double kx = kTwoPi2d * (double)(i < NX / 2 ? i : i - NX);
+            const double kx = kxs * (double)(i < NX / 2 ? i : i - NX);
             const int64_t k = i + (int64_t)NX * j;
             double c = 0.0, d = 0.0;
             if (k != 0) {
@@ -223,6 +232,7 @@ void orc2d_fieldsolve(int32_t NX, int32_t NY, const double* rho, double* Ex, dou
             const double a = pr[k], b = pi[k];
             const double tr = a * c - b * d;
             const double ti = a * d + b * c;
+            bad |= !(isfinite(tr) && isfinite(ti));
             xr[k] = tr * kx;
             xi[k] = ti * kx;
             yr[k] = tr * ky;
@@ -241,6 +251,7 @@ void orc2d_fieldsolve(int32_t NX, int32_t NY, const double* rho, double* Ex, dou
         for (int64_t k = 0; k < G; ++k) phi_out[k] = pr[k] * scale;
     }
     free(pr); free(pi); free(xr); free(xi); free(yr); free(yi);
+    return bad;
 }
 
 /* Diagnostics (src/Electrostatic2D3V.jl:166-170):

@@ -140,6 +140,39 @@ void orc2d_diagnostics(int32_t NX, int32_t NY, int64_t P, const double* Ex, cons
 int  orc2d_run(int32_t NX, int32_t NY, int64_t P, double dt, double B0, double w,
                int32_t textbook, int32_t ns, double* x, double* y, double* vx, double* vy,
                double* vz, double* Ex, double* Ey, int64_t nsteps, double* K, int32_t nthreads);
+int  orc2d_fieldsolve_k(int32_t NX, int32_t NY, double kxs, double kys, const double* rho, double* Ex,
+                        double* Ey, double* phi_out);
+
+/* structured 2D3V module, electrostatic path (src/PIC2D3V.jl), in pic2d3v.c */
+#define ORC23_NGP 0
+#define ORC23_AREA 1
+#define ORC23_BSPLINE0 2 /* B-spline of order n: ORC23_BSPLINE0 + n, n = 0..5 */
+double   orc23_unimod(double x, double n);
+int      orc23_bspline(int32_t n, double x, double* w);
+int      orc23_weights(int32_t shape, double z, double NZ_Lz, int64_t* cell, double* w);
+void     orc23_gather(int32_t NX, int32_t NY, double NX_Lx, double NY_Ly, int32_t shape, const double* Ex,
+                      const double* Ey, double x, double y, double* ex, double* ey);
+void     orc23_boris(int32_t variant, const double* B, double dt, double q_m, double Ex, double Ey, double* vx,
+                     double* vy, double* vz);
+int      orc23_frac_bits(int32_t ns, const int64_t* P, const double* w);
+void     orc23_deposit(int32_t NX, int32_t NY, double NX_Lx, double NY_Ly, int32_t shape, int64_t P,
+                       const double* x, const double* y, double w, double* rho);
+int      orc23_fieldsolve(int32_t NX, int32_t NY, double Lx, double Ly, const double* rho, double* Ex, double* Ey,
+                          double* phi_out);
+uint64_t orc23_order_key(double v);
+int64_t  orc23_ceil_index(double a);
+void     orc23_sort(int64_t P, double* x, double* y, double* vx, double* vy, double* vz, double dx, double dy,
+                    int64_t* perm);
+void     orc23_diagnostics(int32_t NX, int32_t NY, int32_t buffer, const double* Eacc_x, const double* Eacc_y,
+                           int32_t ns, const int64_t* P, const double* mass, const double* weight,
+                           const double* state, double* row);
+int      orc23_run(int32_t NX, int32_t NY, double Lx, double Ly, double dt, const double* B, int32_t buffer,
+                   int32_t ntskip, int32_t ngskip, int32_t accumulate, int32_t boris, int32_t ns,
+                   const int32_t* shape, const int64_t* P, const double* charge, const double* mass,
+                   const double* weight, double* state, double* Eacc_x, double* Eacc_y, int64_t t0,
+                   int64_t nsteps, double* rows, double* Exs, double* Eys, double* phis, int64_t nbins,
+                   int32_t nthreads);
+
 
 #ifdef __cplusplus
 }
