@@ -1,0 +1,154 @@
+/*
+ * pic2d3v.h -- C ABI of the electrostatic path of the structured 2D3V module
+ * of jwscook/ParticleInCellCodeGolf.jl (src/PIC2D3V.jl) on AMD MI355X (gfx950).
+ *
+ * Replaces, for an ElectrostaticField and any number of Species (up to
+ * PIC2D3V_MAX_SPECIES):
+ *   Species(xyv 5 x P, charge, mass, weight, shape)          src/PIC2D3V.jl:148-156
+ *   sort!(species, dx, dy)        per-cell particle sort       :210-215
+ *   depositindicesfractions + the shapes NGP, area, B-spline 0..5
+ *                                                              :1106-1187
+ *   (field)(shape, x, y)          field gather                 :1217-1228
+ *   ElectrostaticBoris            Boris push, uniform B        :39-54
+ *   deposit!(rho, shape, x, y, ..., q*w/dV)                    :1249-1256
+ *   loop!(plasma, field)          particle loop, reduction, spectral solve,
+ *                                 update!                      :530-581, :294-297
+ *   diagnose!(ElectrostaticDiagnostics, ...)                   :1301-1330
+ *
+ * FP64 throughout.  Results are bit-identical to the CPU restatement: per-cell
+ * charge is an exact signed fixed-point sum (independent of the particle
+ * order and of the per-thread grids of :284-285), every other sum uses the
+ * canonical reduction tree.
+ *
+ * Layouts follow Julia's.  A species' particles are the 5 x P column-major
+ * matrix xyv: particle p is (x, y, vx, vy, vz) at xyv[5p .. 5p+4].  Grids are
+ * column-major NX x NY (x fastest).  Positions live in (0, Lx] x (0, Ly]
+ * (unimod, :9).  The ghost layers of the reference's offset arrays hold the
+ * periodic images of the NX x NY field; they are not stored here, but
+ * `buffer` enters the field-energy diagnostic, which averages over them.
+ *
+ * Two behaviours of the reference are kept by default, each with an opt-in
+ * alternative:
+ *   - update! adds every step's field into the gathered field Exy and never
+ *     clears it (:294-297 with applyperiodicity!'s +=, :20-27): the particles
+ *     see the running sum of all fields so far.  PIC2D3V_FIELD_REPLACE uses
+ *     the step's own field.
+ *   - ElectrostaticBoris scales the rotation by q_m^2 (:49-54), which turns
+ *     negative charges the wrong way and keeps |v| only for q_m^2 = 1.
+ *     PIC2D3V_BORIS_TEXTBOOK uses ElectromagneticBoris's form (:61-69).
+ * The shapes keep the reference's cell mirror: r = ceil(zN) - zN is taken as
+ * the distance from the lower cell edge, so a particle's charge centroid is
+ * 2 ceil(zN) - zN.
+ *
+ * Return codes, ownership, threading and error reporting follow pic.h.
+ */
+#ifndef PIC2D3V_H
+#define PIC2D3V_H
+#include <stdint.h>
+
+#include "pic.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIC2D3V_MAX_SPECIES 4
+
+/* shapes (src/PIC2D3V.jl:141-145) */
+#define PIC2D3V_NGP 0
+#define PIC2D3V_AREA 1
+#define PIC2D3V_BSPLINE0 2 /* BSplineWeighting{n}: PIC2D3V_BSPLINE0 + n, n = 0..5 */
+#define PIC2D3V_BSPLINE5 7
+
+#define PIC2D3V_FIELD_ACCUMULATE 0 /* update! as the reference: Exy += E     */
+#define PIC2D3V_FIELD_REPLACE 1    /* Exy = E                                */
+#define PIC2D3V_BORIS_REFERENCE 0  /* ElectrostaticBoris (q_m^2 rotation)    */
+#define PIC2D3V_BORIS_TEXTBOOK 1   /* ElectromagneticBoris with Ez = 0       */
+
+typedef struct pic2d3v_species {
+    int64_t P;        /* particles                                          */
+    int32_t shape;    /* PIC2D3V_NGP, PIC2D3V_AREA or PIC2D3V_BSPLINE0 + n    */
+    double charge;    /* Species.charge                                     */
+    double mass;      /* Species.mass                                       */
+    double weight;    /* Species.weight (n0*Lx*Ly/P in the reference, :189) */
+} pic2d3v_species;
+
+typedef struct pic2d3v_config {
+    int32_t NX, NY;        /* grid, powers of two in [8, 1024]                       */
+    double Lx, Ly;         /* box                                                    */
+    double dt;             /* time step                                              */
+    double B0[3];          /* uniform magnetic field (B0x, B0y, B0z)                 */
+    int32_t buffer;        /* ghost layers of Exy (3 in the reference)               */
+    int32_t ntskip;        /* diagnostics every ntskip steps, fields binned by them  */
+    int32_t ngskip;        /* field subsampling of the binned fields (power of two)  */
+    int32_t field_update;  /* PIC2D3V_FIELD_ACCUMULATE or PIC2D3V_FIELD_REPLACE      */
+    int32_t boris;         /* PIC2D3V_BORIS_REFERENCE or PIC2D3V_BORIS_TEXTBOOK      */
+    int32_t nspecies;      /* 1 .. PIC2D3V_MAX_SPECIES                               */
+    pic2d3v_species species[PIC2D3V_MAX_SPECIES];
+    int32_t device;        /* HIP device ordinal                                     */
+} pic2d3v_config;
+
+typedef struct pic2d3v_ctx pic2d3v_ctx;
+
+/* Context lifetime: a new context holds a zero field, step t = 0 and no
+ * diagnostics (a fresh ElectrostaticField and ElectrostaticDiagnostics).
+ * pic2d3v_last_error(NULL) returns the message of the last failed
+ * pic2d3v_create on this thread. */
+int pic2d3v_create(pic2d3v_ctx** ctx, const pic2d3v_config* cfg);
+int pic2d3v_destroy(pic2d3v_ctx* ctx);
+const char* pic2d3v_last_error(const pic2d3v_ctx* ctx);
+
+/* Species.xyv of one species (5*P doubles, column-major 5 x P) in and out.
+ * Uploading leaves the field, t and the diagnostics alone. */
+int pic2d3v_upload(pic2d3v_ctx* ctx, int32_t species, const double* xyv);
+int pic2d3v_download(pic2d3v_ctx* ctx, int32_t species, double* xyv);
+
+/* Back to a fresh field and diagnostics: Exy = 0, t = 0, no rows, no bins. */
+int pic2d3v_reset(pic2d3v_ctx* ctx);
+
+/* sort!(species, dx, dy): reorders the species' particles by
+ * (ceil(x/dx), ceil(y/dy), vx) under isless, ties by position (the stable
+ * sortperm).  perm (may be NULL) receives the 0-based permutation: new
+ * particle k was particle perm[k] (the reference's species.p minus 1).
+ * Fails with PIC_EINVAL if the cells span more than 2^24 or one cell holds
+ * more than 16384 particles. */
+int pic2d3v_sort(pic2d3v_ctx* ctx, int32_t species, double dx, double dy, int64_t* perm);
+
+/* nsteps of `loop!(plasma, field); diagnose!(d, field, plasma, t)` from the
+ * current t.  rows (may be NULL) receives one row of 8 doubles for every t of
+ * this call with t % ntskip == 0: kinetic energy, field energy, momentum
+ * (x, y, z), characteristic momentum (x, y, z), i.e. kineticenergy[ti],
+ * fieldenergy[ti], particlemomentum[ti], characteristicmomentum[ti].
+ * Returns PIC_ENONFINITE if the solve met a non-finite value (the reference's
+ * `@assert isfinite(tmp)`, :570); the state is then that of the last step. */
+int pic2d3v_run(pic2d3v_ctx* ctx, int64_t nsteps, double* rows);
+
+/* The time-binned fields of the diagnostics, Exs, Eys, phis[:, :, b] for
+ * b < nbins, each (NX/ngskip) x (NY/ngskip) per bin, bins back to back
+ * (Julia's Array{Float64,3} layout).  nbins may not exceed ceil(t/ntskip).
+ * Any output may be NULL. */
+int pic2d3v_fields(pic2d3v_ctx* ctx, double* Exs, double* Eys, double* phis, int64_t nbins);
+
+/* The gathered field Exy: the interior NX x NY of each component (either
+ * may be NULL). */
+int pic2d3v_field(pic2d3v_ctx* ctx, double* Ex, double* Ey);
+
+/* Operators on host arrays, bit-identical to the CPU restatement:
+ * deposit!: rho (NX x NY) = sum over n particles of (wx*wy)*w with the shape,
+ * folded periodically (:1249-1256, :481-486); the gather (f)(shape, x, y) of
+ * field (Ex, Ey) at n positions (:1217-1228); the context's Boris (variant,
+ * B0, dt) on n particles in place with charge-to-mass ratio q_m (:39-54); and
+ * the solve of rho in the context's box (:559-576), phi (may be NULL)
+ * receiving real(ifft(phi)) as diagnose! stores it (:1326). */
+int pic2d3v_deposit(pic2d3v_ctx* ctx, int32_t shape, const double* x, const double* y, int64_t n, double w,
+                    double* rho);
+int pic2d3v_gather(pic2d3v_ctx* ctx, int32_t shape, const double* Ex, const double* Ey, const double* x,
+                   const double* y, int64_t n, double* ex, double* ey);
+int pic2d3v_boris(pic2d3v_ctx* ctx, double* vx, double* vy, double* vz, const double* ex, const double* ey,
+                  int64_t n, double q_m);
+int pic2d3v_fieldsolve(pic2d3v_ctx* ctx, const double* rho, double* Ex, double* Ey, double* phi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -232,6 +232,8 @@ extern "C" int pic2d_create(pic2d_ctx** out, const pic2d_config* cfg) {
     b.D.sy = 32 - b.D.ly;
     b.D.w = cfg->w;
     b.D.rho_scale = cfg->w * std::ldexp(1.0, -2 * kQuanta2D);
+    b.D.kxs = kTwoPi / 1.0;  // the script's unit box: kx = 2pi*m
+    b.D.kys = kTwoPi / 1.0;
     b.precision = cfg->precision;
     b.dt = cfg->dt;
     b.B0 = cfg->B0;

@@ -1,0 +1,709 @@
+// capi2d3v.cpp -- host runtime behind include/pic2d3v.h: the electrostatic
+// PIC2D3V context (species, field, diagnostics), its device buffers, the
+// hipGraphs of one step (with and without the diagnostics row), the
+// per-cell sort and the operator calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pic2d3v.h"
+#include "pic2d3v_internal.h"
+
+#pragma clang fp contract(off)
+
+using namespace pic;
+
+struct pic2d3v_ctx {
+    pic2d3v_config cfg{};
+    hipStream_t stream = nullptr;
+    Grid23 g{};
+    Species23 sp[kMaxSpecies23]{};
+    bool uploaded[kMaxSpecies23] = {};
+    Diag23 d{};
+    Buffers2D b{};           // the 2D solve's descriptor and buffers
+    double2* E = nullptr;    // the gathered field Exy (interior, (Ex, Ey) per cell)
+    std::vector<void*> allocs;
+    double* tw = nullptr;
+    int64_t step = 0;        // t of the next step
+    hipGraphExec_t g_plain = nullptr, g_diag = nullptr;
+    void* op[8] = {};        // scratch (grown on demand)
+    size_t op_bytes[8] = {};
+    std::string err;
+};
+
+static thread_local std::string g23_create_err;
+
+static void set_err(pic2d3v_ctx* c, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    else g23_create_err = buf;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            set_err(ctx, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                    __LINE__);                                                              \
+            return e_ == hipErrorOutOfMemory ? PIC_ENOMEM : PIC_EHIP;                       \
+        }                                                                                   \
+    } while (0)
+
+static int ilog2i(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+
+static bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
+
+// Fraction bits of the signed exact charge sums: the total |charge| of all
+// species times 2^q stays below 2^111 (three 38-bit limbs hold 2^113).
+static int frac_bits23(int ns, const int64_t* P, const double* w) {
+    double total = 0.0;
+    for (int s = 0; s < ns; ++s) total += (double)P[s] * std::fabs(w[s]);
+    if (!(total > 0.0) || !std::isfinite(total)) return 64;
+    const int q = 110 - std::ilogb(total);
+    return q > 1100 ? 1100 : q;
+}
+
+template <typename T>
+static int dalloc(pic2d3v_ctx* c, T** p, size_t count) {
+    void* q = nullptr;
+    HIPCHK(c, hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+    c->allocs.push_back(q);
+    *p = (T*)q;
+    return PIC_OK;
+}
+
+static int scratch(pic2d3v_ctx* c, int slot, size_t bytes, void** out) {
+    if (c->op_bytes[slot] < bytes || !c->op[slot]) {
+        if (c->op[slot]) HIPCHK(c, hipFree(c->op[slot]));
+        c->op[slot] = nullptr;
+        c->op_bytes[slot] = 0;
+        HIPCHK(c, hipMalloc(&c->op[slot], std::max<size_t>(bytes, 8)));
+        c->op_bytes[slot] = bytes;
+    }
+    *out = c->op[slot];
+    return PIC_OK;
+}
+
+static void free_graphs(pic2d3v_ctx* c) {
+    if (c->g_plain) (void)hipGraphExecDestroy(c->g_plain);
+    if (c->g_diag) (void)hipGraphExecDestroy(c->g_diag);
+    c->g_plain = c->g_diag = nullptr;
+}
+
+static void free_bins(pic2d3v_ctx* c) {
+    Diag23& d = c->d;
+    for (double** p : {&d.rows, &d.Exs, &d.Eys, &d.phis}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    d.cap = 0;
+}
+
+static void release(pic2d3v_ctx* c) {
+    if (!c) return;
+    free_graphs(c);
+    free_bins(c);
+    for (void* p : c->allocs) (void)hipFree(p);
+    for (void*& p : c->op)
+        if (p) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+static int validate(const pic2d3v_config* g, std::string& why) {
+    char buf[256];
+    if (!pow2_in(g->NX, 8, kMax2D) || !pow2_in(g->NY, 8, kMax2D)) {
+        snprintf(buf, sizeof buf, "NX, NY must be powers of two in [8, %d] (got %d x %d)", kMax2D, g->NX, g->NY);
+        why = buf;
+        return PIC_EINVAL;
+    }
+    if (!(std::isfinite(g->Lx) && g->Lx > 0.0 && std::isfinite(g->Ly) && g->Ly > 0.0)) {
+        why = "Lx and Ly must be finite and positive";
+        return PIC_EINVAL;
+    }
+    if (!(std::isfinite(g->dt) && g->dt > 0.0)) {
+        why = "dt must be finite and positive";
+        return PIC_EINVAL;
+    }
+    if (!(std::isfinite(g->B0[0]) && std::isfinite(g->B0[1]) && std::isfinite(g->B0[2]))) {
+        why = "B0 must be finite";
+        return PIC_EINVAL;
+    }
+    if (g->buffer < 0 || g->buffer > 1024) {
+        why = "buffer must be in [0, 1024]";
+        return PIC_EINVAL;
+    }
+    if (g->ntskip < 1) {
+        why = "ntskip must be >= 1";
+        return PIC_EINVAL;
+    }
+    if (!pow2_in(g->ngskip, 1, std::min(g->NX, g->NY))) {
+        why = "ngskip must be a power of two no larger than NX and NY";
+        return PIC_EINVAL;
+    }
+    if (g->field_update != PIC2D3V_FIELD_ACCUMULATE && g->field_update != PIC2D3V_FIELD_REPLACE) {
+        why = "field_update must be PIC2D3V_FIELD_ACCUMULATE or PIC2D3V_FIELD_REPLACE";
+        return PIC_EINVAL;
+    }
+    if (g->boris != PIC2D3V_BORIS_REFERENCE && g->boris != PIC2D3V_BORIS_TEXTBOOK) {
+        why = "boris must be PIC2D3V_BORIS_REFERENCE or PIC2D3V_BORIS_TEXTBOOK";
+        return PIC_EINVAL;
+    }
+    if (g->nspecies < 1 || g->nspecies > PIC2D3V_MAX_SPECIES) {
+        snprintf(buf, sizeof buf, "nspecies must be in [1, %d]", PIC2D3V_MAX_SPECIES);
+        why = buf;
+        return PIC_EINVAL;
+    }
+    int64_t total = 0;
+    for (int s = 0; s < g->nspecies; ++s) {
+        const pic2d3v_species& q = g->species[s];
+        if (q.P < 1 || q.P > kMaxP23) {
+            snprintf(buf, sizeof buf, "species %d: P must be in [1, 2^26]", s);
+            why = buf;
+            return PIC_EINVAL;
+        }
+        total += q.P;
+        if (q.shape < PIC2D3V_NGP || q.shape > PIC2D3V_BSPLINE5) {
+            snprintf(buf, sizeof buf, "species %d: shape must be PIC2D3V_NGP, PIC2D3V_AREA or a B-spline", s);
+            why = buf;
+            return PIC_EINVAL;
+        }
+        if (!(std::isfinite(q.charge) && std::isfinite(q.mass) && q.mass != 0.0 && std::isfinite(q.weight) &&
+              q.weight >= 0.0)) {
+            snprintf(buf, sizeof buf, "species %d: charge, mass (nonzero) and weight (>= 0) must be finite", s);
+            why = buf;
+            return PIC_EINVAL;
+        }
+    }
+    if (total > kMaxP23) {
+        why = "the species hold more than 2^26 particles together";
+        return PIC_EINVAL;
+    }
+    return PIC_OK;
+}
+
+static int zero_state(pic2d3v_ctx* c) {
+    const int64_t G = (int64_t)c->cfg.NX * c->cfg.NY;
+    const int64_t zero = 0;
+    HIPCHK(c, hipMemsetAsync(c->E, 0, sizeof(double2) * (size_t)G, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->b.racc, 0, 3 * sizeof(unsigned long long) * (size_t)G, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->b.flag, 0, sizeof(int32_t), c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d.t, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+    Diag23& d = c->d;
+    if (d.cap > 0) {
+        const size_t nb = (size_t)d.cap * d.NXd * d.NYd;
+        HIPCHK(c, hipMemsetAsync(d.rows, 0, sizeof(double) * 8 * (size_t)d.cap, c->stream));
+        HIPCHK(c, hipMemsetAsync(d.Exs, 0, sizeof(double) * nb, c->stream));
+        HIPCHK(c, hipMemsetAsync(d.Eys, 0, sizeof(double) * nb, c->stream));
+        HIPCHK(c, hipMemsetAsync(d.phis, 0, sizeof(double) * nb, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->step = 0;
+    return PIC_OK;
+}
+
+extern "C" int pic2d3v_create(pic2d3v_ctx** out, const pic2d3v_config* cfg) {
+    if (!out || !cfg) {
+        g23_create_err = "pic2d3v_create: null argument";
+        return PIC_EINVAL;
+    }
+    *out = nullptr;
+    std::string why;
+    if (int rc = validate(cfg, why)) {
+        g23_create_err = "pic2d3v_create: " + why;
+        return rc;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) {
+        g23_create_err = "pic2d3v_create: no such HIP device";
+        return PIC_EHIP;
+    }
+    pic2d3v_ctx* c = new pic2d3v_ctx();
+    c->cfg = *cfg;
+    auto fail = [&](int rc) {
+        g23_create_err = c->err.empty() ? std::string("pic2d3v_create failed") : c->err;
+        release(c);
+        return rc;
+    };
+    if (hipSetDevice(cfg->device) != hipSuccess) return fail(PIC_EHIP);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(PIC_EHIP);
+    configure_kernels_2d();
+    const int NX = cfg->NX, NY = cfg->NY, ns = cfg->nspecies;
+    const int64_t G = (int64_t)NX * NY;
+    Grid23& g = c->g;
+    g.NX = NX;
+    g.NY = NY;
+    g.lx = ilog2i(NX);
+    g.ly = ilog2i(NY);
+    g.Lx = cfg->Lx;
+    g.Ly = cfg->Ly;
+    g.NX_Lx = (double)NX / cfg->Lx;
+    g.NY_Ly = (double)NY / cfg->Ly;
+    g.dt = cfg->dt;
+    for (int k = 0; k < 3; ++k) g.B[k] = cfg->B0[k];
+    g.boris = cfg->boris;
+    // qw = (charge*weight)/dV with dV = (Lx/NX)*(Ly/NY) (loop!, :533, :537)
+    const double dV = (cfg->Lx / (double)NX) * (cfg->Ly / (double)NY);
+    int64_t Ps[kMaxSpecies23];
+    double qws[kMaxSpecies23];
+    int rc;
+    for (int s = 0; s < ns; ++s) {
+        const pic2d3v_species& q = cfg->species[s];
+        Species23& sp = c->sp[s];
+        sp.P = q.P;
+        sp.shape = q.shape;
+        sp.q_m = q.charge / q.mass;
+        sp.qw = q.charge * q.weight / dV;
+        sp.mass = q.mass;
+        sp.weight = q.weight;
+        Ps[s] = q.P;
+        qws[s] = sp.qw;
+        if ((rc = dalloc(c, &sp.x, q.P)) || (rc = dalloc(c, &sp.y, q.P)) || (rc = dalloc(c, &sp.vx, q.P)) ||
+            (rc = dalloc(c, &sp.vy, q.P)) || (rc = dalloc(c, &sp.vz, q.P)))
+            return fail(rc);
+        for (double* a : {sp.x, sp.y, sp.vx, sp.vy, sp.vz})
+            if (hipMemset(a, 0, sizeof(double) * (size_t)q.P) != hipSuccess) return fail(PIC_EHIP);
+    }
+    g.q = frac_bits23(ns, Ps, qws);
+    Buffers2D& b = c->b;
+    b.D.NX = NX;
+    b.D.NY = NY;
+    b.D.lx = g.lx;
+    b.D.ly = g.ly;
+    b.D.P = 0;
+    b.D.q = g.q;
+    b.D.kxs = kTwoPi / cfg->Lx;  // kx = 2pi/Lx*m (FFTHelper, :252)
+    b.D.kys = kTwoPi / cfg->Ly;
+    b.precision = PIC_PREC_FP64;
+    if ((rc = dalloc(c, &b.racc, 3 * G)) || (rc = dalloc(c, &b.S, G)) || (rc = dalloc(c, &b.T1, G)) ||
+        (rc = dalloc(c, &b.T2, G)) || (rc = dalloc(c, &b.T3, G)) || (rc = dalloc(c, &b.Ex, G)) ||
+        (rc = dalloc(c, &b.Ey, G)) || (rc = dalloc(c, &b.phi, G)) || (rc = dalloc(c, &b.flag, 1)) ||
+        (rc = dalloc(c, &c->E, G)))
+        return fail(rc);
+    b.E32 = nullptr;
+    const std::vector<double> twp = pass_twiddle_table();
+    if ((rc = dalloc(c, &c->tw, twp.size()))) return fail(rc);
+    if (hipMemcpy(c->tw, twp.data(), sizeof(double) * twp.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(PIC_EHIP);
+    b.tw = reinterpret_cast<const double2*>(c->tw);
+    Diag23& d = c->d;
+    d.buffer = cfg->buffer;
+    d.ntskip = cfg->ntskip;
+    d.ngskip = cfg->ngskip;
+    d.NXd = NX / cfg->ngskip;
+    d.NYd = NY / cfg->ngskip;
+    const int64_t nf = 2 * (int64_t)(NX + 2 * cfg->buffer) * (NY + 2 * cfg->buffer);
+    if ((rc = dalloc(c, &d.t, 1)) || (rc = dalloc(c, &d.fpart, (nf + kTreeChunk - 1) / kTreeChunk))) return fail(rc);
+    for (int s = 0; s < ns; ++s) {
+        const int64_t P = cfg->species[s].P;
+        if ((rc = dalloc(c, &d.kpart[s], (3 * P + kTreeChunk - 1) / kTreeChunk)) ||
+            (rc = dalloc(c, &d.mpart[s], 6 * ((P + kTreeChunk - 1) / kTreeChunk))))
+            return fail(rc);
+    }
+    if ((rc = zero_state(c))) return fail(rc);
+    *out = c;
+    return PIC_OK;
+}
+
+extern "C" int pic2d3v_destroy(pic2d3v_ctx* c) {
+    if (!c) return PIC_EINVAL;
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    release(c);
+    return PIC_OK;
+}
+
+extern "C" const char* pic2d3v_last_error(const pic2d3v_ctx* c) {
+    return c ? c->err.c_str() : g23_create_err.c_str();
+}
+
+static int finish(pic2d3v_ctx* c) {
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+static int check_species(pic2d3v_ctx* c, const char* who, int32_t s) {
+    if (s < 0 || s >= c->cfg.nspecies) {
+        set_err(c, "%s: no species %d", who, (int)s);
+        return PIC_EINVAL;
+    }
+    return PIC_OK;
+}
+
+// ---- state ------------------------------------------------------------------
+
+extern "C" int pic2d3v_upload(pic2d3v_ctx* c, int32_t s, const double* xyv) {
+    if (!c) return PIC_EINVAL;
+    if (int rc = check_species(c, "pic2d3v_upload", s)) return rc;
+    if (!xyv) {
+        set_err(c, "pic2d3v_upload: xyv must not be NULL");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const Species23& sp = c->sp[s];
+    void* st = nullptr;
+    int rc;
+    if ((rc = scratch(c, 0, sizeof(double) * 5 * (size_t)sp.P, &st))) return rc;
+    HIPCHK(c, hipMemcpyAsync(st, xyv, sizeof(double) * 5 * (size_t)sp.P, hipMemcpyHostToDevice, c->stream));
+    launch23_unpack((const double*)st, sp, c->stream);
+    if ((rc = finish(c))) return rc;
+    c->uploaded[s] = true;
+    return PIC_OK;
+}
+
+extern "C" int pic2d3v_download(pic2d3v_ctx* c, int32_t s, double* xyv) {
+    if (!c) return PIC_EINVAL;
+    if (int rc = check_species(c, "pic2d3v_download", s)) return rc;
+    if (!xyv) {
+        set_err(c, "pic2d3v_download: xyv must not be NULL");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const Species23& sp = c->sp[s];
+    void* st = nullptr;
+    int rc;
+    if ((rc = scratch(c, 0, sizeof(double) * 5 * (size_t)sp.P, &st))) return rc;
+    launch23_pack(sp, (double*)st, c->stream);
+    HIPCHK(c, hipMemcpyAsync(xyv, st, sizeof(double) * 5 * (size_t)sp.P, hipMemcpyDeviceToHost, c->stream));
+    return finish(c);
+}
+
+extern "C" int pic2d3v_reset(pic2d3v_ctx* c) {
+    if (!c) return PIC_EINVAL;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    return zero_state(c);
+}
+
+extern "C" int pic2d3v_field(pic2d3v_ctx* c, double* Ex, double* Ey) {
+    if (!c) return PIC_EINVAL;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const int64_t G = (int64_t)c->cfg.NX * c->cfg.NY;
+    void *a = nullptr, *b = nullptr;
+    int rc;
+    if ((rc = scratch(c, 1, sizeof(double) * (size_t)G, &a)) || (rc = scratch(c, 2, sizeof(double) * (size_t)G, &b)))
+        return rc;
+    launch23_split_field(c->E, (double*)a, (double*)b, G, c->stream);
+    if (Ex) HIPCHK(c, hipMemcpyAsync(Ex, a, sizeof(double) * (size_t)G, hipMemcpyDeviceToHost, c->stream));
+    if (Ey) HIPCHK(c, hipMemcpyAsync(Ey, b, sizeof(double) * (size_t)G, hipMemcpyDeviceToHost, c->stream));
+    return finish(c);
+}
+
+// ---- sort!(species, dx, dy) -------------------------------------------------
+
+extern "C" int pic2d3v_sort(pic2d3v_ctx* c, int32_t s, double dx, double dy, int64_t* perm) {
+    if (!c) return PIC_EINVAL;
+    if (int rc = check_species(c, "pic2d3v_sort", s)) return rc;
+    if (!(std::isfinite(dx) && dx > 0.0 && std::isfinite(dy) && dy > 0.0)) {
+        set_err(c, "pic2d3v_sort: dx and dy must be finite and positive");
+        return PIC_EINVAL;
+    }
+    if (!c->uploaded[s]) {
+        set_err(c, "pic2d3v_sort: upload species %d first", (int)s);
+        return PIC_ESTATE;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const Species23& sp = c->sp[s];
+    const int64_t P = sp.P;
+    Sort23 w{};
+    void* p = nullptr;
+    int rc;
+    // slot 3: ext, maxcount; 4: per-particle arrays; 5: per-bucket arrays; 0: applied particles
+    if ((rc = scratch(c, 3, 64, &p))) return rc;
+    w.ext = (int64_t*)p;
+    w.maxcount = (uint32_t*)((char*)p + 32);
+    const int64_t init[4] = {LLONG_MAX, LLONG_MIN, LLONG_MAX, LLONG_MIN};
+    HIPCHK(c, hipMemcpyAsync(w.ext, init, sizeof init, hipMemcpyHostToDevice, c->stream));
+    launch23_sort_extent(sp, dx, dy, w, c->stream);
+    int64_t ext[4];
+    HIPCHK(c, hipMemcpyAsync(ext, w.ext, sizeof ext, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = finish(c))) return rc;
+    const int64_t rx = ext[1] - ext[0] + 1, ry = ext[3] - ext[2] + 1;
+    if (!(rx >= 1 && ry >= 1 && rx <= kMaxSortBuckets && ry <= kMaxSortBuckets && rx * ry <= kMaxSortBuckets)) {
+        set_err(c, "pic2d3v_sort: the particles span more than 2^24 cells of %g x %g", dx, dy);
+        return PIC_EINVAL;
+    }
+    const int64_t nb = rx * ry;
+    // per particle: bucket, vkey, skey, sidx, sbucket, perm
+    const size_t per = (size_t)P;
+    if ((rc = scratch(c, 4, per * (4 + 8 + 8 + 4 + 4 + 4), &p))) return rc;
+    char* q = (char*)p;
+    w.vkey = (unsigned long long*)q;
+    w.skey = (unsigned long long*)(q + 8 * per);
+    w.bucket = (uint32_t*)(q + 16 * per);
+    w.sidx = (uint32_t*)(q + 20 * per);
+    w.sbucket = (uint32_t*)(q + 24 * per);
+    w.perm = (uint32_t*)(q + 28 * per);
+    if ((rc = scratch(c, 5, sizeof(uint32_t) * (3 * (size_t)nb + 1), &p))) return rc;
+    w.count = (uint32_t*)p;
+    w.cursor = w.count + nb;
+    w.start = w.cursor + nb;
+    HIPCHK(c, hipMemsetAsync(w.count, 0, sizeof(uint32_t) * (size_t)nb, c->stream));
+    launch23_sort_buckets(sp, dx, dy, ry, nb, w, c->stream);
+    uint32_t mx = 0;
+    HIPCHK(c, hipMemcpyAsync(&mx, w.maxcount, sizeof mx, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = finish(c))) return rc;
+    if ((int64_t)mx > kMaxSortCell) {
+        set_err(c, "pic2d3v_sort: %u particles share one cell (at most %lld)", mx, (long long)kMaxSortCell);
+        return PIC_EINVAL;
+    }
+    launch23_sort_order(sp, nb, w, c->stream);
+    if ((rc = scratch(c, 0, sizeof(double) * 5 * per, &p))) return rc;
+    double* dst[5];
+    for (int k = 0; k < 5; ++k) dst[k] = (double*)p + k * per;
+    launch23_sort_apply(sp, w.perm, dst, c->stream);
+    double* cols[5] = {sp.x, sp.y, sp.vx, sp.vy, sp.vz};
+    for (int k = 0; k < 5; ++k)
+        HIPCHK(c, hipMemcpyAsync(cols[k], dst[k], sizeof(double) * per, hipMemcpyDeviceToDevice, c->stream));
+    std::vector<uint32_t> hp;
+    if (perm) {
+        hp.resize(per);
+        HIPCHK(c, hipMemcpyAsync(hp.data(), w.perm, sizeof(uint32_t) * per, hipMemcpyDeviceToHost, c->stream));
+    }
+    if ((rc = finish(c))) return rc;
+    for (size_t k = 0; k < hp.size(); ++k) perm[k] = hp[k];
+    return PIC_OK;
+}
+
+// ---- run --------------------------------------------------------------------
+
+// Room for bins 0 .. nbins-1 of the rows and the binned fields (grown
+// geometrically, old bins kept, new ones zero).
+static int ensure_bins(pic2d3v_ctx* c, int64_t nbins) {
+    Diag23& d = c->d;
+    if (nbins <= d.cap) return PIC_OK;
+    const int64_t cap = std::max<int64_t>(nbins, std::max<int64_t>(16, 2 * d.cap));
+    const size_t per = (size_t)d.NXd * d.NYd;
+    double* nr = nullptr;
+    double* nf[3] = {nullptr, nullptr, nullptr};
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMalloc(&nr, sizeof(double) * 8 * (size_t)cap));
+    for (double*& p : nf) {
+        hipError_t e = hipMalloc(&p, sizeof(double) * per * (size_t)cap);
+        if (e != hipSuccess) {
+            (void)hipFree(nr);
+            for (double* z : nf)
+                if (z) (void)hipFree(z);
+            set_err(c, "pic2d3v_run: cannot allocate %lld field bins", (long long)cap);
+            return PIC_ENOMEM;
+        }
+    }
+    HIPCHK(c, hipMemsetAsync(nr, 0, sizeof(double) * 8 * (size_t)cap, c->stream));
+    for (double* p : nf) HIPCHK(c, hipMemsetAsync(p, 0, sizeof(double) * per * (size_t)cap, c->stream));
+    if (d.cap > 0) {
+        HIPCHK(c, hipMemcpyAsync(nr, d.rows, sizeof(double) * 8 * (size_t)d.cap, hipMemcpyDeviceToDevice, c->stream));
+        double* old[3] = {d.Exs, d.Eys, d.phis};
+        for (int k = 0; k < 3; ++k)
+            HIPCHK(c, hipMemcpyAsync(nf[k], old[k], sizeof(double) * per * (size_t)d.cap, hipMemcpyDeviceToDevice,
+                                     c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_bins(c);
+    d.rows = nr;
+    d.Exs = nf[0];
+    d.Eys = nf[1];
+    d.phis = nf[2];
+    d.cap = cap;
+    free_graphs(c);  // the graphs hold the old pointers
+    return PIC_OK;
+}
+
+static void enqueue_step(pic2d3v_ctx* c, bool diag, hipStream_t s) {
+    for (int k = 0; k < c->cfg.nspecies; ++k) launch23_push(c->g, c->sp[k], c->E, c->b.racc, s);
+    launch2d_solve(c->b, 3, nullptr, true, s);
+    launch23_update(c->g, c->d, c->b, c->E, c->cfg.field_update == PIC2D3V_FIELD_ACCUMULATE ? 1 : 0, s);
+    if (diag) launch23_diag(c->g, c->d, c->E, c->sp, c->cfg.nspecies, s);
+    launch23_tick(c->d.t, s);
+}
+
+static int capture(pic2d3v_ctx* c, bool diag, hipGraphExec_t* out) {
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(c, diag, c->stream);
+    HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(c, e);
+    return PIC_OK;
+}
+
+static int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+extern "C" int pic2d3v_run(pic2d3v_ctx* c, int64_t nsteps, double* rows) {
+    if (!c) return PIC_EINVAL;
+    if (nsteps < 0) {
+        set_err(c, "pic2d3v_run: nsteps must be >= 0");
+        return PIC_EINVAL;
+    }
+    for (int s = 0; s < c->cfg.nspecies; ++s)
+        if (!c->uploaded[s]) {
+            set_err(c, "pic2d3v_run: upload species %d first", s);
+            return PIC_ESTATE;
+        }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const int64_t nt = c->cfg.ntskip;
+    const int64_t t0 = c->step, t1 = c->step + nsteps;
+    int rc;
+    if ((rc = ensure_bins(c, cdiv(t1, nt)))) return rc;
+    for (int64_t t = t0; t < t1; ++t) {
+        const bool diag = t % nt == 0;
+        hipGraphExec_t& ge = diag ? c->g_diag : c->g_plain;
+        if (!ge && (rc = capture(c, diag, &ge))) return rc;
+        HIPCHK(c, hipGraphLaunch(ge, c->stream));
+    }
+    if ((rc = finish(c))) return rc;
+    c->step = t1;
+    const int64_t r0 = cdiv(t0, nt), r1 = cdiv(t1, nt);
+    if (rows && r1 > r0)
+        HIPCHK(c, hipMemcpy(rows, c->d.rows + 8 * r0, sizeof(double) * 8 * (size_t)(r1 - r0), hipMemcpyDeviceToHost));
+    int32_t flag = 0;
+    HIPCHK(c, hipMemcpy(&flag, c->b.flag, sizeof flag, hipMemcpyDeviceToHost));
+    if (flag) {
+        set_err(c, "pic2d3v_run: the field solve met a non-finite value");
+        return PIC_ENONFINITE;
+    }
+    return PIC_OK;
+}
+
+extern "C" int pic2d3v_fields(pic2d3v_ctx* c, double* Exs, double* Eys, double* phis, int64_t nbins) {
+    if (!c) return PIC_EINVAL;
+    const int64_t have = cdiv(c->step, c->cfg.ntskip);
+    if (nbins < 0 || nbins > have) {
+        set_err(c, "pic2d3v_fields: nbins must be in [0, %lld]", (long long)have);
+        return PIC_EINVAL;
+    }
+    if (nbins == 0) return PIC_OK;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const size_t n = sizeof(double) * (size_t)c->d.NXd * c->d.NYd * (size_t)nbins;
+    if (Exs) HIPCHK(c, hipMemcpy(Exs, c->d.Exs, n, hipMemcpyDeviceToHost));
+    if (Eys) HIPCHK(c, hipMemcpy(Eys, c->d.Eys, n, hipMemcpyDeviceToHost));
+    if (phis) HIPCHK(c, hipMemcpy(phis, c->d.phis, n, hipMemcpyDeviceToHost));
+    return PIC_OK;
+}
+
+// ---- operators ----------------------------------------------------------------
+
+static int h2d(pic2d3v_ctx* c, int slot, const double* src, int64_t n, double** dev) {
+    void* p = nullptr;
+    int rc = scratch(c, slot, sizeof(double) * (size_t)n, &p);
+    if (rc) return rc;
+    if (src && n > 0)
+        HIPCHK(c, hipMemcpyAsync(p, src, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    *dev = (double*)p;
+    return PIC_OK;
+}
+
+static int d2h(pic2d3v_ctx* c, double* dst, const double* dev, int64_t n) {
+    if (n > 0) HIPCHK(c, hipMemcpyAsync(dst, dev, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    return PIC_OK;
+}
+
+static bool shape_ok(int32_t shape) { return shape >= PIC2D3V_NGP && shape <= PIC2D3V_BSPLINE5; }
+
+extern "C" int pic2d3v_deposit(pic2d3v_ctx* c, int32_t shape, const double* x, const double* y, int64_t n, double w,
+                               double* rho) {
+    if (!c) return PIC_EINVAL;
+    if (!shape_ok(shape) || !x || !y || !rho || n < 0 || n > kMaxP23 || !std::isfinite(w)) {
+        set_err(c, "pic2d3v_deposit: bad argument");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const int64_t G = (int64_t)c->cfg.NX * c->cfg.NY;
+    double *dx, *dy, *dr;
+    int rc;
+    if ((rc = h2d(c, 1, x, n, &dx)) || (rc = h2d(c, 2, y, n, &dy)) || (rc = h2d(c, 3, nullptr, G, &dr))) return rc;
+    Grid23 g = c->g;
+    g.q = frac_bits23(1, &n, &w);
+    // the run's accumulator is clear between steps (the solve empties it)
+    if (n > 0) launch23_deposit_op(g, shape, dx, dy, n, w, c->b.racc, c->stream);
+    launch23_limbs_to_rho(g, c->b.racc, dr, c->stream);
+    if ((rc = d2h(c, rho, dr, G))) return rc;
+    return finish(c);
+}
+
+extern "C" int pic2d3v_gather(pic2d3v_ctx* c, int32_t shape, const double* Ex, const double* Ey, const double* x,
+                              const double* y, int64_t n, double* ex, double* ey) {
+    if (!c) return PIC_EINVAL;
+    if (!shape_ok(shape) || !Ex || !Ey || !x || !y || !ex || !ey || n < 0) {
+        set_err(c, "pic2d3v_gather: bad argument");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const int64_t G = (int64_t)c->cfg.NX * c->cfg.NY;
+    double *dEx, *dEy, *dx, *dy, *dex, *dey, *dE;
+    int rc;
+    if ((rc = h2d(c, 0, Ex, G, &dEx)) || (rc = h2d(c, 1, Ey, G, &dEy)) || (rc = h2d(c, 2, x, n, &dx)) ||
+        (rc = h2d(c, 3, y, n, &dy)) || (rc = h2d(c, 4, nullptr, n, &dex)) || (rc = h2d(c, 5, nullptr, n, &dey)) ||
+        (rc = h2d(c, 6, nullptr, 2 * G, &dE)))
+        return rc;
+    launch23_pair_field(dEx, dEy, (double2*)dE, G, c->stream);
+    if (n > 0) launch23_gather_op(c->g, shape, (const double2*)dE, dx, dy, n, dex, dey, c->stream);
+    if ((rc = d2h(c, ex, dex, n)) || (rc = d2h(c, ey, dey, n))) return rc;
+    return finish(c);
+}
+
+extern "C" int pic2d3v_boris(pic2d3v_ctx* c, double* vx, double* vy, double* vz, const double* ex, const double* ey,
+                             int64_t n, double q_m) {
+    if (!c) return PIC_EINVAL;
+    if (!vx || !vy || !vz || !ex || !ey || n < 0 || !std::isfinite(q_m)) {
+        set_err(c, "pic2d3v_boris: bad argument");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    double *a, *b, *cc, *dex, *dey;
+    int rc;
+    if ((rc = h2d(c, 0, vx, n, &a)) || (rc = h2d(c, 1, vy, n, &b)) || (rc = h2d(c, 2, vz, n, &cc)) ||
+        (rc = h2d(c, 3, ex, n, &dex)) || (rc = h2d(c, 4, ey, n, &dey)))
+        return rc;
+    if (n > 0) launch23_boris_op(c->g, q_m, a, b, cc, dex, dey, n, c->stream);
+    if ((rc = d2h(c, vx, a, n)) || (rc = d2h(c, vy, b, n)) || (rc = d2h(c, vz, cc, n))) return rc;
+    return finish(c);
+}
+
+extern "C" int pic2d3v_fieldsolve(pic2d3v_ctx* c, const double* rho, double* Ex, double* Ey, double* phi) {
+    if (!c) return PIC_EINVAL;
+    if (!rho || !Ex || !Ey) {
+        set_err(c, "pic2d3v_fieldsolve: rho, Ex, Ey must not be NULL");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const int64_t G = (int64_t)c->cfg.NX * c->cfg.NY;
+    double *dr, *ex, *ey, *ph;
+    int rc;
+    if ((rc = h2d(c, 0, rho, G, &dr)) || (rc = h2d(c, 1, nullptr, G, &ex)) || (rc = h2d(c, 2, nullptr, G, &ey)) ||
+        (rc = h2d(c, 3, nullptr, G, &ph)))
+        return rc;
+    Buffers2D b = c->b;
+    b.Ex = ex;
+    b.Ey = ey;
+    b.phi = ph;
+    b.E32 = nullptr;
+    HIPCHK(c, hipMemsetAsync(b.flag, 0, sizeof(int32_t), c->stream));
+    launch2d_solve(b, 2, dr, phi != nullptr, c->stream);
+    if ((rc = d2h(c, Ex, ex, G)) || (rc = d2h(c, Ey, ey, G))) return rc;
+    if (phi && (rc = d2h(c, phi, ph, G))) return rc;
+    if ((rc = finish(c))) return rc;
+    int32_t flag = 0;
+    HIPCHK(c, hipMemcpy(&flag, b.flag, sizeof flag, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemset(b.flag, 0, sizeof(int32_t)));
+    if (flag) {
+        set_err(c, "pic2d3v_fieldsolve: non-finite field");
+        return PIC_ENONFINITE;
+    }
+    return PIC_OK;
+}

@@ -398,7 +398,6 @@ k2d_tile_scatter(Desc2D D, Tiles2D T, const uint32_t* __restrict__ Xq, const uin
     }
 }
 
-__device__ double tree_strided(double* red, double* vals, int stride, int64_t n);
 
 // Mixed kinetic diagnostics, independent of the particle order: per chunk of
 // 4096 held particles the exact integer sums of (vx^2 + vy^2) on a 2^-50 grid
@@ -521,8 +520,18 @@ __device__ void fft_lines(double* re, double* im, const double2* __restrict__ tw
     }
 }
 
+// Signed exact sum held in three 38-bit two's-complement limbs (modulo 2^114),
+// rounded once on the 2^-q grid.
+__device__ __forceinline__ double limbs_s_to_double(u64 a, u64 b, u64 c, int q) {
+    u128 v = (u128)a + ((u128)b << kLimbS) + ((u128)c << (2 * kLimbS));
+    v &= ((u128)1 << (3 * kLimbS)) - 1;
+    const i128 s = (v >> (3 * kLimbS - 1)) ? (i128)v - ((i128)1 << (3 * kLimbS)) : (i128)v;
+    return i128_to_double(s, q);
+}
+
 // Forward transform along x of R grid rows.  SRC 0: FP64 limbs, 1: mixed
-// quanta (both read and cleared for the next step), 2: a rho array.
+// quanta, 3: signed limbs (each read and cleared for the next step), 2: a
+// rho array.
 template <int SRC>
 __global__ void __launch_bounds__(kFFT2DThreads)
 k2d_rows_fwd(Desc2D D, u64* __restrict__ racc, const double* __restrict__ rho, double2* __restrict__ S,
@@ -548,6 +557,12 @@ k2d_rows_fwd(Desc2D D, u64* __restrict__ racc, const double* __restrict__ rho, d
             const u64 a = racc[k];
             racc[k] = 0;
             v = (double)a * D.rho_scale;
+        } else if (SRC == 3) {
+            const u64 a = racc[k], b = racc[G + k], c = racc[2 * G + k];
+            racc[k] = 0;
+            racc[G + k] = 0;
+            racc[2 * G + k] = 0;
+            v = limbs_s_to_double(a, b, c, D.q);
         } else {
             v = rho[k];
         }
@@ -592,8 +607,8 @@ k2d_cols(Desc2D D, const double2* __restrict__ S, double2* __restrict__ T1, doub
         const int c = e & (C - 1), j = e >> lc;
         const int i = i0 + c;
         double a = re[c * st + padi(j)], b = im[c * st + padi(j)];
-        const double kx = kTwoPi * (double)(i < NX / 2 ? i : i - NX);
-        const double ky = kTwoPi * (double)(j < NY / 2 ? j : j - NY);
+        const double kx = D.kxs * (double)(i < NX / 2 ? i : i - NX);
+        const double ky = D.kys * (double)(j < NY / 2 ? j : j - NY);
         double cc = 0.0, d = 0.0;
         if (i == 0 && j == 0) {
             a = 0.0;  // phi[1, 1] = 0
@@ -709,21 +724,6 @@ k2d_diag_parts(const T* __restrict__ VX, const T* __restrict__ VY, int64_t P, do
     }
 }
 
-// Canonical tree over n partials vals[stride*i] (consumed in place).
-__device__ double tree_strided(double* red, double* vals, int stride, int64_t n) {
-    while (n > kTreeChunk) {
-        const int64_t C = (n + kTreeChunk - 1) / kTreeChunk;
-        for (int64_t c = 0; c < C; ++c) {
-            const int64_t m = n - c * kTreeChunk < kTreeChunk ? n - c * kTreeChunk : kTreeChunk;
-            const double s = block_chunk_sum(red, m, [&](int64_t i) { return vals[stride * (c * kTreeChunk + i)]; });
-            if (threadIdx.x == 0) vals[stride * c] = s;
-            __syncthreads();
-        }
-        n = C;
-    }
-    return block_chunk_sum(red, n, [&](int64_t i) { return vals[stride * i]; });
-}
-
 __global__ void __launch_bounds__(256)
 k2d_diag_final(double* __restrict__ gpart, int64_t ng, double* __restrict__ ppart, int64_t np, int64_t G, int64_t P,
                double* __restrict__ out, int64_t* __restrict__ drow, int64_t cap) {
@@ -786,6 +786,8 @@ void launch2d_solve(const Buffers2D& b, int src, const double* rho_in, bool want
         k2d_rows_fwd<0><<<nr, kFFT2DThreads, lr1, s>>>(b.D, b.racc, nullptr, b.S, b.tw);
     else if (src == 1)
         k2d_rows_fwd<1><<<nr, kFFT2DThreads, lr1, s>>>(b.D, b.racc, nullptr, b.S, b.tw);
+    else if (src == 3)
+        k2d_rows_fwd<3><<<nr, kFFT2DThreads, lr1, s>>>(b.D, b.racc, nullptr, b.S, b.tw);
     else
         k2d_rows_fwd<2><<<nr, kFFT2DThreads, lr1, s>>>(b.D, nullptr, rho_in, b.S, b.tw);
     k2d_cols<<<nc, kFFT2DThreads, lc, s>>>(b.D, b.S, b.T1, b.T2, b.T3, b.tw, b.flag, want_phi ? 1 : 0);
@@ -863,7 +865,8 @@ int configure_kernels_2d() {
                          reinterpret_cast<const void*>(k2d_rows_inv),
                          reinterpret_cast<const void*>(k2d_rows_fwd<0>),
                          reinterpret_cast<const void*>(k2d_rows_fwd<1>),
-                         reinterpret_cast<const void*>(k2d_rows_fwd<2>)};
+                         reinterpret_cast<const void*>(k2d_rows_fwd<2>),
+                         reinterpret_cast<const void*>(k2d_rows_fwd<3>)};
     for (const void* f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     (void)hipGetLastError();
     return 0;

@@ -1,0 +1,775 @@
+// kernels_2d3v.hip -- electrostatic path of the structured 2D3V module
+// (src/PIC2D3V.jl): species of any charge and mass in an Lx x Ly box, the
+// NGP, area and B-spline (orders 0..5) shapes, Boris about a uniform B of any
+// direction, exact signed per-cell charge sums, update! of the gathered
+// field, the electrostatic diagnostics and the per-cell sort.
+//
+// FP64, evaluated in the reference's expression order (contraction off; the
+// gather's @muladd is an explicit fma).  Charge contributions are rounded to
+// the 2^-q grid and summed exactly in three 38-bit two's-complement limbs per
+// cell (device-scope integer atomics), so the charge does not depend on the
+// particle order or the schedule.  The field solve is the 2D transform of
+// kernels_2d.hip with wavenumbers 2pi/Lx, 2pi/Ly.
+#include "pic_device.h"
+#include "pic2d3v_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace pic {
+
+// ---------------------------------------------------------------------------
+// shapes: depositindicesfractions + gridinteractiontuple (:1106-1187)
+// SH = 0 NGP, 1 area, 2 + n B-spline of order n; SH_COUNT cells per axis.
+// ---------------------------------------------------------------------------
+template <int SH>
+struct ShapeCount {
+    static constexpr int value = SH == 0 ? 1 : (SH == 1 ? 2 : SH - 1);
+};
+
+__device__ __forceinline__ double q2(double t) { return t * t; }
+__device__ __forceinline__ double q3(double t) { return t * (t * t); }
+__device__ __forceinline__ double q4(double t) {
+    const double s = t * t;
+    return s * s;
+}
+__device__ __forceinline__ double q5(double t) {
+    const double s = t * t;
+    return t * (s * s);
+}
+
+// bspline(BSplineWeighting{n}(), x) (:1124-1162), written out left to right
+// with integer powers as LLVM's powi expands them.
+template <int n>
+__device__ __forceinline__ void bspline_w(double x, double* w) {
+    if constexpr (n == 0) {
+        w[0] = 1.0;
+    } else if constexpr (n == 1) {
+        w[0] = x;
+        w[1] = 1.0 - x;
+    } else if constexpr (n == 2) {
+        const double a = x - 1.5, b = x - 0.5, c = x + 0.5;
+        w[0] = 9.0 / 8.0 + 3.0 / 2.0 * a + 1.0 / 2.0 * q2(a);
+        w[1] = 3.0 / 4.0 - q2(b);
+        w[2] = 9.0 / 8.0 - 3.0 / 2.0 * c + 1.0 / 2.0 * q2(c);
+    } else if constexpr (n == 3) {
+        const double a = x - 2.0, b = x - 1.0, c = x, d = x + 1.0;
+        w[0] = 4.0 / 3.0 + 2.0 * a + q2(a) + 1.0 / 6.0 * q3(a);
+        w[1] = 2.0 / 3.0 - q2(b) - 1.0 / 2.0 * q3(b);
+        w[2] = 2.0 / 3.0 - q2(c) + 1.0 / 2.0 * q3(c);
+        w[3] = 4.0 / 3.0 - 2.0 * d + q2(d) - 1.0 / 6.0 * q3(d);
+    } else if constexpr (n == 4) {
+        const double a = x - 2.5, b = x - 1.5, c = x - 0.5, d = x + 0.5, e = x + 1.5;
+        w[0] = 625.0 / 384.0 + 125.0 / 48.0 * a + 25.0 / 16.0 * q2(a) + 5.0 / 12.0 * q3(a) + 1.0 / 24.0 * q4(a);
+        w[1] = 55.0 / 96.0 - 5.0 / 24.0 * b - 5.0 / 4.0 * q2(b) - 5.0 / 6.0 * q3(b) - 1.0 / 6.0 * q4(b);
+        w[2] = 115.0 / 192.0 - 5.0 / 8.0 * q2(c) + 1.0 / 4.0 * q4(c);
+        w[3] = 55.0 / 96.0 + 5.0 / 24.0 * d - 5.0 / 4.0 * q2(d) + 5.0 / 6.0 * q3(d) - 1.0 / 6.0 * q4(d);
+        w[4] = 625.0 / 384.0 - 125.0 / 48.0 * e + 25.0 / 16.0 * q2(e) - 5.0 / 12.0 * q3(e) + 1.0 / 24.0 * q4(e);
+    } else {
+        const double a = x - 3.0, b = x - 2.0, c = x - 1.0, d0 = x + 0.0, d1 = x - 0.0, e = x + 1.0, f = x + 2.0;
+        w[0] = 243.0 / 120.0 + 81.0 / 24.0 * a + 9.0 / 4.0 * q2(a) + 3.0 / 4.0 * q3(a) + 1.0 / 8.0 * q4(a) +
+               1.0 / 120.0 * q5(a);
+        w[1] = 17.0 / 40.0 - 5.0 / 8.0 * b - 7.0 / 4.0 * q2(b) - 5.0 / 4.0 * q3(b) - 3.0 / 8.0 * q4(b) -
+               1.0 / 24.0 * q5(b);
+        w[2] = 22.0 / 40.0 - 1.0 / 2.0 * q2(c) + 1.0 / 4.0 * q4(c) + 1.0 / 12.0 * q5(c);
+        w[3] = 22.0 / 40.0 - 1.0 / 2.0 * q2(d0) + 1.0 / 4.0 * q4(d1) - 1.0 / 12.0 * q5(d1);
+        w[4] = 17.0 / 40.0 + 5.0 / 8.0 * e - 7.0 / 4.0 * q2(e) + 5.0 / 4.0 * q3(e) - 3.0 / 8.0 * q4(e) +
+               1.0 / 24.0 * q5(e);
+        w[5] = 243.0 / 120.0 - 81.0 / 24.0 * f + 9.0 / 4.0 * q2(f) - 3.0 / 4.0 * q3(f) + 1.0 / 8.0 * q4(f) -
+               1.0 / 120.0 * q5(f);
+    }
+}
+
+// Cells (0-based, wrapped: the ghost layers hold periodic images) and weights
+// along one axis.  zN = z*NZ_Lz, i = ceil(zN), r = i - zN (:1108-1113).
+template <int SH>
+__device__ __forceinline__ void shape_w(double z, double NZ_Lz, int32_t N, int32_t* c, double* w) {
+    const double zN = z * NZ_Lz;
+    const double ci = ceil(zN);
+    const int64_t i = (ci > -4.0e18 && ci < 4.0e18) ? (int64_t)ci : 0;
+    const double r = (double)i - zN;
+    const int64_t m = N - 1;
+    if constexpr (SH == 0) {
+        c[0] = (int32_t)((i - 1) & m);
+        w[0] = 1.0;
+    } else if constexpr (SH == 1) {
+        c[0] = (int32_t)((i - 1) & m);
+        c[1] = (int32_t)(i & m);
+        w[0] = 1.0 - r;
+        w[1] = r;
+    } else {
+        constexpr int n = SH - 2;
+        int64_t j;
+        double x;
+        if constexpr (n & 1) {
+            j = i;
+            x = 1.0 - r;
+        } else {
+            const int q = r > 0.5;
+            j = i + q;
+            x = ((double)q + 0.5) - r;
+        }
+        bspline_w<n>(x, w);
+        const int64_t lo = j - n / 2;
+#pragma unroll
+        for (int k = 0; k <= n; ++k) c[k] = (int32_t)((lo + k - 1) & m);
+    }
+}
+
+// unimod(x, n) = x > n ? x - n : x > 0 ? x : x + n   (:9)
+__device__ __forceinline__ double unimod23(double x, double n) { return x > n ? x - n : (x > 0.0 ? x : x + n); }
+
+// Boris with E = (Ex, Ey, 0) and the uniform B.  BV 0: ElectrostaticBoris
+// (:39-54); BV 1: ElectromagneticBoris with Ez = 0 (:56-69).
+template <int BV>
+__device__ __forceinline__ void boris23(const Grid23& g, double q_m, double Ex, double Ey, double& vx, double& vy,
+                                        double& vz) {
+    double t1, t2, t3, e1, e2, e3, sc = 0.0, den = 0.0, qm2 = 0.0;
+    const double dt_2 = g.dt / 2.0;
+    if constexpr (BV == 0) {
+        t1 = g.B[0] * g.dt / 2.0;
+        t2 = g.B[1] * g.dt / 2.0;
+        t3 = g.B[2] * g.dt / 2.0;
+        const double tt = t1 * t1 + t2 * t2 + t3 * t3;
+        e1 = Ex * dt_2 * q_m;
+        e2 = Ey * dt_2 * q_m;
+        e3 = 0.0 * dt_2 * q_m;
+        qm2 = q_m * q_m;
+        den = 1.0 + qm2 * tt;
+    } else {
+        const double th = dt_2 * q_m;
+        t1 = g.B[0] * th;
+        t2 = g.B[1] * th;
+        t3 = g.B[2] * th;
+        sc = 2.0 / (1.0 + (t1 * t1 + t2 * t2 + t3 * t3));
+        e1 = Ex * th;
+        e2 = Ey * th;
+        e3 = 0.0 * th;
+    }
+    const double m1 = vx + e1, m2 = vy + e2, m3 = vz + e3;
+    const double c1 = m2 * t3 - m3 * t2, c2 = m3 * t1 - m1 * t3, c3 = m1 * t2 - m2 * t1;
+    const double u1 = m1 + c1, u2 = m2 + c2, u3 = m3 + c3;
+    const double d1 = u2 * t3 - u3 * t2, d2 = u3 * t1 - u1 * t3, d3 = u1 * t2 - u2 * t1;
+    double p1, p2, p3;
+    if constexpr (BV == 0) {
+        p1 = m1 + d1 * qm2 * 2.0 / den;
+        p2 = m2 + d2 * qm2 * 2.0 / den;
+        p3 = m3 + d3 * qm2 * 2.0 / den;
+    } else {
+        p1 = m1 + d1 * sc;
+        p2 = m2 + d2 * sc;
+        p3 = m3 + d3 * sc;
+    }
+    vx = p1 + e1;
+    vy = p2 + e2;
+    vz = p3 + e3;
+}
+
+// Signed exact charge: v = rint(c*2^q) as three 38-bit two's-complement limbs,
+// each added to its own u64 counter (no carries between counters).
+__device__ __forceinline__ void add_limbs_s(u64* __restrict__ racc, int64_t G, int64_t k, i128 v) {
+    const u128 u = (u128)v;
+    constexpr u64 M = (1ull << kLimbS) - 1;
+    const u64 l0 = (u64)u & M;
+    const u64 l1 = (u64)(u >> kLimbS) & M;
+    const u64 l2 = (u64)(u >> (2 * kLimbS)) & M;
+    if (l0) atomicAdd(racc + k, l0);
+    if (l1) atomicAdd(racc + G + k, l1);
+    if (l2) atomicAdd(racc + 2 * G + k, l2);
+}
+
+template <int SH>
+__device__ __forceinline__ void gather23(const Grid23& g, const double2* __restrict__ E, double x, double y, double& a,
+                                         double& b) {
+    constexpr int n = ShapeCount<SH>::value;
+    int32_t cx[n], cy[n];
+    double wx[n], wy[n];
+    shape_w<SH>(x, g.NX_Lx, g.NX, cx, wx);
+    shape_w<SH>(y, g.NY_Ly, g.NY, cy, wy);
+    a = 0.0;
+    b = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < n; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < n; ++ii) {
+            const double wxy = wx[ii] * wy[jj];
+            const double2 e = E[cx[ii] + (int64_t)g.NX * cy[jj]];
+            a = fma(e.x, wxy, a);
+            b = fma(e.y, wxy, b);
+        }
+}
+
+// deposit!(rho, shape, x, y, NX_Lx, NY_Ly, w) (:1249-1256): j outer, i inner,
+// contribution (wx*wy)*w.
+template <int SH>
+__device__ __forceinline__ void deposit23(const Grid23& g, u64* __restrict__ racc, double x, double y, double w) {
+    constexpr int n = ShapeCount<SH>::value;
+    const int64_t G = (int64_t)g.NX * g.NY;
+    int32_t cx[n], cy[n];
+    double wx[n], wy[n];
+    shape_w<SH>(x, g.NX_Lx, g.NX, cx, wx);
+    shape_w<SH>(y, g.NY_Ly, g.NY, cy, wy);
+#pragma unroll
+    for (int jj = 0; jj < n; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < n; ++ii) {
+            const double c = wx[ii] * wy[jj] * w;
+            add_limbs_s(racc, G, cx[ii] + (int64_t)g.NX * cy[jj], fix_round128(c, g.q));
+        }
+}
+
+// ---------------------------------------------------------------------------
+// the particle loop of loop! (:545-555) for one species
+// ---------------------------------------------------------------------------
+template <int SH, int BV>
+__global__ void __launch_bounds__(kPush23Threads)
+k23_push(Grid23 g, Species23 s, const double2* __restrict__ E, u64* __restrict__ racc) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < s.P; p += stride) {
+        double x = s.x[p], y = s.y[p], vx = s.vx[p], vy = s.vy[p], vz = s.vz[p];
+        double a, b;
+        gather23<SH>(g, E, x, y, a, b);
+        const double vx0 = vx, vy0 = vy;
+        boris23<BV>(g, s.q_m, a, b, vx, vy, vz);
+        x = unimod23(x + (vx0 + vx) / 2.0 * g.dt, g.Lx);
+        y = unimod23(y + (vy0 + vy) / 2.0 * g.dt, g.Ly);
+        s.x[p] = x;
+        s.y[p] = y;
+        s.vx[p] = vx;
+        s.vy[p] = vy;
+        s.vz[p] = vz;
+        deposit23<SH>(g, racc, x, y, s.qw);
+    }
+}
+
+template <int SH>
+__global__ void __launch_bounds__(kPush23Threads)
+k23_deposit_op(Grid23 g, const double* __restrict__ X, const double* __restrict__ Y, int64_t n, double w,
+               u64* __restrict__ racc) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride)
+        deposit23<SH>(g, racc, X[p], Y[p], w);
+}
+
+template <int SH>
+__global__ void __launch_bounds__(kPush23Threads)
+k23_gather_op(Grid23 g, const double2* __restrict__ E, const double* __restrict__ X, const double* __restrict__ Y,
+              int64_t n, double* __restrict__ ex, double* __restrict__ ey) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        double a, b;
+        gather23<SH>(g, E, X[p], Y[p], a, b);
+        ex[p] = a;
+        ey[p] = b;
+    }
+}
+
+template <int BV>
+__global__ void __launch_bounds__(kPush23Threads)
+k23_boris_op(Grid23 g, double q_m, double* __restrict__ VX, double* __restrict__ VY, double* __restrict__ VZ,
+             const double* __restrict__ ex, const double* __restrict__ ey, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        double vx = VX[p], vy = VY[p], vz = VZ[p];
+        boris23<BV>(g, q_m, ex[p], ey[p], vx, vy, vz);
+        VX[p] = vx;
+        VY[p] = vy;
+        VZ[p] = vz;
+    }
+}
+
+__global__ void __launch_bounds__(256) k23_limbs_to_rho(int64_t G, int q, u64* __restrict__ racc,
+                                                        double* __restrict__ rho) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < G; k += (int64_t)gridDim.x * blockDim.x) {
+        const u64 a = racc[k], b = racc[G + k], c = racc[2 * G + k];
+        racc[k] = 0;
+        racc[G + k] = 0;
+        racc[2 * G + k] = 0;
+        u128 v = (u128)a + ((u128)b << kLimbS) + ((u128)c << (2 * kLimbS));
+        v &= ((u128)1 << (3 * kLimbS)) - 1;
+        const i128 sv = (v >> (3 * kLimbS - 1)) ? (i128)v - ((i128)1 << (3 * kLimbS)) : (i128)v;
+        rho[k] = i128_to_double(sv, q);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// update! (:294-297) and the binned fields of diagnose! (:1323-1327)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k23_update(Grid23 g, Diag23 d, const double* __restrict__ Ex, const double* __restrict__ Ey,
+           const double* __restrict__ phi, double2* __restrict__ E, int accumulate) {
+    const int64_t G = (int64_t)g.NX * g.NY;
+    const int64_t b = *d.t / d.ntskip;
+    const bool bin = b < d.cap;
+    const double nt = (double)d.ntskip;
+    const int32_t gm = d.ngskip - 1;
+    const int lg = __ffs(d.ngskip) - 1;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < G; k += (int64_t)gridDim.x * blockDim.x) {
+        const double ex = Ex[k], ey = Ey[k];
+        if (accumulate) {
+            const double2 e = E[k];
+            E[k] = make_double2(e.x + ex, e.y + ey);
+        } else {
+            E[k] = make_double2(0.0 + ex, 0.0 + ey);
+        }
+        const int32_t i = (int32_t)(k & (g.NX - 1)), j = (int32_t)(k >> g.lx);
+        if (bin && ((i | j) & gm) == 0) {
+            const int64_t o = b * (int64_t)d.NXd * d.NYd + (i >> lg) + (int64_t)d.NXd * (j >> lg);
+            d.Exs[o] = d.Exs[o] + ex / nt;
+            d.Eys[o] = d.Eys[o] + ey / nt;
+            d.phis[o] = d.phis[o] + phi[k] / nt;
+        }
+    }
+}
+
+__global__ void k23_tick(int64_t* t) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *t += 1;
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics (:1301-1321) with the canonical tree
+// ---------------------------------------------------------------------------
+// mean(abs2, Exy) over the ghosted 2 x NXp x NYp array: element e is
+// component e & 1 of padded cell e >> 1, whose periodic image is interior.
+__global__ void __launch_bounds__(256)
+k23_diag_field(Grid23 g, Diag23 d, const double2* __restrict__ E, int64_t nf) {
+    __shared__ double red[kTreeLanes + 8];
+    const int32_t NXp = g.NX + 2 * d.buffer;
+    const int64_t s = (int64_t)blockIdx.x * kTreeChunk;
+    const int64_t n = nf - s < kTreeChunk ? nf - s : kTreeChunk;
+    const double r = block_chunk_sum(red, n, [&](int64_t i) {
+        const int64_t e = s + i;
+        const int32_t q = (int32_t)(e >> 1);
+        const int32_t jp = q / NXp, ip = q - jp * NXp;
+        const int32_t ci = (ip - d.buffer) & (g.NX - 1), cj = (jp - d.buffer) & (g.NY - 1);
+        const double2 v = E[ci + (int64_t)g.NX * cj];
+        const double a = (e & 1) ? v.y : v.x;
+        return a * a;
+    });
+    if (threadIdx.x == 0) d.fpart[blockIdx.x] = r;
+}
+
+// sum(abs2, velocities(s)) over the 3P entries in xyv's column order
+__global__ void __launch_bounds__(256)
+k23_diag_kinetic(Species23 s, double* __restrict__ kpart) {
+    __shared__ double red[kTreeLanes + 8];
+    const int64_t n3 = 3 * s.P;
+    const int64_t b0 = (int64_t)blockIdx.x * kTreeChunk;
+    const int64_t n = n3 - b0 < kTreeChunk ? n3 - b0 : kTreeChunk;
+    const double r = block_chunk_sum(red, n, [&](int64_t i) {
+        const int64_t e = b0 + i;
+        const int64_t p = e / 3;
+        const int c = (int)(e - 3 * p);
+        const double v = c == 0 ? s.vx[p] : (c == 1 ? s.vy[p] : s.vz[p]);
+        return v * v;
+    });
+    if (threadIdx.x == 0) kpart[blockIdx.x] = r;
+}
+
+// momentum and characteristic momentum: six sums over P
+__global__ void __launch_bounds__(256)
+k23_diag_momentum(Species23 s, double* __restrict__ mpart) {
+    __shared__ double red[3 * kTreeLanes + 8];
+    const int64_t b0 = (int64_t)blockIdx.x * kTreeChunk;
+    const int64_t n = s.P - b0 < kTreeChunk ? s.P - b0 : kTreeChunk;
+    double a = 0.0, b = 0.0, c = 0.0, e = 0.0, f = 0.0, h = 0.0;
+    if (threadIdx.x < kTreeLanes)
+        for (int64_t i = threadIdx.x; i < n; i += kTreeLanes) {
+            const double vx = s.vx[b0 + i], vy = s.vy[b0 + i], vz = s.vz[b0 + i];
+            a = a + vx;
+            b = b + vy;
+            c = c + vz;
+            e = e + fabs(vx);
+            f = f + fabs(vy);
+            h = h + fabs(vz);
+        }
+    tree256x3<false>(red, a, b, c);
+    tree256x3<false>(red, e, f, h);
+    if (threadIdx.x == 0) {
+        double* o = mpart + 6 * blockIdx.x;
+        o[0] = a;
+        o[1] = b;
+        o[2] = c;
+        o[3] = e;
+        o[4] = f;
+        o[5] = h;
+    }
+}
+
+struct DiagSpecies23 {
+    int64_t nk, nm;  // chunks of the kinetic and momentum sums
+    double mass, weight;
+};
+struct DiagFinal23 {
+    int ns;
+    int64_t nf, nfc;  // field entries, chunks
+    DiagSpecies23 s[kMaxSpecies23];
+};
+
+__global__ void __launch_bounds__(256) k23_diag_final(Diag23 d, DiagFinal23 f) {
+    __shared__ double red[kTreeLanes + 8];
+    const double fe = tree_strided(red, d.fpart, 1, f.nfc) / (double)f.nf / 2.0;
+    double ke = 0.0, pm[3] = {0.0, 0.0, 0.0}, pc[3] = {0.0, 0.0, 0.0};
+    for (int s = 0; s < f.ns; ++s) {
+        const DiagSpecies23& q = f.s[s];
+        const double kes = tree_strided(red, d.kpart[s], 1, q.nk) * q.mass / 2.0 * q.weight;
+        double ms[3], cs[3];
+        for (int c = 0; c < 3; ++c) {
+            ms[c] = tree_strided(red, d.mpart[s] + c, 6, q.nm) * (q.mass * q.weight);
+            cs[c] = tree_strided(red, d.mpart[s] + 3 + c, 6, q.nm) * (q.mass * q.weight);
+        }
+        if (s == 0) {
+            ke = kes;
+            for (int c = 0; c < 3; ++c) {
+                pm[c] = ms[c];
+                pc[c] = cs[c];
+            }
+        } else {
+            ke = ke + kes;
+            for (int c = 0; c < 3; ++c) {
+                pm[c] = pm[c] + ms[c];
+                pc[c] = pc[c] + cs[c];
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        const int64_t b = *d.t / d.ntskip;
+        if (b < d.cap) {
+            double* o = d.rows + 8 * b;
+            o[0] = ke;
+            o[1] = fe;
+            for (int c = 0; c < 3; ++c) {
+                o[2 + c] = pm[c];
+                o[5 + c] = pc[c];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// xyv (5 x P, column-major) <-> SoA
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k23_unpack(const double* __restrict__ xyv, Species23 s) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < s.P; p += (int64_t)gridDim.x * blockDim.x) {
+        const double* q = xyv + 5 * p;
+        s.x[p] = q[0];
+        s.y[p] = q[1];
+        s.vx[p] = q[2];
+        s.vy[p] = q[3];
+        s.vz[p] = q[4];
+    }
+}
+
+__global__ void __launch_bounds__(256) k23_pack(Species23 s, double* __restrict__ xyv) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < s.P; p += (int64_t)gridDim.x * blockDim.x) {
+        double* q = xyv + 5 * p;
+        q[0] = s.x[p];
+        q[1] = s.y[p];
+        q[2] = s.vx[p];
+        q[3] = s.vy[p];
+        q[4] = s.vz[p];
+    }
+}
+
+__global__ void __launch_bounds__(256) k23_pair(const double* __restrict__ Ex, const double* __restrict__ Ey,
+                                                double2* __restrict__ E, int64_t G) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < G; k += (int64_t)gridDim.x * blockDim.x)
+        E[k] = make_double2(Ex[k], Ey[k]);
+}
+
+__global__ void __launch_bounds__(256) k23_split(const double2* __restrict__ E, double* __restrict__ Ex,
+                                                 double* __restrict__ Ey, int64_t G) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < G; k += (int64_t)gridDim.x * blockDim.x) {
+        const double2 e = E[k];
+        if (Ex) Ex[k] = e.x;
+        if (Ey) Ey[k] = e.y;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// sort!(s, dx, dy) (:210-215): bucket by cell (ceil(x/dx), ceil(y/dy)) with x
+// major, then order each bucket by (vx under isless, index).  Every
+// particle's final slot is its bucket start plus the number of bucket members
+// before it in that order, so the result is the stable sortperm whatever the
+// order of the scatter.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t ceil_index(double a) {
+    const double c = ceil(a);
+    return (c > -4.0e18 && c < 4.0e18) ? (int64_t)c : 0;
+}
+
+// isless on Float64 as an unsigned key: -0.0 < 0.0, every NaN last and equal
+__device__ __forceinline__ u64 order_key(double v) {
+    if (v != v) return ~0ull;
+    const u64 b = (u64)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__global__ void __launch_bounds__(256)
+k23_sort_extent(Species23 s, double dx, double dy, long long* __restrict__ ext) {
+    __shared__ long long red[4][256];
+    long long a = LLONG_MAX, b = LLONG_MIN, c = LLONG_MAX, e = LLONG_MIN;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < s.P; p += (int64_t)gridDim.x * blockDim.x) {
+        const long long cx = ceil_index(s.x[p] / dx), cy = ceil_index(s.y[p] / dy);
+        a = cx < a ? cx : a;
+        b = cx > b ? cx : b;
+        c = cy < c ? cy : c;
+        e = cy > e ? cy : e;
+    }
+    const int t = threadIdx.x;
+    red[0][t] = a;
+    red[1][t] = b;
+    red[2][t] = c;
+    red[3][t] = e;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+            red[0][t] = red[0][t + h] < red[0][t] ? red[0][t + h] : red[0][t];
+            red[1][t] = red[1][t + h] > red[1][t] ? red[1][t + h] : red[1][t];
+            red[2][t] = red[2][t + h] < red[2][t] ? red[2][t + h] : red[2][t];
+            red[3][t] = red[3][t + h] > red[3][t] ? red[3][t + h] : red[3][t];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        atomicMin(ext + 0, red[0][0]);
+        atomicMax(ext + 1, red[1][0]);
+        atomicMin(ext + 2, red[2][0]);
+        atomicMax(ext + 3, red[3][0]);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k23_sort_count(Species23 s, double dx, double dy, const long long* __restrict__ ext, int64_t ry,
+               uint32_t* __restrict__ bucket, u64* __restrict__ vkey, uint32_t* __restrict__ count) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < s.P; p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t cx = ceil_index(s.x[p] / dx) - ext[0], cy = ceil_index(s.y[p] / dy) - ext[2];
+        const uint32_t b = (uint32_t)(cx * ry + cy);
+        bucket[p] = b;
+        vkey[p] = order_key(s.vx[p]);
+        atomicAdd(count + b, 1u);
+    }
+}
+
+// Exclusive scan of count[0..nb) into start[0..nb] and the largest count, by
+// one block in tiles of 4096; clears the scatter cursors.
+__global__ void __launch_bounds__(1024)
+k23_sort_scan(const uint32_t* __restrict__ count, uint32_t* __restrict__ start, uint32_t* __restrict__ cursor,
+              int64_t nb, uint32_t* __restrict__ maxcount) {
+    __shared__ uint32_t sums[1024];
+    __shared__ uint32_t carry_s;
+    const int t = threadIdx.x;
+    uint32_t carry = 0, mx = 0;
+    for (int64_t base = 0; base < nb; base += 4096) {
+        uint32_t v[4], loc = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int64_t i = base + 4 * t + k;
+            v[k] = i < nb ? count[i] : 0u;
+            mx = v[k] > mx ? v[k] : mx;
+            loc += v[k];
+        }
+        sums[t] = loc;
+        __syncthreads();
+        for (int h = 1; h < 1024; h <<= 1) {  // inclusive Hillis-Steele scan of the thread sums
+            const uint32_t a = t >= h ? sums[t - h] : 0u;
+            __syncthreads();
+            sums[t] += a;
+            __syncthreads();
+        }
+        uint32_t run = carry + (t > 0 ? sums[t - 1] : 0u);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int64_t i = base + 4 * t + k;
+            if (i < nb) {
+                start[i] = run;
+                cursor[i] = 0u;
+            }
+            run += v[k];
+        }
+        if (t == 1023) carry_s = carry + sums[1023];
+        __syncthreads();
+        carry = carry_s;
+        __syncthreads();
+    }
+    if (t == 0) start[nb] = carry;
+    sums[t] = mx;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (t < h) sums[t] = sums[t + h] > sums[t] ? sums[t + h] : sums[t];
+        __syncthreads();
+    }
+    if (t == 0) *maxcount = sums[0];
+}
+
+__global__ void __launch_bounds__(256)
+k23_sort_scatter(int64_t P, const uint32_t* __restrict__ bucket, const u64* __restrict__ vkey,
+                 const uint32_t* __restrict__ start, uint32_t* __restrict__ cursor, u64* __restrict__ skey,
+                 uint32_t* __restrict__ sidx, uint32_t* __restrict__ sbucket) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = bucket[p];
+        const uint32_t slot = start[b] + atomicAdd(cursor + b, 1u);
+        skey[slot] = vkey[p];
+        sidx[slot] = (uint32_t)p;
+        sbucket[slot] = b;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k23_sort_rank(int64_t P, const uint32_t* __restrict__ start, const u64* __restrict__ skey,
+              const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ sbucket, uint32_t* __restrict__ perm) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < P; e += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = sbucket[e];
+        const uint32_t s0 = start[b], s1 = start[b + 1];
+        const u64 k = skey[e];
+        const uint32_t i = sidx[e];
+        uint32_t rank = 0;
+        for (uint32_t f = s0; f < s1; ++f) {
+            const u64 kf = skey[f];
+            const uint32_t jf = sidx[f];
+            rank += (kf < k || (kf == k && jf < i)) ? 1u : 0u;
+        }
+        perm[s0 + rank] = i;
+    }
+}
+
+struct Dst23 {
+    double* a[5];
+};
+
+__global__ void __launch_bounds__(256) k23_sort_apply(Species23 s, const uint32_t* __restrict__ perm, Dst23 d) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < s.P; k += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = perm[k];
+        d.a[0][k] = s.x[p];
+        d.a[1][k] = s.y[p];
+        d.a[2][k] = s.vx[p];
+        d.a[3][k] = s.vy[p];
+        d.a[4][k] = s.vz[p];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static unsigned blocks_for(int64_t n, int threads) {
+    const int64_t b = (n + threads - 1) / threads;
+    return (unsigned)(b < 16384 ? (b > 0 ? b : 1) : 16384);
+}
+
+#define PIC23_SHAPES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+
+void launch23_push(const Grid23& g, const Species23& s, const double2* E, unsigned long long* racc, hipStream_t st) {
+    const unsigned nb = blocks_for(s.P, kPush23Threads);
+#define PIC23_PUSH(SH)                                                                                 \
+    case SH:                                                                                           \
+        if (g.boris)                                                                                   \
+            k23_push<SH, 1><<<nb, kPush23Threads, 0, st>>>(g, s, E, racc);                             \
+        else                                                                                           \
+            k23_push<SH, 0><<<nb, kPush23Threads, 0, st>>>(g, s, E, racc);                             \
+        break;
+    switch (s.shape) { PIC23_SHAPES(PIC23_PUSH) }
+#undef PIC23_PUSH
+}
+
+void launch23_deposit_op(const Grid23& g, int shape, const double* x, const double* y, int64_t n, double w,
+                         unsigned long long* racc, hipStream_t st) {
+    const unsigned nb = blocks_for(n, kPush23Threads);
+#define PIC23_DEP(SH)                                                                \
+    case SH:                                                                         \
+        k23_deposit_op<SH><<<nb, kPush23Threads, 0, st>>>(g, x, y, n, w, racc);      \
+        break;
+    switch (shape) { PIC23_SHAPES(PIC23_DEP) }
+#undef PIC23_DEP
+}
+
+void launch23_limbs_to_rho(const Grid23& g, unsigned long long* racc, double* rho, hipStream_t st) {
+    const int64_t G = (int64_t)g.NX * g.NY;
+    k23_limbs_to_rho<<<blocks_for(G, 256), 256, 0, st>>>(G, g.q, racc, rho);
+}
+
+void launch23_gather_op(const Grid23& g, int shape, const double2* E, const double* x, const double* y, int64_t n,
+                        double* ex, double* ey, hipStream_t st) {
+    const unsigned nb = blocks_for(n, kPush23Threads);
+#define PIC23_GAT(SH)                                                                \
+    case SH:                                                                         \
+        k23_gather_op<SH><<<nb, kPush23Threads, 0, st>>>(g, E, x, y, n, ex, ey);     \
+        break;
+    switch (shape) { PIC23_SHAPES(PIC23_GAT) }
+#undef PIC23_GAT
+}
+
+void launch23_boris_op(const Grid23& g, double q_m, double* vx, double* vy, double* vz, const double* ex,
+                       const double* ey, int64_t n, hipStream_t st) {
+    const unsigned nb = blocks_for(n, kPush23Threads);
+    if (g.boris)
+        k23_boris_op<1><<<nb, kPush23Threads, 0, st>>>(g, q_m, vx, vy, vz, ex, ey, n);
+    else
+        k23_boris_op<0><<<nb, kPush23Threads, 0, st>>>(g, q_m, vx, vy, vz, ex, ey, n);
+}
+
+void launch23_update(const Grid23& g, const Diag23& d, const Buffers2D& b, double2* E, int accumulate,
+                     hipStream_t st) {
+    const int64_t G = (int64_t)g.NX * g.NY;
+    k23_update<<<blocks_for(G, 256), 256, 0, st>>>(g, d, b.Ex, b.Ey, b.phi, E, accumulate);
+}
+
+void launch23_tick(int64_t* t, hipStream_t st) { k23_tick<<<1, 64, 0, st>>>(t); }
+
+void launch23_diag(const Grid23& g, const Diag23& d, const double2* E, const Species23* sp, int ns, hipStream_t st) {
+    const int64_t nf = 2 * (int64_t)(g.NX + 2 * d.buffer) * (g.NY + 2 * d.buffer);
+    DiagFinal23 f{};
+    f.ns = ns;
+    f.nf = nf;
+    f.nfc = (nf + kTreeChunk - 1) / kTreeChunk;
+    k23_diag_field<<<(unsigned)f.nfc, 256, 0, st>>>(g, d, E, nf);
+    for (int s = 0; s < ns; ++s) {
+        f.s[s].nk = (3 * sp[s].P + kTreeChunk - 1) / kTreeChunk;
+        f.s[s].nm = (sp[s].P + kTreeChunk - 1) / kTreeChunk;
+        f.s[s].mass = sp[s].mass;
+        f.s[s].weight = sp[s].weight;
+        k23_diag_kinetic<<<(unsigned)f.s[s].nk, 256, 0, st>>>(sp[s], d.kpart[s]);
+        k23_diag_momentum<<<(unsigned)f.s[s].nm, 256, 0, st>>>(sp[s], d.mpart[s]);
+    }
+    k23_diag_final<<<1, 256, 0, st>>>(d, f);
+}
+
+void launch23_unpack(const double* xyv, const Species23& s, hipStream_t st) {
+    k23_unpack<<<blocks_for(s.P, 256), 256, 0, st>>>(xyv, s);
+}
+
+void launch23_pack(const Species23& s, double* xyv, hipStream_t st) {
+    k23_pack<<<blocks_for(s.P, 256), 256, 0, st>>>(s, xyv);
+}
+
+void launch23_pair_field(const double* Ex, const double* Ey, double2* E, int64_t G, hipStream_t st) {
+    k23_pair<<<blocks_for(G, 256), 256, 0, st>>>(Ex, Ey, E, G);
+}
+
+void launch23_split_field(const double2* E, double* Ex, double* Ey, int64_t G, hipStream_t st) {
+    k23_split<<<blocks_for(G, 256), 256, 0, st>>>(E, Ex, Ey, G);
+}
+
+void launch23_sort_extent(const Species23& s, double dx, double dy, Sort23& w, hipStream_t st) {
+    k23_sort_extent<<<blocks_for(s.P, 256), 256, 0, st>>>(s, dx, dy, reinterpret_cast<long long*>(w.ext));
+}
+
+void launch23_sort_buckets(const Species23& s, double dx, double dy, int64_t ry, int64_t nb, Sort23& w,
+                           hipStream_t st) {
+    k23_sort_count<<<blocks_for(s.P, 256), 256, 0, st>>>(s, dx, dy, reinterpret_cast<const long long*>(w.ext), ry,
+                                                         w.bucket, w.vkey, w.count);
+    k23_sort_scan<<<1, 1024, 0, st>>>(w.count, w.start, w.cursor, nb, w.maxcount);
+}
+
+void launch23_sort_order(const Species23& s, int64_t nb, Sort23& w, hipStream_t st) {
+    (void)nb;
+    k23_sort_scatter<<<blocks_for(s.P, 256), 256, 0, st>>>(s.P, w.bucket, w.vkey, w.start, w.cursor, w.skey, w.sidx,
+                                                           w.sbucket);
+    k23_sort_rank<<<blocks_for(s.P, 256), 256, 0, st>>>(s.P, w.start, w.skey, w.sidx, w.sbucket, w.perm);
+}
+
+void launch23_sort_apply(const Species23& s, const uint32_t* perm, double* const* dst, hipStream_t st) {
+    Dst23 d;
+    for (int k = 0; k < 5; ++k) d.a[k] = dst[k];
+    k23_sort_apply<<<blocks_for(s.P, 256), 256, 0, st>>>(s, perm, d);
+}
+
+}  // namespace pic

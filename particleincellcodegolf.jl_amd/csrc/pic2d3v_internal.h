@@ -1,0 +1,91 @@
+// pic2d3v_internal.h -- descriptors and launchers of the electrostatic path
+// of the structured 2D3V module, shared by the host runtime (capi2d3v.cpp)
+// and the kernels (kernels_2d3v.hip).
+#pragma once
+#include "pic2d_internal.h"
+
+namespace pic {
+
+constexpr int kMaxSpecies23 = 4;
+constexpr int kPush23Threads = 256;
+constexpr int64_t kMaxP23 = int64_t(1) << 26;  // all species together: 2^26 contributions per cell
+constexpr int64_t kMaxSortBuckets = int64_t(1) << 24;
+constexpr int64_t kMaxSortCell = 16384;        // particles one cell may hold in a sort
+
+// Grid, box and the step's constants.
+struct Grid23 {
+    int32_t NX, NY, lx, ly;
+    double Lx, Ly;
+    double NX_Lx, NY_Ly;  // NX/Lx, NY/Ly (GridParameters, src/PIC2D3V.jl:234-240)
+    double dt;
+    double B[3];
+    int32_t q;            // fraction bits of the signed exact charge sums
+    int32_t boris;        // 0: ElectrostaticBoris, 1: ElectromagneticBoris form
+};
+
+// One species on the device (SoA).
+struct Species23 {
+    int64_t P;
+    int32_t shape;
+    double q_m;           // charge/mass
+    double qw;            // (charge*weight)/dV
+    double mass, weight;
+    double *x, *y, *vx, *vy, *vz;
+};
+
+// Diagnostics and binned-field state.
+struct Diag23 {
+    int32_t buffer, ntskip, ngskip, NXd, NYd;
+    int64_t* t;           // device step counter
+    double* rows;         // 8 per bin
+    double *Exs, *Eys, *phis;  // NXd*NYd per bin
+    int64_t cap;          // bins allocated
+    double* fpart;        // field-energy chunk partials
+    double* kpart[kMaxSpecies23];  // kinetic chunk partials (3P entries per species)
+    double* mpart[kMaxSpecies23];  // momentum chunk partials, 6 per chunk
+};
+
+// one step's particle loop for one species: gather, Boris, move, deposit
+void launch23_push(const Grid23& g, const Species23& s, const double2* E, unsigned long long* racc, hipStream_t st);
+// update! of the gathered field and the binned fields
+void launch23_update(const Grid23& g, const Diag23& d, const Buffers2D& b, double2* E, int accumulate,
+                     hipStream_t st);
+// the diagnostics row of bin t/ntskip
+void launch23_diag(const Grid23& g, const Diag23& d, const double2* E, const Species23* sp, int ns, hipStream_t st);
+// t += 1
+void launch23_tick(int64_t* t, hipStream_t st);
+// xyv (5 x P, column-major) <-> SoA
+void launch23_unpack(const double* xyv, const Species23& s, hipStream_t st);
+void launch23_pack(const Species23& s, double* xyv, hipStream_t st);
+// operators
+void launch23_deposit_op(const Grid23& g, int shape, const double* x, const double* y, int64_t n, double w,
+                         unsigned long long* racc, hipStream_t st);
+void launch23_limbs_to_rho(const Grid23& g, unsigned long long* racc, double* rho, hipStream_t st);
+void launch23_gather_op(const Grid23& g, int shape, const double2* E, const double* x, const double* y, int64_t n,
+                        double* ex, double* ey, hipStream_t st);
+void launch23_boris_op(const Grid23& g, double q_m, double* vx, double* vy, double* vz, const double* ex,
+                       const double* ey, int64_t n, hipStream_t st);
+void launch23_pair_field(const double* Ex, const double* Ey, double2* E, int64_t G, hipStream_t st);
+void launch23_split_field(const double2* E, double* Ex, double* Ey, int64_t G, hipStream_t st);
+
+// sort!(species, dx, dy)
+struct Sort23 {
+    int64_t* ext;          // [4]: min cx, max cx, min cy, max cy
+    uint32_t* bucket;      // [P]
+    unsigned long long* vkey;  // [P]
+    uint32_t* count;       // [nb]
+    uint32_t* start;       // [nb + 1]
+    uint32_t* cursor;      // [nb]
+    unsigned long long* skey;  // [P] slot keys
+    uint32_t* sidx;        // [P] slot particle
+    uint32_t* sbucket;     // [P] slot bucket
+    uint32_t* perm;        // [P]
+    uint32_t* maxcount;    // [1]
+};
+void launch23_sort_extent(const Species23& s, double dx, double dy, Sort23& w, hipStream_t st);
+void launch23_sort_buckets(const Species23& s, double dx, double dy, int64_t ry, int64_t nb, Sort23& w,
+                           hipStream_t st);
+void launch23_sort_order(const Species23& s, int64_t nb, Sort23& w, hipStream_t st);
+void launch23_sort_apply(const Species23& s, const uint32_t* perm, double* const* dst, hipStream_t st);
+
+}  // namespace pic

@@ -10,6 +10,8 @@ constexpr int kLine2D = 512;        // transform points per block (rows*NX or co
 constexpr int kFFT2DThreads = 256;
 constexpr int kPush2DThreads = 256;
 constexpr int kLimb2D = 42;         // FP64 exact deposit: 3 limbs of 42 bits (P <= 2^22)
+constexpr int kLimbS = 38;          // signed exact deposit: 3 limbs of 38 bits, two's complement
+                                    // (114 bits; up to 2^26 contributions per cell)
 constexpr int kQuanta2D = 20;       // mixed: area weight quanta per axis, 2^-20 of a cell
 constexpr int64_t kMaxP2D64 = int64_t(1) << 22;
 constexpr int64_t kMaxP2D32 = int64_t(1) << 24;
@@ -36,6 +38,7 @@ struct Desc2D {
     int32_t sx, sy;          // mixed: 32 - lx, 32 - ly (fixed-point cell shift)
     double w;                // particle weight
     double rho_scale;        // mixed: charge of one weight quantum, w * 2^-40
+    double kxs, kys;         // wavenumber scales: kx = kxs*m (2pi/Lx), ky = kys*m
 };
 
 // Device buffers of one 2D context.  Grids are NX*NY with x fastest.
@@ -91,7 +94,7 @@ inline size_t cols_lds_bytes(int NX, int NY) {
 void launch2d_push(const Buffers2D& b, hipStream_t s);
 // mixed: regroup the held particles by tile into the _alt buffers
 void launch2d_tile_sort(const Buffers2D& b, hipStream_t s);
-// src: 0 = FP64 limbs, 1 = mixed quanta, 2 = rho array (b.phi reused as input)
+// src: 0 = FP64 limbs, 1 = mixed quanta, 2 = rho array, 3 = signed 38-bit limbs
 void launch2d_solve(const Buffers2D& b, int src, const double* rho_in, bool want_phi, hipStream_t s);
 // diagnostics from b.Ex/b.Ey and the particle velocities, appended at *drow
 void launch2d_diag(const Buffers2D& b, hipStream_t s);

@@ -362,6 +362,21 @@ __device__ __forceinline__ double block_chunk_sum(double* red, int64_t n, F&& el
 }
 
 // NaN-propagating max of |x| over n values (order-independent).
+// Canonical tree over n partials vals[stride*i] (consumed in place).
+__device__ inline double tree_strided(double* red, double* vals, int stride, int64_t n) {
+    while (n > kTreeChunk) {
+        const int64_t C = (n + kTreeChunk - 1) / kTreeChunk;
+        for (int64_t c = 0; c < C; ++c) {
+            const int64_t m = n - c * kTreeChunk < kTreeChunk ? n - c * kTreeChunk : kTreeChunk;
+            const double s = block_chunk_sum(red, m, [&](int64_t i) { return vals[stride * (c * kTreeChunk + i)]; });
+            if (threadIdx.x == 0) vals[stride * c] = s;
+            __syncthreads();
+        }
+        n = C;
+    }
+    return block_chunk_sum(red, n, [&](int64_t i) { return vals[stride * i]; });
+}
+
 template <typename F>
 __device__ __forceinline__ double block_maxabs(double* red, int64_t n, F&& elem) {
     const int t = threadIdx.x;

@@ -46,6 +46,18 @@ EXPORTS_1D2V = [
 ]
 
 
+# every symbol include/pic2d3v.h declares
+EXPORTS_2D3V = [
+    "pic2d3v_create", "pic2d3v_destroy", "pic2d3v_last_error", "pic2d3v_upload", "pic2d3v_download",
+    "pic2d3v_reset", "pic2d3v_sort", "pic2d3v_run", "pic2d3v_fields", "pic2d3v_field", "pic2d3v_deposit",
+    "pic2d3v_gather", "pic2d3v_boris", "pic2d3v_fieldsolve",
+]
+MAX_SPECIES_2D3V = 4
+NGP_2D3V, AREA_2D3V, BSPLINE0_2D3V = 0, 1, 2
+FIELD_ACCUMULATE, FIELD_REPLACE = 0, 1
+BORIS_REFERENCE, BORIS_TEXTBOOK = 0, 1
+
+
 class LibraryMissing(RuntimeError):
     pass
 
@@ -80,6 +92,23 @@ class pic1d2v_config(ctypes.Structure):
         ("P", ctypes.c_int64), ("q", ctypes.c_double * 2), ("q_m", ctypes.c_double * 2),
         ("mass", ctypes.c_double * 2), ("w", ctypes.c_double), ("n0", ctypes.c_double), ("B0", ctypes.c_double),
         ("dt", ctypes.c_double), ("device", ctypes.c_int32),
+    ]
+
+
+class pic2d3v_species(ctypes.Structure):
+    _fields_ = [
+        ("P", ctypes.c_int64), ("shape", ctypes.c_int32), ("charge", ctypes.c_double), ("mass", ctypes.c_double),
+        ("weight", ctypes.c_double),
+    ]
+
+
+class pic2d3v_config(ctypes.Structure):
+    _fields_ = [
+        ("NX", ctypes.c_int32), ("NY", ctypes.c_int32), ("Lx", ctypes.c_double), ("Ly", ctypes.c_double),
+        ("dt", ctypes.c_double), ("B0", ctypes.c_double * 3), ("buffer", ctypes.c_int32),
+        ("ntskip", ctypes.c_int32), ("ngskip", ctypes.c_int32), ("field_update", ctypes.c_int32),
+        ("boris", ctypes.c_int32), ("nspecies", ctypes.c_int32),
+        ("species", pic2d3v_species * MAX_SPECIES_2D3V), ("device", ctypes.c_int32),
     ]
 
 
@@ -155,6 +184,21 @@ def load():
         "pic1d2v_gather": [vp, vp, vp, i64, vp],
         "pic1d2v_boris": [vp, vp, vp, vp, i64, d, d, d],
     })
+    sig.update({
+        "pic2d3v_create": [ctypes.POINTER(vp), ctypes.POINTER(pic2d3v_config)],
+        "pic2d3v_destroy": [vp],
+        "pic2d3v_upload": [vp, i32, vp],
+        "pic2d3v_download": [vp, i32, vp],
+        "pic2d3v_reset": [vp],
+        "pic2d3v_sort": [vp, i32, d, d, vp],
+        "pic2d3v_run": [vp, i64, vp],
+        "pic2d3v_fields": [vp, vp, vp, vp, i64],
+        "pic2d3v_field": [vp, vp, vp],
+        "pic2d3v_deposit": [vp, i32, vp, vp, i64, d, vp],
+        "pic2d3v_gather": [vp, i32, vp, vp, vp, vp, i64, vp, vp],
+        "pic2d3v_boris": [vp, vp, vp, vp, vp, vp, i64, d],
+        "pic2d3v_fieldsolve": [vp, vp, vp, vp, vp],
+    })
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -165,6 +209,8 @@ def load():
     L.pic2d_last_error.restype = ctypes.c_char_p
     L.pic1d2v_last_error.argtypes = [vp]
     L.pic1d2v_last_error.restype = ctypes.c_char_p
+    L.pic2d3v_last_error.argtypes = [vp]
+    L.pic2d3v_last_error.restype = ctypes.c_char_p
     L.pic_strerror.argtypes = [ctypes.c_int]
     L.pic_strerror.restype = ctypes.c_char_p
     L.pic_version.argtypes = []

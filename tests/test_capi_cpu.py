@@ -171,3 +171,65 @@ def test_create1d2v_rejects_bad_config_with_message(field, value, msg):
     with pytest.raises(picgolf.PicError) as e:
         picgolf.Context1D2V(cfg)
     assert e.value.code == -1 and msg in str(e.value)
+
+
+def test_header2d3v_and_binding_agree():
+    import picgolf
+    assert sorted(picgolf.EXPORTS_2D3V) == header_functions("pic2d3v.h", "pic2d3v_")
+
+
+def test_library_exports_every_2d3v_symbol():
+    import picgolf
+    L = picgolf.load()
+    for name in header_functions("pic2d3v.h", "pic2d3v_"):
+        assert hasattr(L, name), name
+
+
+def test_2d3v_struct_layout_matches_the_header(tmp_path):
+    """The ctypes mirrors have the C structs' sizes and field offsets."""
+    import ctypes
+    import subprocess
+    from picgolf import _lib as L
+    structs = {"pic2d3v_config": L.pic2d3v_config, "pic2d3v_species": L.pic2d3v_species}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pic2d3v.h"', "int main(void) {"]
+    for name, cls in structs.items():
+        lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{name} {f} %zu\\n", offsetof({name}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for name, cls in structs.items():
+        assert got[(name, "size")] == ctypes.sizeof(cls), name
+        for f, _ in cls._fields_:
+            assert got[(name, f)] == getattr(cls, f).offset, (name, f)
+
+
+@pytest.mark.parametrize("change,msg", [
+    (dict(NX=100), "powers of two"),
+    (dict(Lx=0.0), "Lx and Ly"),
+    (dict(dt=-1.0), "dt must"),
+    (dict(B0=(0.0, float("nan"), 0.0)), "B0 must"),
+    (dict(ntskip=0), "ntskip"),
+    (dict(ngskip=3), "ngskip"),
+    (dict(field_update=5), "field_update"),
+    (dict(boris=9), "boris"),
+    (dict(species=[]), "nspecies"),
+    (dict(species=[dict(P=0)]), "P must"),
+    (dict(species=[dict(shape=9)]), "shape"),
+    (dict(species=[dict(mass=0.0)]), "mass"),
+    (dict(species=[dict(P=2 ** 25), dict(P=2 ** 25), dict(P=1)]), "2^26"),
+])
+def test_create2d3v_rejects_bad_config_with_message(change, msg):
+    import picgolf
+    from picgolf.core2d3v import Config2D3V, SpeciesConfig
+    kw = dict(NX=16, NY=8, dt=0.01, species=[SpeciesConfig(P=100)])
+    for k, v in change.items():
+        kw[k] = [SpeciesConfig(**{"P": 100, **s}) for s in v] if k == "species" else v
+    with pytest.raises(picgolf.PicError) as e:
+        picgolf.Context2D3V(Config2D3V(**kw))
+    assert e.value.code == -1 and msg in str(e.value)
