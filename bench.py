@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the ensemble and 2D3V workloads reported beside the headline")
     ap.add_argument("--profile-steps", type=int, default=3)
-    ap.add_argument("--workload", default="scaled", choices=["scaled", "ensemble", "es2d3v"],
+    ap.add_argument("--workload", default="scaled", choices=["scaled", "ensemble", "es2d3v", "pic2d3v"],
                     help="scaled: the headline line (default); ensemble / es2d3v: only that workload, "
                          "--steps timed steps, one JSON line (for profiling)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -252,6 +252,40 @@ def bench_2d(steps=500):
             "momentum_x_drift_mixed": float(np.abs(m["K"][:, 3] - m["K"][0, 3]).max())}
 
 
+def bench_pic2d3v(steps=20, warmup=2, N=1024, ppc=16, shape="bspline2"):
+    """The structured 2D3V module's electrostatic loop at the grid and
+    particle count of its driver script (src/2D3V.jl: 1024 x 1024 cells, 16
+    particles per cell per species, electrons and ions of mass 32, quadratic
+    B-splines, both species sorted per cell once), FP64, the field gathered
+    as the reference's update! leaves it."""
+    import numpy as np
+    import picgolf
+    from picgolf.core2d3v import Config2D3V, SpeciesConfig
+    rng = np.random.default_rng(0)
+    P = N * N * ppc
+    n0 = 4 * np.pi ** 2
+    sps, xyvs = [], []
+    for q, m, vth in ((-1.0, 1.0, 0.05), (1.0, 32.0, 0.05 / np.sqrt(32))):
+        sps.append(SpeciesConfig(P=P, shape=shape, charge=q, mass=m, weight=n0 / P))
+        xyvs.append(np.vstack([rng.random(P), rng.random(P), vth * rng.standard_normal((3, P))]))
+    cfg = Config2D3V(NX=N, NY=N, dt=1.0 / (8 * N), species=sps, B0=(np.sqrt(n0) / 4, 0.0, 0.0), ntskip=4,
+                     ngskip=4)
+    with picgolf.Context2D3V(cfg) as c:
+        for k, xyv in enumerate(xyvs):
+            c.upload(k, xyv)
+            c.sort(k, 1.0 / N, 1.0 / N)
+        del xyvs
+        c.run(warmup)
+        t0 = time.perf_counter()
+        rows = c.run(steps)
+        secs = time.perf_counter() - t0
+    return {"workload": f"structured 2D3V electrostatic loop: {N}x{N} cells, 2 species x {P} particles "
+                        f"({ppc} per cell each), {shape}, ions of mass 32, B0 along x, dt=1/(8N), "
+                        f"diagnostics every 4 steps, FP64, particles sorted per cell once before the timed steps",
+            "pushes_per_s": 2 * P * steps / secs, "ms_per_step": secs / steps * 1e3, "steps": steps,
+            "warmup": warmup, "kinetic_energy_first_last": [float(rows[0, 0]), float(rows[-1, 0])]}
+
+
 def cpu_baseline(cfg, x, v, budget_s=15.0):
     """The CPU restatement (FP64) timed on this host on a bounded sample: the
     same N and stencil with the first n particles, steps until ~budget_s."""
@@ -298,6 +332,9 @@ def main():
         return
     if a.workload == "es2d3v":
         print(json.dumps(bench_2d(steps=a.steps)))
+        return
+    if a.workload == "pic2d3v":
+        print(json.dumps(bench_pic2d3v(steps=a.steps, warmup=a.warmup)))
         return
     import numpy as np
     from picgolf import configs
@@ -354,7 +391,8 @@ def main():
         out["field_energy_rel_err_mixed_vs_fp64"] = float(np.max(np.abs(d1m - d1f) / np.abs(d1f)))
         out["momentum_drift_mixed"] = float(np.max(np.abs(m["D"][:, 3] - m["D"][0, 3])))
     if not a.no_extra:
-        out["workloads"] = {"ensemble_512": bench_ensemble(autotune=not a.no_autotune), "es2d3v_256": bench_2d()}
+        out["workloads"] = {"ensemble_512": bench_ensemble(autotune=not a.no_autotune), "es2d3v_256": bench_2d(),
+                            "pic2d3v_1024": bench_pic2d3v()}
         print(f"[bench] extra workloads: {json.dumps(out['workloads'])}", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(*base)

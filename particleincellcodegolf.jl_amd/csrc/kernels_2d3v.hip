@@ -10,6 +10,8 @@
 // cell (device-scope integer atomics), so the charge does not depend on the
 // particle order or the schedule.  The field solve is the 2D transform of
 // kernels_2d.hip with wavenumbers 2pi/Lx, 2pi/Ly.
+#include <climits>
+
 #include "pic_device.h"
 #include "pic2d3v_internal.h"
 
@@ -219,15 +221,169 @@ __device__ __forceinline__ void deposit23(const Grid23& g, u64* __restrict__ rac
 
 // ---------------------------------------------------------------------------
 // the particle loop of loop! (:545-555) for one species
+//
+// A block takes a chunk of kTile23Chunk consecutive particles.  Particles
+// sorted per cell (sort!, as the reference's driver does before its loop)
+// keep a chunk within a thin strip of cells, so the block stages the field of
+// the strip (its window: the chunk's cells plus the stencil and one cell of
+// drift) in LDS and sums the charge of the strip in LDS limbs, adding them to
+// the grid's limbs once at the end.  A particle whose stencil leaves the
+// window (or every particle, when the window would not fit) reads the field
+// and adds its charge through global memory instead.  The charge sums are
+// exact integers and each particle's update depends only on its own state and
+// the field, so the result does not depend on the window.
 // ---------------------------------------------------------------------------
+constexpr int kTile23Threads = 256;
+constexpr int kTile23PPT = 4;
+constexpr int kTile23Chunk = kTile23Threads * kTile23PPT;
+constexpr int kTile23Cells = 1536;  // window cells: 16 B of field + 3 x 8 B of limbs each (60 KB)
+constexpr int kTile23Halo = 1;
+
+// stencil extent about the base cell i = ceil(zN): cells i - lo .. i + hi
+template <int SH>
+struct ShapeExtent {
+    static constexpr int n = SH - 2;
+    static constexpr int lo = SH <= 1 ? 0 : ((n & 1) ? (n - 1) / 2 : n / 2);
+    static constexpr int hi = SH == 0 ? 0 : (SH == 1 ? 1 : ((n & 1) ? (n + 1) / 2 : n / 2 + 1));
+};
+
+__device__ __forceinline__ int32_t base_cell(double z, double NZ_Lz, int32_t N) {
+    const double ci = ceil(z * NZ_Lz);
+    const int64_t i = (ci > -4.0e18 && ci < 4.0e18) ? (int64_t)ci : 0;
+    return (int32_t)((i - 1) & (int64_t)(N - 1));
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+    for (int h = 32; h > 0; h >>= 1) v = min(v, __shfl_xor(v, h, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int h = 32; h > 0; h >>= 1) v = max(v, __shfl_xor(v, h, 64));
+    return v;
+}
+
+__device__ __forceinline__ void add_limbs_lds(u64* Rw, int idx, i128 v) {
+    const u128 u = (u128)v;
+    constexpr u64 M = (1ull << kLimbS) - 1;
+    const u64 l0 = (u64)u & M;
+    const u64 l1 = (u64)(u >> kLimbS) & M;
+    const u64 l2 = (u64)(u >> (2 * kLimbS)) & M;
+    if (l0) atomicAdd(Rw + idx, l0);
+    if (l1) atomicAdd(Rw + kTile23Cells + idx, l1);
+    if (l2) atomicAdd(Rw + 2 * kTile23Cells + idx, l2);
+}
+
 template <int SH, int BV>
-__global__ void __launch_bounds__(kPush23Threads)
+__global__ void __launch_bounds__(kTile23Threads)
 k23_push(Grid23 g, Species23 s, const double2* __restrict__ E, u64* __restrict__ racc) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < s.P; p += stride) {
-        double x = s.x[p], y = s.y[p], vx = s.vx[p], vy = s.vy[p], vz = s.vz[p];
-        double a, b;
-        gather23<SH>(g, E, x, y, a, b);
+    constexpr int n = ShapeCount<SH>::value;
+    constexpr int LO = ShapeExtent<SH>::lo, HI = ShapeExtent<SH>::hi;
+    __shared__ double2 Ew[kTile23Cells];
+    __shared__ u64 Rw[3 * kTile23Cells];
+    __shared__ int red[4][4];
+    __shared__ int win[5];
+    const int t = threadIdx.x;
+    const int NX = g.NX, NY = g.NY;
+    const int64_t G = (int64_t)NX * NY;
+    const int64_t c0 = (int64_t)blockIdx.x * kTile23Chunk;
+    // the window: the chunk's base cells relative to its first particle's
+    const int ax = base_cell(s.x[c0], g.NX_Lx, NX), ay = base_cell(s.y[c0], g.NY_Ly, NY);
+    double xs[kTile23PPT], ys[kTile23PPT];
+    int mnx = INT_MAX, mxx = INT_MIN, mny = INT_MAX, mxy = INT_MIN;
+#pragma unroll
+    for (int k = 0; k < kTile23PPT; ++k) {
+        const int64_t p = c0 + t + (int64_t)k * kTile23Threads;
+        xs[k] = 0.0;
+        ys[k] = 0.0;
+        if (p < s.P) {
+            xs[k] = s.x[p];
+            ys[k] = s.y[p];
+            const int dx = ((base_cell(xs[k], g.NX_Lx, NX) - ax + NX / 2) & (NX - 1)) - NX / 2;
+            const int dy = ((base_cell(ys[k], g.NY_Ly, NY) - ay + NY / 2) & (NY - 1)) - NY / 2;
+            mnx = min(mnx, dx);
+            mxx = max(mxx, dx);
+            mny = min(mny, dy);
+            mxy = max(mxy, dy);
+        }
+    }
+    mnx = wave_min_i(mnx);
+    mxx = wave_max_i(mxx);
+    mny = wave_min_i(mny);
+    mxy = wave_max_i(mxy);
+    const int wv = t >> 6;
+    if ((t & 63) == 0) {
+        red[wv][0] = mnx;
+        red[wv][1] = mxx;
+        red[wv][2] = mny;
+        red[wv][3] = mxy;
+    }
+    __syncthreads();
+    if (t == 0) {
+        for (int w = 1; w < kTile23Threads / 64; ++w) {
+            mnx = min(mnx, red[w][0]);
+            mxx = max(mxx, red[w][1]);
+            mny = min(mny, red[w][2]);
+            mxy = max(mxy, red[w][3]);
+        }
+        int wx = mxx - mnx + LO + HI + 2 * kTile23Halo + 1, sx = ax + mnx - LO - kTile23Halo;
+        int wy = mxy - mny + LO + HI + 2 * kTile23Halo + 1, sy = ay + mny - LO - kTile23Halo;
+        if (wx >= NX) {
+            wx = NX;
+            sx = 0;
+        }
+        if (wy >= NY) {
+            wy = NY;
+            sy = 0;
+        }
+        win[0] = sx & (NX - 1);
+        win[1] = sy & (NY - 1);
+        win[2] = wx;
+        win[3] = wy;
+        win[4] = wx * wy <= kTile23Cells;
+    }
+    __syncthreads();
+    const int ox = win[0], oy = win[1], wx = win[2], wy = win[3];
+    const bool tile = win[4] != 0;
+    if (tile) {
+        for (int e = t; e < wx * wy; e += kTile23Threads) {
+            const int ry = e / wx, rx = e - ry * wx;
+            Ew[e] = E[((ox + rx) & (NX - 1)) + (int64_t)NX * ((oy + ry) & (NY - 1))];
+            Rw[e] = 0;
+            Rw[kTile23Cells + e] = 0;
+            Rw[2 * kTile23Cells + e] = 0;
+        }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int k = 0; k < kTile23PPT; ++k) {
+        const int64_t p = c0 + t + (int64_t)k * kTile23Threads;
+        if (p >= s.P) break;
+        double x = xs[k], y = ys[k], vx = s.vx[p], vy = s.vy[p], vz = s.vz[p];
+        int32_t cx[n], cy[n];
+        double wxs[n], wys[n];
+        int lx[n], ly[n];
+        // gather at the old position
+        shape_w<SH>(x, g.NX_Lx, NX, cx, wxs);
+        shape_w<SH>(y, g.NY_Ly, NY, cy, wys);
+        bool in = tile;
+#pragma unroll
+        for (int q = 0; q < n; ++q) {
+            lx[q] = (cx[q] - ox) & (NX - 1);
+            ly[q] = (cy[q] - oy) & (NY - 1);
+            in = in && lx[q] < wx && ly[q] < wy;
+        }
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < n; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < n; ++ii) {
+                const double wxy = wxs[ii] * wys[jj];
+                const double2 e = in ? Ew[lx[ii] + wx * ly[jj]] : E[cx[ii] + (int64_t)NX * cy[jj]];
+                a = fma(e.x, wxy, a);
+                b = fma(e.y, wxy, b);
+            }
         const double vx0 = vx, vy0 = vy;
         boris23<BV>(g, s.q_m, a, b, vx, vy, vz);
         x = unimod23(x + (vx0 + vx) / 2.0 * g.dt, g.Lx);
@@ -237,7 +393,35 @@ k23_push(Grid23 g, Species23 s, const double2* __restrict__ E, u64* __restrict__
         s.vx[p] = vx;
         s.vy[p] = vy;
         s.vz[p] = vz;
-        deposit23<SH>(g, racc, x, y, s.qw);
+        // deposit at the new position
+        shape_w<SH>(x, g.NX_Lx, NX, cx, wxs);
+        shape_w<SH>(y, g.NY_Ly, NY, cy, wys);
+        in = tile;
+#pragma unroll
+        for (int q = 0; q < n; ++q) {
+            lx[q] = (cx[q] - ox) & (NX - 1);
+            ly[q] = (cy[q] - oy) & (NY - 1);
+            in = in && lx[q] < wx && ly[q] < wy;
+        }
+#pragma unroll
+        for (int jj = 0; jj < n; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < n; ++ii) {
+                const i128 v = fix_round128(wxs[ii] * wys[jj] * s.qw, g.q);
+                if (in) add_limbs_lds(Rw, lx[ii] + wx * ly[jj], v);
+                else add_limbs_s(racc, G, cx[ii] + (int64_t)NX * cy[jj], v);
+            }
+    }
+    __syncthreads();
+    if (tile) {
+        for (int e = t; e < wx * wy; e += kTile23Threads) {
+            const int ry = e / wx, rx = e - ry * wx;
+            const int64_t k = ((ox + rx) & (NX - 1)) + (int64_t)NX * ((oy + ry) & (NY - 1));
+            const u64 l0 = Rw[e], l1 = Rw[kTile23Cells + e], l2 = Rw[2 * kTile23Cells + e];
+            if (l0) atomicAdd(racc + k, l0);
+            if (l1) atomicAdd(racc + G + k, l1);
+            if (l2) atomicAdd(racc + 2 * G + k, l2);
+        }
     }
 }
 
@@ -658,13 +842,13 @@ static unsigned blocks_for(int64_t n, int threads) {
 #define PIC23_SHAPES(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
 
 void launch23_push(const Grid23& g, const Species23& s, const double2* E, unsigned long long* racc, hipStream_t st) {
-    const unsigned nb = blocks_for(s.P, kPush23Threads);
+    const unsigned nb = (unsigned)((s.P + kTile23Chunk - 1) / kTile23Chunk);
 #define PIC23_PUSH(SH)                                                                                 \
     case SH:                                                                                           \
         if (g.boris)                                                                                   \
-            k23_push<SH, 1><<<nb, kPush23Threads, 0, st>>>(g, s, E, racc);                             \
+            k23_push<SH, 1><<<nb, kTile23Threads, 0, st>>>(g, s, E, racc);                             \
         else                                                                                           \
-            k23_push<SH, 0><<<nb, kPush23Threads, 0, st>>>(g, s, E, racc);                             \
+            k23_push<SH, 0><<<nb, kTile23Threads, 0, st>>>(g, s, E, racc);                             \
         break;
     switch (s.shape) { PIC23_SHAPES(PIC23_PUSH) }
 #undef PIC23_PUSH
