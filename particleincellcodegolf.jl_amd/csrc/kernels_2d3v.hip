@@ -236,7 +236,7 @@ __device__ __forceinline__ void deposit23(const Grid23& g, u64* __restrict__ rac
 constexpr int kTile23Threads = 256;
 constexpr int kTile23PPT = 4;
 constexpr int kTile23Chunk = kTile23Threads * kTile23PPT;
-constexpr int kTile23Cells = 1536;  // window cells: 16 B of field + 3 x 8 B of limbs each (60 KB)
+constexpr int kTile23Cells = 1024;  // window cells: 16 B of field + 3 x 8 B of limbs each (40 KB)
 constexpr int kTile23Halo = 1;
 
 // stencil extent about the base cell i = ceil(zN): cells i - lo .. i + hi
@@ -290,16 +290,22 @@ k23_push(Grid23 g, Species23 s, const double2* __restrict__ E, u64* __restrict__
     const int64_t c0 = (int64_t)blockIdx.x * kTile23Chunk;
     // the window: the chunk's base cells relative to its first particle's
     const int ax = base_cell(s.x[c0], g.NX_Lx, NX), ay = base_cell(s.y[c0], g.NY_Ly, NY);
-    double xs[kTile23PPT], ys[kTile23PPT];
+    double xs[kTile23PPT], ys[kTile23PPT], us[kTile23PPT], vs[kTile23PPT], ws[kTile23PPT];
     int mnx = INT_MAX, mxx = INT_MIN, mny = INT_MAX, mxy = INT_MIN;
 #pragma unroll
     for (int k = 0; k < kTile23PPT; ++k) {
         const int64_t p = c0 + t + (int64_t)k * kTile23Threads;
         xs[k] = 0.0;
         ys[k] = 0.0;
+        us[k] = 0.0;
+        vs[k] = 0.0;
+        ws[k] = 0.0;
         if (p < s.P) {
             xs[k] = s.x[p];
             ys[k] = s.y[p];
+            us[k] = s.vx[p];
+            vs[k] = s.vy[p];
+            ws[k] = s.vz[p];
             const int dx = ((base_cell(xs[k], g.NX_Lx, NX) - ax + NX / 2) & (NX - 1)) - NX / 2;
             const int dy = ((base_cell(ys[k], g.NY_Ly, NY) - ay + NY / 2) & (NY - 1)) - NY / 2;
             mnx = min(mnx, dx);
@@ -360,7 +366,7 @@ k23_push(Grid23 g, Species23 s, const double2* __restrict__ E, u64* __restrict__
     for (int k = 0; k < kTile23PPT; ++k) {
         const int64_t p = c0 + t + (int64_t)k * kTile23Threads;
         if (p >= s.P) break;
-        double x = xs[k], y = ys[k], vx = s.vx[p], vy = s.vy[p], vz = s.vz[p];
+        double x = xs[k], y = ys[k], vx = us[k], vy = vs[k], vz = ws[k];
         int32_t cx[n], cy[n];
         double wxs[n], wys[n];
         int lx[n], ly[n];
@@ -539,10 +545,15 @@ k23_diag_kinetic(Species23 s, double* __restrict__ kpart) {
     const int64_t n3 = 3 * s.P;
     const int64_t b0 = (int64_t)blockIdx.x * kTreeChunk;
     const int64_t n = n3 - b0 < kTreeChunk ? n3 - b0 : kTreeChunk;
+    // entry b0 + i is component c of particle p: one 64-bit division per
+    // block, 32-bit ones per entry
+    const int64_t p0 = b0 / 3;
+    const uint32_t r0 = (uint32_t)(b0 - 3 * p0);
     const double r = block_chunk_sum(red, n, [&](int64_t i) {
-        const int64_t e = b0 + i;
-        const int64_t p = e / 3;
-        const int c = (int)(e - 3 * p);
+        const uint32_t q = r0 + (uint32_t)i;
+        const uint32_t dp = q / 3u;
+        const uint32_t c = q - 3u * dp;
+        const int64_t p = p0 + dp;
         const double v = c == 0 ? s.vx[p] : (c == 1 ? s.vy[p] : s.vz[p]);
         return v * v;
     });
