@@ -279,11 +279,28 @@ def bench_pic2d3v(steps=20, warmup=2, N=1024, ppc=16, shape="bspline2"):
         t0 = time.perf_counter()
         rows = c.run(steps)
         secs = time.perf_counter() - t0
+    # the CPU restatement on a sample: the same grid, 1/32 of each species (sorted), two steps
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    n = P // 32
+    sample = []
+    for sp in sps:
+        xyv = np.vstack([rng.random(n), rng.random(n), 0.05 * rng.standard_normal((3, n))])
+        xyv, _ = oracle.sort23(xyv, 1.0 / N, 1.0 / N)
+        sample.append(dict(shape=shape, charge=sp.charge, mass=sp.mass, weight=n0 / n, xyv=xyv))
+    t0 = time.perf_counter()
+    oracle.run23(N, N, 1.0, 1.0, cfg.dt, cfg.B0, sample, 2, ntskip=4, ngskip=4, nthreads=threads)
+    cpu_secs = time.perf_counter() - t0
     return {"workload": f"structured 2D3V electrostatic loop: {N}x{N} cells, 2 species x {P} particles "
                         f"({ppc} per cell each), {shape}, ions of mass 32, B0 along x, dt=1/(8N), "
                         f"diagnostics every 4 steps, FP64, particles sorted per cell once before the timed steps",
             "pushes_per_s": 2 * P * steps / secs, "ms_per_step": secs / steps * 1e3, "steps": steps,
-            "warmup": warmup, "kinetic_energy_first_last": [float(rows[0, 0]), float(rows[-1, 0])]}
+            "warmup": warmup, "kinetic_energy_first_last": [float(rows[0, 0]), float(rows[-1, 0])],
+            "cpu_baseline": {"value": 2 * n * 2 / cpu_secs, "unit": "particle-pushes/s", "cores": threads,
+                             "kind": "port", "sample": f"FP64 CPU restatement, the same {N}x{N} grid and solve, "
+                             f"2 species x {n} particles (1/32), 2 steps; push on {threads} threads, deposit "
+                             f"and solve on one"}}
 
 
 def cpu_baseline(cfg, x, v, budget_s=15.0):

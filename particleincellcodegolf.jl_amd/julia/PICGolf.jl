@@ -38,7 +38,8 @@ module PICGolf
 export Context, PicConfig, deposit!, fieldsolve!, drift!, midpoint_position!, wrap!,
        diagnostics, upload!, download, run!, quiet_start, autotune!, set_tiling!, tiling, PicTuning,
        simpson_kick!, Pic1D2VConfig, Context1D2V, fields,
-       Pic2DConfig, Context2D, gather, boris!
+       Pic2DConfig, Context2D, gather, boris!,
+       Pic2D3VSpecies, Pic2D3VConfig, Context2D3V, reset!, field
 
 const libpicgolf = get(ENV, "PICGOLF_LIB", normpath(joinpath(@__DIR__, "..", "lib", "libpicgolf.so")))
 
@@ -665,6 +666,240 @@ function boris!(ctx::Context1D2V, vx::Vector{Float64}, vy::Vector{Float64}, Ej::
                                       Float64, Float64), ctx.ptr, vx, vy, Ej, length(vx), B, dt, q_m)
     check(ctx, rc, "pic1d2v_boris")
     (vx, vy)
+end
+
+# ---------------------------------------------------------------------------
+# Structured 2D3V module, electrostatic path (src/PIC2D3V.jl), C ABI in
+# include/pic2d3v.h.
+#
+# | PIC2D3V                                                  | PICGolf                                   |
+# |----------------------------------------------------------|-------------------------------------------|
+# | `Species(...)`, `species.xyv` (:148-208)                  | `Pic2D3VSpecies`, `upload!`, `download`    |
+# | `ElectrostaticField(NX, NY, Lx, Ly; dt, B0x, B0y, B0z, buffer)` (:284-292) | `Pic2D3VConfig(...)`     |
+# | `ElectrostaticDiagnostics(NX, NY, NT, ntskip, ngskip)` (:95-101) | `Pic2D3VConfig(...; ntskip, ngskip)` |
+# | `sort!(species, Δx, Δy)` (:210-215)                       | `sort!(ctx, s, Δx, Δy)`                    |
+# | `loop!(plasma, field, to, t, _); diagnose!(d, field, plasma, t, to)` (:530-581, :1310-1330) | `run!` |
+# | `d.Exs, d.Eys, d.ϕs` (:1323-1327)                          | `fields(ctx)`                              |
+# | `field.Exy` (:294-297)                                     | `field(ctx)`                               |
+# | `deposit!(z, shape, x, y, NX_Lx, NY_Ly, w)` (:1249-1256)   | `deposit!(ρ, ctx, shape, x, y, w)`         |
+# | `(field)(shape, x, y)` (:1217-1228)                        | `gather(ctx, shape, Ex, Ey, x, y)`         |
+# | `field.boris(vx, vy, vz, Ex, Ey, q_m)` (:49-54)            | `boris!(ctx, vx, vy, vz, Ex, Ey; q_m)`     |
+# ---------------------------------------------------------------------------
+
+const SHAPES2D3V = Dict(:ngp => Int32(0), :area => Int32(1), :bspline0 => Int32(2), :bspline1 => Int32(3),
+                        :bspline2 => Int32(4), :bspline3 => Int32(5), :bspline4 => Int32(6), :bspline5 => Int32(7))
+const FIELD_UPDATES = Dict(:accumulate => Int32(0), :replace => Int32(1))
+const BORIS_VARIANTS = Dict(:reference => Int32(0), :textbook => Int32(1))
+const MAX_SPECIES_2D3V = 4
+
+"Mirror of `struct pic2d3v_species` in include/pic2d3v.h."
+struct Pic2D3VSpecies
+    P::Int64
+    shape::Int32
+    charge::Float64
+    mass::Float64
+    weight::Float64
+end
+
+"""
+    Pic2D3VSpecies(P; shape=:bspline2, charge=1, mass=1, weight)
+
+One `Species` (src/PIC2D3V.jl:148-156); `weight` is the reference's
+`calculateweight(n0, P, Lx, Ly) = n0*Lx*Ly/P`.
+"""
+Pic2D3VSpecies(P::Integer; shape=:bspline2, charge=1.0, mass=1.0, weight) =
+    Pic2D3VSpecies(Int64(P), SHAPES2D3V[shape], Float64(charge), Float64(mass), Float64(weight))
+
+"Mirror of `struct pic2d3v_config` in include/pic2d3v.h."
+struct Pic2D3VConfig
+    NX::Int32
+    NY::Int32
+    Lx::Float64
+    Ly::Float64
+    dt::Float64
+    B0::NTuple{3,Float64}
+    buffer::Int32
+    ntskip::Int32
+    ngskip::Int32
+    field_update::Int32
+    boris::Int32
+    nspecies::Int32
+    species::NTuple{4,Pic2D3VSpecies}
+    device::Int32
+end
+
+"""
+    Pic2D3VConfig(species; NX, NY=NX, Lx=1, Ly=1, dt, B0x=0, B0y=0, B0z=0, buffer=3, ntskip=1, ngskip=1,
+                  field_update=:accumulate, boris=:reference, device=0)
+
+The `ElectrostaticField(NX, NY, Lx, Ly; dt, B0x, B0y, B0z, buffer)` and
+`ElectrostaticDiagnostics(..., ntskip, ngskip)` of a plasma of up to four
+species.  `field_update=:accumulate` is the reference's `update!`, which adds
+every step's field into `Exy` without clearing it; `:replace` gathers the
+step's own field.  `boris=:reference` is `ElectrostaticBoris` (rotation scaled
+by q_m^2); `:textbook` is `ElectromagneticBoris`'s form with Ez = 0.
+"""
+function Pic2D3VConfig(species::Vector{Pic2D3VSpecies}; NX, NY=NX, Lx=1.0, Ly=1.0, dt, B0x=0.0, B0y=0.0, B0z=0.0,
+                       buffer=3, ntskip=1, ngskip=1, field_update=:accumulate, boris=:reference, device=0)
+    ns = length(species)
+    1 <= ns <= MAX_SPECIES_2D3V || error("Pic2D3VConfig: 1 to $MAX_SPECIES_2D3V species")
+    none = Pic2D3VSpecies(0, 0, 0.0, 0.0, 0.0)
+    sp = ntuple(k -> k <= ns ? species[k] : none, 4)
+    Pic2D3VConfig(Int32(NX), Int32(NY), Float64(Lx), Float64(Ly), Float64(dt),
+                  (Float64(B0x), Float64(B0y), Float64(B0z)), Int32(buffer), Int32(ntskip), Int32(ngskip),
+                  FIELD_UPDATES[field_update], BORIS_VARIANTS[boris], Int32(ns), sp, Int32(device))
+end
+
+mutable struct Context2D3V
+    ptr::Ptr{Cvoid}
+    cfg::Pic2D3VConfig
+    t::Int64
+end
+
+function check(ctx::Context2D3V, rc::Cint, what::AbstractString)
+    rc == 0 || error("$what: ", unsafe_string(ccall((:pic2d3v_last_error, libpicgolf), Cstring, (Ptr{Cvoid},), ctx.ptr)))
+    nothing
+end
+
+function Context2D3V(cfg::Pic2D3VConfig)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:pic2d3v_create, libpicgolf), Cint, (Ref{Ptr{Cvoid}}, Ref{Pic2D3VConfig}), h, cfg)
+    rc == 0 || error("pic2d3v_create: ", unsafe_string(ccall((:pic2d3v_last_error, libpicgolf), Cstring,
+                                                             (Ptr{Cvoid},), C_NULL)))
+    ctx = Context2D3V(h[], cfg, 0)
+    finalizer(destroy!, ctx)
+    ctx
+end
+
+function destroy!(ctx::Context2D3V)
+    if ctx.ptr != C_NULL
+        ccall((:pic2d3v_destroy, libpicgolf), Cint, (Ptr{Cvoid},), ctx.ptr)
+        ctx.ptr = C_NULL
+    end
+    nothing
+end
+
+"`upload!(ctx, s, xyv)`: species `s`'s 5 x P matrix `xyv` (rows x, y, vx, vy, vz)."
+function upload!(ctx::Context2D3V, s::Integer, xyv::Matrix{Float64})
+    size(xyv) == (5, ctx.cfg.species[s].P) || error("upload!: xyv must be 5 x $(ctx.cfg.species[s].P)")
+    rc = GC.@preserve xyv ccall((:pic2d3v_upload, libpicgolf), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}),
+                                ctx.ptr, s - 1, xyv)
+    check(ctx, rc, "pic2d3v_upload")
+    ctx
+end
+
+"`download(ctx, s) -> xyv` (5 x P)."
+function download(ctx::Context2D3V, s::Integer)
+    xyv = zeros(5, ctx.cfg.species[s].P)
+    rc = GC.@preserve xyv ccall((:pic2d3v_download, libpicgolf), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}),
+                                ctx.ptr, s - 1, xyv)
+    check(ctx, rc, "pic2d3v_download")
+    xyv
+end
+
+"`reset!(ctx)`: a fresh field and diagnostics (Exy = 0, t = 0)."
+function reset!(ctx::Context2D3V)
+    check(ctx, ccall((:pic2d3v_reset, libpicgolf), Cint, (Ptr{Cvoid},), ctx.ptr), "pic2d3v_reset")
+    ctx.t = 0
+    ctx
+end
+
+"""
+    sort!(ctx::Context2D3V, s, Δx, Δy) -> p
+
+`sort!(species, Δx, Δy)`: reorders species `s` by `(ceil(x/Δx), ceil(y/Δy), vx)`
+(stable) and returns the permutation `species.p` (1-based).
+"""
+function Base.sort!(ctx::Context2D3V, s::Integer, Δx::Real, Δy::Real)
+    p = zeros(Int64, ctx.cfg.species[s].P)
+    rc = GC.@preserve p ccall((:pic2d3v_sort, libpicgolf), Cint, (Ptr{Cvoid}, Int32, Float64, Float64, Ptr{Int64}),
+                              ctx.ptr, s - 1, Δx, Δy, p)
+    check(ctx, rc, "pic2d3v_sort")
+    p .+ 1
+end
+
+"""
+    run!(ctx::Context2D3V, nsteps) -> rows
+
+`nsteps` of `loop!(plasma, field, ...); diagnose!(d, field, plasma, t, ...)`;
+one row per `t % ntskip == 0` of this call: kineticenergy, fieldenergy,
+particlemomentum (3), characteristicmomentum (3).
+"""
+function run!(ctx::Context2D3V, nsteps::Integer)
+    nt = ctx.cfg.ntskip
+    rows = cld(ctx.t + nsteps, nt) - cld(ctx.t, nt)
+    R = zeros(8, rows)
+    rc = GC.@preserve R ccall((:pic2d3v_run, libpicgolf), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}),
+                              ctx.ptr, nsteps, rows > 0 ? pointer(R) : C_NULL)
+    ctx.t += nsteps
+    check(ctx, rc, "pic2d3v_run")
+    permutedims(R)
+end
+
+"`fields(ctx) -> (Exs, Eys, ϕs)`: the diagnostics' binned fields (NX÷ngskip x NY÷ngskip x bins)."
+function fields(ctx::Context2D3V)
+    c = ctx.cfg
+    nb = cld(ctx.t, c.ntskip)
+    A = [zeros(c.NX ÷ c.ngskip, c.NY ÷ c.ngskip, nb) for _ in 1:3]
+    if nb > 0
+        rc = GC.@preserve A ccall((:pic2d3v_fields, libpicgolf), Cint,
+                                  (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64),
+                                  ctx.ptr, A[1], A[2], A[3], nb)
+        check(ctx, rc, "pic2d3v_fields")
+    end
+    (A[1], A[2], A[3])
+end
+
+"`field(ctx) -> (Ex, Ey)`: the gathered field `Exy` (interior NX x NY)."
+function field(ctx::Context2D3V)
+    Ex, Ey = zeros(ctx.cfg.NX, ctx.cfg.NY), zeros(ctx.cfg.NX, ctx.cfg.NY)
+    rc = GC.@preserve Ex Ey ccall((:pic2d3v_field, libpicgolf), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                                  ctx.ptr, Ex, Ey)
+    check(ctx, rc, "pic2d3v_field")
+    (Ex, Ey)
+end
+
+"`deposit!(ρ, ctx, shape, x, y, w)`: ρ = Σ (wx*wy)*w over the particles, folded periodically."
+function deposit!(ρ::Matrix{Float64}, ctx::Context2D3V, shape::Symbol, x::Vector{Float64}, y::Vector{Float64},
+                  w::Real)
+    rc = GC.@preserve ρ x y ccall((:pic2d3v_deposit, libpicgolf), Cint,
+                                  (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Ptr{Float64}),
+                                  ctx.ptr, SHAPES2D3V[shape], x, y, length(x), w, ρ)
+    check(ctx, rc, "pic2d3v_deposit")
+    ρ
+end
+
+"`gather(ctx, shape, Ex, Ey, x, y) -> (ex, ey)`: `field(shape, x[j], y[j])` for every j."
+function gather(ctx::Context2D3V, shape::Symbol, Ex::Matrix{Float64}, Ey::Matrix{Float64}, x::Vector{Float64},
+                y::Vector{Float64})
+    ex, ey = zeros(length(x)), zeros(length(x))
+    rc = GC.@preserve Ex Ey x y ex ey ccall((:pic2d3v_gather, libpicgolf), Cint,
+                                            (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                                             Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}),
+                                            ctx.ptr, SHAPES2D3V[shape], Ex, Ey, x, y, length(x), ex, ey)
+    check(ctx, rc, "pic2d3v_gather")
+    (ex, ey)
+end
+
+"`boris!(ctx, vx, vy, vz, Ex, Ey; q_m)`: the context's Boris (B0, dt, variant) in place."
+function boris!(ctx::Context2D3V, vx::Vector{Float64}, vy::Vector{Float64}, vz::Vector{Float64},
+                Ex::Vector{Float64}, Ey::Vector{Float64}; q_m::Real)
+    rc = GC.@preserve vx vy vz Ex Ey ccall((:pic2d3v_boris, libpicgolf), Cint,
+                                           (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                                            Ptr{Float64}, Int64, Float64),
+                                           ctx.ptr, vx, vy, vz, Ex, Ey, length(vx), q_m)
+    check(ctx, rc, "pic2d3v_boris")
+    (vx, vy, vz)
+end
+
+"`fieldsolve!(Ex, Ey, ctx, ρ; ϕ=nothing)`: the electrostatic solve of `loop!` in the context's box."
+function fieldsolve!(Ex::Matrix{Float64}, Ey::Matrix{Float64}, ctx::Context2D3V, ρ::Matrix{Float64};
+                     ϕ::Union{Matrix{Float64},Nothing}=nothing)
+    rc = GC.@preserve Ex Ey ρ ϕ ccall((:pic2d3v_fieldsolve, libpicgolf), Cint,
+                                      (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                                      ctx.ptr, ρ, Ex, Ey, ϕ === nothing ? C_NULL : ϕ)
+    check(ctx, rc, "pic2d3v_fieldsolve")
+    (Ex, Ey)
 end
 
 end # module

@@ -7,9 +7,9 @@ import re
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 JL = os.path.join(ROOT, "particleincellcodegolf.jl_amd", "julia", "PICGolf.jl")
-HEADERS = ["pic.h", "pic2d.h", "pic1d2v.h"]
+HEADERS = ["pic.h", "pic2d.h", "pic1d2v.h", "pic2d3v.h"]
 
-C_TYPES = {"int32_t": "Int32", "int64_t": "Int64", "double": "Float64"}
+C_TYPES = {"int32_t": "Int32", "int64_t": "Int64", "double": "Float64", "pic2d3v_species": "Pic2D3VSpecies"}
 
 
 def declared():
@@ -28,12 +28,15 @@ def c_struct(name):
             fields = []
             for line in m.group(1).splitlines():
                 line = re.sub(r"/\*.*?\*/", "", line).strip()
-                m2 = re.match(r"(int32_t|int64_t|double)\s+(.*);", line)
+                m2 = re.match(r"(int32_t|int64_t|double|pic2d3v_species)\s+(.*);", line)
                 if not m2:
                     continue
                 for f in m2.group(2).split(","):
                     f = f.strip()
-                    arr = re.match(r"(\w+)\[(\d+)\]", f)
+                    arr = re.match(r"(\w+)\[(\w+)\]", f)
+                    if arr and not arr.group(2).isdigit():   # a #define'd length
+                        n = re.search(r"#define " + arr.group(2) + r"\s+(\d+)", txt).group(1)
+                        arr = re.match(r"(\w+)\[(\d+)\]", f"{arr.group(1)}[{n}]")
                     t = C_TYPES[m2.group(1)]
                     fields.append((arr.group(1), f"NTuple{{{arr.group(2)},{t}}}") if arr else (f, t))
             return fields
@@ -58,5 +61,6 @@ def test_every_ccall_is_declared_and_exported():
 
 def test_struct_mirrors_match_the_headers():
     for jl, c in (("PicConfig", "pic_config"), ("Pic2DConfig", "pic2d_config"), ("Pic1D2VConfig", "pic1d2v_config"),
-                  ("PicTuning", "pic_tuning")):
+                  ("PicTuning", "pic_tuning"), ("Pic2D3VSpecies", "pic2d3v_species"),
+                  ("Pic2D3VConfig", "pic2d3v_config")):
         assert jl_struct(jl) == c_struct(c), jl
