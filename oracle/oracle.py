@@ -99,6 +99,15 @@ def lib():
         "orc23_ceil_index": (c_i64, [c_d]),
         "orc23_sort": (None, [c_i64, _dp, _dp, _dp, _dp, _dp, c_d, c_d, _ip64]),
         "orc23_diagnostics": (None, [c_i32, c_i32, c_i32, _dp, _dp, c_i32, _ip64, _dp, _dp, _dp, _dp]),
+        "orc_em_irn": (c_i64, [c_d, c_i32]),
+        "orc_em_stencil": (None, [c_i32, c_d, _ip64, _dp]),
+        "orc_em_frac_bits": (ctypes.c_int, [c_i64, c_d, c_d]),
+        "orc_em_deposit": (None, [c_i32, c_i64, _dp, c_vp, c_d, _dp]),
+        "orc_em_fields": (None, [c_i32, c_d, c_d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "orc_em_push": (None, [c_i32, c_d, _dp, _dp, _dp, _dp, _dp, c_d, _dp, _dp]),
+        "orc_em_sort": (None, [c_i64, _dp, _dp, _dp, _dp, _ip64]),
+        "orc_em_run": (ctypes.c_int, [c_i32, c_i64, c_d, c_d, c_d, c_d, c_i32, c_i32, _dp, _dp, _dp, _dp, _dp, _dp,
+                                      _dp, c_i64, c_vp, c_vp]),
         "orc23_run": (ctypes.c_int, [c_i32, c_i32, c_d, c_d, c_d, _dp, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32,
                                      _ip32, _ip64, _dp, _dp, _dp, _dp, _dp, _dp, c_i64, c_i64, c_vp, c_vp, c_vp,
                                      c_vp, c_i64, c_i32]),
@@ -557,3 +566,77 @@ def run23(NX, NY, Lx, Ly, dt, B, species, nsteps, t0=0, buffer=3, ntskip=1, ngsk
         out.append(d)
         o += 5 * n
     return out, (Ex, Ey), rows, bins, bool(bad)
+
+
+# ---- 1D electromagnetic fixed point (src/GaussianEMFixedPoint.jl) ----------
+# Particle velocities are a (3, P) array of rows v[:,1], v[:,2], v[:,3].
+
+def em_irn(c, N):
+    return lib().orc_em_irn(float(c), int(N))
+
+
+def em_stencil(N, c):
+    cells, ff = np.zeros(13, dtype=np.int64), np.zeros(13)
+    lib().orc_em_stencil(int(N), float(c), cells, ff)
+    return cells, ff
+
+
+def em_frac_bits(P, w, M):
+    return lib().orc_em_frac_bits(int(P), float(w), float(M))
+
+
+def em_deposit(N, y, w, u=None):
+    y = _f64(y)
+    rho = np.zeros(N)
+    uu = None if u is None else _f64(u)
+    lib().orc_em_deposit(int(N), y.size, y, _ptr(uu), float(w), rho)
+    return rho
+
+
+def em_fields(eps, dt, rho, Jy, Jz, Fy, Fz, By, Bz, Cy, Cz):
+    """One sweep's field updates from the iterate (By, Bz); returns
+    (Ex, Ey, Ez, By, Bz, ey, ez, by, bz)."""
+    rho, Jy, Jz, Fy, Fz, Cy, Cz = (_f64(a) for a in (rho, Jy, Jz, Fy, Fz, Cy, Cz))
+    N = rho.size
+    E = np.zeros(3 * N)
+    B = np.concatenate([_f64(By), _f64(Bz)])
+    e3, b2 = np.zeros(3 * N), np.zeros(2 * N)
+    lib().orc_em_fields(N, float(eps), float(dt), rho, Jy, Jz, Fy, Fz, Cy, Cz, E, B, e3, b2)
+    return E[:N], E[N:2 * N], E[2 * N:], B[:N], B[N:], e3[N:2 * N], e3[2 * N:], b2[:N], b2[N:]
+
+
+def em_push(c, ex, ey, ez, by, bz, dt, u):
+    ex, ey, ez, by, bz = (_f64(a) for a in (ex, ey, ez, by, bz))
+    v = np.zeros(3)
+    lib().orc_em_push(ex.size, float(c), ex, ey, ez, by, bz, float(dt), _f64(u), v)
+    return v
+
+
+def em_sort(x, v):
+    x = _f64(x).copy()
+    v = [_f64(v[k]).copy() for k in range(3)]
+    perm = np.zeros(x.size, dtype=np.int64)
+    lib().orc_em_sort(x.size, x, v[0], v[1], v[2], perm)
+    return x, np.vstack(v), perm
+
+
+def em_run(N, eps, dt, w, x, v, nsteps, E=None, B=None, Fz=None, rtol=1e-8, max_sweeps=3, sort=1,
+           want_fields=True):
+    """nsteps of the script's loop.  E: (3, N) Ex, Ey, Ez (zeros); B: (2, N)
+    By, Bz (zeros and ones); Fz (zeros).  Returns (x, v, E, B, Fz, fields,
+    sweeps), fields of shape (12, nsteps, N) (fields[k, t] = the script's
+    fields[:, t, k+1])."""
+    x = _f64(x).copy()
+    v = [_f64(v[k]).copy() for k in range(3)]
+    E = np.zeros(3 * N) if E is None else _f64(E).ravel().copy()
+    B = np.concatenate([np.zeros(N), np.ones(N)]) if B is None else _f64(B).ravel().copy()
+    Fz = np.zeros(N) if Fz is None else _f64(Fz).copy()
+    fields = np.zeros(12 * nsteps * N) if want_fields else None
+    sweeps = np.zeros(nsteps, dtype=np.int32)
+    rc = lib().orc_em_run(int(N), x.size, float(eps), float(dt), float(w), float(rtol), int(max_sweeps), int(sort),
+                          x, v[0], v[1], v[2], E, B, Fz, int(nsteps), _ptr(fields),
+                          sweeps.ctypes.data_as(ctypes.c_void_p))
+    if rc != 0:
+        raise ValueError(f"orc_em_run rejected its arguments (rc={rc})")
+    f = None if fields is None else fields.reshape(12, nsteps, N)
+    return x, np.vstack(v), E.reshape(3, N), B.reshape(2, N), Fz, f, sweeps

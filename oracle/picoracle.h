@@ -173,6 +173,21 @@ int      orc23_run(int32_t NX, int32_t NY, double Lx, double Ly, double dt, cons
                    int64_t nsteps, double* rows, double* Exs, double* Eys, double* phis, int64_t nbins,
                    int32_t nthreads);
 
+/* 1D electromagnetic fixed point (src/GaussianEMFixedPoint.jl), in em1d.c */
+int64_t orc_em_irn(double c, int32_t N);
+void    orc_em_stencil(int32_t N, double c, int64_t* cells, double* ff);
+int     orc_em_frac_bits(int64_t P, double w, double M);
+void    orc_em_deposit(int32_t N, int64_t P, const double* y, const double* u, double w, double* rho);
+void    orc_em_fields(int32_t N, double eps, double dt, const double* rho, const double* Jy, const double* Jz,
+                      const double* Fy, const double* Fz, const double* Cy, const double* Cz, double* E, double* B,
+                      double* e3, double* b2);
+void    orc_em_push(int32_t N, double c, const double* ex, const double* ey, const double* ez, const double* by,
+                    const double* bz, double dt, const double* u, double* v);
+void    orc_em_sort(int64_t P, double* x, double* v1, double* v2, double* v3, int64_t* perm);
+int     orc_em_run(int32_t N, int64_t P, double eps, double dt, double w, double rtol, int32_t max_sweeps,
+                   int32_t sort, double* x, double* v1, double* v2, double* v3, double* E, double* B, double* Fz,
+                   int64_t nsteps, double* fields, int32_t* sweeps);
+
 
 #ifdef __cplusplus
 }
