@@ -52,6 +52,11 @@ EXPORTS_2D3V = [
     "pic2d3v_reset", "pic2d3v_sort", "pic2d3v_run", "pic2d3v_fields", "pic2d3v_field", "pic2d3v_deposit",
     "pic2d3v_gather", "pic2d3v_boris", "pic2d3v_fieldsolve",
 ]
+# every symbol include/picem1d.h declares
+EXPORTS_EM1D = [
+    "picem1d_create", "picem1d_destroy", "picem1d_last_error", "picem1d_upload", "picem1d_download",
+    "picem1d_run", "picem1d_deposit", "picem1d_fields", "picem1d_push",
+]
 MAX_SPECIES_2D3V = 4
 NGP_2D3V, AREA_2D3V, BSPLINE0_2D3V = 0, 1, 2
 FIELD_ACCUMULATE, FIELD_REPLACE = 0, 1
@@ -109,6 +114,14 @@ class pic2d3v_config(ctypes.Structure):
         ("ntskip", ctypes.c_int32), ("ngskip", ctypes.c_int32), ("field_update", ctypes.c_int32),
         ("boris", ctypes.c_int32), ("nspecies", ctypes.c_int32),
         ("species", pic2d3v_species * MAX_SPECIES_2D3V), ("device", ctypes.c_int32),
+    ]
+
+
+class picem1d_config(ctypes.Structure):
+    _fields_ = [
+        ("N", ctypes.c_int32), ("max_sweeps", ctypes.c_int32), ("P", ctypes.c_int64), ("eps", ctypes.c_double),
+        ("dt", ctypes.c_double), ("w", ctypes.c_double), ("rtol", ctypes.c_double), ("sort", ctypes.c_int32),
+        ("device", ctypes.c_int32),
     ]
 
 
@@ -199,6 +212,16 @@ def load():
         "pic2d3v_boris": [vp, vp, vp, vp, vp, vp, i64, d],
         "pic2d3v_fieldsolve": [vp, vp, vp, vp, vp],
     })
+    sig.update({
+        "picem1d_create": [ctypes.POINTER(vp), ctypes.POINTER(picem1d_config)],
+        "picem1d_destroy": [vp],
+        "picem1d_upload": [vp, vp, vp, vp, vp],
+        "picem1d_download": [vp, vp, vp, vp, vp],
+        "picem1d_run": [vp, i64, vp, vp],
+        "picem1d_deposit": [vp, vp, vp, i64, d, vp],
+        "picem1d_fields": [vp] + [vp] * 12,
+        "picem1d_push": [vp, vp, vp, vp, vp, i64, vp],
+    })
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -211,6 +234,8 @@ def load():
     L.pic1d2v_last_error.restype = ctypes.c_char_p
     L.pic2d3v_last_error.argtypes = [vp]
     L.pic2d3v_last_error.restype = ctypes.c_char_p
+    L.picem1d_last_error.argtypes = [vp]
+    L.picem1d_last_error.restype = ctypes.c_char_p
     L.pic_strerror.argtypes = [ctypes.c_int]
     L.pic_strerror.restype = ctypes.c_char_p
     L.pic_version.argtypes = []

@@ -233,3 +233,54 @@ def test_create2d3v_rejects_bad_config_with_message(change, msg):
     with pytest.raises(picgolf.PicError) as e:
         picgolf.Context2D3V(Config2D3V(**kw))
     assert e.value.code == -1 and msg in str(e.value)
+
+
+def test_header_em1d_and_binding_agree():
+    import picgolf
+    assert sorted(picgolf.EXPORTS_EM1D) == header_functions("picem1d.h", "picem1d_")
+
+
+def test_library_exports_every_em1d_symbol():
+    import picgolf
+    L = picgolf.load()
+    for name in header_functions("picem1d.h", "picem1d_"):
+        assert hasattr(L, name), name
+
+
+def test_em1d_struct_layout_matches_the_header(tmp_path):
+    import ctypes
+    import subprocess
+    from picgolf import _lib as L
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "picem1d.h"', "int main(void) {",
+             'printf("size %zu\\n", sizeof(picem1d_config));']
+    for f, _ in L.picem1d_config._fields_:
+        lines.append(f'printf("{f} %zu\\n", offsetof(picem1d_config, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                 text=True).stdout.split("\n") if l)
+    assert int(got["size"]) == ctypes.sizeof(L.picem1d_config)
+    for f, _ in L.picem1d_config._fields_:
+        assert int(got[f]) == getattr(L.picem1d_config, f).offset, f
+
+
+@pytest.mark.parametrize("change,msg", [
+    (dict(N=100), "power of two"),
+    (dict(N=2048), "power of two"),
+    (dict(P=0), "P must"),
+    (dict(max_sweeps=0), "max_sweeps"),
+    (dict(eps=0.0), "eps and dt"),
+    (dict(dt=float("inf")), "eps and dt"),
+    (dict(rtol=-1.0), "rtol"),
+])
+def test_create_em1d_rejects_bad_config_with_message(change, msg):
+    import picgolf
+    from picgolf.core_em1d import ConfigEM1D
+    kw = dict(N=64, P=1024, eps=0.1, dt=1e-4, w=25.0)
+    kw.update(change)
+    with pytest.raises(picgolf.PicError) as e:
+        picgolf.ContextEM1D(ConfigEM1D(**kw))
+    assert e.value.code == -1 and msg in str(e.value)
