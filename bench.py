@@ -35,9 +35,11 @@ def parse():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the ensemble and 2D3V workloads reported beside the headline")
     ap.add_argument("--profile-steps", type=int, default=3)
-    ap.add_argument("--workload", default="scaled", choices=["scaled", "ensemble", "es2d3v", "pic2d3v"],
+    ap.add_argument("--workload", default="scaled", choices=["scaled", "ensemble", "es2d3v", "pic2d3v", "em1d"],
                     help="scaled: the headline line (default); ensemble / es2d3v: only that workload, "
                          "--steps timed steps, one JSON line (for profiling)")
+    ap.add_argument("--script-steps", type=int, default=4048,
+                    help="em1d: steps of the script-sized run (the script's T = 4048)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default particle tiling instead of autotuning it before the timed steps")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -303,6 +305,53 @@ def bench_pic2d3v(steps=20, warmup=2, N=1024, ppc=16, shape="bspline2"):
                              f"and solve on one"}}
 
 
+def bench_em1d(T=4048, scaled_steps=20):
+    """The 1D electromagnetic fixed-point script (src/GaussianEMFixedPoint.jl)
+    whole: N=64, P=16N, eps=1/10, dt=eps^2/N/2, W=400, T=4048 steps with the
+    twelve fields of every step and the per-step sort; and the same loop at
+    N=1024 with 2^22 particles.  FP64."""
+    import numpy as np
+    import picgolf
+    from picgolf.core_em1d import ConfigEM1D
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    res = {}
+    for name, N, P, steps in (("script", 64, 1024, T), ("scaled", 1024, 1 << 22, scaled_steps)):
+        eps = 0.1
+        dt = eps ** 2 / N / 2
+        w = 400 / P * N
+        rng = np.random.default_rng(0)
+        x = rng.random(P)
+        a = 2 * np.pi * rng.random(P)
+        v = np.vstack([np.sin(a) / 4, np.cos(a) / 4, np.zeros(P)])
+        with picgolf.ContextEM1D(ConfigEM1D(N=N, P=P, eps=eps, dt=dt, w=w)) as c:
+            c.upload(x, v)
+            c.run(2)                       # graph captured, caches warm
+            c.upload(x, v)
+            t0 = time.perf_counter()
+            f, sw = c.run(steps)
+            secs = time.perf_counter() - t0
+        res[name] = {"N": N, "P": P, "steps": steps, "ms_per_step": secs / steps * 1e3,
+                     "pushes_per_s": P * steps / secs, "mean_sweeps": float(sw.mean())}
+    # the CPU restatement: the script's set-up, 40 steps
+    N, P = 64, 1024
+    eps = 0.1
+    dt = eps ** 2 / N / 2
+    rng = np.random.default_rng(0)
+    x = rng.random(P)
+    a = 2 * np.pi * rng.random(P)
+    v = np.vstack([np.sin(a) / 4, np.cos(a) / 4, np.zeros(P)])
+    t0 = time.perf_counter()
+    oracle.em_run(N, eps, dt, 400 / P * N, x, v, 40)
+    cpu = time.perf_counter() - t0
+    return {"workload": f"1D electromagnetic fixed point (src/GaussianEMFixedPoint.jl): the script "
+                        f"(N=64, P=1024, {T} steps, fields every step, sort every step) and the same loop at "
+                        f"N=1024, P=2^22; FP64",
+            **res,
+            "cpu_baseline": {"value": P * 40 / cpu, "unit": "particle-pushes/s", "cores": 1, "kind": "port",
+                             "sample": "FP64 CPU restatement, the script's set-up, 40 steps, one thread"}}
+
+
 def cpu_baseline(cfg, x, v, budget_s=15.0):
     """The CPU restatement (FP64) timed on this host on a bounded sample: the
     same N and stencil with the first n particles, steps until ~budget_s."""
@@ -352,6 +401,9 @@ def main():
         return
     if a.workload == "pic2d3v":
         print(json.dumps(bench_pic2d3v(steps=a.steps, warmup=a.warmup)))
+        return
+    if a.workload == "em1d":
+        print(json.dumps(bench_em1d(T=a.script_steps, scaled_steps=a.steps)))
         return
     import numpy as np
     from picgolf import configs
@@ -409,7 +461,7 @@ def main():
         out["momentum_drift_mixed"] = float(np.max(np.abs(m["D"][:, 3] - m["D"][0, 3])))
     if not a.no_extra:
         out["workloads"] = {"ensemble_512": bench_ensemble(autotune=not a.no_autotune), "es2d3v_256": bench_2d(),
-                            "pic2d3v_1024": bench_pic2d3v()}
+                            "pic2d3v_1024": bench_pic2d3v(), "em1d": bench_em1d()}
         print(f"[bench] extra workloads: {json.dumps(out['workloads'])}", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(*base)

@@ -39,7 +39,8 @@ export Context, PicConfig, deposit!, fieldsolve!, drift!, midpoint_position!, wr
        diagnostics, upload!, download, run!, quiet_start, autotune!, set_tiling!, tiling, PicTuning,
        simpson_kick!, Pic1D2VConfig, Context1D2V, fields,
        Pic2DConfig, Context2D, gather, boris!,
-       Pic2D3VSpecies, Pic2D3VConfig, Context2D3V, reset!, field
+       Pic2D3VSpecies, Pic2D3VConfig, Context2D3V, reset!, field,
+       PicEM1DConfig, ContextEM1D
 
 const libpicgolf = get(ENV, "PICGOLF_LIB", normpath(joinpath(@__DIR__, "..", "lib", "libpicgolf.so")))
 
@@ -900,6 +901,147 @@ function fieldsolve!(Ex::Matrix{Float64}, Ey::Matrix{Float64}, ctx::Context2D3V,
                                       ctx.ptr, ρ, Ex, Ey, ϕ === nothing ? C_NULL : ϕ)
     check(ctx, rc, "pic2d3v_fieldsolve")
     (Ex, Ey)
+end
+
+# ---------------------------------------------------------------------------
+# 1D electromagnetic fixed point (src/GaussianEMFixedPoint.jl), C ABI in
+# include/picem1d.h.
+#
+# | script expression                                          | PICGolf                              |
+# |------------------------------------------------------------|--------------------------------------|
+# | `d(y, u)` (line 11)                                          | `deposit(ctx, y, w; u)`              |
+# | one sweep's `Ex .= ...` .. `bz=(Bz.+Cz)/2` (lines 20-27)     | `fields!(ctx, ρ, Jy, Jz, F, C, E, B)` |
+# | `v[j,:] .= u .+ (a .+ cross(u .+ a*dt/2, b))*dt` (28-29)     | `push(ctx, c, e3, b2, u)`            |
+# | the `@gif for t ∈ 1:T` loop and `fields` (lines 13-50)       | `upload!`, `run!`, `download`        |
+# ---------------------------------------------------------------------------
+
+"Mirror of `struct picem1d_config` in include/picem1d.h."
+struct PicEM1DConfig
+    N::Int32
+    max_sweeps::Int32
+    P::Int64
+    eps::Float64
+    dt::Float64
+    w::Float64
+    rtol::Float64
+    sort::Int32
+    device::Int32
+end
+
+"""
+    PicEM1DConfig(; N, P, eps, dt, w, rtol=1e-8, max_sweeps=3, sort=true, device=0)
+
+The script's N, P, ϵ, dt = ϵ^2/N/2, w = W/P*N, η, the three sweeps of
+`for _∈0:2`, and the `sortperm(x)` after every step.
+"""
+PicEM1DConfig(; N, P, eps, dt, w, rtol=1e-8, max_sweeps=3, sort=true, device=0) =
+    PicEM1DConfig(Int32(N), Int32(max_sweeps), Int64(P), Float64(eps), Float64(dt), Float64(w), Float64(rtol),
+                  Int32(sort), Int32(device))
+
+mutable struct ContextEM1D
+    ptr::Ptr{Cvoid}
+    cfg::PicEM1DConfig
+end
+
+function check(ctx::ContextEM1D, rc::Cint, what::AbstractString)
+    rc == 0 || error("$what: ", unsafe_string(ccall((:picem1d_last_error, libpicgolf), Cstring, (Ptr{Cvoid},), ctx.ptr)))
+    nothing
+end
+
+function ContextEM1D(cfg::PicEM1DConfig)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:picem1d_create, libpicgolf), Cint, (Ref{Ptr{Cvoid}}, Ref{PicEM1DConfig}), h, cfg)
+    rc == 0 || error("picem1d_create: ", unsafe_string(ccall((:picem1d_last_error, libpicgolf), Cstring,
+                                                             (Ptr{Cvoid},), C_NULL)))
+    ctx = ContextEM1D(h[], cfg)
+    finalizer(destroy!, ctx)
+    ctx
+end
+
+function destroy!(ctx::ContextEM1D)
+    if ctx.ptr != C_NULL
+        ccall((:picem1d_destroy, libpicgolf), Cint, (Ptr{Cvoid},), ctx.ptr)
+        ctx.ptr = C_NULL
+    end
+    nothing
+end
+
+"""
+    upload!(ctx::ContextEM1D, x, v; E=nothing, B=nothing)
+
+The script's `x` and `v` (P x 3); `E` is N x 3 (Ex, Ey, Ez; zeros), `B` is
+N x 2 (By, Bz; the script's zeros and ones).  Restarts the script's Fz.
+"""
+function upload!(ctx::ContextEM1D, x::Vector{Float64}, v::Matrix{Float64};
+                 E::Union{Matrix{Float64},Nothing}=nothing, B::Union{Matrix{Float64},Nothing}=nothing)
+    rc = GC.@preserve x v E B ccall((:picem1d_upload, libpicgolf), Cint,
+                                    (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                                    ctx.ptr, x, v, E === nothing ? C_NULL : E, B === nothing ? C_NULL : B)
+    check(ctx, rc, "picem1d_upload")
+    ctx
+end
+
+"`download(ctx::ContextEM1D) -> (x, v, E, B)` (v: P x 3, E: N x 3, B: N x 2)."
+function download(ctx::ContextEM1D)
+    P, N = ctx.cfg.P, ctx.cfg.N
+    x, v, E, B = zeros(P), zeros(P, 3), zeros(N, 3), zeros(N, 2)
+    rc = GC.@preserve x v E B ccall((:picem1d_download, libpicgolf), Cint,
+                                    (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                                    ctx.ptr, x, v, E, B)
+    check(ctx, rc, "picem1d_download")
+    (x, v, E, B)
+end
+
+"""
+    run!(ctx::ContextEM1D, T) -> (fields, sweeps)
+
+`T` steps of the loop; `fields` is the script's `fields` (N x T x 12) for
+these steps.
+"""
+function run!(ctx::ContextEM1D, T::Integer)
+    f = zeros(ctx.cfg.N, T, 12)
+    sw = zeros(Int32, T)
+    rc = GC.@preserve f sw ccall((:picem1d_run, libpicgolf), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Int32}),
+                                 ctx.ptr, T, f, sw)
+    check(ctx, rc, "picem1d_run")
+    (f, sw)
+end
+
+"`deposit(ctx::ContextEM1D, y, w; u=nothing) -> r`: `d(y, u)`."
+function deposit(ctx::ContextEM1D, y::Vector{Float64}, w::Real; u::Union{Vector{Float64},Nothing}=nothing)
+    r = zeros(ctx.cfg.N)
+    rc = GC.@preserve y u r ccall((:picem1d_deposit, libpicgolf), Cint,
+                                  (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Ptr{Float64}),
+                                  ctx.ptr, y, u === nothing ? C_NULL : u, length(y), w, r)
+    check(ctx, rc, "picem1d_deposit")
+    r
+end
+
+"""
+    fields!(ctx::ContextEM1D, ρ, Jy, Jz, Fx, Fy, Fz, Cy, Cz, E, B) -> (e3, b2)
+
+One sweep's updates: `E` (N x 3) and `B` (N x 2, the iterate's By, Bz) in
+place; returns `ex, ey, ez` (N x 3) and `by, bz` (N x 2).
+"""
+function fields!(ctx::ContextEM1D, ρ, Jy, Jz, Fx, Fy, Fz, Cy, Cz, E::Matrix{Float64}, B::Matrix{Float64})
+    N = ctx.cfg.N
+    e3, b2 = zeros(N, 3), zeros(N, 2)
+    rc = GC.@preserve ρ Jy Jz Fx Fy Fz Cy Cz E B e3 b2 ccall((:picem1d_fields, libpicgolf), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+         Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+        ctx.ptr, ρ, Jy, Jz, Fx, Fy, Fz, Cy, Cz, E, B, e3, b2)
+    check(ctx, rc, "picem1d_fields")
+    (e3, b2)
+end
+
+"`push(ctx::ContextEM1D, c, e3, b2, u) -> v` (u, v: n x 3): the velocity update at midpoints c."
+function push(ctx::ContextEM1D, c::Vector{Float64}, e3::Matrix{Float64}, b2::Matrix{Float64}, u::Matrix{Float64})
+    v = zeros(size(u))
+    rc = GC.@preserve c e3 b2 u v ccall((:picem1d_push, libpicgolf), Cint,
+                                        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64,
+                                         Ptr{Float64}), ctx.ptr, c, e3, b2, u, length(c), v)
+    check(ctx, rc, "picem1d_push")
+    v
 end
 
 end # module

@@ -7,7 +7,7 @@ import re
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 JL = os.path.join(ROOT, "particleincellcodegolf.jl_amd", "julia", "PICGolf.jl")
-HEADERS = ["pic.h", "pic2d.h", "pic1d2v.h", "pic2d3v.h"]
+HEADERS = ["pic.h", "pic2d.h", "pic1d2v.h", "pic2d3v.h", "picem1d.h"]
 
 C_TYPES = {"int32_t": "Int32", "int64_t": "Int64", "double": "Float64", "pic2d3v_species": "Pic2D3VSpecies"}
 
@@ -62,5 +62,5 @@ def test_every_ccall_is_declared_and_exported():
 def test_struct_mirrors_match_the_headers():
     for jl, c in (("PicConfig", "pic_config"), ("Pic2DConfig", "pic2d_config"), ("Pic1D2VConfig", "pic1d2v_config"),
                   ("PicTuning", "pic_tuning"), ("Pic2D3VSpecies", "pic2d3v_species"),
-                  ("Pic2D3VConfig", "pic2d3v_config")):
+                  ("Pic2D3VConfig", "pic2d3v_config"), ("PicEM1DConfig", "picem1d_config")):
         assert jl_struct(jl) == c_struct(c), jl
