@@ -4,7 +4,10 @@
  * wrapped cell, exact signed charge and current sums, the spectral
  * Gauss/Ampere/Faraday updates of Ex, Ey, Ez, By, Bz with c = 1/eps, the
  * implicit-midpoint velocity update, up to three fixed-point sweeps tested on
- * Ex, the wrap, the per-step sort by x and the twelve moment fields.
+ * Ex, the wrap, the per-step sort by x and the twelve moment fields; and the
+ * explicit step of src/GaussianEMExplicit.jl on the same operators (Gauss,
+ * leapfrog Faraday then Ampere, the split-kick velocity update, the current
+ * at (X + x)/2 with X left unsorted).
  *
  * TEST INFRASTRUCTURE ONLY (see picoracle.h): the oracle the GPU EM path is
  * checked against.
@@ -22,6 +25,12 @@
  *                     current term each step); one sweep when Ex does not move;
  *                     the moment fields equal deposits of the final state; the
  *                     sort equals numpy's stable argsort; two calls equal one.
+ *   orc_emx_*         single-mode closed forms of Gauss and both curls (numpy
+ *                     FFT), the split kick's closed form in complex notation
+ *                     (E kick, B turn by (1 - i b dt), E kick), the vacuum
+ *                     leapfrog's per-mode matrix, the unsorted-X pairing of
+ *                     the current against a numpy restatement of the step,
+ *                     and two calls equal one.
  */
 #include "picoracle.h"
 
@@ -135,34 +144,12 @@ static void rif(int32_t N, double* re, double* im, double* out) {
     for (int32_t j = 0; j < N; ++j) out[j] = re[j] * s;
 }
 
-/* One sweep's field updates (:20-27), given rho, Jy, Jz at the sweep's
- * midpoints and the saved Fx, Fy, Fz, Cy, Cz:
- *   Ex = real(ifft((xi = fft(rho) ./ ik; xi[1] *= false; xi)))
- *   by = (By + Cy)/2, bz = (Bz + Cz)/2
- *   Ey = Fy + dt*rif(-ik .* fft(bz)/eps^2 .- fft(Jy))
- *   Ez = Fz + dt*rif(+ik .* fft(by)/eps^2 .- fft(Jz))
- *   ex, ey, ez = (E + F)/2
- *   By = Cy - dt*rif(-ik .* fft(ez)),  Bz = Cz - dt*rif(ik .* fft(ey))
- *   by, bz = (B + C)/2
- * Complex products are Julia's (ac - bd, ad + bc), complex/real divides both
- * parts.  fft(rho)[1]/ik[1] = (0,0)/(0,0) is NaN, and `*= false` makes it
+/* Gauss (:20): Ex = real(ifft((xi = fft(rho) ./ ik; xi[1] *= false; xi))).
+ * fft(rho)[1]/ik[1] = (0,0)/(0,0) is NaN, and `*= false` makes it
  * copysign(0, NaN): x86's default NaN is negative, so xi[1] = (-0.0, -0.0).
  * The other ratios are Julia's robust division by (0.0*m, 2pi*m), which
- * reduces to t = 1/(2pi*m), (Im*t, -(Re*t)).  Outputs: E[3N] (Ex, Ey, Ez),
- * B[2N] (By, Bz), e3[3N] (ex, ey, ez), b2[2N] (by, bz). */
-void orc_em_fields(int32_t N, double eps, double dt, const double* rho, const double* Jy, const double* Jz,
-                   const double* Fy, const double* Fz, const double* Cy, const double* Cz, double* E, double* B,
-                   double* e3, double* b2) {
-    const size_t n = (size_t)N;
-    double* re = (double*)malloc(sizeof(double) * n);
-    double* im = (double*)malloc(sizeof(double) * n);
-    double* jr = (double*)malloc(sizeof(double) * n);
-    double* ji = (double*)malloc(sizeof(double) * n);
-    double* tmp = (double*)malloc(sizeof(double) * n);
-    double *Ex = E, *Ey = E + N, *Ez = E + 2 * N, *By = B, *Bz = B + N;
-    double *ex = e3, *ey = e3 + N, *ez = e3 + 2 * N, *by = b2, *bz = b2 + N;
-    const double e2 = eps * eps;
-    /* Gauss */
+ * reduces to t = 1/(2pi*m), (Im*t, -(Re*t)).  re, im: N scratch each. */
+static void gauss(int32_t N, const double* rho, double* re, double* im, double* Ex) {
     fft_real(N, rho, re, im);
     for (int32_t j = 0; j < N; ++j) {
         if (j == 0) {
@@ -178,6 +165,33 @@ void orc_em_fields(int32_t N, double eps, double dt, const double* rho, const do
         im[j] = xi;
     }
     rif(N, re, im, Ex);
+}
+
+/* One sweep's field updates (:20-27), given rho, Jy, Jz at the sweep's
+ * midpoints and the saved Fx, Fy, Fz, Cy, Cz:
+ *   Ex = real(ifft((xi = fft(rho) ./ ik; xi[1] *= false; xi)))
+ *   by = (By + Cy)/2, bz = (Bz + Cz)/2
+ *   Ey = Fy + dt*rif(-ik .* fft(bz)/eps^2 .- fft(Jy))
+ *   Ez = Fz + dt*rif(+ik .* fft(by)/eps^2 .- fft(Jz))
+ *   ex, ey, ez = (E + F)/2
+ *   By = Cy - dt*rif(-ik .* fft(ez)),  Bz = Cz - dt*rif(ik .* fft(ey))
+ *   by, bz = (B + C)/2
+ * Complex products are Julia's (ac - bd, ad + bc), complex/real divides both
+ * parts; Gauss is gauss() above.  Outputs: E[3N] (Ex, Ey, Ez),
+ * B[2N] (By, Bz), e3[3N] (ex, ey, ez), b2[2N] (by, bz). */
+void orc_em_fields(int32_t N, double eps, double dt, const double* rho, const double* Jy, const double* Jz,
+                   const double* Fy, const double* Fz, const double* Cy, const double* Cz, double* E, double* B,
+                   double* e3, double* b2) {
+    const size_t n = (size_t)N;
+    double* re = (double*)malloc(sizeof(double) * n);
+    double* im = (double*)malloc(sizeof(double) * n);
+    double* jr = (double*)malloc(sizeof(double) * n);
+    double* ji = (double*)malloc(sizeof(double) * n);
+    double* tmp = (double*)malloc(sizeof(double) * n);
+    double *Ex = E, *Ey = E + N, *Ez = E + 2 * N, *By = B, *Bz = B + N;
+    double *ex = e3, *ey = e3 + N, *ez = e3 + 2 * N, *by = b2, *bz = b2 + N;
+    const double e2 = eps * eps;
+    gauss(N, rho, re, im, Ex);
     /* the averaged B of the current iterate */
     for (int32_t j = 0; j < N; ++j) {
         by[j] = (By[j] + Cy[j]) / 2.0;
@@ -376,5 +390,165 @@ int orc_em_run(int32_t N, int64_t P, double eps, double dt, double w, double rto
     }
     free(X); free(V1); free(V2); free(V3); free(c); free(uy); free(uz); free(sq);
     free(rho); free(Jy); free(Jz); free(Fx); free(Fy); free(Cy); free(Cz); free(e3); free(b2);
+    return 0;
+}
+
+
+/* ------------------------------------------------------------------------ */
+/* The explicit script, src/GaussianEMExplicit.jl:12-30.  One step:          */
+/*   X .= x; Ex .= Gauss(d(X))                                          :13  */
+/*   x .= X .+ v[:,1]*dt; x .= mod.(x, 1); p = sortperm(x); x = x[p];        */
+/*   v = v[p, :]                                                        :14  */
+/*   Cy .= By; Cz .= Bz; By .-= dt.*rif(-ik.*fft(Ez));                        */
+/*   Bz .-= dt.*rif(ik.*fft(Ey)); by, bz = (B .+ C)./2                :15-16 */
+/*   v[j] += a*dt/2; v[j] += cross(v[j], b)*dt; v[j] += a*dt/2 at s(x[j]) :17-22 */
+/*   Ey .+= dt.*rif(-ik.*fft(Bz)/eps^2 .- fft(d((X.+x)/2, v[:,2])))           */
+/*   Ez .+= dt.*rif(+ik.*fft(By)/eps^2 .- fft(d((X.+x)/2, v[:,3])))    :23-24 */
+/*   fields[:, t, 1:12] as in the fixed point                         :25-29 */
+/* X is not permuted with x: (X .+ x)/2 pairs the pre-sort position of one   */
+/* particle with the sorted position of another, and the script does so.     */
+/* ------------------------------------------------------------------------ */
+
+/* out = rif(sign*ik .* fft(a)) (sign < 0: -ik), or, with J, rif(sign*ik .*
+ * fft(a)/e2 .- fft(J)) */
+static void curl_term(int32_t N, int sign, const double* a, double e2, const double* J, double* out) {
+    const size_t n = (size_t)N;
+    double* re = (double*)malloc(sizeof(double) * n);
+    double* im = (double*)malloc(sizeof(double) * n);
+    double* jr = (double*)malloc(sizeof(double) * n);
+    double* ji = (double*)malloc(sizeof(double) * n);
+    fft_real(N, a, re, im);
+    if (J) fft_real(N, J, jr, ji);
+    for (int32_t j = 0; j < N; ++j) {
+        double kr, ki;
+        ik_of(N, j, &kr, &ki);
+        if (sign < 0) {
+            kr = -kr;
+            ki = -ki;
+        }
+        double pr = kr * re[j] - ki * im[j];
+        double pi = kr * im[j] + ki * re[j];
+        if (J) {
+            pr = pr / e2 - jr[j];
+            pi = pi / e2 - ji[j];
+        }
+        re[j] = pr;
+        im[j] = pi;
+    }
+    rif(N, re, im, out);
+    free(re); free(im); free(jr); free(ji);
+}
+
+/* :13 and :15-16: Ex from rho; By, Bz from the old Ey, Ez; b2 = (by, bz). */
+void orc_emx_gauss_faraday(int32_t N, double dt, const double* rho, double* E, double* B, double* b2) {
+    const size_t n = (size_t)N;
+    double* re = (double*)malloc(sizeof(double) * n);
+    double* im = (double*)malloc(sizeof(double) * n);
+    double* ty = (double*)malloc(sizeof(double) * n);
+    double* tz = (double*)malloc(sizeof(double) * n);
+    double *Ex = E, *Ey = E + N, *Ez = E + 2 * N, *By = B, *Bz = B + N;
+    gauss(N, rho, re, im, Ex);
+    curl_term(N, -1, Ez, 0.0, NULL, ty);
+    curl_term(N, +1, Ey, 0.0, NULL, tz);
+    for (int32_t j = 0; j < N; ++j) {
+        const double cy = By[j], cz = Bz[j];
+        By[j] = cy - dt * ty[j];
+        Bz[j] = cz - dt * tz[j];
+        b2[j] = (By[j] + cy) / 2.0;
+        b2[N + j] = (Bz[j] + cz) / 2.0;
+    }
+    free(re); free(im); free(ty); free(tz);
+}
+
+/* :23-24: Ey, Ez from the new By, Bz and the currents Jy, Jz. */
+void orc_emx_ampere(int32_t N, double eps, double dt, const double* Jy, const double* Jz, double* E, const double* B) {
+    const size_t n = (size_t)N;
+    double* ty = (double*)malloc(sizeof(double) * n);
+    double* tz = (double*)malloc(sizeof(double) * n);
+    const double e2 = eps * eps;
+    curl_term(N, -1, B + N, e2, Jy, ty);
+    curl_term(N, +1, B, e2, Jz, tz);
+    for (int32_t j = 0; j < N; ++j) {
+        E[N + j] = E[N + j] + dt * ty[j];
+        E[2 * N + j] = E[2 * N + j] + dt * tz[j];
+    }
+    free(ty); free(tz);
+}
+
+/* :17-22 for one particle at x: a and b are left folds over the stencil from
+ * its first term (Bx is the script's zero array); then
+ * v += a*dt/2; v += cross(v, b)*dt; v += a*dt/2. */
+void orc_emx_push(int32_t N, double x, const double* E, const double* b2, double dt, double* v) {
+    int64_t cells[13];
+    double ff[13];
+    orc_em_stencil(N, x, cells, ff);
+    const double *Ex = E, *Ey = E + N, *Ez = E + 2 * N, *by = b2, *bz = b2 + N;
+    double a1 = 0, a2 = 0, a3 = 0, b1 = 0, b2y = 0, b3 = 0;
+    for (int k = 0; k < 13; ++k) {
+        const int64_t m = cells[k];
+        const double t1 = Ex[m] * ff[k], t2 = Ey[m] * ff[k], t3 = Ez[m] * ff[k];
+        const double s1 = 0.0 * ff[k], s2 = by[m] * ff[k], s3 = bz[m] * ff[k];
+        if (k == 0) {
+            a1 = t1; a2 = t2; a3 = t3;
+            b1 = s1; b2y = s2; b3 = s3;
+        } else {
+            a1 = a1 + t1; a2 = a2 + t2; a3 = a3 + t3;
+            b1 = b1 + s1; b2y = b2y + s2; b3 = b3 + s3;
+        }
+    }
+    const double u1 = v[0] + a1 * dt / 2.0, u2 = v[1] + a2 * dt / 2.0, u3 = v[2] + a3 * dt / 2.0;
+    const double c1 = u2 * b3 - u3 * b2y, c2 = u3 * b1 - u1 * b3, c3 = u1 * b2y - u2 * b1;
+    const double w1 = u1 + c1 * dt, w2 = u2 + c2 * dt, w3 = u3 + c3 * dt;
+    v[0] = w1 + a1 * dt / 2.0;
+    v[1] = w2 + a2 * dt / 2.0;
+    v[2] = w3 + a3 * dt / 2.0;
+}
+
+int orc_emx_run(int32_t N, int64_t P, double eps, double dt, double w, int32_t sort, double* x, double* v1,
+                double* v2, double* v3, double* E, double* B, int64_t nsteps, double* fields) {
+    if (N < 8 || (N & (N - 1)) || P < 1) return -1;
+    const size_t n = (size_t)N, np = (size_t)P;
+    double* X = (double*)malloc(sizeof(double) * np);
+    double* c = (double*)malloc(sizeof(double) * np);
+    double* sq = (double*)malloc(sizeof(double) * np);
+    double* rho = (double*)malloc(sizeof(double) * n);
+    double* Jy = (double*)malloc(sizeof(double) * n);
+    double* Jz = (double*)malloc(sizeof(double) * n);
+    double* b2 = (double*)malloc(sizeof(double) * 2 * n);
+    for (int64_t t = 0; t < nsteps; ++t) {
+        memcpy(X, x, sizeof(double) * np);
+        orc_em_deposit(N, P, X, NULL, w, rho);
+        for (int64_t j = 0; j < P; ++j) x[j] = orc_jl_mod1f(X[j] + v1[j] * dt);
+        if (sort) orc_em_sort(P, x, v1, v2, v3, NULL);
+        orc_emx_gauss_faraday(N, dt, rho, E, B, b2);
+        for (int64_t j = 0; j < P; ++j) {
+            double vv[3] = {v1[j], v2[j], v3[j]};
+            orc_emx_push(N, x[j], E, b2, dt, vv);
+            v1[j] = vv[0];
+            v2[j] = vv[1];
+            v3[j] = vv[2];
+        }
+        for (int64_t j = 0; j < P; ++j) c[j] = (X[j] + x[j]) / 2.0;
+        orc_em_deposit(N, P, c, v2, w, Jy);
+        orc_em_deposit(N, P, c, v3, w, Jz);
+        orc_emx_ampere(N, eps, dt, Jy, Jz, E, B);
+        if (fields) {
+            double* o = fields + (size_t)t * n;
+            const size_t stride = (size_t)nsteps * n;
+            memcpy(o, E, sizeof(double) * n);
+            memcpy(o + stride, E + N, sizeof(double) * n);
+            memcpy(o + 2 * stride, E + 2 * N, sizeof(double) * n);
+            memcpy(o + 3 * stride, B, sizeof(double) * n);
+            memcpy(o + 4 * stride, B + N, sizeof(double) * n);
+            orc_em_deposit(N, P, x, NULL, w, o + 5 * stride);
+            const double* vs[3] = {v1, v2, v3};
+            for (int k = 0; k < 3; ++k) orc_em_deposit(N, P, x, vs[k], w, o + (6 + k) * stride);
+            for (int k = 0; k < 3; ++k) {
+                for (int64_t j = 0; j < P; ++j) sq[j] = vs[k][j] * vs[k][j];
+                orc_em_deposit(N, P, x, sq, w, o + (9 + k) * stride);
+            }
+        }
+    }
+    free(X); free(c); free(sq); free(rho); free(Jy); free(Jz); free(b2);
     return 0;
 }

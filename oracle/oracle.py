@@ -108,6 +108,11 @@ def lib():
         "orc_em_sort": (None, [c_i64, _dp, _dp, _dp, _dp, _ip64]),
         "orc_em_run": (ctypes.c_int, [c_i32, c_i64, c_d, c_d, c_d, c_d, c_i32, c_i32, _dp, _dp, _dp, _dp, _dp, _dp,
                                       _dp, c_i64, c_vp, c_vp]),
+        "orc_emx_gauss_faraday": (None, [c_i32, c_d, _dp, _dp, _dp, _dp]),
+        "orc_emx_ampere": (None, [c_i32, c_d, c_d, _dp, _dp, _dp, _dp]),
+        "orc_emx_push": (None, [c_i32, c_d, _dp, _dp, c_d, _dp]),
+        "orc_emx_run": (ctypes.c_int, [c_i32, c_i64, c_d, c_d, c_d, c_i32, _dp, _dp, _dp, _dp, _dp, _dp, c_i64,
+                                       c_vp]),
         "orc23_run": (ctypes.c_int, [c_i32, c_i32, c_d, c_d, c_d, _dp, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32,
                                      _ip32, _ip64, _dp, _dp, _dp, _dp, _dp, _dp, c_i64, c_i64, c_vp, c_vp, c_vp,
                                      c_vp, c_i64, c_i32]),
@@ -640,3 +645,49 @@ def em_run(N, eps, dt, w, x, v, nsteps, E=None, B=None, Fz=None, rtol=1e-8, max_
         raise ValueError(f"orc_em_run rejected its arguments (rc={rc})")
     f = None if fields is None else fields.reshape(12, nsteps, N)
     return x, np.vstack(v), E.reshape(3, N), B.reshape(2, N), Fz, f, sweeps
+
+
+def emx_gauss_faraday(dt, rho, E, B):
+    """src/GaussianEMExplicit.jl:13, :15-16: Ex from rho, (By, Bz) from the
+    old (Ey, Ez).  E (3, N), B (2, N).  Returns (E, B, b2) with b2 = (by, bz)."""
+    rho = _f64(rho)
+    N = rho.size
+    E = _f64(E).ravel().copy()
+    B = _f64(B).ravel().copy()
+    b2 = np.zeros(2 * N)
+    lib().orc_emx_gauss_faraday(N, float(dt), rho, E, B, b2)
+    return E.reshape(3, N), B.reshape(2, N), b2.reshape(2, N)
+
+
+def emx_ampere(eps, dt, Jy, Jz, E, B):
+    """src/GaussianEMExplicit.jl:23-24: (Ey, Ez) from (By, Bz) and the currents."""
+    Jy, Jz = _f64(Jy), _f64(Jz)
+    N = Jy.size
+    E = _f64(E).ravel().copy()
+    lib().orc_emx_ampere(N, float(eps), float(dt), Jy, Jz, E, _f64(B).ravel().copy())
+    return E.reshape(3, N)
+
+
+def emx_push(x, E, b2, dt, u):
+    """src/GaussianEMExplicit.jl:17-22 for one particle at x."""
+    E, b2 = _f64(E).ravel(), _f64(b2).ravel()
+    v = _f64(u).copy()
+    lib().orc_emx_push(E.size // 3, float(x), E, b2, float(dt), v)
+    return v
+
+
+def emx_run(N, eps, dt, w, x, v, nsteps, E=None, B=None, sort=1, want_fields=True):
+    """nsteps of src/GaussianEMExplicit.jl's loop.  E: (3, N) (zeros); B:
+    (2, N) By, Bz (zeros and ones).  Returns (x, v, E, B, fields), fields of
+    shape (12, nsteps, N)."""
+    x = _f64(x).copy()
+    v = [_f64(v[k]).copy() for k in range(3)]
+    E = np.zeros(3 * N) if E is None else _f64(E).ravel().copy()
+    B = np.concatenate([np.zeros(N), np.ones(N)]) if B is None else _f64(B).ravel().copy()
+    fields = np.zeros(12 * nsteps * N) if want_fields else None
+    rc = lib().orc_emx_run(int(N), x.size, float(eps), float(dt), float(w), int(sort), x, v[0], v[1], v[2], E, B,
+                           int(nsteps), _ptr(fields))
+    if rc != 0:
+        raise ValueError(f"orc_emx_run rejected its arguments (rc={rc})")
+    f = None if fields is None else fields.reshape(12, nsteps, N)
+    return x, np.vstack(v), E.reshape(3, N), B.reshape(2, N), f

@@ -187,6 +187,13 @@ void    orc_em_sort(int64_t P, double* x, double* v1, double* v2, double* v3, in
 int     orc_em_run(int32_t N, int64_t P, double eps, double dt, double w, double rtol, int32_t max_sweeps,
                    int32_t sort, double* x, double* v1, double* v2, double* v3, double* E, double* B, double* Fz,
                    int64_t nsteps, double* fields, int32_t* sweeps);
+/* 1D electromagnetic explicit step (src/GaussianEMExplicit.jl), in em1d.c */
+void    orc_emx_gauss_faraday(int32_t N, double dt, const double* rho, double* E, double* B, double* b2);
+void    orc_emx_ampere(int32_t N, double eps, double dt, const double* Jy, const double* Jz, double* E,
+                       const double* B);
+void    orc_emx_push(int32_t N, double x, const double* E, const double* b2, double dt, double* v);
+int     orc_emx_run(int32_t N, int64_t P, double eps, double dt, double w, int32_t sort, double* x, double* v1,
+                    double* v2, double* v3, double* E, double* B, int64_t nsteps, double* fields);
 
 
 #ifdef __cplusplus

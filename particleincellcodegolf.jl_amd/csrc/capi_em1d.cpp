@@ -1,6 +1,7 @@
 // capi_em1d.cpp -- host runtime behind include/picem1d.h: the 1D
-// electromagnetic fixed-point context, its device buffers, the hipGraph of one
-// step (every sweep predicated on the device), and the operator calls.
+// electromagnetic context (fixed-point or explicit scheme), its device
+// buffers, the hipGraph of one step (every fixed-point sweep predicated on the
+// device), and the operator calls.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -103,7 +104,11 @@ static int validate(const picem1d_config* g, std::string& why) {
         why = "P must be in [1, 2^26]";
         return PIC_EINVAL;
     }
-    if (g->max_sweeps < 1 || g->max_sweeps > kEMMaxSweeps) {
+    if (g->scheme != PICEM1D_FIXED_POINT && g->scheme != PICEM1D_EXPLICIT) {
+        why = "scheme must be PICEM1D_FIXED_POINT (0) or PICEM1D_EXPLICIT (1)";
+        return PIC_EINVAL;
+    }
+    if (g->scheme == PICEM1D_FIXED_POINT && (g->max_sweeps < 1 || g->max_sweeps > kEMMaxSweeps)) {
         snprintf(buf, sizeof buf, "max_sweeps must be in [1, %d]", kEMMaxSweeps);
         why = buf;
         return PIC_EINVAL;
@@ -169,8 +174,9 @@ extern "C" int picem1d_create(picem1d_ctx** out, const picem1d_config* cfg) {
     s.nb = (int32_t)std::min<int64_t>(nb, int64_t(1) << 20);
     int rc;
     double** parts[] = {&s.x, &s.v1, &s.v2, &s.v3, &s.X, &s.V1, &s.V2, &s.V3, &s.c};
-    for (double** p : parts)
-        if ((rc = dalloc(c, p, P))) return fail(rc);
+    const int nparts = cfg->scheme == PICEM1D_EXPLICIT ? 5 : 9;  // the explicit step keeps X only
+    for (int k = 0; k < nparts; ++k)
+        if ((rc = dalloc(c, parts[k], P))) return fail(rc);
     double** grids[] = {&s.Ex, &s.Ey, &s.Ez, &s.By, &s.Bz, &s.Fx, &s.Fy, &s.Fz,
                         &s.Cy, &s.Cz, &s.ex, &s.ey, &s.ez, &s.by, &s.bz};
     for (double** p : grids)
@@ -287,8 +293,12 @@ extern "C" int picem1d_download(picem1d_ctx* c, double* x, double* v, double* E,
 static int capture(picem1d_ctx* c, bool moments, hipGraphExec_t* out) {
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int sw = 0; sw < c->s.max_sweeps; ++sw) em_launch_sweep(c->s, sw, c->stream);
-    em_launch_end(c->s, c->cfg.sort != 0, moments, c->stream);
+    if (c->cfg.scheme == PICEM1D_EXPLICIT) {
+        emx_launch_step(c->s, c->cfg.sort != 0, moments, c->stream);
+    } else {
+        for (int sw = 0; sw < c->s.max_sweeps; ++sw) em_launch_sweep(c->s, sw, c->stream);
+        em_launch_end(c->s, c->cfg.sort != 0, moments, c->stream);
+    }
     HIPCHK(c, hipStreamEndCapture(c->stream, &g));
     hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -434,6 +444,62 @@ extern "C" int picem1d_push(picem1d_ctx* c, const double* cc, const double* e3, 
     s.by = fe + 3 * N;
     s.bz = fe + 4 * N;
     em_launch_push_op(s, dc, du, n, dv, c->stream);
+    if ((rc = d2h(c, v, dv, 3 * n))) return rc;
+    return finish(c);
+}
+
+// ---- operators of the explicit scheme -------------------------------------------------
+
+extern "C" int picem1d_explicit_fields(picem1d_ctx* c, const double* rho, const double* Jy, const double* Jz,
+                                       double* E, double* B, double* b2) {
+    if (!c) return PIC_EINVAL;
+    if (!rho || !Jy || !Jz || !E || !B || !b2) {
+        set_err(c, "picem1d_explicit_fields: arguments must not be NULL");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const int N = c->s.N;
+    double *dr, *djy, *djz, *buf;
+    int rc;
+    // slot 6: Ex Ey Ez By Bz by bz (7 N)
+    if ((rc = h2d(c, 4, rho, N, &dr)) || (rc = h2d(c, 5, Jy, N, &djy)) || (rc = h2d(c, 7, Jz, N, &djz)) ||
+        (rc = h2d(c, 6, nullptr, 15 * (int64_t)N, &buf)))
+        return rc;
+    EM1D s = c->s;
+    double** f[7] = {&s.Ex, &s.Ey, &s.Ez, &s.By, &s.Bz, &s.by, &s.bz};
+    for (int k = 0; k < 7; ++k) *f[k] = buf + (size_t)k * N;
+    HIPCHK(c, hipMemcpyAsync(s.Ex, E, 24 * (size_t)N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.By, B, 16 * (size_t)N, hipMemcpyHostToDevice, c->stream));
+    emx_launch_fields_op(s, dr, djy, djz, c->stream);
+    if ((rc = d2h(c, E, s.Ex, 3 * (int64_t)N)) || (rc = d2h(c, B, s.By, 2 * (int64_t)N)) ||
+        (rc = d2h(c, b2, s.by, 2 * (int64_t)N)))
+        return rc;
+    return finish(c);
+}
+
+extern "C" int picem1d_explicit_push(picem1d_ctx* c, const double* x, const double* E, const double* b2,
+                                     const double* u, int64_t n, double* v) {
+    if (!c) return PIC_EINVAL;
+    if (!x || !E || !b2 || !u || !v || n < 0) {
+        set_err(c, "picem1d_explicit_push: bad argument");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const int N = c->s.N;
+    double *dx, *du, *dv, *fe;
+    int rc;
+    if ((rc = h2d(c, 8, x, n, &dx)) || (rc = h2d(c, 9, u, 3 * n, &du)) || (rc = h2d(c, 10, nullptr, 3 * n, &dv)) ||
+        (rc = h2d(c, 11, nullptr, 5 * (int64_t)N, &fe)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(fe, E, 24 * (size_t)N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(fe + 3 * N, b2, 16 * (size_t)N, hipMemcpyHostToDevice, c->stream));
+    EM1D s = c->s;
+    s.Ex = fe;
+    s.Ey = fe + N;
+    s.Ez = fe + 2 * N;
+    s.by = fe + 3 * N;
+    s.bz = fe + 4 * N;
+    emx_launch_push_op(s, dx, du, n, dv, c->stream);
     if ((rc = d2h(c, v, dv, 3 * n))) return rc;
     return finish(c);
 }

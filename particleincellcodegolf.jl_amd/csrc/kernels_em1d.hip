@@ -9,6 +9,12 @@
 // velocity update).  The step's end wraps x, sorts the particles by x
 // (bucket by position, then each particle's rank in its bucket by its key and
 // index), deposits the seven moments, and writes the step's twelve fields.
+//
+// The explicit scheme (src/GaussianEMExplicit.jl) reuses the stencil, the
+// exact limbs, the sort and the moments: begin (X, the charge, the move and the
+// sort keys), a one-block Gauss and Faraday solve, the sort (X stays
+// unsorted), the split-kick push, the current at (X + x)/2, a one-block Ampere
+// solve, the moments and the step's fields.
 #include <climits>
 
 #include "fft_lines.h"
@@ -558,7 +564,8 @@ __global__ void __launch_bounds__(256) k_em_sort_apply(EM1D s) {
     }
 }
 
-// the sorted particles back in place, and the moment maxima
+// the sorted particles back in place, and (MAX) the moment maxima
+template <bool MAX>
 __global__ void __launch_bounds__(kEMThreads) k_em_sort_back(EM1D s) {
     __shared__ u64 red[kEMThreads / 64];
     const int64_t P = s.P;
@@ -573,6 +580,7 @@ __global__ void __launch_bounds__(kEMThreads) k_em_sort_back(EM1D s) {
         m2 = fmax_num(m2, fabs(b));
         m3 = fmax_num(m3, fabs(c));
     }
+    if (!MAX) return;
     block_max_into(m1, s.umax + 2, red);
     block_max_into(m2, s.umax + 3, red);
     block_max_into(m3, s.umax + 4, red);
@@ -668,6 +676,271 @@ __global__ void __launch_bounds__(256) k_em_finish(EM1D s) {
 }
 
 // ---------------------------------------------------------------------------
+// the explicit scheme (src/GaussianEMExplicit.jl:12-30)
+// ---------------------------------------------------------------------------
+// X .= x; d(X) into the exact limbs (:13); x .= mod.(X .+ v[:,1]*dt, 1)
+// (:14); SORT: the particle's bucket and key for sortperm(x).
+template <bool SORT>
+__global__ void __launch_bounds__(kEMThreads) k_emx_begin(EM1D s) {
+    extern __shared__ u64 ldsu[];
+    const int N = s.N;
+    for (int i = threadIdx.x; i < 3 * N; i += blockDim.x) ldsu[i] = 0;
+    __syncthreads();
+    const double Nd = (double)N;
+    const int q0 = em_frac_bits(s.P, s.w, 1.0);
+    const int64_t p0 = (int64_t)blockIdx.x * s.tile;
+    const int64_t p1 = p0 + s.tile < s.P ? p0 + s.tile : s.P;
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        const double X = s.x[p];
+        s.X[p] = X;
+        int32_t cell[13];
+        double ff[13];
+        em_stencil(X, N, Nd, cell, ff);
+#pragma unroll
+        for (int k = 0; k < 13; ++k) {
+            const double a = ff[k] * s.w;
+            if (a != 0.0) em_add(ldsu, N, cell[k], fix_round128(a, q0));
+        }
+        const double x = jl_mod1f(X + s.v1[p] * s.dt);
+        s.x[p] = x;
+        if (SORT) {
+            const uint32_t b = em_bucket(x, s.nb);
+            s.bucket[p] = b;
+            s.vkey[p] = em_order_key(x);
+            atomicAdd(s.count + b, 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * N; i += blockDim.x) {
+        const u64 v = ldsu[i];
+        if (v) atomicAdd(s.racc + i, v);
+    }
+}
+
+// One block.  PHASE 0 (:13, :15-16): Ex = Gauss(rho); By .-= dt*rif(-ik.*
+// fft(Ez)); Bz .-= dt*rif(ik.*fft(Ey)); by, bz = (B + C)/2 with C the old B.
+// PHASE 1 (:23-24): Ey .+= dt*rif(-ik.*fft(Bz)/eps^2 .- fft(Jy)); Ez .+=
+// dt*rif(+ik.*fft(By)/eps^2 .- fft(Jz)).  SRC 0: rho, Jy, Jz from the exact
+// limbs (cleared here); 1: from the arrays rin, jyin, jzin.
+template <int PHASE, int SRC>
+__global__ void __launch_bounds__(kEMSolveThreads)
+k_emx_solve(EM1D s, const double* __restrict__ rin, const double* __restrict__ jyin,
+            const double* __restrict__ jzin) {
+    extern __shared__ double sm[];
+    const int N = s.N, lN = s.lN, st = em_stride(N);
+    const int t = threadIdx.x;
+    double* re = sm;
+    double* im = sm + 5 * st;
+    const double Nd = (double)N, sc = 1.0 / Nd, dt = s.dt, e2 = s.eps * s.eps;
+    constexpr int NL = PHASE == 0 ? 3 : 4;
+    int q0 = 0, q1 = 0, q2 = 0;
+    if (SRC == 0) {
+        q0 = em_frac_bits(s.P, s.w, 1.0);
+        q1 = em_frac_bits(s.P, s.w, bits_d(s.umax[3]));
+        q2 = em_frac_bits(s.P, s.w, bits_d(s.umax[4]));
+    }
+    // PHASE 0 lines: rho, Ez, Ey; PHASE 1: Bz, Jy, By, Jz
+    for (int j = t; j < N; j += blockDim.x) {
+        double v[NL];
+        u64* g = s.racc;
+        if (PHASE == 0) {
+            if (SRC == 0) {
+                v[0] = em_sum(g[j], g[N + j], g[2 * N + j], q0);
+                for (int l = 0; l < 3; ++l) g[l * N + j] = 0;
+            } else {
+                v[0] = rin[j];
+            }
+            v[1] = s.Ez[j];
+            v[2] = s.Ey[j];
+        } else {
+            if (SRC == 0) {
+                v[1] = em_sum(g[3 * N + j], g[4 * N + j], g[5 * N + j], q1);
+                v[3] = em_sum(g[6 * N + j], g[7 * N + j], g[8 * N + j], q2);
+                for (int l = 3; l < 9; ++l) g[l * N + j] = 0;
+            } else {
+                v[1] = jyin[j];
+                v[3] = jzin[j];
+            }
+            v[0] = s.Bz[j];
+            v[2] = s.By[j];
+        }
+        const int pos = padi((int)brev((uint32_t)j, lN));
+#pragma unroll
+        for (int l = 0; l < NL; ++l) {
+            re[l * st + pos] = v[l];
+            im[l * st + pos] = 0.0;
+        }
+    }
+    __syncthreads();
+    fft_lines(re, im, s.tw, lN, NL, st, false);
+    constexpr int JPT = kMaxNEM / kEMSolveThreads;
+    constexpr int NO = PHASE == 0 ? 3 : 2;
+    double o[JPT][2 * NO];
+#pragma unroll
+    for (int q = 0; q < JPT; ++q) {
+        const int j = t + q * kEMSolveThreads;
+        if (j >= N) break;
+        const int pj = padi(j);
+        double kr, ki, pr, pi;
+        ik_of(N, j, kr, ki);
+        if (PHASE == 0) {
+            if (j == 0) {  // (0,0)/(0,0) is NaN; `*= false` keeps its (x86: negative) sign
+                o[q][0] = -0.0;
+                o[q][1] = -0.0;
+            } else {
+                const double tt = 1.0 / ki;
+                o[q][0] = im[pj] * tt;
+                o[q][1] = -(re[pj] * tt);
+            }
+            cmul(-kr, -ki, re[st + pj], im[st + pj], o[q][2], o[q][3]);
+            cmul(kr, ki, re[2 * st + pj], im[2 * st + pj], o[q][4], o[q][5]);
+        } else {
+            cmul(-kr, -ki, re[pj], im[pj], pr, pi);
+            o[q][0] = pr / e2 - re[st + pj];
+            o[q][1] = pi / e2 - im[st + pj];
+            cmul(kr, ki, re[2 * st + pj], im[2 * st + pj], pr, pi);
+            o[q][2] = pr / e2 - re[3 * st + pj];
+            o[q][3] = pi / e2 - im[3 * st + pj];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < JPT; ++q) {
+        const int j = t + q * kEMSolveThreads;
+        if (j >= N) break;
+        const int pos = padi((int)brev((uint32_t)j, lN));
+#pragma unroll
+        for (int l = 0; l < NO; ++l) {
+            re[l * st + pos] = o[q][2 * l];
+            im[l * st + pos] = o[q][2 * l + 1];
+        }
+    }
+    __syncthreads();
+    fft_lines(re, im, s.tw, lN, NO, st, true);
+    for (int j = t; j < N; j += blockDim.x) {
+        const int pj = padi(j);
+        if (PHASE == 0) {
+            const double cy = s.By[j], cz = s.Bz[j];
+            const double By = cy - dt * (re[st + pj] * sc);
+            const double Bz = cz - dt * (re[2 * st + pj] * sc);
+            s.Ex[j] = re[pj] * sc;
+            s.By[j] = By;
+            s.Bz[j] = Bz;
+            s.by[j] = (By + cy) / 2.0;
+            s.bz[j] = (Bz + cz) / 2.0;
+        } else {
+            s.Ey[j] = s.Ey[j] + dt * (re[pj] * sc);
+            s.Ez[j] = s.Ez[j] + dt * (re[st + pj] * sc);
+        }
+    }
+    if (PHASE == 1 && SRC == 0 && t == 0) s.ctl[kEMMaxSweeps] = 1;
+}
+
+// v += a*dt/2; v += cross(v, b)*dt; v += a*dt/2 with a, b gathered at x
+// (:17-22; fe: Ex, Ey, Ez, by, bz).
+__device__ __forceinline__ void emx_velocity(const double* fe, int N, double Nd, double x, double dt, double& v1,
+                                             double& v2, double& v3) {
+    int32_t cell[13];
+    double ff[13];
+    em_stencil(x, N, Nd, cell, ff);
+    double a1 = 0, a2 = 0, a3 = 0, b1 = 0, b2 = 0, b3 = 0;
+#pragma unroll
+    for (int k = 0; k < 13; ++k) {
+        const int m = cell[k];
+        const double t1 = fe[m] * ff[k], t2 = fe[N + m] * ff[k], t3 = fe[2 * N + m] * ff[k];
+        const double s1 = 0.0 * ff[k], s2 = fe[3 * N + m] * ff[k], s3 = fe[4 * N + m] * ff[k];
+        if (k == 0) {
+            a1 = t1;
+            a2 = t2;
+            a3 = t3;
+            b1 = s1;
+            b2 = s2;
+            b3 = s3;
+        } else {
+            a1 = a1 + t1;
+            a2 = a2 + t2;
+            a3 = a3 + t3;
+            b1 = b1 + s1;
+            b2 = b2 + s2;
+            b3 = b3 + s3;
+        }
+    }
+    const double u1 = v1 + a1 * dt / 2.0, u2 = v2 + a2 * dt / 2.0, u3 = v3 + a3 * dt / 2.0;
+    const double c1 = u2 * b3 - u3 * b2, c2 = u3 * b1 - u1 * b3, c3 = u1 * b2 - u2 * b1;
+    const double w1 = u1 + c1 * dt, w2 = u2 + c2 * dt, w3 = u3 + c3 * dt;
+    v1 = w1 + a1 * dt / 2.0;
+    v2 = w2 + a2 * dt / 2.0;
+    v3 = w3 + a3 * dt / 2.0;
+}
+
+__device__ __forceinline__ void stage_fields_x(double* fe, const EM1D& s) {
+    const int N = s.N;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        fe[j] = s.Ex[j];
+        fe[N + j] = s.Ey[j];
+        fe[2 * N + j] = s.Ez[j];
+        fe[3 * N + j] = s.by[j];
+        fe[4 * N + j] = s.bz[j];
+    }
+    __syncthreads();
+}
+
+// the velocity update of every particle at its (sorted) x, and the maxima of
+// |v1|, |v2|, |v3| for the currents' and the moments' fixed-point grids
+__global__ void __launch_bounds__(kEMThreads) k_emx_push(EM1D s) {
+    extern __shared__ double fe[];
+    __shared__ u64 red[kEMThreads / 64];
+    stage_fields_x(fe, s);
+    const double Nd = (double)s.N;
+    double m1 = 0.0, m2 = 0.0, m3 = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < s.P; p += stride) {
+        double v1 = s.v1[p], v2 = s.v2[p], v3 = s.v3[p];
+        emx_velocity(fe, s.N, Nd, s.x[p], s.dt, v1, v2, v3);
+        s.v1[p] = v1;
+        s.v2[p] = v2;
+        s.v3[p] = v3;
+        m1 = fmax_num(m1, fabs(v1));
+        m2 = fmax_num(m2, fabs(v2));
+        m3 = fmax_num(m3, fabs(v3));
+    }
+    block_max_into(m1, s.umax + 2, red);
+    block_max_into(m2, s.umax + 3, red);
+    block_max_into(m3, s.umax + 4, red);
+}
+
+// d((X.+x)/2, v[:,2]) and d((X.+x)/2, v[:,3]) (:23-24) into limbs 3..8
+__global__ void __launch_bounds__(kEMThreads) k_emx_current(EM1D s) {
+    extern __shared__ u64 ldsu[];
+    const int N = s.N;
+    for (int i = threadIdx.x; i < 6 * N; i += blockDim.x) ldsu[i] = 0;
+    __syncthreads();
+    const double Nd = (double)N;
+    const int q1 = em_frac_bits(s.P, s.w, bits_d(s.umax[3]));
+    const int q2 = em_frac_bits(s.P, s.w, bits_d(s.umax[4]));
+    const int64_t p0 = (int64_t)blockIdx.x * s.tile;
+    const int64_t p1 = p0 + s.tile < s.P ? p0 + s.tile : s.P;
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        const double c = (s.X[p] + s.x[p]) / 2.0;
+        const double uy = s.v2[p], uz = s.v3[p];
+        int32_t cell[13];
+        double ff[13];
+        em_stencil(c, N, Nd, cell, ff);
+#pragma unroll
+        for (int k = 0; k < 13; ++k) {
+            const double a1 = uy * ff[k] * s.w, a2 = uz * ff[k] * s.w;
+            if (a1 != 0.0) em_add(ldsu, N, cell[k], fix_round128(a1, q1));
+            if (a2 != 0.0) em_add(ldsu + 3 * N, N, cell[k], fix_round128(a2, q2));
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 6 * N; i += blockDim.x) {
+        const u64 v = ldsu[i];
+        if (v) atomicAdd(s.racc + 3 * N + i, v);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // operators
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kEMThreads) k_em_umax(const double* __restrict__ u, int64_t n, u64* dst) {
@@ -731,6 +1004,21 @@ k_em_push_op(EM1D s, const double* __restrict__ c, const double* __restrict__ u,
     }
 }
 
+__global__ void __launch_bounds__(kEMThreads)
+k_emx_push_op(EM1D s, const double* __restrict__ xs, const double* __restrict__ u, int64_t n,
+              double* __restrict__ v) {
+    extern __shared__ double fe[];
+    stage_fields_x(fe, s);
+    const double Nd = (double)s.N;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        double a = u[p], b = u[n + p], d = u[2 * n + p];
+        emx_velocity(fe, s.N, Nd, xs[p], s.dt, a, b, d);
+        v[p] = a;
+        v[n + p] = b;
+        v[2 * n + p] = d;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -763,7 +1051,7 @@ void em_launch_end(const EM1D& s, bool sort, bool moments, hipStream_t st) {
         k_em_sort_scatter<<<nb, 256, 0, st>>>(s);
         k_em_sort_rank<<<nb, 256, 0, st>>>(s);
         k_em_sort_apply<<<nb, 256, 0, st>>>(s);
-        k_em_sort_back<<<nb, kEMThreads, 0, st>>>(s);
+        k_em_sort_back<true><<<nb, kEMThreads, 0, st>>>(s);
     } else {
         k_em_wrap<false><<<nb, kEMThreads, 0, st>>>(s);
     }
@@ -774,6 +1062,48 @@ void em_launch_end(const EM1D& s, bool sort, bool moments, hipStream_t st) {
     } else {
         k_em_finish<false><<<1, 256, 0, st>>>(s);
     }
+}
+
+void emx_launch_step(const EM1D& s, bool sort, bool moments, hipStream_t st) {
+    const unsigned nb = em_blocks(s.P);
+    const unsigned nt = (unsigned)((s.P + s.tile - 1) / s.tile);
+    const size_t solve = em_solve_lds_bytes(s.N);
+    if (sort) {
+        const int ntiles = (s.nb + kEMScanTile - 1) / kEMScanTile;
+        const unsigned nbk = (unsigned)((s.nb + 255) / 256 < 1024 ? (s.nb + 255) / 256 : 1024);
+        k_em_sort_zero<<<nbk, 256, 0, st>>>(s);
+        k_emx_begin<true><<<nt, kEMThreads, sizeof(u64) * 3 * (size_t)s.N, st>>>(s);
+        k_emx_solve<0, 0><<<1, kEMSolveThreads, solve, st>>>(s, nullptr, nullptr, nullptr);
+        k_em_scan_tiles<<<(unsigned)ntiles, 1024, 0, st>>>(s);
+        k_em_scan_top<<<1, 256, 0, st>>>(s, ntiles);
+        k_em_scan_add<<<nbk, 256, 0, st>>>(s);
+        k_em_sort_scatter<<<nb, 256, 0, st>>>(s);
+        k_em_sort_rank<<<nb, 256, 0, st>>>(s);
+        k_em_sort_apply<<<nb, 256, 0, st>>>(s);
+        k_em_sort_back<false><<<nb, kEMThreads, 0, st>>>(s);
+    } else {
+        k_emx_begin<false><<<nt, kEMThreads, sizeof(u64) * 3 * (size_t)s.N, st>>>(s);
+        k_emx_solve<0, 0><<<1, kEMSolveThreads, solve, st>>>(s, nullptr, nullptr, nullptr);
+    }
+    k_emx_push<<<nb, kEMThreads, sizeof(double) * 5 * (size_t)s.N, st>>>(s);
+    k_emx_current<<<nt, kEMThreads, sizeof(u64) * 6 * (size_t)s.N, st>>>(s);
+    k_emx_solve<1, 0><<<1, kEMSolveThreads, solve, st>>>(s, nullptr, nullptr, nullptr);
+    if (moments) {
+        k_em_moments<<<dim3(nt, kEMMomentGroups), kEMThreads, sizeof(u64) * 6 * (size_t)s.N, st>>>(s);
+        k_em_finish<true><<<1, 256, 0, st>>>(s);
+    } else {
+        k_em_finish<false><<<1, 256, 0, st>>>(s);
+    }
+}
+
+void emx_launch_fields_op(const EM1D& s, const double* rho, const double* Jy, const double* Jz, hipStream_t st) {
+    const size_t solve = em_solve_lds_bytes(s.N);
+    k_emx_solve<0, 1><<<1, kEMSolveThreads, solve, st>>>(s, rho, Jy, Jz);
+    k_emx_solve<1, 1><<<1, kEMSolveThreads, solve, st>>>(s, rho, Jy, Jz);
+}
+
+void emx_launch_push_op(const EM1D& s, const double* x, const double* u, int64_t n, double* v, hipStream_t st) {
+    if (n > 0) k_emx_push_op<<<em_blocks(n), kEMThreads, sizeof(double) * 5 * (size_t)s.N, st>>>(s, x, u, n, v);
 }
 
 void em_launch_deposit_op(const EM1D& s, const double* y, const double* u, int64_t n, double w, double* rho,
@@ -800,6 +1130,14 @@ int configure_kernels_em1d() {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_em_solve<0, true>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_em_solve<1, false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_emx_solve<0, 0>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_emx_solve<1, 0>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_emx_solve<0, 1>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_emx_solve<1, 1>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     const int dep = (int)(sizeof(u64) * 9 * kMaxNEM);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_em_deposit3), hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -1,5 +1,5 @@
 // picem1d_internal.h -- descriptors and launchers of the 1D electromagnetic
-// fixed-point step, shared by the host runtime (capi_em1d.cpp) and the kernels
+// fixed-point and explicit steps, shared by the host runtime (capi_em1d.cpp) and the kernels
 // (kernels_em1d.hip).
 #pragma once
 #include "pic_internal.h"
@@ -43,12 +43,18 @@ struct EM1D {
 
 void em_launch_sweep(const EM1D& s, int sweep, hipStream_t st);
 void em_launch_end(const EM1D& s, bool sort, bool moments, hipStream_t st);
+// one step of the explicit scheme
+void emx_launch_step(const EM1D& s, bool sort, bool moments, hipStream_t st);
 // operators
 void em_launch_deposit_op(const EM1D& s, const double* y, const double* u, int64_t n, double w, double* rho,
                           unsigned long long* scratch_max, hipStream_t st);
 // one sweep's field updates from rho, Jy, Jz arrays (s points at scratch copies)
 void em_launch_fields_op(const EM1D& s, const double* rho, const double* Jy, const double* Jz, hipStream_t st);
 void em_launch_push_op(const EM1D& s, const double* c, const double* u, int64_t n, double* v, hipStream_t st);
+// the explicit scheme's field updates (Gauss, Faraday, then Ampere) from rho,
+// Jy, Jz arrays, and its velocity update at positions x
+void emx_launch_fields_op(const EM1D& s, const double* rho, const double* Jy, const double* Jz, hipStream_t st);
+void emx_launch_push_op(const EM1D& s, const double* x, const double* u, int64_t n, double* v, hipStream_t st);
 int configure_kernels_em1d();
 size_t em_solve_lds_bytes(int N);
 

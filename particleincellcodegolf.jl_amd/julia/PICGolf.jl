@@ -40,7 +40,7 @@ export Context, PicConfig, deposit!, fieldsolve!, drift!, midpoint_position!, wr
        simpson_kick!, Pic1D2VConfig, Context1D2V, fields,
        Pic2DConfig, Context2D, gather, boris!,
        Pic2D3VSpecies, Pic2D3VConfig, Context2D3V, reset!, field,
-       PicEM1DConfig, ContextEM1D
+       PicEM1DConfig, ContextEM1D, explicit_fields!, explicit_push
 
 const libpicgolf = get(ENV, "PICGOLF_LIB", normpath(joinpath(@__DIR__, "..", "lib", "libpicgolf.so")))
 
@@ -904,8 +904,9 @@ function fieldsolve!(Ex::Matrix{Float64}, Ey::Matrix{Float64}, ctx::Context2D3V,
 end
 
 # ---------------------------------------------------------------------------
-# 1D electromagnetic fixed point (src/GaussianEMFixedPoint.jl), C ABI in
-# include/picem1d.h.
+# 1D electromagnetic fixed point (src/GaussianEMFixedPoint.jl, scheme
+# :fixed_point) and explicit step (src/GaussianEMExplicit.jl, scheme
+# :explicit), C ABI in include/picem1d.h.
 #
 # | script expression                                          | PICGolf                              |
 # |------------------------------------------------------------|--------------------------------------|
@@ -913,7 +914,12 @@ end
 # | one sweep's `Ex .= ...` .. `bz=(Bz.+Cz)/2` (lines 20-27)     | `fields!(ctx, ρ, Jy, Jz, F, C, E, B)` |
 # | `v[j,:] .= u .+ (a .+ cross(u .+ a*dt/2, b))*dt` (28-29)     | `push(ctx, c, e3, b2, u)`            |
 # | the `@gif for t ∈ 1:T` loop and `fields` (lines 13-50)       | `upload!`, `run!`, `download`        |
+# | explicit: Gauss, Faraday, Ampere (lines 13, 15-16, 23-24)    | `explicit_fields!(ctx, ρ, Jy, Jz, E, B)` |
+# | explicit: `v[j,:].+=a*dt/2` .. (lines 17-22)                 | `explicit_push(ctx, x, E, b2, u)`    |
+# | explicit: the `for t ∈ 1:T` loop and `fields` (lines 12-30)  | `upload!`, `run!`, `download`        |
 # ---------------------------------------------------------------------------
+
+const EM_SCHEMES = Dict(:fixed_point => Int32(0), :explicit => Int32(1))
 
 "Mirror of `struct picem1d_config` in include/picem1d.h."
 struct PicEM1DConfig
@@ -926,17 +932,19 @@ struct PicEM1DConfig
     rtol::Float64
     sort::Int32
     device::Int32
+    scheme::Int32
 end
 
 """
-    PicEM1DConfig(; N, P, eps, dt, w, rtol=1e-8, max_sweeps=3, sort=true, device=0)
+    PicEM1DConfig(; N, P, eps, dt, w, rtol=1e-8, max_sweeps=3, sort=true, device=0, scheme=:fixed_point)
 
 The script's N, P, ϵ, dt = ϵ^2/N/2, w = W/P*N, η, the three sweeps of
-`for _∈0:2`, and the `sortperm(x)` after every step.
+`for _∈0:2`, and the `sortperm(x)` after every step; `scheme=:explicit` runs
+src/GaussianEMExplicit.jl instead (η and the sweeps are then unused).
 """
-PicEM1DConfig(; N, P, eps, dt, w, rtol=1e-8, max_sweeps=3, sort=true, device=0) =
+PicEM1DConfig(; N, P, eps, dt, w, rtol=1e-8, max_sweeps=3, sort=true, device=0, scheme=:fixed_point) =
     PicEM1DConfig(Int32(N), Int32(max_sweeps), Int64(P), Float64(eps), Float64(dt), Float64(w), Float64(rtol),
-                  Int32(sort), Int32(device))
+                  Int32(sort), Int32(device), EM_SCHEMES[scheme])
 
 mutable struct ContextEM1D
     ptr::Ptr{Cvoid}
@@ -1041,6 +1049,34 @@ function push(ctx::ContextEM1D, c::Vector{Float64}, e3::Matrix{Float64}, b2::Mat
                                         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64,
                                          Ptr{Float64}), ctx.ptr, c, e3, b2, u, length(c), v)
     check(ctx, rc, "picem1d_push")
+    v
+end
+
+"""
+    explicit_fields!(ctx::ContextEM1D, ρ, Jy, Jz, E, B) -> b2
+
+The explicit step's field updates: `Ex` from `ρ` (line 13), `By, Bz` from the
+given `Ey, Ez` (line 15), then `Ey, Ez` from the new `By, Bz` and `Jy, Jz`
+(lines 23-24), with `E` (N x 3) and `B` (N x 2) in place; returns `by, bz`
+(N x 2) of line 16.
+"""
+function explicit_fields!(ctx::ContextEM1D, ρ, Jy, Jz, E::Matrix{Float64}, B::Matrix{Float64})
+    b2 = zeros(ctx.cfg.N, 2)
+    rc = GC.@preserve ρ Jy Jz E B b2 ccall((:picem1d_explicit_fields, libpicgolf), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+        ctx.ptr, ρ, Jy, Jz, E, B, b2)
+    check(ctx, rc, "picem1d_explicit_fields")
+    b2
+end
+
+"`explicit_push(ctx::ContextEM1D, x, E, b2, u) -> v` (u, v: n x 3): the split kick of lines 17-22 at x."
+function explicit_push(ctx::ContextEM1D, x::Vector{Float64}, E::Matrix{Float64}, b2::Matrix{Float64},
+                       u::Matrix{Float64})
+    v = zeros(size(u))
+    rc = GC.@preserve x E b2 u v ccall((:picem1d_explicit_push, libpicgolf), Cint,
+                                       (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64,
+                                        Ptr{Float64}), ctx.ptr, x, E, b2, u, length(x), v)
+    check(ctx, rc, "picem1d_explicit_push")
     v
 end
 

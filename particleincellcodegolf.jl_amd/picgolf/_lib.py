@@ -55,7 +55,8 @@ EXPORTS_2D3V = [
 # every symbol include/picem1d.h declares
 EXPORTS_EM1D = [
     "picem1d_create", "picem1d_destroy", "picem1d_last_error", "picem1d_upload", "picem1d_download",
-    "picem1d_run", "picem1d_deposit", "picem1d_fields", "picem1d_push",
+    "picem1d_run", "picem1d_deposit", "picem1d_fields", "picem1d_push", "picem1d_explicit_fields",
+    "picem1d_explicit_push",
 ]
 MAX_SPECIES_2D3V = 4
 NGP_2D3V, AREA_2D3V, BSPLINE0_2D3V = 0, 1, 2
@@ -121,8 +122,12 @@ class picem1d_config(ctypes.Structure):
     _fields_ = [
         ("N", ctypes.c_int32), ("max_sweeps", ctypes.c_int32), ("P", ctypes.c_int64), ("eps", ctypes.c_double),
         ("dt", ctypes.c_double), ("w", ctypes.c_double), ("rtol", ctypes.c_double), ("sort", ctypes.c_int32),
-        ("device", ctypes.c_int32),
+        ("device", ctypes.c_int32), ("scheme", ctypes.c_int32),
     ]
+
+
+PICEM1D_FIXED_POINT = 0
+PICEM1D_EXPLICIT = 1
 
 
 class pic_tuning(ctypes.Structure):
@@ -221,6 +226,8 @@ def load():
         "picem1d_deposit": [vp, vp, vp, i64, d, vp],
         "picem1d_fields": [vp] + [vp] * 12,
         "picem1d_push": [vp, vp, vp, vp, vp, i64, vp],
+        "picem1d_explicit_fields": [vp] * 7,
+        "picem1d_explicit_push": [vp, vp, vp, vp, vp, i64, vp],
     })
     for name, args in sig.items():
         f = getattr(L, name)

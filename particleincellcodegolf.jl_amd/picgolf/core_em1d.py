@@ -1,10 +1,17 @@
-"""Python mirror of the 1D electromagnetic fixed-point script
-(src/GaussianEMFixedPoint.jl) on top of the C ABI in include/picem1d.h.
+"""Python mirror of the 1D electromagnetic scripts on top of the C ABI in
+include/picem1d.h: the fixed point (src/GaussianEMFixedPoint.jl, scheme
+"fixed_point")
 
     d(y, u)                                   :11        -> deposit
     one sweep's Ex, Ey, Ez, By, Bz updates    :20-27     -> fields
     v[j,:] .= u .+ (a .+ cross(...))*dt       :28-29     -> push
     the `@gif for t ∈ 1:T` loop and `fields`  :13-50     -> upload, run, download
+
+and the explicit step (src/GaussianEMExplicit.jl, scheme "explicit")
+
+    Gauss, Faraday (:13, :15-16), Ampere (:23-24)        -> explicit_fields
+    v += a*dt/2; v += cross(v, b)*dt; v += a*dt/2 :17-22 -> explicit_push
+    the `for t ∈ 1:T` loop and `fields`       :12-30     -> upload, run, download
 
 Velocities are a (3, P) array with rows v[:,1], v[:,2], v[:,3].  run()
 returns the script's fields[:, t, 1:12] for the steps of the call as an
@@ -23,7 +30,8 @@ from . import _lib as L
 @dataclass
 class ConfigEM1D:
     """N cells, P particles, eps = 1/c, dt, weight w (W/P*N), the fixed
-    point's rtol and sweeps, and whether to sort by x after every step."""
+    point's rtol and sweeps, whether to sort by x after every step, and the
+    scheme: "fixed_point" or "explicit"."""
     N: int
     P: int
     eps: float
@@ -33,9 +41,14 @@ class ConfigEM1D:
     max_sweeps: int = 3
     sort: bool = True
     device: int = 0
+    scheme: str = "fixed_point"
 
     def to_c(self):
+        schemes = {"fixed_point": L.PICEM1D_FIXED_POINT, "explicit": L.PICEM1D_EXPLICIT}
+        if self.scheme not in schemes:
+            raise ValueError(f"scheme must be 'fixed_point' or 'explicit' (got {self.scheme!r})")
         c = L.picem1d_config()
+        c.scheme = schemes[self.scheme]
         c.N, c.max_sweeps, c.P = int(self.N), int(self.max_sweeps), int(self.P)
         c.eps, c.dt, c.w, c.rtol = float(self.eps), float(self.dt), float(self.w), float(self.rtol)
         c.sort, c.device = int(bool(self.sort)), int(self.device)
@@ -124,4 +137,27 @@ class ContextEM1D:
         self._check(self._lib.picem1d_push(self._h, L.ptr(c), L.ptr(np.ascontiguousarray(L.f64(e3))),
                                            L.ptr(np.ascontiguousarray(L.f64(b2))), L.ptr(u), c.size, L.ptr(v)),
                     "picem1d_push")
+        return v
+
+    def explicit_fields(self, rho, Jy, Jz, E, B):
+        """The explicit step's field updates from its rho and currents:
+        returns (E (3, N), B (2, N), b2 (2, N): by, bz)."""
+        N = self.cfg.N
+        a = [L.f64(z) for z in (rho, Jy, Jz)]
+        E = L.f64(E).reshape(3, N).copy()
+        B = L.f64(B).reshape(2, N).copy()
+        b2 = np.zeros((2, N))
+        self._check(self._lib.picem1d_explicit_fields(self._h, *(L.ptr(z) for z in a), L.ptr(E), L.ptr(B),
+                                                      L.ptr(b2)), "picem1d_explicit_fields")
+        return E, B, b2
+
+    def explicit_push(self, x, E, b2, u):
+        """v (3, n): the split kick of u at positions x with E (3, N) and
+        b2 (2, N)."""
+        x = L.f64(x)
+        u = np.ascontiguousarray(L.f64(u).reshape(3, -1))
+        v = np.zeros_like(u)
+        self._check(self._lib.picem1d_explicit_push(self._h, L.ptr(x), L.ptr(np.ascontiguousarray(L.f64(E))),
+                                                    L.ptr(np.ascontiguousarray(L.f64(b2))), L.ptr(u), x.size,
+                                                    L.ptr(v)), "picem1d_explicit_push")
         return v

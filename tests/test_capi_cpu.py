@@ -275,6 +275,8 @@ def test_em1d_struct_layout_matches_the_header(tmp_path):
     (dict(eps=0.0), "eps and dt"),
     (dict(dt=float("inf")), "eps and dt"),
     (dict(rtol=-1.0), "rtol"),
+    (dict(scheme="explicit", N=48), "power of two"),
+    (dict(scheme="explicit", dt=0.0), "eps and dt"),
 ])
 def test_create_em1d_rejects_bad_config_with_message(change, msg):
     import picgolf
@@ -284,3 +286,18 @@ def test_create_em1d_rejects_bad_config_with_message(change, msg):
     with pytest.raises(picgolf.PicError) as e:
         picgolf.ContextEM1D(ConfigEM1D(**kw))
     assert e.value.code == -1 and msg in str(e.value)
+
+
+def test_create_em1d_rejects_unknown_scheme():
+    import picgolf
+    from picgolf import _lib as L
+    from picgolf.core_em1d import ConfigEM1D
+    with pytest.raises(ValueError):
+        ConfigEM1D(N=64, P=1024, eps=0.1, dt=1e-4, w=25.0, scheme="implicit").to_c()
+    c = ConfigEM1D(N=64, P=1024, eps=0.1, dt=1e-4, w=25.0).to_c()
+    c.scheme = 7
+    lib = L.load()
+    import ctypes
+    h = ctypes.c_void_p()
+    assert lib.picem1d_create(ctypes.byref(h), ctypes.byref(c)) == -1
+    assert b"scheme" in lib.picem1d_last_error(None)
