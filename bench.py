@@ -35,11 +35,12 @@ def parse():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the ensemble and 2D3V workloads reported beside the headline")
     ap.add_argument("--profile-steps", type=int, default=3)
-    ap.add_argument("--workload", default="scaled", choices=["scaled", "ensemble", "es2d3v", "pic2d3v", "em1d"],
+    ap.add_argument("--workload", default="scaled",
+                    choices=["scaled", "ensemble", "es2d3v", "pic2d3v", "em1d", "em1d_explicit"],
                     help="scaled: the headline line (default); ensemble / es2d3v: only that workload, "
                          "--steps timed steps, one JSON line (for profiling)")
     ap.add_argument("--script-steps", type=int, default=4048,
-                    help="em1d: steps of the script-sized run (the script's T = 4048)")
+                    help="em1d, em1d_explicit: steps of the script-sized run (the scripts' T = 4048)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default particle tiling instead of autotuning it before the timed steps")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -305,26 +306,38 @@ def bench_pic2d3v(steps=20, warmup=2, N=1024, ppc=16, shape="bspline2"):
                              f"and solve on one"}}
 
 
-def bench_em1d(T=4048, scaled_steps=20):
-    """The 1D electromagnetic fixed-point script (src/GaussianEMFixedPoint.jl)
-    whole: N=64, P=16N, eps=1/10, dt=eps^2/N/2, W=400, T=4048 steps with the
-    twelve fields of every step and the per-step sort; and the same loop at
-    N=1024 with 2^22 particles.  FP64."""
+def em1d_setup(N, P, scheme, seed=0):
+    """The scripts' set-up: eps = 1/10, dt = eps^2/N/2, W = 400, w = W/P*N,
+    x = rand(P); the fixed point's v = (sin a/4, cos a/4, 0) with a = 2pi*rand,
+    the explicit script's v = randn(P, 3)."""
     import numpy as np
+    eps = 0.1
+    dt = eps ** 2 / N / 2
+    w = 400 / P * N
+    rng = np.random.default_rng(seed)
+    x = rng.random(P)
+    if scheme == "explicit":
+        v = rng.standard_normal((3, P))
+    else:
+        a = 2 * np.pi * rng.random(P)
+        v = np.vstack([np.sin(a) / 4, np.cos(a) / 4, np.zeros(P)])
+    return eps, dt, w, x, v
+
+
+def bench_em1d(T=4048, scaled_steps=20, scheme="fixed_point"):
+    """A 1D electromagnetic script whole (src/GaussianEMFixedPoint.jl, or
+    src/GaussianEMExplicit.jl with scheme "explicit"): N=64, P=16N,
+    eps=1/10, dt=eps^2/N/2, W=400, T=4048 steps with the twelve fields of
+    every step and the per-step sort; and the same loop at N=1024 with 2^22
+    particles.  FP64."""
     import picgolf
     from picgolf.core_em1d import ConfigEM1D
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     res = {}
     for name, N, P, steps in (("script", 64, 1024, T), ("scaled", 1024, 1 << 22, scaled_steps)):
-        eps = 0.1
-        dt = eps ** 2 / N / 2
-        w = 400 / P * N
-        rng = np.random.default_rng(0)
-        x = rng.random(P)
-        a = 2 * np.pi * rng.random(P)
-        v = np.vstack([np.sin(a) / 4, np.cos(a) / 4, np.zeros(P)])
-        with picgolf.ContextEM1D(ConfigEM1D(N=N, P=P, eps=eps, dt=dt, w=w)) as c:
+        eps, dt, w, x, v = em1d_setup(N, P, scheme)
+        with picgolf.ContextEM1D(ConfigEM1D(N=N, P=P, eps=eps, dt=dt, w=w, scheme=scheme)) as c:
             c.upload(x, v)
             c.run(2)                       # graph captured, caches warm
             c.upload(x, v)
@@ -335,16 +348,16 @@ def bench_em1d(T=4048, scaled_steps=20):
                      "pushes_per_s": P * steps / secs, "mean_sweeps": float(sw.mean())}
     # the CPU restatement: the script's set-up, 40 steps
     N, P = 64, 1024
-    eps = 0.1
-    dt = eps ** 2 / N / 2
-    rng = np.random.default_rng(0)
-    x = rng.random(P)
-    a = 2 * np.pi * rng.random(P)
-    v = np.vstack([np.sin(a) / 4, np.cos(a) / 4, np.zeros(P)])
+    eps, dt, w, x, v = em1d_setup(N, P, scheme)
     t0 = time.perf_counter()
-    oracle.em_run(N, eps, dt, 400 / P * N, x, v, 40)
+    if scheme == "explicit":
+        oracle.emx_run(N, eps, dt, w, x, v, 40)
+    else:
+        oracle.em_run(N, eps, dt, w, x, v, 40)
     cpu = time.perf_counter() - t0
-    return {"workload": f"1D electromagnetic fixed point (src/GaussianEMFixedPoint.jl): the script "
+    script = "src/GaussianEMExplicit.jl" if scheme == "explicit" else "src/GaussianEMFixedPoint.jl"
+    what = "explicit step" if scheme == "explicit" else "fixed point"
+    return {"workload": f"1D electromagnetic {what} ({script}): the script "
                         f"(N=64, P=1024, {T} steps, fields every step, sort every step) and the same loop at "
                         f"N=1024, P=2^22; FP64",
             **res,
@@ -402,8 +415,9 @@ def main():
     if a.workload == "pic2d3v":
         print(json.dumps(bench_pic2d3v(steps=a.steps, warmup=a.warmup)))
         return
-    if a.workload == "em1d":
-        print(json.dumps(bench_em1d(T=a.script_steps, scaled_steps=a.steps)))
+    if a.workload in ("em1d", "em1d_explicit"):
+        scheme = "explicit" if a.workload == "em1d_explicit" else "fixed_point"
+        print(json.dumps(bench_em1d(T=a.script_steps, scaled_steps=a.steps, scheme=scheme)))
         return
     import numpy as np
     from picgolf import configs
@@ -461,7 +475,8 @@ def main():
         out["momentum_drift_mixed"] = float(np.max(np.abs(m["D"][:, 3] - m["D"][0, 3])))
     if not a.no_extra:
         out["workloads"] = {"ensemble_512": bench_ensemble(autotune=not a.no_autotune), "es2d3v_256": bench_2d(),
-                            "pic2d3v_1024": bench_pic2d3v(), "em1d": bench_em1d()}
+                            "pic2d3v_1024": bench_pic2d3v(), "em1d": bench_em1d(),
+                            "em1d_explicit": bench_em1d(scheme="explicit")}
         print(f"[bench] extra workloads: {json.dumps(out['workloads'])}", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(*base)
