@@ -1,0 +1,489 @@
+/*
+ * lorenz2d.c -- CPU FP64 restatement of the Lorenz-gauge electromagnetic path
+ * of the structured 2D3V module src/PIC2D3V.jl as src/2D3V.jl:102-106,123-126
+ * drives it: LorenzGaugeStaggeredField (:347-392) with any ImEx theta,
+ * warmup! (:826-859), loop! (:875-979), ElectromagneticBoris (:56-69), the
+ * six-component gather (:1231-1247), the one- and three-weight deposits
+ * (:1258-1284), update! (:468-479), reduction! (:504-522) and
+ * diagnose!(::LorenzGaugeDiagnostics) (:1332-1398), whose in-place inverse
+ * transforms of the potentials and sources are part of the dynamics.
+ *
+ * TEST INFRASTRUCTURE ONLY (see picoracle.h): the oracle the GPU Lorenz-gauge
+ * path is checked against.
+ *
+ * Field state: 18 complex NX x NY arrays (column-major, interleaved re, im as
+ * Julia's ComplexF64), in the order of ORCLG_* in picoracle.h: phi0, phi-,
+ * Ax0, Ay0, Az0, Ax-, Ay-, Az-, rho0, Jx0, Jy0, Jz0, Ex, Ey, Ez, Bx, By, Bz.
+ * phi+ and A+ are temporaries: loop! writes them in full before it reads
+ * them, so the forward transforms diagnose! applies to them (:1388-1391) are
+ * not observable and are not restated; nor is the forward transform and
+ * smoothing of rho0, J0 at :891-903, which the reduction at :964-966
+ * overwrites before anything reads it.
+ *
+ * What loop! does, in the reference's order:
+ *   1. lorenzgauge! (:629-635) for phi (source rho0) and Ax, Ay, Az (sources
+ *      J0): x+ = (num*x0 + dt^2*s)/den - x-, num = 2 - dt^2*k^2*(1 - theta),
+ *      den = 1 + dt^2*k^2*theta/2 (:610-615; Explicit is theta 0 and
+ *      Implicit theta 1 bit for bit).  The sources are whatever rho0, J0
+ *      hold: the real-space reduction of the previous step, or its inverse
+ *      transform when diagnose! ran.
+ *   2. E = -im*k*phi+ - (A+ - A0)/dt, B = curl((A+ + A0)/2) in the spectrum
+ *      (:917-925), then ifft of the six (:927-934) and update! (:935):
+ *      EBxyz = (0.0 + real(F)) (+ B0 for B), ghost layers periodic copies.
+ *   3. the particle loop (:937-968): gather, Boris, half move, deposit rho,
+ *      half move, deposit J; reduction into rho0, J0 (imaginary parts 0).
+ *   4. phi- = phi0, phi0 = phi+, A- = A0, A0 = A+ (:969-978).
+ *
+ * Charge and current sums are exact (each contribution rounded to 2^-q,
+ * summed exactly, rounded once): q from sum_s P_s |qw_s| for rho and
+ * sum_s P_s (|qw_s| max|v_c|) for each current component, with the maxima of
+ * the velocities the deposit uses.  The reference's `@muladd z += wxy*w` is
+ * taken as the rounded product wxy*w added exactly.  Diagnostic sums over
+ * particles and cells use the canonical tree (orc_tree_sum).
+ *
+ * Pinned by (tests/test_oracle_lorenz2d.py):
+ *   orclg_wavenumbers  the FFTHelper closed form (kx = 2pi/Lx*[0:N/2-1, -N/2:-1]).
+ *   orclg_boris        E = 0: |v| kept, turn angle 2 atan(|t|) about B; B = 0:
+ *                      v += q_m E dt; E x B drift of the average velocity.
+ *   orclg_gather       constant fields, linear reproduction of the B-splines.
+ *   orclg_solve        single-mode closed forms of E = -grad phi - dA/dt and
+ *                      B = curl A in an Lx x Ly box against numpy's FFT; the
+ *                      per-mode recurrence of the vacuum wave (determinant 1,
+ *                      |lambda| = 1 for theta >= 1/2) over many steps.
+ *   orclg_warmup       rho0 stays zero, J0 is the deposit of the round-trip
+ *                      advected particles.
+ *   orclg_run          a numpy restatement of loop! and diagnose! (FFT in
+ *                      numpy, deposits in floating point) over a few steps;
+ *                      diagnose!'s in-place transforms; the binned averages
+ *                      with ngskip = 2; two calls equal one.
+ */
+#include "picoracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef __int128 i128;
+typedef unsigned __int128 u128;
+
+static const double kTwoPiLG = 6.283185307179586; /* Julia 2pi = 2*Float64(pi) */
+
+static inline int64_t wrap_cell(int64_t c, int32_t N) { return (c - 1) & (int64_t)(N - 1); }
+
+/* FFTHelper (:251-267): kx = 2pi/Lx * vcat(0:NX/2-1, -NX/2:-1), ky likewise,
+ * k2 = kx.^2 .+ ky'.^2. */
+void orclg_wavenumbers(int32_t NX, int32_t NY, double Lx, double Ly, double* kx, double* ky, double* k2) {
+    const double sx = kTwoPiLG / Lx, sy = kTwoPiLG / Ly;
+    for (int32_t i = 0; i < NX; ++i) kx[i] = sx * (double)(i < NX / 2 ? i : i - NX);
+    for (int32_t j = 0; j < NY; ++j) ky[j] = sy * (double)(j < NY / 2 ? j : j - NY);
+    for (int32_t j = 0; j < NY; ++j)
+        for (int32_t i = 0; i < NX; ++i) k2[i + (int64_t)NX * j] = kx[i] * kx[i] + ky[j] * ky[j];
+}
+
+/* ElectromagneticBoris (:56-69): theta = (dt/2)*q_m, t = B*theta,
+ * s = 2/(1 + dot(t, t)), e = E*theta, m = v + e,
+ * v+ = m + cross(m + cross(m, t), t)*s, v = v+ + e. */
+void orclg_boris(double dt, double q_m, const double* E, const double* B, double* v) {
+    const double th = dt / 2.0 * q_m;
+    const double t1 = B[0] * th, t2 = B[1] * th, t3 = B[2] * th;
+    const double sc = 2.0 / (1.0 + (t1 * t1 + t2 * t2 + t3 * t3));
+    const double e1 = E[0] * th, e2 = E[1] * th, e3 = E[2] * th;
+    const double m1 = v[0] + e1, m2 = v[1] + e2, m3 = v[2] + e3;
+    const double c1 = m2 * t3 - m3 * t2, c2 = m3 * t1 - m1 * t3, c3 = m1 * t2 - m2 * t1;
+    const double u1 = m1 + c1, u2 = m2 + c2, u3 = m3 + c3;
+    const double d1 = u2 * t3 - u3 * t2, d2 = u3 * t1 - u1 * t3, d3 = u1 * t2 - u2 * t1;
+    v[0] = (m1 + d1 * sc) + e1;
+    v[1] = (m2 + d2 * sc) + e2;
+    v[2] = (m3 + d3 * sc) + e3;
+}
+
+/* (f::AbstractLorenzGaugeField)(shape, x, y) (:1231-1247): j outer, i inner,
+ * wxy = wx*wy, F = muladd(EBxyz[c, i, j], wxy, F) taken as a fused
+ * multiply-add.  EB holds the six NX x NY components back to back. */
+void orclg_gather(int32_t NX, int32_t NY, double NX_Lx, double NY_Ly, int32_t shape, const double* EB, double x,
+                  double y, double* out) {
+    int64_t cx[6], cy[6];
+    double wx[6], wy[6];
+    const int nx = orc23_weights(shape, x, NX_Lx, cx, wx);
+    const int ny = orc23_weights(shape, y, NY_Ly, cy, wy);
+    const int64_t G = (int64_t)NX * NY;
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (int jj = 0; jj < ny; ++jj)
+        for (int ii = 0; ii < nx; ++ii) {
+            const double wxy = wx[ii] * wy[jj];
+            const int64_t k = wrap_cell(cx[ii], NX) + (int64_t)NX * wrap_cell(cy[jj], NY);
+            for (int c = 0; c < 6; ++c) a[c] = fma(EB[c * G + k], wxy, a[c]);
+        }
+    for (int c = 0; c < 6; ++c) out[c] = a[c];
+}
+
+/* ifft! in place of one complex array (FFTW's plan_ifft!: the unnormalised
+ * inverse, then every element times 1/(NX*NY)). */
+static void ifft2_inplace(int32_t NX, int32_t NY, double* z) {
+    const int64_t G = (int64_t)NX * NY;
+    double* re = (double*)malloc(sizeof(double) * (size_t)G);
+    double* im = (double*)malloc(sizeof(double) * (size_t)G);
+    for (int64_t k = 0; k < G; ++k) {
+        re[k] = z[2 * k];
+        im[k] = z[2 * k + 1];
+    }
+    orc2d_fft(NX, NY, re, im, 1);
+    const double s = 1.0 / (double)G;
+    for (int64_t k = 0; k < G; ++k) {
+        z[2 * k] = re[k] * s;
+        z[2 * k + 1] = im[k] * s;
+    }
+    free(re);
+    free(im);
+}
+
+void orclg_ifft2(int32_t NX, int32_t NY, double* z) { ifft2_inplace(NX, NY, z); }
+
+static inline void cmul(double ar, double ai, double br, double bi, double* cr, double* ci) {
+    *cr = ar * br - ai * bi;
+    *ci = ar * bi + ai * br;
+}
+
+/* Steps 1 and 2 of loop!: plus (4 complex arrays: phi+, Ax+, Ay+, Az+) from
+ * the state, then Ex..Bz (state slots 12..17) in the spectrum, then their
+ * inverse transforms.  -im*k is (0*k, -k) and im*k is (0*k, k), as Julia
+ * multiplies Complex{Int}(0,-1) and Complex{Bool}(false,true) by a Float64. */
+void orclg_solve(int32_t NX, int32_t NY, double Lx, double Ly, double dt, double theta, double* F, double* plus) {
+    const int64_t G = (int64_t)NX * NY;
+    double* kx = (double*)malloc(sizeof(double) * (size_t)NX);
+    double* ky = (double*)malloc(sizeof(double) * (size_t)NY);
+    double* k2 = (double*)malloc(sizeof(double) * (size_t)G);
+    orclg_wavenumbers(NX, NY, Lx, Ly, kx, ky, k2);
+    const double dt2 = dt * dt;
+    double* Z[18];
+    for (int a = 0; a < 18; ++a) Z[a] = F + (size_t)a * 2 * G;
+    double* Pp[4];
+    for (int a = 0; a < 4; ++a) Pp[a] = plus + (size_t)a * 2 * G;
+    static const int x0s[4] = {ORCLG_PHI0, ORCLG_AX0, ORCLG_AY0, ORCLG_AZ0};
+    static const int xms[4] = {ORCLG_PHIM, ORCLG_AXM, ORCLG_AYM, ORCLG_AZM};
+    static const int srcs[4] = {ORCLG_RHO0, ORCLG_JX0, ORCLG_JY0, ORCLG_JZ0};
+    for (int32_t j = 0; j < NY; ++j)
+        for (int32_t i = 0; i < NX; ++i) {
+            const int64_t k = i + (int64_t)NX * j;
+            const double num = 2.0 - dt2 * k2[k] * (1.0 - theta);
+            const double den = 1.0 + dt2 * k2[k] * theta / 2.0;
+            for (int a = 0; a < 4; ++a) {
+                const double* x0 = Z[x0s[a]] + 2 * k;
+                const double* xm = Z[xms[a]] + 2 * k;
+                const double* sr = Z[srcs[a]] + 2 * k;
+                Pp[a][2 * k] = (num * x0[0] + dt2 * sr[0]) / den - xm[0];
+                Pp[a][2 * k + 1] = (num * x0[1] + dt2 * sr[1]) / den - xm[1];
+            }
+            const double kxi = kx[i], kyj = ky[j];
+            const double* ph = Pp[0] + 2 * k;
+            const double *axp = Pp[1] + 2 * k, *ayp = Pp[2] + 2 * k, *azp = Pp[3] + 2 * k;
+            const double *ax0 = Z[ORCLG_AX0] + 2 * k, *ay0 = Z[ORCLG_AY0] + 2 * k, *az0 = Z[ORCLG_AZ0] + 2 * k;
+            double er, ei;
+            /* Ex = -im*kx*phi+ ; Ex -= (Ax+ - Ax0)/dt */
+            cmul(0.0 * kxi, -kxi, ph[0], ph[1], &er, &ei);
+            Z[ORCLG_EX][2 * k] = er - (axp[0] - ax0[0]) / dt;
+            Z[ORCLG_EX][2 * k + 1] = ei - (axp[1] - ax0[1]) / dt;
+            cmul(0.0 * kyj, -kyj, ph[0], ph[1], &er, &ei);
+            Z[ORCLG_EY][2 * k] = er - (ayp[0] - ay0[0]) / dt;
+            Z[ORCLG_EY][2 * k + 1] = ei - (ayp[1] - ay0[1]) / dt;
+            /* Ez = -(Az+ - Az0)/dt */
+            Z[ORCLG_EZ][2 * k] = -(azp[0] - az0[0]) / dt;
+            Z[ORCLG_EZ][2 * k + 1] = -(azp[1] - az0[1]) / dt;
+            /* Bx = im*ky*(Az+ + Az0)/2 ; By = -im*kx*(Az+ + Az0)/2 */
+            const double szr = azp[0] + az0[0], szi = azp[1] + az0[1];
+            cmul(0.0 * kyj, kyj, szr, szi, &er, &ei);
+            Z[ORCLG_BX][2 * k] = er / 2.0;
+            Z[ORCLG_BX][2 * k + 1] = ei / 2.0;
+            cmul(0.0 * kxi, -kxi, szr, szi, &er, &ei);
+            Z[ORCLG_BY][2 * k] = er / 2.0;
+            Z[ORCLG_BY][2 * k + 1] = ei / 2.0;
+            /* Bz = im*kx*(Ay+ + Ay0)/2 ; Bz -= im*ky*(Ax+ + Ax0)/2 */
+            double fr, fi;
+            cmul(0.0 * kxi, kxi, ayp[0] + ay0[0], ayp[1] + ay0[1], &er, &ei);
+            cmul(0.0 * kyj, kyj, axp[0] + ax0[0], axp[1] + ax0[1], &fr, &fi);
+            Z[ORCLG_BZ][2 * k] = er / 2.0 - fr / 2.0;
+            Z[ORCLG_BZ][2 * k + 1] = ei / 2.0 - fi / 2.0;
+        }
+    for (int a = ORCLG_EX; a <= ORCLG_BZ; ++a) ifft2_inplace(NX, NY, Z[a]);
+    free(kx);
+    free(ky);
+    free(k2);
+}
+
+/* update! (:468-479) on the NX x NY grid: EB = 0.0 + real(F), plus B0 for
+ * the magnetic components. */
+void orclg_update(int32_t NX, int32_t NY, const double* F, const double* B0, double* EB) {
+    const int64_t G = (int64_t)NX * NY;
+    for (int c = 0; c < 6; ++c) {
+        const double* z = F + (size_t)(ORCLG_EX + c) * 2 * G;
+        for (int64_t k = 0; k < G; ++k) {
+            const double e = 0.0 + z[2 * k];
+            EB[c * G + k] = c < 3 ? e : e + B0[c - 3];
+        }
+    }
+}
+
+static i128 fix_signed(double a, int q) {
+    uint64_t lo, hi;
+    orc_fxp_from_double(fabs(a), q, &lo, &hi);
+    const i128 v = (i128)(((u128)hi << 64) | lo);
+    return a < 0.0 ? -v : v;
+}
+
+static double unfix_signed(i128 v, int q) {
+    if (v < 0) {
+        const u128 u = (u128)(-v);
+        return -orc_fxp_to_double((uint64_t)u, (uint64_t)(u >> 64), q);
+    }
+    return orc_fxp_to_double((uint64_t)v, (uint64_t)((u128)v >> 64), q);
+}
+
+/* q of an exact sum whose contributions are bounded by sum_s P_s*(|w_s|*M_s) */
+int orclg_frac_bits(int32_t ns, const int64_t* P, const double* w, const double* M) {
+    double total = 0.0;
+    for (int s = 0; s < ns; ++s) total += (double)P[s] * (fabs(w[s]) * (M ? M[s] : 1.0));
+    if (!(total > 0.0) || !isfinite(total)) return 64;
+    const int q = 110 - ilogb(total);
+    return q > 1100 ? 1100 : q;
+}
+
+/* deposit (wx*wy)*w[p] (or *w0 when w is NULL) of n particles into acc */
+static void deposit_into(i128* acc, int32_t NX, int32_t NY, double NX_Lx, double NY_Ly, int32_t shape, int64_t n,
+                         const double* x, const double* y, const double* w, double w0, int q) {
+    for (int64_t p = 0; p < n; ++p) {
+        int64_t cx[6], cy[6];
+        double wx[6], wy[6];
+        const int nx = orc23_weights(shape, x[p], NX_Lx, cx, wx);
+        const int ny = orc23_weights(shape, y[p], NY_Ly, cy, wy);
+        const double ww = w ? w[p] : w0;
+        for (int jj = 0; jj < ny; ++jj)
+            for (int ii = 0; ii < nx; ++ii) {
+                const double c = wx[ii] * wy[jj] * ww;
+                acc[wrap_cell(cx[ii], NX) + (int64_t)NX * wrap_cell(cy[jj], NY)] += fix_signed(c, q);
+            }
+    }
+}
+
+/* J0 (slots 9..11, imaginary parts 0) as the exact deposit of
+ * (vx, vy, vz)*qw_s of every species at its current position. */
+static void deposit_currents(int32_t NX, int32_t NY, double Lx, double Ly, int32_t ns, const int32_t* shape,
+                             const int64_t* P, const double* qw, const double* state, double* F) {
+    const int64_t G = (int64_t)NX * NY;
+    const double NX_Lx = (double)NX / Lx, NY_Ly = (double)NY / Ly;
+    double M[3][8];
+    const double* st = state;
+    for (int s = 0; s < ns; ++s) {
+        for (int c = 0; c < 3; ++c) {
+            double m = 0.0;
+            for (int64_t p = 0; p < P[s]; ++p) {
+                const double a = fabs(st[(2 + c) * P[s] + p]);
+                m = a > m ? a : m;
+            }
+            M[c][s] = m;
+        }
+        st += 5 * P[s];
+    }
+    i128* acc = (i128*)malloc(sizeof(i128) * (size_t)G);
+    for (int c = 0; c < 3; ++c) {
+        const int q = orclg_frac_bits(ns, P, qw, M[c]);
+        memset(acc, 0, sizeof(i128) * (size_t)G);
+        st = state;
+        for (int s = 0; s < ns; ++s) {
+            const int64_t n = P[s];
+            double* w = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+            for (int64_t p = 0; p < n; ++p) w[p] = st[(2 + c) * n + p] * qw[s];
+            deposit_into(acc, NX, NY, NX_Lx, NY_Ly, shape[s], n, st, st + n, w, 0.0, q);
+            free(w);
+            st += 5 * n;
+        }
+        double* z = F + (size_t)(ORCLG_JX0 + c) * 2 * G;
+        for (int64_t k = 0; k < G; ++k) {
+            z[2 * k] = unfix_signed(acc[k], q);
+            z[2 * k + 1] = 0.0;
+        }
+    }
+    free(acc);
+}
+
+/* warmup!(::LorenzGaugeStaggeredField) (:826-859): advect by -dt/2, deposit
+ * rho (which the second deposit!'s `rhoJs .= 0` discards, :803), advect by
+ * +dt/2, deposit J; the reduction leaves rho0 = 0 and J0 = the currents. */
+void orclg_warmup(int32_t NX, int32_t NY, double Lx, double Ly, double dt, int32_t ns, const int32_t* shape,
+                  const int64_t* P, const double* charge, const double* weight, double* state, double* F) {
+    const int64_t G = (int64_t)NX * NY;
+    const double dV = (Lx / (double)NX) * (Ly / (double)NY);
+    double qw[8];
+    for (int s = 0; s < ns; ++s) qw[s] = charge[s] * weight[s] / dV;
+    for (int pass = 0; pass < 2; ++pass) {
+        const double h = pass == 0 ? -dt / 2.0 : dt / 2.0;
+        double* st = state;
+        for (int s = 0; s < ns; ++s) {
+            const int64_t n = P[s];
+            for (int64_t p = 0; p < n; ++p) {
+                st[p] = orc23_unimod(st[p] + st[2 * n + p] * h, Lx);
+                st[n + p] = orc23_unimod(st[n + p] + st[3 * n + p] * h, Ly);
+            }
+            st += 5 * n;
+        }
+    }
+    double* rho = F + (size_t)ORCLG_RHO0 * 2 * G;
+    for (int64_t k = 0; k < 2 * G; ++k) rho[k] = 0.0;
+    deposit_currents(NX, NY, Lx, Ly, ns, shape, P, qw, state, F);
+}
+
+/* the row of diagnose! (:1336-1357): kinetic energy, field energy (mean of
+ * abs2 over the six ghosted EBxyz components, /2), particle momentum (3),
+ * field momentum (sum of real(E) x real(B) over the cells / NX*NY, 3),
+ * characteristic momentum (3). */
+void orclg_row(int32_t NX, int32_t NY, int32_t buffer, const double* F, const double* EB, int32_t ns,
+               const int64_t* P, const double* mass, const double* weight, const double* state, double* row) {
+    const int64_t G = (int64_t)NX * NY;
+    const int64_t NXp = NX + 2 * (int64_t)buffer, NYp = NY + 2 * (int64_t)buffer;
+    const int64_t nf = 6 * NXp * NYp;
+    double* t = (double*)malloc(sizeof(double) * (size_t)(nf > 3 * G ? nf : 3 * G));
+    int64_t e = 0;
+    for (int64_t jp = 0; jp < NYp; ++jp)
+        for (int64_t ip = 0; ip < NXp; ++ip) {
+            const int64_t k = wrap_cell(ip - (buffer - 1), NX) + (int64_t)NX * wrap_cell(jp - (buffer - 1), NY);
+            for (int c = 0; c < 6; ++c) {
+                const double v = EB[c * G + k];
+                t[e++] = v * v;
+            }
+        }
+    double pm[3], pc[3];
+    orc23_species_moments(ns, P, mass, weight, state, &row[0], pm, pc);
+    row[1] = orc_tree_sum(t, nf) / (double)nf / 2.0;
+    const double *ex = F + (size_t)ORCLG_EX * 2 * G, *ey = F + (size_t)ORCLG_EY * 2 * G,
+                 *ez = F + (size_t)ORCLG_EZ * 2 * G, *bx = F + (size_t)ORCLG_BX * 2 * G,
+                 *by = F + (size_t)ORCLG_BY * 2 * G, *bz = F + (size_t)ORCLG_BZ * 2 * G;
+    for (int c = 0; c < 3; ++c) {
+        for (int64_t k = 0; k < G; ++k) {
+            const double Ex = ex[2 * k], Ey = ey[2 * k], Ez = ez[2 * k];
+            const double Bx = bx[2 * k], By = by[2 * k], Bz = bz[2 * k];
+            t[k] = c == 0 ? Ey * Bz - Ez * By : (c == 1 ? Ez * Bx - Ex * Bz : Ex * By - Ey * Bx);
+        }
+        row[5 + c] = orc_tree_sum(t, G) / (double)G;
+    }
+    for (int c = 0; c < 3; ++c) {
+        row[2 + c] = pm[c];
+        row[8 + c] = pc[c];
+    }
+    free(t);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Time loop: for t = t0 .. t0+nsteps-1, loop!(plasma, field) (:875-979) and, */
+/* if diagnose, diagnose!(d, field, plasma, t) (:1332-1398):                 */
+/*   if t % ntskip == 0 the row of bin t/ntskip (11 doubles, orclg_row);      */
+/*   ifft! in place of Ax0, Ay0, Az0, phi0, rho0, Jx0, Jy0, Jz0 (:1359-1368); */
+/*   every step, bins[f][t/ntskip][il, jl] += real(Z[il*g + ii, jl*g + jj]) *  */
+/*   (1/(ntskip*g^2)) for jj outer, ii inner (:1370-1386), g = ngskip, for   */
+/*   the fourteen fields Ex, Ey, Ez, Bx, By, Bz, Ax0, Ay0, Az0, phi0, rho0,   */
+/*   Jx0, Jy0, Jz0 (after the transforms).                                    */
+/* state: species after species, each x, y, vx, vy, vz (P[s] each).  rows     */
+/* receives one row per t % ntskip == 0 step of this call; bins (may be NULL) */
+/* holds 14 x nbins x (NX/ngskip) x (NY/ngskip).                             */
+/* ------------------------------------------------------------------------ */
+int orclg_run(int32_t NX, int32_t NY, double Lx, double Ly, double dt, const double* B0, int32_t buffer,
+              double theta, int32_t ntskip, int32_t ngskip, int32_t ns, const int32_t* shape, const int64_t* P,
+              const double* charge, const double* mass, const double* weight, double* state, double* F,
+              int64_t t0, int64_t nsteps, int32_t diagnose, double* rows, double* bins, int64_t nbins) {
+    if (ns < 1 || ns > 8 || NX < 2 || NY < 2 || (NX & (NX - 1)) || (NY & (NY - 1)) || ntskip < 1 || ngskip < 1 ||
+        NX % ngskip || NY % ngskip)
+        return -1;
+    const int64_t G = (int64_t)NX * NY;
+    const double NX_Lx = (double)NX / Lx, NY_Ly = (double)NY / Ly;
+    const double dV = (Lx / (double)NX) * (Ly / (double)NY);
+    double qw[8];
+    for (int s = 0; s < ns; ++s) qw[s] = charge[s] * weight[s] / dV;
+    const int qr = orclg_frac_bits(ns, P, qw, NULL);
+    const int32_t NXd = NX / ngskip, NYd = NY / ngskip;
+    double* plus = (double*)malloc(sizeof(double) * 8 * (size_t)G);
+    double* EB = (double*)malloc(sizeof(double) * 6 * (size_t)G);
+    i128* acc = (i128*)malloc(sizeof(i128) * (size_t)G);
+    int64_t Ptot = 0;
+    for (int s = 0; s < ns; ++s) Ptot += P[s];
+    double* xh = (double*)malloc(sizeof(double) * (size_t)(Ptot > 0 ? Ptot : 1));
+    double* yh = (double*)malloc(sizeof(double) * (size_t)(Ptot > 0 ? Ptot : 1));
+    int64_t nrow = 0;
+    for (int64_t t = t0; t < t0 + nsteps; ++t) {
+        orclg_solve(NX, NY, Lx, Ly, dt, theta, F, plus);
+        orclg_update(NX, NY, F, B0, EB);
+        /* particles: gather, Boris, half move (the rho position), half move */
+        double* st = state;
+        int64_t off = 0;
+        for (int s = 0; s < ns; ++s) {
+            const int64_t n = P[s];
+            double *x = st, *y = st + n, *vx = st + 2 * n, *vy = st + 3 * n, *vz = st + 4 * n;
+            const double q_m = charge[s] / mass[s];
+            for (int64_t p = 0; p < n; ++p) {
+                double f[6], v[3] = {vx[p], vy[p], vz[p]};
+                orclg_gather(NX, NY, NX_Lx, NY_Ly, shape[s], EB, x[p], y[p], f);
+                orclg_boris(dt, q_m, f, f + 3, v);
+                vx[p] = v[0];
+                vy[p] = v[1];
+                vz[p] = v[2];
+                x[p] = orc23_unimod(x[p] + vx[p] * dt / 2.0, Lx);
+                y[p] = orc23_unimod(y[p] + vy[p] * dt / 2.0, Ly);
+                xh[off + p] = x[p];
+                yh[off + p] = y[p];
+                x[p] = orc23_unimod(x[p] + vx[p] * dt / 2.0, Lx);
+                y[p] = orc23_unimod(y[p] + vy[p] * dt / 2.0, Ly);
+            }
+            st += 5 * n;
+            off += n;
+        }
+        /* reduction: rho0 from the half-step positions, J0 from the new ones */
+        memset(acc, 0, sizeof(i128) * (size_t)G);
+        off = 0;
+        for (int s = 0; s < ns; ++s) {
+            deposit_into(acc, NX, NY, NX_Lx, NY_Ly, shape[s], P[s], xh + off, yh + off, NULL, qw[s], qr);
+            off += P[s];
+        }
+        double* rho = F + (size_t)ORCLG_RHO0 * 2 * G;
+        for (int64_t k = 0; k < G; ++k) {
+            rho[2 * k] = unfix_signed(acc[k], qr);
+            rho[2 * k + 1] = 0.0;
+        }
+        deposit_currents(NX, NY, Lx, Ly, ns, shape, P, qw, state, F);
+        /* phi- = phi0, phi0 = phi+, A- = A0, A0 = A+ */
+        static const int x0s[4] = {ORCLG_PHI0, ORCLG_AX0, ORCLG_AY0, ORCLG_AZ0};
+        static const int xms[4] = {ORCLG_PHIM, ORCLG_AXM, ORCLG_AYM, ORCLG_AZM};
+        for (int a = 0; a < 4; ++a) {
+            memcpy(F + (size_t)xms[a] * 2 * G, F + (size_t)x0s[a] * 2 * G, sizeof(double) * 2 * (size_t)G);
+            memcpy(F + (size_t)x0s[a] * 2 * G, plus + (size_t)a * 2 * G, sizeof(double) * 2 * (size_t)G);
+        }
+        if (!diagnose) continue;
+        if (t % ntskip == 0 && rows)
+            orclg_row(NX, NY, buffer, F, EB, ns, P, mass, weight, state, rows + 11 * nrow++);
+        static const int tr[8] = {ORCLG_AX0, ORCLG_AY0, ORCLG_AZ0, ORCLG_PHI0,
+                                  ORCLG_RHO0, ORCLG_JX0, ORCLG_JY0, ORCLG_JZ0};
+        for (int a = 0; a < 8; ++a) ifft2_inplace(NX, NY, F + (size_t)tr[a] * 2 * G);
+        const int64_t b = t / ntskip;
+        if (!bins || b >= nbins) continue;
+        static const int av[14] = {ORCLG_EX,  ORCLG_EY,   ORCLG_EZ,  ORCLG_BX,  ORCLG_BY,
+                                   ORCLG_BZ,  ORCLG_AX0,  ORCLG_AY0, ORCLG_AZ0, ORCLG_PHI0,
+                                   ORCLG_RHO0, ORCLG_JX0, ORCLG_JY0, ORCLG_JZ0};
+        const double factor = 1.0 / (double)((int64_t)ntskip * ngskip * ngskip);
+        for (int a = 0; a < 14; ++a) {
+            const double* z = F + (size_t)av[a] * 2 * G;
+            double* o = bins + ((size_t)a * nbins + b) * (size_t)NXd * NYd;
+            for (int32_t jl = 0; jl < NYd; ++jl)
+                for (int32_t il = 0; il < NXd; ++il) {
+                    double acc2 = o[il + (int64_t)NXd * jl];
+                    for (int32_t jj = 0; jj < ngskip; ++jj)
+                        for (int32_t ii = 0; ii < ngskip; ++ii) {
+                            const int64_t k = (int64_t)il * ngskip + ii + (int64_t)NX * ((int64_t)jl * ngskip + jj);
+                            acc2 = acc2 + z[2 * k] * factor;
+                        }
+                    o[il + (int64_t)NXd * jl] = acc2;
+                }
+        }
+    }
+    free(plus);
+    free(EB);
+    free(acc);
+    free(xh);
+    free(yh);
+    return 0;
+}

@@ -113,6 +113,17 @@ def lib():
         "orc_emx_push": (None, [c_i32, c_d, _dp, _dp, c_d, _dp]),
         "orc_emx_run": (ctypes.c_int, [c_i32, c_i64, c_d, c_d, c_d, c_i32, _dp, _dp, _dp, _dp, _dp, _dp, c_i64,
                                        c_vp]),
+        "orclg_wavenumbers": (None, [c_i32, c_i32, c_d, c_d, _dp, _dp, _dp]),
+        "orclg_boris": (None, [c_d, c_d, _dp, _dp, _dp]),
+        "orclg_gather": (None, [c_i32, c_i32, c_d, c_d, c_i32, _dp, c_d, c_d, _dp]),
+        "orclg_ifft2": (None, [c_i32, c_i32, _dp]),
+        "orclg_solve": (None, [c_i32, c_i32, c_d, c_d, c_d, c_d, _dp, _dp]),
+        "orclg_update": (None, [c_i32, c_i32, _dp, _dp, _dp]),
+        "orclg_frac_bits": (ctypes.c_int, [c_i32, _ip64, _dp, c_vp]),
+        "orclg_warmup": (None, [c_i32, c_i32, c_d, c_d, c_d, c_i32, _ip32, _ip64, _dp, _dp, _dp, _dp]),
+        "orclg_row": (None, [c_i32, c_i32, c_i32, _dp, _dp, c_i32, _ip64, _dp, _dp, _dp, _dp]),
+        "orclg_run": (ctypes.c_int, [c_i32, c_i32, c_d, c_d, c_d, _dp, c_i32, c_d, c_i32, c_i32, c_i32, _ip32,
+                                     _ip64, _dp, _dp, _dp, _dp, _dp, c_i64, c_i64, c_i32, c_vp, c_vp, c_i64]),
         "orc23_run": (ctypes.c_int, [c_i32, c_i32, c_d, c_d, c_d, _dp, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32,
                                      _ip32, _ip64, _dp, _dp, _dp, _dp, _dp, _dp, c_i64, c_i64, c_vp, c_vp, c_vp,
                                      c_vp, c_i64, c_i32]),
@@ -571,6 +582,131 @@ def run23(NX, NY, Lx, Ly, dt, B, species, nsteps, t0=0, buffer=3, ntskip=1, ngsk
         out.append(d)
         o += 5 * n
     return out, (Ex, Ey), rows, bins, bool(bad)
+
+
+# ---- Lorenz-gauge electromagnetic 2D3V (src/PIC2D3V.jl) -------------------
+# Field state: complex128 array (18, NY, NX) in the order of LG_FIELDS.
+LG_FIELDS = ["phi0", "phim", "Ax0", "Ay0", "Az0", "Axm", "Aym", "Azm", "rho0", "Jx0", "Jy0", "Jz0",
+             "Ex", "Ey", "Ez", "Bx", "By", "Bz"]
+LG_BINS = ["Ex", "Ey", "Ez", "Bx", "By", "Bz", "Ax", "Ay", "Az", "phi", "rho", "Jx", "Jy", "Jz"]
+
+
+def lg_wavenumbers(NX, NY, Lx=1.0, Ly=1.0):
+    kx, ky, k2 = np.zeros(NX), np.zeros(NY), np.zeros((NY, NX))
+    lib().orclg_wavenumbers(int(NX), int(NY), float(Lx), float(Ly), kx, ky, k2.ravel())
+    return kx, ky, k2
+
+
+def lg_boris(dt, q_m, E, B, v):
+    v = _f64(v).copy()
+    lib().orclg_boris(float(dt), float(q_m), _f64(E), _f64(B), v)
+    return v
+
+
+def lg_gather(EB, x, y, shape, Lx=1.0, Ly=1.0):
+    """EB: (6, NY, NX) real: Ex, Ey, Ez, Bx, By, Bz."""
+    EB = _f64(EB)
+    _, NY, NX = EB.shape
+    out = np.zeros(6)
+    lib().orclg_gather(NX, NY, NX / Lx, NY / Ly, _shape23(shape), EB.ravel(), float(x), float(y), out)
+    return out
+
+
+def _c128(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+
+
+def _cp(a):
+    return a.view(np.float64).ravel()
+
+
+def lg_ifft2(z):
+    z = _c128(z).copy()
+    NY, NX = z.shape
+    lib().orclg_ifft2(NX, NY, _cp(z))
+    return z
+
+
+def lg_solve(F, dt, theta, Lx=1.0, Ly=1.0):
+    """Steps 1-2 of loop!: returns (F with Ex..Bz replaced, plus (4, NY, NX))."""
+    F = _c128(F).copy()
+    _, NY, NX = F.shape
+    plus = np.zeros((4, NY, NX), dtype=np.complex128)
+    lib().orclg_solve(NX, NY, float(Lx), float(Ly), float(dt), float(theta), _cp(F), _cp(plus))
+    return F, plus
+
+
+def lg_update(F, B0):
+    F = _c128(F)
+    _, NY, NX = F.shape
+    EB = np.zeros((6, NY, NX))
+    lib().orclg_update(NX, NY, _cp(F), _f64(B0), EB.ravel())
+    return EB
+
+
+def lg_frac_bits(P, w, M=None):
+    P = np.asarray(P, dtype=np.int64)
+    return lib().orclg_frac_bits(P.size, P, _f64(w), _ptr(None if M is None else _f64(M)))
+
+
+def _lg_species(species):
+    P = np.array([s["xyv"].shape[1] for s in species], dtype=np.int64)
+    state = np.concatenate([_f64(s["xyv"]).ravel() for s in species])
+    shape = np.array([_shape23(s["shape"]) for s in species], dtype=np.int32)
+    return P, state, shape
+
+
+def _lg_unpack(species, P, state):
+    out, o = [], 0
+    for s, n in zip(species, P):
+        d = dict(s)
+        d["xyv"] = state[o:o + 5 * n].reshape(5, n).copy()
+        out.append(d)
+        o += 5 * n
+    return out
+
+
+def lg_warmup(NX, NY, Lx, Ly, dt, species, F=None):
+    """warmup!(::LorenzGaugeStaggeredField): returns (species, F)."""
+    P, state, shape = _lg_species(species)
+    F = np.zeros((18, NY, NX), dtype=np.complex128) if F is None else _c128(F).copy()
+    lib().orclg_warmup(int(NX), int(NY), float(Lx), float(Ly), float(dt), len(species), shape, P,
+                       _f64([s["charge"] for s in species]), _f64([s["weight"] for s in species]), state, _cp(F))
+    return _lg_unpack(species, P, state), F
+
+
+def lg_row(F, EB, species, buffer):
+    P, state, _ = _lg_species(species)
+    _, NY, NX = EB.shape
+    row = np.zeros(11)
+    lib().orclg_row(NX, NY, int(buffer), _cp(_c128(F)), _f64(EB).ravel(), len(species), P,
+                    _f64([s["mass"] for s in species]), _f64([s["weight"] for s in species]), state, row)
+    return row
+
+
+def lg_run(NX, NY, Lx, Ly, dt, B0, species, F, nsteps, theta=1.0, t0=0, buffer=10, ntskip=1, ngskip=1,
+           diagnose=True, nbins=None, bins=None):
+    """nsteps of loop!(plasma, ::LorenzGaugeStaggeredField) and (diagnose)
+    diagnose!(::LorenzGaugeDiagnostics).  F: (18, NY, NX) complex field
+    state.  bins: (14, nbins, NY/ngskip, NX/ngskip), accumulated in place
+    (zeros when None).  Returns (species, F, rows (n, 11), bins)."""
+    P, state, shape = _lg_species(species)
+    F = _c128(F).copy()
+    t1 = t0 + nsteps
+    nrows = (-(-t1 // ntskip) - (-(-t0 // ntskip))) if diagnose else 0
+    rows = np.zeros((max(nrows, 0), 11))
+    if nbins is None:
+        nbins = -(-t1 // ntskip)
+    if bins is None:
+        bins = np.zeros((14, nbins, NY // ngskip, NX // ngskip))
+    bins = _f64(bins).copy()
+    rc = lib().orclg_run(int(NX), int(NY), float(Lx), float(Ly), float(dt), _f64(B0), int(buffer), float(theta),
+                         int(ntskip), int(ngskip), len(species), shape, P, _f64([s["charge"] for s in species]),
+                         _f64([s["mass"] for s in species]), _f64([s["weight"] for s in species]), state, _cp(F),
+                         int(t0), int(nsteps), int(bool(diagnose)), _ptr(rows), _ptr(bins), int(nbins))
+    if rc != 0:
+        raise ValueError(f"orclg_run rejected its arguments (rc={rc})")
+    return _lg_unpack(species, P, state), F, rows, bins
 
 
 # ---- 1D electromagnetic fixed point (src/GaussianEMFixedPoint.jl) ----------

@@ -388,6 +388,19 @@ void orc23_diagnostics(int32_t NX, int32_t NY, int32_t buffer, const double* Eac
         }
     row[1] = orc_tree_sum(t, nf) / (double)nf / 2.0;
     free(t);
+    double pm[3], pc[3];
+    orc23_species_moments(ns, P, mass, weight, state, &row[0], pm, pc);
+    for (int c = 0; c < 3; ++c) {
+        row[2 + c] = pm[c];
+        row[5 + c] = pc[c];
+    }
+}
+
+/* kineticenergy, momentum and characteristicmomentum summed over the
+ * species (:174-200, :1304-1309): state holds each species as x, y, vx, vy,
+ * vz (P[s] each), species after species. */
+void orc23_species_moments(int32_t ns, const int64_t* P, const double* mass, const double* weight,
+                           const double* state, double* ke_out, double* pm_out, double* pc_out) {
     double ke = 0.0, pm[3] = {0, 0, 0}, pc[3] = {0, 0, 0};
     const double* st = state;
     for (int s = 0; s < ns; ++s) {
@@ -413,10 +426,10 @@ void orc23_diagnostics(int32_t NX, int32_t NY, int32_t buffer, const double* Eac
         }
         st += 5 * n;
     }
-    row[0] = ke;
+    *ke_out = ke;
     for (int c = 0; c < 3; ++c) {
-        row[2 + c] = pm[c];
-        row[5 + c] = pc[c];
+        pm_out[c] = pm[c];
+        pc_out[c] = pc[c];
     }
 }
 

@@ -163,6 +163,8 @@ uint64_t orc23_order_key(double v);
 int64_t  orc23_ceil_index(double a);
 void     orc23_sort(int64_t P, double* x, double* y, double* vx, double* vy, double* vz, double dx, double dy,
                     int64_t* perm);
+void     orc23_species_moments(int32_t ns, const int64_t* P, const double* mass, const double* weight,
+                               const double* state, double* ke_out, double* pm_out, double* pc_out);
 void     orc23_diagnostics(int32_t NX, int32_t NY, int32_t buffer, const double* Eacc_x, const double* Eacc_y,
                            int32_t ns, const int64_t* P, const double* mass, const double* weight,
                            const double* state, double* row);
@@ -172,6 +174,43 @@ int      orc23_run(int32_t NX, int32_t NY, double Lx, double Ly, double dt, cons
                    const double* weight, double* state, double* Eacc_x, double* Eacc_y, int64_t t0,
                    int64_t nsteps, double* rows, double* Exs, double* Eys, double* phis, int64_t nbins,
                    int32_t nthreads);
+
+/* Lorenz-gauge electromagnetic 2D3V (src/PIC2D3V.jl LorenzGaugeStaggeredField), in
+ * lorenz2d.c.  Field state: 18 complex NX x NY arrays (interleaved re, im). */
+#define ORCLG_PHI0 0
+#define ORCLG_PHIM 1
+#define ORCLG_AX0 2
+#define ORCLG_AY0 3
+#define ORCLG_AZ0 4
+#define ORCLG_AXM 5
+#define ORCLG_AYM 6
+#define ORCLG_AZM 7
+#define ORCLG_RHO0 8
+#define ORCLG_JX0 9
+#define ORCLG_JY0 10
+#define ORCLG_JZ0 11
+#define ORCLG_EX 12
+#define ORCLG_EY 13
+#define ORCLG_EZ 14
+#define ORCLG_BX 15
+#define ORCLG_BY 16
+#define ORCLG_BZ 17
+void orclg_wavenumbers(int32_t NX, int32_t NY, double Lx, double Ly, double* kx, double* ky, double* k2);
+void orclg_boris(double dt, double q_m, const double* E, const double* B, double* v);
+void orclg_gather(int32_t NX, int32_t NY, double NX_Lx, double NY_Ly, int32_t shape, const double* EB, double x,
+                  double y, double* out);
+void orclg_ifft2(int32_t NX, int32_t NY, double* z);
+void orclg_solve(int32_t NX, int32_t NY, double Lx, double Ly, double dt, double theta, double* F, double* plus);
+void orclg_update(int32_t NX, int32_t NY, const double* F, const double* B0, double* EB);
+int  orclg_frac_bits(int32_t ns, const int64_t* P, const double* w, const double* M);
+void orclg_warmup(int32_t NX, int32_t NY, double Lx, double Ly, double dt, int32_t ns, const int32_t* shape,
+                  const int64_t* P, const double* charge, const double* weight, double* state, double* F);
+void orclg_row(int32_t NX, int32_t NY, int32_t buffer, const double* F, const double* EB, int32_t ns,
+               const int64_t* P, const double* mass, const double* weight, const double* state, double* row);
+int  orclg_run(int32_t NX, int32_t NY, double Lx, double Ly, double dt, const double* B0, int32_t buffer,
+               double theta, int32_t ntskip, int32_t ngskip, int32_t ns, const int32_t* shape, const int64_t* P,
+               const double* charge, const double* mass, const double* weight, double* state, double* F,
+               int64_t t0, int64_t nsteps, int32_t diagnose, double* rows, double* bins, int64_t nbins);
 
 /* 1D electromagnetic fixed point (src/GaussianEMFixedPoint.jl), in em1d.c */
 int64_t orc_em_irn(double c, int32_t N);
