@@ -1,6 +1,7 @@
 /*
- * pic2d3v.h -- C ABI of the electrostatic path of the structured 2D3V module
- * of jwscook/ParticleInCellCodeGolf.jl (src/PIC2D3V.jl) on AMD MI355X (gfx950).
+ * pic2d3v.h -- C ABI of the structured 2D3V module of
+ * jwscook/ParticleInCellCodeGolf.jl (src/PIC2D3V.jl) on AMD MI355X (gfx950):
+ * the electrostatic path and the Lorenz-gauge electromagnetic path.
  *
  * Replaces, for an ElectrostaticField and any number of Species (up to
  * PIC2D3V_MAX_SPECIES):
@@ -14,6 +15,17 @@
  *   loop!(plasma, field)          particle loop, reduction, spectral solve,
  *                                 update!                      :530-581, :294-297
  *   diagnose!(ElectrostaticDiagnostics, ...)                   :1301-1330
+ * and, for a LorenzGaugeStaggeredField (field = PIC2D3V_LORENZ_STAGGERED, as
+ * src/2D3V.jl:102-106 builds it):
+ *   warmup!(field, plasma)        advect -dt/2, +dt/2, the currents   :826-859
+ *   lorenzgauge!(imex, ...)       per-mode wave update, any ImEx theta :610-635
+ *   E = -grad phi+ - dA/dt, B = curl (A+ + A0)/2, ifft, update!     :917-935
+ *   (field)(shape, x, y)          six-component gather             :1231-1247
+ *   ElectromagneticBoris                                           :56-69
+ *   the two half moves, deposit!(rho) at the half step and
+ *   deposit!(Jx, Jy, Jz) at the new positions                      :937-968
+ *   diagnose!(LorenzGaugeDiagnostics, ...)  with its in-place inverse
+ *                                 transforms of the state          :1332-1398
  *
  * FP64 throughout.  Results are bit-identical to the CPU restatement: per-cell
  * charge is an exact signed fixed-point sum (independent of the particle
@@ -40,6 +52,19 @@
  * the distance from the lower cell edge, so a particle's charge centroid is
  * 2 ceil(zN) - zN.
  *
+ * The Lorenz-gauge path keeps the reference's data flow exactly:
+ *   - the wave update reads rho0 and J0 as they are: the real-space
+ *     reduction of the previous step, or its inverse transform once
+ *     diagnose! has run (the forward transform and smoothing of :891-903 are
+ *     overwritten by the reduction before anything reads them);
+ *   - diagnose! inverse-transforms Ax0, Ay0, Az0, phi0, rho0, J0 in place
+ *     (:1359-1368), which changes the next step; pic2d3v_loop runs loop!
+ *     alone;
+ *   - warmup!'s second deposit! clears the first one's charge, so rho0
+ *     starts at zero.
+ * Current sums are exact like the charge, with the fraction bits set from the
+ * largest |v| of each component after the push.
+ *
  * Return codes, ownership, threading and error reporting follow pic.h.
  */
 #ifndef PIC2D3V_H
@@ -65,6 +90,46 @@ extern "C" {
 #define PIC2D3V_BORIS_REFERENCE 0  /* ElectrostaticBoris (q_m^2 rotation)    */
 #define PIC2D3V_BORIS_TEXTBOOK 1   /* ElectromagneticBoris with Ez = 0       */
 
+#define PIC2D3V_ELECTROSTATIC 0    /* ElectrostaticField (:270-297)          */
+#define PIC2D3V_LORENZ_STAGGERED 1 /* LorenzGaugeStaggeredField (:347-392)   */
+
+/* Lorenz-gauge field state: complex NX x NY arrays (pic2d3v_lg_get/set) */
+#define PIC2D3V_LG_PHI0 0
+#define PIC2D3V_LG_PHIM 1  /* phi at the previous step */
+#define PIC2D3V_LG_AX0 2
+#define PIC2D3V_LG_AY0 3
+#define PIC2D3V_LG_AZ0 4
+#define PIC2D3V_LG_AXM 5
+#define PIC2D3V_LG_AYM 6
+#define PIC2D3V_LG_AZM 7
+#define PIC2D3V_LG_RHO0 8
+#define PIC2D3V_LG_JX0 9
+#define PIC2D3V_LG_JY0 10
+#define PIC2D3V_LG_JZ0 11
+#define PIC2D3V_LG_EX 12
+#define PIC2D3V_LG_EY 13
+#define PIC2D3V_LG_EZ 14
+#define PIC2D3V_LG_BX 15
+#define PIC2D3V_LG_BY 16
+#define PIC2D3V_LG_BZ 17
+#define PIC2D3V_LG_NSTATE 18
+/* binned fields of LorenzGaugeDiagnostics, in its order (pic2d3v_lg_bins) */
+#define PIC2D3V_LG_BIN_EX 0
+#define PIC2D3V_LG_BIN_EY 1
+#define PIC2D3V_LG_BIN_EZ 2
+#define PIC2D3V_LG_BIN_BX 3
+#define PIC2D3V_LG_BIN_BY 4
+#define PIC2D3V_LG_BIN_BZ 5
+#define PIC2D3V_LG_BIN_AX 6
+#define PIC2D3V_LG_BIN_AY 7
+#define PIC2D3V_LG_BIN_AZ 8
+#define PIC2D3V_LG_BIN_PHI 9
+#define PIC2D3V_LG_BIN_RHO 10
+#define PIC2D3V_LG_BIN_JX 11
+#define PIC2D3V_LG_BIN_JY 12
+#define PIC2D3V_LG_BIN_JZ 13
+#define PIC2D3V_LG_NBINS 14
+
 typedef struct pic2d3v_species {
     int64_t P;        /* particles                                          */
     int32_t shape;    /* PIC2D3V_NGP, PIC2D3V_AREA or PIC2D3V_BSPLINE0 + n    */
@@ -86,6 +151,8 @@ typedef struct pic2d3v_config {
     int32_t nspecies;      /* 1 .. PIC2D3V_MAX_SPECIES                               */
     pic2d3v_species species[PIC2D3V_MAX_SPECIES];
     int32_t device;        /* HIP device ordinal                                     */
+    int32_t field;         /* PIC2D3V_ELECTROSTATIC or PIC2D3V_LORENZ_STAGGERED       */
+    double theta;          /* Lorenz gauge: ImEx theta (0 Explicit(), 1 Implicit())   */
 } pic2d3v_config;
 
 typedef struct pic2d3v_ctx pic2d3v_ctx;
@@ -103,7 +170,8 @@ const char* pic2d3v_last_error(const pic2d3v_ctx* ctx);
 int pic2d3v_upload(pic2d3v_ctx* ctx, int32_t species, const double* xyv);
 int pic2d3v_download(pic2d3v_ctx* ctx, int32_t species, double* xyv);
 
-/* Back to a fresh field and diagnostics: Exy = 0, t = 0, no rows, no bins. */
+/* Back to a fresh field and diagnostics: Exy (or the Lorenz-gauge state) = 0,
+ * t = 0, no rows, no bins. */
 int pic2d3v_reset(pic2d3v_ctx* ctx);
 
 /* sort!(species, dx, dy): reorders the species' particles by
@@ -118,10 +186,25 @@ int pic2d3v_sort(pic2d3v_ctx* ctx, int32_t species, double dx, double dy, int64_
  * current t.  rows (may be NULL) receives one row of 8 doubles for every t of
  * this call with t % ntskip == 0: kinetic energy, field energy, momentum
  * (x, y, z), characteristic momentum (x, y, z), i.e. kineticenergy[ti],
- * fieldenergy[ti], particlemomentum[ti], characteristicmomentum[ti].
+ * fieldenergy[ti], particlemomentum[ti], characteristicmomentum[ti].  The
+ * Lorenz-gauge path writes rows of 11: kinetic energy, field energy,
+ * particle momentum (3), field momentum (3), characteristic momentum (3).
  * Returns PIC_ENONFINITE if the solve met a non-finite value (the reference's
  * `@assert isfinite(tmp)`, :570); the state is then that of the last step. */
 int pic2d3v_run(pic2d3v_ctx* ctx, int64_t nsteps, double* rows);
+
+/* Lorenz gauge only: warmup!(field, plasma) (:826-859), which the reference's
+ * driver calls once before its loop (src/2D3V.jl:123); nsteps of loop!
+ * without diagnose!; the field state slot `which` (PIC2D3V_LG_*: 2*NX*NY
+ * doubles, complex interleaved as Julia's ComplexF64) out and in; and the
+ * binned field `which` (PIC2D3V_LG_BIN_*) of bins 0 .. nbins-1, laid out as
+ * pic2d3v_fields lays out Exs.  pic2d3v_warmup on an electrostatic context
+ * does nothing, as warmup!(::AbstractField) (:745). */
+int pic2d3v_warmup(pic2d3v_ctx* ctx);
+int pic2d3v_loop(pic2d3v_ctx* ctx, int64_t nsteps);
+int pic2d3v_lg_get(pic2d3v_ctx* ctx, int32_t which, double* z);
+int pic2d3v_lg_set(pic2d3v_ctx* ctx, int32_t which, const double* z);
+int pic2d3v_lg_bins(pic2d3v_ctx* ctx, int32_t which, double* out, int64_t nbins);
 
 /* The time-binned fields of the diagnostics, Exs, Eys, phis[:, :, b] for
  * b < nbins, each (NX/ngskip) x (NY/ngskip) per bin, bins back to back
@@ -130,7 +213,7 @@ int pic2d3v_run(pic2d3v_ctx* ctx, int64_t nsteps, double* rows);
 int pic2d3v_fields(pic2d3v_ctx* ctx, double* Exs, double* Eys, double* phis, int64_t nbins);
 
 /* The gathered field Exy: the interior NX x NY of each component (either
- * may be NULL). */
+ * may be NULL); for the Lorenz gauge, EBxyz's Ex and Ey. */
 int pic2d3v_field(pic2d3v_ctx* ctx, double* Ex, double* Ey);
 
 /* Operators on host arrays, bit-identical to the CPU restatement:

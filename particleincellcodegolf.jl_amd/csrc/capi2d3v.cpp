@@ -1,7 +1,7 @@
-// capi2d3v.cpp -- host runtime behind include/pic2d3v.h: the electrostatic
-// PIC2D3V context (species, field, diagnostics), its device buffers, the
-// hipGraphs of one step (with and without the diagnostics row), the
-// per-cell sort and the operator calls.
+// capi2d3v.cpp -- host runtime behind include/pic2d3v.h: the PIC2D3V context
+// (species, an electrostatic or Lorenz-gauge field, diagnostics), its device
+// buffers, the hipGraphs of one step (with and without the diagnostics row),
+// the per-cell sort, warmup! and the operator calls.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,6 +15,7 @@
 
 #include "../../include/pic2d3v.h"
 #include "pic2d3v_internal.h"
+#include "pic2d3v_lg_internal.h"
 
 #pragma clang fp contract(off)
 
@@ -33,6 +34,13 @@ struct pic2d3v_ctx {
     double* tw = nullptr;
     int64_t step = 0;        // t of the next step
     hipGraphExec_t g_plain = nullptr, g_diag = nullptr;
+    // Lorenz gauge: the state, the half-step positions, and the step graphs
+    // (loop! alone, with diagnose! without and with the row)
+    bool lorenz = false;
+    LG23 lg{};
+    double* xh[kMaxSpecies23] = {};
+    double* yh[kMaxSpecies23] = {};
+    hipGraphExec_t g_lg[3] = {nullptr, nullptr, nullptr};
     void* op[8] = {};        // scratch (grown on demand)
     size_t op_bytes[8] = {};
     std::string err;
@@ -103,15 +111,20 @@ static void free_graphs(pic2d3v_ctx* c) {
     if (c->g_plain) (void)hipGraphExecDestroy(c->g_plain);
     if (c->g_diag) (void)hipGraphExecDestroy(c->g_diag);
     c->g_plain = c->g_diag = nullptr;
+    for (auto& g : c->g_lg) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
 }
 
 static void free_bins(pic2d3v_ctx* c) {
     Diag23& d = c->d;
-    for (double** p : {&d.rows, &d.Exs, &d.Eys, &d.phis}) {
+    for (double** p : {&d.rows, &d.Exs, &d.Eys, &d.phis, &c->lg.bins, &c->lg.rows}) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
     }
     d.cap = 0;
+    c->lg.cap = 0;
 }
 
 static void release(pic2d3v_ctx* c) {
@@ -194,6 +207,14 @@ static int validate(const pic2d3v_config* g, std::string& why) {
         why = "the species hold more than 2^26 particles together";
         return PIC_EINVAL;
     }
+    if (g->field != PIC2D3V_ELECTROSTATIC && g->field != PIC2D3V_LORENZ_STAGGERED) {
+        why = "field must be PIC2D3V_ELECTROSTATIC or PIC2D3V_LORENZ_STAGGERED";
+        return PIC_EINVAL;
+    }
+    if (!std::isfinite(g->theta)) {
+        why = "theta must be finite";
+        return PIC_EINVAL;
+    }
     return PIC_OK;
 }
 
@@ -204,6 +225,19 @@ static int zero_state(pic2d3v_ctx* c) {
     HIPCHK(c, hipMemsetAsync(c->b.racc, 0, 3 * sizeof(unsigned long long) * (size_t)G, c->stream));
     HIPCHK(c, hipMemsetAsync(c->b.flag, 0, sizeof(int32_t), c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d.t, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+    if (c->lorenz) {
+        LG23& L = c->lg;
+        HIPCHK(c, hipMemsetAsync(L.F, 0, sizeof(double2) * kLGSlots * (size_t)G, c->stream));
+        HIPCHK(c, hipMemsetAsync(L.plus, 0, sizeof(double2) * 4 * (size_t)G, c->stream));
+        HIPCHK(c, hipMemsetAsync(L.EB, 0, sizeof(double) * 6 * (size_t)G, c->stream));
+        HIPCHK(c, hipMemsetAsync(L.racc, 0, sizeof(unsigned long long) * 12 * (size_t)G, c->stream));
+        HIPCHK(c, hipMemsetAsync(L.umax, 0, sizeof(unsigned long long) * 3 * kMaxSpecies23, c->stream));
+        if (L.cap > 0) {
+            HIPCHK(c, hipMemsetAsync(L.rows, 0, sizeof(double) * kLGRow * (size_t)L.cap, c->stream));
+            HIPCHK(c, hipMemsetAsync(L.bins, 0, sizeof(double) * kLGBins * (size_t)L.cap * c->d.NXd * c->d.NYd,
+                                     c->stream));
+        }
+    }
     Diag23& d = c->d;
     if (d.cap > 0) {
         const size_t nb = (size_t)d.cap * d.NXd * d.NYd;
@@ -315,6 +349,46 @@ extern "C" int pic2d3v_create(pic2d3v_ctx** out, const pic2d3v_config* cfg) {
             (rc = dalloc(c, &d.mpart[s], 6 * ((P + kTreeChunk - 1) / kTreeChunk))))
             return fail(rc);
     }
+    if (cfg->field == PIC2D3V_LORENZ_STAGGERED) {
+        c->lorenz = true;
+        configure_kernels_2d3v_lg();
+        LG23& L = c->lg;
+        L.NX = NX;
+        L.NY = NY;
+        L.lx = g.lx;
+        L.ly = g.ly;
+        L.dt = cfg->dt;
+        L.theta = cfg->theta;
+        for (int k = 0; k < 3; ++k) L.B0[k] = cfg->B0[k];
+        L.ns = ns;
+        L.qr = g.q;  // the charge's fraction bits: sum_s P_s |qw_s|, as the electrostatic path
+        for (int s = 0; s < ns; ++s) {
+            L.P[s] = Ps[s];
+            L.qw[s] = qws[s];
+        }
+        L.tw = b.tw;
+        double *kx = nullptr, *ky = nullptr;
+        const int64_t nf6 = 6 * (int64_t)(NX + 2 * cfg->buffer) * (NY + 2 * cfg->buffer);
+        if ((rc = dalloc(c, &L.F, kLGSlots * G)) || (rc = dalloc(c, &L.plus, 4 * G)) || (rc = dalloc(c, &L.EB, 6 * G)) ||
+            (rc = dalloc(c, &kx, NX)) || (rc = dalloc(c, &ky, NY)) || (rc = dalloc(c, &L.racc, 12 * G)) ||
+            (rc = dalloc(c, &L.umax, 3 * kMaxSpecies23)) ||
+            (rc = dalloc(c, &L.fpart, (nf6 + kTreeChunk - 1) / kTreeChunk)) ||
+            (rc = dalloc(c, &L.mfpart, 3 * ((G + kTreeChunk - 1) / kTreeChunk))))
+            return fail(rc);
+        for (int s = 0; s < ns; ++s)
+            if ((rc = dalloc(c, &c->xh[s], cfg->species[s].P)) || (rc = dalloc(c, &c->yh[s], cfg->species[s].P)))
+                return fail(rc);
+        // FFTHelper (:251-254): kx = 2pi/Lx * vcat(0:NX/2-1, -NX/2:-1), ky likewise
+        std::vector<double> hx(NX), hy(NY);
+        const double sx = kTwoPi / cfg->Lx, sy = kTwoPi / cfg->Ly;
+        for (int i = 0; i < NX; ++i) hx[i] = sx * (double)(i < NX / 2 ? i : i - NX);
+        for (int j = 0; j < NY; ++j) hy[j] = sy * (double)(j < NY / 2 ? j : j - NY);
+        if (hipMemcpy(kx, hx.data(), sizeof(double) * NX, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(ky, hy.data(), sizeof(double) * NY, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(PIC_EHIP);
+        L.kx = kx;
+        L.ky = ky;
+    }
     if ((rc = zero_state(c))) return fail(rc);
     *out = c;
     return PIC_OK;
@@ -394,6 +468,12 @@ extern "C" int pic2d3v_field(pic2d3v_ctx* c, double* Ex, double* Ey) {
     if (!c) return PIC_EINVAL;
     HIPCHK(c, hipSetDevice(c->cfg.device));
     const int64_t G = (int64_t)c->cfg.NX * c->cfg.NY;
+    if (c->lorenz) {
+        if (Ex) HIPCHK(c, hipMemcpyAsync(Ex, c->lg.EB, sizeof(double) * (size_t)G, hipMemcpyDeviceToHost, c->stream));
+        if (Ey)
+            HIPCHK(c, hipMemcpyAsync(Ey, c->lg.EB + G, sizeof(double) * (size_t)G, hipMemcpyDeviceToHost, c->stream));
+        return finish(c);
+    }
     void *a = nullptr, *b = nullptr;
     int rc;
     if ((rc = scratch(c, 1, sizeof(double) * (size_t)G, &a)) || (rc = scratch(c, 2, sizeof(double) * (size_t)G, &b)))
@@ -484,7 +564,42 @@ extern "C" int pic2d3v_sort(pic2d3v_ctx* c, int32_t s, double dx, double dy, int
 
 // Room for bins 0 .. nbins-1 of the rows and the binned fields (grown
 // geometrically, old bins kept, new ones zero).
+static int ensure_bins_lg(pic2d3v_ctx* c, int64_t nbins) {
+    LG23& L = c->lg;
+    if (nbins <= L.cap) return PIC_OK;
+    const int64_t cap = std::max<int64_t>(nbins, std::max<int64_t>(16, 2 * L.cap));
+    const size_t per = (size_t)c->d.NXd * c->d.NYd;
+    double *nr = nullptr, *nb = nullptr;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    hipError_t e1 = hipMalloc(&nr, sizeof(double) * kLGRow * (size_t)cap);
+    hipError_t e2 = hipMalloc(&nb, sizeof(double) * kLGBins * per * (size_t)cap);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        if (nr) (void)hipFree(nr);
+        if (nb) (void)hipFree(nb);
+        set_err(c, "pic2d3v_run: cannot allocate %lld field bins", (long long)cap);
+        return PIC_ENOMEM;
+    }
+    HIPCHK(c, hipMemsetAsync(nr, 0, sizeof(double) * kLGRow * (size_t)cap, c->stream));
+    HIPCHK(c, hipMemsetAsync(nb, 0, sizeof(double) * kLGBins * per * (size_t)cap, c->stream));
+    if (L.cap > 0) {
+        HIPCHK(c, hipMemcpyAsync(nr, L.rows, sizeof(double) * kLGRow * (size_t)L.cap, hipMemcpyDeviceToDevice,
+                                 c->stream));
+        for (int a = 0; a < kLGBins; ++a)
+            HIPCHK(c, hipMemcpyAsync(nb + (size_t)a * cap * per, L.bins + (size_t)a * L.cap * per,
+                                     sizeof(double) * per * (size_t)L.cap, hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (L.rows) HIPCHK(c, hipFree(L.rows));
+    if (L.bins) HIPCHK(c, hipFree(L.bins));
+    L.rows = nr;
+    L.bins = nb;
+    L.cap = cap;
+    free_graphs(c);  // the graphs hold the old pointers
+    return PIC_OK;
+}
+
 static int ensure_bins(pic2d3v_ctx* c, int64_t nbins) {
+    if (c->lorenz) return ensure_bins_lg(c, nbins);
     Diag23& d = c->d;
     if (nbins <= d.cap) return PIC_OK;
     const int64_t cap = std::max<int64_t>(nbins, std::max<int64_t>(16, 2 * d.cap));
@@ -523,6 +638,21 @@ static int ensure_bins(pic2d3v_ctx* c, int64_t nbins) {
     return PIC_OK;
 }
 
+// One Lorenz-gauge step: loop! (:875-979) and, variant 1 or 2, diagnose!
+// (variant 2 with the row).
+static void enqueue_step_lg(pic2d3v_ctx* c, int variant, hipStream_t s) {
+    const int ns = c->cfg.nspecies;
+    const LG23& L = c->lg;
+    lg_launch_solve(L, s);
+    for (int k = 0; k < ns; ++k) lg_launch_push(L, c->g, c->sp[k], k, c->xh[k], c->yh[k], s);
+    for (int k = 0; k < ns; ++k) lg_launch_deposit_rho(L, c->g, c->sp[k], c->xh[k], c->yh[k], s);
+    for (int k = 0; k < ns; ++k) lg_launch_deposit_j(L, c->g, c->sp[k], s);
+    lg_launch_finish(L, s);
+    if (variant == 2) lg_launch_diag_row(L, c->g, c->d, c->sp, ns, s);
+    if (variant >= 1) lg_launch_diag_fields(L, c->d, s);
+    launch23_tick(c->d.t, s);
+}
+
 static void enqueue_step(pic2d3v_ctx* c, bool diag, hipStream_t s) {
     for (int k = 0; k < c->cfg.nspecies; ++k) launch23_push(c->g, c->sp[k], c->E, c->b.racc, s);
     launch2d_solve(c->b, 3, nullptr, true, s);
@@ -531,10 +661,13 @@ static void enqueue_step(pic2d3v_ctx* c, bool diag, hipStream_t s) {
     launch23_tick(c->d.t, s);
 }
 
-static int capture(pic2d3v_ctx* c, bool diag, hipGraphExec_t* out) {
+// variant: electrostatic 0/1 = without/with the row; Lorenz gauge 0 = loop!
+// alone, 1/2 = with diagnose! without/with the row
+static int capture(pic2d3v_ctx* c, int variant, hipGraphExec_t* out) {
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(c, diag, c->stream);
+    if (c->lorenz) enqueue_step_lg(c, variant, c->stream);
+    else enqueue_step(c, variant != 0, c->stream);
     HIPCHK(c, hipStreamEndCapture(c->stream, &g));
     hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -562,15 +695,17 @@ extern "C" int pic2d3v_run(pic2d3v_ctx* c, int64_t nsteps, double* rows) {
     if ((rc = ensure_bins(c, cdiv(t1, nt)))) return rc;
     for (int64_t t = t0; t < t1; ++t) {
         const bool diag = t % nt == 0;
-        hipGraphExec_t& ge = diag ? c->g_diag : c->g_plain;
-        if (!ge && (rc = capture(c, diag, &ge))) return rc;
+        hipGraphExec_t& ge = c->lorenz ? c->g_lg[diag ? 2 : 1] : (diag ? c->g_diag : c->g_plain);
+        if (!ge && (rc = capture(c, c->lorenz ? (diag ? 2 : 1) : (diag ? 1 : 0), &ge))) return rc;
         HIPCHK(c, hipGraphLaunch(ge, c->stream));
     }
     if ((rc = finish(c))) return rc;
     c->step = t1;
     const int64_t r0 = cdiv(t0, nt), r1 = cdiv(t1, nt);
+    const int rl = c->lorenz ? kLGRow : 8;
+    const double* dr = c->lorenz ? c->lg.rows : c->d.rows;
     if (rows && r1 > r0)
-        HIPCHK(c, hipMemcpy(rows, c->d.rows + 8 * r0, sizeof(double) * 8 * (size_t)(r1 - r0), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(rows, dr + rl * r0, sizeof(double) * rl * (size_t)(r1 - r0), hipMemcpyDeviceToHost));
     int32_t flag = 0;
     HIPCHK(c, hipMemcpy(&flag, c->b.flag, sizeof flag, hipMemcpyDeviceToHost));
     if (flag) {
@@ -590,9 +725,124 @@ extern "C" int pic2d3v_fields(pic2d3v_ctx* c, double* Exs, double* Eys, double* 
     if (nbins == 0) return PIC_OK;
     HIPCHK(c, hipSetDevice(c->cfg.device));
     const size_t n = sizeof(double) * (size_t)c->d.NXd * c->d.NYd * (size_t)nbins;
+    if (c->lorenz) {
+        const size_t blk = (size_t)c->d.NXd * c->d.NYd * (size_t)c->lg.cap;
+        if (Exs) HIPCHK(c, hipMemcpy(Exs, c->lg.bins + PIC2D3V_LG_BIN_EX * blk, n, hipMemcpyDeviceToHost));
+        if (Eys) HIPCHK(c, hipMemcpy(Eys, c->lg.bins + PIC2D3V_LG_BIN_EY * blk, n, hipMemcpyDeviceToHost));
+        if (phis) HIPCHK(c, hipMemcpy(phis, c->lg.bins + PIC2D3V_LG_BIN_PHI * blk, n, hipMemcpyDeviceToHost));
+        return PIC_OK;
+    }
     if (Exs) HIPCHK(c, hipMemcpy(Exs, c->d.Exs, n, hipMemcpyDeviceToHost));
     if (Eys) HIPCHK(c, hipMemcpy(Eys, c->d.Eys, n, hipMemcpyDeviceToHost));
     if (phis) HIPCHK(c, hipMemcpy(phis, c->d.phis, n, hipMemcpyDeviceToHost));
+    return PIC_OK;
+}
+
+// ---- Lorenz gauge: warmup!, loop! alone, the state, the bins ----------------
+
+static int need_lorenz(pic2d3v_ctx* c, const char* who) {
+    if (!c->lorenz) {
+        set_err(c, "%s: the context's field is not PIC2D3V_LORENZ_STAGGERED", who);
+        return PIC_ESTATE;
+    }
+    return PIC_OK;
+}
+
+static int need_uploaded(pic2d3v_ctx* c, const char* who) {
+    for (int s = 0; s < c->cfg.nspecies; ++s)
+        if (!c->uploaded[s]) {
+            set_err(c, "%s: upload species %d first", who, s);
+            return PIC_ESTATE;
+        }
+    return PIC_OK;
+}
+
+extern "C" int pic2d3v_warmup(pic2d3v_ctx* c) {
+    if (!c) return PIC_EINVAL;
+    if (!c->lorenz) return PIC_OK;  // warmup!(::AbstractField) = field
+    if (int rc = need_uploaded(c, "pic2d3v_warmup")) return rc;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const int ns = c->cfg.nspecies;
+    const double h0 = -c->cfg.dt / 2.0, h1 = c->cfg.dt / 2.0;
+    for (int k = 0; k < ns; ++k) lg_launch_advect(c->lg, c->g, c->sp[k], h0, c->stream);
+    for (int k = 0; k < ns; ++k) lg_launch_advect(c->lg, c->g, c->sp[k], h1, c->stream);
+    for (int k = 0; k < ns; ++k) lg_launch_vmax(c->lg, c->sp[k], k, k == 0, c->stream);
+    for (int k = 0; k < ns; ++k) lg_launch_deposit_j(c->lg, c->g, c->sp[k], c->stream);
+    lg_launch_warm_finish(c->lg, c->stream);
+    return finish(c);
+}
+
+extern "C" int pic2d3v_loop(pic2d3v_ctx* c, int64_t nsteps) {
+    if (!c) return PIC_EINVAL;
+    if (int rc = need_lorenz(c, "pic2d3v_loop")) return rc;
+    if (nsteps < 0) {
+        set_err(c, "pic2d3v_loop: nsteps must be >= 0");
+        return PIC_EINVAL;
+    }
+    if (int rc = need_uploaded(c, "pic2d3v_loop")) return rc;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    int rc;
+    if (!c->g_lg[0] && (rc = capture(c, 0, &c->g_lg[0]))) return rc;
+    for (int64_t t = 0; t < nsteps; ++t) HIPCHK(c, hipGraphLaunch(c->g_lg[0], c->stream));
+    if ((rc = finish(c))) return rc;
+    c->step += nsteps;
+    return PIC_OK;
+}
+
+static int check_slot(pic2d3v_ctx* c, const char* who, int32_t which, int32_t n) {
+    if (which < 0 || which >= n) {
+        set_err(c, "%s: no slot %d", who, (int)which);
+        return PIC_EINVAL;
+    }
+    return PIC_OK;
+}
+
+extern "C" int pic2d3v_lg_get(pic2d3v_ctx* c, int32_t which, double* z) {
+    if (!c) return PIC_EINVAL;
+    if (int rc = need_lorenz(c, "pic2d3v_lg_get")) return rc;
+    if (int rc = check_slot(c, "pic2d3v_lg_get", which, PIC2D3V_LG_NSTATE)) return rc;
+    if (!z) {
+        set_err(c, "pic2d3v_lg_get: z must not be NULL");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const size_t G = (size_t)c->cfg.NX * c->cfg.NY;
+    HIPCHK(c, hipMemcpyAsync(z, c->lg.F + which * G, sizeof(double2) * G, hipMemcpyDeviceToHost, c->stream));
+    return finish(c);
+}
+
+extern "C" int pic2d3v_lg_set(pic2d3v_ctx* c, int32_t which, const double* z) {
+    if (!c) return PIC_EINVAL;
+    if (int rc = need_lorenz(c, "pic2d3v_lg_set")) return rc;
+    if (int rc = check_slot(c, "pic2d3v_lg_set", which, PIC2D3V_LG_NSTATE)) return rc;
+    if (!z) {
+        set_err(c, "pic2d3v_lg_set: z must not be NULL");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const size_t G = (size_t)c->cfg.NX * c->cfg.NY;
+    HIPCHK(c, hipMemcpyAsync(c->lg.F + which * G, z, sizeof(double2) * G, hipMemcpyHostToDevice, c->stream));
+    return finish(c);
+}
+
+extern "C" int pic2d3v_lg_bins(pic2d3v_ctx* c, int32_t which, double* out, int64_t nbins) {
+    if (!c) return PIC_EINVAL;
+    if (int rc = need_lorenz(c, "pic2d3v_lg_bins")) return rc;
+    if (int rc = check_slot(c, "pic2d3v_lg_bins", which, PIC2D3V_LG_NBINS)) return rc;
+    const int64_t have = std::min<int64_t>(cdiv(c->step, c->cfg.ntskip), c->lg.cap);
+    if (nbins < 0 || nbins > have) {
+        set_err(c, "pic2d3v_lg_bins: nbins must be in [0, %lld]", (long long)have);
+        return PIC_EINVAL;
+    }
+    if (nbins == 0) return PIC_OK;
+    if (!out) {
+        set_err(c, "pic2d3v_lg_bins: out must not be NULL");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const size_t per = (size_t)c->d.NXd * c->d.NYd;
+    HIPCHK(c, hipMemcpy(out, c->lg.bins + (size_t)which * c->lg.cap * per, sizeof(double) * per * (size_t)nbins,
+                        hipMemcpyDeviceToHost));
     return PIC_OK;
 }
 

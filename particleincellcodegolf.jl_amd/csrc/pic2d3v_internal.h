@@ -52,6 +52,8 @@ void launch23_update(const Grid23& g, const Diag23& d, const Buffers2D& b, doubl
                      hipStream_t st);
 // the diagnostics row of bin t/ntskip
 void launch23_diag(const Grid23& g, const Diag23& d, const double2* E, const Species23* sp, int ns, hipStream_t st);
+// the species sums of the diagnostics: kinetic and momentum chunk partials
+void launch23_diag_species(const Species23& s, double* kpart, double* mpart, hipStream_t st);
 // t += 1
 void launch23_tick(int64_t* t, hipStream_t st);
 // xyv (5 x P, column-major) <-> SoA

@@ -1,0 +1,65 @@
+// pic2d3v_lg_internal.h -- descriptors and launchers of the Lorenz-gauge
+// electromagnetic path of the structured 2D3V module (LorenzGaugeStaggeredField
+// of src/PIC2D3V.jl), shared by the host runtime (capi2d3v.cpp) and the
+// kernels (kernels_2d3v_lg.hip).
+#pragma once
+#include "pic2d3v_internal.h"
+
+namespace pic {
+
+// Field state slots (complex NX x NY arrays), in the order of include/pic2d3v.h.
+enum LGSlot {
+    kLGPhi0 = 0, kLGPhiM, kLGAx0, kLGAy0, kLGAz0, kLGAxM, kLGAyM, kLGAzM,
+    kLGRho0, kLGJx0, kLGJy0, kLGJz0, kLGEx, kLGEy, kLGEz, kLGBx, kLGBy, kLGBz, kLGSlots
+};
+constexpr int kLGBins = 14;      // binned fields of LorenzGaugeDiagnostics
+constexpr int kLGRow = 11;       // doubles per diagnostics row
+constexpr int kLGLinePoints = 4096;  // transform points per block (whole lines)
+
+struct LG23 {
+    int32_t NX, NY, lx, ly;
+    double dt, theta;
+    double B0[3];
+    double2* F;          // kLGSlots x G: the field state
+    double2* plus;       // 4 x G: phi+, Ax+, Ay+, Az+
+    double* EB;          // 6 x G: the interior of EBxyz (0.0 + real(F), + B0 for B)
+    const double* kx;    // NX wavenumbers (FFTHelper)
+    const double* ky;    // NY
+    unsigned long long* racc;  // 4 grids (rho, Jx, Jy, Jz) x 3 limbs x G
+    unsigned long long* umax;  // kMaxSpecies23 x 3: max |v_c| per species (bits)
+    int32_t ns;
+    int32_t qr;          // fraction bits of the exact charge sums
+    int64_t P[kMaxSpecies23];
+    double qw[kMaxSpecies23];
+    const double2* tw;
+    double* bins;        // kLGBins x cap x NXd x NYd
+    double* rows;        // cap x kLGRow
+    int64_t cap;
+    double* fpart;       // chunk partials of the field energy
+    double* mfpart;      // 3 x chunk partials of the field momentum
+};
+
+// step 1-2 of loop!: the wave update, E and B in the spectrum, their inverse
+// transforms and update!
+void lg_launch_solve(const LG23& L, hipStream_t st);
+// step 3: gather, Boris, half move (xh, yh), half move; per-species maxima
+void lg_launch_push(const LG23& L, const Grid23& g, const Species23& s, int sidx, double* xh, double* yh,
+                    hipStream_t st);
+// the charge at (xh, yh) and the currents at (x, y)
+void lg_launch_deposit_rho(const LG23& L, const Grid23& g, const Species23& s, const double* xh, const double* yh,
+                           hipStream_t st);
+void lg_launch_deposit_j(const LG23& L, const Grid23& g, const Species23& s, hipStream_t st);
+// the reduction into rho0, J0 and the potentials' buffer copies (step 4)
+void lg_launch_finish(const LG23& L, hipStream_t st);
+// warmup!: advect by h, the velocity maxima, and the reduction (rho0 = 0)
+void lg_launch_advect(const LG23& L, const Grid23& g, const Species23& s, double h, hipStream_t st);
+void lg_launch_vmax(const LG23& L, const Species23& s, int sidx, bool clear, hipStream_t st);
+void lg_launch_warm_finish(const LG23& L, hipStream_t st);
+// diagnose!: the row (when t % ntskip == 0), the in-place inverse transforms
+// and the binned fields
+void lg_launch_diag_row(const LG23& L, const Grid23& g, const Diag23& d, const Species23* sp, int ns,
+                        hipStream_t st);
+void lg_launch_diag_fields(const LG23& L, const Diag23& d, hipStream_t st);
+int configure_kernels_2d3v_lg();
+
+}  // namespace pic

@@ -39,7 +39,7 @@ export Context, PicConfig, deposit!, fieldsolve!, drift!, midpoint_position!, wr
        diagnostics, upload!, download, run!, quiet_start, autotune!, set_tiling!, tiling, PicTuning,
        simpson_kick!, Pic1D2VConfig, Context1D2V, fields,
        Pic2DConfig, Context2D, gather, boris!,
-       Pic2D3VSpecies, Pic2D3VConfig, Context2D3V, reset!, field,
+       Pic2D3VSpecies, Pic2D3VConfig, Context2D3V, reset!, field, warmup!, loop!, lg_field, lg_field!, lg_bins,
        PicEM1DConfig, ContextEM1D, explicit_fields!, explicit_push
 
 const libpicgolf = get(ENV, "PICGOLF_LIB", normpath(joinpath(@__DIR__, "..", "lib", "libpicgolf.so")))
@@ -691,7 +691,11 @@ const SHAPES2D3V = Dict(:ngp => Int32(0), :area => Int32(1), :bspline0 => Int32(
                         :bspline2 => Int32(4), :bspline3 => Int32(5), :bspline4 => Int32(6), :bspline5 => Int32(7))
 const FIELD_UPDATES = Dict(:accumulate => Int32(0), :replace => Int32(1))
 const BORIS_VARIANTS = Dict(:reference => Int32(0), :textbook => Int32(1))
+const FIELDS2D3V = Dict(:electrostatic => Int32(0), :lorenz => Int32(1))
 const MAX_SPECIES_2D3V = 4
+# the Lorenz-gauge state slots and binned fields of include/pic2d3v.h
+const LG_STATE = (:ϕ⁰, :ϕ⁻, :Ax⁰, :Ay⁰, :Az⁰, :Ax⁻, :Ay⁻, :Az⁻, :ρ⁰, :Jx⁰, :Jy⁰, :Jz⁰, :Ex, :Ey, :Ez, :Bx, :By, :Bz)
+const LG_BINS = (:Exs, :Eys, :Ezs, :Bxs, :Bys, :Bzs, :Axs, :Ays, :Azs, :ϕs, :ρs, :Jxs, :Jys, :Jzs)
 
 "Mirror of `struct pic2d3v_species` in include/pic2d3v.h."
 struct Pic2D3VSpecies
@@ -727,11 +731,13 @@ struct Pic2D3VConfig
     nspecies::Int32
     species::NTuple{4,Pic2D3VSpecies}
     device::Int32
+    field::Int32
+    theta::Float64
 end
 
 """
     Pic2D3VConfig(species; NX, NY=NX, Lx=1, Ly=1, dt, B0x=0, B0y=0, B0z=0, buffer=3, ntskip=1, ngskip=1,
-                  field_update=:accumulate, boris=:reference, device=0)
+                  field_update=:accumulate, boris=:reference, device=0, field=:electrostatic, theta=1)
 
 The `ElectrostaticField(NX, NY, Lx, Ly; dt, B0x, B0y, B0z, buffer)` and
 `ElectrostaticDiagnostics(..., ntskip, ngskip)` of a plasma of up to four
@@ -739,16 +745,21 @@ species.  `field_update=:accumulate` is the reference's `update!`, which adds
 every step's field into `Exy` without clearing it; `:replace` gathers the
 step's own field.  `boris=:reference` is `ElectrostaticBoris` (rotation scaled
 by q_m^2); `:textbook` is `ElectromagneticBoris`'s form with Ez = 0.
+`field=:lorenz` builds `LorenzGaugeStaggeredField(NX, NY, Lx, Ly; dt, B0x, B0y,
+B0z, imex=ImEx(theta), buffer)` and `LorenzGaugeDiagnostics` instead
+(`theta=0` is `Explicit()`, `1` is `Implicit()`; src/2D3V.jl uses `ImEx(1)`).
 """
 function Pic2D3VConfig(species::Vector{Pic2D3VSpecies}; NX, NY=NX, Lx=1.0, Ly=1.0, dt, B0x=0.0, B0y=0.0, B0z=0.0,
-                       buffer=3, ntskip=1, ngskip=1, field_update=:accumulate, boris=:reference, device=0)
+                       buffer=3, ntskip=1, ngskip=1, field_update=:accumulate, boris=:reference, device=0,
+                       field=:electrostatic, theta=1.0)
     ns = length(species)
     1 <= ns <= MAX_SPECIES_2D3V || error("Pic2D3VConfig: 1 to $MAX_SPECIES_2D3V species")
     none = Pic2D3VSpecies(0, 0, 0.0, 0.0, 0.0)
     sp = ntuple(k -> k <= ns ? species[k] : none, 4)
     Pic2D3VConfig(Int32(NX), Int32(NY), Float64(Lx), Float64(Ly), Float64(dt),
                   (Float64(B0x), Float64(B0y), Float64(B0z)), Int32(buffer), Int32(ntskip), Int32(ngskip),
-                  FIELD_UPDATES[field_update], BORIS_VARIANTS[boris], Int32(ns), sp, Int32(device))
+                  FIELD_UPDATES[field_update], BORIS_VARIANTS[boris], Int32(ns), sp, Int32(device),
+                  FIELDS2D3V[field], Float64(theta))
 end
 
 mutable struct Context2D3V
@@ -824,12 +835,13 @@ end
 
 `nsteps` of `loop!(plasma, field, ...); diagnose!(d, field, plasma, t, ...)`;
 one row per `t % ntskip == 0` of this call: kineticenergy, fieldenergy,
-particlemomentum (3), characteristicmomentum (3).
+particlemomentum (3), characteristicmomentum (3); the Lorenz gauge adds
+fieldmomentum (3) before characteristicmomentum.
 """
 function run!(ctx::Context2D3V, nsteps::Integer)
     nt = ctx.cfg.ntskip
     rows = cld(ctx.t + nsteps, nt) - cld(ctx.t, nt)
-    R = zeros(8, rows)
+    R = zeros(ctx.cfg.field == FIELDS2D3V[:lorenz] ? 11 : 8, rows)
     rc = GC.@preserve R ccall((:pic2d3v_run, libpicgolf), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}),
                               ctx.ptr, nsteps, rows > 0 ? pointer(R) : C_NULL)
     ctx.t += nsteps
@@ -891,6 +903,53 @@ function boris!(ctx::Context2D3V, vx::Vector{Float64}, vy::Vector{Float64}, vz::
                                            ctx.ptr, vx, vy, vz, Ex, Ey, length(vx), q_m)
     check(ctx, rc, "pic2d3v_boris")
     (vx, vy, vz)
+end
+
+"`warmup!(ctx::Context2D3V)`: `warmup!(field, plasma, to)` (nothing for an electrostatic field)."
+function warmup!(ctx::Context2D3V)
+    check(ctx, ccall((:pic2d3v_warmup, libpicgolf), Cint, (Ptr{Cvoid},), ctx.ptr), "pic2d3v_warmup")
+    ctx
+end
+
+"`loop!(ctx::Context2D3V, nsteps)`: `nsteps` of the Lorenz-gauge `loop!` without `diagnose!`."
+function loop!(ctx::Context2D3V, nsteps::Integer)
+    rc = ccall((:pic2d3v_loop, libpicgolf), Cint, (Ptr{Cvoid}, Int64), ctx.ptr, nsteps)
+    check(ctx, rc, "pic2d3v_loop")
+    ctx.t += nsteps
+    ctx
+end
+
+"`lg_field(ctx, which) -> Matrix{ComplexF64}`: the Lorenz-gauge field array `which` (a symbol of `LG_STATE`)."
+function lg_field(ctx::Context2D3V, which::Symbol)
+    z = zeros(ComplexF64, ctx.cfg.NX, ctx.cfg.NY)
+    k = Int32(findfirst(==(which), LG_STATE) - 1)
+    rc = GC.@preserve z ccall((:pic2d3v_lg_get, libpicgolf), Cint, (Ptr{Cvoid}, Int32, Ptr{ComplexF64}),
+                              ctx.ptr, k, z)
+    check(ctx, rc, "pic2d3v_lg_get")
+    z
+end
+
+"`lg_field!(ctx, which, z)`: sets the Lorenz-gauge field array `which`."
+function lg_field!(ctx::Context2D3V, which::Symbol, z::Matrix{ComplexF64})
+    k = Int32(findfirst(==(which), LG_STATE) - 1)
+    rc = GC.@preserve z ccall((:pic2d3v_lg_set, libpicgolf), Cint, (Ptr{Cvoid}, Int32, Ptr{ComplexF64}),
+                              ctx.ptr, k, z)
+    check(ctx, rc, "pic2d3v_lg_set")
+    ctx
+end
+
+"`lg_bins(ctx, which) -> Array{Float64,3}`: `LorenzGaugeDiagnostics`' binned field `which` (e.g. `:Ezs`)."
+function lg_bins(ctx::Context2D3V, which::Symbol)
+    c = ctx.cfg
+    nb = cld(ctx.t, c.ntskip)
+    A = zeros(c.NX ÷ c.ngskip, c.NY ÷ c.ngskip, nb)
+    k = Int32(findfirst(==(which), LG_BINS) - 1)
+    if nb > 0
+        rc = GC.@preserve A ccall((:pic2d3v_lg_bins, libpicgolf), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}, Int64),
+                                  ctx.ptr, k, A, nb)
+        check(ctx, rc, "pic2d3v_lg_bins")
+    end
+    A
 end
 
 "`fieldsolve!(Ex, Ey, ctx, ρ; ϕ=nothing)`: the electrostatic solve of `loop!` in the context's box."

@@ -1,5 +1,6 @@
-"""Python mirror of the electrostatic path of the structured 2D3V module
-(src/PIC2D3V.jl) on top of the C ABI in include/pic2d3v.h.
+"""Python mirror of the structured 2D3V module (src/PIC2D3V.jl), its
+electrostatic and Lorenz-gauge paths, on top of the C ABI in
+include/pic2d3v.h.
 
     Species(P, ...; charge, mass) / xyv    src/PIC2D3V.jl:148-208   -> SpeciesConfig, upload, download
     sort!(species, dx, dy)                 :210-215                 -> sort
@@ -11,6 +12,12 @@
                                            :530-581, :1301-1330     -> run
     d.Exs, d.Eys, d.phis                   :1323-1327               -> fields
     deposit!, (field)(shape, x, y), boris  :1249, :1217, :49        -> deposit, gather, boris
+    LorenzGaugeStaggeredField(NX, NY, Lx, Ly; dt, B0x, B0y, B0z, imex, buffer)
+                                           :347-392                 -> Config2D3V(field="lorenz", theta)
+    warmup!(field, plasma, to)             :826-859                 -> warmup
+    loop! alone (no diagnose!)             :875-979                 -> loop
+    the field's phi, A, rho, J, E, B       :347-378                 -> lg_get, lg_set
+    LorenzGaugeDiagnostics' Exs .. Jzs     :104-128, :1370-1386     -> lg_bins
 
 Particles of one species are the reference's xyv: a (5, P) array with rows
 x, y, vx, vy, vz (Julia's 5 x P matrix; the C side takes it column-major).
@@ -28,6 +35,7 @@ from . import _lib as L
 SHAPES = {"ngp": L.NGP_2D3V, "area": L.AREA_2D3V, **{f"bspline{n}": L.BSPLINE0_2D3V + n for n in range(6)}}
 FIELD_UPDATES = {"accumulate": L.FIELD_ACCUMULATE, "replace": L.FIELD_REPLACE}
 BORIS = {"reference": L.BORIS_REFERENCE, "textbook": L.BORIS_TEXTBOOK}
+FIELDS = {"electrostatic": L.ELECTROSTATIC_2D3V, "lorenz": L.LORENZ_STAGGERED_2D3V}
 
 
 def _code(table, v):
@@ -50,7 +58,9 @@ class Config2D3V:
     dt, ghost layers `buffer`), the diagnostics' ntskip / ngskip, and the
     plasma's species.  field_update "accumulate" (the reference's update!,
     which never clears Exy) or "replace"; boris "reference"
-    (ElectrostaticBoris) or "textbook" (ElectromagneticBoris with Ez = 0)."""
+    (ElectrostaticBoris) or "textbook" (ElectromagneticBoris with Ez = 0).
+    field "lorenz" builds a LorenzGaugeStaggeredField instead, with the ImEx
+    theta (0: Explicit(), 1: Implicit(); src/2D3V.jl uses ImEx(1))."""
     NX: int
     NY: int
     dt: float
@@ -64,6 +74,8 @@ class Config2D3V:
     field_update: str = "accumulate"
     boris: str = "reference"
     device: int = 0
+    field: str = "electrostatic"
+    theta: float = 1.0
 
     def to_c(self):
         c = L.pic2d3v_config()
@@ -81,6 +93,8 @@ class Config2D3V:
             c.species[s].charge, c.species[s].mass = float(sp.charge), float(sp.mass)
             c.species[s].weight = float(sp.weight)
         c.device = int(self.device)
+        c.field = _code(FIELDS, self.field)
+        c.theta = float(self.theta)
         return c
 
 
@@ -143,11 +157,12 @@ class Context2D3V:
     def run(self, nsteps):
         """nsteps of loop! + diagnose!; returns the rows of this call (one per t
         with t % ntskip == 0): kinetic energy, field energy, momentum (3),
-        characteristic momentum (3)."""
+        characteristic momentum (3); the Lorenz gauge adds the field momentum
+        (3) before the characteristic momentum."""
         nt = self.cfg.ntskip
         t0, t1 = self.t, self.t + int(nsteps)
         n = -(-t1 // nt) - (-(-t0 // nt))
-        rows = np.zeros((max(n, 0), 8))
+        rows = np.zeros((max(n, 0), 11 if self.cfg.field == "lorenz" else 8))
         rc = self._lib.pic2d3v_run(self._h, int(nsteps), L.ptr(rows))
         self.t = t1
         self._check(rc, "pic2d3v_run")
@@ -167,6 +182,47 @@ class Context2D3V:
         Ex, Ey = np.zeros((self.cfg.NY, self.cfg.NX)), np.zeros((self.cfg.NY, self.cfg.NX))
         self._check(self._lib.pic2d3v_field(self._h, L.ptr(Ex), L.ptr(Ey)), "pic2d3v_field")
         return Ex, Ey
+
+    # ---- Lorenz gauge --------------------------------------------------------
+    def warmup(self):
+        """warmup!(field, plasma) of the staggered Lorenz-gauge field."""
+        self._check(self._lib.pic2d3v_warmup(self._h), "pic2d3v_warmup")
+
+    def loop(self, nsteps):
+        """nsteps of loop! without diagnose!."""
+        rc = self._lib.pic2d3v_loop(self._h, int(nsteps))
+        self._check(rc, "pic2d3v_loop")
+        self.t += int(nsteps)
+
+    def _slot(self, which, table):
+        return table.index(which) if isinstance(which, str) else int(which)
+
+    def lg_get(self, which):
+        """The state slot `which` (a name of LG_STATE or its index) as a
+        complex (NY, NX) array."""
+        z = np.zeros((self.cfg.NY, self.cfg.NX), dtype=np.complex128)
+        self._check(self._lib.pic2d3v_lg_get(self._h, self._slot(which, L.LG_STATE), L.ptr(z)), "pic2d3v_lg_get")
+        return z
+
+    def lg_state(self):
+        """All 18 slots as a complex (18, NY, NX) array."""
+        return np.stack([self.lg_get(k) for k in range(len(L.LG_STATE))])
+
+    def lg_set(self, which, z):
+        z = np.ascontiguousarray(np.asarray(z, dtype=np.complex128).reshape(self.cfg.NY, self.cfg.NX))
+        self._check(self._lib.pic2d3v_lg_set(self._h, self._slot(which, L.LG_STATE), L.ptr(z)), "pic2d3v_lg_set")
+
+    def lg_bins(self, which=None, nbins=None):
+        """The binned field `which` (a name of LG_BINS or its index), shape
+        (nbins, NY/ngskip, NX/ngskip); all fourteen when which is None."""
+        if nbins is None:
+            nbins = -(-self.t // self.cfg.ntskip)
+        if which is None:
+            return np.stack([self.lg_bins(k, nbins) for k in range(len(L.LG_BINS))])
+        out = np.zeros((int(nbins), self.cfg.NY // self.cfg.ngskip, self.cfg.NX // self.cfg.ngskip))
+        self._check(self._lib.pic2d3v_lg_bins(self._h, self._slot(which, L.LG_BINS), L.ptr(out), int(nbins)),
+                    "pic2d3v_lg_bins")
+        return out
 
     # ---- operators ---------------------------------------------------------
     def deposit(self, shape, x, y, w):
