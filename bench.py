@@ -36,7 +36,7 @@ def parse():
                     help="skip the ensemble and 2D3V workloads reported beside the headline")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--workload", default="scaled",
-                    choices=["scaled", "ensemble", "es2d3v", "pic2d3v", "em1d", "em1d_explicit"],
+                    choices=["scaled", "ensemble", "es2d3v", "pic2d3v", "lorenz2d3v", "em1d", "em1d_explicit"],
                     help="scaled: the headline line (default); ensemble / es2d3v: only that workload, "
                          "--steps timed steps, one JSON line (for profiling)")
     ap.add_argument("--script-steps", type=int, default=4048,
@@ -306,6 +306,80 @@ def bench_pic2d3v(steps=20, warmup=2, N=1024, ppc=16, shape="bspline2"):
                              f"and solve on one"}}
 
 
+def driver_2d3v_parameters(N=1024):
+    """The normalised parameters src/2D3V.jl:12-70 derives (n = 1e16, B = 2.1 T,
+    Te = 10 keV, kresolution = 5, L0 = 1): dt = 1/(8N), B0, n0, vth."""
+    import math
+    c, me, e = 299792458.0, 9.1093837e-31, 1.60217663e-19
+    mu0 = 4.0e-7 * math.pi
+    eps0 = 1.0 / mu0 / c / c
+    n, B, M, TeeV = 1e16, 2.1, 2 * 1836, 1e4
+    Va = B / math.sqrt(mu0 * M * me * n)
+    Oi0 = e * B / M / me
+    L = Va / Oi0 * 2 * math.pi * 5
+    T = L / c
+    phi_s = me * (L / T) ** 2 / e
+    n_s = phi_s * eps0 / L ** 2 / e
+    A_s = T * phi_s / L
+    return {"dt": L / N / c / T / 8, "B0": B / (A_s / L), "n0": n / n_s, "vth": math.sqrt(TeeV * e * 2 / me) / L * T}
+
+
+def bench_lorenz2d3v(steps=10, warmup=2, N=1024, ppc=16, shape="bspline2"):
+    """The structured 2D3V module's driver whole (src/2D3V.jl:50-127):
+    LorenzGaugeStaggeredField(1024, 1024; ImEx(1), buffer = 10, B0 along x),
+    electrons and ions of mass 32 with 16 particles per cell each, quadratic
+    B-splines, sorted per cell once, warmup!, then loop! and diagnose! every
+    step (ntskip = ngskip = 1).  FP64.  Particles are uniform random with
+    Gaussian velocities of the driver's vth (not its Halton layout)."""
+    import numpy as np
+    import picgolf
+    from picgolf.core2d3v import Config2D3V, SpeciesConfig
+    q = driver_2d3v_parameters(N)
+    rng = np.random.default_rng(0)
+    P = N * N * ppc
+    sps, xyvs = [], []
+    for ch, m in ((-1.0, 1.0), (1.0, 32.0)):
+        vth = q["vth"] / np.sqrt(m)
+        sps.append(SpeciesConfig(P=P, shape=shape, charge=ch, mass=m, weight=q["n0"] / P))
+        xyvs.append(np.vstack([rng.random(P), rng.random(P), vth / np.sqrt(2) * rng.standard_normal((3, P))]))
+    cfg = Config2D3V(NX=N, NY=N, dt=q["dt"], species=sps, B0=(q["B0"], 0.0, 0.0), buffer=10, ntskip=1, ngskip=1,
+                     field="lorenz", theta=1.0)
+    with picgolf.Context2D3V(cfg) as c:
+        for k, xyv in enumerate(xyvs):
+            c.upload(k, xyv)
+            c.sort(k, 1.0 / N, 1.0 / N)
+        c.warmup()
+        c.run(warmup)
+        t0 = time.perf_counter()
+        rows = c.run(steps)
+        secs = time.perf_counter() - t0
+    # the CPU restatement on a sample: the same grid and transforms, 1/32 of
+    # each species (sorted), warmup! and one step
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    n = P // 32
+    sample = []
+    for sp in sps:
+        xyv = np.vstack([rng.random(n), rng.random(n), q["vth"] / np.sqrt(2 * sp.mass) *
+                         rng.standard_normal((3, n))])
+        xyv, _ = oracle.sort23(xyv, 1.0 / N, 1.0 / N)
+        sample.append(dict(shape=shape, charge=sp.charge, mass=sp.mass, weight=q["n0"] / n, xyv=xyv))
+    sample, F = oracle.lg_warmup(N, N, 1.0, 1.0, cfg.dt, sample)
+    t0 = time.perf_counter()
+    oracle.lg_run(N, N, 1.0, 1.0, cfg.dt, np.array(cfg.B0), sample, F, 1, buffer=10)
+    cpu_secs = time.perf_counter() - t0
+    return {"workload": f"structured 2D3V Lorenz-gauge driver (src/2D3V.jl): {N}x{N} cells, "
+                        f"LorenzGaugeStaggeredField ImEx(1), buffer 10, 2 species x {P} particles ({ppc} per cell "
+                        f"each), {shape}, ions of mass 32, B0 along x, dt=1/(8N), diagnose! every step, FP64, "
+                        f"particles sorted per cell once",
+            "pushes_per_s": 2 * P * steps / secs, "ms_per_step": secs / steps * 1e3, "steps": steps,
+            "warmup": warmup, "kinetic_energy_first_last": [float(rows[0, 0]), float(rows[-1, 0])],
+            "field_energy_first_last": [float(rows[0, 1]), float(rows[-1, 1])],
+            "cpu_baseline": {"value": 2 * n / cpu_secs, "unit": "particle-pushes/s", "cores": 1, "kind": "port",
+                             "sample": f"FP64 CPU restatement, the same {N}x{N} grid and transforms, 2 species x "
+                             f"{n} particles (1/32), one step with diagnose!, one thread"}}
+
+
 def em1d_setup(N, P, scheme, seed=0):
     """The scripts' set-up: eps = 1/10, dt = eps^2/N/2, W = 400, w = W/P*N,
     x = rand(P); the fixed point's v = (sin a/4, cos a/4, 0) with a = 2pi*rand,
@@ -415,6 +489,9 @@ def main():
     if a.workload == "pic2d3v":
         print(json.dumps(bench_pic2d3v(steps=a.steps, warmup=a.warmup)))
         return
+    if a.workload == "lorenz2d3v":
+        print(json.dumps(bench_lorenz2d3v(steps=a.steps, warmup=a.warmup)))
+        return
     if a.workload in ("em1d", "em1d_explicit"):
         scheme = "explicit" if a.workload == "em1d_explicit" else "fixed_point"
         print(json.dumps(bench_em1d(T=a.script_steps, scaled_steps=a.steps, scheme=scheme)))
@@ -475,7 +552,8 @@ def main():
         out["momentum_drift_mixed"] = float(np.max(np.abs(m["D"][:, 3] - m["D"][0, 3])))
     if not a.no_extra:
         out["workloads"] = {"ensemble_512": bench_ensemble(autotune=not a.no_autotune), "es2d3v_256": bench_2d(),
-                            "pic2d3v_1024": bench_pic2d3v(), "em1d": bench_em1d(),
+                            "pic2d3v_1024": bench_pic2d3v(), "lorenz2d3v_1024": bench_lorenz2d3v(),
+                            "em1d": bench_em1d(),
                             "em1d_explicit": bench_em1d(scheme="explicit")}
         print(f"[bench] extra workloads: {json.dumps(out['workloads'])}", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:

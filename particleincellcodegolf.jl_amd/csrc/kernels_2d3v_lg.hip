@@ -403,11 +403,15 @@ k_lg_deposit(LG23 L, Grid23 g, Species23 s, const double* __restrict__ X, const 
     const int NX = g.NX, NY = g.NY;
     const int64_t G = (int64_t)NX * NY;
     const int64_t c0 = (int64_t)blockIdx.x * kTile23Chunk;
+    // Lane l of wave w takes particles l*16 + w*4 + k: the particles are
+    // sorted per cell, so in each iteration a wave's lanes sit in different
+    // cells and few of them add to the same LDS word at once.
+    const int64_t pb = c0 + (int64_t)(t & 63) * (kTile23Threads / 64 * kTile23PPT) + (t >> 6) * kTile23PPT;
     double xs[kTile23PPT], ys[kTile23PPT];
     int nv = 0;
 #pragma unroll
     for (int k = 0; k < kTile23PPT; ++k) {
-        const int64_t p = c0 + t + (int64_t)k * kTile23Threads;
+        const int64_t p = pb + k;
         xs[k] = 0.0;
         ys[k] = 0.0;
         if (p < s.P) {
@@ -429,7 +433,7 @@ k_lg_deposit(LG23 L, Grid23 g, Species23 s, const double* __restrict__ X, const 
     constexpr u64 M = (1ull << kLimbS) - 1;
 #pragma unroll 1
     for (int k = 0; k < kTile23PPT; ++k) {
-        const int64_t p = c0 + t + (int64_t)k * kTile23Threads;
+        const int64_t p = pb + k;
         if (p >= s.P) break;
         double w[NG];
         if (NG == 1) {
