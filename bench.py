@@ -343,7 +343,7 @@ def bench_lorenz2d3v(steps=10, warmup=2, N=1024, ppc=16, shape="bspline2"):
         sps.append(SpeciesConfig(P=P, shape=shape, charge=ch, mass=m, weight=q["n0"] / P))
         xyvs.append(np.vstack([rng.random(P), rng.random(P), vth / np.sqrt(2) * rng.standard_normal((3, P))]))
     cfg = Config2D3V(NX=N, NY=N, dt=q["dt"], species=sps, B0=(q["B0"], 0.0, 0.0), buffer=10, ntskip=1, ngskip=1,
-                     field="lorenz", theta=1.0)
+                     field="lorenz_staggered", theta=1.0)
     with picgolf.Context2D3V(cfg) as c:
         for k, xyv in enumerate(xyvs):
             c.upload(k, xyv)

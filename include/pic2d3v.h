@@ -16,7 +16,9 @@
  *                                 update!                      :530-581, :294-297
  *   diagnose!(ElectrostaticDiagnostics, ...)                   :1301-1330
  * and, for a LorenzGaugeStaggeredField (field = PIC2D3V_LORENZ_STAGGERED, as
- * src/2D3V.jl:102-106 builds it):
+ * src/2D3V.jl:102-106 builds it) or a LorenzGaugeField (PIC2D3V_LORENZ, the
+ * centred loop! of :665-745, run with pic2d3v_loop only: the reference's
+ * diagnose! reads field.rho0, which that type lacks):
  *   warmup!(field, plasma)        advect -dt/2, +dt/2, the currents   :826-859
  *   lorenzgauge!(imex, ...)       per-mode wave update, any ImEx theta :610-635
  *   E = -grad phi+ - dA/dt, B = curl (A+ + A0)/2, ifft, update!     :917-935
@@ -92,8 +94,12 @@ extern "C" {
 
 #define PIC2D3V_ELECTROSTATIC 0    /* ElectrostaticField (:270-297)          */
 #define PIC2D3V_LORENZ_STAGGERED 1 /* LorenzGaugeStaggeredField (:347-392)   */
+#define PIC2D3V_LORENZ 2           /* LorenzGaugeField (:307-345)            */
 
-/* Lorenz-gauge field state: complex NX x NY arrays (pic2d3v_lg_get/set) */
+/* Lorenz-gauge field state: complex NX x NY arrays (pic2d3v_lg_get/set).
+ * For PIC2D3V_LORENZ the slots hold that type's arrays: PHI0 = phi+ (the
+ * latest), PHIM = phi0 (the one before), AX0.. = A+, AXM.. = A0,
+ * RHO0.. = rho+, J+ (in the spectrum, as loop! leaves them). */
 #define PIC2D3V_LG_PHI0 0
 #define PIC2D3V_LG_PHIM 1  /* phi at the previous step */
 #define PIC2D3V_LG_AX0 2
@@ -151,7 +157,7 @@ typedef struct pic2d3v_config {
     int32_t nspecies;      /* 1 .. PIC2D3V_MAX_SPECIES                               */
     pic2d3v_species species[PIC2D3V_MAX_SPECIES];
     int32_t device;        /* HIP device ordinal                                     */
-    int32_t field;         /* PIC2D3V_ELECTROSTATIC or PIC2D3V_LORENZ_STAGGERED       */
+    int32_t field;         /* PIC2D3V_ELECTROSTATIC, _LORENZ_STAGGERED or _LORENZ     */
     double theta;          /* Lorenz gauge: ImEx theta (0 Explicit(), 1 Implicit())   */
 } pic2d3v_config;
 
@@ -193,7 +199,9 @@ int pic2d3v_sort(pic2d3v_ctx* ctx, int32_t species, double dx, double dy, int64_
  * `@assert isfinite(tmp)`, :570); the state is then that of the last step. */
 int pic2d3v_run(pic2d3v_ctx* ctx, int64_t nsteps, double* rows);
 
-/* Lorenz gauge only: warmup!(field, plasma) (:826-859), which the reference's
+/* Lorenz gauge only (a LorenzGaugeField's EBxyz, which its loop! gathers
+ * before it solves, follows the fields set with pic2d3v_lg_set; a fresh one
+ * is zero): warmup!(field, plasma) (:826-859), which the reference's
  * driver calls once before its loop (src/2D3V.jl:123); nsteps of loop!
  * without diagnose!; the field state slot `which` (PIC2D3V_LG_*: 2*NX*NY
  * doubles, complex interleaved as Julia's ComplexF64) out and in; and the

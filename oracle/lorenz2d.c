@@ -6,7 +6,8 @@
  * six-component gather (:1231-1247), the one- and three-weight deposits
  * (:1258-1284), update! (:468-479), reduction! (:504-522) and
  * diagnose!(::LorenzGaugeDiagnostics) (:1332-1398), whose in-place inverse
- * transforms of the potentials and sources are part of the dynamics.
+ * transforms of the potentials and sources are part of the dynamics; and
+ * LorenzGaugeField's centred loop! (:307-345, :665-745) at the end of the file.
  *
  * TEST INFRASTRUCTURE ONLY (see picoracle.h): the oracle the GPU Lorenz-gauge
  * path is checked against.
@@ -56,6 +57,11 @@
  *                      numpy, deposits in floating point) over a few steps;
  *                      diagnose!'s in-place transforms; the binned averages
  *                      with ngskip = 2; two calls equal one.
+ *   orclg_run_centred  (LorenzGaugeField) a numpy restatement of loop! over a
+ *                      few steps; the vacuum recurrence; the centred E of a
+ *                      single mode; uncharged particles stream freely in the
+ *                      first step (EBxyz is still zero) and turn about B0 in
+ *                      the second.
  */
 #include "picoracle.h"
 
@@ -485,5 +491,160 @@ int orclg_run(int32_t NX, int32_t NY, double Lx, double Ly, double dt, const dou
     free(acc);
     free(xh);
     free(yh);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LorenzGaugeField (:307-345) and its loop! (:665-745), the centred scheme. */
+/* The state slots hold: PHI0 = phi+ (the latest potential), PHIM = phi0     */
+/* (the one before), AX0.. = A+, AXM.. = A0, RHO0 = rho+, J0 = J+ (in the    */
+/* spectrum, [1,1] = 0, as the step leaves them), EX..BZ as before.  EB (the */
+/* 6 x NX x NY interior of EBxyz) is state too: the particle loop runs first */
+/* and gathers what the previous step's update! left (zeros at the start).  */
+/*   1. particles: gather at x, ElectromagneticBoris, x = unimod(x +          */
+/*      (vx0 + vx)/2*dt, Lx), deposit qw, vx*qw, vy*qw, vz*qw at the new x   */
+/*      (:672-694); reduction into rho+, J+ (:696-699)                       */
+/*   2. fft! of rho+, J+ and their [1,1] = 0 (:700-709)                      */
+/*   3. lorenzgauge!(imex, x+, x0, s, k2, dt^2) (:619-627): new = (num*x+ +  */
+/*      dt^2*s)/den - x0; x0 = x+; x+ = new (:711-714)                       */
+/*   4. E = -im*k*(phi+ + phi0)/2 - (A+ - A0)/dt, B = curl (A+ + A0)/2        */
+/*      (:717-725), ifft of the six, update! (:727-735)                      */
+/* The reference has no warmup! for this type (warmup!(::AbstractField) is   */
+/* the identity, :745) and its diagnose! reads f.rho0, which the type does   */
+/* not have (:1366), so only loop! runs.                                      */
+/* ------------------------------------------------------------------------ */
+static void fft2_inplace(int32_t NX, int32_t NY, double* z) {
+    const int64_t G = (int64_t)NX * NY;
+    double* re = (double*)malloc(sizeof(double) * (size_t)G);
+    double* im = (double*)malloc(sizeof(double) * (size_t)G);
+    for (int64_t k = 0; k < G; ++k) {
+        re[k] = z[2 * k];
+        im[k] = z[2 * k + 1];
+    }
+    orc2d_fft(NX, NY, re, im, 0);
+    for (int64_t k = 0; k < G; ++k) {
+        z[2 * k] = re[k];
+        z[2 * k + 1] = im[k];
+    }
+    free(re);
+    free(im);
+}
+
+void orclg_fft2(int32_t NX, int32_t NY, double* z) { fft2_inplace(NX, NY, z); }
+
+/* steps 3-4 on the state (the sources already in the spectrum) */
+void orclg_solve_centred(int32_t NX, int32_t NY, double Lx, double Ly, double dt, double theta, double* F) {
+    const int64_t G = (int64_t)NX * NY;
+    double* kx = (double*)malloc(sizeof(double) * (size_t)NX);
+    double* ky = (double*)malloc(sizeof(double) * (size_t)NY);
+    double* k2 = (double*)malloc(sizeof(double) * (size_t)G);
+    orclg_wavenumbers(NX, NY, Lx, Ly, kx, ky, k2);
+    const double dt2 = dt * dt;
+    double* Z[18];
+    for (int a = 0; a < 18; ++a) Z[a] = F + (size_t)a * 2 * G;
+    static const int xps[4] = {ORCLG_PHI0, ORCLG_AX0, ORCLG_AY0, ORCLG_AZ0};
+    static const int x0s[4] = {ORCLG_PHIM, ORCLG_AXM, ORCLG_AYM, ORCLG_AZM};
+    static const int srcs[4] = {ORCLG_RHO0, ORCLG_JX0, ORCLG_JY0, ORCLG_JZ0};
+    for (int32_t j = 0; j < NY; ++j)
+        for (int32_t i = 0; i < NX; ++i) {
+            const int64_t k = i + (int64_t)NX * j;
+            const double num = 2.0 - dt2 * k2[k] * (1.0 - theta);
+            const double den = 1.0 + dt2 * k2[k] * theta / 2.0;
+            for (int a = 0; a < 4; ++a) {
+                double* xp = Z[xps[a]] + 2 * k;
+                double* x0 = Z[x0s[a]] + 2 * k;
+                const double* sr = Z[srcs[a]] + 2 * k;
+                const double nr = (num * xp[0] + dt2 * sr[0]) / den - x0[0];
+                const double ni = (num * xp[1] + dt2 * sr[1]) / den - x0[1];
+                x0[0] = xp[0];
+                x0[1] = xp[1];
+                xp[0] = nr;
+                xp[1] = ni;
+            }
+            const double kxi = kx[i], kyj = ky[j];
+            const double *php = Z[ORCLG_PHI0] + 2 * k, *ph0 = Z[ORCLG_PHIM] + 2 * k;
+            const double *axp = Z[ORCLG_AX0] + 2 * k, *ayp = Z[ORCLG_AY0] + 2 * k, *azp = Z[ORCLG_AZ0] + 2 * k;
+            const double *ax0 = Z[ORCLG_AXM] + 2 * k, *ay0 = Z[ORCLG_AYM] + 2 * k, *az0 = Z[ORCLG_AZM] + 2 * k;
+            const double sr = php[0] + ph0[0], si = php[1] + ph0[1];
+            double er, ei, fr, fi;
+            /* Ex = -im*kx*(phi+ + phi0)/2 ; Ex -= (Ax+ - Ax0)/dt */
+            cmul(0.0 * kxi, -kxi, sr, si, &er, &ei);
+            Z[ORCLG_EX][2 * k] = er / 2.0 - (axp[0] - ax0[0]) / dt;
+            Z[ORCLG_EX][2 * k + 1] = ei / 2.0 - (axp[1] - ax0[1]) / dt;
+            cmul(0.0 * kyj, -kyj, sr, si, &er, &ei);
+            Z[ORCLG_EY][2 * k] = er / 2.0 - (ayp[0] - ay0[0]) / dt;
+            Z[ORCLG_EY][2 * k + 1] = ei / 2.0 - (ayp[1] - ay0[1]) / dt;
+            Z[ORCLG_EZ][2 * k] = -(azp[0] - az0[0]) / dt;
+            Z[ORCLG_EZ][2 * k + 1] = -(azp[1] - az0[1]) / dt;
+            const double szr = azp[0] + az0[0], szi = azp[1] + az0[1];
+            cmul(0.0 * kyj, kyj, szr, szi, &er, &ei);
+            Z[ORCLG_BX][2 * k] = er / 2.0;
+            Z[ORCLG_BX][2 * k + 1] = ei / 2.0;
+            cmul(0.0 * kxi, -kxi, szr, szi, &er, &ei);
+            Z[ORCLG_BY][2 * k] = er / 2.0;
+            Z[ORCLG_BY][2 * k + 1] = ei / 2.0;
+            cmul(0.0 * kxi, kxi, ayp[0] + ay0[0], ayp[1] + ay0[1], &er, &ei);
+            cmul(0.0 * kyj, kyj, axp[0] + ax0[0], axp[1] + ax0[1], &fr, &fi);
+            Z[ORCLG_BZ][2 * k] = er / 2.0 - fr / 2.0;
+            Z[ORCLG_BZ][2 * k + 1] = ei / 2.0 - fi / 2.0;
+        }
+    for (int a = ORCLG_EX; a <= ORCLG_BZ; ++a) ifft2_inplace(NX, NY, Z[a]);
+    free(kx);
+    free(ky);
+    free(k2);
+}
+
+int orclg_run_centred(int32_t NX, int32_t NY, double Lx, double Ly, double dt, const double* B0, double theta,
+                      int32_t ns, const int32_t* shape, const int64_t* P, const double* charge, const double* mass,
+                      const double* weight, double* state, double* F, double* EB, int64_t nsteps) {
+    if (ns < 1 || ns > 8 || NX < 2 || NY < 2 || (NX & (NX - 1)) || (NY & (NY - 1))) return -1;
+    const int64_t G = (int64_t)NX * NY;
+    const double NX_Lx = (double)NX / Lx, NY_Ly = (double)NY / Ly;
+    const double dV = (Lx / (double)NX) * (Ly / (double)NY);
+    double qw[8];
+    for (int s = 0; s < ns; ++s) qw[s] = charge[s] * weight[s] / dV;
+    const int qr = orclg_frac_bits(ns, P, qw, NULL);
+    i128* acc = (i128*)malloc(sizeof(i128) * (size_t)G);
+    for (int64_t t = 0; t < nsteps; ++t) {
+        double* st = state;
+        for (int s = 0; s < ns; ++s) {
+            const int64_t n = P[s];
+            double *x = st, *y = st + n, *vx = st + 2 * n, *vy = st + 3 * n, *vz = st + 4 * n;
+            const double q_m = charge[s] / mass[s];
+            for (int64_t p = 0; p < n; ++p) {
+                double f[6], v[3] = {vx[p], vy[p], vz[p]};
+                orclg_gather(NX, NY, NX_Lx, NY_Ly, shape[s], EB, x[p], y[p], f);
+                const double vx0 = vx[p], vy0 = vy[p];
+                orclg_boris(dt, q_m, f, f + 3, v);
+                vx[p] = v[0];
+                vy[p] = v[1];
+                vz[p] = v[2];
+                x[p] = orc23_unimod(x[p] + (vx0 + vx[p]) / 2.0 * dt, Lx);
+                y[p] = orc23_unimod(y[p] + (vy0 + vy[p]) / 2.0 * dt, Ly);
+            }
+            st += 5 * n;
+        }
+        memset(acc, 0, sizeof(i128) * (size_t)G);
+        st = state;
+        for (int s = 0; s < ns; ++s) {
+            deposit_into(acc, NX, NY, NX_Lx, NY_Ly, shape[s], P[s], st, st + P[s], NULL, qw[s], qr);
+            st += 5 * P[s];
+        }
+        double* rho = F + (size_t)ORCLG_RHO0 * 2 * G;
+        for (int64_t k = 0; k < G; ++k) {
+            rho[2 * k] = unfix_signed(acc[k], qr);
+            rho[2 * k + 1] = 0.0;
+        }
+        deposit_currents(NX, NY, Lx, Ly, ns, shape, P, qw, state, F);
+        for (int a = ORCLG_RHO0; a <= ORCLG_JZ0; ++a) {
+            double* z = F + (size_t)a * 2 * G;
+            fft2_inplace(NX, NY, z);
+            z[0] = 0.0;
+            z[1] = 0.0;
+        }
+        orclg_solve_centred(NX, NY, Lx, Ly, dt, theta, F);
+        orclg_update(NX, NY, F, B0, EB);
+    }
+    free(acc);
     return 0;
 }

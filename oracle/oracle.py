@@ -122,6 +122,10 @@ def lib():
         "orclg_frac_bits": (ctypes.c_int, [c_i32, _ip64, _dp, c_vp]),
         "orclg_warmup": (None, [c_i32, c_i32, c_d, c_d, c_d, c_i32, _ip32, _ip64, _dp, _dp, _dp, _dp]),
         "orclg_row": (None, [c_i32, c_i32, c_i32, _dp, _dp, c_i32, _ip64, _dp, _dp, _dp, _dp]),
+        "orclg_fft2": (None, [c_i32, c_i32, _dp]),
+        "orclg_solve_centred": (None, [c_i32, c_i32, c_d, c_d, c_d, c_d, _dp]),
+        "orclg_run_centred": (ctypes.c_int, [c_i32, c_i32, c_d, c_d, c_d, _dp, c_d, c_i32, _ip32, _ip64, _dp, _dp,
+                                             _dp, _dp, _dp, _dp, c_i64]),
         "orclg_run": (ctypes.c_int, [c_i32, c_i32, c_d, c_d, c_d, _dp, c_i32, c_d, c_i32, c_i32, c_i32, _ip32,
                                      _ip64, _dp, _dp, _dp, _dp, _dp, c_i64, c_i64, c_i32, c_vp, c_vp, c_i64]),
         "orc23_run": (ctypes.c_int, [c_i32, c_i32, c_d, c_d, c_d, _dp, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32,
@@ -707,6 +711,38 @@ def lg_run(NX, NY, Lx, Ly, dt, B0, species, F, nsteps, theta=1.0, t0=0, buffer=1
     if rc != 0:
         raise ValueError(f"orclg_run rejected its arguments (rc={rc})")
     return _lg_unpack(species, P, state), F, rows, bins
+
+
+def lg_fft2(z):
+    z = _c128(z).copy()
+    NY, NX = z.shape
+    lib().orclg_fft2(NX, NY, _cp(z))
+    return z
+
+
+def lg_solve_centred(F, dt, theta, Lx=1.0, Ly=1.0):
+    """Steps 3-4 of LorenzGaugeField's loop! on the state (sources already in
+    the spectrum).  Returns the new state."""
+    F = _c128(F).copy()
+    _, NY, NX = F.shape
+    lib().orclg_solve_centred(NX, NY, float(Lx), float(Ly), float(dt), float(theta), _cp(F))
+    return F
+
+
+def lg_run_centred(NX, NY, Lx, Ly, dt, B0, species, F, EB, nsteps, theta=0.0):
+    """nsteps of loop!(plasma, ::LorenzGaugeField).  F: (18, NY, NX) complex
+    state (PHI0 = phi+, PHIM = phi0, ...); EB: (6, NY, NX) the gathered
+    fields (zeros for a fresh field).  Returns (species, F, EB)."""
+    P, state, shape = _lg_species(species)
+    F = _c128(F).copy()
+    EB = _f64(EB).copy()
+    rc = lib().orclg_run_centred(int(NX), int(NY), float(Lx), float(Ly), float(dt), _f64(B0), float(theta),
+                                 len(species), shape, P, _f64([s["charge"] for s in species]),
+                                 _f64([s["mass"] for s in species]), _f64([s["weight"] for s in species]), state,
+                                 _cp(F), EB.ravel(), int(nsteps))
+    if rc != 0:
+        raise ValueError(f"orclg_run_centred rejected its arguments (rc={rc})")
+    return _lg_unpack(species, P, state), F, EB
 
 
 # ---- 1D electromagnetic fixed point (src/GaussianEMFixedPoint.jl) ----------

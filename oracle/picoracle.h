@@ -207,6 +207,11 @@ void orclg_warmup(int32_t NX, int32_t NY, double Lx, double Ly, double dt, int32
                   const int64_t* P, const double* charge, const double* weight, double* state, double* F);
 void orclg_row(int32_t NX, int32_t NY, int32_t buffer, const double* F, const double* EB, int32_t ns,
                const int64_t* P, const double* mass, const double* weight, const double* state, double* row);
+void orclg_fft2(int32_t NX, int32_t NY, double* z);
+void orclg_solve_centred(int32_t NX, int32_t NY, double Lx, double Ly, double dt, double theta, double* F);
+int  orclg_run_centred(int32_t NX, int32_t NY, double Lx, double Ly, double dt, const double* B0, double theta,
+                       int32_t ns, const int32_t* shape, const int64_t* P, const double* charge, const double* mass,
+                       const double* weight, double* state, double* F, double* EB, int64_t nsteps);
 int  orclg_run(int32_t NX, int32_t NY, double Lx, double Ly, double dt, const double* B0, int32_t buffer,
                double theta, int32_t ntskip, int32_t ngskip, int32_t ns, const int32_t* shape, const int64_t* P,
                const double* charge, const double* mass, const double* weight, double* state, double* F,

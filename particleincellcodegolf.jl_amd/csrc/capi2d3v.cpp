@@ -37,6 +37,7 @@ struct pic2d3v_ctx {
     // Lorenz gauge: the state, the half-step positions, and the step graphs
     // (loop! alone, with diagnose! without and with the row)
     bool lorenz = false;
+    bool centred = false;    // LorenzGaugeField (else LorenzGaugeStaggeredField)
     LG23 lg{};
     double* xh[kMaxSpecies23] = {};
     double* yh[kMaxSpecies23] = {};
@@ -207,8 +208,8 @@ static int validate(const pic2d3v_config* g, std::string& why) {
         why = "the species hold more than 2^26 particles together";
         return PIC_EINVAL;
     }
-    if (g->field != PIC2D3V_ELECTROSTATIC && g->field != PIC2D3V_LORENZ_STAGGERED) {
-        why = "field must be PIC2D3V_ELECTROSTATIC or PIC2D3V_LORENZ_STAGGERED";
+    if (g->field != PIC2D3V_ELECTROSTATIC && g->field != PIC2D3V_LORENZ_STAGGERED && g->field != PIC2D3V_LORENZ) {
+        why = "field must be PIC2D3V_ELECTROSTATIC, PIC2D3V_LORENZ_STAGGERED or PIC2D3V_LORENZ";
         return PIC_EINVAL;
     }
     if (!std::isfinite(g->theta)) {
@@ -349,8 +350,9 @@ extern "C" int pic2d3v_create(pic2d3v_ctx** out, const pic2d3v_config* cfg) {
             (rc = dalloc(c, &d.mpart[s], 6 * ((P + kTreeChunk - 1) / kTreeChunk))))
             return fail(rc);
     }
-    if (cfg->field == PIC2D3V_LORENZ_STAGGERED) {
+    if (cfg->field == PIC2D3V_LORENZ_STAGGERED || cfg->field == PIC2D3V_LORENZ) {
         c->lorenz = true;
+        c->centred = cfg->field == PIC2D3V_LORENZ;
         configure_kernels_2d3v_lg();
         LG23& L = c->lg;
         L.NX = NX;
@@ -638,16 +640,21 @@ static int ensure_bins(pic2d3v_ctx* c, int64_t nbins) {
     return PIC_OK;
 }
 
-// One Lorenz-gauge step: loop! (:875-979) and, variant 1 or 2, diagnose!
-// (variant 2 with the row).
+// One Lorenz-gauge step: loop! (staggered :875-979, centred :665-745) and,
+// variant 1 or 2, diagnose! (variant 2 with the row).
 static void enqueue_step_lg(pic2d3v_ctx* c, int variant, hipStream_t s) {
     const int ns = c->cfg.nspecies;
     const LG23& L = c->lg;
-    lg_launch_solve(L, s);
-    for (int k = 0; k < ns; ++k) lg_launch_push(L, c->g, c->sp[k], k, c->xh[k], c->yh[k], s);
-    for (int k = 0; k < ns; ++k) lg_launch_deposit_rho(L, c->g, c->sp[k], c->xh[k], c->yh[k], s);
+    const bool cen = c->centred;
+    if (!cen) lg_launch_solve(L, false, s);
+    for (int k = 0; k < ns; ++k) lg_launch_push(L, c->g, c->sp[k], k, c->xh[k], c->yh[k], cen, s);
+    for (int k = 0; k < ns; ++k) {
+        const Species23& sp = c->sp[k];
+        lg_launch_deposit_rho(L, c->g, sp, cen ? sp.x : c->xh[k], cen ? sp.y : c->yh[k], s);
+    }
     for (int k = 0; k < ns; ++k) lg_launch_deposit_j(L, c->g, c->sp[k], s);
-    lg_launch_finish(L, s);
+    lg_launch_finish(L, cen, s);
+    if (cen) lg_launch_solve(L, true, s);
     if (variant == 2) lg_launch_diag_row(L, c->g, c->d, c->sp, ns, s);
     if (variant >= 1) lg_launch_diag_fields(L, c->d, s);
     launch23_tick(c->d.t, s);
@@ -688,6 +695,11 @@ extern "C" int pic2d3v_run(pic2d3v_ctx* c, int64_t nsteps, double* rows) {
             set_err(c, "pic2d3v_run: upload species %d first", s);
             return PIC_ESTATE;
         }
+    if (c->centred) {
+        set_err(c, "pic2d3v_run: LorenzGaugeField has no diagnose! (the reference's reads field.rho0, which the "
+                   "type lacks); use pic2d3v_loop");
+        return PIC_ESTATE;
+    }
     HIPCHK(c, hipSetDevice(c->cfg.device));
     const int64_t nt = c->cfg.ntskip;
     const int64_t t0 = c->step, t1 = c->step + nsteps;
@@ -759,7 +771,7 @@ static int need_uploaded(pic2d3v_ctx* c, const char* who) {
 
 extern "C" int pic2d3v_warmup(pic2d3v_ctx* c) {
     if (!c) return PIC_EINVAL;
-    if (!c->lorenz) return PIC_OK;  // warmup!(::AbstractField) = field
+    if (!c->lorenz || c->centred) return PIC_OK;  // warmup!(::AbstractField) = field
     if (int rc = need_uploaded(c, "pic2d3v_warmup")) return rc;
     HIPCHK(c, hipSetDevice(c->cfg.device));
     const int ns = c->cfg.nspecies;
@@ -822,6 +834,9 @@ extern "C" int pic2d3v_lg_set(pic2d3v_ctx* c, int32_t which, const double* z) {
     HIPCHK(c, hipSetDevice(c->cfg.device));
     const size_t G = (size_t)c->cfg.NX * c->cfg.NY;
     HIPCHK(c, hipMemcpyAsync(c->lg.F + which * G, z, sizeof(double2) * G, hipMemcpyHostToDevice, c->stream));
+    // the centred loop gathers EBxyz before it solves: setting a field sets
+    // what the next step gathers
+    if (c->centred && which >= PIC2D3V_LG_EX) lg_launch_update_eb(c->lg, c->stream);
     return finish(c);
 }
 

@@ -22,6 +22,11 @@
 // diagnose! (:1332-1398): the row (field energy over the ghosted EBxyz, field
 // momentum, the species sums), the in-place inverse transforms of Ax0, Ay0,
 // Az0, phi0, rho0, Jx0, Jy0, Jz0, and the binned averages.
+//
+// LorenzGaugeField's centred loop! (:665-745) reuses them: the push (with
+// the centred move) comes first, then the deposits at the new positions,
+// k_lg_finish<2>, the forward transforms of rho+, J+, k_lg_spectral<true> and
+// the inverse transforms with update!.
 #include <climits>
 
 #include "fft_lines.h"
@@ -82,6 +87,11 @@ __device__ __forceinline__ void cmul_lg(double ar, double ai, double br, double 
     ci = ar * bi + ai * br;
 }
 
+// CENTRED (LorenzGaugeField, :711-725): the sources are already in the
+// spectrum and their [1,1] is cleared here; new = (num*x+ + dt^2*s)/den - x0,
+// x0 = x+, x+ = new (the state slots PHI0.. hold x+, PHIM.. x0), and E, B use
+// (phi+ + phi0)/2 and A+ - A0, A+ + A0.
+template <bool CENTRED>
 __global__ void __launch_bounds__(256) k_lg_spectral(LG23 L) {
     const int NX = L.NX;
     const int64_t G = (int64_t)NX * L.NY;
@@ -97,19 +107,45 @@ __global__ void __launch_bounds__(256) k_lg_spectral(LG23 L) {
         const int x0s[4] = {kLGPhi0, kLGAx0, kLGAy0, kLGAz0};
         const int xms[4] = {kLGPhiM, kLGAxM, kLGAyM, kLGAzM};
         const int srcs[4] = {kLGRho0, kLGJx0, kLGJy0, kLGJz0};
-        double2 p[4];
+        double2 p[4], q0[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            const double2 x0 = F[x0s[a] * G + k], xm = F[xms[a] * G + k], sr = F[srcs[a] * G + k];
+            const double2 x0 = F[x0s[a] * G + k], xm = F[xms[a] * G + k];
+            double2 sr = F[srcs[a] * G + k];
+            if (CENTRED && k == 0) {
+                sr = make_double2(0.0, 0.0);
+                F[srcs[a] * G] = sr;
+            }
             p[a].x = (num * x0.x + dt2 * sr.x) / den - xm.x;
             p[a].y = (num * x0.y + dt2 * sr.y) / den - xm.y;
-            L.plus[a * G + k] = p[a];
+            if (CENTRED) {
+                q0[a] = x0;
+                F[xms[a] * G + k] = x0;
+                F[x0s[a] * G + k] = p[a];
+            } else {
+                q0[a] = x0;
+                L.plus[a * G + k] = p[a];
+            }
         }
-        const double2 ax0 = F[kLGAx0 * G + k], ay0 = F[kLGAy0 * G + k], az0 = F[kLGAz0 * G + k];
+        // A0 of the E and B formulas: the state's A0 (staggered) or the old A+ (centred)
+        const double2 ax0 = q0[1], ay0 = q0[2], az0 = q0[3];
         double er, ei, fr, fi;
-        cmul_lg(0.0 * kxi, -kxi, p[0].x, p[0].y, er, ei);
+        double phr = p[0].x, phi_ = p[0].y;
+        if (CENTRED) {
+            phr = p[0].x + q0[0].x;
+            phi_ = p[0].y + q0[0].y;
+        }
+        cmul_lg(0.0 * kxi, -kxi, phr, phi_, er, ei);
+        if (CENTRED) {
+            er = er / 2.0;
+            ei = ei / 2.0;
+        }
         F[kLGEx * G + k] = make_double2(er - (p[1].x - ax0.x) / dt, ei - (p[1].y - ax0.y) / dt);
-        cmul_lg(0.0 * kyj, -kyj, p[0].x, p[0].y, er, ei);
+        cmul_lg(0.0 * kyj, -kyj, phr, phi_, er, ei);
+        if (CENTRED) {
+            er = er / 2.0;
+            ei = ei / 2.0;
+        }
         F[kLGEy * G + k] = make_double2(er - (p[2].x - ay0.x) / dt, ei - (p[2].y - ay0.y) / dt);
         F[kLGEz * G + k] = make_double2(-(p[3].x - az0.x) / dt, -(p[3].y - az0.y) / dt);
         const double szr = p[3].x + az0.x, szi = p[3].y + az0.y;
@@ -123,8 +159,10 @@ __global__ void __launch_bounds__(256) k_lg_spectral(LG23 L) {
     }
 }
 
-// In-place inverse transform along y of C columns of array blockIdx.y.
-__global__ void __launch_bounds__(256) k_lg_cols_inv(LGList lst, int NX, int NY, int ly, const double2* tw) {
+// In-place transform along y of C columns of array blockIdx.y (INV: the
+// unnormalised inverse).
+template <bool INV>
+__global__ void __launch_bounds__(256) k_lg_cols(LGList lst, int NX, int NY, int ly, const double2* tw) {
     extern __shared__ double lds_lg[];
     const int C = lg_cols(NX, NY), st = line_stride(NY), lc = __ffs(C) - 1;
     double* re = lds_lg;
@@ -139,7 +177,7 @@ __global__ void __launch_bounds__(256) k_lg_cols_inv(LGList lst, int NX, int NY,
         im[pos] = v.y;
     }
     __syncthreads();
-    fft_lines(re, im, tw, ly, C, st, true);
+    fft_lines(re, im, tw, ly, C, st, INV);
     for (int e = threadIdx.x; e < C * NY; e += blockDim.x) {
         const int c = e & (C - 1), j = e >> lc;
         const int pos = c * st + padi(j);
@@ -147,11 +185,12 @@ __global__ void __launch_bounds__(256) k_lg_cols_inv(LGList lst, int NX, int NY,
     }
 }
 
-// In-place inverse transform along x of R rows of array blockIdx.y, scaled by
-// 1/(NX*NY) (FFTW's ifft).  EBOUT: the list is Ex, Ey, Ez, Bx, By, Bz and the
-// pass also writes update!'s EBxyz = 0.0 + real(F) (+ B0 for B).
-template <bool EBOUT>
-__global__ void __launch_bounds__(256) k_lg_rows_inv(LGList lst, LG23 L) {
+// In-place transform along x of R rows of array blockIdx.y.  INV: the inverse,
+// scaled by 1/(NX*NY) (FFTW's ifft); else the unnormalised forward transform.
+// EBOUT: the list is Ex, Ey, Ez, Bx, By, Bz and the pass also writes update!'s
+// EBxyz = 0.0 + real(F) (+ B0 for B).
+template <bool INV, bool EBOUT>
+__global__ void __launch_bounds__(256) k_lg_rows(LGList lst, LG23 L) {
     extern __shared__ double lds_lg[];
     const int NX = L.NX, lx = L.lx;
     const int R = lg_rows(NX, L.NY), st = line_stride(NX);
@@ -169,12 +208,12 @@ __global__ void __launch_bounds__(256) k_lg_rows_inv(LGList lst, LG23 L) {
         im[pos] = v.y;
     }
     __syncthreads();
-    fft_lines(re, im, L.tw, lx, R, st, true);
+    fft_lines(re, im, L.tw, lx, R, st, INV);
     const double scale = 1.0 / (double)G;
     for (int e = threadIdx.x; e < R * NX; e += blockDim.x) {
         const int r = e >> lx, i = e & (NX - 1);
         const int pos = r * st + padi(i);
-        const double zr = re[pos] * scale, zi = im[pos] * scale;
+        const double zr = INV ? re[pos] * scale : re[pos], zi = INV ? im[pos] * scale : im[pos];
         Z[k0 + e] = make_double2(zr, zi);
         if (EBOUT) {
             const double ev = 0.0 + zr;
@@ -183,14 +222,21 @@ __global__ void __launch_bounds__(256) k_lg_rows_inv(LGList lst, LG23 L) {
     }
 }
 
+// ifft! in place (along y, then along x and the scale)
 static void lg_inverse(const LGList& lst, int n, const LG23& L, bool ebout, hipStream_t st) {
     const int C = lg_cols(L.NX, L.NY), R = lg_rows(L.NX, L.NY);
-    k_lg_cols_inv<<<dim3((unsigned)(L.NX / C), (unsigned)n), 256, lg_cols_lds(L.NX, L.NY), st>>>(lst, L.NX, L.NY,
-                                                                                                   L.ly, L.tw);
-    if (ebout)
-        k_lg_rows_inv<true><<<dim3((unsigned)(L.NY / R), (unsigned)n), 256, lg_rows_lds(L.NX, L.NY), st>>>(lst, L);
-    else
-        k_lg_rows_inv<false><<<dim3((unsigned)(L.NY / R), (unsigned)n), 256, lg_rows_lds(L.NX, L.NY), st>>>(lst, L);
+    const dim3 gc((unsigned)(L.NX / C), (unsigned)n), gr((unsigned)(L.NY / R), (unsigned)n);
+    k_lg_cols<true><<<gc, 256, lg_cols_lds(L.NX, L.NY), st>>>(lst, L.NX, L.NY, L.ly, L.tw);
+    if (ebout) k_lg_rows<true, true><<<gr, 256, lg_rows_lds(L.NX, L.NY), st>>>(lst, L);
+    else k_lg_rows<true, false><<<gr, 256, lg_rows_lds(L.NX, L.NY), st>>>(lst, L);
+}
+
+// fft! in place (along x, then along y)
+static void lg_forward(const LGList& lst, int n, const LG23& L, hipStream_t st) {
+    const int C = lg_cols(L.NX, L.NY), R = lg_rows(L.NX, L.NY);
+    const dim3 gc((unsigned)(L.NX / C), (unsigned)n), gr((unsigned)(L.NY / R), (unsigned)n);
+    k_lg_rows<false, false><<<gr, 256, lg_rows_lds(L.NX, L.NY), st>>>(lst, L);
+    k_lg_cols<false><<<gc, 256, lg_cols_lds(L.NX, L.NY), st>>>(lst, L.NX, L.NY, L.ly, L.tw);
 }
 
 static unsigned grid_blocks(int64_t n) {
@@ -198,12 +244,33 @@ static unsigned grid_blocks(int64_t n) {
     return (unsigned)(b < 8192 ? (b > 0 ? b : 1) : 8192);
 }
 
-void lg_launch_solve(const LG23& L, hipStream_t st) {
+void lg_launch_solve(const LG23& L, bool centred, hipStream_t st) {
     const int64_t G = (int64_t)L.NX * L.NY;
-    k_lg_spectral<<<grid_blocks(G), 256, 0, st>>>(L);
+    if (centred) {
+        LGList src{};
+        for (int c = 0; c < 4; ++c) src.a[c] = L.F + (size_t)(kLGRho0 + c) * G;
+        lg_forward(src, 4, L, st);
+        k_lg_spectral<true><<<grid_blocks(G), 256, 0, st>>>(L);
+    } else {
+        k_lg_spectral<false><<<grid_blocks(G), 256, 0, st>>>(L);
+    }
     LGList lst{};
     for (int c = 0; c < 6; ++c) lst.a[c] = L.F + (size_t)(kLGEx + c) * G;
     lg_inverse(lst, 6, L, true, st);
+}
+
+// update! alone: EBxyz from the six fields of the state
+__global__ void __launch_bounds__(256) k_lg_update_eb(LG23 L) {
+    const int64_t G = (int64_t)L.NX * L.NY;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 6 * G; e += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(e / G);
+        const double ev = 0.0 + L.F[(kLGEx + c) * G + (e - c * G)].x;
+        L.EB[e] = c < 3 ? ev : ev + L.B0[c - 3];
+    }
+}
+
+void lg_launch_update_eb(const LG23& L, hipStream_t st) {
+    k_lg_update_eb<<<grid_blocks(6 * (int64_t)L.NX * L.NY), 256, 0, st>>>(L);
 }
 
 // ---------------------------------------------------------------------------
@@ -281,7 +348,9 @@ __device__ __forceinline__ void lg_window(const Grid23& g, const double* xs, con
     __syncthreads();
 }
 
-template <int SH>
+// CENTRED (LorenzGaugeField, :683-690): x = unimod(x + (vx0 + vx)/2*dt);
+// else the staggered loop's two half moves, the first written to (xh, yh).
+template <int SH, bool CENTRED>
 __global__ void __launch_bounds__(kTile23Threads)
 k_lg_push(LG23 L, Grid23 g, Species23 s, int sidx, double* __restrict__ xh, double* __restrict__ yh) {
     constexpr int n = ShapeCount<SH>::value;
@@ -353,13 +422,19 @@ k_lg_push(LG23 L, Grid23 g, Species23 s, int sidx, double* __restrict__ xh, doub
                     for (int c = 0; c < 6; ++c) f[c] = fma(L.EB[c * G + kk], wxy, f[c]);
                 }
             }
+        const double vx0 = vx, vy0 = vy;
         lg_boris(dt, s.q_m, f, vx, vy, vz);
-        x = unimod23(x + vx * dt / 2.0, g.Lx);
-        y = unimod23(y + vy * dt / 2.0, g.Ly);
-        xh[p] = x;
-        yh[p] = y;
-        x = unimod23(x + vx * dt / 2.0, g.Lx);
-        y = unimod23(y + vy * dt / 2.0, g.Ly);
+        if (CENTRED) {
+            x = unimod23(x + (vx0 + vx) / 2.0 * dt, g.Lx);
+            y = unimod23(y + (vy0 + vy) / 2.0 * dt, g.Ly);
+        } else {
+            x = unimod23(x + vx * dt / 2.0, g.Lx);
+            y = unimod23(y + vy * dt / 2.0, g.Ly);
+            xh[p] = x;
+            yh[p] = y;
+            x = unimod23(x + vx * dt / 2.0, g.Lx);
+            y = unimod23(y + vy * dt / 2.0, g.Ly);
+        }
         s.x[p] = x;
         s.y[p] = y;
         s.vx[p] = vx;
@@ -496,10 +571,12 @@ k_lg_deposit(LG23 L, Grid23 g, Species23 s, const double* __restrict__ X, const 
         }
 }
 
-// rho0, J0 from the limbs (cleared); WARM: rho0 = 0 (warmup!'s second
-// deposit! discards the charge); else the buffer copies of the potentials.
-template <bool WARM>
+// rho0, J0 from the limbs (cleared).  MODE 0: then the staggered buffer
+// copies of the potentials; 1 (warmup!): rho0 = 0, as warmup!'s second
+// deposit! discards the charge; 2 (centred): rho+, J+ only.
+template <int MODE>
 __global__ void __launch_bounds__(256) k_lg_finish(LG23 L) {
+    constexpr bool WARM = MODE == 1;
     const int64_t G = (int64_t)L.NX * L.NY;
     int qj[3];
     for (int c = 0; c < 3; ++c) qj[c] = lg_frac_bits(L, c);
@@ -517,7 +594,7 @@ __global__ void __launch_bounds__(256) k_lg_finish(LG23 L) {
             F[(kLGJx0 + c) * G + k] = make_double2(lg_limbs(r, G, k, qj[c]), 0.0);
             for (int b = 0; b < 3; ++b) r[b * G + k] = 0;
         }
-        if (!WARM) {
+        if (MODE == 0) {
             const int x0s[4] = {kLGPhi0, kLGAx0, kLGAy0, kLGAz0};
             const int xms[4] = {kLGPhiM, kLGAxM, kLGAyM, kLGAzM};
 #pragma unroll
@@ -586,8 +663,9 @@ static void by_shape(int shape, A... a) {
 template <int SH>
 struct PushL {
     static void go(const LG23& L, const Grid23& g, const Species23& s, int sidx, double* xh, double* yh,
-                   hipStream_t st) {
-        k_lg_push<SH><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s, sidx, xh, yh);
+                   bool centred, hipStream_t st) {
+        if (centred) k_lg_push<SH, true><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s, sidx, xh, yh);
+        else k_lg_push<SH, false><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s, sidx, xh, yh);
     }
 };
 template <int SH>
@@ -605,8 +683,8 @@ struct DepJL {
 };
 
 void lg_launch_push(const LG23& L, const Grid23& g, const Species23& s, int sidx, double* xh, double* yh,
-                    hipStream_t st) {
-    if (s.P > 0) by_shape<PushL>(s.shape, L, g, s, sidx, xh, yh, st);
+                    bool centred, hipStream_t st) {
+    if (s.P > 0) by_shape<PushL>(s.shape, L, g, s, sidx, xh, yh, centred, st);
 }
 
 void lg_launch_deposit_rho(const LG23& L, const Grid23& g, const Species23& s, const double* xh, const double* yh,
@@ -618,8 +696,9 @@ void lg_launch_deposit_j(const LG23& L, const Grid23& g, const Species23& s, hip
     if (s.P > 0) by_shape<DepJL>(s.shape, L, g, s, st);
 }
 
-void lg_launch_finish(const LG23& L, hipStream_t st) {
-    k_lg_finish<false><<<grid_blocks((int64_t)L.NX * L.NY), 256, 0, st>>>(L);
+void lg_launch_finish(const LG23& L, bool centred, hipStream_t st) {
+    if (centred) k_lg_finish<2><<<grid_blocks((int64_t)L.NX * L.NY), 256, 0, st>>>(L);
+    else k_lg_finish<0><<<grid_blocks((int64_t)L.NX * L.NY), 256, 0, st>>>(L);
 }
 
 void lg_launch_advect(const LG23& L, const Grid23& g, const Species23& s, double h, hipStream_t st) {
@@ -633,7 +712,7 @@ void lg_launch_vmax(const LG23& L, const Species23& s, int sidx, bool clear, hip
 }
 
 void lg_launch_warm_finish(const LG23& L, hipStream_t st) {
-    k_lg_finish<true><<<grid_blocks((int64_t)L.NX * L.NY), 256, 0, st>>>(L);
+    k_lg_finish<1><<<grid_blocks((int64_t)L.NX * L.NY), 256, 0, st>>>(L);
 }
 
 // ---------------------------------------------------------------------------
@@ -787,12 +866,11 @@ void lg_launch_diag_fields(const LG23& L, const Diag23& d, hipStream_t st) {
 }
 
 int configure_kernels_2d3v_lg() {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_lg_cols_inv), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              160 * 1024);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_lg_rows_inv<true>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_lg_rows_inv<false>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    const void* ks[] = {reinterpret_cast<const void*>(k_lg_cols<true>), reinterpret_cast<const void*>(k_lg_cols<false>),
+                        reinterpret_cast<const void*>(k_lg_rows<true, true>),
+                        reinterpret_cast<const void*>(k_lg_rows<true, false>),
+                        reinterpret_cast<const void*>(k_lg_rows<false, false>)};
+    for (const void* k : ks) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)hipGetLastError();
     return 0;
 }

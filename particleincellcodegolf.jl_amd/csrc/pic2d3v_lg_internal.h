@@ -1,6 +1,6 @@
 // pic2d3v_lg_internal.h -- descriptors and launchers of the Lorenz-gauge
 // electromagnetic path of the structured 2D3V module (LorenzGaugeStaggeredField
-// of src/PIC2D3V.jl), shared by the host runtime (capi2d3v.cpp) and the
+// and LorenzGaugeField of src/PIC2D3V.jl), shared by the host runtime (capi2d3v.cpp) and the
 // kernels (kernels_2d3v_lg.hip).
 #pragma once
 #include "pic2d3v_internal.h"
@@ -40,17 +40,22 @@ struct LG23 {
 };
 
 // step 1-2 of loop!: the wave update, E and B in the spectrum, their inverse
-// transforms and update!
-void lg_launch_solve(const LG23& L, hipStream_t st);
-// step 3: gather, Boris, half move (xh, yh), half move; per-species maxima
+// transforms and update! (centred: LorenzGaugeField's, after the forward
+// transforms of the sources)
+void lg_launch_solve(const LG23& L, bool centred, hipStream_t st);
+// update! alone, from the state's six fields
+void lg_launch_update_eb(const LG23& L, hipStream_t st);
+// step 3: gather, Boris, half move (xh, yh), half move (centred: one move);
+// per-species maxima
 void lg_launch_push(const LG23& L, const Grid23& g, const Species23& s, int sidx, double* xh, double* yh,
-                    hipStream_t st);
+                    bool centred, hipStream_t st);
 // the charge at (xh, yh) and the currents at (x, y)
 void lg_launch_deposit_rho(const LG23& L, const Grid23& g, const Species23& s, const double* xh, const double* yh,
                            hipStream_t st);
 void lg_launch_deposit_j(const LG23& L, const Grid23& g, const Species23& s, hipStream_t st);
-// the reduction into rho0, J0 and the potentials' buffer copies (step 4)
-void lg_launch_finish(const LG23& L, hipStream_t st);
+// the reduction into rho0, J0 and the potentials' buffer copies (step 4;
+// centred: the reduction alone)
+void lg_launch_finish(const LG23& L, bool centred, hipStream_t st);
 // warmup!: advect by h, the velocity maxima, and the reduction (rho0 = 0)
 void lg_launch_advect(const LG23& L, const Grid23& g, const Species23& s, double h, hipStream_t st);
 void lg_launch_vmax(const LG23& L, const Species23& s, int sidx, bool clear, hipStream_t st);

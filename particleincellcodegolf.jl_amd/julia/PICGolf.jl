@@ -691,7 +691,7 @@ const SHAPES2D3V = Dict(:ngp => Int32(0), :area => Int32(1), :bspline0 => Int32(
                         :bspline2 => Int32(4), :bspline3 => Int32(5), :bspline4 => Int32(6), :bspline5 => Int32(7))
 const FIELD_UPDATES = Dict(:accumulate => Int32(0), :replace => Int32(1))
 const BORIS_VARIANTS = Dict(:reference => Int32(0), :textbook => Int32(1))
-const FIELDS2D3V = Dict(:electrostatic => Int32(0), :lorenz => Int32(1))
+const FIELDS2D3V = Dict(:electrostatic => Int32(0), :lorenz_staggered => Int32(1), :lorenz => Int32(2))
 const MAX_SPECIES_2D3V = 4
 # the Lorenz-gauge state slots and binned fields of include/pic2d3v.h
 const LG_STATE = (:ϕ⁰, :ϕ⁻, :Ax⁰, :Ay⁰, :Az⁰, :Ax⁻, :Ay⁻, :Az⁻, :ρ⁰, :Jx⁰, :Jy⁰, :Jz⁰, :Ex, :Ey, :Ez, :Bx, :By, :Bz)
@@ -745,9 +745,10 @@ species.  `field_update=:accumulate` is the reference's `update!`, which adds
 every step's field into `Exy` without clearing it; `:replace` gathers the
 step's own field.  `boris=:reference` is `ElectrostaticBoris` (rotation scaled
 by q_m^2); `:textbook` is `ElectromagneticBoris`'s form with Ez = 0.
-`field=:lorenz` builds `LorenzGaugeStaggeredField(NX, NY, Lx, Ly; dt, B0x, B0y,
-B0z, imex=ImEx(theta), buffer)` and `LorenzGaugeDiagnostics` instead
-(`theta=0` is `Explicit()`, `1` is `Implicit()`; src/2D3V.jl uses `ImEx(1)`).
+`field=:lorenz_staggered` builds `LorenzGaugeStaggeredField(NX, NY, Lx, Ly; dt,
+B0x, B0y, B0z, imex=ImEx(theta), buffer)` and `LorenzGaugeDiagnostics` instead
+(`theta=0` is `Explicit()`, `1` is `Implicit()`; src/2D3V.jl uses `ImEx(1)`);
+`field=:lorenz` builds a `LorenzGaugeField`, which runs with `loop!` only.
 """
 function Pic2D3VConfig(species::Vector{Pic2D3VSpecies}; NX, NY=NX, Lx=1.0, Ly=1.0, dt, B0x=0.0, B0y=0.0, B0z=0.0,
                        buffer=3, ntskip=1, ngskip=1, field_update=:accumulate, boris=:reference, device=0,
@@ -841,7 +842,7 @@ fieldmomentum (3) before characteristicmomentum.
 function run!(ctx::Context2D3V, nsteps::Integer)
     nt = ctx.cfg.ntskip
     rows = cld(ctx.t + nsteps, nt) - cld(ctx.t, nt)
-    R = zeros(ctx.cfg.field == FIELDS2D3V[:lorenz] ? 11 : 8, rows)
+    R = zeros(ctx.cfg.field == FIELDS2D3V[:lorenz_staggered] ? 11 : 8, rows)
     rc = GC.@preserve R ccall((:pic2d3v_run, libpicgolf), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}),
                               ctx.ptr, nsteps, rows > 0 ? pointer(R) : C_NULL)
     ctx.t += nsteps

@@ -120,7 +120,7 @@ class pic2d3v_config(ctypes.Structure):
     ]
 
 
-ELECTROSTATIC_2D3V, LORENZ_STAGGERED_2D3V = 0, 1
+ELECTROSTATIC_2D3V, LORENZ_STAGGERED_2D3V, LORENZ_2D3V = 0, 1, 2
 # Lorenz-gauge state slots and binned fields, in include/pic2d3v.h's order
 LG_STATE = ["phi0", "phim", "Ax0", "Ay0", "Az0", "Axm", "Aym", "Azm", "rho0", "Jx0", "Jy0", "Jz0",
             "Ex", "Ey", "Ez", "Bx", "By", "Bz"]

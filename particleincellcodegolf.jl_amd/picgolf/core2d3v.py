@@ -13,7 +13,8 @@ include/pic2d3v.h.
     d.Exs, d.Eys, d.phis                   :1323-1327               -> fields
     deposit!, (field)(shape, x, y), boris  :1249, :1217, :49        -> deposit, gather, boris
     LorenzGaugeStaggeredField(NX, NY, Lx, Ly; dt, B0x, B0y, B0z, imex, buffer)
-                                           :347-392                 -> Config2D3V(field="lorenz", theta)
+                                           :347-392                 -> Config2D3V(field="lorenz_staggered", theta)
+    LorenzGaugeField(...; imex, buffer)    :307-345                 -> Config2D3V(field="lorenz", theta)
     warmup!(field, plasma, to)             :826-859                 -> warmup
     loop! alone (no diagnose!)             :875-979                 -> loop
     the field's phi, A, rho, J, E, B       :347-378                 -> lg_get, lg_set
@@ -35,7 +36,8 @@ from . import _lib as L
 SHAPES = {"ngp": L.NGP_2D3V, "area": L.AREA_2D3V, **{f"bspline{n}": L.BSPLINE0_2D3V + n for n in range(6)}}
 FIELD_UPDATES = {"accumulate": L.FIELD_ACCUMULATE, "replace": L.FIELD_REPLACE}
 BORIS = {"reference": L.BORIS_REFERENCE, "textbook": L.BORIS_TEXTBOOK}
-FIELDS = {"electrostatic": L.ELECTROSTATIC_2D3V, "lorenz": L.LORENZ_STAGGERED_2D3V}
+FIELDS = {"electrostatic": L.ELECTROSTATIC_2D3V, "lorenz_staggered": L.LORENZ_STAGGERED_2D3V,
+          "lorenz": L.LORENZ_2D3V}
 
 
 def _code(table, v):
@@ -59,8 +61,9 @@ class Config2D3V:
     plasma's species.  field_update "accumulate" (the reference's update!,
     which never clears Exy) or "replace"; boris "reference"
     (ElectrostaticBoris) or "textbook" (ElectromagneticBoris with Ez = 0).
-    field "lorenz" builds a LorenzGaugeStaggeredField instead, with the ImEx
-    theta (0: Explicit(), 1: Implicit(); src/2D3V.jl uses ImEx(1))."""
+    field "lorenz_staggered" builds a LorenzGaugeStaggeredField instead, with
+    the ImEx theta (0: Explicit(), 1: Implicit(); src/2D3V.jl uses ImEx(1));
+    "lorenz" a LorenzGaugeField, run with loop() only."""
     NX: int
     NY: int
     dt: float
@@ -162,7 +165,7 @@ class Context2D3V:
         nt = self.cfg.ntskip
         t0, t1 = self.t, self.t + int(nsteps)
         n = -(-t1 // nt) - (-(-t0 // nt))
-        rows = np.zeros((max(n, 0), 11 if self.cfg.field == "lorenz" else 8))
+        rows = np.zeros((max(n, 0), 11 if self.cfg.field == "lorenz_staggered" else 8))
         rc = self._lib.pic2d3v_run(self._h, int(nsteps), L.ptr(rows))
         self.t = t1
         self._check(rc, "pic2d3v_run")

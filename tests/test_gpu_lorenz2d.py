@@ -4,7 +4,9 @@ through the C ABI in include/pic2d3v.h: warmup!, steps of loop! with and
 without diagnose! (rows, the 18 state arrays, the particles and the fourteen
 binned fields) bit for bit against the CPU restatement, for every shape and
 for explicit, semi-implicit and implicit theta; a step from an arbitrary
-complex state; a 128 x 128 run with 2 x 2^18 particles; and, at 512 x 512
+complex state; a 128 x 128 run with 2 x 2^18 particles; LorenzGaugeField's
+centred loop! (every shape tested, theta 0 and 1, from a fresh and from an
+arbitrary state); and, at 512 x 512
 with 2 x 2^22 particles, the size-independent properties that the fields do
 not depend on the particle order (the LDS-window deposit against the global
 fallback) and that two calls equal one."""
@@ -38,12 +40,12 @@ def make_species(P, Lx, Ly, shape="bspline2", seed=0, sort_dxdy=None):
     return out
 
 
-def context(pg, NX, NY, Lx, Ly, dt, B0, species, theta=1.0, buffer=3, ntskip=1, ngskip=1):
+def context(pg, NX, NY, Lx, Ly, dt, B0, species, theta=1.0, buffer=3, ntskip=1, ngskip=1, field="lorenz_staggered"):
     from picgolf.core2d3v import Config2D3V, SpeciesConfig
     sc = [SpeciesConfig(P=s["xyv"].shape[1], shape=s["shape"], charge=s["charge"], mass=s["mass"],
                         weight=s["weight"]) for s in species]
     cfg = Config2D3V(NX=NX, NY=NY, dt=dt, species=sc, Lx=Lx, Ly=Ly, B0=tuple(B0), buffer=buffer, ntskip=ntskip,
-                     ngskip=ngskip, field="lorenz", theta=theta)
+                     ngskip=ngskip, field=field, theta=theta)
     c = pg.Context2D3V(cfg)
     for k, s in enumerate(species):
         c.upload(k, s["xyv"])
@@ -173,3 +175,41 @@ def test_order_independence_and_split_calls_at_scale(pg):
         r1 = c.run(1)
         r2 = c.run(1)
         assert same(np.concatenate([r1, r2]), a[0]) and same(c.lg_state(), a[1])
+
+
+@pytest.mark.parametrize("shape,theta,dt", [("bspline2", 1.0, 0.02), ("area", 0.0, 0.002), ("bspline3", 0.5, 0.01),
+                                            ("ngp", 1.0, 0.01)])
+def test_centred_loop_bitwise(pg, orc, shape, theta, dt):
+    """LorenzGaugeField: a fresh field (EBxyz zero until the first update!),
+    then loop! over two calls."""
+    NX, NY, Lx, Ly = 32, 16, 1.0, 2.0
+    B0 = (0.4, 0.0, 0.3)
+    sp = make_species(1536, Lx, Ly, shape=shape, seed=7)
+    with context(pg, NX, NY, Lx, Ly, dt, B0, sp, theta=theta, field="lorenz") as c:
+        c.warmup()                                   # warmup!(::AbstractField): nothing
+        c.loop(2)
+        c.loop(2)
+        F0, EB0 = np.zeros((18, NY, NX), dtype=complex), np.zeros((6, NY, NX))
+        sp1, F1, EB1 = orc.lg_run_centred(NX, NY, Lx, Ly, dt, np.array(B0), sp, F0, EB0, 4, theta=theta)
+        check_state(c, sp1, F1)
+        Ex, Ey = c.field()
+        assert same(Ex, EB1[0]) and same(Ey, EB1[1])
+        with pytest.raises(pg.PicError):
+            c.run(1)
+
+
+def test_centred_from_an_arbitrary_state(pg, orc):
+    NX, NY, Lx, Ly, dt = 16, 32, 2.0, 1.0, 0.01
+    B0 = (0.3, -0.2, 0.7)
+    sp = make_species(512, Lx, Ly, seed=8)
+    rng = np.random.default_rng(8)
+    F = 0.1 * (rng.standard_normal((18, NY, NX)) + 1j * rng.standard_normal((18, NY, NX)))
+    with context(pg, NX, NY, Lx, Ly, dt, B0, sp, field="lorenz") as c:
+        for k in range(18):
+            c.lg_set(k, F[k])
+        c.loop(3)
+        EB = orc.lg_update(F, np.array(B0))
+        sp1, F1, EB1 = orc.lg_run_centred(NX, NY, Lx, Ly, dt, np.array(B0), sp, F, EB, 3, theta=1.0)
+        check_state(c, sp1, F1)
+        Ex, Ey = c.field()
+        assert same(Ex, EB1[0]) and same(Ey, EB1[1])

@@ -309,3 +309,137 @@ def test_two_calls_equal_one(orc):
     assert np.array_equal(b[1], whole[1])
     assert np.array_equal(np.concatenate([a[2], b[2]]), whole[2])
     assert np.array_equal(b[3], whole[3])
+
+
+# ---- LorenzGaugeField (the centred scheme, :665-745) ------------------------
+
+def test_centred_single_mode(orc):
+    """phi+ = cos, phi0 = 0, no source: the new phi+ is r cos and phi0 the old
+    phi+, so Ex = -d((phi+ + phi0)/2)/dx = (r + 1)/2 k sin; Az+ = cos gives
+    Ez = -(r - 1)/dt cos and By = (r + 1)/2 k sin."""
+    NX, NY, Lx, Ly, dt = 16, 8, 2.0, 0.5, 0.01
+    for theta in (0.0, 1.0):
+        for mx, my in ((1, 0), (3, 0), (0, 2)):
+            kx0, ky0 = 2 * np.pi * mx / Lx, 2 * np.pi * my / Ly
+            k2 = kx0 ** 2 + ky0 ** 2
+            r = (2 - dt ** 2 * k2 * (1 - theta)) / (1 + dt ** 2 * k2 * theta / 2)
+            c, s = mode(NX, NY, Lx, Ly, mx, my), mode(NX, NY, Lx, Ly, mx, my, "sin")
+            F = np.zeros((18, NY, NX), dtype=complex)
+            F[0] = np.fft.fft2(c)
+            F1 = orc.lg_solve_centred(F, dt, theta, Lx, Ly)
+            assert np.allclose(F1[0], r * F[0], atol=1e-11) and np.array_equal(F1[1], F[0])
+            assert np.allclose(F1[12].real, (r + 1) / 2 * kx0 * s, atol=1e-11)
+            assert np.allclose(F1[13].real, (r + 1) / 2 * ky0 * s, atol=1e-11)
+            F = np.zeros((18, NY, NX), dtype=complex)
+            F[4] = np.fft.fft2(c)
+            F1 = orc.lg_solve_centred(F, dt, theta, Lx, Ly)
+            assert np.allclose(F1[14].real, -(r - 1) / dt * c, atol=1e-9)
+            assert np.allclose(F1[16].real, (r + 1) / 2 * kx0 * s, atol=1e-11)
+
+
+@pytest.mark.parametrize("theta,dt", [(1.0, 0.05), (0.0, 0.005)])
+def test_centred_vacuum_recurrence(orc, theta, dt):
+    NX, NY, Lx, Ly, T, m = 16, 8, 1.0, 1.0, 50, 5
+    k2 = (2 * np.pi * m / Lx) ** 2
+    r = (2 - dt ** 2 * k2 * (1 - theta)) / (1 + dt ** 2 * k2 * theta / 2)
+    F = np.zeros((18, NY, NX), dtype=complex)
+    F[0, 0, m] = 1.0
+    F[1, 0, m] = 0.5
+    sp = inert_species(Lx, Ly)
+    _, F1, _ = orc.lg_run_centred(NX, NY, Lx, Ly, dt, [0, 0, 0], sp, F, np.zeros((6, NY, NX)), T, theta=theta)
+    s = np.linalg.matrix_power(np.array([[r, -1.0], [1.0, 0.0]]), T) @ np.array([1.0, 0.5])
+    assert np.allclose([F1[0, 0, m], F1[1, 0, m]], s, rtol=1e-9, atol=1e-12)
+    # the sources were transformed and their [1, 1] cleared
+    assert not np.any(F1[8:12])
+
+
+def test_centred_particles_move_before_the_first_update(orc):
+    """The particle loop comes first and EBxyz is zero in a fresh field:
+    uncharged particles stream in the first step and turn about B0 only in
+    the second (the Boris turn 2 atan(|t|))."""
+    NX, NY, Lx, Ly, dt = 16, 16, 1.0, 1.0, 0.01
+    B0 = np.array([0.0, 0.0, 3.0])
+    sp = inert_species(Lx, Ly, P=8, seed=11)
+    x0 = sp[0]["xyv"].copy()
+    F = np.zeros((18, NY, NX), dtype=complex)
+    sp1, F1, EB1 = orc.lg_run_centred(NX, NY, Lx, Ly, dt, B0, sp, F, np.zeros((6, NY, NX)), 1)
+    v = x0[2:]
+    assert np.array_equal(sp1[0]["xyv"][2:], v)
+    assert np.array_equal(sp1[0]["xyv"][0], [orc.unimod23(a + (b + b) / 2 * dt, Lx) for a, b in zip(x0[0], v[0])])
+    assert np.array_equal(EB1[5], np.full((NY, NX), 3.0)) and not np.any(EB1[:5])
+    sp2, _, _ = orc.lg_run_centred(NX, NY, Lx, Ly, dt, B0, sp1, F1, EB1, 1)
+    S = np.array([orc.weights23("bspline2", x, NX / Lx)[1] for x in sp1[0]["xyv"][0]]).sum(axis=1)
+    S *= np.array([sum(orc.weights23("bspline2", y, NY / Ly)[1]) for y in sp1[0]["xyv"][1]])
+    phi = 2 * np.arctan(3.0 * S * dt / 2)
+    z0 = v[0] + 1j * v[1]
+    assert np.allclose(sp2[0]["xyv"][2] + 1j * sp2[0]["xyv"][3], z0 * np.exp(-1j * phi), rtol=0, atol=1e-14)
+
+
+def numpy_steps_centred(orc, NX, NY, Lx, Ly, dt, B0, theta, species, F, EB, T):
+    kx = (2 * np.pi / Lx) * np.concatenate([np.arange(NX // 2), np.arange(-NX // 2, 0)])
+    ky = (2 * np.pi / Ly) * np.concatenate([np.arange(NY // 2), np.arange(-NY // 2, 0)])
+    KX, KY = kx[None, :], ky[:, None]
+    k2 = KX ** 2 + KY ** 2
+    dV = (Lx / NX) * (Ly / NY)
+    F, EB = F.copy(), EB.copy()
+    sp = [dict(s, xyv=s["xyv"].copy()) for s in species]
+    for _ in range(T):
+        rho, J = np.zeros((NY, NX)), np.zeros((3, NY, NX))
+        for s in sp:
+            x, y, vx, vy, vz = s["xyv"]
+            qw = s["charge"] * s["weight"] / dV
+            q_m = s["charge"] / s["mass"]
+            for p in range(x.size):
+                g = np.array([orc.gather23(EB[c], EB[c + 1], x[p], y[p], s["shape"], Lx, Ly) for c in (0, 2, 4)]).ravel()
+                u = np.array([vx[p], vy[p], vz[p]])
+                v = numpy_boris(dt, q_m, g[:3], g[3:], u)
+                vx[p], vy[p], vz[p] = v
+                x[p] = orc.unimod23(x[p] + (u[0] + v[0]) / 2 * dt, Lx)
+                y[p] = orc.unimod23(y[p] + (u[1] + v[1]) / 2 * dt, Ly)
+                cx, wx = orc.weights23(s["shape"], x[p], NX / Lx)
+                cy, wy = orc.weights23(s["shape"], y[p], NY / Ly)
+                for j, b in zip(cy, wy):
+                    for i, a in zip(cx, wx):
+                        k = ((j - 1) % NY, (i - 1) % NX)
+                        rho[k] += a * b * qw
+                        J[:, k[0], k[1]] += a * b * v * qw
+        src = [np.fft.fft2(rho)] + [np.fft.fft2(J[c]) for c in range(3)]
+        for a in src:
+            a[0, 0] = 0
+        num = 2 - dt ** 2 * k2 * (1 - theta)
+        den = 1 + dt ** 2 * k2 * theta / 2
+        for a, (xp, x0) in enumerate(((0, 1), (2, 5), (3, 6), (4, 7))):
+            new = (num * F[xp] + dt ** 2 * src[a]) / den - F[x0]
+            F[x0] = F[xp]
+            F[xp] = new
+        F[8:12] = src
+        ph = F[0] + F[1]
+        E = [-1j * KX * ph / 2 - (F[2] - F[5]) / dt, -1j * KY * ph / 2 - (F[3] - F[6]) / dt, -(F[4] - F[7]) / dt]
+        B = [1j * KY * (F[4] + F[7]) / 2, -1j * KX * (F[4] + F[7]) / 2,
+             1j * KX * (F[3] + F[6]) / 2 - 1j * KY * (F[2] + F[5]) / 2]
+        for c in range(3):
+            F[12 + c] = np.fft.ifft2(E[c])
+            F[15 + c] = np.fft.ifft2(B[c])
+        EB = np.array([F[12 + c].real + (B0[c - 3] if c >= 3 else 0.0) for c in range(6)])
+    return sp, F, EB
+
+
+def test_centred_steps_against_numpy(orc):
+    NX, NY, Lx, Ly, dt, T = 16, 16, 1.0, 2.0, 0.02, 3
+    B0 = np.array([0.6, 0.0, 0.2])
+    sp = random_species(Lx, Ly, P=48, seed=12)
+    F, EB = np.zeros((18, NY, NX), dtype=complex), np.zeros((6, NY, NX))
+    o = orc.lg_run_centred(NX, NY, Lx, Ly, dt, B0, sp, F, EB, T, theta=1.0)
+    n = numpy_steps_centred(orc, NX, NY, Lx, Ly, dt, B0, 1.0, sp, F, EB, T)
+    for a, b in zip(o[0], n[0]):
+        assert np.allclose(a["xyv"], b["xyv"], rtol=1e-10, atol=1e-13)
+    scale = np.abs(n[1]).max(axis=(1, 2), keepdims=True) + 1e-300
+    assert np.all(np.abs(o[1] - n[1]) <= 1e-9 * scale)
+    assert np.allclose(o[2], n[2], rtol=1e-9, atol=1e-12)
+    assert np.abs(o[1][12:]).max() > 0
+    # two calls equal one
+    a = orc.lg_run_centred(NX, NY, Lx, Ly, dt, B0, sp, F, EB, 1, theta=1.0)
+    b = orc.lg_run_centred(NX, NY, Lx, Ly, dt, B0, a[0], a[1], a[2], T - 1, theta=1.0)
+    assert np.array_equal(b[1], o[1]) and np.array_equal(b[2], o[2])
+    for p, q in zip(b[0], o[0]):
+        assert np.array_equal(p["xyv"], q["xyv"])
