@@ -729,7 +729,8 @@ extern "C" int pic2d3v_run(pic2d3v_ctx* c, int64_t nsteps, double* rows) {
 
 extern "C" int pic2d3v_fields(pic2d3v_ctx* c, double* Exs, double* Eys, double* phis, int64_t nbins) {
     if (!c) return PIC_EINVAL;
-    const int64_t have = cdiv(c->step, c->cfg.ntskip);
+    // bins exist for the steps pic2d3v_run took (pic2d3v_loop writes none)
+    const int64_t have = std::min<int64_t>(cdiv(c->step, c->cfg.ntskip), c->lorenz ? c->lg.cap : c->d.cap);
     if (nbins < 0 || nbins > have) {
         set_err(c, "pic2d3v_fields: nbins must be in [0, %lld]", (long long)have);
         return PIC_EINVAL;

@@ -188,18 +188,18 @@ __device__ __forceinline__ void ik_of(int N, int j, double& kr, double& ki) {
     ki = kTwoPi * m;
 }
 
-// One sweep's field updates (:20-27) in one block.  SRC 0: rho, Jy, Jz from
-// the exact limbs (cleared here); 1: from the arrays rin, jyin, jzin.  RUN:
-// the step's saves at the first sweep, Fx .= Ex, and the test that decides
-// the next sweep.
-template <int SRC, bool RUN>
-__global__ void __launch_bounds__(kEMSolveThreads)
-k_em_solve(EM1D s, int sweep, const double* __restrict__ rin, const double* __restrict__ jyin,
-           const double* __restrict__ jzin) {
-    extern __shared__ double sm[];
-    if (RUN && s.ctl[sweep] == 0) return;
+// One sweep's field updates (:20-27) by one block of any size with
+// N <= JPT*blockDim.x.  SRC 0: rho, Jy, Jz from the exact limbs g (rho, Jy, Jz
+// at 0, 3N, 6N, three limbs each; cleared here) with fraction bits q0..q2; 1:
+// from the arrays rin, jyin, jzin.  RUN: the step's saves at the first sweep,
+// Fx .= Ex, and the return value is the test of the next sweep (:14).  sm:
+// em_solve_lds_bytes(N) of LDS.
+template <int SRC, bool RUN, int JPT>
+__device__ bool em_solve_body(const EM1D& s, int sweep, u64* g, const double* __restrict__ rin,
+                              const double* __restrict__ jyin, const double* __restrict__ jzin, int q0, int q1, int q2,
+                              double* sm) {
     const int N = s.N, lN = s.lN, st = em_stride(N);
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, nth = blockDim.x;
     double* re = sm;
     double* im = sm + 5 * st;
     double* red = sm + 10 * st;
@@ -214,18 +214,11 @@ k_em_solve(EM1D s, int sweep, const double* __restrict__ rin, const double* __re
             }
         for (int j = t; j < N; j += blockDim.x) s.Fx[j] = s.Ex[j];  // Fx .= Ex (:14)
     }
-    int q0 = 0, q1 = 0, q2 = 0;
-    if (SRC == 0) {
-        q0 = em_frac_bits(s.P, s.w, 1.0);
-        q1 = em_frac_bits(s.P, s.w, bits_d(s.umax[0]));
-        q2 = em_frac_bits(s.P, s.w, bits_d(s.umax[1]));
-    }
     __syncthreads();
     // lines: 0 rho, 1 bz, 2 Jy, 3 by, 4 Jz (bit-reversed, imaginary zero)
     for (int j = t; j < N; j += blockDim.x) {
         double r, jy, jz;
         if (SRC == 0) {
-            u64* g = s.racc;
             r = em_sum(g[j], g[N + j], g[2 * N + j], q0);
             jy = em_sum(g[3 * N + j], g[4 * N + j], g[5 * N + j], q1);
             jz = em_sum(g[6 * N + j], g[7 * N + j], g[8 * N + j], q2);
@@ -247,11 +240,10 @@ k_em_solve(EM1D s, int sweep, const double* __restrict__ rin, const double* __re
     __syncthreads();
     fft_lines(re, im, s.tw, lN, 5, st, false);
     // Gauss, Ampere y, Ampere z in the spectrum
-    constexpr int JPT = kMaxNEM / kEMSolveThreads;
     double o[JPT][6];
 #pragma unroll
     for (int q = 0; q < JPT; ++q) {
-        const int j = t + q * kEMSolveThreads;
+        const int j = t + q * nth;
         if (j >= N) break;
         const int pj = padi(j);
         double kr, ki;
@@ -275,7 +267,7 @@ k_em_solve(EM1D s, int sweep, const double* __restrict__ rin, const double* __re
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < JPT; ++q) {
-        const int j = t + q * kEMSolveThreads;
+        const int j = t + q * nth;
         if (j >= N) break;
         const int pos = padi((int)brev((uint32_t)j, lN));
 #pragma unroll
@@ -289,7 +281,7 @@ k_em_solve(EM1D s, int sweep, const double* __restrict__ rin, const double* __re
     double ezr[JPT], eyr[JPT];
 #pragma unroll
     for (int q = 0; q < JPT; ++q) {
-        const int j = t + q * kEMSolveThreads;
+        const int j = t + q * nth;
         if (j >= N) break;
         const int pj = padi(j);
         const double Ex = re[pj] * sc;
@@ -308,7 +300,7 @@ k_em_solve(EM1D s, int sweep, const double* __restrict__ rin, const double* __re
     // Faraday: lines 0 ez, 1 ey
 #pragma unroll
     for (int q = 0; q < JPT; ++q) {
-        const int j = t + q * kEMSolveThreads;
+        const int j = t + q * nth;
         if (j >= N) break;
         const int pos = padi((int)brev((uint32_t)j, lN));
         re[pos] = ezr[q];
@@ -320,7 +312,7 @@ k_em_solve(EM1D s, int sweep, const double* __restrict__ rin, const double* __re
     fft_lines(re, im, s.tw, lN, 2, st, false);
 #pragma unroll
     for (int q = 0; q < JPT; ++q) {
-        const int j = t + q * kEMSolveThreads;
+        const int j = t + q * nth;
         if (j >= N) break;
         const int pj = padi(j);
         double kr, ki;
@@ -331,7 +323,7 @@ k_em_solve(EM1D s, int sweep, const double* __restrict__ rin, const double* __re
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < JPT; ++q) {
-        const int j = t + q * kEMSolveThreads;
+        const int j = t + q * nth;
         if (j >= N) break;
         const int pos = padi((int)brev((uint32_t)j, lN));
         re[pos] = o[q][0];
@@ -350,19 +342,35 @@ k_em_solve(EM1D s, int sweep, const double* __restrict__ rin, const double* __re
         s.by[j] = (By + s.Cy[j]) / 2.0;
         s.bz[j] = (Bz + s.Cz[j]) / 2.0;
     }
-    if (!RUN) return;
+    if (!RUN) return false;
     __syncthreads();
     // isapprox(Fx, Ex): the test at the top of the next sweep (:14)
     double me;
-    const bool conv = block_isapprox(red, [&](int i) { return s.Fx[i]; }, [&](int i) { return s.Ex[i]; }, N,
-                                     s.rtol, 0.0, me);
-    if (t == 0) {
+    return block_isapprox(red, [&](int i) { return s.Fx[i]; }, [&](int i) { return s.Ex[i]; }, N, s.rtol, 0.0, me);
+}
+
+template <int SRC, bool RUN>
+__global__ void __launch_bounds__(kEMSolveThreads)
+k_em_solve(EM1D s, int sweep, const double* __restrict__ rin, const double* __restrict__ jyin,
+           const double* __restrict__ jzin) {
+    extern __shared__ double sm[];
+    if (RUN && s.ctl[sweep] == 0) return;
+    int q0 = 0, q1 = 0, q2 = 0;
+    if (SRC == 0) {
+        q0 = em_frac_bits(s.P, s.w, 1.0);
+        q1 = em_frac_bits(s.P, s.w, bits_d(s.umax[0]));
+        q2 = em_frac_bits(s.P, s.w, bits_d(s.umax[1]));
+    }
+    const bool conv =
+        em_solve_body<SRC, RUN, kMaxNEM / kEMSolveThreads>(s, sweep, s.racc, rin, jyin, jzin, q0, q1, q2, sm);
+    if (RUN && threadIdx.x == 0) {
         s.ctl[kEMMaxSweeps] += 1;
         if (sweep + 1 < s.max_sweeps) s.ctl[sweep + 1] = conv ? 0 : 1;
         s.umax[0] = 0;
         s.umax[1] = 0;
     }
 }
+
 
 // v = u + (a + cross(u + a*dt/2, b))*dt (:28-29) with a, b gathered at c.
 __device__ __forceinline__ void em_velocity(const double* fe, int N, double Nd, double c, double dt, double u1,
@@ -717,32 +725,24 @@ __global__ void __launch_bounds__(kEMThreads) k_emx_begin(EM1D s) {
     }
 }
 
-// One block.  PHASE 0 (:13, :15-16): Ex = Gauss(rho); By .-= dt*rif(-ik.*
-// fft(Ez)); Bz .-= dt*rif(ik.*fft(Ey)); by, bz = (B + C)/2 with C the old B.
-// PHASE 1 (:23-24): Ey .+= dt*rif(-ik.*fft(Bz)/eps^2 .- fft(Jy)); Ez .+=
-// dt*rif(+ik.*fft(By)/eps^2 .- fft(Jz)).  SRC 0: rho, Jy, Jz from the exact
-// limbs (cleared here); 1: from the arrays rin, jyin, jzin.
-template <int PHASE, int SRC>
-__global__ void __launch_bounds__(kEMSolveThreads)
-k_emx_solve(EM1D s, const double* __restrict__ rin, const double* __restrict__ jyin,
-            const double* __restrict__ jzin) {
-    extern __shared__ double sm[];
+// One block of any size with N <= JPT*blockDim.x.  PHASE 0 (:13, :15-16):
+// Ex = Gauss(rho); By .-= dt*rif(-ik.*fft(Ez)); Bz .-= dt*rif(ik.*fft(Ey));
+// by, bz = (B + C)/2 with C the old B.  PHASE 1 (:23-24): Ey .+= dt*rif(-ik.*
+// fft(Bz)/eps^2 .- fft(Jy)); Ez .+= dt*rif(+ik.*fft(By)/eps^2 .- fft(Jz)).
+// SRC 0: rho, Jy, Jz from the exact limbs g (rho at 0, Jy at 3N, Jz at 6N;
+// cleared here) with fraction bits q0..q2; 1: from the arrays rin, jyin, jzin.
+template <int PHASE, int SRC, int JPT>
+__device__ void emx_solve_body(const EM1D& s, u64* g, const double* __restrict__ rin, const double* __restrict__ jyin,
+                               const double* __restrict__ jzin, int q0, int q1, int q2, double* sm) {
     const int N = s.N, lN = s.lN, st = em_stride(N);
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, nth = blockDim.x;
     double* re = sm;
     double* im = sm + 5 * st;
     const double Nd = (double)N, sc = 1.0 / Nd, dt = s.dt, e2 = s.eps * s.eps;
     constexpr int NL = PHASE == 0 ? 3 : 4;
-    int q0 = 0, q1 = 0, q2 = 0;
-    if (SRC == 0) {
-        q0 = em_frac_bits(s.P, s.w, 1.0);
-        q1 = em_frac_bits(s.P, s.w, bits_d(s.umax[3]));
-        q2 = em_frac_bits(s.P, s.w, bits_d(s.umax[4]));
-    }
     // PHASE 0 lines: rho, Ez, Ey; PHASE 1: Bz, Jy, By, Jz
     for (int j = t; j < N; j += blockDim.x) {
         double v[NL];
-        u64* g = s.racc;
         if (PHASE == 0) {
             if (SRC == 0) {
                 v[0] = em_sum(g[j], g[N + j], g[2 * N + j], q0);
@@ -773,12 +773,11 @@ k_emx_solve(EM1D s, const double* __restrict__ rin, const double* __restrict__ j
     }
     __syncthreads();
     fft_lines(re, im, s.tw, lN, NL, st, false);
-    constexpr int JPT = kMaxNEM / kEMSolveThreads;
     constexpr int NO = PHASE == 0 ? 3 : 2;
     double o[JPT][2 * NO];
 #pragma unroll
     for (int q = 0; q < JPT; ++q) {
-        const int j = t + q * kEMSolveThreads;
+        const int j = t + q * nth;
         if (j >= N) break;
         const int pj = padi(j);
         double kr, ki, pr, pi;
@@ -806,7 +805,7 @@ k_emx_solve(EM1D s, const double* __restrict__ rin, const double* __restrict__ j
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < JPT; ++q) {
-        const int j = t + q * kEMSolveThreads;
+        const int j = t + q * nth;
         if (j >= N) break;
         const int pos = padi((int)brev((uint32_t)j, lN));
 #pragma unroll
@@ -833,8 +832,23 @@ k_emx_solve(EM1D s, const double* __restrict__ rin, const double* __restrict__ j
             s.Ez[j] = s.Ez[j] + dt * (re[st + pj] * sc);
         }
     }
-    if (PHASE == 1 && SRC == 0 && t == 0) s.ctl[kEMMaxSweeps] = 1;
 }
+
+template <int PHASE, int SRC>
+__global__ void __launch_bounds__(kEMSolveThreads)
+k_emx_solve(EM1D s, const double* __restrict__ rin, const double* __restrict__ jyin,
+            const double* __restrict__ jzin) {
+    extern __shared__ double sm[];
+    int q0 = 0, q1 = 0, q2 = 0;
+    if (SRC == 0) {
+        q0 = em_frac_bits(s.P, s.w, 1.0);
+        q1 = em_frac_bits(s.P, s.w, bits_d(s.umax[3]));
+        q2 = em_frac_bits(s.P, s.w, bits_d(s.umax[4]));
+    }
+    emx_solve_body<PHASE, SRC, kMaxNEM / kEMSolveThreads>(s, s.racc, rin, jyin, jzin, q0, q1, q2, sm);
+    if (PHASE == 1 && SRC == 0 && threadIdx.x == 0) s.ctl[kEMMaxSweeps] = 1;
+}
+
 
 // v += a*dt/2; v += cross(v, b)*dt; v += a*dt/2 with a, b gathered at x
 // (:17-22; fe: Ex, Ey, Ez, by, bz).

@@ -69,12 +69,14 @@ def test_fields_and_push_bitwise(pg, orc):
         assert same(vv, ref)
 
 
+@pytest.mark.parametrize("P", [1024, 8192])
 @pytest.mark.parametrize("sort", [1, 0])
-def test_run_bitwise_over_two_calls(pg, orc, sort):
+def test_run_bitwise_over_two_calls(pg, orc, sort, P):
+    """The scripts' P = 1024, and 8192 (eight deposit blocks per sweep)."""
     from picgolf.core_em1d import ConfigEM1D
-    N, eps, dt, w, x, v = setup()
+    N, eps, dt, w, x, v = setup(P=P)
     T1, T2 = 5, 7
-    with pg.ContextEM1D(ConfigEM1D(N=N, P=1024, eps=eps, dt=dt, w=w, sort=bool(sort))) as c:
+    with pg.ContextEM1D(ConfigEM1D(N=N, P=P, eps=eps, dt=dt, w=w, sort=bool(sort))) as c:
         c.upload(x, v)
         f1, s1 = c.run(T1)
         f2, s2 = c.run(T2)

@@ -59,11 +59,13 @@ def test_fields_and_push_bitwise(pg, orc):
         assert same(vv[:, :2], u[:, :2] + 0.0)
 
 
+@pytest.mark.parametrize("P", [1024, 8192])
 @pytest.mark.parametrize("sort", [1, 0])
-def test_run_bitwise_over_two_calls(pg, orc, sort):
-    N, eps, dt, w, x, v = setup()
+def test_run_bitwise_over_two_calls(pg, orc, sort, P):
+    """The scripts' P = 1024, and 8192 (eight deposit blocks per sweep)."""
+    N, eps, dt, w, x, v = setup(P=P)
     T1, T2 = 4, 6
-    with pg.ContextEM1D(cfg(N, 1024, eps, dt, w, sort=bool(sort))) as c:
+    with pg.ContextEM1D(cfg(N, P, eps, dt, w, sort=bool(sort))) as c:
         c.upload(x, v)
         f1, s1 = c.run(T1)
         f2, s2 = c.run(T2)
