@@ -377,6 +377,36 @@ void orclg_row(int32_t NX, int32_t NY, int32_t buffer, const double* F, const do
     free(t);
 }
 
+/* diagnose!(d, field, plasma, t) (:1332-1398) on the state F, EB after a step */
+static void lg_diagnose(int32_t NX, int32_t NY, int32_t buffer, int32_t ntskip, int32_t ngskip, double* F,
+                        const double* EB, int32_t ns, const int64_t* P, const double* mass, const double* weight,
+                        const double* state, int64_t t, double* rows, int64_t* nrow, double* bins, int64_t nbins) {
+    const int64_t G = (int64_t)NX * NY;
+    const int32_t NXd = NX / ngskip, NYd = NY / ngskip;
+    if (t % ntskip == 0 && rows) orclg_row(NX, NY, buffer, F, EB, ns, P, mass, weight, state, rows + 11 * (*nrow)++);
+    static const int tr[8] = {ORCLG_AX0, ORCLG_AY0, ORCLG_AZ0, ORCLG_PHI0, ORCLG_RHO0, ORCLG_JX0, ORCLG_JY0, ORCLG_JZ0};
+    for (int a = 0; a < 8; ++a) ifft2_inplace(NX, NY, F + (size_t)tr[a] * 2 * G);
+    const int64_t b = t / ntskip;
+    if (!bins || b >= nbins) return;
+    static const int av[14] = {ORCLG_EX,  ORCLG_EY,  ORCLG_EZ,  ORCLG_BX,   ORCLG_BY,   ORCLG_BZ,  ORCLG_AX0,
+                               ORCLG_AY0, ORCLG_AZ0, ORCLG_PHI0, ORCLG_RHO0, ORCLG_JX0, ORCLG_JY0, ORCLG_JZ0};
+    const double factor = 1.0 / (double)((int64_t)ntskip * ngskip * ngskip);
+    for (int a = 0; a < 14; ++a) {
+        const double* z = F + (size_t)av[a] * 2 * G;
+        double* o = bins + ((size_t)a * nbins + b) * (size_t)NXd * NYd;
+        for (int32_t jl = 0; jl < NYd; ++jl)
+            for (int32_t il = 0; il < NXd; ++il) {
+                double acc2 = o[il + (int64_t)NXd * jl];
+                for (int32_t jj = 0; jj < ngskip; ++jj)
+                    for (int32_t ii = 0; ii < ngskip; ++ii) {
+                        const int64_t k = (int64_t)il * ngskip + ii + (int64_t)NX * ((int64_t)jl * ngskip + jj);
+                        acc2 = acc2 + z[2 * k] * factor;
+                    }
+                o[il + (int64_t)NXd * jl] = acc2;
+            }
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Time loop: for t = t0 .. t0+nsteps-1, loop!(plasma, field) (:875-979) and, */
 /* if diagnose, diagnose!(d, field, plasma, t) (:1332-1398):                 */
@@ -403,7 +433,6 @@ int orclg_run(int32_t NX, int32_t NY, double Lx, double Ly, double dt, const dou
     double qw[8];
     for (int s = 0; s < ns; ++s) qw[s] = charge[s] * weight[s] / dV;
     const int qr = orclg_frac_bits(ns, P, qw, NULL);
-    const int32_t NXd = NX / ngskip, NYd = NY / ngskip;
     double* plus = (double*)malloc(sizeof(double) * 8 * (size_t)G);
     double* EB = (double*)malloc(sizeof(double) * 6 * (size_t)G);
     i128* acc = (i128*)malloc(sizeof(i128) * (size_t)G);
@@ -459,32 +488,9 @@ int orclg_run(int32_t NX, int32_t NY, double Lx, double Ly, double dt, const dou
             memcpy(F + (size_t)xms[a] * 2 * G, F + (size_t)x0s[a] * 2 * G, sizeof(double) * 2 * (size_t)G);
             memcpy(F + (size_t)x0s[a] * 2 * G, plus + (size_t)a * 2 * G, sizeof(double) * 2 * (size_t)G);
         }
-        if (!diagnose) continue;
-        if (t % ntskip == 0 && rows)
-            orclg_row(NX, NY, buffer, F, EB, ns, P, mass, weight, state, rows + 11 * nrow++);
-        static const int tr[8] = {ORCLG_AX0, ORCLG_AY0, ORCLG_AZ0, ORCLG_PHI0,
-                                  ORCLG_RHO0, ORCLG_JX0, ORCLG_JY0, ORCLG_JZ0};
-        for (int a = 0; a < 8; ++a) ifft2_inplace(NX, NY, F + (size_t)tr[a] * 2 * G);
-        const int64_t b = t / ntskip;
-        if (!bins || b >= nbins) continue;
-        static const int av[14] = {ORCLG_EX,  ORCLG_EY,   ORCLG_EZ,  ORCLG_BX,  ORCLG_BY,
-                                   ORCLG_BZ,  ORCLG_AX0,  ORCLG_AY0, ORCLG_AZ0, ORCLG_PHI0,
-                                   ORCLG_RHO0, ORCLG_JX0, ORCLG_JY0, ORCLG_JZ0};
-        const double factor = 1.0 / (double)((int64_t)ntskip * ngskip * ngskip);
-        for (int a = 0; a < 14; ++a) {
-            const double* z = F + (size_t)av[a] * 2 * G;
-            double* o = bins + ((size_t)a * nbins + b) * (size_t)NXd * NYd;
-            for (int32_t jl = 0; jl < NYd; ++jl)
-                for (int32_t il = 0; il < NXd; ++il) {
-                    double acc2 = o[il + (int64_t)NXd * jl];
-                    for (int32_t jj = 0; jj < ngskip; ++jj)
-                        for (int32_t ii = 0; ii < ngskip; ++ii) {
-                            const int64_t k = (int64_t)il * ngskip + ii + (int64_t)NX * ((int64_t)jl * ngskip + jj);
-                            acc2 = acc2 + z[2 * k] * factor;
-                        }
-                    o[il + (int64_t)NXd * jl] = acc2;
-                }
-        }
+        if (diagnose)
+            lg_diagnose(NX, NY, buffer, ntskip, ngskip, F, EB, ns, P, mass, weight, state, t, rows, &nrow, bins,
+                        nbins);
     }
     free(plus);
     free(EB);
@@ -647,4 +653,315 @@ int orclg_run_centred(int32_t NX, int32_t NY, double Lx, double Ly, double dt, c
     }
     free(acc);
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LorenzGaugeSemiImplicitField (:403-461): warmup! (:864-871) and loop!     */
+/* (:993-1100).  State: the 18 slots above (PHI0/PHIM = phi0/phi-, A0/A-,    */
+/* RHO0, J0 the n sources, EX..BZ) and four more, RHOM, JXM, JYM, JZM = rho-, */
+/* J-; the buffered deposit grid rhoJs+ (4 x (NX+2b) x (NY+2b), axes         */
+/* -(b-1):NX+b, in the reference's element order, component fastest), which  */
+/* the next step's first convergence test reads; and EB, the gathered fields */
+/* update! left (zeros until the first update!).                             */
+/* warmup!: three passes of the four-weight warmup! (:746-775) -- move by h, */
+/* deposit qw, vx*qw, vy*qw, vz*qw, move back by h -- with h = -dt (into     */
+/* rho-, J-), +dt (rho0, J0) and +dt (rho+, J+ and the grid rhoJs+), then    */
+/* advect! by -dt.  rho+, J+ (temporaries the loop overwrites before it      */
+/* reads them) and rhoJs-, rhoJs0 (which loop! only copies into each other)  */
+/* are not observable and are not kept.                                      */
+/* loop!: from the step's start state (x, v), iteration k:                   */
+/*   1. gather at (x, y) from EB, xw = unimod(x + vx*dt) with the old v,     */
+/*      vw = ElectromagneticBoris(v); the currents vw*qw at xw into the      */
+/*      buffered grid (:1018-1045; its charge component stays zero), the    */
+/*      reduction into rho+ (zero) and J+ (:1046-1049);                      */
+/*   2. fft! of rho+, J+ and [1,1] = 0 (:1050-1059);                         */
+/*   3. lorenzgauge! (:638-648) of phi, Ax, Ay, Az with the field theta_f    */
+/*      and the source theta_s/2*s- + (1-theta_s)*s0 + theta_s/2*s+          */
+/*      (:1060-1065);                                                        */
+/*   4. E = -im k (theta_f/2 (phi+ + phi-) + (1-theta_f) phi0) -             */
+/*      (A+ - A-)/2dt, B = curl (theta_f/2 (A+ + A-) + (1-theta_f) A0)       */
+/*      (:1066-1077), ifft of the six, update! (:1078-1086).                 */
+/* The loop stops after iteration k when k > maxiters or                     */
+/* isapprox(rhoJs_prev, rhoJs+; rtol, atol = 0) (:1000-1012, orc_isapprox),  */
+/* rhoJs_prev being the grid before iteration k (the previous step's, or     */
+/* warmup!'s, at k = 1); then x, v = xw, vw and phi- = phi0, phi0 = phi+,    */
+/* A- = A0, A0 = A+ (:1089-1100).  The sources rho0, J0, rho-, J- keep their */
+/* warmup! values, in real space, except where diagnose! transforms rho0, J0 */
+/* in place.  The reference keeps one buffered grid per thread and its test  */
+/* compares them all, so its iteration counts depend on nthreads(); this is  */
+/* the one-thread grid (julia's default).  Charge and current sums are exact */
+/* per buffered cell and per periodic cell (the exact sum over the cell's    */
+/* images), each rounded once.  A stencil cell outside the buffer (the       */
+/* reference's BoundsError) returns -2.                                      */
+/* ------------------------------------------------------------------------ */
+static int deposit_buffered(int32_t NX, int32_t NY, int32_t b, double Lx, double Ly, int32_t ns,
+                            const int32_t* shape, const int64_t* P, const double* qw, const double* state, int c0,
+                            double* grid, double* const* per) {
+    const int64_t G = (int64_t)NX * NY;
+    const int64_t NXe = NX + 2 * (int64_t)b, NYe = NY + 2 * (int64_t)b, Ge = NXe * NYe;
+    const double NX_Lx = (double)NX / Lx, NY_Ly = (double)NY / Ly;
+    double M[3][8];
+    const double* st = state;
+    for (int s = 0; s < ns; ++s) {
+        for (int c = 0; c < 3; ++c) {
+            double m = 0.0;
+            for (int64_t p = 0; p < P[s]; ++p) {
+                const double a = fabs(st[(2 + c) * P[s] + p]);
+                m = a > m ? a : m;
+            }
+            M[c][s] = m;
+        }
+        st += 5 * P[s];
+    }
+    i128* acc = (i128*)malloc(sizeof(i128) * (size_t)Ge);
+    i128* pacc = (i128*)malloc(sizeof(i128) * (size_t)G);
+    int rc = 0;
+    if (grid)
+        for (int64_t e = 0; e < 4 * Ge; ++e) grid[e] = 0.0;
+    for (int c = c0; c < 4; ++c) {
+        const int q = orclg_frac_bits(ns, P, qw, c == 0 ? NULL : M[c - 1]);
+        memset(acc, 0, sizeof(i128) * (size_t)Ge);
+        memset(pacc, 0, sizeof(i128) * (size_t)G);
+        st = state;
+        for (int s = 0; s < ns; ++s) {
+            const int64_t n = P[s];
+            for (int64_t p = 0; p < n; ++p) {
+                const double w = c == 0 ? qw[s] : st[(1 + c) * n + p] * qw[s];
+                int64_t cx[6], cy[6];
+                double wx[6], wy[6];
+                const int nx = orc23_weights(shape[s], st[p], NX_Lx, cx, wx);
+                const int ny = orc23_weights(shape[s], st[n + p], NY_Ly, cy, wy);
+                for (int jj = 0; jj < ny; ++jj)
+                    for (int ii = 0; ii < nx; ++ii) {
+                        const double term = wx[ii] * wy[jj] * w;
+                        const int64_t u = cx[ii] - 1 + b, v = cy[jj] - 1 + b;
+                        if (u < 0 || u >= NXe || v < 0 || v >= NYe) {
+                            rc = -2;
+                            continue;
+                        }
+                        const i128 f = fix_signed(term, q);
+                        acc[u + NXe * v] += f;
+                        pacc[wrap_cell(cx[ii], NX) + (int64_t)NX * wrap_cell(cy[jj], NY)] += f;
+                    }
+            }
+            st += 5 * n;
+        }
+        if (grid)
+            for (int64_t e = 0; e < Ge; ++e) grid[c + 4 * e] = unfix_signed(acc[e], q);
+        if (per && per[c])
+            for (int64_t k = 0; k < G; ++k) {
+                per[c][2 * k] = unfix_signed(pacc[k], q);
+                per[c][2 * k + 1] = 0.0;
+            }
+    }
+    free(acc);
+    free(pacc);
+    return rc;
+}
+
+/* the deposits alone (tests): c0 = 0 all four weights, 1 the currents */
+int orclg_si_deposit(int32_t NX, int32_t NY, int32_t buffer, double Lx, double Ly, int32_t ns, const int32_t* shape,
+                     const int64_t* P, const double* charge, const double* weight, const double* state, int32_t c0,
+                     double* grid, double* per) {
+    const double dV = (Lx / (double)NX) * (Ly / (double)NY);
+    double qw[8];
+    for (int s = 0; s < ns; ++s) qw[s] = charge[s] * weight[s] / dV;
+    const int64_t G = (int64_t)NX * NY;
+    double* pp[4] = {per, per + 2 * G, per + 4 * G, per + 6 * G};
+    return deposit_buffered(NX, NY, buffer, Lx, Ly, ns, shape, P, qw, state, c0, grid, per ? pp : NULL);
+}
+
+static void advect_all(int32_t ns, const int64_t* P, double Lx, double Ly, double h, double* state) {
+    double* st = state;
+    for (int s = 0; s < ns; ++s) {
+        const int64_t n = P[s];
+        for (int64_t p = 0; p < n; ++p) {
+            st[p] = orc23_unimod(st[p] + st[2 * n + p] * h, Lx);
+            st[n + p] = orc23_unimod(st[n + p] + st[3 * n + p] * h, Ly);
+        }
+        st += 5 * n;
+    }
+}
+
+/* F: 22 complex slots; grid: 4 x (NX+2b) x (NY+2b) */
+int orclg_si_warmup(int32_t NX, int32_t NY, double Lx, double Ly, double dt, int32_t buffer, int32_t ns,
+                    const int32_t* shape, const int64_t* P, const double* charge, const double* weight, double* state,
+                    double* F, double* grid) {
+    if (ns < 1 || ns > 8 || NX < 2 || NY < 2 || (NX & (NX - 1)) || (NY & (NY - 1)) || buffer < 0) return -1;
+    const int64_t G = (int64_t)NX * NY;
+    const double dV = (Lx / (double)NX) * (Ly / (double)NY);
+    double qw[8];
+    for (int s = 0; s < ns; ++s) qw[s] = charge[s] * weight[s] / dV;
+    int rc = 0;
+    for (int pass = 0; pass < 3; ++pass) {
+        const double h = pass == 0 ? -dt : dt;
+        const int first = pass == 0 ? ORCLG_RHOM : ORCLG_RHO0;
+        double* pp[4] = {F + (size_t)first * 2 * G, F + (size_t)(first + 1) * 2 * G, F + (size_t)(first + 2) * 2 * G,
+                         F + (size_t)(first + 3) * 2 * G};
+        advect_all(ns, P, Lx, Ly, h, state);
+        const int r = deposit_buffered(NX, NY, buffer, Lx, Ly, ns, shape, P, qw, state, 0, pass == 2 ? grid : NULL,
+                                       pass == 2 ? NULL : pp);
+        if (r) rc = r;
+        advect_all(ns, P, Lx, Ly, -h, state);
+    }
+    advect_all(ns, P, Lx, Ly, -dt, state);
+    return rc;
+}
+
+/* steps 2-4 of an iteration: src (4 complex: rho+, Jx+, Jy+, Jz+ in real
+ * space) is transformed in place and its [1,1] cleared; plus gets phi+, A+;
+ * the six E, B slots of F get their real-space values. */
+void orclg_si_solve(int32_t NX, int32_t NY, double Lx, double Ly, double dt, double theta_f, double theta_s,
+                    double* F, double* src, double* plus) {
+    const int64_t G = (int64_t)NX * NY;
+    for (int a = 0; a < 4; ++a) {
+        double* z = src + (size_t)a * 2 * G;
+        fft2_inplace(NX, NY, z);
+        z[0] = 0.0;
+        z[1] = 0.0;
+    }
+    double* kx = (double*)malloc(sizeof(double) * (size_t)NX);
+    double* ky = (double*)malloc(sizeof(double) * (size_t)NY);
+    double* k2 = (double*)malloc(sizeof(double) * (size_t)G);
+    orclg_wavenumbers(NX, NY, Lx, Ly, kx, ky, k2);
+    const double dt2 = dt * dt, dt_2 = 2.0 * dt;
+    const double hs = theta_s / 2.0, ws = 1.0 - theta_s, hf = theta_f / 2.0, wf = 1.0 - theta_f;
+    static const int x0s[4] = {ORCLG_PHI0, ORCLG_AX0, ORCLG_AY0, ORCLG_AZ0};
+    static const int xms[4] = {ORCLG_PHIM, ORCLG_AXM, ORCLG_AYM, ORCLG_AZM};
+    static const int s0s[4] = {ORCLG_RHO0, ORCLG_JX0, ORCLG_JY0, ORCLG_JZ0};
+    static const int sms[4] = {ORCLG_RHOM, ORCLG_JXM, ORCLG_JYM, ORCLG_JZM};
+    for (int32_t j = 0; j < NY; ++j)
+        for (int32_t i = 0; i < NX; ++i) {
+            const int64_t k = i + (int64_t)NX * j;
+            const double num = 2.0 - dt2 * k2[k] * (1.0 - theta_f);
+            const double den = 1.0 + dt2 * k2[k] * theta_f / 2.0;
+            double pl[4][2], c0[4][2];
+            for (int a = 0; a < 4; ++a) {
+                const double* sm = F + (size_t)sms[a] * 2 * G + 2 * k;
+                const double* s0 = F + (size_t)s0s[a] * 2 * G + 2 * k;
+                const double* sp = src + (size_t)a * 2 * G + 2 * k;
+                const double* x0 = F + (size_t)x0s[a] * 2 * G + 2 * k;
+                const double* xm = F + (size_t)xms[a] * 2 * G + 2 * k;
+                for (int r = 0; r < 2; ++r) {
+                    const double sv = hs * sm[r] + ws * s0[r] + hs * sp[r];
+                    pl[a][r] = (num * x0[r] + dt2 * sv) / den - xm[r];
+                    /* theta_f/2*(x+ + x-) + (1-theta_f)*x0 */
+                    c0[a][r] = hf * (pl[a][r] + xm[r]) + wf * x0[r];
+                    plus[(size_t)a * 2 * G + 2 * k + r] = pl[a][r];
+                }
+            }
+            const double kxi = kx[i], kyj = ky[j];
+            const double* axm = F + (size_t)ORCLG_AXM * 2 * G + 2 * k;
+            const double* aym = F + (size_t)ORCLG_AYM * 2 * G + 2 * k;
+            const double* azm = F + (size_t)ORCLG_AZM * 2 * G + 2 * k;
+            double er, ei, fr, fi;
+            double* Z;
+            cmul(0.0 * kxi, -kxi, c0[0][0], c0[0][1], &er, &ei);
+            Z = F + (size_t)ORCLG_EX * 2 * G + 2 * k;
+            Z[0] = er - (pl[1][0] - axm[0]) / dt_2;
+            Z[1] = ei - (pl[1][1] - axm[1]) / dt_2;
+            cmul(0.0 * kyj, -kyj, c0[0][0], c0[0][1], &er, &ei);
+            Z = F + (size_t)ORCLG_EY * 2 * G + 2 * k;
+            Z[0] = er - (pl[2][0] - aym[0]) / dt_2;
+            Z[1] = ei - (pl[2][1] - aym[1]) / dt_2;
+            Z = F + (size_t)ORCLG_EZ * 2 * G + 2 * k;
+            Z[0] = -(pl[3][0] - azm[0]) / dt_2;
+            Z[1] = -(pl[3][1] - azm[1]) / dt_2;
+            cmul(0.0 * kyj, kyj, c0[3][0], c0[3][1], &er, &ei);
+            Z = F + (size_t)ORCLG_BX * 2 * G + 2 * k;
+            Z[0] = er;
+            Z[1] = ei;
+            cmul(0.0 * kxi, -kxi, c0[3][0], c0[3][1], &er, &ei);
+            Z = F + (size_t)ORCLG_BY * 2 * G + 2 * k;
+            Z[0] = er;
+            Z[1] = ei;
+            cmul(0.0 * kxi, kxi, c0[2][0], c0[2][1], &er, &ei);
+            cmul(0.0 * kyj, kyj, c0[1][0], c0[1][1], &fr, &fi);
+            Z = F + (size_t)ORCLG_BZ * 2 * G + 2 * k;
+            Z[0] = er - fr;
+            Z[1] = ei - fi;
+        }
+    for (int a = ORCLG_EX; a <= ORCLG_BZ; ++a) ifft2_inplace(NX, NY, F + (size_t)a * 2 * G);
+    free(kx);
+    free(ky);
+    free(k2);
+}
+
+/* nsteps of loop! and (diagnose) diagnose!; iters[t - t0] gets the step's
+ * iteration count (may be NULL). */
+int orclg_si_run(int32_t NX, int32_t NY, double Lx, double Ly, double dt, const double* B0, int32_t buffer,
+                 double theta_f, double theta_s, double rtol, int64_t maxiters, int32_t ntskip, int32_t ngskip,
+                 int32_t ns, const int32_t* shape, const int64_t* P, const double* charge, const double* mass,
+                 const double* weight, double* state, double* F, double* grid, double* EB, int64_t t0,
+                 int64_t nsteps, int32_t diagnose, double* rows, double* bins, int64_t nbins, int64_t* iters) {
+    if (ns < 1 || ns > 8 || NX < 2 || NY < 2 || (NX & (NX - 1)) || (NY & (NY - 1)) || ntskip < 1 || ngskip < 1 ||
+        NX % ngskip || NY % ngskip || buffer < 0 || maxiters < 0)
+        return -1;
+    const int64_t G = (int64_t)NX * NY;
+    const int64_t Ge = (NX + 2 * (int64_t)buffer) * (NY + 2 * (int64_t)buffer);
+    const double NX_Lx = (double)NX / Lx, NY_Ly = (double)NY / Ly;
+    const double dV = (Lx / (double)NX) * (Ly / (double)NY);
+    double qw[8];
+    for (int s = 0; s < ns; ++s) qw[s] = charge[s] * weight[s] / dV;
+    int64_t Ptot = 0;
+    for (int s = 0; s < ns; ++s) Ptot += P[s];
+    double* work = (double*)malloc(sizeof(double) * 5 * (size_t)(Ptot > 0 ? Ptot : 1));
+    double* src = (double*)malloc(sizeof(double) * 8 * (size_t)G);
+    double* plus = (double*)malloc(sizeof(double) * 8 * (size_t)G);
+    double* prev = (double*)malloc(sizeof(double) * 4 * (size_t)Ge);
+    double* pp[4] = {src, src + 2 * G, src + 4 * G, src + 6 * G};
+    int rc = 0;
+    int64_t nrow = 0;
+    for (int64_t t = t0; t < t0 + nsteps && rc == 0; ++t) {
+        int64_t k = 0;
+        int conv = 0;
+        while (!(k > 0 && (k > maxiters || conv))) {
+            ++k;
+            memcpy(prev, grid, sizeof(double) * 4 * (size_t)Ge);
+            const double* st = state;
+            double* wk = work;
+            for (int s = 0; s < ns; ++s) {
+                const int64_t n = P[s];
+                const double q_m = charge[s] / mass[s];
+                for (int64_t p = 0; p < n; ++p) {
+                    double f[6], v[3] = {st[2 * n + p], st[3 * n + p], st[4 * n + p]};
+                    orclg_gather(NX, NY, NX_Lx, NY_Ly, shape[s], EB, st[p], st[n + p], f);
+                    wk[p] = orc23_unimod(st[p] + v[0] * dt, Lx);
+                    wk[n + p] = orc23_unimod(st[n + p] + v[1] * dt, Ly);
+                    orclg_boris(dt, q_m, f, f + 3, v);
+                    wk[2 * n + p] = v[0];
+                    wk[3 * n + p] = v[1];
+                    wk[4 * n + p] = v[2];
+                }
+                st += 5 * n;
+                wk += 5 * n;
+            }
+            for (int64_t e = 0; e < 2 * G; ++e) src[e] = 0.0;
+            const int r = deposit_buffered(NX, NY, buffer, Lx, Ly, ns, shape, P, qw, work, 1, grid, pp);
+            if (r) {
+                rc = r;
+                break;
+            }
+            conv = orc_isapprox(prev, grid, 4 * Ge, rtol, 0.0);
+            orclg_si_solve(NX, NY, Lx, Ly, dt, theta_f, theta_s, F, src, plus);
+            orclg_update(NX, NY, F, B0, EB);
+        }
+        if (rc) break;
+        if (iters) iters[t - t0] = k;
+        memcpy(state, work, sizeof(double) * 5 * (size_t)Ptot);
+        static const int x0s[4] = {ORCLG_PHI0, ORCLG_AX0, ORCLG_AY0, ORCLG_AZ0};
+        static const int xms[4] = {ORCLG_PHIM, ORCLG_AXM, ORCLG_AYM, ORCLG_AZM};
+        for (int a = 0; a < 4; ++a) {
+            memcpy(F + (size_t)xms[a] * 2 * G, F + (size_t)x0s[a] * 2 * G, sizeof(double) * 2 * (size_t)G);
+            memcpy(F + (size_t)x0s[a] * 2 * G, plus + (size_t)a * 2 * G, sizeof(double) * 2 * (size_t)G);
+        }
+        if (diagnose)
+            lg_diagnose(NX, NY, buffer, ntskip, ngskip, F, EB, ns, P, mass, weight, state, t, rows, &nrow, bins,
+                        nbins);
+    }
+    free(work);
+    free(src);
+    free(plus);
+    free(prev);
+    return rc;
 }

@@ -128,6 +128,14 @@ def lib():
                                              _dp, _dp, _dp, _dp, c_i64]),
         "orclg_run": (ctypes.c_int, [c_i32, c_i32, c_d, c_d, c_d, _dp, c_i32, c_d, c_i32, c_i32, c_i32, _ip32,
                                      _ip64, _dp, _dp, _dp, _dp, _dp, c_i64, c_i64, c_i32, c_vp, c_vp, c_i64]),
+        "orclg_si_deposit": (ctypes.c_int, [c_i32, c_i32, c_i32, c_d, c_d, c_i32, _ip32, _ip64, _dp, _dp, _dp,
+                                            c_i32, _dp, _dp]),
+        "orclg_si_warmup": (ctypes.c_int, [c_i32, c_i32, c_d, c_d, c_d, c_i32, c_i32, _ip32, _ip64, _dp, _dp, _dp,
+                                           _dp, _dp]),
+        "orclg_si_solve": (None, [c_i32, c_i32, c_d, c_d, c_d, c_d, c_d, _dp, _dp, _dp]),
+        "orclg_si_run": (ctypes.c_int, [c_i32, c_i32, c_d, c_d, c_d, _dp, c_i32, c_d, c_d, c_d, c_i64, c_i32, c_i32,
+                                        c_i32, _ip32, _ip64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, c_i64, c_i64, c_i32,
+                                        c_vp, c_vp, c_i64, _ip64]),
         "orc23_run": (ctypes.c_int, [c_i32, c_i32, c_d, c_d, c_d, _dp, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32,
                                      _ip32, _ip64, _dp, _dp, _dp, _dp, _dp, _dp, c_i64, c_i64, c_vp, c_vp, c_vp,
                                      c_vp, c_i64, c_i32]),
@@ -743,6 +751,80 @@ def lg_run_centred(NX, NY, Lx, Ly, dt, B0, species, F, EB, nsteps, theta=0.0):
     if rc != 0:
         raise ValueError(f"orclg_run_centred rejected its arguments (rc={rc})")
     return _lg_unpack(species, P, state), F, EB
+
+
+# LorenzGaugeSemiImplicitField: 22 slots (LG_FIELDS + rho-, J-), the buffered
+# deposit grid (NY + 2b, NX + 2b, 4) (component fastest, as rhoJs[c, i, j])
+# and EB (6, NY, NX).
+LG_SI_FIELDS = LG_FIELDS + ["rhom", "Jxm", "Jym", "Jzm"]
+
+
+def lg_si_deposit(NX, NY, Lx, Ly, species, buffer, currents_only=False):
+    """The exact deposits of the species at their positions into the
+    buffered grid and the periodic cells: returns (rc, grid (NY+2b, NX+2b, 4),
+    per (4, NY, NX) complex)."""
+    P, state, shape = _lg_species(species)
+    b = int(buffer)
+    grid = np.zeros((NY + 2 * b, NX + 2 * b, 4))
+    per = np.zeros((4, NY, NX), dtype=np.complex128)
+    rc = lib().orclg_si_deposit(int(NX), int(NY), b, float(Lx), float(Ly), len(species), shape, P,
+                                _f64([s["charge"] for s in species]), _f64([s["weight"] for s in species]), state,
+                                int(bool(currents_only)), grid.ravel(), _cp(per))
+    return rc, grid, per
+
+
+def lg_si_warmup(NX, NY, Lx, Ly, dt, species, buffer):
+    """warmup!(::LorenzGaugeSemiImplicitField): returns (species, F (22, NY, NX), grid)."""
+    P, state, shape = _lg_species(species)
+    b = int(buffer)
+    F = np.zeros((22, NY, NX), dtype=np.complex128)
+    grid = np.zeros((NY + 2 * b, NX + 2 * b, 4))
+    rc = lib().orclg_si_warmup(int(NX), int(NY), float(Lx), float(Ly), float(dt), b, len(species), shape, P,
+                               _f64([s["charge"] for s in species]), _f64([s["weight"] for s in species]), state,
+                               _cp(F), grid.ravel())
+    if rc != 0:
+        raise ValueError(f"orclg_si_warmup failed (rc={rc})")
+    return _lg_unpack(species, P, state), F, grid
+
+
+def lg_si_solve(F, src, dt, theta_f, theta_s, Lx=1.0, Ly=1.0):
+    """Steps 2-4 of an iteration: returns (F with real-space E, B; src in the
+    spectrum with [1,1] = 0; plus (4, NY, NX))."""
+    F = _c128(F).copy()
+    src = _c128(src).copy()
+    _, NY, NX = F.shape
+    plus = np.zeros((4, NY, NX), dtype=np.complex128)
+    lib().orclg_si_solve(NX, NY, float(Lx), float(Ly), float(dt), float(theta_f), float(theta_s), _cp(F), _cp(src),
+                         _cp(plus))
+    return F, src, plus
+
+
+def lg_si_run(NX, NY, Lx, Ly, dt, B0, species, F, grid, EB, nsteps, theta=1.0, theta_source=0.0,
+              rtol=float(np.sqrt(np.finfo(float).eps)), maxiters=10, t0=0, buffer=10, ntskip=1, ngskip=1,
+              diagnose=True, nbins=None, bins=None):
+    """nsteps of loop!(plasma, ::LorenzGaugeSemiImplicitField) and (diagnose)
+    diagnose!.  Returns (species, F, grid, EB, rows, bins, iterations)."""
+    P, state, shape = _lg_species(species)
+    F = _c128(F).copy()
+    grid = _f64(grid).copy()
+    EB = _f64(EB).copy()
+    t1 = t0 + nsteps
+    nrows = (-(-t1 // ntskip) - (-(-t0 // ntskip))) if diagnose else 0
+    rows = np.zeros((max(nrows, 0), 11))
+    if nbins is None:
+        nbins = -(-t1 // ntskip)
+    if bins is None:
+        bins = np.zeros((14, nbins, NY // ngskip, NX // ngskip))
+    bins = _f64(bins).copy()
+    iters = np.zeros(max(int(nsteps), 1), dtype=np.int64)
+    rc = lib().orclg_si_run(int(NX), int(NY), float(Lx), float(Ly), float(dt), _f64(B0), int(buffer), float(theta),
+                            float(theta_source), float(rtol), int(maxiters), int(ntskip), int(ngskip), len(species),
+                            shape, P, _f64([s["charge"] for s in species]), _f64([s["mass"] for s in species]),
+                            _f64([s["weight"] for s in species]), state, _cp(F), grid.ravel(), EB.ravel(), int(t0),
+                            int(nsteps), int(bool(diagnose)), _ptr(rows), _ptr(bins), int(nbins), iters)
+    if rc != 0:
+        raise ValueError(f"orclg_si_run failed (rc={rc})")
+    return _lg_unpack(species, P, state), F, grid, EB, rows, bins, iters[:nsteps]
 
 
 # ---- 1D electromagnetic fixed point (src/GaussianEMFixedPoint.jl) ----------

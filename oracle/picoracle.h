@@ -195,6 +195,10 @@ int      orc23_run(int32_t NX, int32_t NY, double Lx, double Ly, double dt, cons
 #define ORCLG_BX 15
 #define ORCLG_BY 16
 #define ORCLG_BZ 17
+#define ORCLG_RHOM 18 /* LorenzGaugeSemiImplicitField: rho-, Jx-, Jy-, Jz- */
+#define ORCLG_JXM 19
+#define ORCLG_JYM 20
+#define ORCLG_JZM 21
 void orclg_wavenumbers(int32_t NX, int32_t NY, double Lx, double Ly, double* kx, double* ky, double* k2);
 void orclg_boris(double dt, double q_m, const double* E, const double* B, double* v);
 void orclg_gather(int32_t NX, int32_t NY, double NX_Lx, double NY_Ly, int32_t shape, const double* EB, double x,
@@ -216,6 +220,19 @@ int  orclg_run(int32_t NX, int32_t NY, double Lx, double Ly, double dt, const do
                double theta, int32_t ntskip, int32_t ngskip, int32_t ns, const int32_t* shape, const int64_t* P,
                const double* charge, const double* mass, const double* weight, double* state, double* F,
                int64_t t0, int64_t nsteps, int32_t diagnose, double* rows, double* bins, int64_t nbins);
+int  orclg_si_deposit(int32_t NX, int32_t NY, int32_t buffer, double Lx, double Ly, int32_t ns, const int32_t* shape,
+                      const int64_t* P, const double* charge, const double* weight, const double* state, int32_t c0,
+                      double* grid, double* per);
+int  orclg_si_warmup(int32_t NX, int32_t NY, double Lx, double Ly, double dt, int32_t buffer, int32_t ns,
+                     const int32_t* shape, const int64_t* P, const double* charge, const double* weight,
+                     double* state, double* F, double* grid);
+void orclg_si_solve(int32_t NX, int32_t NY, double Lx, double Ly, double dt, double theta_f, double theta_s,
+                    double* F, double* src, double* plus);
+int  orclg_si_run(int32_t NX, int32_t NY, double Lx, double Ly, double dt, const double* B0, int32_t buffer,
+                  double theta_f, double theta_s, double rtol, int64_t maxiters, int32_t ntskip, int32_t ngskip,
+                  int32_t ns, const int32_t* shape, const int64_t* P, const double* charge, const double* mass,
+                  const double* weight, double* state, double* F, double* grid, double* EB, int64_t t0,
+                  int64_t nsteps, int32_t diagnose, double* rows, double* bins, int64_t nbins, int64_t* iters);
 
 /* 1D electromagnetic fixed point (src/GaussianEMFixedPoint.jl), in em1d.c */
 int64_t orc_em_irn(double c, int32_t N);
