@@ -95,6 +95,7 @@ extern "C" {
 #define PIC2D3V_ELECTROSTATIC 0    /* ElectrostaticField (:270-297)          */
 #define PIC2D3V_LORENZ_STAGGERED 1 /* LorenzGaugeStaggeredField (:347-392)   */
 #define PIC2D3V_LORENZ 2           /* LorenzGaugeField (:307-345)            */
+#define PIC2D3V_LORENZ_SEMI_IMPLICIT 3 /* LorenzGaugeSemiImplicitField (:403-461) */
 
 /* Lorenz-gauge field state: complex NX x NY arrays (pic2d3v_lg_get/set).
  * For PIC2D3V_LORENZ the slots hold that type's arrays: PHI0 = phi+ (the
@@ -119,6 +120,14 @@ extern "C" {
 #define PIC2D3V_LG_BY 16
 #define PIC2D3V_LG_BZ 17
 #define PIC2D3V_LG_NSTATE 18
+/* PIC2D3V_LORENZ_SEMI_IMPLICIT has four more: the sources at n-1 (rho-, J-),
+ * which its warmup! deposits at -dt and nothing changes after; its RHO0, J0
+ * are the sources at n (warmup! at +dt), changed only by diagnose!. */
+#define PIC2D3V_LG_RHOM 18
+#define PIC2D3V_LG_JXM 19
+#define PIC2D3V_LG_JYM 20
+#define PIC2D3V_LG_JZM 21
+#define PIC2D3V_LG_NSTATE_SI 22
 /* binned fields of LorenzGaugeDiagnostics, in its order (pic2d3v_lg_bins) */
 #define PIC2D3V_LG_BIN_EX 0
 #define PIC2D3V_LG_BIN_EY 1
@@ -157,8 +166,11 @@ typedef struct pic2d3v_config {
     int32_t nspecies;      /* 1 .. PIC2D3V_MAX_SPECIES                               */
     pic2d3v_species species[PIC2D3V_MAX_SPECIES];
     int32_t device;        /* HIP device ordinal                                     */
-    int32_t field;         /* PIC2D3V_ELECTROSTATIC, _LORENZ_STAGGERED or _LORENZ     */
+    int32_t field;         /* PIC2D3V_ELECTROSTATIC, _LORENZ_STAGGERED, _LORENZ, ... */
     double theta;          /* Lorenz gauge: ImEx theta (0 Explicit(), 1 Implicit())   */
+    double theta_source;   /* semi-implicit: sourceimex theta                          */
+    double rtol;           /* semi-implicit: rtol of the convergence test (sqrt(eps))  */
+    int32_t maxiters;      /* semi-implicit: at most maxiters + 1 iterations (10)      */
 } pic2d3v_config;
 
 typedef struct pic2d3v_ctx pic2d3v_ctx;
@@ -213,6 +225,31 @@ int pic2d3v_loop(pic2d3v_ctx* ctx, int64_t nsteps);
 int pic2d3v_lg_get(pic2d3v_ctx* ctx, int32_t which, double* z);
 int pic2d3v_lg_set(pic2d3v_ctx* ctx, int32_t which, const double* z);
 int pic2d3v_lg_bins(pic2d3v_ctx* ctx, int32_t which, double* out, int64_t nbins);
+
+/* LorenzGaugeSemiImplicitField (field = PIC2D3V_LORENZ_SEMI_IMPLICIT,
+ * :403-461; src/2D3V.jl:104-105 builds one with fieldimex = ImEx(1.0),
+ * sourceimex = ImEx(0.05), buffer = 10): warmup! (:864-871) and loop!
+ * (:993-1100) through pic2d3v_warmup, pic2d3v_run and pic2d3v_loop.  Each
+ * step iterates from its start state: push (xw = unimod(x + vx*dt) with the
+ * old v, vw = ElectromagneticBoris), the exact current deposits into the
+ * buffered grid rhoJs+, its reduction and forward transform, lorenzgauge!
+ * with the source theta_source/2*s- + (1 - theta_source)*s0 +
+ * theta_source/2*s+ and the field theta, E and B with theta/2 (x+ + x-) +
+ * (1 - theta) x0, their inverse transforms and update!; it stops after
+ * iteration k when k > maxiters or isapprox(grid before, grid after; rtol).
+ * The reference keeps one grid per thread and its test compares them all,
+ * so its iteration counts depend on nthreads(); this is the one-thread grid
+ * (julia's default).  buffer must cover every species' stencil; a stencil
+ * cell that still falls outside (a position rounding onto NX + 1, the
+ * reference's BoundsError) fails the call with PIC_EINVAL.
+ * pic2d3v_lg_grid / pic2d3v_lg_set_grid: the grid rhoJs+ (4*(NX+2*buffer)*
+ * (NY+2*buffer) doubles, the reference's element order: component fastest,
+ * then x, then y; axes -(buffer-1):NX+buffer), which the next step's first
+ * test compares with.  pic2d3v_lg_iterations: the iteration counts of the
+ * last n steps of the most recent pic2d3v_run or pic2d3v_loop call. */
+int pic2d3v_lg_grid(pic2d3v_ctx* ctx, double* out);
+int pic2d3v_lg_set_grid(pic2d3v_ctx* ctx, const double* in);
+int pic2d3v_lg_iterations(pic2d3v_ctx* ctx, int64_t* out, int64_t n);
 
 /* The time-binned fields of the diagnostics, Exs, Eys, phis[:, :, b] for
  * b < nbins, each (NX/ngskip) x (NY/ngskip) per bin, bins back to back

@@ -38,7 +38,11 @@ struct pic2d3v_ctx {
     // (loop! alone, with diagnose! without and with the row)
     bool lorenz = false;
     bool centred = false;    // LorenzGaugeField (else LorenzGaugeStaggeredField)
+    bool semi = false;       // LorenzGaugeSemiImplicitField
     LG23 lg{};
+    Species23 wk[kMaxSpecies23]{};  // semi-implicit: the work arrays xyvwork
+    double* hres = nullptr;  // pinned: the iteration's test result
+    std::vector<int64_t> iters;  // semi-implicit: iterations of the last call's steps
     double* xh[kMaxSpecies23] = {};
     double* yh[kMaxSpecies23] = {};
     hipGraphExec_t g_lg[3] = {nullptr, nullptr, nullptr};
@@ -133,6 +137,7 @@ static void release(pic2d3v_ctx* c) {
     free_graphs(c);
     free_bins(c);
     for (void* p : c->allocs) (void)hipFree(p);
+    if (c->hres) (void)hipHostFree(c->hres);
     for (void*& p : c->op)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -208,13 +213,33 @@ static int validate(const pic2d3v_config* g, std::string& why) {
         why = "the species hold more than 2^26 particles together";
         return PIC_EINVAL;
     }
-    if (g->field != PIC2D3V_ELECTROSTATIC && g->field != PIC2D3V_LORENZ_STAGGERED && g->field != PIC2D3V_LORENZ) {
-        why = "field must be PIC2D3V_ELECTROSTATIC, PIC2D3V_LORENZ_STAGGERED or PIC2D3V_LORENZ";
+    if (g->field != PIC2D3V_ELECTROSTATIC && g->field != PIC2D3V_LORENZ_STAGGERED && g->field != PIC2D3V_LORENZ &&
+        g->field != PIC2D3V_LORENZ_SEMI_IMPLICIT) {
+        why = "field must be PIC2D3V_ELECTROSTATIC, PIC2D3V_LORENZ_STAGGERED, PIC2D3V_LORENZ or "
+              "PIC2D3V_LORENZ_SEMI_IMPLICIT";
         return PIC_EINVAL;
     }
     if (!std::isfinite(g->theta)) {
         why = "theta must be finite";
         return PIC_EINVAL;
+    }
+    if (g->field == PIC2D3V_LORENZ_SEMI_IMPLICIT) {
+        if (!std::isfinite(g->theta_source) || !(std::isfinite(g->rtol) && g->rtol >= 0.0) || g->maxiters < 0) {
+            why = "theta_source must be finite, rtol finite and >= 0, maxiters >= 0";
+            return PIC_EINVAL;
+        }
+        // the deposit grid's axes -(b-1):NX+b hold every stencil cell of a
+        // particle in (0, L] only for b >= the shape's upper reach
+        for (int s = 0; s < g->nspecies; ++s) {
+            const int sh = g->species[s].shape;
+            const int n = sh - PIC2D3V_BSPLINE0;
+            const int reach = sh == PIC2D3V_NGP ? 0 : (sh == PIC2D3V_AREA ? 1 : ((n & 1) ? (n + 1) / 2 : n / 2 + 1));
+            if (g->buffer < reach) {
+                snprintf(buf, sizeof buf, "species %d: the semi-implicit deposit grid needs buffer >= %d", s, reach);
+                why = buf;
+                return PIC_EINVAL;
+            }
+        }
     }
     return PIC_OK;
 }
@@ -228,7 +253,15 @@ static int zero_state(pic2d3v_ctx* c) {
     HIPCHK(c, hipMemcpyAsync(c->d.t, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
     if (c->lorenz) {
         LG23& L = c->lg;
-        HIPCHK(c, hipMemsetAsync(L.F, 0, sizeof(double2) * kLGSlots * (size_t)G, c->stream));
+        HIPCHK(c, hipMemsetAsync(L.F, 0, sizeof(double2) * (c->semi ? kLGSlotsSI : kLGSlots) * (size_t)G, c->stream));
+        if (c->semi) {
+            const size_t Ge = (size_t)L.NXe * L.NYe;
+            HIPCHK(c, hipMemsetAsync(L.src, 0, sizeof(double2) * 4 * (size_t)G, c->stream));
+            HIPCHK(c, hipMemsetAsync(L.eacc, 0, sizeof(unsigned long long) * 12 * Ge, c->stream));
+            HIPCHK(c, hipMemsetAsync(L.gridT, 0, sizeof(double) * 4 * Ge, c->stream));
+            HIPCHK(c, hipMemsetAsync(L.gridP, 0, sizeof(double) * 4 * Ge, c->stream));
+            HIPCHK(c, hipMemsetAsync(L.eflag, 0, sizeof(int32_t), c->stream));
+        }
         HIPCHK(c, hipMemsetAsync(L.plus, 0, sizeof(double2) * 4 * (size_t)G, c->stream));
         HIPCHK(c, hipMemsetAsync(L.EB, 0, sizeof(double) * 6 * (size_t)G, c->stream));
         HIPCHK(c, hipMemsetAsync(L.racc, 0, sizeof(unsigned long long) * 12 * (size_t)G, c->stream));
@@ -350,9 +383,11 @@ extern "C" int pic2d3v_create(pic2d3v_ctx** out, const pic2d3v_config* cfg) {
             (rc = dalloc(c, &d.mpart[s], 6 * ((P + kTreeChunk - 1) / kTreeChunk))))
             return fail(rc);
     }
-    if (cfg->field == PIC2D3V_LORENZ_STAGGERED || cfg->field == PIC2D3V_LORENZ) {
+    if (cfg->field == PIC2D3V_LORENZ_STAGGERED || cfg->field == PIC2D3V_LORENZ ||
+        cfg->field == PIC2D3V_LORENZ_SEMI_IMPLICIT) {
         c->lorenz = true;
         c->centred = cfg->field == PIC2D3V_LORENZ;
+        c->semi = cfg->field == PIC2D3V_LORENZ_SEMI_IMPLICIT;
         configure_kernels_2d3v_lg();
         LG23& L = c->lg;
         L.NX = NX;
@@ -371,7 +406,8 @@ extern "C" int pic2d3v_create(pic2d3v_ctx** out, const pic2d3v_config* cfg) {
         L.tw = b.tw;
         double *kx = nullptr, *ky = nullptr;
         const int64_t nf6 = 6 * (int64_t)(NX + 2 * cfg->buffer) * (NY + 2 * cfg->buffer);
-        if ((rc = dalloc(c, &L.F, kLGSlots * G)) || (rc = dalloc(c, &L.plus, 4 * G)) || (rc = dalloc(c, &L.EB, 6 * G)) ||
+        if ((rc = dalloc(c, &L.F, (c->semi ? kLGSlotsSI : kLGSlots) * G)) || (rc = dalloc(c, &L.plus, 4 * G)) ||
+            (rc = dalloc(c, &L.EB, 6 * G)) ||
             (rc = dalloc(c, &kx, NX)) || (rc = dalloc(c, &ky, NY)) || (rc = dalloc(c, &L.racc, 12 * G)) ||
             (rc = dalloc(c, &L.umax, 3 * kMaxSpecies23)) ||
             (rc = dalloc(c, &L.fpart, (nf6 + kTreeChunk - 1) / kTreeChunk)) ||
@@ -380,6 +416,29 @@ extern "C" int pic2d3v_create(pic2d3v_ctx** out, const pic2d3v_config* cfg) {
         for (int s = 0; s < ns; ++s)
             if ((rc = dalloc(c, &c->xh[s], cfg->species[s].P)) || (rc = dalloc(c, &c->yh[s], cfg->species[s].P)))
                 return fail(rc);
+        if (c->semi) {
+            L.theta_s = cfg->theta_source;
+            L.rtol = cfg->rtol;
+            L.b = cfg->buffer;
+            L.NXe = NX + 2 * cfg->buffer;
+            L.NYe = NY + 2 * cfg->buffer;
+            const int64_t Ge = (int64_t)L.NXe * L.NYe;
+            if ((rc = dalloc(c, &L.src, 4 * G)) || (rc = dalloc(c, &L.eacc, 12 * Ge)) ||
+                (rc = dalloc(c, &L.gridT, 4 * Ge)) || (rc = dalloc(c, &L.gridP, 4 * Ge)) ||
+                (rc = dalloc(c, &L.npart, 6 * ((4 * Ge + kTreeChunk - 1) / kTreeChunk))) ||
+                (rc = dalloc(c, &L.res, 8)) || (rc = dalloc(c, &L.eflag, 1)))
+                return fail(rc);
+            for (int s = 0; s < ns; ++s) {
+                Species23& w = c->wk[s];
+                w = c->sp[s];
+                const int64_t P = cfg->species[s].P;
+                if ((rc = dalloc(c, &w.x, P)) || (rc = dalloc(c, &w.y, P)) || (rc = dalloc(c, &w.vx, P)) ||
+                    (rc = dalloc(c, &w.vy, P)) || (rc = dalloc(c, &w.vz, P)))
+                    return fail(rc);
+            }
+            if (hipHostMalloc((void**)&c->hres, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+                return fail(PIC_ENOMEM);
+        }
         // FFTHelper (:251-254): kx = 2pi/Lx * vcat(0:NX/2-1, -NX/2:-1), ky likewise
         std::vector<double> hx(NX), hy(NY);
         const double sx = kTwoPi / cfg->Lx, sy = kTwoPi / cfg->Ly;
@@ -684,6 +743,52 @@ static int capture(pic2d3v_ctx* c, int variant, hipGraphExec_t* out) {
 
 static int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// One LorenzGaugeSemiImplicitField step, loop! (:993-1100): iterations from
+// the step's start state (the particles stay unchanged, the pushes go to the
+// work arrays) until iteration k > maxiters or isapprox(grid before, grid
+// after) (:1000), each read back here; then x, v = xw, vw (the arrays swap),
+// the potentials' rotation, and diag 1/2 as in enqueue_step_lg.
+static int step_si(pic2d3v_ctx* c, int diag) {
+    LG23& L = c->lg;
+    const int ns = c->cfg.nspecies;
+    const int64_t G = (int64_t)c->cfg.NX * c->cfg.NY;
+    hipStream_t st = c->stream;
+    LGOut out{};
+    for (int a = 0; a < 4; ++a) out.a[a] = L.src + (size_t)a * G;
+    int64_t k = 0;
+    for (;;) {
+        ++k;
+        lg_launch_zero_umax(L, st);
+        for (int s = 0; s < ns; ++s) lg_launch_si_push(L, c->g, c->sp[s], c->wk[s], s, st);
+        for (int s = 0; s < ns; ++s) lg_launch_si_deposit(L, c->g, c->wk[s], false, st);
+        lg_launch_si_reduce(L, out, true, st);
+        lg_launch_si_solve(L, st);
+        std::swap(L.gridT, L.gridP);  // the iteration's grid is the next one's "before"
+        HIPCHK(c, hipMemcpyAsync(c->hres, L.res, 5 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        if (c->hres[4] != 0.0) {
+            set_err(c, "a deposit stencil reached outside the buffered grid (the reference's BoundsError); "
+                       "raise buffer");
+            return PIC_EINVAL;
+        }
+        if (k > c->cfg.maxiters || c->hres[3] != 0.0) break;
+    }
+    c->iters.push_back(k);
+    for (int s = 0; s < ns; ++s) {
+        std::swap(c->sp[s].x, c->wk[s].x);
+        std::swap(c->sp[s].y, c->wk[s].y);
+        std::swap(c->sp[s].vx, c->wk[s].vx);
+        std::swap(c->sp[s].vy, c->wk[s].vy);
+        std::swap(c->sp[s].vz, c->wk[s].vz);
+    }
+    lg_launch_rotate(L, st);
+    if (diag == 2) lg_launch_diag_row(L, c->g, c->d, c->sp, ns, st);
+    if (diag >= 1) lg_launch_diag_fields(L, c->d, st);
+    launch23_tick(c->d.t, st);
+    HIPCHK(c, hipGetLastError());
+    return PIC_OK;
+}
+
 extern "C" int pic2d3v_run(pic2d3v_ctx* c, int64_t nsteps, double* rows) {
     if (!c) return PIC_EINVAL;
     if (nsteps < 0) {
@@ -705,11 +810,19 @@ extern "C" int pic2d3v_run(pic2d3v_ctx* c, int64_t nsteps, double* rows) {
     const int64_t t0 = c->step, t1 = c->step + nsteps;
     int rc;
     if ((rc = ensure_bins(c, cdiv(t1, nt)))) return rc;
-    for (int64_t t = t0; t < t1; ++t) {
-        const bool diag = t % nt == 0;
-        hipGraphExec_t& ge = c->lorenz ? c->g_lg[diag ? 2 : 1] : (diag ? c->g_diag : c->g_plain);
-        if (!ge && (rc = capture(c, c->lorenz ? (diag ? 2 : 1) : (diag ? 1 : 0), &ge))) return rc;
-        HIPCHK(c, hipGraphLaunch(ge, c->stream));
+    if (c->semi) {
+        c->iters.clear();
+        for (int64_t t = t0; t < t1; ++t) {
+            if ((rc = step_si(c, t % nt == 0 ? 2 : 1))) return rc;
+            c->step = t + 1;
+        }
+    } else {
+        for (int64_t t = t0; t < t1; ++t) {
+            const bool diag = t % nt == 0;
+            hipGraphExec_t& ge = c->lorenz ? c->g_lg[diag ? 2 : 1] : (diag ? c->g_diag : c->g_plain);
+            if (!ge && (rc = capture(c, c->lorenz ? (diag ? 2 : 1) : (diag ? 1 : 0), &ge))) return rc;
+            HIPCHK(c, hipGraphLaunch(ge, c->stream));
+        }
     }
     if ((rc = finish(c))) return rc;
     c->step = t1;
@@ -755,7 +868,15 @@ extern "C" int pic2d3v_fields(pic2d3v_ctx* c, double* Exs, double* Eys, double* 
 
 static int need_lorenz(pic2d3v_ctx* c, const char* who) {
     if (!c->lorenz) {
-        set_err(c, "%s: the context's field is not PIC2D3V_LORENZ_STAGGERED", who);
+        set_err(c, "%s: the context's field is not a Lorenz-gauge field", who);
+        return PIC_ESTATE;
+    }
+    return PIC_OK;
+}
+
+static int need_semi(pic2d3v_ctx* c, const char* who) {
+    if (!c->semi) {
+        set_err(c, "%s: the context's field is not PIC2D3V_LORENZ_SEMI_IMPLICIT", who);
         return PIC_ESTATE;
     }
     return PIC_OK;
@@ -770,11 +891,48 @@ static int need_uploaded(pic2d3v_ctx* c, const char* who) {
     return PIC_OK;
 }
 
+// warmup!(::LorenzGaugeSemiImplicitField) (:864-871): three passes of the
+// four-weight warmup! (:746-775) -- advect by h, deposit the charge and
+// currents, advect by -h -- into rho-, J- (h = -dt), rho0, J0 (+dt) and the
+// grid rhoJs+ (+dt; its periodic reduction, rho+ and J+, is a temporary),
+// then advect! by -dt.
+static int warmup_si(pic2d3v_ctx* c) {
+    LG23& L = c->lg;
+    const int ns = c->cfg.nspecies;
+    const int64_t G = (int64_t)c->cfg.NX * c->cfg.NY;
+    const double dt = c->cfg.dt;
+    hipStream_t st = c->stream;
+    for (int s = 0; s < ns; ++s) lg_launch_vmax(L, c->sp[s], s, s == 0, st);
+    for (int pass = 0; pass < 3; ++pass) {
+        const double h = pass == 0 ? -dt : dt;
+        const int first = pass == 0 ? kLGRhoM : kLGRho0;
+        LGOut out{};
+        for (int a = 0; a < 4; ++a) out.a[a] = pass == 2 ? L.src + (size_t)a * G : L.F + (size_t)(first + a) * G;
+        for (int s = 0; s < ns; ++s) lg_launch_advect(L, c->g, c->sp[s], h, st);
+        for (int s = 0; s < ns; ++s) lg_launch_si_deposit(L, c->g, c->sp[s], true, st);
+        lg_launch_si_reduce(L, out, false, st);
+        for (int s = 0; s < ns; ++s) lg_launch_advect(L, c->g, c->sp[s], -h, st);
+    }
+    std::swap(L.gridT, L.gridP);
+    for (int s = 0; s < ns; ++s) lg_launch_advect(L, c->g, c->sp[s], -dt, st);
+    int rc;
+    if ((rc = finish(c))) return rc;
+    int32_t flag = 0;
+    HIPCHK(c, hipMemcpy(&flag, L.eflag, sizeof flag, hipMemcpyDeviceToHost));
+    if (flag) {
+        set_err(c, "pic2d3v_warmup: a deposit stencil reached outside the buffered grid (the reference's "
+                   "BoundsError); raise buffer");
+        return PIC_EINVAL;
+    }
+    return PIC_OK;
+}
+
 extern "C" int pic2d3v_warmup(pic2d3v_ctx* c) {
     if (!c) return PIC_EINVAL;
     if (!c->lorenz || c->centred) return PIC_OK;  // warmup!(::AbstractField) = field
     if (int rc = need_uploaded(c, "pic2d3v_warmup")) return rc;
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->semi) return warmup_si(c);
     const int ns = c->cfg.nspecies;
     const double h0 = -c->cfg.dt / 2.0, h1 = c->cfg.dt / 2.0;
     for (int k = 0; k < ns; ++k) lg_launch_advect(c->lg, c->g, c->sp[k], h0, c->stream);
@@ -795,6 +953,14 @@ extern "C" int pic2d3v_loop(pic2d3v_ctx* c, int64_t nsteps) {
     if (int rc = need_uploaded(c, "pic2d3v_loop")) return rc;
     HIPCHK(c, hipSetDevice(c->cfg.device));
     int rc;
+    if (c->semi) {
+        c->iters.clear();
+        for (int64_t t = 0; t < nsteps; ++t) {
+            if ((rc = step_si(c, 0))) return rc;
+            c->step += 1;
+        }
+        return finish(c);
+    }
     if (!c->g_lg[0] && (rc = capture(c, 0, &c->g_lg[0]))) return rc;
     for (int64_t t = 0; t < nsteps; ++t) HIPCHK(c, hipGraphLaunch(c->g_lg[0], c->stream));
     if ((rc = finish(c))) return rc;
@@ -813,7 +979,8 @@ static int check_slot(pic2d3v_ctx* c, const char* who, int32_t which, int32_t n)
 extern "C" int pic2d3v_lg_get(pic2d3v_ctx* c, int32_t which, double* z) {
     if (!c) return PIC_EINVAL;
     if (int rc = need_lorenz(c, "pic2d3v_lg_get")) return rc;
-    if (int rc = check_slot(c, "pic2d3v_lg_get", which, PIC2D3V_LG_NSTATE)) return rc;
+    if (int rc = check_slot(c, "pic2d3v_lg_get", which, c->semi ? PIC2D3V_LG_NSTATE_SI : PIC2D3V_LG_NSTATE))
+        return rc;
     if (!z) {
         set_err(c, "pic2d3v_lg_get: z must not be NULL");
         return PIC_EINVAL;
@@ -827,7 +994,8 @@ extern "C" int pic2d3v_lg_get(pic2d3v_ctx* c, int32_t which, double* z) {
 extern "C" int pic2d3v_lg_set(pic2d3v_ctx* c, int32_t which, const double* z) {
     if (!c) return PIC_EINVAL;
     if (int rc = need_lorenz(c, "pic2d3v_lg_set")) return rc;
-    if (int rc = check_slot(c, "pic2d3v_lg_set", which, PIC2D3V_LG_NSTATE)) return rc;
+    if (int rc = check_slot(c, "pic2d3v_lg_set", which, c->semi ? PIC2D3V_LG_NSTATE_SI : PIC2D3V_LG_NSTATE))
+        return rc;
     if (!z) {
         set_err(c, "pic2d3v_lg_set: z must not be NULL");
         return PIC_EINVAL;
@@ -835,10 +1003,52 @@ extern "C" int pic2d3v_lg_set(pic2d3v_ctx* c, int32_t which, const double* z) {
     HIPCHK(c, hipSetDevice(c->cfg.device));
     const size_t G = (size_t)c->cfg.NX * c->cfg.NY;
     HIPCHK(c, hipMemcpyAsync(c->lg.F + which * G, z, sizeof(double2) * G, hipMemcpyHostToDevice, c->stream));
-    // the centred loop gathers EBxyz before it solves: setting a field sets
-    // what the next step gathers
-    if (c->centred && which >= PIC2D3V_LG_EX) lg_launch_update_eb(c->lg, c->stream);
+    // the centred and semi-implicit loops gather EBxyz before they solve:
+    // setting a field sets what the next step gathers
+    if ((c->centred || c->semi) && which >= PIC2D3V_LG_EX && which <= PIC2D3V_LG_BZ)
+        lg_launch_update_eb(c->lg, c->stream);
     return finish(c);
+}
+
+extern "C" int pic2d3v_lg_grid(pic2d3v_ctx* c, double* out) {
+    if (!c) return PIC_EINVAL;
+    if (int rc = need_semi(c, "pic2d3v_lg_grid")) return rc;
+    if (!out) {
+        set_err(c, "pic2d3v_lg_grid: out must not be NULL");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const size_t n = 4 * (size_t)c->lg.NXe * c->lg.NYe;
+    HIPCHK(c, hipMemcpyAsync(out, c->lg.gridT, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    return finish(c);
+}
+
+extern "C" int pic2d3v_lg_set_grid(pic2d3v_ctx* c, const double* in) {
+    if (!c) return PIC_EINVAL;
+    if (int rc = need_semi(c, "pic2d3v_lg_set_grid")) return rc;
+    if (!in) {
+        set_err(c, "pic2d3v_lg_set_grid: in must not be NULL");
+        return PIC_EINVAL;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const size_t n = 4 * (size_t)c->lg.NXe * c->lg.NYe;
+    HIPCHK(c, hipMemcpyAsync(c->lg.gridT, in, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    return finish(c);
+}
+
+extern "C" int pic2d3v_lg_iterations(pic2d3v_ctx* c, int64_t* out, int64_t n) {
+    if (!c) return PIC_EINVAL;
+    if (int rc = need_semi(c, "pic2d3v_lg_iterations")) return rc;
+    if (n < 0 || n > (int64_t)c->iters.size()) {
+        set_err(c, "pic2d3v_lg_iterations: n must be in [0, %lld]", (long long)c->iters.size());
+        return PIC_EINVAL;
+    }
+    if (n > 0 && !out) {
+        set_err(c, "pic2d3v_lg_iterations: out must not be NULL");
+        return PIC_EINVAL;
+    }
+    for (int64_t k = 0; k < n; ++k) out[k] = c->iters[c->iters.size() - (size_t)n + (size_t)k];
+    return PIC_OK;
 }
 
 extern "C" int pic2d3v_lg_bins(pic2d3v_ctx* c, int32_t which, double* out, int64_t nbins) {

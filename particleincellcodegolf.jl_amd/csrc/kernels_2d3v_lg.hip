@@ -27,6 +27,11 @@
 // the centred move) comes first, then the deposits at the new positions,
 // k_lg_finish<2>, the forward transforms of rho+, J+, k_lg_spectral<true> and
 // the inverse transforms with update!.
+// LorenzGaugeSemiImplicitField's iterated loop! (:993-1100) has kernels of its
+// own at the end of the file: the push into work arrays, exact deposits into
+// the buffered grid, the grid and its isapprox test, the periodic reduction,
+// the wave update with three source weights and the rotation of the
+// potentials.
 #include <climits>
 
 #include "fft_lines.h"
@@ -62,11 +67,14 @@ __device__ __forceinline__ double fmax_nn(double a, double b) { return b > a ? b
 
 // the exact sum of a cell (three 38-bit two's-complement limb counters,
 // modulo 2^114) rounded once
-__device__ __forceinline__ double lg_limbs(const u64* r, int64_t G, int64_t k, int q) {
-    u128 v = (u128)r[k] + ((u128)r[G + k] << kLimbS) + ((u128)r[2 * G + k] << (2 * kLimbS));
+__device__ __forceinline__ double lg_limbs_v(u64 a0, u64 a1, u64 a2, int q) {
+    u128 v = (u128)a0 + ((u128)a1 << kLimbS) + ((u128)a2 << (2 * kLimbS));
     v &= ((u128)1 << (3 * kLimbS)) - 1;
     const i128 s = (v >> (3 * kLimbS - 1)) ? (i128)v - ((i128)1 << (3 * kLimbS)) : (i128)v;
     return i128_to_double(s, q);
+}
+__device__ __forceinline__ double lg_limbs(const u64* r, int64_t G, int64_t k, int q) {
+    return lg_limbs_v(r[k], r[G + k], r[2 * G + k], q);
 }
 
 // q of a sum bounded by sum_s P_s*(|qw_s|*M_s) (M = 1 for the charge)
@@ -348,11 +356,14 @@ __device__ __forceinline__ void lg_window(const Grid23& g, const double* xs, con
     __syncthreads();
 }
 
-// CENTRED (LorenzGaugeField, :683-690): x = unimod(x + (vx0 + vx)/2*dt);
-// else the staggered loop's two half moves, the first written to (xh, yh).
-template <int SH, bool CENTRED>
+// MODE 0: the staggered loop's two half moves, the first written to (xh, yh);
+// 1 (LorenzGaugeField, :683-690): x = unimod(x + (vx0 + vx)/2*dt); 2
+// (LorenzGaugeSemiImplicitField, :1036-1040): xw = unimod(x + vx0*dt) and vw
+// into the work arrays o, s unchanged (modes 0 and 1 write s, o unused).
+template <int SH, int MODE>
 __global__ void __launch_bounds__(kTile23Threads)
-k_lg_push(LG23 L, Grid23 g, Species23 s, int sidx, double* __restrict__ xh, double* __restrict__ yh) {
+k_lg_push(LG23 L, Grid23 g, Species23 s, Species23 o, int sidx, double* __restrict__ xh, double* __restrict__ yh) {
+    constexpr bool CENTRED = MODE == 1;
     constexpr int n = ShapeCount<SH>::value;
     __shared__ double Ew[6 * kLGWinCells];
     __shared__ int red[4][4];
@@ -424,7 +435,13 @@ k_lg_push(LG23 L, Grid23 g, Species23 s, int sidx, double* __restrict__ xh, doub
             }
         const double vx0 = vx, vy0 = vy;
         lg_boris(dt, s.q_m, f, vx, vy, vz);
-        if (CENTRED) {
+        if (MODE == 2) {
+            o.x[p] = unimod23(x + vx0 * dt, g.Lx);
+            o.y[p] = unimod23(y + vy0 * dt, g.Ly);
+            o.vx[p] = vx;
+            o.vy[p] = vy;
+            o.vz[p] = vz;
+        } else if (CENTRED) {
             x = unimod23(x + (vx0 + vx) / 2.0 * dt, g.Lx);
             y = unimod23(y + (vy0 + vy) / 2.0 * dt, g.Ly);
         } else {
@@ -435,11 +452,13 @@ k_lg_push(LG23 L, Grid23 g, Species23 s, int sidx, double* __restrict__ xh, doub
             x = unimod23(x + vx * dt / 2.0, g.Lx);
             y = unimod23(y + vy * dt / 2.0, g.Ly);
         }
-        s.x[p] = x;
-        s.y[p] = y;
-        s.vx[p] = vx;
-        s.vy[p] = vy;
-        s.vz[p] = vz;
+        if (MODE != 2) {
+            s.x[p] = x;
+            s.y[p] = y;
+            s.vx[p] = vx;
+            s.vy[p] = vy;
+            s.vz[p] = vz;
+        }
         m1 = fmax_nn(m1, fabs(vx));
         m2 = fmax_nn(m2, fabs(vy));
         m3 = fmax_nn(m3, fabs(vz));
@@ -664,8 +683,8 @@ template <int SH>
 struct PushL {
     static void go(const LG23& L, const Grid23& g, const Species23& s, int sidx, double* xh, double* yh,
                    bool centred, hipStream_t st) {
-        if (centred) k_lg_push<SH, true><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s, sidx, xh, yh);
-        else k_lg_push<SH, false><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s, sidx, xh, yh);
+        if (centred) k_lg_push<SH, 1><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s, s, sidx, xh, yh);
+        else k_lg_push<SH, 0><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s, s, sidx, xh, yh);
     }
 };
 template <int SH>
@@ -864,6 +883,447 @@ void lg_launch_diag_fields(const LG23& L, const Diag23& d, hipStream_t st) {
     lg_inverse(lst, 8, L, false, st);
     k_lg_bins<<<grid_blocks(kLGBins * (int64_t)d.NXd * d.NYd), 256, 0, st>>>(L, d);
 }
+
+// ---------------------------------------------------------------------------
+// LorenzGaugeSemiImplicitField (:403-461, warmup! :864-871, loop! :993-1100)
+// ---------------------------------------------------------------------------
+// The deposit grid is the reference's buffered one: Julia cell c (1-based,
+// unwrapped, axes -(b-1):NX+b) is row c - 1 + b of the (NX+2b)-wide grid.
+// The exact limbs are summed per buffered cell; the periodic reduction adds
+// a cell's images' counters (exact) and rounds once.
+
+// the buffered row of the base cell ceil(z*NZ_Lz) (clamped far outside)
+__device__ __forceinline__ int si_base(double z, double NZ_Lz, int b) {
+    double ci = ceil(z * NZ_Lz);
+    ci = ci == ci ? (ci > 1.0e8 ? 1.0e8 : (ci < -1.0e8 ? -1.0e8 : ci)) : 0.0;
+    return (int)ci - 1 + b;
+}
+
+// The window of a chunk in the buffered grid: the block-reduced extent of the
+// particles' base cells widened by the stencil and a halo, clipped to the
+// grid; win = (ox, oy, wx, wy, fits in CELLS).
+template <int SH, int CELLS>
+__device__ __forceinline__ void si_window(const LG23& L, const Grid23& g, const double* xs, const double* ys, int nv,
+                                          int ax, int ay, int (*red)[4], int* win) {
+    constexpr int LO = ShapeExtent<SH>::lo, HI = ShapeExtent<SH>::hi;
+    const int t = threadIdx.x;
+    int mnx = INT_MAX / 2, mxx = INT_MIN / 2, mny = INT_MAX / 2, mxy = INT_MIN / 2;
+#pragma unroll
+    for (int k = 0; k < kTile23PPT; ++k) {
+        if (k >= nv) break;
+        const int dx = si_base(xs[k], g.NX_Lx, L.b) - ax, dy = si_base(ys[k], g.NY_Ly, L.b) - ay;
+        mnx = min(mnx, dx);
+        mxx = max(mxx, dx);
+        mny = min(mny, dy);
+        mxy = max(mxy, dy);
+    }
+    mnx = wave_min_i(mnx);
+    mxx = wave_max_i(mxx);
+    mny = wave_min_i(mny);
+    mxy = wave_max_i(mxy);
+    const int wv = t >> 6;
+    if ((t & 63) == 0) {
+        red[wv][0] = mnx;
+        red[wv][1] = mxx;
+        red[wv][2] = mny;
+        red[wv][3] = mxy;
+    }
+    __syncthreads();
+    if (t == 0) {
+        for (int w = 1; w < kTile23Threads / 64; ++w) {
+            mnx = min(mnx, red[w][0]);
+            mxx = max(mxx, red[w][1]);
+            mny = min(mny, red[w][2]);
+            mxy = max(mxy, red[w][3]);
+        }
+        int64_t sx = (int64_t)ax + mnx - LO - kTile23Halo, ex = (int64_t)ax + mxx + HI + kTile23Halo + 1;
+        int64_t sy = (int64_t)ay + mny - LO - kTile23Halo, ey = (int64_t)ay + mxy + HI + kTile23Halo + 1;
+        sx = sx < 0 ? 0 : sx;
+        sy = sy < 0 ? 0 : sy;
+        ex = ex > L.NXe ? L.NXe : ex;
+        ey = ey > L.NYe ? L.NYe : ey;
+        const int64_t wx = ex - sx, wy = ey - sy;
+        win[0] = (int)(wx > 0 ? sx : 0);
+        win[1] = (int)(wy > 0 ? sy : 0);
+        win[2] = (int)(wx > 0 ? wx : 0);
+        win[3] = (int)(wy > 0 ? wy : 0);
+        win[4] = wx > 0 && wy > 0 && wx * wy <= CELLS;
+    }
+    __syncthreads();
+}
+
+// The exact deposits of a species at its positions into the buffered limbs.
+// NG = 4: qw, vx*qw, vy*qw, vz*qw (warmup!, :759-760); NG = 3: the currents
+// (the loop, :1042-1043).  A stencil cell outside the buffered grid (the
+// reference's BoundsError) drops the particle and sets eflag.
+template <int SH, int NG>
+__global__ void __launch_bounds__(kTile23Threads) k_lg_si_deposit(LG23 L, Grid23 g, Species23 s) {
+    constexpr int n = ShapeCount<SH>::value;
+    constexpr int c0 = 4 - NG;                      // first component: 0 rho, 1 Jx
+    constexpr int CELLS = NG == 4 ? 512 : kLGDepCells;
+    __shared__ u64 Rw[NG * 3 * CELLS];
+    __shared__ int red[4][4];
+    __shared__ int win[5];
+    const int t = threadIdx.x;
+    const int NXe = L.NXe, NYe = L.NYe, b = L.b;
+    const int64_t Ge = (int64_t)NXe * NYe;
+    const int64_t c0p = (int64_t)blockIdx.x * kTile23Chunk;
+    // lanes of a wave in different cells, as in k_lg_deposit
+    const int64_t pb = c0p + (int64_t)(t & 63) * (kTile23Threads / 64 * kTile23PPT) + (t >> 6) * kTile23PPT;
+    double xs[kTile23PPT], ys[kTile23PPT];
+    int nv = 0;
+#pragma unroll
+    for (int k = 0; k < kTile23PPT; ++k) {
+        const int64_t p = pb + k;
+        xs[k] = 0.0;
+        ys[k] = 0.0;
+        if (p < s.P) {
+            xs[k] = s.x[p];
+            ys[k] = s.y[p];
+            nv = k + 1;
+        }
+    }
+    const int ax = si_base(s.x[c0p], g.NX_Lx, b), ay = si_base(s.y[c0p], g.NY_Ly, b);
+    si_window<SH, CELLS>(L, g, xs, ys, nv, ax, ay, red, win);
+    const int ox = win[0], oy = win[1], wx = win[2], wy = win[3];
+    const bool tile = win[4] != 0;
+    if (tile)
+        for (int e = t; e < NG * 3 * CELLS; e += kTile23Threads) Rw[e] = 0;
+    __syncthreads();
+    int q[NG];
+#pragma unroll
+    for (int c = 0; c < NG; ++c) q[c] = c0 + c == 0 ? L.qr : lg_frac_bits(L, c0 + c - 1);
+    constexpr u64 M = (1ull << kLimbS) - 1;
+#pragma unroll 1
+    for (int k = 0; k < kTile23PPT; ++k) {
+        const int64_t p = pb + k;
+        if (p >= s.P) break;
+        double w[NG];
+        {
+            const double v[4] = {1.0, s.vx[p], s.vy[p], s.vz[p]};
+#pragma unroll
+            for (int c = 0; c < NG; ++c) w[c] = c0 + c == 0 ? s.qw : v[c0 + c] * s.qw;
+        }
+        int32_t cx[n], cy[n];
+        double wxs[n], wys[n];
+        shape_w<SH>(xs[k], g.NX_Lx, 0, cx, wxs);
+        shape_w<SH>(ys[k], g.NY_Ly, 0, cy, wys);
+        bool ok = true, in = tile;
+        int ux[n], uy[n];
+#pragma unroll
+        for (int qq = 0; qq < n; ++qq) {
+            ux[qq] = cx[qq] + b;
+            uy[qq] = cy[qq] + b;
+            ok = ok && (unsigned)ux[qq] < (unsigned)NXe && (unsigned)uy[qq] < (unsigned)NYe;
+            in = in && (unsigned)(ux[qq] - ox) < (unsigned)wx && (unsigned)(uy[qq] - oy) < (unsigned)wy;
+        }
+        if (!ok) {
+            atomicOr(L.eflag, 1);
+            continue;
+        }
+#pragma unroll
+        for (int jj = 0; jj < n; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < n; ++ii) {
+                const double wxy = wxs[ii] * wys[jj];
+#pragma unroll
+                for (int c = 0; c < NG; ++c) {
+                    const double term = wxy * w[c];
+                    if (term == 0.0) continue;
+                    const u128 u = (u128)fix_round128(term, q[c]);
+                    const u64 l[3] = {(u64)u & M, (u64)(u >> kLimbS) & M, (u64)(u >> (2 * kLimbS)) & M};
+                    if (in) {
+                        const int e = (ux[ii] - ox) + wx * (uy[jj] - oy);
+#pragma unroll
+                        for (int bb = 0; bb < 3; ++bb)
+                            if (l[bb]) atomicAdd(Rw + (c * 3 + bb) * CELLS + e, l[bb]);
+                    } else {
+                        u64* r = L.eacc + (size_t)(c0 + c) * 3 * Ge;
+                        const int64_t kk = ux[ii] + (int64_t)NXe * uy[jj];
+#pragma unroll
+                        for (int bb = 0; bb < 3; ++bb)
+                            if (l[bb]) atomicAdd(r + bb * Ge + kk, l[bb]);
+                    }
+                }
+            }
+    }
+    __syncthreads();
+    if (tile)
+        for (int e = t; e < wx * wy; e += kTile23Threads) {
+            const int ry = e / wx, rx = e - ry * wx;
+            const int64_t k = (ox + rx) + (int64_t)NXe * (oy + ry);
+#pragma unroll
+            for (int c = 0; c < NG; ++c)
+#pragma unroll
+                for (int bb = 0; bb < 3; ++bb) {
+                    const u64 v = Rw[(c * 3 + bb) * CELLS + e];
+                    if (v) atomicAdd(L.eacc + (size_t)(c0 + c) * 3 * Ge + bb * Ge + k, v);
+                }
+        }
+}
+
+// The grid rhoJs+ from the limbs into gridP (element i = component i % 4 of
+// buffered cell i / 4, the reference's order) and, TEST, the chunk partials
+// of norm(gridT - gridP), norm(gridT), norm(gridP): canonical sums of squares
+// and the maxima of |.| (generic_norm2).
+template <bool TEST>
+__global__ void __launch_bounds__(256) k_lg_si_norm(LG23 L, int64_t nch) {
+    __shared__ double red[3 * kTreeLanes + 8];
+    const int64_t Ge = (int64_t)L.NXe * L.NYe, n = 4 * Ge;
+    const int64_t s0 = (int64_t)blockIdx.x * kTreeChunk;
+    const int64_t m = n - s0 < kTreeChunk ? n - s0 : kTreeChunk;
+    int q[4];
+    q[0] = L.qr;
+    for (int c = 0; c < 3; ++c) q[c + 1] = lg_frac_bits(L, c);
+    double sd = 0.0, st = 0.0, sp = 0.0, md = 0.0, mt = 0.0, mp = 0.0;
+    if (threadIdx.x < kTreeLanes)
+        for (int64_t i = threadIdx.x; i < m; i += kTreeLanes) {
+            const int64_t idx = s0 + i;
+            const int comp = (int)(idx & 3);
+            const double p = lg_limbs(L.eacc + (size_t)comp * 3 * Ge, Ge, idx >> 2, q[comp]);
+            L.gridP[idx] = p;
+            if (TEST) {
+                const double tv = L.gridT[idx], d = tv - p;
+                sd = sd + d * d;
+                st = st + tv * tv;
+                sp = sp + p * p;
+                md = nanmax(md, fabs(d));
+                mt = nanmax(mt, fabs(tv));
+                mp = nanmax(mp, fabs(p));
+            }
+        }
+    if (!TEST) return;
+    tree256x3<false>(red, sd, st, sp);
+    tree256x3<true>(red, md, mt, mp);
+    if (threadIdx.x == 0) {
+        const double v[6] = {sd, st, sp, md, mt, mp};
+        for (int k = 0; k < 6; ++k) L.npart[k * nch + blockIdx.x] = v[k];
+    }
+}
+
+// isapprox(gridT, gridP; rtol, atol = 0) (:1000) as LinearAlgebra's array
+// isapprox: d = norm(T - P) finite -> d <= rtol*max(norm(T), norm(P)), else
+// elementwise.  One block; res = (d, norm T, norm P, converged, eflag).
+__global__ void __launch_bounds__(256) k_lg_si_test(LG23 L, int64_t nch) {
+    __shared__ double red[kTreeLanes + 8];
+    __shared__ int bad;
+    const int t = threadIdx.x;
+    const int64_t Ge = (int64_t)L.NXe * L.NYe, n = 4 * Ge;
+    double sums[3], maxs[3], nrm[3];
+    for (int k = 0; k < 3; ++k) sums[k] = tree_strided(red, L.npart + (size_t)k * nch, 1, nch);
+    for (int k = 0; k < 3; ++k) {
+        double mm = 0.0;
+        if (t < kTreeLanes)
+            for (int64_t i = t; i < nch; i += kTreeLanes) mm = nanmax(mm, L.npart[(size_t)(3 + k) * nch + i]);
+        maxs[k] = tree256<true>(red, mm);
+    }
+    for (int k = 0; k < 3; ++k) {
+        const double m = maxs[k];
+        if (m == 0.0 || isinf(m)) {
+            nrm[k] = m;
+        } else if (isfinite((double)n * m * m) && m * m != 0.0) {
+            nrm[k] = sqrt(sums[k]);
+        } else {
+            // scaled: m*sqrt(sum (|x|/m)^2), the partials' storage reused
+            for (int64_t c = 0; c < nch; ++c) {
+                const int64_t s0 = c * kTreeChunk, mc = n - s0 < kTreeChunk ? n - s0 : kTreeChunk;
+                const double r = block_chunk_sum(red, mc, [&](int64_t i) {
+                    const double tv = L.gridT[s0 + i], pv = L.gridP[s0 + i];
+                    const double x = k == 0 ? tv - pv : (k == 1 ? tv : pv);
+                    const double qv = fabs(x) / m;
+                    return qv * qv;
+                });
+                if (t == 0) L.npart[(size_t)k * nch + c] = r;
+                __syncthreads();
+            }
+            nrm[k] = m * sqrt(tree_strided(red, L.npart + (size_t)k * nch, 1, nch));
+        }
+    }
+    const double dn = nrm[0], rtol = L.rtol;
+    int conv;
+    if (isfinite(dn)) {
+        const double nf = nrm[1], ne = nrm[2];
+        double mx = nf > ne ? nf : ne;
+        if (nf != nf || ne != ne) mx = NAN;
+        const double tol = rtol * mx;
+        double lim = 0.0 > tol ? 0.0 : tol;
+        if (tol != tol) lim = tol;
+        conv = dn <= lim;
+    } else {
+        if (t == 0) bad = 0;
+        __syncthreads();
+        for (int64_t i = t; i < n; i += blockDim.x) {
+            const double a = L.gridT[i], c = L.gridP[i];
+            if (a == c) continue;
+            if (!(isfinite(a) && isfinite(c))) {
+                bad = 1;
+                continue;
+            }
+            const double ma = fabs(a) > fabs(c) ? fabs(a) : fabs(c);
+            const double tt = rtol * ma;
+            if (!(fabs(a - c) <= (0.0 > tt ? 0.0 : tt))) bad = 1;
+        }
+        __syncthreads();
+        conv = !bad;
+    }
+    if (t == 0) {
+        L.res[0] = dn;
+        L.res[1] = nrm[1];
+        L.res[2] = nrm[2];
+        L.res[3] = conv ? 1.0 : 0.0;
+        L.res[4] = (double)*L.eflag;
+    }
+}
+
+// reduction! (:504-522): each periodic cell's exact sum over its images in
+// the buffered grid, rounded once, into out (imaginary parts 0); the limbs
+// of the images are cleared.
+__global__ void __launch_bounds__(256) k_lg_si_wrap(LG23 L, LGOut out) {
+    const int NX = L.NX, NY = L.NY, NXe = L.NXe, NYe = L.NYe, b = L.b;
+    const int64_t G = (int64_t)NX * NY, Ge = (int64_t)NXe * NYe;
+    int q[4];
+    q[0] = L.qr;
+    for (int c = 0; c < 3; ++c) q[c + 1] = lg_frac_bits(L, c);
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < G; k += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(k & (NX - 1)), j = (int)(k >> L.lx);
+        const int u0 = (i + b) & (NX - 1), v0 = (j + b) & (NY - 1);
+#pragma unroll
+        for (int comp = 0; comp < 4; ++comp) {
+            u64* r = L.eacc + (size_t)comp * 3 * Ge;
+            u64 a0 = 0, a1 = 0, a2 = 0;
+            for (int v = v0; v < NYe; v += NY)
+                for (int u = u0; u < NXe; u += NX) {
+                    const int64_t e = u + (int64_t)NXe * v;
+                    a0 += r[e];
+                    a1 += r[Ge + e];
+                    a2 += r[2 * Ge + e];
+                    r[e] = 0;
+                    r[Ge + e] = 0;
+                    r[2 * Ge + e] = 0;
+                }
+            out.a[comp][k] = make_double2(lg_limbs_v(a0, a1, a2, q[comp]), 0.0);
+        }
+    }
+}
+
+// lorenzgauge! with the three source weights (:638-648) for phi (rho) and A
+// (J), then E and B with the field theta (:1066-1077).  src is in the
+// spectrum; its [1,1] is cleared here (:1055-1059).
+__global__ void __launch_bounds__(256) k_lg_spectral_si(LG23 L) {
+    const int NX = L.NX;
+    const int64_t G = (int64_t)NX * L.NY;
+    const double dt = L.dt, dt2 = dt * dt, dt_2 = 2.0 * dt, thf = L.theta, ths = L.theta_s;
+    const double hs = ths / 2.0, ws = 1.0 - ths, hf = thf / 2.0, wf = 1.0 - thf;
+    double2* F = L.F;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < G; k += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(k & (NX - 1)), j = (int)(k >> L.lx);
+        const double kxi = L.kx[i], kyj = L.ky[j];
+        const double k2 = kxi * kxi + kyj * kyj;
+        const double num = 2.0 - dt2 * k2 * (1.0 - thf);
+        const double den = 1.0 + dt2 * k2 * thf / 2.0;
+        const int x0s[4] = {kLGPhi0, kLGAx0, kLGAy0, kLGAz0};
+        const int xms[4] = {kLGPhiM, kLGAxM, kLGAyM, kLGAzM};
+        const int s0s[4] = {kLGRho0, kLGJx0, kLGJy0, kLGJz0};
+        const int sms[4] = {kLGRhoM, kLGJxM, kLGJyM, kLGJzM};
+        double2 pl[4], cc[4], xm[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const double2 sm = F[sms[a] * G + k], s0 = F[s0s[a] * G + k];
+            double2 sp = L.src[a * G + k];
+            if (k == 0) {
+                sp = make_double2(0.0, 0.0);
+                L.src[a * G] = sp;
+            }
+            const double2 x0 = F[x0s[a] * G + k];
+            xm[a] = F[xms[a] * G + k];
+            const double svr = hs * sm.x + ws * s0.x + hs * sp.x;
+            const double svi = hs * sm.y + ws * s0.y + hs * sp.y;
+            pl[a].x = (num * x0.x + dt2 * svr) / den - xm[a].x;
+            pl[a].y = (num * x0.y + dt2 * svi) / den - xm[a].y;
+            cc[a].x = hf * (pl[a].x + xm[a].x) + wf * x0.x;
+            cc[a].y = hf * (pl[a].y + xm[a].y) + wf * x0.y;
+            L.plus[a * G + k] = pl[a];
+        }
+        double er, ei, fr, fi;
+        cmul_lg(0.0 * kxi, -kxi, cc[0].x, cc[0].y, er, ei);
+        F[kLGEx * G + k] = make_double2(er - (pl[1].x - xm[1].x) / dt_2, ei - (pl[1].y - xm[1].y) / dt_2);
+        cmul_lg(0.0 * kyj, -kyj, cc[0].x, cc[0].y, er, ei);
+        F[kLGEy * G + k] = make_double2(er - (pl[2].x - xm[2].x) / dt_2, ei - (pl[2].y - xm[2].y) / dt_2);
+        F[kLGEz * G + k] = make_double2(-(pl[3].x - xm[3].x) / dt_2, -(pl[3].y - xm[3].y) / dt_2);
+        cmul_lg(0.0 * kyj, kyj, cc[3].x, cc[3].y, er, ei);
+        F[kLGBx * G + k] = make_double2(er, ei);
+        cmul_lg(0.0 * kxi, -kxi, cc[3].x, cc[3].y, er, ei);
+        F[kLGBy * G + k] = make_double2(er, ei);
+        cmul_lg(0.0 * kxi, kxi, cc[2].x, cc[2].y, er, ei);
+        cmul_lg(0.0 * kyj, kyj, cc[1].x, cc[1].y, fr, fi);
+        F[kLGBz * G + k] = make_double2(er - fr, ei - fi);
+    }
+}
+
+// phi- = phi0, phi0 = phi+, A- = A0, A0 = A+ (:1089-1100)
+__global__ void __launch_bounds__(256) k_lg_rotate(LG23 L) {
+    const int64_t G = (int64_t)L.NX * L.NY;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < G; k += (int64_t)gridDim.x * blockDim.x) {
+        const int x0s[4] = {kLGPhi0, kLGAx0, kLGAy0, kLGAz0};
+        const int xms[4] = {kLGPhiM, kLGAxM, kLGAyM, kLGAzM};
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            L.F[xms[a] * G + k] = L.F[x0s[a] * G + k];
+            L.F[x0s[a] * G + k] = L.plus[a * G + k];
+        }
+    }
+}
+
+template <int SH>
+struct PushSIL {
+    static void go(const LG23& L, const Grid23& g, const Species23& s, const Species23& o, int sidx, hipStream_t st) {
+        k_lg_push<SH, 2><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s, o, sidx, nullptr, nullptr);
+    }
+};
+template <int SH>
+struct DepSIL {
+    static void go(const LG23& L, const Grid23& g, const Species23& s, bool charge, hipStream_t st) {
+        if (charge) k_lg_si_deposit<SH, 4><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s);
+        else k_lg_si_deposit<SH, 3><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s);
+    }
+};
+
+void lg_launch_si_push(const LG23& L, const Grid23& g, const Species23& s, const Species23& o, int sidx,
+                       hipStream_t st) {
+    if (s.P > 0) by_shape<PushSIL>(s.shape, L, g, s, o, sidx, st);
+}
+
+void lg_launch_si_deposit(const LG23& L, const Grid23& g, const Species23& s, bool charge, hipStream_t st) {
+    if (s.P > 0) by_shape<DepSIL>(s.shape, L, g, s, charge, st);
+}
+
+void lg_launch_si_reduce(const LG23& L, const LGOut& out, bool test, hipStream_t st) {
+    const int64_t n = 4 * (int64_t)L.NXe * L.NYe, nch = (n + kTreeChunk - 1) / kTreeChunk;
+    if (test) {
+        k_lg_si_norm<true><<<(unsigned)nch, 256, 0, st>>>(L, nch);
+        k_lg_si_test<<<1, 256, 0, st>>>(L, nch);
+    } else {
+        k_lg_si_norm<false><<<(unsigned)nch, 256, 0, st>>>(L, nch);
+    }
+    k_lg_si_wrap<<<grid_blocks((int64_t)L.NX * L.NY), 256, 0, st>>>(L, out);
+}
+
+void lg_launch_si_solve(const LG23& L, hipStream_t st) {
+    const int64_t G = (int64_t)L.NX * L.NY;
+    LGList src{};
+    for (int c = 0; c < 4; ++c) src.a[c] = L.src + (size_t)c * G;
+    lg_forward(src, 4, L, st);
+    k_lg_spectral_si<<<grid_blocks(G), 256, 0, st>>>(L);
+    LGList lst{};
+    for (int c = 0; c < 6; ++c) lst.a[c] = L.F + (size_t)(kLGEx + c) * G;
+    lg_inverse(lst, 6, L, true, st);
+}
+
+void lg_launch_rotate(const LG23& L, hipStream_t st) {
+    k_lg_rotate<<<grid_blocks((int64_t)L.NX * L.NY), 256, 0, st>>>(L);
+}
+
+void lg_launch_zero_umax(const LG23& L, hipStream_t st) { k_lg_zero_umax<<<1, 64, 0, st>>>(L); }
 
 int configure_kernels_2d3v_lg() {
     const void* ks[] = {reinterpret_cast<const void*>(k_lg_cols<true>), reinterpret_cast<const void*>(k_lg_cols<false>),

@@ -1,7 +1,8 @@
 // pic2d3v_lg_internal.h -- descriptors and launchers of the Lorenz-gauge
 // electromagnetic path of the structured 2D3V module (LorenzGaugeStaggeredField
-// and LorenzGaugeField of src/PIC2D3V.jl), shared by the host runtime (capi2d3v.cpp) and the
-// kernels (kernels_2d3v_lg.hip).
+// LorenzGaugeField and LorenzGaugeSemiImplicitField of src/PIC2D3V.jl),
+// shared by the host runtime (capi2d3v.cpp) and the kernels
+// (kernels_2d3v_lg.hip).
 #pragma once
 #include "pic2d3v_internal.h"
 
@@ -12,6 +13,8 @@ enum LGSlot {
     kLGPhi0 = 0, kLGPhiM, kLGAx0, kLGAy0, kLGAz0, kLGAxM, kLGAyM, kLGAzM,
     kLGRho0, kLGJx0, kLGJy0, kLGJz0, kLGEx, kLGEy, kLGEz, kLGBx, kLGBy, kLGBz, kLGSlots
 };
+// LorenzGaugeSemiImplicitField's sources at n-1 (warmup! at -dt), after the 18
+enum { kLGRhoM = kLGSlots, kLGJxM, kLGJyM, kLGJzM, kLGSlotsSI };
 constexpr int kLGBins = 14;      // binned fields of LorenzGaugeDiagnostics
 constexpr int kLGRow = 11;       // doubles per diagnostics row
 constexpr int kLGLinePoints = 4096;  // transform points per block (whole lines)
@@ -37,6 +40,18 @@ struct LG23 {
     int64_t cap;
     double* fpart;       // chunk partials of the field energy
     double* mfpart;      // 3 x chunk partials of the field momentum
+    // LorenzGaugeSemiImplicitField: the deposit grid is the buffered
+    // (NX+2b) x (NY+2b) one of the reference's offset arrays
+    double theta_s;      // sourceimex theta
+    double rtol;         // the convergence test's rtol (atol = 0)
+    int32_t b, NXe, NYe;
+    double2* src;        // 4 x G: rho+, Jx+, Jy+, Jz+
+    unsigned long long* eacc;  // 4 components x 3 limbs x Ge: the buffered exact sums
+    double* gridT;       // 4 x Ge, component fastest: the grid before the iteration
+    double* gridP;       // 4 x Ge: the iteration's grid
+    double* npart;       // 6 x chunk partials of the three norms (sums, then maxima)
+    double* res;         // norm(T - P), norm(T), norm(P), converged, stencil outside the buffer
+    int32_t* eflag;      // set by a deposit whose stencil leaves the buffered grid
 };
 
 // step 1-2 of loop!: the wave update, E and B in the spectrum, their inverse
@@ -65,6 +80,27 @@ void lg_launch_warm_finish(const LG23& L, hipStream_t st);
 void lg_launch_diag_row(const LG23& L, const Grid23& g, const Diag23& d, const Species23* sp, int ns,
                         hipStream_t st);
 void lg_launch_diag_fields(const LG23& L, const Diag23& d, hipStream_t st);
+// LorenzGaugeSemiImplicitField (loop! :993-1100, warmup! :864-871):
+// the push from the step's start state s into the work arrays o
+// (xw = unimod(x + vx*dt) with the old v, vw = Boris), per-species maxima
+void lg_launch_si_push(const LG23& L, const Grid23& g, const Species23& s, const Species23& o, int sidx,
+                       hipStream_t st);
+// the exact deposits into the buffered grid at s's positions: the charge and
+// currents (charge) or the currents alone
+void lg_launch_si_deposit(const LG23& L, const Grid23& g, const Species23& s, bool charge, hipStream_t st);
+// the buffered grid into gridP (test: with isapprox(gridT, gridP) into res)
+// and its periodic reduction into out[0..3] (imaginary parts 0); the limbs
+// are cleared
+struct LGOut {
+    double2* a[4];
+};
+void lg_launch_si_reduce(const LG23& L, const LGOut& out, bool test, hipStream_t st);
+// fft! of rho+, J+, the wave update with the three source weights, E and B
+// in the spectrum, their inverse transforms and update!
+void lg_launch_si_solve(const LG23& L, hipStream_t st);
+// phi- = phi0, phi0 = phi+, A- = A0, A0 = A+
+void lg_launch_rotate(const LG23& L, hipStream_t st);
+void lg_launch_zero_umax(const LG23& L, hipStream_t st);
 int configure_kernels_2d3v_lg();
 
 }  // namespace pic

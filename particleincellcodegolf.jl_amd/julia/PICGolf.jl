@@ -40,6 +40,7 @@ export Context, PicConfig, deposit!, fieldsolve!, drift!, midpoint_position!, wr
        simpson_kick!, Pic1D2VConfig, Context1D2V, fields,
        Pic2DConfig, Context2D, gather, boris!,
        Pic2D3VSpecies, Pic2D3VConfig, Context2D3V, reset!, field, warmup!, loop!, lg_field, lg_field!, lg_bins,
+       lg_grid, lg_grid!, lg_iterations,
        PicEM1DConfig, ContextEM1D, explicit_fields!, explicit_push
 
 const libpicgolf = get(ENV, "PICGOLF_LIB", normpath(joinpath(@__DIR__, "..", "lib", "libpicgolf.so")))
@@ -691,10 +692,12 @@ const SHAPES2D3V = Dict(:ngp => Int32(0), :area => Int32(1), :bspline0 => Int32(
                         :bspline2 => Int32(4), :bspline3 => Int32(5), :bspline4 => Int32(6), :bspline5 => Int32(7))
 const FIELD_UPDATES = Dict(:accumulate => Int32(0), :replace => Int32(1))
 const BORIS_VARIANTS = Dict(:reference => Int32(0), :textbook => Int32(1))
-const FIELDS2D3V = Dict(:electrostatic => Int32(0), :lorenz_staggered => Int32(1), :lorenz => Int32(2))
+const FIELDS2D3V = Dict(:electrostatic => Int32(0), :lorenz_staggered => Int32(1), :lorenz => Int32(2),
+                        :lorenz_semi_implicit => Int32(3))
 const MAX_SPECIES_2D3V = 4
 # the Lorenz-gauge state slots and binned fields of include/pic2d3v.h
-const LG_STATE = (:ϕ⁰, :ϕ⁻, :Ax⁰, :Ay⁰, :Az⁰, :Ax⁻, :Ay⁻, :Az⁻, :ρ⁰, :Jx⁰, :Jy⁰, :Jz⁰, :Ex, :Ey, :Ez, :Bx, :By, :Bz)
+const LG_STATE = (:ϕ⁰, :ϕ⁻, :Ax⁰, :Ay⁰, :Az⁰, :Ax⁻, :Ay⁻, :Az⁻, :ρ⁰, :Jx⁰, :Jy⁰, :Jz⁰, :Ex, :Ey, :Ez, :Bx, :By, :Bz,
+                  :ρ⁻, :Jx⁻, :Jy⁻, :Jz⁻)  # the last four: LorenzGaugeSemiImplicitField only
 const LG_BINS = (:Exs, :Eys, :Ezs, :Bxs, :Bys, :Bzs, :Axs, :Ays, :Azs, :ϕs, :ρs, :Jxs, :Jys, :Jzs)
 
 "Mirror of `struct pic2d3v_species` in include/pic2d3v.h."
@@ -733,11 +736,15 @@ struct Pic2D3VConfig
     device::Int32
     field::Int32
     theta::Float64
+    theta_source::Float64
+    rtol::Float64
+    maxiters::Int32
 end
 
 """
     Pic2D3VConfig(species; NX, NY=NX, Lx=1, Ly=1, dt, B0x=0, B0y=0, B0z=0, buffer=3, ntskip=1, ngskip=1,
-                  field_update=:accumulate, boris=:reference, device=0, field=:electrostatic, theta=1)
+                  field_update=:accumulate, boris=:reference, device=0, field=:electrostatic, theta=1,
+                  theta_source=0, rtol=sqrt(eps()), maxiters=10)
 
 The `ElectrostaticField(NX, NY, Lx, Ly; dt, B0x, B0y, B0z, buffer)` and
 `ElectrostaticDiagnostics(..., ntskip, ngskip)` of a plasma of up to four
@@ -748,11 +755,13 @@ by q_m^2); `:textbook` is `ElectromagneticBoris`'s form with Ez = 0.
 `field=:lorenz_staggered` builds `LorenzGaugeStaggeredField(NX, NY, Lx, Ly; dt,
 B0x, B0y, B0z, imex=ImEx(theta), buffer)` and `LorenzGaugeDiagnostics` instead
 (`theta=0` is `Explicit()`, `1` is `Implicit()`; src/2D3V.jl uses `ImEx(1)`);
-`field=:lorenz` builds a `LorenzGaugeField`, which runs with `loop!` only.
+`field=:lorenz` builds a `LorenzGaugeField`, which runs with `loop!` only;
+`field=:lorenz_semi_implicit` a `LorenzGaugeSemiImplicitField(...; fieldimex=ImEx(theta),
+sourceimex=ImEx(theta_source), buffer, rtol, maxiters)`.
 """
 function Pic2D3VConfig(species::Vector{Pic2D3VSpecies}; NX, NY=NX, Lx=1.0, Ly=1.0, dt, B0x=0.0, B0y=0.0, B0z=0.0,
                        buffer=3, ntskip=1, ngskip=1, field_update=:accumulate, boris=:reference, device=0,
-                       field=:electrostatic, theta=1.0)
+                       field=:electrostatic, theta=1.0, theta_source=0.0, rtol=sqrt(eps()), maxiters=10)
     ns = length(species)
     1 <= ns <= MAX_SPECIES_2D3V || error("Pic2D3VConfig: 1 to $MAX_SPECIES_2D3V species")
     none = Pic2D3VSpecies(0, 0, 0.0, 0.0, 0.0)
@@ -760,7 +769,7 @@ function Pic2D3VConfig(species::Vector{Pic2D3VSpecies}; NX, NY=NX, Lx=1.0, Ly=1.
     Pic2D3VConfig(Int32(NX), Int32(NY), Float64(Lx), Float64(Ly), Float64(dt),
                   (Float64(B0x), Float64(B0y), Float64(B0z)), Int32(buffer), Int32(ntskip), Int32(ngskip),
                   FIELD_UPDATES[field_update], BORIS_VARIANTS[boris], Int32(ns), sp, Int32(device),
-                  FIELDS2D3V[field], Float64(theta))
+                  FIELDS2D3V[field], Float64(theta), Float64(theta_source), Float64(rtol), Int32(maxiters))
 end
 
 mutable struct Context2D3V
@@ -842,7 +851,7 @@ fieldmomentum (3) before characteristicmomentum.
 function run!(ctx::Context2D3V, nsteps::Integer)
     nt = ctx.cfg.ntskip
     rows = cld(ctx.t + nsteps, nt) - cld(ctx.t, nt)
-    R = zeros(ctx.cfg.field == FIELDS2D3V[:lorenz_staggered] ? 11 : 8, rows)
+    R = zeros(ctx.cfg.field in (FIELDS2D3V[:lorenz_staggered], FIELDS2D3V[:lorenz_semi_implicit]) ? 11 : 8, rows)
     rc = GC.@preserve R ccall((:pic2d3v_run, libpicgolf), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}),
                               ctx.ptr, nsteps, rows > 0 ? pointer(R) : C_NULL)
     ctx.t += nsteps
@@ -937,6 +946,32 @@ function lg_field!(ctx::Context2D3V, which::Symbol, z::Matrix{ComplexF64})
                               ctx.ptr, k, z)
     check(ctx, rc, "pic2d3v_lg_set")
     ctx
+end
+
+"""`lg_grid(ctx) -> Array{Float64,3}`: a `LorenzGaugeSemiImplicitField`'s deposit grid `ρJs⁺`
+(4 x (NX+2buffer) x (NY+2buffer), axes 1:4, -(buffer-1):NX+buffer, ... as the reference's)."""
+function lg_grid(ctx::Context2D3V)
+    c = ctx.cfg
+    A = zeros(4, c.NX + 2c.buffer, c.NY + 2c.buffer)
+    rc = GC.@preserve A ccall((:pic2d3v_lg_grid, libpicgolf), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.ptr, A)
+    check(ctx, rc, "pic2d3v_lg_grid")
+    A
+end
+
+"`lg_grid!(ctx, A)`: sets the deposit grid `ρJs⁺` (the next step's first convergence test compares with it)."
+function lg_grid!(ctx::Context2D3V, A::Array{Float64,3})
+    rc = GC.@preserve A ccall((:pic2d3v_lg_set_grid, libpicgolf), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.ptr, A)
+    check(ctx, rc, "pic2d3v_lg_set_grid")
+    ctx
+end
+
+"`lg_iterations(ctx, n) -> Vector{Int64}`: the semi-implicit iteration counts of the last `n` steps."
+function lg_iterations(ctx::Context2D3V, n::Integer)
+    v = zeros(Int64, max(n, 1))
+    rc = GC.@preserve v ccall((:pic2d3v_lg_iterations, libpicgolf), Cint, (Ptr{Cvoid}, Ptr{Int64}, Int64),
+                              ctx.ptr, v, n)
+    check(ctx, rc, "pic2d3v_lg_iterations")
+    v[1:n]
 end
 
 "`lg_bins(ctx, which) -> Array{Float64,3}`: `LorenzGaugeDiagnostics`' binned field `which` (e.g. `:Ezs`)."

@@ -51,7 +51,7 @@ EXPORTS_2D3V = [
     "pic2d3v_create", "pic2d3v_destroy", "pic2d3v_last_error", "pic2d3v_upload", "pic2d3v_download",
     "pic2d3v_reset", "pic2d3v_sort", "pic2d3v_run", "pic2d3v_fields", "pic2d3v_field", "pic2d3v_deposit",
     "pic2d3v_gather", "pic2d3v_boris", "pic2d3v_fieldsolve", "pic2d3v_warmup", "pic2d3v_loop", "pic2d3v_lg_get",
-    "pic2d3v_lg_set", "pic2d3v_lg_bins",
+    "pic2d3v_lg_set", "pic2d3v_lg_bins", "pic2d3v_lg_grid", "pic2d3v_lg_set_grid", "pic2d3v_lg_iterations",
 ]
 # every symbol include/picem1d.h declares
 EXPORTS_EM1D = [
@@ -116,14 +116,16 @@ class pic2d3v_config(ctypes.Structure):
         ("ntskip", ctypes.c_int32), ("ngskip", ctypes.c_int32), ("field_update", ctypes.c_int32),
         ("boris", ctypes.c_int32), ("nspecies", ctypes.c_int32),
         ("species", pic2d3v_species * MAX_SPECIES_2D3V), ("device", ctypes.c_int32), ("field", ctypes.c_int32),
-        ("theta", ctypes.c_double),
+        ("theta", ctypes.c_double), ("theta_source", ctypes.c_double), ("rtol", ctypes.c_double),
+        ("maxiters", ctypes.c_int32),
     ]
 
 
-ELECTROSTATIC_2D3V, LORENZ_STAGGERED_2D3V, LORENZ_2D3V = 0, 1, 2
+ELECTROSTATIC_2D3V, LORENZ_STAGGERED_2D3V, LORENZ_2D3V, LORENZ_SEMI_IMPLICIT_2D3V = 0, 1, 2, 3
 # Lorenz-gauge state slots and binned fields, in include/pic2d3v.h's order
 LG_STATE = ["phi0", "phim", "Ax0", "Ay0", "Az0", "Axm", "Aym", "Azm", "rho0", "Jx0", "Jy0", "Jz0",
             "Ex", "Ey", "Ez", "Bx", "By", "Bz"]
+LG_STATE_SI = LG_STATE + ["rhom", "Jxm", "Jym", "Jzm"]   # LorenzGaugeSemiImplicitField
 LG_BINS = ["Ex", "Ey", "Ez", "Bx", "By", "Bz", "Ax", "Ay", "Az", "phi", "rho", "Jx", "Jy", "Jz"]
 
 
@@ -230,6 +232,9 @@ def load():
         "pic2d3v_lg_get": [vp, i32, vp],
         "pic2d3v_lg_set": [vp, i32, vp],
         "pic2d3v_lg_bins": [vp, i32, vp, i64],
+        "pic2d3v_lg_grid": [vp, vp],
+        "pic2d3v_lg_set_grid": [vp, vp],
+        "pic2d3v_lg_iterations": [vp, vp, i64],
     })
     sig.update({
         "picem1d_create": [ctypes.POINTER(vp), ctypes.POINTER(picem1d_config)],

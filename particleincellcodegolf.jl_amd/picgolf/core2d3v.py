@@ -15,6 +15,10 @@ include/pic2d3v.h.
     LorenzGaugeStaggeredField(NX, NY, Lx, Ly; dt, B0x, B0y, B0z, imex, buffer)
                                            :347-392                 -> Config2D3V(field="lorenz_staggered", theta)
     LorenzGaugeField(...; imex, buffer)    :307-345                 -> Config2D3V(field="lorenz", theta)
+    LorenzGaugeSemiImplicitField(...; fieldimex, sourceimex, buffer, rtol, maxiters)
+                                           :403-461                 -> Config2D3V(field="lorenz_semi_implicit",
+                                                                        theta, theta_source, rtol, maxiters)
+    its rhoJs+ and iteration counts        :1000-1012               -> lg_grid, lg_set_grid, lg_iterations
     warmup!(field, plasma, to)             :826-859                 -> warmup
     loop! alone (no diagnose!)             :875-979                 -> loop
     the field's phi, A, rho, J, E, B       :347-378                 -> lg_get, lg_set
@@ -37,7 +41,7 @@ SHAPES = {"ngp": L.NGP_2D3V, "area": L.AREA_2D3V, **{f"bspline{n}": L.BSPLINE0_2
 FIELD_UPDATES = {"accumulate": L.FIELD_ACCUMULATE, "replace": L.FIELD_REPLACE}
 BORIS = {"reference": L.BORIS_REFERENCE, "textbook": L.BORIS_TEXTBOOK}
 FIELDS = {"electrostatic": L.ELECTROSTATIC_2D3V, "lorenz_staggered": L.LORENZ_STAGGERED_2D3V,
-          "lorenz": L.LORENZ_2D3V}
+          "lorenz": L.LORENZ_2D3V, "lorenz_semi_implicit": L.LORENZ_SEMI_IMPLICIT_2D3V}
 
 
 def _code(table, v):
@@ -63,7 +67,10 @@ class Config2D3V:
     (ElectrostaticBoris) or "textbook" (ElectromagneticBoris with Ez = 0).
     field "lorenz_staggered" builds a LorenzGaugeStaggeredField instead, with
     the ImEx theta (0: Explicit(), 1: Implicit(); src/2D3V.jl uses ImEx(1));
-    "lorenz" a LorenzGaugeField, run with loop() only."""
+    "lorenz" a LorenzGaugeField, run with loop() only; "lorenz_semi_implicit"
+    a LorenzGaugeSemiImplicitField with fieldimex theta, sourceimex
+    theta_source and the convergence test's rtol and maxiters (the
+    reference's defaults sqrt(eps()) and 10)."""
     NX: int
     NY: int
     dt: float
@@ -79,6 +86,9 @@ class Config2D3V:
     device: int = 0
     field: str = "electrostatic"
     theta: float = 1.0
+    theta_source: float = 0.0
+    rtol: float = float(np.sqrt(np.finfo(float).eps))
+    maxiters: int = 10
 
     def to_c(self):
         c = L.pic2d3v_config()
@@ -98,6 +108,9 @@ class Config2D3V:
         c.device = int(self.device)
         c.field = _code(FIELDS, self.field)
         c.theta = float(self.theta)
+        c.theta_source = float(self.theta_source)
+        c.rtol = float(self.rtol)
+        c.maxiters = int(self.maxiters)
         return c
 
 
@@ -112,6 +125,7 @@ class Context2D3V:
             raise L.PicError(rc, f"pic2d3v_create: {self._lib.pic2d3v_last_error(None).decode()}")
         self._h = h
         self.t = 0
+        self._last_steps = 0
 
     def _check(self, rc, what):
         if rc != 0:
@@ -165,9 +179,10 @@ class Context2D3V:
         nt = self.cfg.ntskip
         t0, t1 = self.t, self.t + int(nsteps)
         n = -(-t1 // nt) - (-(-t0 // nt))
-        rows = np.zeros((max(n, 0), 11 if self.cfg.field == "lorenz_staggered" else 8))
+        rows = np.zeros((max(n, 0), 11 if self.cfg.field in ("lorenz_staggered", "lorenz_semi_implicit") else 8))
         rc = self._lib.pic2d3v_run(self._h, int(nsteps), L.ptr(rows))
         self.t = t1
+        self._last_steps = int(nsteps)
         self._check(rc, "pic2d3v_run")
         return rows
 
@@ -188,7 +203,8 @@ class Context2D3V:
 
     # ---- Lorenz gauge --------------------------------------------------------
     def warmup(self):
-        """warmup!(field, plasma) of the staggered Lorenz-gauge field."""
+        """warmup!(field, plasma) of a Lorenz-gauge field (nothing for
+        LorenzGaugeField and the electrostatic field)."""
         self._check(self._lib.pic2d3v_warmup(self._h), "pic2d3v_warmup")
 
     def loop(self, nsteps):
@@ -196,24 +212,55 @@ class Context2D3V:
         rc = self._lib.pic2d3v_loop(self._h, int(nsteps))
         self._check(rc, "pic2d3v_loop")
         self.t += int(nsteps)
+        self._last_steps = int(nsteps)
 
     def _slot(self, which, table):
         return table.index(which) if isinstance(which, str) else int(which)
 
+    def _states(self):
+        return L.LG_STATE_SI if self.cfg.field == "lorenz_semi_implicit" else L.LG_STATE
+
     def lg_get(self, which):
-        """The state slot `which` (a name of LG_STATE or its index) as a
-        complex (NY, NX) array."""
+        """The state slot `which` (a name of LG_STATE, or of LG_STATE_SI for
+        the semi-implicit field, or its index) as a complex (NY, NX) array."""
         z = np.zeros((self.cfg.NY, self.cfg.NX), dtype=np.complex128)
-        self._check(self._lib.pic2d3v_lg_get(self._h, self._slot(which, L.LG_STATE), L.ptr(z)), "pic2d3v_lg_get")
+        self._check(self._lib.pic2d3v_lg_get(self._h, self._slot(which, self._states()), L.ptr(z)),
+                    "pic2d3v_lg_get")
         return z
 
     def lg_state(self):
-        """All 18 slots as a complex (18, NY, NX) array."""
-        return np.stack([self.lg_get(k) for k in range(len(L.LG_STATE))])
+        """All slots (18, or 22 for the semi-implicit field) as a complex
+        (n, NY, NX) array."""
+        return np.stack([self.lg_get(k) for k in range(len(self._states()))])
 
     def lg_set(self, which, z):
         z = np.ascontiguousarray(np.asarray(z, dtype=np.complex128).reshape(self.cfg.NY, self.cfg.NX))
-        self._check(self._lib.pic2d3v_lg_set(self._h, self._slot(which, L.LG_STATE), L.ptr(z)), "pic2d3v_lg_set")
+        self._check(self._lib.pic2d3v_lg_set(self._h, self._slot(which, self._states()), L.ptr(z)),
+                    "pic2d3v_lg_set")
+
+    def _grid_shape(self):
+        b = self.cfg.buffer
+        return (self.cfg.NY + 2 * b, self.cfg.NX + 2 * b, 4)
+
+    def lg_grid(self):
+        """The semi-implicit field's deposit grid rhoJs+: (NY+2b, NX+2b, 4),
+        component fastest as rhoJs[c, i, j]."""
+        out = np.zeros(self._grid_shape())
+        self._check(self._lib.pic2d3v_lg_grid(self._h, L.ptr(out)), "pic2d3v_lg_grid")
+        return out
+
+    def lg_set_grid(self, grid):
+        g = np.ascontiguousarray(np.asarray(grid, dtype=np.float64).reshape(self._grid_shape()))
+        self._check(self._lib.pic2d3v_lg_set_grid(self._h, L.ptr(g)), "pic2d3v_lg_set_grid")
+
+    def lg_iterations(self, n=None):
+        """The semi-implicit iteration counts of the last n steps (all steps
+        of the most recent run/loop call when None)."""
+        if n is None:
+            n = self._last_steps
+        out = np.zeros(max(int(n), 1), dtype=np.int64)
+        self._check(self._lib.pic2d3v_lg_iterations(self._h, L.ptr(out), int(n)), "pic2d3v_lg_iterations")
+        return out[:int(n)]
 
     def lg_bins(self, which=None, nbins=None):
         """The binned field `which` (a name of LG_BINS or its index), shape

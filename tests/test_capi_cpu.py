@@ -223,6 +223,12 @@ def test_2d3v_struct_layout_matches_the_header(tmp_path):
     (dict(species=[dict(shape=9)]), "shape"),
     (dict(species=[dict(mass=0.0)]), "mass"),
     (dict(species=[dict(P=2 ** 25), dict(P=2 ** 25), dict(P=1)]), "2^26"),
+    (dict(field="lorenz_semi_implicit", maxiters=-1), "maxiters"),
+    (dict(field="lorenz_semi_implicit", rtol=-1e-8), "rtol"),
+    (dict(field="lorenz_semi_implicit", theta_source=float("inf")), "theta_source"),
+    # the buffered deposit grid must hold every stencil cell (the reference's BoundsError)
+    (dict(field="lorenz_semi_implicit", buffer=1, species=[dict(shape=4)]), "buffer >= 2"),
+    (dict(field="lorenz_semi_implicit", buffer=2, species=[dict(shape=1), dict(shape=7)]), "buffer >= 3"),
 ])
 def test_create2d3v_rejects_bad_config_with_message(change, msg):
     import picgolf
