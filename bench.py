@@ -36,7 +36,8 @@ def parse():
                     help="skip the ensemble and 2D3V workloads reported beside the headline")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--workload", default="scaled",
-                    choices=["scaled", "ensemble", "es2d3v", "pic2d3v", "lorenz2d3v", "em1d", "em1d_explicit"],
+                    choices=["scaled", "ensemble", "es2d3v", "pic2d3v", "lorenz2d3v", "lorenz2d3v_si", "em1d",
+                             "em1d_explicit"],
                     help="scaled: the headline line (default); ensemble / es2d3v: only that workload, "
                          "--steps timed steps, one JSON line (for profiling)")
     ap.add_argument("--script-steps", type=int, default=4048,
@@ -324,13 +325,17 @@ def driver_2d3v_parameters(N=1024):
     return {"dt": L / N / c / T / 8, "B0": B / (A_s / L), "n0": n / n_s, "vth": math.sqrt(TeeV * e * 2 / me) / L * T}
 
 
-def bench_lorenz2d3v(steps=10, warmup=2, N=1024, ppc=16, shape="bspline2"):
+def bench_lorenz2d3v(steps=10, warmup=2, N=1024, ppc=16, shape="bspline2", field="lorenz_staggered"):
     """The structured 2D3V module's driver whole (src/2D3V.jl:50-127):
     LorenzGaugeStaggeredField(1024, 1024; ImEx(1), buffer = 10, B0 along x),
     electrons and ions of mass 32 with 16 particles per cell each, quadratic
     B-splines, sorted per cell once, warmup!, then loop! and diagnose! every
     step (ntskip = ngskip = 1).  FP64.  Particles are uniform random with
-    Gaussian velocities of the driver's vth (not its Halton layout)."""
+    Gaussian velocities of the driver's vth (not its Halton layout).
+    field "lorenz_semi_implicit": the driver's commented-out alternative
+    (src/2D3V.jl:104-105), LorenzGaugeSemiImplicitField with fieldimex
+    ImEx(1.0), sourceimex ImEx(0.05), buffer 10, rtol sqrt(eps()),
+    maxiters 1000; a push counts once per step whatever the iterations."""
     import numpy as np
     import picgolf
     from picgolf.core2d3v import Config2D3V, SpeciesConfig
@@ -342,8 +347,9 @@ def bench_lorenz2d3v(steps=10, warmup=2, N=1024, ppc=16, shape="bspline2"):
         vth = q["vth"] / np.sqrt(m)
         sps.append(SpeciesConfig(P=P, shape=shape, charge=ch, mass=m, weight=q["n0"] / P))
         xyvs.append(np.vstack([rng.random(P), rng.random(P), vth / np.sqrt(2) * rng.standard_normal((3, P))]))
+    semi = field == "lorenz_semi_implicit"
     cfg = Config2D3V(NX=N, NY=N, dt=q["dt"], species=sps, B0=(q["B0"], 0.0, 0.0), buffer=10, ntskip=1, ngskip=1,
-                     field="lorenz_staggered", theta=1.0)
+                     field=field, theta=1.0, theta_source=0.05, maxiters=1000)
     with picgolf.Context2D3V(cfg) as c:
         for k, xyv in enumerate(xyvs):
             c.upload(k, xyv)
@@ -353,6 +359,7 @@ def bench_lorenz2d3v(steps=10, warmup=2, N=1024, ppc=16, shape="bspline2"):
         t0 = time.perf_counter()
         rows = c.run(steps)
         secs = time.perf_counter() - t0
+        iters = c.lg_iterations() if semi else None
     # the CPU restatement on a sample: the same grid and transforms, 1/32 of
     # each species (sorted), warmup! and one step
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -364,20 +371,34 @@ def bench_lorenz2d3v(steps=10, warmup=2, N=1024, ppc=16, shape="bspline2"):
                          rng.standard_normal((3, n))])
         xyv, _ = oracle.sort23(xyv, 1.0 / N, 1.0 / N)
         sample.append(dict(shape=shape, charge=sp.charge, mass=sp.mass, weight=q["n0"] / n, xyv=xyv))
-    sample, F = oracle.lg_warmup(N, N, 1.0, 1.0, cfg.dt, sample)
-    t0 = time.perf_counter()
-    oracle.lg_run(N, N, 1.0, 1.0, cfg.dt, np.array(cfg.B0), sample, F, 1, buffer=10)
-    cpu_secs = time.perf_counter() - t0
-    return {"workload": f"structured 2D3V Lorenz-gauge driver (src/2D3V.jl): {N}x{N} cells, "
-                        f"LorenzGaugeStaggeredField ImEx(1), buffer 10, 2 species x {P} particles ({ppc} per cell "
-                        f"each), {shape}, ions of mass 32, B0 along x, dt=1/(8N), diagnose! every step, FP64, "
-                        f"particles sorted per cell once",
-            "pushes_per_s": 2 * P * steps / secs, "ms_per_step": secs / steps * 1e3, "steps": steps,
-            "warmup": warmup, "kinetic_energy_first_last": [float(rows[0, 0]), float(rows[-1, 0])],
-            "field_energy_first_last": [float(rows[0, 1]), float(rows[-1, 1])],
-            "cpu_baseline": {"value": 2 * n / cpu_secs, "unit": "particle-pushes/s", "cores": 1, "kind": "port",
-                             "sample": f"FP64 CPU restatement, the same {N}x{N} grid and transforms, 2 species x "
-                             f"{n} particles (1/32), one step with diagnose!, one thread"}}
+    if semi:
+        sample, F, grid = oracle.lg_si_warmup(N, N, 1.0, 1.0, cfg.dt, sample, 10)
+        t0 = time.perf_counter()
+        o = oracle.lg_si_run(N, N, 1.0, 1.0, cfg.dt, np.array(cfg.B0), sample, F, grid, np.zeros((6, N, N)), 1,
+                             theta=1.0, theta_source=0.05, maxiters=1000, buffer=10)
+        cpu_secs = time.perf_counter() - t0
+        cpu_iters = int(o[6][0])
+        fname = "LorenzGaugeSemiImplicitField ImEx(1.0), sourceimex ImEx(0.05), rtol sqrt(eps), maxiters 1000"
+    else:
+        sample, F = oracle.lg_warmup(N, N, 1.0, 1.0, cfg.dt, sample)
+        t0 = time.perf_counter()
+        oracle.lg_run(N, N, 1.0, 1.0, cfg.dt, np.array(cfg.B0), sample, F, 1, buffer=10)
+        cpu_secs = time.perf_counter() - t0
+        fname = "LorenzGaugeStaggeredField ImEx(1)"
+    out = {"workload": f"structured 2D3V Lorenz-gauge driver (src/2D3V.jl): {N}x{N} cells, "
+                       f"{fname}, buffer 10, 2 species x {P} particles ({ppc} per cell "
+                       f"each), {shape}, ions of mass 32, B0 along x, dt=1/(8N), diagnose! every step, FP64, "
+                       f"particles sorted per cell once",
+           "pushes_per_s": 2 * P * steps / secs, "ms_per_step": secs / steps * 1e3, "steps": steps,
+           "warmup": warmup, "kinetic_energy_first_last": [float(rows[0, 0]), float(rows[-1, 0])],
+           "field_energy_first_last": [float(rows[0, 1]), float(rows[-1, 1])],
+           "cpu_baseline": {"value": 2 * n / cpu_secs, "unit": "particle-pushes/s", "cores": 1, "kind": "port",
+                            "sample": f"FP64 CPU restatement, the same {N}x{N} grid and transforms, 2 species x "
+                            f"{n} particles (1/32), one step with diagnose!, one thread"}}
+    if semi:
+        out["iterations_per_step"] = [int(k) for k in iters]
+        out["cpu_baseline"]["sample"] += f" ({cpu_iters} iterations)"
+    return out
 
 
 def em1d_setup(N, P, scheme, seed=0):
@@ -489,8 +510,9 @@ def main():
     if a.workload == "pic2d3v":
         print(json.dumps(bench_pic2d3v(steps=a.steps, warmup=a.warmup)))
         return
-    if a.workload == "lorenz2d3v":
-        print(json.dumps(bench_lorenz2d3v(steps=a.steps, warmup=a.warmup)))
+    if a.workload in ("lorenz2d3v", "lorenz2d3v_si"):
+        f = "lorenz_semi_implicit" if a.workload == "lorenz2d3v_si" else "lorenz_staggered"
+        print(json.dumps(bench_lorenz2d3v(steps=a.steps, warmup=a.warmup, field=f)))
         return
     if a.workload in ("em1d", "em1d_explicit"):
         scheme = "explicit" if a.workload == "em1d_explicit" else "fixed_point"
@@ -553,6 +575,7 @@ def main():
     if not a.no_extra:
         out["workloads"] = {"ensemble_512": bench_ensemble(autotune=not a.no_autotune), "es2d3v_256": bench_2d(),
                             "pic2d3v_1024": bench_pic2d3v(), "lorenz2d3v_1024": bench_lorenz2d3v(),
+                            "lorenz2d3v_si_1024": bench_lorenz2d3v(field="lorenz_semi_implicit"),
                             "em1d": bench_em1d(),
                             "em1d_explicit": bench_em1d(scheme="explicit")}
         print(f"[bench] extra workloads: {json.dumps(out['workloads'])}", file=sys.stderr, flush=True)

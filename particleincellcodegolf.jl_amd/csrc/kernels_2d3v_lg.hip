@@ -484,8 +484,11 @@ k_lg_push(LG23 L, Grid23 g, Species23 s, Species23 o, int sidx, double* __restri
 }
 
 // NG = 1: the charge (weight qw) at (X, Y); NG = 3: the currents (weights
-// vx*qw, vy*qw, vz*qw) at (X, Y) = the species' positions.
-template <int SH, int NG>
+// vx*qw, vy*qw, vz*qw) at (X, Y) = the species' positions.  GHOST (the
+// semi-implicit field's buffered grid): a stencil cell outside the NX x NY
+// interior also adds its limbs to that ghost cell of eacc, and a stencil
+// cell outside the buffered grid drops the particle and sets eflag.
+template <int SH, int NG, bool GHOST>
 __global__ void __launch_bounds__(kTile23Threads)
 k_lg_deposit(LG23 L, Grid23 g, Species23 s, const double* __restrict__ X, const double* __restrict__ Y) {
     constexpr int n = ShapeCount<SH>::value;
@@ -540,8 +543,27 @@ k_lg_deposit(LG23 L, Grid23 g, Species23 s, const double* __restrict__ X, const 
         int32_t cx[n], cy[n];
         double wxs[n], wys[n];
         int lx[n], ly[n];
-        shape_w<SH>(xs[k], g.NX_Lx, NX, cx, wxs);
-        shape_w<SH>(ys[k], g.NY_Ly, NY, cy, wys);
+        int ux[n], uy[n];  // GHOST: the buffered rows (unwrapped cell + b)
+        if (GHOST) {
+            shape_w<SH>(xs[k], g.NX_Lx, 0, cx, wxs);
+            shape_w<SH>(ys[k], g.NY_Ly, 0, cy, wys);
+            bool ok = true;
+#pragma unroll
+            for (int qq = 0; qq < n; ++qq) {
+                ux[qq] = cx[qq] + L.b;
+                uy[qq] = cy[qq] + L.b;
+                ok = ok && (unsigned)ux[qq] < (unsigned)L.NXe && (unsigned)uy[qq] < (unsigned)L.NYe;
+                cx[qq] &= NX - 1;
+                cy[qq] &= NY - 1;
+            }
+            if (!ok) {
+                atomicOr(L.eflag, 1);
+                continue;
+            }
+        } else {
+            shape_w<SH>(xs[k], g.NX_Lx, NX, cx, wxs);
+            shape_w<SH>(ys[k], g.NY_Ly, NY, cy, wys);
+        }
         bool in = tile;
 #pragma unroll
         for (int qq = 0; qq < n; ++qq) {
@@ -554,12 +576,23 @@ k_lg_deposit(LG23 L, Grid23 g, Species23 s, const double* __restrict__ X, const 
 #pragma unroll
             for (int ii = 0; ii < n; ++ii) {
                 const double wxy = wxs[ii] * wys[jj];
+                // GHOST: the cell lies outside the interior of the buffered grid
+                const bool ghost = GHOST && ((unsigned)(ux[ii] - L.b) >= (unsigned)NX ||
+                                             (unsigned)(uy[jj] - L.b) >= (unsigned)NY);
 #pragma unroll
                 for (int c = 0; c < NG; ++c) {
                     const double term = wxy * w[c];
                     if (term == 0.0) continue;
                     const u128 u = (u128)fix_round128(term, q[c]);
                     const u64 l[3] = {(u64)u & M, (u64)(u >> kLimbS) & M, (u64)(u >> (2 * kLimbS)) & M};
+                    if (ghost) {
+                        const int64_t Ge = (int64_t)L.NXe * L.NYe;
+                        u64* r = L.eacc + (size_t)(g0 + c) * 3 * Ge;
+                        const int64_t kk = ux[ii] + (int64_t)L.NXe * uy[jj];
+#pragma unroll
+                        for (int b = 0; b < 3; ++b)
+                            if (l[b]) atomicAdd(r + b * Ge + kk, l[b]);
+                    }
                     if (in) {
                         const int e = lx[ii] + wx * ly[jj];
 #pragma unroll
@@ -691,13 +724,13 @@ template <int SH>
 struct DepRhoL {
     static void go(const LG23& L, const Grid23& g, const Species23& s, const double* xh, const double* yh,
                    hipStream_t st) {
-        k_lg_deposit<SH, 1><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s, xh, yh);
+        k_lg_deposit<SH, 1, false><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s, xh, yh);
     }
 };
 template <int SH>
 struct DepJL {
     static void go(const LG23& L, const Grid23& g, const Species23& s, hipStream_t st) {
-        k_lg_deposit<SH, 3><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s, s.x, s.y);
+        k_lg_deposit<SH, 3, false><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s, s.x, s.y);
     }
 };
 
@@ -889,177 +922,33 @@ void lg_launch_diag_fields(const LG23& L, const Diag23& d, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // The deposit grid is the reference's buffered one: Julia cell c (1-based,
 // unwrapped, axes -(b-1):NX+b) is row c - 1 + b of the (NX+2b)-wide grid.
-// The exact limbs are summed per buffered cell; the periodic reduction adds
-// a cell's images' counters (exact) and rounds once.
+// The deposits (k_lg_deposit<.., GHOST>) add every contribution to the
+// periodic limbs racc, as the other fields' deposits do, and a contribution
+// to a ghost cell also to that cell of eacc.  The periodic reduction is then
+// racc itself; an interior buffered cell's counters are its periodic cell's
+// minus those of the cell's ghost images (exact: the counters are sums
+// modulo 2^64), a ghost cell's are eacc's.
 
-// the buffered row of the base cell ceil(z*NZ_Lz) (clamped far outside)
-__device__ __forceinline__ int si_base(double z, double NZ_Lz, int b) {
-    double ci = ceil(z * NZ_Lz);
-    ci = ci == ci ? (ci > 1.0e8 ? 1.0e8 : (ci < -1.0e8 ? -1.0e8 : ci)) : 0.0;
-    return (int)ci - 1 + b;
-}
-
-// The window of a chunk in the buffered grid: the block-reduced extent of the
-// particles' base cells widened by the stencil and a halo, clipped to the
-// grid; win = (ox, oy, wx, wy, fits in CELLS).
-template <int SH, int CELLS>
-__device__ __forceinline__ void si_window(const LG23& L, const Grid23& g, const double* xs, const double* ys, int nv,
-                                          int ax, int ay, int (*red)[4], int* win) {
-    constexpr int LO = ShapeExtent<SH>::lo, HI = ShapeExtent<SH>::hi;
-    const int t = threadIdx.x;
-    int mnx = INT_MAX / 2, mxx = INT_MIN / 2, mny = INT_MAX / 2, mxy = INT_MIN / 2;
-#pragma unroll
-    for (int k = 0; k < kTile23PPT; ++k) {
-        if (k >= nv) break;
-        const int dx = si_base(xs[k], g.NX_Lx, L.b) - ax, dy = si_base(ys[k], g.NY_Ly, L.b) - ay;
-        mnx = min(mnx, dx);
-        mxx = max(mxx, dx);
-        mny = min(mny, dy);
-        mxy = max(mxy, dy);
-    }
-    mnx = wave_min_i(mnx);
-    mxx = wave_max_i(mxx);
-    mny = wave_min_i(mny);
-    mxy = wave_max_i(mxy);
-    const int wv = t >> 6;
-    if ((t & 63) == 0) {
-        red[wv][0] = mnx;
-        red[wv][1] = mxx;
-        red[wv][2] = mny;
-        red[wv][3] = mxy;
-    }
-    __syncthreads();
-    if (t == 0) {
-        for (int w = 1; w < kTile23Threads / 64; ++w) {
-            mnx = min(mnx, red[w][0]);
-            mxx = max(mxx, red[w][1]);
-            mny = min(mny, red[w][2]);
-            mxy = max(mxy, red[w][3]);
+// The value of buffered cell e of component comp (see above)
+__device__ __forceinline__ double si_cell(const LG23& L, int64_t Ge, int comp, int64_t e, const int* q) {
+    const int NX = L.NX, NY = L.NY, NXe = L.NXe, NYe = L.NYe, b = L.b;
+    const int v = (int)(e / NXe), u = (int)(e - (int64_t)v * NXe);
+    const u64* ge = L.eacc + (size_t)comp * 3 * Ge;
+    if ((unsigned)(u - b) >= (unsigned)NX || (unsigned)(v - b) >= (unsigned)NY)
+        return lg_limbs(ge, Ge, e, q[comp]);
+    const int64_t G = (int64_t)NX * NY;
+    const int64_t k = (u - b) + (int64_t)NX * (v - b);
+    const u64* r = L.racc + (size_t)comp * 3 * G;
+    u64 a0 = r[k], a1 = r[G + k], a2 = r[2 * G + k];
+    // the ghost images (u + m NX, v + n NY) of the cell; the cell itself holds 0
+    for (int vv = v & (NY - 1); vv < NYe; vv += NY)
+        for (int uu = u & (NX - 1); uu < NXe; uu += NX) {
+            const int64_t f = uu + (int64_t)NXe * vv;
+            a0 -= ge[f];
+            a1 -= ge[Ge + f];
+            a2 -= ge[2 * Ge + f];
         }
-        int64_t sx = (int64_t)ax + mnx - LO - kTile23Halo, ex = (int64_t)ax + mxx + HI + kTile23Halo + 1;
-        int64_t sy = (int64_t)ay + mny - LO - kTile23Halo, ey = (int64_t)ay + mxy + HI + kTile23Halo + 1;
-        sx = sx < 0 ? 0 : sx;
-        sy = sy < 0 ? 0 : sy;
-        ex = ex > L.NXe ? L.NXe : ex;
-        ey = ey > L.NYe ? L.NYe : ey;
-        const int64_t wx = ex - sx, wy = ey - sy;
-        win[0] = (int)(wx > 0 ? sx : 0);
-        win[1] = (int)(wy > 0 ? sy : 0);
-        win[2] = (int)(wx > 0 ? wx : 0);
-        win[3] = (int)(wy > 0 ? wy : 0);
-        win[4] = wx > 0 && wy > 0 && wx * wy <= CELLS;
-    }
-    __syncthreads();
-}
-
-// The exact deposits of a species at its positions into the buffered limbs.
-// NG = 4: qw, vx*qw, vy*qw, vz*qw (warmup!, :759-760); NG = 3: the currents
-// (the loop, :1042-1043).  A stencil cell outside the buffered grid (the
-// reference's BoundsError) drops the particle and sets eflag.
-template <int SH, int NG>
-__global__ void __launch_bounds__(kTile23Threads) k_lg_si_deposit(LG23 L, Grid23 g, Species23 s) {
-    constexpr int n = ShapeCount<SH>::value;
-    constexpr int c0 = 4 - NG;                      // first component: 0 rho, 1 Jx
-    constexpr int CELLS = NG == 4 ? 512 : kLGDepCells;
-    __shared__ u64 Rw[NG * 3 * CELLS];
-    __shared__ int red[4][4];
-    __shared__ int win[5];
-    const int t = threadIdx.x;
-    const int NXe = L.NXe, NYe = L.NYe, b = L.b;
-    const int64_t Ge = (int64_t)NXe * NYe;
-    const int64_t c0p = (int64_t)blockIdx.x * kTile23Chunk;
-    // lanes of a wave in different cells, as in k_lg_deposit
-    const int64_t pb = c0p + (int64_t)(t & 63) * (kTile23Threads / 64 * kTile23PPT) + (t >> 6) * kTile23PPT;
-    double xs[kTile23PPT], ys[kTile23PPT];
-    int nv = 0;
-#pragma unroll
-    for (int k = 0; k < kTile23PPT; ++k) {
-        const int64_t p = pb + k;
-        xs[k] = 0.0;
-        ys[k] = 0.0;
-        if (p < s.P) {
-            xs[k] = s.x[p];
-            ys[k] = s.y[p];
-            nv = k + 1;
-        }
-    }
-    const int ax = si_base(s.x[c0p], g.NX_Lx, b), ay = si_base(s.y[c0p], g.NY_Ly, b);
-    si_window<SH, CELLS>(L, g, xs, ys, nv, ax, ay, red, win);
-    const int ox = win[0], oy = win[1], wx = win[2], wy = win[3];
-    const bool tile = win[4] != 0;
-    if (tile)
-        for (int e = t; e < NG * 3 * CELLS; e += kTile23Threads) Rw[e] = 0;
-    __syncthreads();
-    int q[NG];
-#pragma unroll
-    for (int c = 0; c < NG; ++c) q[c] = c0 + c == 0 ? L.qr : lg_frac_bits(L, c0 + c - 1);
-    constexpr u64 M = (1ull << kLimbS) - 1;
-#pragma unroll 1
-    for (int k = 0; k < kTile23PPT; ++k) {
-        const int64_t p = pb + k;
-        if (p >= s.P) break;
-        double w[NG];
-        {
-            const double v[4] = {1.0, s.vx[p], s.vy[p], s.vz[p]};
-#pragma unroll
-            for (int c = 0; c < NG; ++c) w[c] = c0 + c == 0 ? s.qw : v[c0 + c] * s.qw;
-        }
-        int32_t cx[n], cy[n];
-        double wxs[n], wys[n];
-        shape_w<SH>(xs[k], g.NX_Lx, 0, cx, wxs);
-        shape_w<SH>(ys[k], g.NY_Ly, 0, cy, wys);
-        bool ok = true, in = tile;
-        int ux[n], uy[n];
-#pragma unroll
-        for (int qq = 0; qq < n; ++qq) {
-            ux[qq] = cx[qq] + b;
-            uy[qq] = cy[qq] + b;
-            ok = ok && (unsigned)ux[qq] < (unsigned)NXe && (unsigned)uy[qq] < (unsigned)NYe;
-            in = in && (unsigned)(ux[qq] - ox) < (unsigned)wx && (unsigned)(uy[qq] - oy) < (unsigned)wy;
-        }
-        if (!ok) {
-            atomicOr(L.eflag, 1);
-            continue;
-        }
-#pragma unroll
-        for (int jj = 0; jj < n; ++jj)
-#pragma unroll
-            for (int ii = 0; ii < n; ++ii) {
-                const double wxy = wxs[ii] * wys[jj];
-#pragma unroll
-                for (int c = 0; c < NG; ++c) {
-                    const double term = wxy * w[c];
-                    if (term == 0.0) continue;
-                    const u128 u = (u128)fix_round128(term, q[c]);
-                    const u64 l[3] = {(u64)u & M, (u64)(u >> kLimbS) & M, (u64)(u >> (2 * kLimbS)) & M};
-                    if (in) {
-                        const int e = (ux[ii] - ox) + wx * (uy[jj] - oy);
-#pragma unroll
-                        for (int bb = 0; bb < 3; ++bb)
-                            if (l[bb]) atomicAdd(Rw + (c * 3 + bb) * CELLS + e, l[bb]);
-                    } else {
-                        u64* r = L.eacc + (size_t)(c0 + c) * 3 * Ge;
-                        const int64_t kk = ux[ii] + (int64_t)NXe * uy[jj];
-#pragma unroll
-                        for (int bb = 0; bb < 3; ++bb)
-                            if (l[bb]) atomicAdd(r + bb * Ge + kk, l[bb]);
-                    }
-                }
-            }
-    }
-    __syncthreads();
-    if (tile)
-        for (int e = t; e < wx * wy; e += kTile23Threads) {
-            const int ry = e / wx, rx = e - ry * wx;
-            const int64_t k = (ox + rx) + (int64_t)NXe * (oy + ry);
-#pragma unroll
-            for (int c = 0; c < NG; ++c)
-#pragma unroll
-                for (int bb = 0; bb < 3; ++bb) {
-                    const u64 v = Rw[(c * 3 + bb) * CELLS + e];
-                    if (v) atomicAdd(L.eacc + (size_t)(c0 + c) * 3 * Ge + bb * Ge + k, v);
-                }
-        }
+    return lg_limbs_v(a0, a1, a2, q[comp]);
 }
 
 // The grid rhoJs+ from the limbs into gridP (element i = component i % 4 of
@@ -1079,8 +968,7 @@ __global__ void __launch_bounds__(256) k_lg_si_norm(LG23 L, int64_t nch) {
     if (threadIdx.x < kTreeLanes)
         for (int64_t i = threadIdx.x; i < m; i += kTreeLanes) {
             const int64_t idx = s0 + i;
-            const int comp = (int)(idx & 3);
-            const double p = lg_limbs(L.eacc + (size_t)comp * 3 * Ge, Ge, idx >> 2, q[comp]);
+            const double p = si_cell(L, Ge, (int)(idx & 3), idx >> 2, q);
             L.gridP[idx] = p;
             if (TEST) {
                 const double tv = L.gridT[idx], d = tv - p;
@@ -1176,8 +1064,8 @@ __global__ void __launch_bounds__(256) k_lg_si_test(LG23 L, int64_t nch) {
 }
 
 // reduction! (:504-522): each periodic cell's exact sum over its images in
-// the buffered grid, rounded once, into out (imaginary parts 0); the limbs
-// of the images are cleared.
+// the buffered grid (the periodic limbs), rounded once, into out (imaginary
+// parts 0); the limbs, periodic and ghost, are cleared.
 __global__ void __launch_bounds__(256) k_lg_si_wrap(LG23 L, LGOut out) {
     const int NX = L.NX, NY = L.NY, NXe = L.NXe, NYe = L.NYe, b = L.b;
     const int64_t G = (int64_t)NX * NY, Ge = (int64_t)NXe * NYe;
@@ -1189,19 +1077,19 @@ __global__ void __launch_bounds__(256) k_lg_si_wrap(LG23 L, LGOut out) {
         const int u0 = (i + b) & (NX - 1), v0 = (j + b) & (NY - 1);
 #pragma unroll
         for (int comp = 0; comp < 4; ++comp) {
-            u64* r = L.eacc + (size_t)comp * 3 * Ge;
-            u64 a0 = 0, a1 = 0, a2 = 0;
+            u64* r = L.racc + (size_t)comp * 3 * G;
+            out.a[comp][k] = make_double2(lg_limbs(r, G, k, q[comp]), 0.0);
+            r[k] = 0;
+            r[G + k] = 0;
+            r[2 * G + k] = 0;
+            u64* ge = L.eacc + (size_t)comp * 3 * Ge;
             for (int v = v0; v < NYe; v += NY)
                 for (int u = u0; u < NXe; u += NX) {
                     const int64_t e = u + (int64_t)NXe * v;
-                    a0 += r[e];
-                    a1 += r[Ge + e];
-                    a2 += r[2 * Ge + e];
-                    r[e] = 0;
-                    r[Ge + e] = 0;
-                    r[2 * Ge + e] = 0;
+                    ge[e] = 0;
+                    ge[Ge + e] = 0;
+                    ge[2 * Ge + e] = 0;
                 }
-            out.a[comp][k] = make_double2(lg_limbs_v(a0, a1, a2, q[comp]), 0.0);
         }
     }
 }
@@ -1283,8 +1171,8 @@ struct PushSIL {
 template <int SH>
 struct DepSIL {
     static void go(const LG23& L, const Grid23& g, const Species23& s, bool charge, hipStream_t st) {
-        if (charge) k_lg_si_deposit<SH, 4><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s);
-        else k_lg_si_deposit<SH, 3><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s);
+        if (charge) k_lg_deposit<SH, 1, true><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s, s.x, s.y);
+        k_lg_deposit<SH, 3, true><<<chunks23(s.P), kTile23Threads, 0, st>>>(L, g, s, s.x, s.y);
     }
 };
 

@@ -46,7 +46,7 @@ struct LG23 {
     double rtol;         // the convergence test's rtol (atol = 0)
     int32_t b, NXe, NYe;
     double2* src;        // 4 x G: rho+, Jx+, Jy+, Jz+
-    unsigned long long* eacc;  // 4 components x 3 limbs x Ge: the buffered exact sums
+    unsigned long long* eacc;  // 4 components x 3 limbs x Ge: the ghost cells' exact sums (interior 0)
     double* gridT;       // 4 x Ge, component fastest: the grid before the iteration
     double* gridP;       // 4 x Ge: the iteration's grid
     double* npart;       // 6 x chunk partials of the three norms (sums, then maxima)
@@ -85,12 +85,14 @@ void lg_launch_diag_fields(const LG23& L, const Diag23& d, hipStream_t st);
 // (xw = unimod(x + vx*dt) with the old v, vw = Boris), per-species maxima
 void lg_launch_si_push(const LG23& L, const Grid23& g, const Species23& s, const Species23& o, int sidx,
                        hipStream_t st);
-// the exact deposits into the buffered grid at s's positions: the charge and
-// currents (charge) or the currents alone
+// the exact deposits at s's positions into the periodic limbs racc and the
+// ghost cells of the buffered grid (eacc): the charge and currents (charge)
+// or the currents alone
 void lg_launch_si_deposit(const LG23& L, const Grid23& g, const Species23& s, bool charge, hipStream_t st);
-// the buffered grid into gridP (test: with isapprox(gridT, gridP) into res)
-// and its periodic reduction into out[0..3] (imaginary parts 0); the limbs
-// are cleared
+// the buffered grid into gridP (an interior cell's limbs are its periodic
+// cell's minus its ghost images'; test: with isapprox(gridT, gridP) into
+// res) and the periodic reduction into out[0..3] (imaginary parts 0); the
+// limbs are cleared
 struct LGOut {
     double2* a[4];
 };
