@@ -7,7 +7,9 @@
  * (:1258-1284), update! (:468-479), reduction! (:504-522) and
  * diagnose!(::LorenzGaugeDiagnostics) (:1332-1398), whose in-place inverse
  * transforms of the potentials and sources are part of the dynamics; and
- * LorenzGaugeField's centred loop! (:307-345, :665-745) at the end of the file.
+ * LorenzGaugeField's centred loop! (:307-345, :665-745) and
+ * LorenzGaugeSemiImplicitField (:403-461, :864-871, :993-1100) at the end of
+ * the file.
  *
  * TEST INFRASTRUCTURE ONLY (see picoracle.h): the oracle the GPU Lorenz-gauge
  * path is checked against.
@@ -62,6 +64,18 @@
  *                      single mode; uncharged particles stream freely in the
  *                      first step (EBxyz is still zero) and turn about B0 in
  *                      the second.
+ * (tests/test_oracle_lorenz2d_si.py, LorenzGaugeSemiImplicitField):
+ *   orclg_si_deposit   exact rational sums of the contributions on the 2^-q
+ *                      grid per buffered cell and per periodic cell, every
+ *                      shape, ghost cells included; the BoundsError.
+ *   orclg_si_warmup    its three passes rebuilt from the pinned deposits.
+ *   orclg_si_solve     single-mode closed forms of the theta-weighted E and B
+ *                      and of the three source weights.
+ *   orclg_si_run       a numpy restatement (floating-point deposits, numpy's
+ *                      FFT, isapprox through numpy norms) with equal
+ *                      iteration counts; the stopping rule; free streaming
+ *                      before the first update!; diagnose!'s transforms of
+ *                      rho0, J0; split calls.
  */
 #include "picoracle.h"
 
