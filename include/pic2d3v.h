@@ -16,9 +16,11 @@
  *                                 update!                      :530-581, :294-297
  *   diagnose!(ElectrostaticDiagnostics, ...)                   :1301-1330
  * and, for a LorenzGaugeStaggeredField (field = PIC2D3V_LORENZ_STAGGERED, as
- * src/2D3V.jl:102-106 builds it) or a LorenzGaugeField (PIC2D3V_LORENZ, the
+ * src/2D3V.jl:102-106 builds it), a LorenzGaugeField (PIC2D3V_LORENZ, the
  * centred loop! of :665-745, run with pic2d3v_loop only: the reference's
- * diagnose! reads field.rho0, which that type lacks):
+ * diagnose! reads field.rho0, which that type lacks) or a
+ * LorenzGaugeSemiImplicitField (PIC2D3V_LORENZ_SEMI_IMPLICIT, :403-461, its
+ * iterated loop! described at pic2d3v_lg_grid below):
  *   warmup!(field, plasma)        advect -dt/2, +dt/2, the currents   :826-859
  *   lorenzgauge!(imex, ...)       per-mode wave update, any ImEx theta :610-635
  *   E = -grad phi+ - dA/dt, B = curl (A+ + A0)/2, ifft, update!     :917-935
