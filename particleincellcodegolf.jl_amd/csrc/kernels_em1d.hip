@@ -594,6 +594,88 @@ __global__ void __launch_bounds__(kEMThreads) k_em_sort_back(EM1D s) {
     block_max_into(m3, s.umax + 4, red);
 }
 
+// sortperm(x) (:41) of at most kEMSmallSort particles in one block: a bitonic
+// sort of the (key, index) pairs in LDS, a total order that equals the stable
+// sort, then the particles permuted in place.  It replaces the nine launches
+// of the bucket sort at the scripts' sizes.  WRAP: x = mod(x, 1) first; MAX:
+// the moment maxima of the sorted velocities.
+template <bool WRAP, bool MAX>
+__global__ void __launch_bounds__(1024) k_em_sort_small(EM1D s) {
+    __shared__ u64 key[kEMSmallSort];
+    __shared__ uint32_t idx[kEMSmallSort];
+    __shared__ u64 red[1024 / 64];
+    constexpr int R = kEMSmallSort / 1024;
+    const int t = threadIdx.x;
+    const int P = (int)s.P;
+    int n = 1;
+    while (n < P) n <<= 1;
+    for (int i = t; i < n; i += 1024) {
+        if (i < P) {
+            double x = s.x[i];
+            if (WRAP) {
+                x = jl_mod1f(x);
+                s.x[i] = x;
+            }
+            key[i] = em_order_key(x);
+            idx[i] = (uint32_t)i;
+        } else {
+            key[i] = ~0ull;  // after every particle, NaN keys included
+            idx[i] = 0xFFFFFFFFu;
+        }
+    }
+    __syncthreads();
+    for (int k = 2; k <= n; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = t; i < n; i += 1024) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const u64 ka = key[i], kb = key[l];
+                    const uint32_t ia = idx[i], ib = idx[l];
+                    const bool gt = ka > kb || (ka == kb && ia > ib);
+                    if (gt == ((i & k) == 0)) {
+                        key[i] = kb;
+                        key[l] = ka;
+                        idx[i] = ib;
+                        idx[l] = ia;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    double xv[R], a[R], b[R], c[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int k = t + r * 1024;
+        if (k < P) {
+            const uint32_t p = idx[k];
+            s.perm[k] = p;
+            xv[r] = s.x[p];
+            a[r] = s.v1[p];
+            b[r] = s.v2[p];
+            c[r] = s.v3[p];
+        }
+    }
+    __syncthreads();  // every read of the old order before the first write
+    double m1 = 0.0, m2 = 0.0, m3 = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int k = t + r * 1024;
+        if (k < P) {
+            s.x[k] = xv[r];
+            s.v1[k] = a[r];
+            s.v2[k] = b[r];
+            s.v3[k] = c[r];
+            m1 = fmax_num(m1, fabs(a[r]));
+            m2 = fmax_num(m2, fabs(b[r]));
+            m3 = fmax_num(m3, fabs(c[r]));
+        }
+    }
+    if (!MAX) return;
+    block_max_into(m1, s.umax + 2, red);
+    block_max_into(m2, s.umax + 3, red);
+    block_max_into(m3, s.umax + 4, red);
+}
+
 // moment grid g (0: d(x); 1-3: d(x, v_g); 4-6: d(x, v_{g-3}.^2)) of the
 // particle's velocity, and the fraction bits of its grid
 __device__ __forceinline__ double moment_u(int g, double v1, double v2, double v3) {
@@ -1055,7 +1137,9 @@ void em_launch_sweep(const EM1D& s, int sweep, hipStream_t st) {
 
 void em_launch_end(const EM1D& s, bool sort, bool moments, hipStream_t st) {
     const unsigned nb = em_blocks(s.P);
-    if (sort) {
+    if (sort && s.P <= kEMSmallSort) {
+        k_em_sort_small<true, true><<<1, 1024, 0, st>>>(s);
+    } else if (sort) {
         const int ntiles = (s.nb + kEMScanTile - 1) / kEMScanTile;
         k_em_sort_zero<<<(unsigned)((s.nb + 255) / 256 < 1024 ? (s.nb + 255) / 256 : 1024), 256, 0, st>>>(s);
         k_em_wrap<true><<<nb, kEMThreads, 0, st>>>(s);
@@ -1082,7 +1166,11 @@ void emx_launch_step(const EM1D& s, bool sort, bool moments, hipStream_t st) {
     const unsigned nb = em_blocks(s.P);
     const unsigned nt = (unsigned)((s.P + s.tile - 1) / s.tile);
     const size_t solve = em_solve_lds_bytes(s.N);
-    if (sort) {
+    if (sort && s.P <= kEMSmallSort) {
+        k_emx_begin<false><<<nt, kEMThreads, sizeof(u64) * 3 * (size_t)s.N, st>>>(s);
+        k_emx_solve<0, 0><<<1, kEMSolveThreads, solve, st>>>(s, nullptr, nullptr, nullptr);
+        k_em_sort_small<false, false><<<1, 1024, 0, st>>>(s);
+    } else if (sort) {
         const int ntiles = (s.nb + kEMScanTile - 1) / kEMScanTile;
         const unsigned nbk = (unsigned)((s.nb + 255) / 256 < 1024 ? (s.nb + 255) / 256 : 1024);
         k_em_sort_zero<<<nbk, 256, 0, st>>>(s);

@@ -12,6 +12,7 @@ constexpr int kEMThreads = 256;
 constexpr int kEMSolveThreads = 256;
 constexpr int kEMMomentGroups = 4;  // moment grids per group: 2, 2, 2, 1
 constexpr int kEMScanTile = 4096;
+constexpr int kEMSmallSort = 4096;  // particles the one-block sort takes (the bucket sort beyond)
 constexpr int kLimbEM = 38;         // exact sums: three 38-bit two's-complement limbs per cell
 constexpr int kEMMaxSweeps = 8;     // ctl[s] = sweep s active; ctl[kEMMaxSweeps] = sweeps run this step
 
