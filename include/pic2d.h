@@ -44,8 +44,8 @@ typedef struct pic2d_config {
     double B0;          /* magnetic field along x (:24)                                */
     int32_t ns;         /* diagnostics every ns steps (the script's NS = 2, :25)       */
     int32_t device;     /* HIP device ordinal                                          */
-    int32_t sort_every; /* mixed: steps between regroupings of the particles by 16x16
-                           cell tile (0 = 8); results do not depend on it          */
+    int32_t sort_every; /* steps between regroupings of the particles by 16x16 cell
+                           tile (0 = 8); results do not depend on it               */
 } pic2d_config;
 
 typedef struct pic2d_ctx pic2d_ctx;

@@ -24,13 +24,14 @@ struct pic2d_ctx {
     double* tw = nullptr;
     bool uploaded = false;
     int64_t step = 0;                // steps since upload
-    // step graphs per buffer parity of the mixed particle arrays
+    // step graphs per buffer parity of the particle arrays
     hipGraphExec_t g_plain[2] = {nullptr, nullptr};
     hipGraphExec_t g_diag[2] = {nullptr, nullptr};
-    int par = 0;                     // mixed: which buffer set holds the particles
-    int64_t since_sort = 0;          // mixed: steps since the last tile sort
+    int par = 0;                     // which buffer set holds the particles
+    int64_t since_sort = 0;          // steps since the last tile sort
     uint32_t *xq[2] = {}, *yq[2] = {}, *pid[2] = {};
     float *vx[2] = {}, *vy[2] = {}, *vz[2] = {};
+    double *X64[2] = {}, *Y64[2] = {}, *VX64[2] = {}, *VY64[2] = {}, *VZ64[2] = {};
     // operator scratch (grown on demand)
     void* op[8] = {};
     size_t op_bytes[8] = {};
@@ -104,10 +105,20 @@ static void free_graphs(pic2d_ctx* c) {
     }
 }
 
-// Points the mixed particle arrays of the launch descriptors at buffer set par.
+// Points the particle arrays of the launch descriptors at buffer set par.
 static void set_par(pic2d_ctx* c, int par) {
     Buffers2D& b = c->b;
     c->par = par;
+    b.X = c->X64[par];
+    b.Y = c->Y64[par];
+    b.VX = c->VX64[par];
+    b.VY = c->VY64[par];
+    b.VZ = c->VZ64[par];
+    b.X_alt = c->X64[1 - par];
+    b.Y_alt = c->Y64[1 - par];
+    b.VX_alt = c->VX64[1 - par];
+    b.VY_alt = c->VY64[1 - par];
+    b.VZ_alt = c->VZ64[1 - par];
     b.Xq = c->xq[par];
     b.Yq = c->yq[par];
     b.VXf = c->vx[par];
@@ -122,7 +133,7 @@ static void set_par(pic2d_ctx* c, int par) {
     b.pid_alt = c->pid[1 - par];
 }
 
-// Regroups the mixed particles by tile (into the other buffer set).
+// Regroups the particles by tile (into the other buffer set).
 static int tile_sort(pic2d_ctx* c) {
     launch2d_tile_sort(c->b, c->stream);
     HIPCHK(c, hipGetLastError());
@@ -242,13 +253,25 @@ extern "C" int pic2d_create(pic2d_ctx** out, const pic2d_config* cfg) {
         b.blocks_push = (int)std::min<int64_t>(bl, 8192);
     }
     int rc;
-    if (cfg->precision == PIC_PREC_MIXED) {
-        for (int k = 0; k < 2; ++k)
+    // both precisions hold the particles by tile (two buffer sets, the
+    // caller's index of each held particle)
+    for (int k = 0; k < 2; ++k) {
+        if ((rc = dalloc(c, &c->pid[k], P))) return fail(rc);
+        if (cfg->precision == PIC_PREC_MIXED) {
             if ((rc = dalloc(c, &c->xq[k], P)) || (rc = dalloc(c, &c->yq[k], P)) || (rc = dalloc(c, &c->vx[k], P)) ||
-                (rc = dalloc(c, &c->vy[k], P)) || (rc = dalloc(c, &c->vz[k], P)) || (rc = dalloc(c, &c->pid[k], P)))
+                (rc = dalloc(c, &c->vy[k], P)) || (rc = dalloc(c, &c->vz[k], P)))
                 return fail(rc);
-        if ((rc = dalloc(c, &b.racc, G))) return fail(rc);
-        set_par(c, 0);
+        } else {
+            if ((rc = dalloc(c, &c->X64[k], P)) || (rc = dalloc(c, &c->Y64[k], P)) ||
+                (rc = dalloc(c, &c->VX64[k], P)) || (rc = dalloc(c, &c->VY64[k], P)) ||
+                (rc = dalloc(c, &c->VZ64[k], P)))
+                return fail(rc);
+        }
+    }
+    if (cfg->precision != PIC_PREC_MIXED && (rc = dalloc(c, &b.slot, P))) return fail(rc);
+    if ((rc = dalloc(c, &b.racc, (cfg->precision == PIC_PREC_MIXED ? 1 : 3) * G))) return fail(rc);
+    set_par(c, 0);
+    {
         Tiles2D& T = b.T;
         T.TX = std::min(kTile2D, NX);
         T.TY = std::min(kTile2D, NY);
@@ -261,10 +284,6 @@ extern "C" int pic2d_create(pic2d_ctx** out, const pic2d_config* cfg) {
             (rc = dalloc(c, &T.cursor, T.ntiles)))
             return fail(rc);
         if (hipMemset(T.count, 0, sizeof(int32_t) * T.ntiles) != hipSuccess) return fail(PIC_EHIP);
-    } else {
-        if ((rc = dalloc(c, &b.X, P)) || (rc = dalloc(c, &b.Y, P)) || (rc = dalloc(c, &b.VX, P)) ||
-            (rc = dalloc(c, &b.VY, P)) || (rc = dalloc(c, &b.VZ, P)) || (rc = dalloc(c, &b.racc, 3 * G)))
-            return fail(rc);
     }
     if ((rc = dalloc(c, &b.S, G)) || (rc = dalloc(c, &b.T1, G)) || (rc = dalloc(c, &b.T2, G)) ||
         (rc = dalloc(c, &b.T3, G)) || (rc = dalloc(c, &b.Ex, G)) || (rc = dalloc(c, &b.Ey, G)) ||
@@ -336,6 +355,9 @@ extern "C" int pic2d_upload(pic2d_ctx* c, const double* x, const double* y, cons
         HIPCHK(c, hipMemcpy(b.VX, vx, 8 * P, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(b.VY, vy, 8 * P, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(b.VZ, vz, 8 * P, hipMemcpyHostToDevice));
+        std::vector<uint32_t> q(P);
+        for (int64_t p = 0; p < P; ++p) q[p] = (uint32_t)p;
+        HIPCHK(c, hipMemcpy(b.pid, q.data(), 4 * P, hipMemcpyHostToDevice));
     }
     std::vector<double> ex(G, 0.0), ey(G, 0.0);
     if (Ex) std::copy(Ex, Ex + G, ex.begin());
@@ -346,7 +368,7 @@ extern "C" int pic2d_upload(pic2d_ctx* c, const double* x, const double* y, cons
     HIPCHK(c, hipMemcpy(b.Ey, ey.data(), 8 * G, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(b.E32, e32.data(), sizeof(float2) * G, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemset(b.flag, 0, sizeof(int32_t)));
-    if (c->cfg.precision == PIC_PREC_MIXED) {
+    {
         int rc = tile_sort(c);
         if (rc) return rc;
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -388,11 +410,17 @@ extern "C" int pic2d_download(pic2d_ctx* c, double* x, double* y, double* vx, do
             for (int64_t p = 0; p < P; ++p) ds[k][id[p]] = (double)f[p];
         }
     } else {
-        if (x) HIPCHK(c, hipMemcpy(x, b.X, 8 * P, hipMemcpyDeviceToHost));
-        if (y) HIPCHK(c, hipMemcpy(y, b.Y, 8 * P, hipMemcpyDeviceToHost));
-        if (vx) HIPCHK(c, hipMemcpy(vx, b.VX, 8 * P, hipMemcpyDeviceToHost));
-        if (vy) HIPCHK(c, hipMemcpy(vy, b.VY, 8 * P, hipMemcpyDeviceToHost));
-        if (vz) HIPCHK(c, hipMemcpy(vz, b.VZ, 8 * P, hipMemcpyDeviceToHost));
+        // held by tile too: back to the caller's order through pid
+        std::vector<uint32_t> id(P);
+        std::vector<double> f(P);
+        HIPCHK(c, hipMemcpy(id.data(), b.pid, 4 * P, hipMemcpyDeviceToHost));
+        const double* ss[5] = {b.X, b.Y, b.VX, b.VY, b.VZ};
+        double* ds[5] = {x, y, vx, vy, vz};
+        for (int k = 0; k < 5; ++k) {
+            if (!ds[k]) continue;
+            HIPCHK(c, hipMemcpy(f.data(), ss[k], 8 * P, hipMemcpyDeviceToHost));
+            for (int64_t p = 0; p < P; ++p) ds[k][id[p]] = f[p];
+        }
     }
     if (Ex) HIPCHK(c, hipMemcpy(Ex, b.Ex, 8 * G, hipMemcpyDeviceToHost));
     if (Ey) HIPCHK(c, hipMemcpy(Ey, b.Ey, 8 * G, hipMemcpyDeviceToHost));
@@ -446,10 +474,9 @@ extern "C" int pic2d_run(pic2d_ctx* c, int64_t nsteps, double* K) {
     }
     const int64_t zero = 0;
     HIPCHK(c, hipMemcpyAsync(b.drow, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
-    const bool mixed = c->cfg.precision == PIC_PREC_MIXED;
     const int64_t every = c->cfg.sort_every > 0 ? c->cfg.sort_every : kSortEvery2D;
     for (int64_t t = t0 + 1; t <= t1; ++t) {
-        if (mixed && c->since_sort >= every && (rc = tile_sort(c))) return rc;
+        if (c->since_sort >= every && (rc = tile_sort(c))) return rc;
         const int k = c->par;
         if (!c->g_plain[k] && (rc = capture(c, false, &c->g_plain[k]))) return rc;
         if (!c->g_diag[k] && (rc = capture(c, true, &c->g_diag[k]))) return rc;

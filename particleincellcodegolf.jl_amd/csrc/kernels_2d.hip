@@ -399,6 +399,164 @@ k2d_tile_scatter(Desc2D D, Tiles2D T, const uint32_t* __restrict__ Xq, const uin
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 particles held by tile.  The tile of a particle is that of its area64
+// cell c0 on each axis.  k2d_push64t is k2d_push64<0>'s arithmetic with the
+// tile's field (and kHalo2D cells around it) read from LDS and the charge's
+// three limbs summed in LDS, then added to the grid's limbs; a particle whose
+// cells fall outside the window reads and adds through global memory.  Every
+// particle's update depends only on its own state and the field, and the limb
+// sums are exact, so the results are k2d_push64<0>'s.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int32_t tile_of64(const Desc2D& D, const Tiles2D& T, double x, double y) {
+    const int32_t cx = area64(x, D.NX, D.pairing).c0, cy = area64(y, D.NY, D.pairing).c0;
+    return (cx / T.TX) + T.ntx * (cy / T.TY);
+}
+
+__global__ void __launch_bounds__(256)
+k2d_tile_count64(Desc2D D, Tiles2D T, const double* __restrict__ X, const double* __restrict__ Y) {
+    extern __shared__ int32_t hist[];
+    for (int i = threadIdx.x; i < T.ntiles; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < D.P; p += stride)
+        atomicAdd(&hist[tile_of64(D, T, X[p], Y[p])], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < T.ntiles; i += blockDim.x)
+        if (hist[i]) atomicAdd(&T.count[i], hist[i]);
+}
+
+// as k2d_tile_scatter; slot[i] gets the new slot of caller index i
+__global__ void __launch_bounds__(256)
+k2d_tile_scatter64(Desc2D D, Tiles2D T, const double* __restrict__ X, const double* __restrict__ Y,
+                   const double* __restrict__ VX, const double* __restrict__ VY, const double* __restrict__ VZ,
+                   const uint32_t* __restrict__ pid, double* __restrict__ Xo, double* __restrict__ Yo,
+                   double* __restrict__ VXo, double* __restrict__ VYo, double* __restrict__ VZo,
+                   uint32_t* __restrict__ pido, uint32_t* __restrict__ slot, int64_t per_block) {
+    extern __shared__ int32_t hist[];
+    for (int i = threadIdx.x; i < T.ntiles; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * per_block;
+    const int64_t b1 = b0 + per_block < D.P ? b0 + per_block : D.P;
+    for (int64_t p = b0 + threadIdx.x; p < b1; p += blockDim.x) atomicAdd(&hist[tile_of64(D, T, X[p], Y[p])], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < T.ntiles; i += blockDim.x)
+        if (hist[i]) hist[i] = atomicAdd(&T.cursor[i], hist[i]);
+    __syncthreads();
+    for (int64_t p = b0 + threadIdx.x; p < b1; p += blockDim.x) {
+        const double x = X[p], y = Y[p];
+        const int32_t d = atomicAdd(&hist[tile_of64(D, T, x, y)], 1);
+        const uint32_t id = pid[p];
+        Xo[d] = x;
+        Yo[d] = y;
+        VXo[d] = VX[p];
+        VYo[d] = VY[p];
+        VZo[d] = VZ[p];
+        pido[d] = id;
+        slot[id] = (uint32_t)d;
+    }
+}
+
+__global__ void __launch_bounds__(kTileThreads)
+k2d_push64t(Desc2D D, Tiles2D T, double* __restrict__ X, double* __restrict__ Y, double* __restrict__ VX,
+            double* __restrict__ VY, double* __restrict__ VZ, const double* __restrict__ Ex,
+            const double* __restrict__ Ey, u64* __restrict__ racc, double dt, double B0) {
+    constexpr int W = kTile2D + 2 * kHalo2D;
+    __shared__ double exw[W * W], eyw[W * W];
+    __shared__ u64 aw[3 * W * W];
+    const int tile = blockIdx.x / T.splits, part = blockIdx.x % T.splits;
+    const int tx = tile % T.ntx, ty = tile / T.ntx;
+    const int wx = T.TX + 2 * kHalo2D, wy = T.TY + 2 * kHalo2D, nw = wx * wy;
+    const int ox = tx * T.TX - kHalo2D, oy = ty * T.TY - kHalo2D;
+    const int64_t G = (int64_t)D.NX * D.NY;
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) {
+        const int gx = (ox + i % wx + D.NX) % D.NX, gy = (oy + i / wx + D.NY) % D.NY;
+        exw[i] = Ex[gx + (int64_t)D.NX * gy];
+        eyw[i] = Ey[gx + (int64_t)D.NX * gy];
+        aw[i] = 0;
+        aw[nw + i] = 0;
+        aw[2 * nw + i] = 0;
+    }
+    __syncthreads();
+    double t1, tscale;
+    boris_consts(dt, B0, t1, tscale);
+    const int64_t a0 = T.start[tile], a1 = T.start[tile + 1];
+    const int64_t len = a1 - a0;
+    const int64_t p0 = a0 + len * part / T.splits, p1 = a0 + len * (part + 1) / T.splits;
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        double x = X[p], y = Y[p], vx = VX[p], vy = VY[p], vz = VZ[p];
+        {
+            const Area64 ax = area64(x, D.NX, D.pairing), ay = area64(y, D.NY, D.pairing);
+            const int32_t ci[2] = {ax.c0, ax.c1}, cj[2] = {ay.c0, ay.c1};
+            const double wi[2] = {ax.w0, ax.w1}, wj[2] = {ay.w0, ay.w1};
+            const int32_t ri[2] = {win_rel(ax.c0, ox, D.NX), win_rel(ax.c1, ox, D.NX)};
+            const int32_t rj[2] = {win_rel(ay.c0, oy, D.NY), win_rel(ay.c1, oy, D.NY)};
+            const bool in = ri[0] < wx && ri[1] < wx && rj[0] < wy && rj[1] < wy;
+            double f1 = 0.0, f2 = 0.0;  // gather64's order: j outer, i inner
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int a = 0; a < 2; ++a) {
+                    const double wxy = wi[a] * wj[b];
+                    double e1, e2;
+                    if (in) {
+                        e1 = exw[ri[a] + wx * rj[b]];
+                        e2 = eyw[ri[a] + wx * rj[b]];
+                    } else {
+                        const int64_t k = (int64_t)ci[a] + (int64_t)D.NX * cj[b];
+                        e1 = Ex[k];
+                        e2 = Ey[k];
+                    }
+                    f1 = fma(e1, wxy, f1);
+                    f2 = fma(e2, wxy, f2);
+                }
+            boris64(vx, vy, vz, f1, f2, dt, t1, tscale);
+        }
+        x = unimod1(x + vx * dt);
+        y = unimod1(y + vy * dt);
+        X[p] = x;
+        Y[p] = y;
+        VX[p] = vx;
+        VY[p] = vy;
+        VZ[p] = vz;
+        // deposit64's contributions (wx*wy)*w, j outer, i inner
+        const Area64 bx = area64(x, D.NX, D.pairing), by = area64(y, D.NY, D.pairing);
+        const int32_t ci[2] = {bx.c0, bx.c1}, cj[2] = {by.c0, by.c1};
+        const double wi[2] = {bx.w0, bx.w1}, wj[2] = {by.w0, by.w1};
+        const int32_t ri[2] = {win_rel(bx.c0, ox, D.NX), win_rel(bx.c1, ox, D.NX)};
+        const int32_t rj[2] = {win_rel(by.c0, oy, D.NY), win_rel(by.c1, oy, D.NY)};
+        const bool in = ri[0] < wx && ri[1] < wx && rj[0] < wy && rj[1] < wy;
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const double c = wi[a] * wj[b] * D.w;
+                const u128 v = fxp_from(c, D.q);
+                if (in) {
+                    const int e = ri[a] + wx * rj[b];
+                    const u64 l0 = (u64)v & kLimb2DMask;
+                    const u64 l1 = (u64)(v >> kLimb2D) & kLimb2DMask;
+                    const u64 l2 = (u64)(v >> (2 * kLimb2D));
+                    if (l0) atomicAdd(&aw[e], l0);
+                    if (l1) atomicAdd(&aw[nw + e], l1);
+                    if (l2) atomicAdd(&aw[2 * nw + e], l2);
+                } else {
+                    add_limbs42(racc, G, (int64_t)ci[a] + (int64_t)D.NX * cj[b], v);
+                }
+            }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) {
+        const int gx = (ox + i % wx + D.NX) % D.NX, gy = (oy + i / wx + D.NY) % D.NY;
+        const int64_t k = gx + (int64_t)D.NX * gy;
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            const u64 v = aw[l * nw + i];
+            if (v) atomicAdd(racc + l * G + k, v);
+        }
+    }
+}
+
 
 // Mixed kinetic diagnostics, independent of the particle order: per chunk of
 // 4096 held particles the exact integer sums of (vx^2 + vy^2) on a 2^-50 grid
@@ -655,16 +813,20 @@ k2d_diag_grid(const double* __restrict__ Ex, const double* __restrict__ Ey, int6
     if (threadIdx.x == 0) gpart[blockIdx.x] = r;
 }
 
+// slot (may be NULL): where each caller index's particle is held, so that the
+// sums run in the caller's order
 template <typename T>
 __global__ void __launch_bounds__(256)
-k2d_diag_parts(const T* __restrict__ VX, const T* __restrict__ VY, int64_t P, double w, double* __restrict__ ppart) {
+k2d_diag_parts(const T* __restrict__ VX, const T* __restrict__ VY, int64_t P, double w, double* __restrict__ ppart,
+               const uint32_t* __restrict__ slot) {
     __shared__ double red[3 * kTreeLanes + 8];
     const int64_t s = (int64_t)blockIdx.x * kTreeChunk;
     const int64_t n = P - s < kTreeChunk ? P - s : kTreeChunk;
     double a = 0.0, b = 0.0, c = 0.0;
     if (threadIdx.x < kTreeLanes)
         for (int64_t i = threadIdx.x; i < n; i += kTreeLanes) {
-            const double vx = (double)VX[s + i], vy = (double)VY[s + i];
+            const int64_t q = slot ? (int64_t)slot[s + i] : s + i;
+            const double vx = (double)VX[q], vy = (double)VY[q];
             a = a + (vx * vx + vy * vy) * w;
             b = b + vx;
             c = c + vy;
@@ -715,8 +877,8 @@ void launch2d_push(const Buffers2D& b, hipStream_t s) {
             b.D, b.T, b.Xq, b.Yq, b.VXf, b.VYf, b.VZf, b.E32, b.racc, (float)(b.dt / 2.0), (float)t1,
             (float)tscale, (float)(b.dt * 4294967296.0));
     } else {
-        k2d_push64<0><<<b.blocks_push, kPush2DThreads, 0, s>>>(b.D, b.X, b.Y, b.VX, b.VY, b.VZ, b.Ex, b.Ey, b.racc,
-                                                             b.dt, b.B0);
+        k2d_push64t<<<b.T.ntiles * b.T.splits, kTileThreads, 0, s>>>(b.D, b.T, b.X, b.Y, b.VX, b.VY, b.VZ, b.Ex, b.Ey,
+                                                                     b.racc, b.dt, b.B0);
     }
 }
 
@@ -724,10 +886,17 @@ void launch2d_tile_sort(const Buffers2D& b, hipStream_t s) {
     const size_t lds = sizeof(int32_t) * (size_t)b.T.ntiles;
     const int64_t per = 16384;
     const int nb = (int)((b.D.P + per - 1) / per);
-    k2d_tile_count<<<nb, 256, lds, s>>>(b.D, b.T, b.Xq, b.Yq);
-    k2d_tile_scan<<<1, 1024, 0, s>>>(b.T);
-    k2d_tile_scatter<<<nb, 256, lds, s>>>(b.D, b.T, b.Xq, b.Yq, b.VXf, b.VYf, b.VZf, b.pid, b.Xq_alt, b.Yq_alt,
-                                          b.VXf_alt, b.VYf_alt, b.VZf_alt, b.pid_alt, per);
+    if (b.precision == 1) {
+        k2d_tile_count<<<nb, 256, lds, s>>>(b.D, b.T, b.Xq, b.Yq);
+        k2d_tile_scan<<<1, 1024, 0, s>>>(b.T);
+        k2d_tile_scatter<<<nb, 256, lds, s>>>(b.D, b.T, b.Xq, b.Yq, b.VXf, b.VYf, b.VZf, b.pid, b.Xq_alt, b.Yq_alt,
+                                              b.VXf_alt, b.VYf_alt, b.VZf_alt, b.pid_alt, per);
+    } else {
+        k2d_tile_count64<<<nb, 256, lds, s>>>(b.D, b.T, b.X, b.Y);
+        k2d_tile_scan<<<1, 1024, 0, s>>>(b.T);
+        k2d_tile_scatter64<<<nb, 256, lds, s>>>(b.D, b.T, b.X, b.Y, b.VX, b.VY, b.VZ, b.pid, b.X_alt, b.Y_alt,
+                                                b.VX_alt, b.VY_alt, b.VZ_alt, b.pid_alt, b.slot, per);
+    }
 }
 
 void launch2d_solve(const Buffers2D& b, int src, const double* rho_in, bool want_phi, hipStream_t s) {
@@ -752,7 +921,7 @@ void launch2d_diag_op(const double* Ex, const double* Ey, int64_t G, const doubl
                       double w, double* gpart, double* ppart, double* out, int64_t* drow, hipStream_t s) {
     const int64_t ng = (G + kTreeChunk - 1) / kTreeChunk, np = (P + kTreeChunk - 1) / kTreeChunk;
     k2d_diag_grid<<<(unsigned)ng, 256, 0, s>>>(Ex, Ey, G, gpart);
-    k2d_diag_parts<double><<<(unsigned)np, 256, 0, s>>>(vx, vy, P, w, ppart);
+    k2d_diag_parts<double><<<(unsigned)np, 256, 0, s>>>(vx, vy, P, w, ppart, nullptr);
     k2d_diag_final<<<1, 256, 0, s>>>(gpart, ng, ppart, np, G, P, out, drow, drow ? INT64_MAX : 1);
 }
 
@@ -767,7 +936,7 @@ void launch2d_diag(const Buffers2D& b, hipStream_t s) {
         k2d_diag_final32x<<<1, 256, 0, s>>>(b.gpart, ng, pp, np, G, P, b.D.w, b.diag, b.drow, b.diag_cap);
         return;
     }
-    k2d_diag_parts<double><<<(unsigned)np, 256, 0, s>>>(b.VX, b.VY, P, b.D.w, b.ppart);
+    k2d_diag_parts<double><<<(unsigned)np, 256, 0, s>>>(b.VX, b.VY, P, b.D.w, b.ppart, b.slot);
     k2d_diag_final<<<1, 256, 0, s>>>(b.gpart, ng, b.ppart, np, G, P, b.diag, b.drow, b.diag_cap);
 }
 

@@ -15,7 +15,7 @@ constexpr int kLimbS = 38;          // signed exact deposit: 3 limbs of 38 bits,
 constexpr int kQuanta2D = 20;       // mixed: area weight quanta per axis, 2^-20 of a cell
 constexpr int64_t kMaxP2D64 = int64_t(1) << 22;
 constexpr int64_t kMaxP2D32 = int64_t(1) << 24;
-// mixed path: particles held in tiles of kTile2D x kTile2D cells; a push
+// particles held in tiles of kTile2D x kTile2D cells; a push
 // block stages its tile's field and charge with kHalo2D cells around it
 constexpr int kTile2D = 16;
 constexpr int kHalo2D = 4;
@@ -51,11 +51,13 @@ struct Buffers2D {
     double *X, *Y, *VX, *VY, *VZ;
     uint32_t *Xq, *Yq;
     float *VXf, *VYf, *VZf;
-    uint32_t* pid;           // mixed: caller's index of each held particle
-    // mixed: the other buffer of each particle array (tile sorts move the
-    // particles there)
+    uint32_t* pid;           // caller's index of each held particle
+    // the other buffer of each particle array (tile sorts move the particles
+    // there)
     uint32_t *Xq_alt, *Yq_alt, *pid_alt;
     float *VXf_alt, *VYf_alt, *VZf_alt;
+    double *X_alt, *Y_alt, *VX_alt, *VY_alt, *VZ_alt;
+    uint32_t* slot;          // FP64: the slot holding caller index i (diagnostics run in the caller's order)
     Tiles2D T;
     unsigned long long* racc;  // FP64: 3 limbs x NX*NY; mixed: NX*NY quanta
     double2 *S, *T1, *T2, *T3;

@@ -203,14 +203,15 @@ def test_upload_rejects_positions_outside_unit_box(pg):
     assert e.value.code == -1 and "outside" in str(e.value)
 
 
-def test_mixed_results_do_not_depend_on_the_tile_sorts(pg):
-    """The mixed path regroups its particles by tile every few steps; the
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_results_do_not_depend_on_the_tile_sorts(pg, precision):
+    """Both paths regroup their particles by tile every few steps; the
     particles, the field and every diagnostic are the same bit for bit whether
-    it regroups every step, every 8 steps or never again after the upload."""
+    they regroup every step, every 8 steps or never again after the upload."""
     from picgolf import configs
     outs = []
     for every in (1, 8, 10 ** 6):
-        cfg, st, _ = configs.es2d3v(precision="mixed", NX=64, NY=64, ppc=16)
+        cfg, st, _ = configs.es2d3v(precision=precision, NX=64, NY=64, ppc=16)
         cfg.sort_every = every
         with pg.Context2D(cfg) as c:
             c.upload(*st)
