@@ -27,6 +27,14 @@ struct pic2d3v_ctx {
     Grid23 g{};
     Species23 sp[kMaxSpecies23]{};
     bool uploaded[kMaxSpecies23] = {};
+    // regrouping by tile: the other buffer set, the caller's index of each
+    // held particle (and its other buffer), the tiles
+    Species23 alt[kMaxSpecies23]{};
+    uint32_t* pid[kMaxSpecies23] = {};
+    uint32_t* pid_alt[kMaxSpecies23] = {};
+    bool held_identity[kMaxSpecies23] = {};
+    Tiles23 tiles{};
+    int64_t since_regroup = 0;
     Diag23 d{};
     Buffers2D b{};           // the 2D solve's descriptor and buffers
     double2* E = nullptr;    // the gathered field Exy (interior, (Ex, Ey) per cell)
@@ -346,6 +354,27 @@ extern "C" int pic2d3v_create(pic2d3v_ctx** out, const pic2d3v_config* cfg) {
             return fail(rc);
         for (double* a : {sp.x, sp.y, sp.vx, sp.vy, sp.vz})
             if (hipMemset(a, 0, sizeof(double) * (size_t)q.P) != hipSuccess) return fail(PIC_EHIP);
+        Species23& al = c->alt[s];
+        al = sp;
+        if ((rc = dalloc(c, &al.x, q.P)) || (rc = dalloc(c, &al.y, q.P)) || (rc = dalloc(c, &al.vx, q.P)) ||
+            (rc = dalloc(c, &al.vy, q.P)) || (rc = dalloc(c, &al.vz, q.P)) || (rc = dalloc(c, &sp.slot, q.P)) ||
+            (rc = dalloc(c, &c->pid[s], q.P)) || (rc = dalloc(c, &c->pid_alt[s], q.P)))
+            return fail(rc);
+        al.slot = sp.slot;
+        launch23_identity(c->pid[s], sp.slot, q.P, c->stream);
+        c->held_identity[s] = true;
+    }
+    {
+        Tiles23& T = c->tiles;
+        T.TX = std::min(kRegroupTile23, NX);
+        T.TY = std::min(kRegroupTile23, NY);
+        T.ntx = NX / T.TX;
+        T.nty = NY / T.TY;
+        T.ntiles = T.ntx * T.nty;
+        if ((rc = dalloc(c, &T.count, T.ntiles)) || (rc = dalloc(c, &T.start, T.ntiles + 1)) ||
+            (rc = dalloc(c, &T.cursor, T.ntiles)))
+            return fail(rc);
+        if (hipMemset(T.count, 0, sizeof(int32_t) * T.ntiles) != hipSuccess) return fail(PIC_EHIP);
     }
     g.q = frac_bits23(ns, Ps, qws);
     Buffers2D& b = c->b;
@@ -497,8 +526,11 @@ extern "C" int pic2d3v_upload(pic2d3v_ctx* c, int32_t s, const double* xyv) {
     if ((rc = scratch(c, 0, sizeof(double) * 5 * (size_t)sp.P, &st))) return rc;
     HIPCHK(c, hipMemcpyAsync(st, xyv, sizeof(double) * 5 * (size_t)sp.P, hipMemcpyHostToDevice, c->stream));
     launch23_unpack((const double*)st, sp, c->stream);
+    launch23_identity(c->pid[s], sp.slot, sp.P, c->stream);
     if ((rc = finish(c))) return rc;
     c->uploaded[s] = true;
+    c->held_identity[s] = true;
+    c->since_regroup = kRegroupEvery23;  // regroup before the next step
     return PIC_OK;
 }
 
@@ -547,6 +579,45 @@ extern "C" int pic2d3v_field(pic2d3v_ctx* c, double* Ex, double* Ey) {
 
 // ---- sort!(species, dx, dy) -------------------------------------------------
 
+// Regroups every species' held particles by tile (see launch23_regroup); the
+// captured step graphs hold the old arrays and are dropped.
+static int regroup(pic2d3v_ctx* c) {
+    for (int s = 0; s < c->cfg.nspecies; ++s) {
+        Species23& sp = c->sp[s];
+        Species23& al = c->alt[s];
+        launch23_regroup(c->g, c->tiles, sp, al, c->pid[s], c->pid_alt[s], c->stream);
+        std::swap(sp.x, al.x);
+        std::swap(sp.y, al.y);
+        std::swap(sp.vx, al.vx);
+        std::swap(sp.vy, al.vy);
+        std::swap(sp.vz, al.vz);
+        std::swap(c->pid[s], c->pid_alt[s]);
+        c->held_identity[s] = false;
+    }
+    HIPCHK(c, hipGetLastError());
+    free_graphs(c);
+    c->since_regroup = 0;
+    return PIC_OK;
+}
+
+// The held particles of species s back in the caller's order.
+static int to_caller(pic2d3v_ctx* c, int32_t s) {
+    if (c->held_identity[s]) return PIC_OK;
+    Species23& sp = c->sp[s];
+    Species23& al = c->alt[s];
+    launch23_to_caller(sp, al, c->stream);
+    std::swap(sp.x, al.x);
+    std::swap(sp.y, al.y);
+    std::swap(sp.vx, al.vx);
+    std::swap(sp.vy, al.vy);
+    std::swap(sp.vz, al.vz);
+    launch23_identity(c->pid[s], sp.slot, sp.P, c->stream);
+    HIPCHK(c, hipGetLastError());
+    c->held_identity[s] = true;
+    free_graphs(c);
+    return PIC_OK;
+}
+
 extern "C" int pic2d3v_sort(pic2d3v_ctx* c, int32_t s, double dx, double dy, int64_t* perm) {
     if (!c) return PIC_EINVAL;
     if (int rc = check_species(c, "pic2d3v_sort", s)) return rc;
@@ -559,11 +630,13 @@ extern "C" int pic2d3v_sort(pic2d3v_ctx* c, int32_t s, double dx, double dy, int
         return PIC_ESTATE;
     }
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    int rc;
+    // sort!(species) permutes the caller's order: from that order
+    if ((rc = to_caller(c, s))) return rc;
     const Species23& sp = c->sp[s];
     const int64_t P = sp.P;
     Sort23 w{};
     void* p = nullptr;
-    int rc;
     // slot 3: ext, maxcount; 4: per-particle arrays; 5: per-bucket arrays; 0: applied particles
     if ((rc = scratch(c, 3, 64, &p))) return rc;
     w.ext = (int64_t*)p;
@@ -813,11 +886,15 @@ extern "C" int pic2d3v_run(pic2d3v_ctx* c, int64_t nsteps, double* rows) {
     if (c->semi) {
         c->iters.clear();
         for (int64_t t = t0; t < t1; ++t) {
+            if (c->since_regroup >= kRegroupEvery23 && (rc = regroup(c))) return rc;
             if ((rc = step_si(c, t % nt == 0 ? 2 : 1))) return rc;
             c->step = t + 1;
+            ++c->since_regroup;
         }
     } else {
         for (int64_t t = t0; t < t1; ++t) {
+            if (c->since_regroup >= kRegroupEvery23 && (rc = regroup(c))) return rc;
+            ++c->since_regroup;
             const bool diag = t % nt == 0;
             hipGraphExec_t& ge = c->lorenz ? c->g_lg[diag ? 2 : 1] : (diag ? c->g_diag : c->g_plain);
             if (!ge && (rc = capture(c, c->lorenz ? (diag ? 2 : 1) : (diag ? 1 : 0), &ge))) return rc;
@@ -956,13 +1033,19 @@ extern "C" int pic2d3v_loop(pic2d3v_ctx* c, int64_t nsteps) {
     if (c->semi) {
         c->iters.clear();
         for (int64_t t = 0; t < nsteps; ++t) {
+            if (c->since_regroup >= kRegroupEvery23 && (rc = regroup(c))) return rc;
             if ((rc = step_si(c, 0))) return rc;
             c->step += 1;
+            ++c->since_regroup;
         }
         return finish(c);
     }
-    if (!c->g_lg[0] && (rc = capture(c, 0, &c->g_lg[0]))) return rc;
-    for (int64_t t = 0; t < nsteps; ++t) HIPCHK(c, hipGraphLaunch(c->g_lg[0], c->stream));
+    for (int64_t t = 0; t < nsteps; ++t) {
+        if (c->since_regroup >= kRegroupEvery23 && (rc = regroup(c))) return rc;
+        ++c->since_regroup;
+        if (!c->g_lg[0] && (rc = capture(c, 0, &c->g_lg[0]))) return rc;
+        HIPCHK(c, hipGraphLaunch(c->g_lg[0], c->stream));
+    }
     if ((rc = finish(c))) return rc;
     c->step += nsteps;
     return PIC_OK;

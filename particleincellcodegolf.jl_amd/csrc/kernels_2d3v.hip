@@ -317,7 +317,8 @@ k23_diag_kinetic(Species23 s, double* __restrict__ kpart) {
         const uint32_t dp = q / 3u;
         const uint32_t c = q - 3u * dp;
         const int64_t p = p0 + dp;
-        const double v = c == 0 ? s.vx[p] : (c == 1 ? s.vy[p] : s.vz[p]);
+        const int64_t h = s.slot ? (int64_t)s.slot[p] : p;  // caller index p's slot
+        const double v = c == 0 ? s.vx[h] : (c == 1 ? s.vy[h] : s.vz[h]);
         return v * v;
     });
     if (threadIdx.x == 0) kpart[blockIdx.x] = r;
@@ -332,7 +333,8 @@ k23_diag_momentum(Species23 s, double* __restrict__ mpart) {
     double a = 0.0, b = 0.0, c = 0.0, e = 0.0, f = 0.0, h = 0.0;
     if (threadIdx.x < kTreeLanes)
         for (int64_t i = threadIdx.x; i < n; i += kTreeLanes) {
-            const double vx = s.vx[b0 + i], vy = s.vy[b0 + i], vz = s.vz[b0 + i];
+            const int64_t hs = s.slot ? (int64_t)s.slot[b0 + i] : b0 + i;  // caller index b0 + i's slot
+            const double vx = s.vx[hs], vy = s.vy[hs], vz = s.vz[hs];
             a = a + vx;
             b = b + vy;
             c = c + vz;
@@ -417,14 +419,119 @@ __global__ void __launch_bounds__(256) k23_unpack(const double* __restrict__ xyv
     }
 }
 
+// caller index p from its held slot
 __global__ void __launch_bounds__(256) k23_pack(Species23 s, double* __restrict__ xyv) {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < s.P; p += (int64_t)gridDim.x * blockDim.x) {
         double* q = xyv + 5 * p;
-        q[0] = s.x[p];
-        q[1] = s.y[p];
-        q[2] = s.vx[p];
-        q[3] = s.vy[p];
-        q[4] = s.vz[p];
+        const int64_t h = s.slot ? (int64_t)s.slot[p] : p;
+        q[0] = s.x[h];
+        q[1] = s.y[h];
+        q[2] = s.vx[h];
+        q[3] = s.vy[h];
+        q[4] = s.vz[h];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Regrouping.  The deposits' LDS windows cover a chunk of consecutive held
+// particles; particles drift out of the order a sort left them in, and a
+// chunk then outgrows its window and falls back to global atomics.  Every few
+// steps the held particles are regrouped by 8 x 8-cell tile (a counting sort
+// by tile; the order inside a tile is arbitrary).  Every result is
+// independent of the held order (exact deposit sums, per-particle updates),
+// and the diagnostics' particle sums and the downloads read through slot, in
+// the caller's order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int32_t tile23(const Grid23& g, const Tiles23& T, double x, double y) {
+    return base_cell(x, g.NX_Lx, g.NX) / T.TX + T.ntx * (base_cell(y, g.NY_Ly, g.NY) / T.TY);
+}
+
+__global__ void __launch_bounds__(256) k23_regroup_count(Grid23 g, Tiles23 T, Species23 s, int64_t per_block) {
+    extern __shared__ int32_t hist[];
+    for (int i = threadIdx.x; i < T.ntiles; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * per_block;
+    const int64_t b1 = b0 + per_block < s.P ? b0 + per_block : s.P;
+    for (int64_t p = b0 + threadIdx.x; p < b1; p += blockDim.x) atomicAdd(&hist[tile23(g, T, s.x[p], s.y[p])], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < T.ntiles; i += blockDim.x)
+        if (hist[i]) atomicAdd(&T.count[i], hist[i]);
+}
+
+// one block: exclusive scan of the counts into start and the cursors; the
+// counts cleared for the next regrouping
+__global__ void __launch_bounds__(1024) k23_regroup_scan(Tiles23 T) {
+    __shared__ int32_t part[1024];
+    const int per = (T.ntiles + blockDim.x - 1) / blockDim.x;
+    const int i0 = threadIdx.x * per;
+    int32_t sum = 0;
+    for (int i = i0; i < i0 + per && i < T.ntiles; ++i) sum += T.count[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t run = 0;
+        for (int k = 0; k < (int)blockDim.x; ++k) {
+            const int32_t v = part[k];
+            part[k] = run;
+            run += v;
+        }
+        T.start[T.ntiles] = run;
+    }
+    __syncthreads();
+    int32_t run = part[threadIdx.x];
+    for (int i = i0; i < i0 + per && i < T.ntiles; ++i) {
+        T.start[i] = run;
+        T.cursor[i] = run;
+        run += T.count[i];
+        T.count[i] = 0;
+    }
+}
+
+// every held particle to a free slot of its tile's range in o; pido and the
+// slot of each caller index follow
+__global__ void __launch_bounds__(256)
+k23_regroup_scatter(Grid23 g, Tiles23 T, Species23 s, Species23 o, const uint32_t* __restrict__ pid,
+                    uint32_t* __restrict__ pido, int64_t per_block) {
+    extern __shared__ int32_t hist[];
+    for (int i = threadIdx.x; i < T.ntiles; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * per_block;
+    const int64_t b1 = b0 + per_block < s.P ? b0 + per_block : s.P;
+    for (int64_t p = b0 + threadIdx.x; p < b1; p += blockDim.x) atomicAdd(&hist[tile23(g, T, s.x[p], s.y[p])], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < T.ntiles; i += blockDim.x)
+        if (hist[i]) hist[i] = atomicAdd(&T.cursor[i], hist[i]);
+    __syncthreads();
+    for (int64_t p = b0 + threadIdx.x; p < b1; p += blockDim.x) {
+        const double x = s.x[p], y = s.y[p];
+        const int32_t d = atomicAdd(&hist[tile23(g, T, x, y)], 1);
+        const uint32_t id = pid[p];
+        o.x[d] = x;
+        o.y[d] = y;
+        o.vx[d] = s.vx[p];
+        o.vy[d] = s.vy[p];
+        o.vz[d] = s.vz[p];
+        pido[d] = id;
+        s.slot[id] = (uint32_t)d;
+    }
+}
+
+// the held particles of s in the caller's order into o
+__global__ void __launch_bounds__(256) k23_to_caller(Species23 s, Species23 o) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < s.P; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t h = s.slot[i];
+        o.x[i] = s.x[h];
+        o.y[i] = s.y[h];
+        o.vx[i] = s.vx[h];
+        o.vy[i] = s.vy[h];
+        o.vz[i] = s.vz[h];
+    }
+}
+
+__global__ void __launch_bounds__(256) k23_identity(uint32_t* __restrict__ pid, uint32_t* __restrict__ slot, int64_t P) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
+        pid[i] = (uint32_t)i;
+        slot[i] = (uint32_t)i;
     }
 }
 
@@ -702,6 +809,29 @@ void launch23_unpack(const double* xyv, const Species23& s, hipStream_t st) {
 
 void launch23_pack(const Species23& s, double* xyv, hipStream_t st) {
     k23_pack<<<blocks_for(s.P, 256), 256, 0, st>>>(s, xyv);
+}
+
+void launch23_regroup(const Grid23& g, const Tiles23& T, const Species23& s, const Species23& o, const uint32_t* pid,
+                      uint32_t* pido, hipStream_t st) {
+    if (s.P < 1) return;
+    const size_t lds = sizeof(int32_t) * (size_t)T.ntiles;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k23_regroup_count),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k23_regroup_scatter),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const int64_t per = 16384;
+    const unsigned nb = (unsigned)((s.P + per - 1) / per);
+    k23_regroup_count<<<nb, 256, lds, st>>>(g, T, s, per);
+    k23_regroup_scan<<<1, 1024, 0, st>>>(T);
+    k23_regroup_scatter<<<nb, 256, lds, st>>>(g, T, s, o, pid, pido, per);
+}
+
+void launch23_to_caller(const Species23& s, const Species23& o, hipStream_t st) {
+    if (s.P > 0) k23_to_caller<<<blocks_for(s.P, 256), 256, 0, st>>>(s, o);
+}
+
+void launch23_identity(uint32_t* pid, uint32_t* slot, int64_t P, hipStream_t st) {
+    if (P > 0) k23_identity<<<blocks_for(P, 256), 256, 0, st>>>(pid, slot, P);
 }
 
 void launch23_pair_field(const double* Ex, const double* Ey, double2* E, int64_t G, hipStream_t st) {

@@ -31,6 +31,17 @@ struct Species23 {
     double qw;            // (charge*weight)/dV
     double mass, weight;
     double *x, *y, *vx, *vy, *vz;
+    // the slot holding caller index i (the particles are regrouped by tile
+    // every few steps; diagnostics and downloads run in the caller's order)
+    uint32_t* slot;
+};
+
+// Regrouping of held particles by kRegroupTile23 x kRegroupTile23-cell tile.
+constexpr int kRegroupTile23 = 8;
+constexpr int kRegroupEvery23 = 16;  // steps between regroupings
+struct Tiles23 {
+    int32_t TX, TY, ntx, nty, ntiles;
+    int32_t *count, *start, *cursor;
 };
 
 // Diagnostics and binned-field state.
@@ -59,6 +70,13 @@ void launch23_tick(int64_t* t, hipStream_t st);
 // xyv (5 x P, column-major) <-> SoA
 void launch23_unpack(const double* xyv, const Species23& s, hipStream_t st);
 void launch23_pack(const Species23& s, double* xyv, hipStream_t st);
+// regrouping: the held particles of s by tile into o (and pid into pido,
+// s.slot updated); the held particles back in the caller's order (into o);
+// pid = slot = 0 .. P-1
+void launch23_regroup(const Grid23& g, const Tiles23& T, const Species23& s, const Species23& o, const uint32_t* pid,
+                      uint32_t* pido, hipStream_t st);
+void launch23_to_caller(const Species23& s, const Species23& o, hipStream_t st);
+void launch23_identity(uint32_t* pid, uint32_t* slot, int64_t P, hipStream_t st);
 // operators
 void launch23_deposit_op(const Grid23& g, int shape, const double* x, const double* y, int64_t n, double w,
                          unsigned long long* racc, hipStream_t st);

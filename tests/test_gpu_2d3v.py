@@ -159,6 +159,26 @@ def test_sort_bitwise(pg, orc):
     assert same(got, ref)
 
 
+def test_sort_after_steps_follows_the_caller_order(pg, orc):
+    """The context regroups its particles by tile while it steps; sort!
+    afterwards permutes the caller's order (the downloaded one), and a second
+    run continues from it."""
+    rng = np.random.default_rng(16)
+    pl = plasma(rng, 20000, 2.0, 0.75)
+    with pg.Context2D3V(config(pl)) as c:
+        for s, (_, xyv) in enumerate(pl):
+            c.upload(s, xyv)
+        c.run(3)
+        before = c.download(0)
+        perm = c.sort(0, 2.0 / 32, 0.75 / 16)
+        got = c.download(0)
+        ref, operm = orc.sort23(before, 2.0 / 32, 0.75 / 16)
+        assert np.array_equal(perm, operm) and same(got, ref)
+        r = c.run(2)
+        assert np.all(np.isfinite(r))
+        assert same(c.download(0)[2:], c.download(0)[2:])
+
+
 def test_upload_download_roundtrip_and_reset(pg, orc):
     rng = np.random.default_rng(7)
     pl = plasma(rng, 1000, 2.0, 0.75)
