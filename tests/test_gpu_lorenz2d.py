@@ -4,7 +4,8 @@ through the C ABI in include/pic2d3v.h: warmup!, steps of loop! with and
 without diagnose! (rows, the 18 state arrays, the particles and the fourteen
 binned fields) bit for bit against the CPU restatement, for every shape and
 for explicit, semi-implicit and implicit theta; a step from an arbitrary
-complex state; a 128 x 128 run with 2 x 2^18 particles; LorenzGaugeField's
+complex state; 20 steps across two regroupings of the held particles; a
+128 x 128 run with 2 x 2^18 particles; LorenzGaugeField's
 centred loop! (every shape tested, theta 0 and 1, from a fresh and from an
 arbitrary state); and, at 512 x 512
 with 2 x 2^22 particles, the size-independent properties that the fields do
@@ -114,6 +115,22 @@ def test_step_from_an_arbitrary_state(pg, orc):
         check_state(c, sp1, F1)
         assert same(rows, rows1) and rows.shape == (2, 11)
         assert same(c.lg_bins(), bins1)
+
+
+def test_regroupings_over_many_steps_bitwise(pg, orc):
+    """The context regroups its particles by tile before the first step and
+    every 16 steps; 20 diagnosed steps (two regroupings) match the oracle,
+    which keeps the caller's order, in the rows, state, particles and bins."""
+    NX, NY, Lx, Ly, dt = 32, 32, 1.0, 1.0, 0.02
+    B0 = (0.5, 0.0, 0.3)
+    sp = make_species(2048, Lx, Ly, seed=9)
+    with context(pg, NX, NY, Lx, Ly, dt, B0, sp, buffer=3) as c:
+        c.warmup()
+        rows = c.run(20)
+        sp1, F1 = orc.lg_warmup(NX, NY, Lx, Ly, dt, sp)
+        sp2, F2, rows2, bins2 = orc.lg_run(NX, NY, Lx, Ly, dt, np.array(B0), sp1, F1, 20, buffer=3)
+        check_state(c, sp2, F2)
+        assert same(rows, rows2) and same(c.lg_bins(), bins2)
 
 
 def test_mid_size_bitwise(pg, orc):
