@@ -199,7 +199,11 @@ int pic2d3v_reset(pic2d3v_ctx* ctx);
  * sortperm).  perm (may be NULL) receives the 0-based permutation: new
  * particle k was particle perm[k] (the reference's species.p minus 1).
  * Fails with PIC_EINVAL if the cells span more than 2^24 or one cell holds
- * more than 16384 particles. */
+ * more than 16384 particles.  The context itself keeps its particles grouped
+ * by 8 x 8-cell tile for the LDS windows of its deposits (before the first
+ * step after an upload and every 16 steps); that order is internal:
+ * pic2d3v_download, the diagnostics and this sort see the caller's order,
+ * and no result depends on it. */
 int pic2d3v_sort(pic2d3v_ctx* ctx, int32_t species, double dx, double dy, int64_t* perm);
 
 /* nsteps of `loop!(plasma, field); diagnose!(d, field, plasma, t)` from the
