@@ -70,6 +70,26 @@ int64_t orc_jl_mod1i(int64_t i, int64_t n) {
 double orc_jl_round(double x) { return rint(x); }
 
 /* ------------------------------------------------------------------------ */
+/* The exp and erf below restate FreeBSD msun e_exp.c and s_erf.c, whose      */
+/* copyright notices are kept here as their licence asks:                    */
+/*                                                                           */
+/* ====================================================                      */
+/* Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.         */
+/*                                                                           */
+/* Developed at SunPro, a Sun Microsystems, Inc. business.                   */
+/* Permission to use, copy, modify, and distribute this                      */
+/* software is freely granted, provided that this notice                     */
+/* is preserved.                                                             */
+/* ====================================================                      */
+/*                                                                           */
+/* ====================================================                      */
+/* Copyright (C) 2004 by Sun Microsystems, Inc. All rights reserved.         */
+/*                                                                           */
+/* Permission to use, copy, modify, and distribute this                      */
+/* software is freely granted, provided that this notice                     */
+/* is preserved.                                                             */
+/* ====================================================                      */
+/* ------------------------------------------------------------------------ */
 /* exp: restatement of FreeBSD msun e_exp.c (the __ieee754_exp that OpenLibm  */
 /* 0.8.1 ships and its erf calls).  Argument reduction x = k ln2 + r with     */
 /* ln2 split hi/lo, a degree-5 Remez polynomial in r^2, reconstruction by     */

@@ -595,6 +595,9 @@ static int regroup(pic2d3v_ctx* c) {
         c->held_identity[s] = false;
     }
     HIPCHK(c, hipGetLastError());
+    // a graph launched before the regroup may still run on the stream: wait
+    // for it before its executable is destroyed
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     free_graphs(c);
     c->since_regroup = 0;
     return PIC_OK;
@@ -614,6 +617,7 @@ static int to_caller(pic2d3v_ctx* c, int32_t s) {
     launch23_identity(c->pid[s], sp.slot, sp.P, c->stream);
     HIPCHK(c, hipGetLastError());
     c->held_identity[s] = true;
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // no graph in flight when it is destroyed
     free_graphs(c);
     return PIC_OK;
 }

@@ -23,6 +23,25 @@ __device__ __forceinline__ double from_hiword(uint32_t hi) {
     return __longlong_as_double((long long)((u64)hi << 32));
 }
 
+// The exp and erf below restate FreeBSD msun e_exp.c and s_erf.c, whose
+// copyright notices are kept here as their licence asks:
+//
+// ====================================================
+// Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+//
+// Developed at SunPro, a Sun Microsystems, Inc. business.
+// Permission to use, copy, modify, and distribute this
+// software is freely granted, provided that this notice
+// is preserved.
+// ====================================================
+//
+// ====================================================
+// Copyright (C) 2004 by Sun Microsystems, Inc. All rights reserved.
+//
+// Permission to use, copy, modify, and distribute this
+// software is freely granted, provided that this notice
+// is preserved.
+// ====================================================
 // ---------------------------------------------------------------------------
 // exp and erf: FreeBSD msun e_exp.c / s_erf.c as shipped by OpenLibm 0.8.1,
 // the erf SpecialFunctions 2.2.0 calls in the Gaussian shape
