@@ -36,7 +36,7 @@ def parse():
                     help="skip the ensemble and 2D3V workloads reported beside the headline")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--workload", default="scaled",
-                    choices=["scaled", "ensemble", "es2d3v", "pic2d3v", "lorenz2d3v", "lorenz2d3v_si", "em1d",
+                    choices=["scaled", "small", "ensemble", "es2d3v", "pic2d3v", "lorenz2d3v", "lorenz2d3v_si", "em1d",
                              "em1d_explicit"],
                     help="scaled: the headline line (default); ensemble / es2d3v: only that workload, "
                          "--steps timed steps, one JSON line (for profiling)")
@@ -286,7 +286,7 @@ def bench_pic2d3v(steps=20, warmup=2, N=1024, ppc=16, shape="bspline2"):
     # the CPU restatement on a sample: the same grid, 1/32 of each species (sorted), two steps
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_cpu()[2]
     n = P // 32
     sample = []
     for sp in sps:
@@ -460,28 +460,97 @@ def bench_em1d(T=4048, scaled_steps=20, scheme="fixed_point"):
                              "sample": "FP64 CPU restatement, the script's set-up, 40 steps, one thread"}}
 
 
-def cpu_baseline(cfg, x, v, budget_s=15.0):
+def host_cpu():
+    """(nproc, CPU model name, threads the CPU baselines use).  The threads are
+    the job's CPU share: OMP_NUM_THREADS where the host sets it (16 per GPU on
+    the MI355X boxes), else every core nproc reports."""
+    nproc = os.cpu_count() or 1
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True).stdout.strip() or nproc)
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    share = os.environ.get("OMP_NUM_THREADS")
+    threads = int(share) if share and share.isdigit() and int(share) > 0 else nproc
+    return nproc, model, max(1, min(threads, nproc))
+
+
+def cpu_baseline(cfg, x, v, budget_s=12.0):
     """The CPU restatement (FP64) timed on this host on a bounded sample: the
-    same N and stencil with the first n particles, steps until ~budget_s."""
+    same N and stencil with the first n particles, steps until ~budget_s, on
+    the job's CPU share (OpenMP) and on one thread (the scripts' own loop is
+    single-threaded)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
+    nproc, model, threads = host_cpu()
     n = 2 ** 14
-    xs, vs = x[:n].copy(), v[:n].copy()
     w = cfg.W / n * cfg.N
-    t0 = time.perf_counter()
-    steps = 0
-    E = None
-    while True:
-        xs, vs, E, _, _ = oracle.run(cfg.N, 7, 1, cfg.W, w, cfg.dt, xs, vs, 1, rtol=cfg.rtol, E=E,
-                                     nthreads=threads)
-        steps += 1
-        if time.perf_counter() - t0 > budget_s or steps >= 200:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": n * steps / dt, "unit": "particle-pushes/s", "cores": threads, "kind": "port",
+
+    def timed(nthreads, budget, max_steps):
+        xs, vs = x[:n].copy(), v[:n].copy()
+        t0 = time.perf_counter()
+        steps = 0
+        E = None
+        while True:
+            xs, vs, E, _, _ = oracle.run(cfg.N, 7, 1, cfg.W, w, cfg.dt, xs, vs, 1, rtol=cfg.rtol, E=E,
+                                         nthreads=nthreads)
+            steps += 1
+            if time.perf_counter() - t0 > budget or steps >= max_steps:
+                break
+        return n * steps / (time.perf_counter() - t0), steps
+
+    value, steps = timed(threads, budget_s, 200)
+    single, steps1 = timed(1, budget_s / 3, 50)
+    return {"value": value, "unit": "particle-pushes/s", "cores": threads, "kind": "port",
             "sample": f"FP64 CPU restatement, N=4096, first {n} particles of the workload (w rescaled), "
-                      f"{steps} steps, OpenMP {threads} threads"}
+                      f"{steps} steps, OpenMP {threads} threads (the job's CPU share; nproc {nproc})",
+            "nproc": nproc, "cpu_model": model,
+            "single_thread": {"value": single, "steps": steps1, "cores": 1}}
+
+
+def bench_small():
+    """The two small workloads whole, through both run paths: the NGP leapfrog
+    two-stream run (src/NGPFourier.jl: N=128, P=8192, 1024 steps) and the quiet
+    Gaussian fixed-point run (src/GaussianFixedPointQuiet.jl: N=64, P=2048,
+    8192 steps, rtol=4eps), FP64; the CPU restatement on one thread beside
+    them (the scripts are single-threaded), on the whole NGP run and on the
+    first 512 steps of the quiet run."""
+    import picgolf
+    from picgolf import configs
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    out = {}
+    for name, fn, shape, scheme, cpu_steps in (("ngp_two_stream", configs.ngp_two_stream, 0, 0, 1024),
+                                               ("quiet_two_stream", configs.quiet_two_stream, 7, 1, 512)):
+        cfg, x, v, T = fn()
+        rec = {"N": cfg.N, "P": cfg.P, "steps": T}
+        for path in ("graph", "persistent"):
+            with picgolf.Context(cfg) as c:
+                c.set_run_path(path)
+                c.upload(x, v)
+                c.run(min(T, 64))                 # graphs captured, code loaded
+                c.upload(x, v)
+                t0 = time.perf_counter()
+                D, S = c.run(T)
+                secs = time.perf_counter() - t0
+            rec[path] = {"us_per_step": secs / T * 1e6, "pushes_per_s": cfg.P * T / secs,
+                         "mean_sweeps": float(S.mean())}
+        rec["persistent_over_graph"] = rec["graph"]["us_per_step"] / rec["persistent"]["us_per_step"]
+        t0 = time.perf_counter()
+        oracle.run(cfg.N, shape, scheme, cfg.W, cfg.w, cfg.dt, x, v, cpu_steps, rtol=cfg.rtol, nthreads=1)
+        cpu = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": cfg.P * cpu_steps / cpu, "unit": "particle-pushes/s", "cores": 1,
+                               "kind": "port", "sample": f"FP64 CPU restatement, the same run, first {cpu_steps} "
+                                                         f"steps, one thread"}
+        out[name] = rec
+    return out
 
 
 def main():
@@ -501,6 +570,9 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU sharding is a design "
               f"note, not built", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "small":
+        print(json.dumps(bench_small()))
+        return
     if a.workload == "ensemble":
         print(json.dumps(bench_ensemble(steps=a.steps, warmup=a.warmup, autotune=not a.no_autotune)))
         return
@@ -573,7 +645,8 @@ def main():
         out["field_energy_rel_err_mixed_vs_fp64"] = float(np.max(np.abs(d1m - d1f) / np.abs(d1f)))
         out["momentum_drift_mixed"] = float(np.max(np.abs(m["D"][:, 3] - m["D"][0, 3])))
     if not a.no_extra:
-        out["workloads"] = {"ensemble_512": bench_ensemble(autotune=not a.no_autotune), "es2d3v_256": bench_2d(),
+        out["workloads"] = {"small": bench_small(),
+                            "ensemble_512": bench_ensemble(autotune=not a.no_autotune), "es2d3v_256": bench_2d(),
                             "pic2d3v_1024": bench_pic2d3v(), "lorenz2d3v_1024": bench_lorenz2d3v(),
                             "lorenz2d3v_si_1024": bench_lorenz2d3v(field="lorenz_semi_implicit"),
                             "em1d": bench_em1d(),

@@ -104,6 +104,27 @@ int pic_run(pic_ctx* ctx, int64_t nsteps, double* diag, int32_t* sweeps);
 int pic_run_profiled(pic_ctx* ctx, int64_t nsteps, double* diag, int32_t* sweeps,
                      pic_kernel_stat* stats, int32_t* n_stats);
 
+/* ---- run path ---------------------------------------------------------- */
+/* How pic_run advances the steps.  The reference has no counterpart: its
+ * step loop is the script's own `for t in 1:T` (src/NGPFourier.jl:4,
+ * src/GaussianFixedPointQuiet.jl:7).  Both paths give the same bits.
+ *  GRAPH       captured step graphs: one launch per kernel per sweep slot
+ *  PERSISTENT  one launch for the whole call: the step loop, the Picard
+ *              sweeps and their test stay on the device.  Single FP64
+ *              instances only: NGP leapfrog with N <= 512 and P <= 8192, or
+ *              the Gaussian shapes (leapfrog or midpoint) with N <= 512 and
+ *              P <= 16384
+ *  AUTO        PERSISTENT where it applies, else GRAPH (the default)
+ * workgroups: the Gaussian persistent run's particle workgroups (1..128,
+ * each 32 particles per slot, at most 4 slots), 0 for the default. */
+#define PIC_RUN_AUTO 0
+#define PIC_RUN_GRAPH 1
+#define PIC_RUN_PERSISTENT 2
+int pic_set_run_path(pic_ctx* ctx, int32_t path, int32_t workgroups);
+/* The path the next pic_run takes (GRAPH or PERSISTENT) and its workgroups
+ * (0 for GRAPH). */
+int pic_get_run_path(const pic_ctx* ctx, int32_t* path, int32_t* workgroups);
+
 /* ---- particle tiling of the push launches ------------------------------ */
 /* The reference has no counterpart: its particle loops are serial
  * (src/GaussianFixedPointQuiet.jl:10).  A push launch runs one workgroup of

@@ -6,6 +6,7 @@ The library lands in particleincellcodegolf.jl_amd/lib/.  All device code is
 compiled with -ffp-contract=off (the FP64 path must round exactly as the CPU
 restatement does) and with packed FP32 instructions disabled.
 """
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -17,7 +18,7 @@ LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libpicgolf.so")
 ARCH = os.environ.get("PIC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels_1d.hip", "kernels_mixed.hip", "kernels_2d.hip", "kernels_1d2v.hip", "kernels_2d3v.hip",
+SOURCES = ["kernels_1d.hip", "kernels_mixed.hip", "kernels_small.hip", "kernels_2d.hip", "kernels_1d2v.hip", "kernels_2d3v.hip",
            "kernels_2d3v_lg.hip", "kernels_em1d.hip", "capi.cpp", "capi2d.cpp", "capi1d2v.cpp", "capi2d3v.cpp",
            "capi_em1d.cpp"]
 
@@ -34,16 +35,24 @@ def build(verbose=False):
     objs = []
     common = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall",
               "-Wno-unused-function", "-I", os.path.join(HERE, "..", "include")]
+    cmds = []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(LIB_DIR, os.path.splitext(src)[0] + ".o")
         cmd = [hipcc(), *common, "-c", path, "-o", obj]
         # hipcc compiles .cpp as HIP too: every unit gets the same target flags
         cmd[1:1] = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-        run(cmd, verbose)
+        cmds.append(cmd)
         objs.append(obj)
+    # the units are independent: compile them side by side
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
+        for f in [ex.submit(run, cmd, verbose) for cmd in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
-    run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs], verbose)
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, f"-L{rocm_lib}",
+         "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{rocm_lib}"], verbose)
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)

@@ -13,6 +13,7 @@
 
 #include "../../include/pic.h"
 #include "pic_internal.h"
+#include "trace.h"
 
 using namespace pic;
 
@@ -52,6 +53,12 @@ struct pic_ctx {
     bool primed = false;       // the accumulator holds the next step's first charge
     int sweeps_seen = 0;       // most sweeps any step has needed (0: none run yet)
     int64_t steps_done = 0;
+    // persistent runs of a small single instance (pic_set_run_path)
+    int run_path = PIC_RUN_AUTO;
+    bool small_ok = false;         // the instance fits the persistent kernels
+    int64_t small_K = 0;           // their particle workgroups (Gaussian)
+    unsigned long long* small_gacc = nullptr;
+    uint32_t* small_bar = nullptr;
     // particle tiling of the push launches (pic_set_tiling / pic_autotune)
     int64_t tileP = 0;
     DevBuf tiles_buf, dpart_buf, snap_buf;
@@ -304,6 +311,9 @@ static int build_tables(pic_ctx* c) {
         d.W = k.W;
         d.rtol = k.rtol;
         d.wq = k.w * 0x1p-30;
+        d.vstep = k.dt * 4294967296.0;
+        d.inv_vstep = 1.0 / d.vstep;
+        d.escale2 = (k.dt * k.dt) * 4.0;
         d.red0 = red0;
         red0 += k.N;
         d.chunk0 = (int32_t)c->hchunks.size();
@@ -352,6 +362,8 @@ static int build_tables(pic_ctx* c) {
         }
         if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.racc), 3 * (size_t)red0))) return rc;
         HIPCHK(c, hipMemset(b.racc, 0, 3 * 8 * (size_t)red0));
+        b.racc_cells = 3 * red0;
+        b.prime_next = c->scheme == PIC_SCHEME_MIDPOINT ? 1 : 0;
     } else {
         for (int j = 0; j < 2; ++j) {
             if ((rc = dalloc(c, &b.Xq[j], P))) return rc;
@@ -394,6 +406,7 @@ static int build_tables(pic_ctx* c) {
 }
 
 int pic_create_batch(pic_ctx** out, const pic_config* cfgs, int32_t n) {
+    PIC_TRACE("pic_create_batch");
     if (!out || !cfgs || n < 1) return PIC_EINVAL;
     *out = nullptr;
     pic_ctx* c = new pic_ctx();
@@ -438,13 +451,22 @@ int pic_create_batch(pic_ctx** out, const pic_config* cfgs, int32_t n) {
     if (hipMemcpy(c->tw, twp.data(), sizeof(double) * twp.size(), hipMemcpyHostToDevice) != hipSuccess)
         return fail(PIC_EHIP);
     if ((rc = build_tables(c))) return fail(rc);
+    if (c->ninst == 1 && small_eligible(c->hinst[0], c->scheme, c->precision, &c->small_K)) {
+        c->small_ok = true;
+        if ((rc = dalloc(c, &c->small_gacc, 9 * (size_t)c->hinst[0].N))) return fail(rc);
+        if ((rc = dalloc(c, &c->small_bar, 4))) return fail(rc);
+    }
     *out = c;
     return PIC_OK;
 }
 
-int pic_create(pic_ctx** out, const pic_config* cfg) { return pic_create_batch(out, cfg, 1); }
+int pic_create(pic_ctx** out, const pic_config* cfg) {
+    PIC_TRACE("pic_create");
+    return pic_create_batch(out, cfg, 1);
+}
 
 int pic_destroy(pic_ctx* c) {
+    PIC_TRACE("pic_destroy");
     if (!c) return PIC_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -476,6 +498,7 @@ static int write_states(pic_ctx* c, const std::vector<InstState>& st) {
 }
 
 int pic_upload(pic_ctx* c, int32_t inst, const double* x, const double* v, const double* E) {
+    PIC_TRACE("pic_upload");
     if (!c) return PIC_EINVAL;
     if (inst < 0 || inst >= c->ninst || !x || !v) {
         set_err(c, "pic_upload: bad instance or null array");
@@ -495,8 +518,9 @@ int pic_upload(pic_ctx* c, int32_t inst, const double* x, const double* v, const
             double f = x[j] - std::floor(x[j]);
             double s = std::nearbyint(f * 4294967296.0);
             xq[j] = (uint32_t)(uint64_t)(int64_t)s;  // 2^32 wraps to 0
-            vh[j] = (float)v[j];
-            vl[j] = (float)(v[j] - (double)vh[j]);
+            const double vs = v[j] * d.vstep;  // steps of the fixed-point position
+            vh[j] = (float)vs;
+            vl[j] = (float)(vs - (double)vh[j]);
         }
         HIPCHK(c, hipMemcpyAsync(c->rb.Xq[par] + d.poff, xq.data(), P * 4, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->rb.Vh[par] + d.poff, vh.data(), P * 4, hipMemcpyHostToDevice, c->stream));
@@ -512,7 +536,7 @@ int pic_upload(pic_ctx* c, int32_t inst, const double* x, const double* v, const
                                      c->stream));
     }
     std::vector<float> e32((size_t)d.N);
-    for (int i = 0; i < d.N; ++i) e32[i] = (float)e[i];
+    for (int i = 0; i < d.N; ++i) e32[i] = (float)(e[i] * d.escale2);  // the mixed gather's units
     HIPCHK(c, hipMemcpyAsync(c->rb.E[0] + d.goff, e.data(), sizeof(double) * d.N, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->rb.E32 + d.goff, e32.data(), sizeof(float) * d.N, hipMemcpyHostToDevice, c->stream));
     std::vector<InstState> st;
@@ -525,6 +549,7 @@ int pic_upload(pic_ctx* c, int32_t inst, const double* x, const double* v, const
 }
 
 int pic_download(pic_ctx* c, int32_t inst, double* x, double* v, double* E) {
+    PIC_TRACE("pic_download");
     if (!c) return PIC_EINVAL;
     if (inst < 0 || inst >= c->ninst) {
         set_err(c, "pic_download: bad instance");
@@ -547,7 +572,7 @@ int pic_download(pic_ctx* c, int32_t inst, double* x, double* v, double* E) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         for (size_t j = 0; j < P; ++j) {
             if (x) x[j] = (double)xq[j] * 0x1p-32;
-            if (v) v[j] = (double)vh[j] + (double)vl[j];
+            if (v) v[j] = ((double)vh[j] + (double)vl[j]) * d.inv_vstep;
         }
     }
     if (E) {
@@ -708,7 +733,84 @@ static int note_sweeps(pic_ctx* c, int64_t row0, int64_t row1) {
     return PIC_OK;
 }
 
+static bool use_small(const pic_ctx* c) { return c->small_ok && c->run_path != PIC_RUN_GRAPH; }
+
+// The whole call in one persistent launch (kernels_small.hip).
+static int run_small(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps) {
+    if (nsteps > INT32_MAX - 1) {
+        set_err(c, "pic_run: nsteps too large for one persistent run");
+        return PIC_EINVAL;
+    }
+    int rc = begin_run(c, nsteps);
+    if (rc) return rc;
+    const RunBuffers& b = c->rb;
+    SmallRun a{};
+    a.I = c->hinst[0];
+    a.st = b.st;
+    a.X0 = b.X[0];
+    a.X1 = b.X[1];
+    a.V0 = b.V[0];
+    a.V1 = b.V[1];
+    a.E0 = b.E[0];
+    a.E1 = b.E[1];
+    a.tw = reinterpret_cast<const double2*>(c->tw);
+    a.diag = b.diag;
+    a.sweeps = b.sweeps;
+    a.nsteps = nsteps;
+    a.gacc = c->small_gacc;
+    a.bar = c->small_bar;
+    a.K = (int32_t)c->small_K;
+    a.spin_limit = 1u << 22;
+    HIPCHK(c, hipMemsetAsync(c->small_gacc, 0, 9 * sizeof(unsigned long long) * (size_t)a.I.N, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->small_bar, 0, 4 * sizeof(uint32_t), c->stream));
+    HIPCHK(c, launch_small_run(a, c->scheme, c->stream));
+    uint32_t bar[4] = {0, 0, 0, 0};
+    HIPCHK(c, hipMemcpyAsync(bar, c->small_bar, sizeof bar, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // the run moved the state past any charge a graph step primed
+    c->primed = false;
+    if (bar[2]) {
+        set_err(c, "pic_run: a grid barrier of the persistent run timed out (run state undefined)");
+        return PIC_EHIP;
+    }
+    if ((rc = note_sweeps(c, 0, nsteps))) return rc;
+    return finish_run(c, nsteps, diag, sweeps);
+}
+
+int pic_set_run_path(pic_ctx* c, int32_t path, int32_t workgroups) {
+    PIC_TRACE("pic_set_run_path");
+    if (!c) return PIC_EINVAL;
+    if (path != PIC_RUN_AUTO && path != PIC_RUN_GRAPH && path != PIC_RUN_PERSISTENT) {
+        set_err(c, "pic_set_run_path: unknown path %d", (int)path);
+        return PIC_EINVAL;
+    }
+    int64_t K = 0;
+    const bool fits = c->ninst == 1 && small_eligible(c->hinst[0], c->scheme, c->precision, &K);
+    if (path == PIC_RUN_PERSISTENT && !fits) {
+        set_err(c, "pic_set_run_path: the persistent run takes one FP64 instance with N <= 512 and NGP "
+                   "leapfrog (P <= 8192) or a Gaussian shape (P <= 16384)");
+        return PIC_EINVAL;
+    }
+    if (workgroups != 0 && (!fits || !small_workgroups_ok(c->hinst[0], workgroups))) {
+        set_err(c, "pic_set_run_path: %d workgroups do not fit this instance", (int)workgroups);
+        return PIC_EINVAL;
+    }
+    c->run_path = path;
+    if (fits) c->small_K = workgroups ? workgroups : K;
+    return PIC_OK;
+}
+
+int pic_get_run_path(const pic_ctx* c, int32_t* path, int32_t* workgroups) {
+    PIC_TRACE("pic_get_run_path");
+    if (!c) return PIC_EINVAL;
+    const bool small = use_small(c);
+    if (path) *path = small ? PIC_RUN_PERSISTENT : PIC_RUN_GRAPH;
+    if (workgroups) *workgroups = small ? (int32_t)c->small_K : 0;
+    return PIC_OK;
+}
+
 int pic_run(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps) {
+    PIC_TRACE("pic_run");
     if (!c) return PIC_EINVAL;
     if (nsteps < 0) {
         set_err(c, "pic_run: nsteps < 0");
@@ -716,6 +818,7 @@ int pic_run(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps) {
     }
     if (nsteps == 0) return PIC_OK;
     HIPCHK(c, hipSetDevice(c->device));
+    if (use_small(c)) return run_small(c, nsteps, diag, sweeps);
     int rc = begin_run(c, nsteps);
     if (rc) return rc;
     if ((rc = prime(c))) return rc;
@@ -765,6 +868,7 @@ int pic_run(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps) {
 
 int pic_run_profiled(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps, pic_kernel_stat* stats,
                      int32_t* n_stats) {
+    PIC_TRACE("pic_run_profiled");
     if (!c) return PIC_EINVAL;
     if (nsteps <= 0) {
         set_err(c, "pic_run_profiled: nsteps must be > 0");
@@ -892,6 +996,7 @@ static int copy_run_state(pic_ctx* c, bool save) {
 }
 
 int pic_set_tiling(pic_ctx* c, int64_t tile_particles, int32_t block) {
+    PIC_TRACE("pic_set_tiling");
     if (!c) return PIC_EINVAL;
     if (tile_particles < 1 || block < 64 || block > 1024 || (block & (block - 1)) != 0) {
         set_err(c, "pic_set_tiling: tile_particles must be >= 1 and block a power of two in [64, 1024]");
@@ -903,6 +1008,7 @@ int pic_set_tiling(pic_ctx* c, int64_t tile_particles, int32_t block) {
 }
 
 int pic_get_tiling(const pic_ctx* c, int64_t* tile_particles, int32_t* block, int32_t* n_tiles) {
+    PIC_TRACE("pic_get_tiling");
     if (!c) return PIC_EINVAL;
     if (tile_particles) *tile_particles = c->tileP;
     if (block) *block = c->rb.block_push;
@@ -939,6 +1045,7 @@ static int time_trial(pic_ctx* c, int steps, double* ms_per_step) {
 }
 
 int pic_autotune(pic_ctx* c, int32_t trial_steps, pic_tuning* out) {
+    PIC_TRACE("pic_autotune");
     if (!c) return PIC_EINVAL;
     if (trial_steps < 1 || trial_steps > 64) {
         set_err(c, "pic_autotune: trial_steps must be in [1, 64]");
@@ -1041,6 +1148,7 @@ static int check_shape(pic_ctx* c, int32_t shape, bool allow_ngp, bool allow_are
 
 int pic_deposit(pic_ctx* c, const double* x, const double* y, int64_t P, double w, int32_t shape,
                 double* rho) {
+    PIC_TRACE("pic_deposit");
     if (!c) return PIC_EINVAL;
     if (!x || !rho || P < 1 || !(w >= 0.0) || !std::isfinite(w)) {
         set_err(c, "pic_deposit: null array, P < 1 or bad w");
@@ -1061,6 +1169,7 @@ int pic_deposit(pic_ctx* c, const double* x, const double* y, int64_t P, double 
 }
 
 int pic_fieldsolve(pic_ctx* c, const double* rho, double* E) {
+    PIC_TRACE("pic_fieldsolve");
     if (!c) return PIC_EINVAL;
     if (!rho || !E) {
         set_err(c, "pic_fieldsolve: null array");
@@ -1077,6 +1186,7 @@ int pic_fieldsolve(pic_ctx* c, const double* rho, double* E) {
 }
 
 int pic_drift_half(pic_ctx* c, double* x, const double* v, int64_t P, double dt) {
+    PIC_TRACE("pic_drift_half");
     if (!c) return PIC_EINVAL;
     if (!x || !v || P < 1) {
         set_err(c, "pic_drift_half: null array or P < 1");
@@ -1092,6 +1202,7 @@ int pic_drift_half(pic_ctx* c, double* x, const double* v, int64_t P, double dt)
 
 int pic_push_leapfrog(pic_ctx* c, double* x, double* v, const double* E, int64_t P, double dt,
                       int32_t shape) {
+    PIC_TRACE("pic_push_leapfrog");
     if (!c) return PIC_EINVAL;
     if (!x || !v || !E || P < 1) {
         set_err(c, "pic_push_leapfrog: null array or P < 1");
@@ -1113,6 +1224,7 @@ int pic_push_leapfrog(pic_ctx* c, double* x, double* v, const double* E, int64_t
 
 int pic_midpoint_position(pic_ctx* c, const double* X, const double* V, const double* v, double* x,
                           int64_t P, double dt) {
+    PIC_TRACE("pic_midpoint_position");
     if (!c) return PIC_EINVAL;
     if (!X || !V || !v || !x || P < 1) {
         set_err(c, "pic_midpoint_position: null array or P < 1");
@@ -1131,6 +1243,7 @@ int pic_midpoint_position(pic_ctx* c, const double* X, const double* V, const do
 
 int pic_push_midpoint(pic_ctx* c, const double* X, const double* V, const double* x, double* v,
                       const double* E, int64_t P, double dt, int32_t shape) {
+    PIC_TRACE("pic_push_midpoint");
     if (!c) return PIC_EINVAL;
     if (!X || !V || !x || !v || !E || P < 1) {
         set_err(c, "pic_push_midpoint: null array or P < 1");
@@ -1153,6 +1266,7 @@ int pic_push_midpoint(pic_ctx* c, const double* X, const double* V, const double
 
 int pic_simpson_kick(pic_ctx* c, const double* a, const double* b, const double* v_in, double* v_out,
                      const double* E, int64_t P, double coef, int32_t shape) {
+    PIC_TRACE("pic_simpson_kick");
     if (!c) return PIC_EINVAL;
     if (!a || !b || !v_in || !v_out || !E || P < 1) {
         set_err(c, "pic_simpson_kick: null array or P < 1");
@@ -1174,6 +1288,7 @@ int pic_simpson_kick(pic_ctx* c, const double* a, const double* b, const double*
 }
 
 int pic_wrap(pic_ctx* c, double* x, int64_t P) {
+    PIC_TRACE("pic_wrap");
     if (!c) return PIC_EINVAL;
     if (!x || P < 1) {
         set_err(c, "pic_wrap: null array or P < 1");
@@ -1188,6 +1303,7 @@ int pic_wrap(pic_ctx* c, double* x, int64_t P) {
 
 int pic_isapprox(pic_ctx* c, const double* F, const double* E, int32_t n, double rtol, double atol,
                  int32_t* result) {
+    PIC_TRACE("pic_isapprox");
     if (!c) return PIC_EINVAL;
     if (!F || !E || !result || n < 1 || n > kTreeChunk) {
         set_err(c, "pic_isapprox: null array or n outside [1, 4096]");
@@ -1206,6 +1322,7 @@ int pic_isapprox(pic_ctx* c, const double* F, const double* E, int32_t n, double
 }
 
 int pic_diagnostics(pic_ctx* c, const double* E, const double* v, int64_t P, double W, double* out4) {
+    PIC_TRACE("pic_diagnostics");
     if (!c) return PIC_EINVAL;
     if (!E || !v || !out4 || P < 1) {
         set_err(c, "pic_diagnostics: null array or P < 1");

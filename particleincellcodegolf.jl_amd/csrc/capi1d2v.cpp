@@ -13,6 +13,7 @@
 
 #include "../../include/pic1d2v.h"
 #include "pic1d2v_internal.h"
+#include "trace.h"
 
 using namespace pic;
 
@@ -118,6 +119,7 @@ extern "C" {
 const char* pic1d2v_last_error(const pic1d2v_ctx* c) { return c ? c->err.c_str() : g12_create_err.c_str(); }
 
 int pic1d2v_destroy(pic1d2v_ctx* c) {
+    PIC_TRACE("pic1d2v_destroy");
     if (!c) return PIC_OK;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -133,6 +135,7 @@ int pic1d2v_destroy(pic1d2v_ctx* c) {
 }
 
 int pic1d2v_create(pic1d2v_ctx** out, const pic1d2v_config* cfg) {
+    PIC_TRACE("pic1d2v_create");
     if (!out || !cfg) {
         set_err(nullptr, "pic1d2v_create: null argument");
         return PIC_EINVAL;
@@ -232,6 +235,7 @@ int pic1d2v_create(pic1d2v_ctx** out, const pic1d2v_config* cfg) {
 }
 
 int pic1d2v_upload(pic1d2v_ctx* c, int32_t species, const double* x, const double* vx, const double* vy) {
+    PIC_TRACE("pic1d2v_upload");
     if (!c) return PIC_EINVAL;
     if (species < 0 || species >= c->cfg.nspecies || !x || !vx || !vy) {
         set_err(c, "pic1d2v_upload: bad species or null array");
@@ -255,6 +259,7 @@ int pic1d2v_upload(pic1d2v_ctx* c, int32_t species, const double* x, const doubl
 }
 
 int pic1d2v_download(pic1d2v_ctx* c, int32_t species, double* x, double* vx, double* vy) {
+    PIC_TRACE("pic1d2v_download");
     if (!c) return PIC_EINVAL;
     if (species < 0 || species >= c->cfg.nspecies) {
         set_err(c, "pic1d2v_download: bad species");
@@ -316,6 +321,7 @@ static int capture(pic1d2v_ctx* c, int64_t steps, hipGraphExec_t* out) {
 }
 
 int pic1d2v_run(pic1d2v_ctx* c, int64_t nsteps, double* D) {
+    PIC_TRACE("pic1d2v_run");
     if (!c) return PIC_EINVAL;
     if (nsteps < 0) {
         set_err(c, "pic1d2v_run: nsteps < 0");
@@ -363,6 +369,7 @@ int pic1d2v_run(pic1d2v_ctx* c, int64_t nsteps, double* D) {
 }
 
 int pic1d2v_fields(pic1d2v_ctx* c, double* Es, int64_t nbins) {
+    PIC_TRACE("pic1d2v_fields");
     if (!c) return PIC_EINVAL;
     const int64_t have = (c->step + c->b.bin - 1) / c->b.bin;
     if (!Es || nbins < 0 || nbins > have) {
@@ -376,6 +383,7 @@ int pic1d2v_fields(pic1d2v_ctx* c, double* Es, int64_t nbins) {
 }
 
 int pic1d2v_deposit(pic1d2v_ctx* c, const double* x, double* rho) {
+    PIC_TRACE("pic1d2v_deposit");
     if (!c) return PIC_EINVAL;
     if (!x || !rho) {
         set_err(c, "pic1d2v_deposit: null array");
@@ -399,6 +407,7 @@ int pic1d2v_deposit(pic1d2v_ctx* c, const double* x, double* rho) {
 }
 
 int pic1d2v_gather(pic1d2v_ctx* c, const double* E, const double* x, int64_t n, double* Ej) {
+    PIC_TRACE("pic1d2v_gather");
     if (!c) return PIC_EINVAL;
     if (!E || !x || !Ej || n < 1) {
         set_err(c, "pic1d2v_gather: null array or n < 1");
@@ -420,6 +429,7 @@ int pic1d2v_gather(pic1d2v_ctx* c, const double* E, const double* x, int64_t n, 
 
 int pic1d2v_boris(pic1d2v_ctx* c, double* vx, double* vy, const double* Ej, int64_t n, double B, double dt,
                   double q_m) {
+    PIC_TRACE("pic1d2v_boris");
     if (!c) return PIC_EINVAL;
     if (!vx || !vy || !Ej || n < 1) {
         set_err(c, "pic1d2v_boris: null array or n < 1");

@@ -13,6 +13,7 @@
 
 #include "../../include/pic2d.h"
 #include "pic2d_internal.h"
+#include "trace.h"
 
 using namespace pic;
 
@@ -204,6 +205,7 @@ static bool in_unit_box(const double* x, const double* y, int64_t P, int64_t* ba
     } while (0)
 
 extern "C" int pic2d_create(pic2d_ctx** out, const pic2d_config* cfg) {
+    PIC_TRACE("pic2d_create");
     if (!out || !cfg) {
         g2_create_err = "pic2d_create: null argument";
         return PIC_EINVAL;
@@ -307,6 +309,7 @@ extern "C" int pic2d_create(pic2d_ctx** out, const pic2d_config* cfg) {
 }
 
 extern "C" int pic2d_destroy(pic2d_ctx* c) {
+    PIC_TRACE("pic2d_destroy");
     if (!c) return PIC_EINVAL;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -322,6 +325,7 @@ extern "C" const char* pic2d_last_error(const pic2d_ctx* c) {
 
 extern "C" int pic2d_upload(pic2d_ctx* c, const double* x, const double* y, const double* vx, const double* vy,
                             const double* vz, const double* Ex, const double* Ey) {
+    PIC_TRACE("pic2d_upload");
     if (!c) return PIC_EINVAL;
     if (!x || !y || !vx || !vy || !vz) {
         set_err(c, "pic2d_upload: particle arrays must not be NULL");
@@ -380,6 +384,7 @@ extern "C" int pic2d_upload(pic2d_ctx* c, const double* x, const double* y, cons
 
 extern "C" int pic2d_download(pic2d_ctx* c, double* x, double* y, double* vx, double* vy, double* vz, double* Ex,
                               double* Ey) {
+    PIC_TRACE("pic2d_download");
     if (!c) return PIC_EINVAL;
     if (!c->uploaded) {
         set_err(c, "pic2d_download: nothing uploaded");
@@ -447,6 +452,7 @@ static int capture(pic2d_ctx* c, bool diag, hipGraphExec_t* out) {
 }
 
 extern "C" int pic2d_run(pic2d_ctx* c, int64_t nsteps, double* K) {
+    PIC_TRACE("pic2d_run");
     if (!c) return PIC_EINVAL;
     if (nsteps < 0) {
         set_err(c, "pic2d_run: nsteps must be >= 0");
@@ -526,6 +532,7 @@ static int finish(pic2d_ctx* c) {
 }
 
 extern "C" int pic2d_deposit(pic2d_ctx* c, const double* x, const double* y, int64_t P, double w, double* rho) {
+    PIC_TRACE("pic2d_deposit");
     if (!c) return PIC_EINVAL;
     if (int rc = need_fp64_ops(c, "pic2d_deposit")) return rc;
     if (!x || !y || !rho || P < 0 || P > kMaxP2D64 || !(std::isfinite(w) && w > 0.0)) {
@@ -548,6 +555,7 @@ extern "C" int pic2d_deposit(pic2d_ctx* c, const double* x, const double* y, int
 }
 
 extern "C" int pic2d_fieldsolve(pic2d_ctx* c, const double* rho, double* Ex, double* Ey, double* phi) {
+    PIC_TRACE("pic2d_fieldsolve");
     if (!c) return PIC_EINVAL;
     if (int rc = need_fp64_ops(c, "pic2d_fieldsolve")) return rc;
     if (!rho || !Ex || !Ey) {
@@ -582,6 +590,7 @@ extern "C" int pic2d_fieldsolve(pic2d_ctx* c, const double* rho, double* Ex, dou
 
 extern "C" int pic2d_gather(pic2d_ctx* c, const double* Ex, const double* Ey, const double* x, const double* y,
                             int64_t P, double* ex, double* ey) {
+    PIC_TRACE("pic2d_gather");
     if (!c) return PIC_EINVAL;
     if (int rc = need_fp64_ops(c, "pic2d_gather")) return rc;
     if (!Ex || !Ey || !x || !y || !ex || !ey || P < 0) {
@@ -603,6 +612,7 @@ extern "C" int pic2d_gather(pic2d_ctx* c, const double* Ex, const double* Ey, co
 
 extern "C" int pic2d_boris(pic2d_ctx* c, double* vx, double* vy, double* vz, const double* ex, const double* ey,
                            int64_t P, double dt, double B0) {
+    PIC_TRACE("pic2d_boris");
     if (!c) return PIC_EINVAL;
     if (int rc = need_fp64_ops(c, "pic2d_boris")) return rc;
     if (!vx || !vy || !vz || !ex || !ey || P < 0) {
@@ -622,6 +632,7 @@ extern "C" int pic2d_boris(pic2d_ctx* c, double* vx, double* vy, double* vz, con
 
 extern "C" int pic2d_push(pic2d_ctx* c, double* x, double* y, double* vx, double* vy, double* vz, int64_t P,
                           const double* Ex, const double* Ey, double dt, double B0) {
+    PIC_TRACE("pic2d_push");
     if (!c) return PIC_EINVAL;
     if (int rc = need_fp64_ops(c, "pic2d_push")) return rc;
     if (!x || !y || !vx || !vy || !vz || !Ex || !Ey || P < 0) {
@@ -646,6 +657,7 @@ extern "C" int pic2d_push(pic2d_ctx* c, double* x, double* y, double* vx, double
 
 extern "C" int pic2d_diagnostics(pic2d_ctx* c, const double* Ex, const double* Ey, const double* vx, const double* vy,
                                  int64_t P, double w, double* K) {
+    PIC_TRACE("pic2d_diagnostics");
     if (!c) return PIC_EINVAL;
     if (!Ex || !Ey || !vx || !vy || !K || P < 1) {
         set_err(c, "pic2d_diagnostics: bad argument");

@@ -15,6 +15,7 @@
 
 #include "../../include/pic2d3v.h"
 #include "pic2d3v_internal.h"
+#include "trace.h"
 #include "pic2d3v_lg_internal.h"
 
 #pragma clang fp contract(off)
@@ -294,6 +295,7 @@ static int zero_state(pic2d3v_ctx* c) {
 }
 
 extern "C" int pic2d3v_create(pic2d3v_ctx** out, const pic2d3v_config* cfg) {
+    PIC_TRACE("pic2d3v_create");
     if (!out || !cfg) {
         g23_create_err = "pic2d3v_create: null argument";
         return PIC_EINVAL;
@@ -485,6 +487,7 @@ extern "C" int pic2d3v_create(pic2d3v_ctx** out, const pic2d3v_config* cfg) {
 }
 
 extern "C" int pic2d3v_destroy(pic2d3v_ctx* c) {
+    PIC_TRACE("pic2d3v_destroy");
     if (!c) return PIC_EINVAL;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -513,6 +516,7 @@ static int check_species(pic2d3v_ctx* c, const char* who, int32_t s) {
 // ---- state ------------------------------------------------------------------
 
 extern "C" int pic2d3v_upload(pic2d3v_ctx* c, int32_t s, const double* xyv) {
+    PIC_TRACE("pic2d3v_upload");
     if (!c) return PIC_EINVAL;
     if (int rc = check_species(c, "pic2d3v_upload", s)) return rc;
     if (!xyv) {
@@ -535,6 +539,7 @@ extern "C" int pic2d3v_upload(pic2d3v_ctx* c, int32_t s, const double* xyv) {
 }
 
 extern "C" int pic2d3v_download(pic2d3v_ctx* c, int32_t s, double* xyv) {
+    PIC_TRACE("pic2d3v_download");
     if (!c) return PIC_EINVAL;
     if (int rc = check_species(c, "pic2d3v_download", s)) return rc;
     if (!xyv) {
@@ -552,12 +557,14 @@ extern "C" int pic2d3v_download(pic2d3v_ctx* c, int32_t s, double* xyv) {
 }
 
 extern "C" int pic2d3v_reset(pic2d3v_ctx* c) {
+    PIC_TRACE("pic2d3v_reset");
     if (!c) return PIC_EINVAL;
     HIPCHK(c, hipSetDevice(c->cfg.device));
     return zero_state(c);
 }
 
 extern "C" int pic2d3v_field(pic2d3v_ctx* c, double* Ex, double* Ey) {
+    PIC_TRACE("pic2d3v_field");
     if (!c) return PIC_EINVAL;
     HIPCHK(c, hipSetDevice(c->cfg.device));
     const int64_t G = (int64_t)c->cfg.NX * c->cfg.NY;
@@ -623,6 +630,7 @@ static int to_caller(pic2d3v_ctx* c, int32_t s) {
 }
 
 extern "C" int pic2d3v_sort(pic2d3v_ctx* c, int32_t s, double dx, double dy, int64_t* perm) {
+    PIC_TRACE("pic2d3v_sort");
     if (!c) return PIC_EINVAL;
     if (int rc = check_species(c, "pic2d3v_sort", s)) return rc;
     if (!(std::isfinite(dx) && dx > 0.0 && std::isfinite(dy) && dy > 0.0)) {
@@ -867,6 +875,7 @@ static int step_si(pic2d3v_ctx* c, int diag) {
 }
 
 extern "C" int pic2d3v_run(pic2d3v_ctx* c, int64_t nsteps, double* rows) {
+    PIC_TRACE("pic2d3v_run");
     if (!c) return PIC_EINVAL;
     if (nsteps < 0) {
         set_err(c, "pic2d3v_run: nsteps must be >= 0");
@@ -922,6 +931,7 @@ extern "C" int pic2d3v_run(pic2d3v_ctx* c, int64_t nsteps, double* rows) {
 }
 
 extern "C" int pic2d3v_fields(pic2d3v_ctx* c, double* Exs, double* Eys, double* phis, int64_t nbins) {
+    PIC_TRACE("pic2d3v_fields");
     if (!c) return PIC_EINVAL;
     // bins exist for the steps pic2d3v_run took (pic2d3v_loop writes none)
     const int64_t have = std::min<int64_t>(cdiv(c->step, c->cfg.ntskip), c->lorenz ? c->lg.cap : c->d.cap);
@@ -1009,6 +1019,7 @@ static int warmup_si(pic2d3v_ctx* c) {
 }
 
 extern "C" int pic2d3v_warmup(pic2d3v_ctx* c) {
+    PIC_TRACE("pic2d3v_warmup");
     if (!c) return PIC_EINVAL;
     if (!c->lorenz || c->centred) return PIC_OK;  // warmup!(::AbstractField) = field
     if (int rc = need_uploaded(c, "pic2d3v_warmup")) return rc;
@@ -1025,6 +1036,7 @@ extern "C" int pic2d3v_warmup(pic2d3v_ctx* c) {
 }
 
 extern "C" int pic2d3v_loop(pic2d3v_ctx* c, int64_t nsteps) {
+    PIC_TRACE("pic2d3v_loop");
     if (!c) return PIC_EINVAL;
     if (int rc = need_lorenz(c, "pic2d3v_loop")) return rc;
     if (nsteps < 0) {
@@ -1064,6 +1076,7 @@ static int check_slot(pic2d3v_ctx* c, const char* who, int32_t which, int32_t n)
 }
 
 extern "C" int pic2d3v_lg_get(pic2d3v_ctx* c, int32_t which, double* z) {
+    PIC_TRACE("pic2d3v_lg_get");
     if (!c) return PIC_EINVAL;
     if (int rc = need_lorenz(c, "pic2d3v_lg_get")) return rc;
     if (int rc = check_slot(c, "pic2d3v_lg_get", which, c->semi ? PIC2D3V_LG_NSTATE_SI : PIC2D3V_LG_NSTATE))
@@ -1079,6 +1092,7 @@ extern "C" int pic2d3v_lg_get(pic2d3v_ctx* c, int32_t which, double* z) {
 }
 
 extern "C" int pic2d3v_lg_set(pic2d3v_ctx* c, int32_t which, const double* z) {
+    PIC_TRACE("pic2d3v_lg_set");
     if (!c) return PIC_EINVAL;
     if (int rc = need_lorenz(c, "pic2d3v_lg_set")) return rc;
     if (int rc = check_slot(c, "pic2d3v_lg_set", which, c->semi ? PIC2D3V_LG_NSTATE_SI : PIC2D3V_LG_NSTATE))
@@ -1098,6 +1112,7 @@ extern "C" int pic2d3v_lg_set(pic2d3v_ctx* c, int32_t which, const double* z) {
 }
 
 extern "C" int pic2d3v_lg_grid(pic2d3v_ctx* c, double* out) {
+    PIC_TRACE("pic2d3v_lg_grid");
     if (!c) return PIC_EINVAL;
     if (int rc = need_semi(c, "pic2d3v_lg_grid")) return rc;
     if (!out) {
@@ -1111,6 +1126,7 @@ extern "C" int pic2d3v_lg_grid(pic2d3v_ctx* c, double* out) {
 }
 
 extern "C" int pic2d3v_lg_set_grid(pic2d3v_ctx* c, const double* in) {
+    PIC_TRACE("pic2d3v_lg_set_grid");
     if (!c) return PIC_EINVAL;
     if (int rc = need_semi(c, "pic2d3v_lg_set_grid")) return rc;
     if (!in) {
@@ -1124,6 +1140,7 @@ extern "C" int pic2d3v_lg_set_grid(pic2d3v_ctx* c, const double* in) {
 }
 
 extern "C" int pic2d3v_lg_iterations(pic2d3v_ctx* c, int64_t* out, int64_t n) {
+    PIC_TRACE("pic2d3v_lg_iterations");
     if (!c) return PIC_EINVAL;
     if (int rc = need_semi(c, "pic2d3v_lg_iterations")) return rc;
     if (n < 0 || n > (int64_t)c->iters.size()) {
@@ -1139,6 +1156,7 @@ extern "C" int pic2d3v_lg_iterations(pic2d3v_ctx* c, int64_t* out, int64_t n) {
 }
 
 extern "C" int pic2d3v_lg_bins(pic2d3v_ctx* c, int32_t which, double* out, int64_t nbins) {
+    PIC_TRACE("pic2d3v_lg_bins");
     if (!c) return PIC_EINVAL;
     if (int rc = need_lorenz(c, "pic2d3v_lg_bins")) return rc;
     if (int rc = check_slot(c, "pic2d3v_lg_bins", which, PIC2D3V_LG_NBINS)) return rc;
@@ -1180,6 +1198,7 @@ static bool shape_ok(int32_t shape) { return shape >= PIC2D3V_NGP && shape <= PI
 
 extern "C" int pic2d3v_deposit(pic2d3v_ctx* c, int32_t shape, const double* x, const double* y, int64_t n, double w,
                                double* rho) {
+    PIC_TRACE("pic2d3v_deposit");
     if (!c) return PIC_EINVAL;
     if (!shape_ok(shape) || !x || !y || !rho || n < 0 || n > kMaxP23 || !std::isfinite(w)) {
         set_err(c, "pic2d3v_deposit: bad argument");
@@ -1201,6 +1220,7 @@ extern "C" int pic2d3v_deposit(pic2d3v_ctx* c, int32_t shape, const double* x, c
 
 extern "C" int pic2d3v_gather(pic2d3v_ctx* c, int32_t shape, const double* Ex, const double* Ey, const double* x,
                               const double* y, int64_t n, double* ex, double* ey) {
+    PIC_TRACE("pic2d3v_gather");
     if (!c) return PIC_EINVAL;
     if (!shape_ok(shape) || !Ex || !Ey || !x || !y || !ex || !ey || n < 0) {
         set_err(c, "pic2d3v_gather: bad argument");
@@ -1222,6 +1242,7 @@ extern "C" int pic2d3v_gather(pic2d3v_ctx* c, int32_t shape, const double* Ex, c
 
 extern "C" int pic2d3v_boris(pic2d3v_ctx* c, double* vx, double* vy, double* vz, const double* ex, const double* ey,
                              int64_t n, double q_m) {
+    PIC_TRACE("pic2d3v_boris");
     if (!c) return PIC_EINVAL;
     if (!vx || !vy || !vz || !ex || !ey || n < 0 || !std::isfinite(q_m)) {
         set_err(c, "pic2d3v_boris: bad argument");
@@ -1239,6 +1260,7 @@ extern "C" int pic2d3v_boris(pic2d3v_ctx* c, double* vx, double* vy, double* vz,
 }
 
 extern "C" int pic2d3v_fieldsolve(pic2d3v_ctx* c, const double* rho, double* Ex, double* Ey, double* phi) {
+    PIC_TRACE("pic2d3v_fieldsolve");
     if (!c) return PIC_EINVAL;
     if (!rho || !Ex || !Ey) {
         set_err(c, "pic2d3v_fieldsolve: rho, Ex, Ey must not be NULL");

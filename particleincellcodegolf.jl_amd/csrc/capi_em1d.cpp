@@ -14,6 +14,7 @@
 
 #include "../../include/picem1d.h"
 #include "picem1d_internal.h"
+#include "trace.h"
 
 using namespace pic;
 
@@ -132,6 +133,7 @@ static int ilog2i(int n) {
 }
 
 extern "C" int picem1d_create(picem1d_ctx** out, const picem1d_config* cfg) {
+    PIC_TRACE("picem1d_create");
     if (!out || !cfg) {
         gem_create_err = "picem1d_create: null argument";
         return PIC_EINVAL;
@@ -206,6 +208,7 @@ extern "C" int picem1d_create(picem1d_ctx** out, const picem1d_config* cfg) {
 }
 
 extern "C" int picem1d_destroy(picem1d_ctx* c) {
+    PIC_TRACE("picem1d_destroy");
     if (!c) return PIC_EINVAL;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -226,6 +229,7 @@ static int finish(picem1d_ctx* c) {
 // ---- state --------------------------------------------------------------------
 
 extern "C" int picem1d_upload(picem1d_ctx* c, const double* x, const double* v, const double* E, const double* B) {
+    PIC_TRACE("picem1d_upload");
     if (!c) return PIC_EINVAL;
     if (!x || !v) {
         set_err(c, "picem1d_upload: x and v must not be NULL");
@@ -261,6 +265,7 @@ extern "C" int picem1d_upload(picem1d_ctx* c, const double* x, const double* v, 
 }
 
 extern "C" int picem1d_download(picem1d_ctx* c, double* x, double* v, double* E, double* B) {
+    PIC_TRACE("picem1d_download");
     if (!c) return PIC_EINVAL;
     if (!c->uploaded) {
         set_err(c, "picem1d_download: nothing uploaded");
@@ -307,6 +312,7 @@ static int capture(picem1d_ctx* c, bool moments, hipGraphExec_t* out) {
 }
 
 extern "C" int picem1d_run(picem1d_ctx* c, int64_t nsteps, double* fields, int32_t* sweeps) {
+    PIC_TRACE("picem1d_run");
     if (!c) return PIC_EINVAL;
     if (nsteps < 0) {
         set_err(c, "picem1d_run: nsteps must be >= 0");
@@ -371,6 +377,7 @@ static int d2h(picem1d_ctx* c, double* dst, const double* dev, int64_t n) {
 }
 
 extern "C" int picem1d_deposit(picem1d_ctx* c, const double* y, const double* u, int64_t n, double w, double* rho) {
+    PIC_TRACE("picem1d_deposit");
     if (!c) return PIC_EINVAL;
     if (!y || !rho || n < 0 || n > kMaxPEM || !std::isfinite(w)) {
         set_err(c, "picem1d_deposit: bad argument");
@@ -393,6 +400,7 @@ extern "C" int picem1d_deposit(picem1d_ctx* c, const double* y, const double* u,
 extern "C" int picem1d_fields(picem1d_ctx* c, const double* rho, const double* Jy, const double* Jz, const double* Fx,
                               const double* Fy, const double* Fz, const double* Cy, const double* Cz, double* E,
                               double* B, double* e3, double* b2) {
+    PIC_TRACE("picem1d_fields");
     if (!c) return PIC_EINVAL;
     if (!rho || !Jy || !Jz || !Fx || !Fy || !Fz || !Cy || !Cz || !E || !B || !e3 || !b2) {
         set_err(c, "picem1d_fields: arguments must not be NULL");
@@ -423,6 +431,7 @@ extern "C" int picem1d_fields(picem1d_ctx* c, const double* rho, const double* J
 
 extern "C" int picem1d_push(picem1d_ctx* c, const double* cc, const double* e3, const double* b2, const double* u,
                             int64_t n, double* v) {
+    PIC_TRACE("picem1d_push");
     if (!c) return PIC_EINVAL;
     if (!cc || !e3 || !b2 || !u || !v || n < 0) {
         set_err(c, "picem1d_push: bad argument");
@@ -452,6 +461,7 @@ extern "C" int picem1d_push(picem1d_ctx* c, const double* cc, const double* e3, 
 
 extern "C" int picem1d_explicit_fields(picem1d_ctx* c, const double* rho, const double* Jy, const double* Jz,
                                        double* E, double* B, double* b2) {
+    PIC_TRACE("picem1d_explicit_fields");
     if (!c) return PIC_EINVAL;
     if (!rho || !Jy || !Jz || !E || !B || !b2) {
         set_err(c, "picem1d_explicit_fields: arguments must not be NULL");
@@ -479,6 +489,7 @@ extern "C" int picem1d_explicit_fields(picem1d_ctx* c, const double* rho, const 
 
 extern "C" int picem1d_explicit_push(picem1d_ctx* c, const double* x, const double* E, const double* b2,
                                      const double* u, int64_t n, double* v) {
+    PIC_TRACE("picem1d_explicit_push");
     if (!c) return PIC_EINVAL;
     if (!x || !E || !b2 || !u || !v || n < 0) {
         set_err(c, "picem1d_explicit_push: bad argument");

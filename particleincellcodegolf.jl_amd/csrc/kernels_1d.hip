@@ -34,7 +34,7 @@ k_mid_push64(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
              const TileDesc* __restrict__ tiles, double* X0, double* X1, double* V0,
              double* V1, double* vit0, double* vit1, const double* __restrict__ E0,
              const double* __restrict__ E1, u64* __restrict__ racc, int k, const int32_t* __restrict__ halt,
-             int last_slot) {
+             int last_slot, int prime) {
     extern __shared__ u64 lds[];
     if (halted(halt, MODE == 1 && last_slot)) return;
     const TileDesc T = tiles[blockIdx.x];
@@ -53,7 +53,10 @@ k_mid_push64(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
         const double* Ein = (S.e_cur ? E1 : E0) + I.goff;
         for (int i = threadIdx.x; i < N; i += blockDim.x) Eld[i] = Ein[i];
     }
-    if (!last) zero_lds(acc, 3 * N);
+    // prime: the final sweep also deposits the next step's first charge (the
+    // deposit of a MODE 0 launch on the state it writes)
+    const bool deposit = !last || prime;
+    if (deposit) zero_lds(acc, 3 * N);
     __syncthreads();
 
     const int par = S.step & 1;
@@ -78,8 +81,14 @@ k_mid_push64(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
             const double Sg = gather_gauss<R>(Eld, c, Nd, nmask);
             vn = Vp + Sg * dt;
             if (last) {
-                Xo[g] = jl_mod1f(x);
+                const double Xn = jl_mod1f(x);
+                Xo[g] = Xn;
                 Vo[g] = vn;
+                if (prime) {
+                    // next step: X = Xn, V = vn, x_0 = X + ((V + V)/2)*dt
+                    const double x0 = Xn + ((vn + vn) / 2.0) * dt;
+                    deposit_gauss<R>(acc, N, (x0 + Xn) / 2.0, Nd, nmask, w, q);
+                }
                 continue;
             }
             vout[g] = vn;
@@ -88,7 +97,7 @@ k_mid_push64(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
         const double c2 = (x2 + Xp) / 2.0;
         deposit_gauss<R>(acc, N, c2, Nd, nmask, w, q);
     }
-    if (last) return;
+    if (!deposit) return;
     __syncthreads();
     flush_limbs(acc, N, racc + 3 * I.red0);
 }
@@ -236,7 +245,7 @@ k_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
         const double e = solve_at(re, j);
         Eo[j] = e;
-        E32[I.goff + j] = (float)e;
+        if (precision != 0) E32[I.goff + j] = (float)(e * I.escale2);  // E*dt in steps of the position, x 2^-30
         if (precision == 0) {
             u64* a = (u64*)racc + 3 * (I.red0 + j);
             a[0] = 0;
@@ -282,58 +291,6 @@ k_diag_chunks64(const InstDesc* __restrict__ insts, const InstState* __restrict_
         dpart[2 * blockIdx.x] = s2;
         dpart[2 * blockIdx.x + 1] = s1;
     }
-}
-
-// Canonical tree over n values spread with stride 2 (in place in dpart).
-__device__ double tree_over_partials(double* red, double* vals, int64_t n) {
-    while (n > kTreeChunk) {
-        const int64_t C = (n + kTreeChunk - 1) / kTreeChunk;
-        for (int64_t c = 0; c < C; ++c) {
-            const int64_t m = n - c * kTreeChunk < kTreeChunk ? n - c * kTreeChunk : kTreeChunk;
-            const double s = block_chunk_sum(red, m, [&](int64_t i) { return vals[2 * (c * kTreeChunk + i)]; });
-            if (threadIdx.x == 0) vals[2 * c] = s;
-            __syncthreads();
-        }
-        n = C;
-    }
-    return block_chunk_sum(red, n, [&](int64_t i) { return vals[2 * i]; });
-}
-
-// sum(E.^2) over N <= 4096 cells and the canonical tree over the chunk
-// partials of v^2 and v/P; the common case (<= 4096 chunks) in one pass.
-__device__ void diag_sums(double* red, const double* E, int N, double* dp, int64_t nchunks, double& se,
-                          double& sv, double& sp) {
-    if (nchunks > kTreeChunk) {
-        se = block_chunk_sum(red, N, [&](int64_t i) { return E[i] * E[i]; });
-        sv = tree_over_partials(red, dp, nchunks);
-        sp = tree_over_partials(red, dp + 1, nchunks);
-        return;
-    }
-    double a = 0.0, b = 0.0, c = 0.0;
-    if (threadIdx.x < kTreeLanes) {
-        constexpr int K = kTreeChunk / kTreeLanes;
-        double e[K], x[K], y[K];
-#pragma unroll
-        for (int i = 0; i < K; ++i) {
-            const int64_t j = threadIdx.x + (int64_t)i * kTreeLanes;
-            e[i] = j < N ? E[j] : 0.0;
-            x[i] = j < nchunks ? dp[2 * j] : 0.0;
-            y[i] = j < nchunks ? dp[2 * j + 1] : 0.0;
-        }
-#pragma unroll
-        for (int i = 0; i < K; ++i) {
-            const int64_t j = threadIdx.x + (int64_t)i * kTreeLanes;
-            if (j < N) a = a + e[i] * e[i];
-            if (j < nchunks) {
-                b = b + x[i];
-                c = c + y[i];
-            }
-        }
-    }
-    tree256x3<false>(red, a, b, c);
-    se = a;
-    sv = b;
-    sp = c;
 }
 
 __global__ void __launch_bounds__(256)
@@ -398,6 +355,10 @@ k_diag_final(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
             o[3] = sp;
             sweeps[(int64_t)blockIdx.x * cap + row] = scheme != 0 ? S.last_sweep + 1 : 1;
         }
+        // the step is complete: the next one starts a fresh Picard loop (no
+        // first-deposit launch does this where the final sweep primes the
+        // next step's charge)
+        S.last_sweep = -1;
         S.step += 1;
         st[blockIdx.x] = S;
     }
@@ -642,10 +603,10 @@ void launch_mid_push0(const RunBuffers& b, hipStream_t s) {
     u64* part = (u64*)b.racc;
     if (b.R == 6)
         k_mid_push64<6, 0><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
-                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, 0, b.halt, 0);
+                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, 0, b.halt, 0, 0);
     else
         k_mid_push64<7, 0><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
-                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, 0, b.halt, 0);
+                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, 0, b.halt, 0, 0);
 }
 
 void launch_mid_push(const RunBuffers& b, int k, bool last_slot, hipStream_t s) {
@@ -655,10 +616,12 @@ void launch_mid_push(const RunBuffers& b, int k, bool last_slot, hipStream_t s) 
     u64* part = (u64*)b.racc;
     if (b.R == 6)
         k_mid_push64<6, 1><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
-                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, k, b.halt, ls);
+                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, k, b.halt, ls,
+                                                   b.prime_next);
     else
         k_mid_push64<7, 1><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1],
-                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, k, b.halt, ls);
+                                                   b.vit[0], b.vit[1], b.E[0], b.E[1], part, k, b.halt, ls,
+                                                   b.prime_next);
 }
 
 static int solve_lds_bytes(int N) { return solve_lds_doubles(N) * 8; }
@@ -702,8 +665,6 @@ void launch_diag(const RunBuffers& b, hipStream_t s) {
     const bool fused = b.precision == 1 && b.scheme == 1;
     if (b.precision == 0)
         k_diag_chunks64<<<b.nchunks, 256, 0, s>>>(b.insts, b.st, b.chunks, b.V[0], b.V[1], b.dpart, b.halt);
-    else if (!fused)
-        launch_diag_chunks32(b, s);
     k_diag_final<<<b.ninst, 256, 0, s>>>(b.insts, b.st, b.dpart, b.E[0], b.E[1], b.diag, b.sweeps,
                                          b.diag_cap, b.scheme, b.halt, fused ? 1 : 0);
 }
@@ -1041,6 +1002,7 @@ int configure_kernels() {
     chk(allow_lds(k_op_fieldsolve, big));
     (void)hipGetLastError();
     (void)e;
+    if (configure_kernels_small() != 0) return -1;
     return configure_kernels_mixed();
 }
 

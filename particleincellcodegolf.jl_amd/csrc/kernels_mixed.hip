@@ -5,10 +5,15 @@
 // box (2^-32 resolution, wrap-around for free, cell index = top log2 N bits,
 // sub-cell offset = the rest), velocity as an FP32 hi + lo pair (the lo word
 // is read and written once per step to keep the time integration compensated),
-// Picard iterate in FP32.  Shape weights are FP32 polynomials in the sub-cell
-// offset (weights32.h); the gather sums in FP32; the charge is accumulated
-// exactly as integer multiples of 2^-30 of a particle's charge, so the per-cell
-// total is exact and independent of scheduling.  The FFT, the convergence
+// Picard iterate in FP32.  Velocities are held in steps of the fixed-point
+// position per time step (v * dt * 2^32), so the half and quarter steps of
+// the midpoint rule are exact power-of-two scalings and no rounded dt factor
+// biases every particle of a run the same way; the field arrives scaled the
+// same way (E * dt^2 * 4, once per cell in FP64, by the solve).  Shape
+// weights are FP32 polynomials in the sub-cell offset (weights32.h); the
+// gather sums in FP32; the charge is accumulated exactly as integer multiples
+// of 2^-30 of a particle's charge, so the per-cell total is exact and
+// independent of scheduling.  The FFT, the convergence
 // test and the diagnostics stay FP64 (kernels_1d.hip).
 #include "pic_device.h"
 #include "pic_internal.h"
@@ -57,6 +62,8 @@ __device__ __forceinline__ void stencil32(uint32_t cq, int shift, Stencil32& s) 
 #undef PIC_PAIR
 }
 
+// x + (v_a + v_b) * scale * dt on the fixed-point grid, velocities in steps
+// of the grid: scale 1/2 gives the position, 1/4 the midpoint centre (exact).
 __device__ __forceinline__ uint32_t advance(uint32_t xq, float v_sum, float scale) {
     return xq + (uint32_t)(int32_t)rintf(v_sum * scale);
 }
@@ -118,13 +125,11 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     const bool last = (MODE == 1) && (S.last_sweep == k);
     const int32_t N = I.N, nmask = I.nmask;
     const int NPAD = pad_cells(N);
-    // E*dt in units of 2^30 so that quanta weights give the kick directly
-    const float escale = (float)(I.dt * 0x1p-30);
     u64* acc = lds;
     float* Eld = reinterpret_cast<float*>(lds + NPAD);
     if (MODE == 1)
         for (int i = threadIdx.x; i < NPAD; i += blockDim.x)
-            Eld[i] = E32[I.goff + ((i - kHalo) & nmask)] * escale;
+            Eld[i] = E32[I.goff + ((i - kHalo) & nmask)];  // the kick per 2^-30 of weight
     // prime: the final sweep also deposits the next step's first charge
     // (centre x + V*dt/2 with v_0 = V), which a first-deposit launch would
     // otherwise compute from the state it writes
@@ -143,8 +148,8 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     const float* vin = (k & 1) ? vf1 : vf0;
     float* vout = ((k + 1) & 1) ? vf1 : vf0;
     const int shift = 32 - I.log2N;
-    const float cscale = (float)(I.dt * 1073741824.0);  // (v + V)/4*dt in 2^-32 units
-    const float xscale = (float)(I.dt * 2147483648.0);  // (v + V)/2*dt in 2^-32 units
+    constexpr float cscale = 0.25f;  // centre (x + X)/2 = X + (v + V)/4*dt
+    constexpr float xscale = 0.5f;   // x = X + (v + V)/2*dt
     const int64_t base = I.poff + T.start;
     // final sweep: exact sums of v^2 (2^-kV2Bits grid) and v (2^-kV1Bits grid)
     // over this thread's particles
@@ -185,7 +190,7 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
                     Xo[g] = xn;
                     Vho[g] = hi;
                     Vlo[g] = lo;
-                    const double vd = (double)hi + (double)lo;  // exact
+                    const double vd = ((double)hi + (double)lo) * I.inv_vstep;  // the sum is exact
                     q2 += fix_round128(vd * vd, kV2Bits);
                     q1 += fix_round128(vd, kV1Bits);
                     dep = prime != 0;
@@ -235,26 +240,6 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     flush_charge32(acc, N, NPAD, racc + I.red0);
 }
 
-__global__ void __launch_bounds__(256)
-k_diag_chunks32(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
-                const ChunkDesc* __restrict__ chunks, const float* Vh0, const float* Vh1,
-                const float* Vl0, const float* Vl1, double* __restrict__ dpart, const int32_t* __restrict__ halt) {
-    __shared__ double red[4 * kTreeLanes + 8];
-    if (*halt) return;
-    const ChunkDesc C = chunks[blockIdx.x];
-    const InstDesc I = insts[C.inst];
-    const InstState S = st[C.inst];
-    const int nxt = (S.step + 1) & 1;
-    const float* vh = (nxt ? Vh1 : Vh0) + I.poff + C.start;
-    const float* vl = (nxt ? Vl1 : Vl0) + I.poff + C.start;
-    double s2, s1;
-    chunk_sums_v(red, C.count, (double)I.P, [&](int64_t i) { return (double)vh[i] + (double)vl[i]; }, s2, s1);
-    if (threadIdx.x == 0) {
-        dpart[2 * blockIdx.x] = s2;
-        dpart[2 * blockIdx.x + 1] = s1;
-    }
-}
-
 void launch_mid_push32(const RunBuffers& b, int mode, int k, bool last_slot, hipStream_t s) {
     const int ls = last_slot ? 1 : 0;
     const dim3 g(b.ntiles), t(b.block_push);
@@ -267,11 +252,6 @@ void launch_mid_push32(const RunBuffers& b, int mode, int k, bool last_slot, hip
         k_mid_push32<1><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.Xq[0], b.Xq[1], b.Vh[0], b.Vh[1],
                                                  b.Vl[0], b.Vl[1], b.vf[0], b.vf[1], b.E32, part, k, b.halt, ls,
                                                  b.dpart, b.prime_next);
-}
-
-void launch_diag_chunks32(const RunBuffers& b, hipStream_t s) {
-    k_diag_chunks32<<<b.nchunks, 256, 0, s>>>(b.insts, b.st, b.chunks, b.Vh[0], b.Vh[1], b.Vl[0], b.Vl[1],
-                                              b.dpart, b.halt);
 }
 
 int configure_kernels_mixed() {

@@ -37,6 +37,11 @@ struct InstDesc {
     int64_t red0;                // first cell of this instance in the charge accumulator
     double w, dt, W, rtol;
     double wq;                   // mixed deposit: w * 2^-30 (charge per weight quantum)
+    // mixed: velocities are held in steps of the 2^-32 fixed-point position,
+    // v_s = v * vstep with vstep = dt * 2^32, so every position update is a
+    // power-of-two scaling; the gather reads E * escale2 (escale2 = dt^2 * 4,
+    // applied in FP64 once per cell) against weights in 2^-30 units
+    double vstep, inv_vstep, escale2;
 };
 
 // Device-mutable per-instance loop state.
@@ -94,10 +99,10 @@ struct RunBuffers {
     // into 2 and every later kernel returns at once.  The host then finishes
     // the step with the remaining sweeps and resumes.
     int32_t* halt;
-    // mixed midpoint: the final sweep also deposits the next step's first
-    // charge, so steps after the first carry no first-deposit launch
+    // midpoint: the final sweep also deposits the next step's first charge,
+    // so steps after the first carry no first-deposit launch
     int32_t prime_next;
-    int64_t racc_cells;  // u64 words of the mixed accumulator
+    int64_t racc_cells;  // u64 words of the charge accumulator
     // Simpson-1/3 (FP64): E[0..1] hold E[3,:] (ping-pong by e_cur), Es1 is
     // E[1,:], Es2[0..1] E[2,:]; racc collects rho(X, X), racc2 rho(X, x) and
     // racc3 rho(x, x); Vg[p] = V + (gather(E[1,:], X)*dt)/6 of the step.
@@ -130,8 +135,32 @@ bool mixed_available();
 // kernels_mixed.hip
 void launch_mid_push32(const RunBuffers& b, int mode, int k, bool last_slot, hipStream_t s);
 int push32_lds_bytes(int N);
-void launch_diag_chunks32(const RunBuffers& b, hipStream_t s);
 int configure_kernels_mixed();
+
+// kernels_small.hip: the whole run of a small single FP64 instance in one
+// launch (the step loop, the Picard sweeps and their test on the device)
+constexpr int kSmallMaxN = 512;       // every workgroup holds the grid, its solve and its tables in LDS
+constexpr int kSmallNgpSlots = 16;    // NGP: particles per thread of the one 512-thread workgroup (P <= 8192)
+constexpr int kSmallMaxSlots = 4;     // Gaussian: particles per 16-lane segment
+constexpr int kSmallMaxGroups = 128;  // Gaussian: particle workgroups (32 segments each; P <= 16384)
+struct SmallRun {
+    InstDesc I;
+    InstState* st;
+    double *X0, *X1, *V0, *V1, *E0, *E1;
+    const double2* tw;
+    double* diag;                // nsteps x 4
+    int32_t* sweeps;             // nsteps
+    int64_t nsteps;
+    unsigned long long* gacc;    // Gaussian: three device charge accumulators of 3N limbs
+    uint32_t* bar;               // Gaussian: barrier counter, generation, failure flag
+    int32_t K;                   // Gaussian: particle workgroups (one more computes diagnostics)
+    uint32_t spin_limit;         // Gaussian: polls before a barrier wait gives up
+};
+// whether the instance can run persistently; *K gets the default workgroups
+bool small_eligible(const InstDesc& I, int scheme, int precision, int64_t* K);
+int small_workgroups_ok(const InstDesc& I, int64_t K);
+hipError_t launch_small_run(const SmallRun& a, int scheme, hipStream_t s);
+int configure_kernels_small();
 
 // operators (single grid of N cells, contiguous particle arrays)
 struct OpBuffers {

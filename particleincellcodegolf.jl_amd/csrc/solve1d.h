@@ -131,8 +131,10 @@ struct SolveRegs {
 };
 
 // Issues the solve's table loads (call before the charge loads so that their
-// latency overlaps).  Requires blockDim.x == kSolveThreads.
-__device__ __forceinline__ void solve_prefetch(SolveRegs& R, const double2* __restrict__ tw, int N, int L) {
+// latency overlaps).  Requires blockDim.x == kSolveThreads.  xi: the
+// 1/(2 pi k) factors, k = 0..N/2 (NULL: the ones after the twiddle table).
+__device__ __forceinline__ void solve_prefetch(SolveRegs& R, const double2* __restrict__ tw, int N, int L,
+                                               const double* __restrict__ xi_in = nullptr) {
     const int r = L % 3, np = (L + 2) / 3;
 #pragma unroll
     for (int p = 0; p < 4; ++p)
@@ -141,7 +143,7 @@ __device__ __forceinline__ void solve_prefetch(SolveRegs& R, const double2* __re
             load_pass_tw(R.w[p], tw, s0, (p == 0 && r) ? r : 3, threadIdx.x & ((1 << s0) - 1));
         }
     load_pass_tw(R.w0, tw, 0, 3, 0);
-    const double* xi = reinterpret_cast<const double*>(tw) + kXiOffset;
+    const double* xi = xi_in ? xi_in : reinterpret_cast<const double*>(tw) + kXiOffset;
     const int g = threadIdx.x, s = L - 3;
     if (g < (N >> 3)) {
 #pragma unroll
@@ -377,5 +379,63 @@ __device__ bool block_isapprox(double* red, FF&& F, FE&& E, int n, double rtol, 
 // and (TEST) runs the midpoint's convergence test between the previous field
 // (e_cur) and the new one -- the test the scripts run at the top of sweep
 // k+1 -- ending the loop on success or at the last sweep.
+
+// ---------------------------------------------------------------------------
+// Diagnostics sums (src/GaussianFixedPointQuiet.jl:11,13) with the canonical
+// tree: sum(E.^2) over the grid and the trees over the per-chunk partials of
+// v^2 and v/P (chunks of 4096 particles).
+// ---------------------------------------------------------------------------
+// Canonical tree over n values spread with stride 2 (in place in dpart).
+__device__ inline double tree_over_partials(double* red, double* vals, int64_t n) {
+    while (n > kTreeChunk) {
+        const int64_t C = (n + kTreeChunk - 1) / kTreeChunk;
+        for (int64_t c = 0; c < C; ++c) {
+            const int64_t m = n - c * kTreeChunk < kTreeChunk ? n - c * kTreeChunk : kTreeChunk;
+            const double s = block_chunk_sum(red, m, [&](int64_t i) { return vals[2 * (c * kTreeChunk + i)]; });
+            if (threadIdx.x == 0) vals[2 * c] = s;
+            __syncthreads();
+        }
+        n = C;
+    }
+    return block_chunk_sum(red, n, [&](int64_t i) { return vals[2 * i]; });
+}
+
+// sum(E.^2) over N <= 4096 cells and the canonical tree over the chunk
+// partials of v^2 and v/P; the common case (<= 4096 chunks) in one pass.
+__device__ inline void diag_sums(double* red, const double* E, int N, double* dp, int64_t nchunks, double& se,
+                          double& sv, double& sp) {
+    if (nchunks > kTreeChunk) {
+        se = block_chunk_sum(red, N, [&](int64_t i) { return E[i] * E[i]; });
+        sv = tree_over_partials(red, dp, nchunks);
+        sp = tree_over_partials(red, dp + 1, nchunks);
+        return;
+    }
+    double a = 0.0, b = 0.0, c = 0.0;
+    if (threadIdx.x < kTreeLanes) {
+        constexpr int K = kTreeChunk / kTreeLanes;
+        double e[K], x[K], y[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const int64_t j = threadIdx.x + (int64_t)i * kTreeLanes;
+            e[i] = j < N ? E[j] : 0.0;
+            x[i] = j < nchunks ? dp[2 * j] : 0.0;
+            y[i] = j < nchunks ? dp[2 * j + 1] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const int64_t j = threadIdx.x + (int64_t)i * kTreeLanes;
+            if (j < N) a = a + e[i] * e[i];
+            if (j < nchunks) {
+                b = b + x[i];
+                c = c + y[i];
+            }
+        }
+    }
+    tree256x3<false>(red, a, b, c);
+    se = a;
+    sv = b;
+    sp = c;
+}
+
 
 }  // namespace pic

@@ -19,6 +19,7 @@ PIC_ENONFINITE = -4
 PIC_ESTATE = -5
 
 SHAPE_NGP, SHAPE_AREA, SHAPE_GAUSS6, SHAPE_GAUSS7 = 0, 1, 6, 7
+RUN_AUTO, RUN_GRAPH, RUN_PERSISTENT = 0, 1, 2
 LEAPFROG, MIDPOINT, SIMPSON = 0, 1, 2
 FP64, MIXED = 0, 1
 
@@ -28,7 +29,8 @@ EXPORTS = [
     "pic_version", "pic_upload", "pic_download", "pic_run", "pic_run_profiled",
     "pic_deposit", "pic_fieldsolve", "pic_drift_half", "pic_push_leapfrog",
     "pic_midpoint_position", "pic_push_midpoint", "pic_simpson_kick", "pic_wrap", "pic_isapprox",
-    "pic_diagnostics", "pic_set_tiling", "pic_get_tiling", "pic_autotune",
+    "pic_diagnostics", "pic_set_tiling", "pic_get_tiling", "pic_autotune", "pic_set_run_path",
+    "pic_get_run_path",
 ]
 
 # every symbol include/pic2d.h declares
@@ -186,6 +188,8 @@ def load():
         "pic_isapprox": [vp, vp, vp, i32, d, d, ip],
         "pic_diagnostics": [vp, vp, vp, i64, d, vp],
         "pic_set_tiling": [vp, i64, i32],
+        "pic_set_run_path": [vp, i32, i32],
+        "pic_get_run_path": [vp, vp, vp],
         "pic_get_tiling": [vp, ctypes.POINTER(i64), ip, ip],
         "pic_autotune": [vp, i32, ctypes.POINTER(pic_tuning)],
     }

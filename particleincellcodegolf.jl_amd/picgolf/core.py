@@ -214,6 +214,22 @@ class Context:
         self._check(self._lib.pic_run(self._h, int(nsteps), L.ptr(diag), L.ptr(sweeps)), "pic_run")
         return diag, sweeps
 
+    # -- run path ------------------------------------------------------------
+    RUN_PATHS = {"auto": L.RUN_AUTO, "graph": L.RUN_GRAPH, "persistent": L.RUN_PERSISTENT}
+
+    def set_run_path(self, path="auto", workgroups=0):
+        """How run() advances: "graph" (captured step graphs), "persistent"
+        (the whole call in one launch; one small FP64 instance) or "auto"."""
+        self._check(self._lib.pic_set_run_path(self._h, _code(path, self.RUN_PATHS, "run path"), int(workgroups)),
+                    "pic_set_run_path")
+        return self.run_path()
+
+    def run_path(self):
+        """(path the next run() takes, "graph" or "persistent"; its workgroups)."""
+        p, k = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self._lib.pic_get_run_path(self._h, ctypes.byref(p), ctypes.byref(k)), "pic_get_run_path")
+        return ("persistent" if p.value == L.RUN_PERSISTENT else "graph"), k.value
+
     # -- particle tiling of the push launches ---------------------------------
     def tiling(self):
         """(particles per tile, threads per push workgroup, tiles)."""

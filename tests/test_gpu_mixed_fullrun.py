@@ -1,0 +1,137 @@
+"""The mixed-precision form over the configured run lengths, against the
+FP64 path (the oracle's bits), at the bars it is quoted at: field-energy
+relative error <= 1e-6 over the run and the fitted growth rate within 1% of
+FP64 (src/GaussianFixedPoint.jl:7-11; the analytic rate of
+src/GaussianFixedPointQuiet.jl:19-20 reported beside it).
+
+  scaled      N=4096, P=2^24, all 2000 steps; the field every 50 steps for
+              the growth of mode 2 (least squares on log|E^_2|)
+  ensemble    512 instances, all 1000 steps; each instance compared up to its
+              saturation (the peak of its FP64 field energy), growth rates
+              fitted on the exponential phase of the field energy
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def gamma(W, m=1, u=1.0):
+    """gamma of src/GaussianFixedPointQuiet.jl:19 for mode m, beams +-u."""
+    x = 2 * np.pi * m * u / np.sqrt(W / 2)
+    return np.imag(np.sqrt(complex(x ** 2 + 1 - np.sqrt(4 * x ** 2 + 1)))) * np.sqrt(W / 2)
+
+
+@pytest.fixture(scope="module")
+def scaled():
+    import picgolf
+    from picgolf import configs
+    out = {}
+    for prec in ("mixed", "fp64"):
+        cfg, x, v, T = configs.scaled_two_stream(precision=prec)
+        Ds, Es = [], []
+        with picgolf.Context(cfg) as c:
+            c.upload(x, v)
+            for _ in range(T // 50):
+                D, _ = c.run(50)
+                Ds.append(D[0])
+                Es.append(c.download()[2])
+        out[prec] = (np.concatenate(Ds), np.array(Es))
+        print(f"scaled {prec}: {T} steps done", flush=True)
+    return cfg, T, out
+
+
+def test_scaled_field_energy_over_the_run(scaled):
+    cfg, T, out = scaled
+    (Dm, _), (Df, _) = out["mixed"], out["fp64"]
+    rel = np.abs(Dm[:, 0] - Df[:, 0]) / Df[:, 0]
+    print(f"scaled: field-energy rel err over {T} steps: max {rel.max():.3e} (step {rel.argmax() + 1}), "
+          f"final {rel[-1]:.3e}; momentum drift mixed {np.abs(Dm[:, 3] - Dm[0, 3]).max():.2e}")
+    assert rel.size == T
+    assert rel.max() <= 1e-6
+    assert np.abs(Dm[:, 3] - Dm[0, 3]).max() < 1e-9
+
+
+def test_scaled_growth_rate_of_mode_2(scaled):
+    cfg, T, out = scaled
+    t = np.arange(1, T // 50 + 1) * 50 * cfg.dt
+    fits = {}
+    for prec, (_, E) in out.items():
+        a2 = np.abs(np.fft.fft(E, axis=1)[:, 2])
+        fits[prec] = np.polyfit(t, np.log(a2), 1)[0]
+    g2 = gamma(cfg.W, m=2)
+    a2 = np.abs(np.fft.fft(out["fp64"][1], axis=1)[:, 2])
+    print("scaled: |E^_2| (FP64) every 400 steps: " + " ".join(f"{a:.4e}" for a in a2[7::8]))
+    print(f"scaled: mode-2 growth mixed {fits['mixed']:.5f} fp64 {fits['fp64']:.5f} analytic {g2:.4f} "
+          f"(gamma T = {g2 * T * cfg.dt:.2f}: the run is linear, its mode 2 starts from noise)")
+    assert fits["mixed"] == pytest.approx(fits["fp64"], rel=0.01)
+
+
+@pytest.fixture(scope="module")
+def ensemble():
+    import picgolf
+    from picgolf import configs
+    out = {}
+    for prec in ("mixed", "fp64"):
+        cfgs, states, T = configs.ensemble(512, precision=prec)
+        with picgolf.Context(cfgs) as c:
+            for i, (x, v) in enumerate(states):
+                c.upload(x, v, inst=i)
+            D, S = c.run(T)
+        out[prec] = (D, S)
+        print(f"ensemble {prec}: {T} steps done", flush=True)
+    return cfgs, states, T, out
+
+
+def test_ensemble_field_energy_up_to_saturation(ensemble):
+    cfgs, states, T, out = ensemble
+    Dm, Df = out["mixed"][0], out["fp64"][0]
+    rel = np.abs(Dm[:, :, 0] - Df[:, :, 0]) / Df[:, :, 0]
+    sat = np.argmax(Df[:, :, 0], axis=1)
+    worst = np.array([rel[i, :sat[i] + 1].max() for i in range(len(cfgs))])
+    print(f"ensemble: field-energy rel err up to saturation: max {worst.max():.3e} (instance {worst.argmax()}), "
+          f"median {np.median(worst):.3e}; {np.sum(sat < T - 1)} of {len(cfgs)} instances saturate within {T} steps")
+    assert worst.max() <= 1e-6
+
+
+def test_ensemble_growth_rates(ensemble):
+    cfgs, states, T, out = ensemble
+    Dm, Df = out["mixed"][0], out["fp64"][0]
+    ratios, analytic = [], []
+    for i, k in enumerate(cfgs):
+        d1f, d1m = Df[i, :, 0], Dm[i, :, 0]
+        sat = int(np.argmax(d1f))
+        lo = int(np.argmin(d1f[:sat + 1]))
+        # the exponential phase: from 10x the trough to 1/10 of the peak
+        sel = np.arange(T)
+        sel = (sel > lo) & (sel < sat) & (d1f > 10 * d1f[lo]) & (d1f < d1f[sat] / 10)
+        if sel.sum() < 30:
+            continue
+        t = np.arange(1, T + 1) * k.dt
+        sf = np.polyfit(t[sel], np.log(d1f[sel]), 1)[0] / 2
+        sm = np.polyfit(t[sel], np.log(d1m[sel]), 1)[0] / 2
+        u = abs(states[i][1][0])
+        g = max(gamma(k.W, m, u) for m in range(1, k.N // 2))
+        ratios.append(sm / sf)
+        analytic.append(sf / g)
+    ratios, analytic = np.array(ratios), np.array(analytic)
+    print(f"ensemble: {ratios.size} instances with an exponential phase; mixed/FP64 growth rate "
+          f"{ratios.min():.5f}..{ratios.max():.5f}; FP64/analytic median {np.median(analytic):.3f}")
+    assert ratios.size >= 50
+    assert np.all(np.abs(ratios - 1) <= 0.01)
+
+
+def test_ensemble_sweep_counts_follow_fp64(ensemble):
+    """Each step runs its own Picard loop: the mixed sweep counts track the
+    FP64 ones step by step and rise where the field's growth needs it (a
+    step never inherits the previous step's last sweep)."""
+    cfgs, states, T, out = ensemble
+    Sm, Sf = out["mixed"][1], out["fp64"][1]
+    agree = np.mean(Sm == Sf)
+    rises = int(np.sum(np.diff(Sm, axis=1) > 0))
+    print(f"ensemble: mixed sweep counts equal FP64's on {agree:.4f} of instance-steps; {rises} rises; "
+          f"mean {Sm.mean():.3f} vs {Sf.mean():.3f}")
+    # the test ||F - E|| <= 1e-8 ||E|| flips where a step sits at the threshold
+    assert agree > 0.95
+    assert abs(Sm.mean() - Sf.mean()) <= 0.01 * Sf.mean()
+    assert rises > 0
