@@ -30,15 +30,23 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+OBJ_DIR = os.path.join(LIB_DIR, "obj")
+SAN_LIB = os.path.join(LIB_DIR, "san", "libpicgolf.so")
+# AddressSanitizer + UndefinedBehaviorSanitizer on the host code only (each
+# -fsanitize= behind -Xarch_host: the device code is built as always)
+HOST_SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                 "-Xarch_host", "-fno-sanitize-recover=all", "-Xarch_host", "-fno-omit-frame-pointer"]
+
+
 def build(verbose=False):
-    os.makedirs(LIB_DIR, exist_ok=True)
+    os.makedirs(OBJ_DIR, exist_ok=True)
     objs = []
     common = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall",
               "-Wno-unused-function", "-I", os.path.join(HERE, "..", "include")]
     cmds = []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
-        obj = os.path.join(LIB_DIR, os.path.splitext(src)[0] + ".o")
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
         cmd = [hipcc(), *common, "-c", path, "-o", obj]
         # hipcc compiles .cpp as HIP too: every unit gets the same target flags
         cmd[1:1] = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -54,9 +62,42 @@ def build(verbose=False):
     run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, f"-L{rocm_lib}",
          "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{rocm_lib}"], verbose)
     os.replace(tmp, LIB)
-    for o in objs:
-        os.remove(o)
     return LIB
+
+
+def build_sanitized(verbose=False):
+    """libpicgolf with the host runtime (the C ABI, capi*.cpp) built under
+    ASan + UBSan into lib/san/, linked with the kernels of the last build().
+    It needs the clang ASan runtime preloaded (tests/test_sanitizers_cpu.py)."""
+    if not all(os.path.exists(os.path.join(OBJ_DIR, os.path.splitext(f)[0] + ".o")) for f in SOURCES):
+        build(verbose)
+    san_dir = os.path.dirname(SAN_LIB)
+    os.makedirs(san_dir, exist_ok=True)
+    common = ["-O1", "-g", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+              "-I", os.path.join(HERE, "..", "include")]
+    cmds, objs = [], []
+    for src in SOURCES:
+        stem = os.path.splitext(src)[0]
+        if not src.endswith(".cpp"):
+            objs.append(os.path.join(OBJ_DIR, stem + ".o"))
+            continue
+        obj = os.path.join(san_dir, stem + ".o")
+        cmds.append([hipcc(), f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                     *HOST_SANITIZE, *common, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    with concurrent.futures.ThreadPoolExecutor(max(1, min(len(cmds), os.cpu_count() or 1))) as ex:
+        for f in [ex.submit(run, cmd, verbose) for cmd in cmds]:
+            f.result()
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", SAN_LIB, *objs, f"-L{rocm_lib}",
+         "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{rocm_lib}"], verbose)
+    return SAN_LIB
+
+
+def asan_runtime():
+    """The clang ASan runtime matching hipcc (to preload for SAN_LIB)."""
+    r = subprocess.run([hipcc(), "-print-file-name=libclang_rt.asan-x86_64.so"], capture_output=True, text=True)
+    return r.stdout.strip()
 
 
 def run(cmd, verbose):
@@ -73,3 +114,5 @@ def run(cmd, verbose):
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    if len(sys.argv) > 1 and sys.argv[1] == "sanitize":
+        print(build_sanitized(verbose=True))

@@ -208,6 +208,7 @@ __device__ __forceinline__ u128 fxp_from(double a, int q) {
     int ex = (int)((b >> 52) & 0x7ff);
     u64 m = b & 0xFFFFFFFFFFFFFull;
     int e;
+    if (ex == 0x7ff) return 0;  // Inf / NaN add nothing (a defined result; no grid holds them)
     if (ex == 0) {
         if (m == 0) return 0;
         e = -1074;
@@ -216,7 +217,7 @@ __device__ __forceinline__ u128 fxp_from(double a, int q) {
         e = ex - 1075;
     }
     int s = e + q;
-    if (s >= 0) return (u128)m << s;
+    if (s >= 0) return s <= 74 ? (u128)m << s : ~(u128)0;  // beyond 2^128 of the grid: saturate
     int r = -s;
     if (r >= 54) return 0;
     u64 qv = m >> r;

@@ -121,3 +121,18 @@ def test_fixed_point_conversion_exact(orc):
     got = orc.fxp_to_double(S, q)
     assert got == float(Fraction(S, 2 ** q))
     assert got == float(sum(Fraction(v) for v in vals))  # exact sum, rounded once
+
+
+def test_fixed_point_edges_are_defined(orc):
+    """Rounding onto the 2^-q grid at the edges of its domain: Inf and NaN
+    add nothing, a value at or beyond 2^128 grid units saturates, the
+    largest representable shift is exact, and ties round to even."""
+    q = 64
+    assert orc.fxp_from_double(float("nan"), q) == 0
+    assert orc.fxp_from_double(float("inf"), q) == 0
+    assert orc.fxp_from_double(2.0 ** 70, q) == (1 << 128) - 1          # 2^134 units
+    big = (2 ** 53 - 1) * 2.0 ** 10                                       # mantissa shifted by 74
+    assert orc.fxp_from_double(big, q) == (2 ** 53 - 1) << 74
+    assert orc.fxp_from_double(2.5 * 2.0 ** -q, q) == 2                   # half to even
+    assert orc.fxp_from_double(3.5 * 2.0 ** -q, q) == 4
+    assert orc.fxp_from_double(2.0 ** -(q + 2), q) == 0                   # below half a unit
