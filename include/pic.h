@@ -191,6 +191,24 @@ int pic_isapprox(pic_ctx* ctx, const double* F, const double* E, int32_t n,
 int pic_diagnostics(pic_ctx* ctx, const double* E, const double* v, int64_t P,
                     double W, double* out4);
 
+/* ---- allocation-free deposit ------------------------------------------ */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops -- the deposit
+ * of pic_deposit (src/GaussianFixedPointQuiet.jl:7, src/NGPFourier.jl:5) on
+ * DEVICE arrays x, y (y = NULL means y = x) and rho (N cells), enqueued on
+ * `stream` (a hipStream_t; NULL = the null stream) and returning without
+ * waiting.  No context: N is an argument (a power of two, 8 <= N <= 4096).
+ * Its scratch, an exact per-cell accumulator of 3 x 4096 u64 limbs (96 KiB),
+ * is a static device array each GPU holds for the life of the process; the
+ * call allocates, frees and synchronises nothing and clears the accumulator
+ * with a kernel (safe to capture).  Calls that overlap on one GPU share that
+ * accumulator and race: serialise them.  rho is bit for bit pic_deposit's.
+ * Errors: PIC_EINVAL (null array, P < 1, bad N, shape or w), PIC_EHIP; the
+ * message of the calling thread's last failure is pic_noalloc_last_error(). */
+int pic_deposit_noalloc(const double* x, const double* y, int64_t P, double w, int32_t N,
+                        int32_t shape, double* rho, void* stream);
+const char* pic_noalloc_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif
