@@ -99,6 +99,15 @@ int pic_download(pic_ctx* ctx, int32_t inst, double* x, double* v, double* E);
  * D[t,1:4] trace of src/GaussianFixedPointQuiet.jl:11,13.  sweeps (nsteps per
  * instance; may be NULL): Picard sweeps taken each step (1 for leapfrog). */
 int pic_run(pic_ctx* ctx, int64_t nsteps, double* diag, int32_t* sweeps);
+/* As pic_run, but returns at once: the steps run on a thread of the library
+ * while the caller goes on.  diag and sweeps must stay valid until pic_sync.
+ * Until then every other call on the context returns PIC_ESTATE (pic_destroy
+ * waits for the run first).  The reference has no counterpart: its step loop
+ * is the script's own (src/GaussianFixedPointQuiet.jl:7). */
+int pic_run_async(pic_ctx* ctx, int64_t nsteps, double* diag, int32_t* sweeps);
+/* Waits for the context's pic_run_async run and returns its status (PIC_OK
+ * when none is in flight); pic_last_error then describes a failure. */
+int pic_sync(pic_ctx* ctx);
 /* As pic_run, but launched eagerly with HIP events around every kernel.
  * Fills up to *n_stats entries of stats and sets *n_stats to the count. */
 int pic_run_profiled(pic_ctx* ctx, int64_t nsteps, double* diag, int32_t* sweeps,
@@ -111,8 +120,8 @@ int pic_run_profiled(pic_ctx* ctx, int64_t nsteps, double* diag, int32_t* sweeps
  *  GRAPH       captured step graphs: one launch per kernel per sweep slot
  *  PERSISTENT  one launch for the whole call: the step loop, the Picard
  *              sweeps and their test stay on the device.  Single FP64
- *              instances only: NGP leapfrog with N <= 512 and P <= 8192, or
- *              the Gaussian shapes (leapfrog or midpoint) with N <= 512 and
+ *              instances only: NGP leapfrog with N <= 256 and P <= 8192, or
+ *              the Gaussian shapes (leapfrog or midpoint) with N <= 256 and
  *              P <= 16384
  *  AUTO        PERSISTENT where it applies, else GRAPH (the default)
  * workgroups: the Gaussian persistent run's particle workgroups (1..128,

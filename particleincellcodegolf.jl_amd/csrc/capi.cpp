@@ -3,12 +3,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pic.h"
@@ -63,9 +65,18 @@ struct pic_ctx {
     int64_t tileP = 0;
     DevBuf tiles_buf, dpart_buf, snap_buf;
     std::string err;
+    // pic_run_async: the run's host thread, and whether it is still in flight
+    std::thread async_thread;
+    std::atomic<bool> busy{false};
+    int async_rc = 0;
 };
 
 static thread_local std::string g_create_err;
+
+// Every call but pic_sync, pic_destroy and the getters of fixed settings is
+// refused while a pic_run_async run is in flight on the context (the run's
+// thread owns c->err until pic_sync, so nothing is written there).
+static int in_flight(pic_ctx*) { return PIC_ESTATE; }
 
 static void set_err(pic_ctx* c, const char* fmt, ...) {
     if (!c) return;
@@ -232,7 +243,12 @@ const char* pic_strerror(int code) {
 
 // pic_last_error(NULL) reports why the last pic_create / pic_create_batch on
 // this thread failed.
-const char* pic_last_error(const pic_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+// While a pic_run_async run is in flight its thread owns ctx->err, and the
+// refusal is reported instead.
+const char* pic_last_error(const pic_ctx* ctx) {
+    if (ctx && ctx->busy.load()) return "a pic_run_async run is in flight; call pic_sync first";
+    return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
 
 // Default tiling: every tile of every instance holds at most tileP particles,
 // so the launch is balanced, with enough tiles to cover the chip.  FP64: as
@@ -468,6 +484,7 @@ int pic_create(pic_ctx** out, const pic_config* cfg) {
 int pic_destroy(pic_ctx* c) {
     PIC_TRACE("pic_destroy");
     if (!c) return PIC_OK;
+    if (c->async_thread.joinable()) c->async_thread.join();
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graphs(c);
@@ -500,6 +517,7 @@ static int write_states(pic_ctx* c, const std::vector<InstState>& st) {
 int pic_upload(pic_ctx* c, int32_t inst, const double* x, const double* v, const double* E) {
     PIC_TRACE("pic_upload");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (inst < 0 || inst >= c->ninst || !x || !v) {
         set_err(c, "pic_upload: bad instance or null array");
         return PIC_EINVAL;
@@ -551,6 +569,7 @@ int pic_upload(pic_ctx* c, int32_t inst, const double* x, const double* v, const
 int pic_download(pic_ctx* c, int32_t inst, double* x, double* v, double* E) {
     PIC_TRACE("pic_download");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (inst < 0 || inst >= c->ninst) {
         set_err(c, "pic_download: bad instance");
         return PIC_EINVAL;
@@ -780,6 +799,7 @@ static int run_small(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps) 
 int pic_set_run_path(pic_ctx* c, int32_t path, int32_t workgroups) {
     PIC_TRACE("pic_set_run_path");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (path != PIC_RUN_AUTO && path != PIC_RUN_GRAPH && path != PIC_RUN_PERSISTENT) {
         set_err(c, "pic_set_run_path: unknown path %d", (int)path);
         return PIC_EINVAL;
@@ -787,7 +807,7 @@ int pic_set_run_path(pic_ctx* c, int32_t path, int32_t workgroups) {
     int64_t K = 0;
     const bool fits = c->ninst == 1 && small_eligible(c->hinst[0], c->scheme, c->precision, &K);
     if (path == PIC_RUN_PERSISTENT && !fits) {
-        set_err(c, "pic_set_run_path: the persistent run takes one FP64 instance with N <= 512 and NGP "
+        set_err(c, "pic_set_run_path: the persistent run takes one FP64 instance with N <= 256 and NGP "
                    "leapfrog (P <= 8192) or a Gaussian shape (P <= 16384)");
         return PIC_EINVAL;
     }
@@ -809,9 +829,46 @@ int pic_get_run_path(const pic_ctx* c, int32_t* path, int32_t* workgroups) {
     return PIC_OK;
 }
 
+static int run_impl(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps);
+
 int pic_run(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps) {
     PIC_TRACE("pic_run");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
+    return run_impl(c, nsteps, diag, sweeps);
+}
+
+int pic_run_async(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps) {
+    PIC_TRACE("pic_run_async");
+    if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
+    if (c->async_thread.joinable()) c->async_thread.join();
+    c->busy.store(true);
+    try {
+        c->async_thread = std::thread([c, nsteps, diag, sweeps] {
+            c->async_rc = run_impl(c, nsteps, diag, sweeps);
+            c->busy.store(false);
+        });
+    } catch (...) {
+        c->busy.store(false);
+        set_err(c, "pic_run_async: could not start the run's thread");
+        return PIC_ESTATE;
+    }
+    return PIC_OK;
+}
+
+int pic_sync(pic_ctx* c) {
+    PIC_TRACE("pic_sync");
+    if (!c) return PIC_EINVAL;
+    if (!c->async_thread.joinable()) return PIC_OK;
+    c->async_thread.join();
+    c->busy.store(false);
+    const int rc = c->async_rc;
+    c->async_rc = PIC_OK;
+    return rc;
+}
+
+static int run_impl(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps) {
     if (nsteps < 0) {
         set_err(c, "pic_run: nsteps < 0");
         return PIC_EINVAL;
@@ -870,6 +927,7 @@ int pic_run_profiled(pic_ctx* c, int64_t nsteps, double* diag, int32_t* sweeps, 
                      int32_t* n_stats) {
     PIC_TRACE("pic_run_profiled");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (nsteps <= 0) {
         set_err(c, "pic_run_profiled: nsteps must be > 0");
         return PIC_EINVAL;
@@ -998,6 +1056,7 @@ static int copy_run_state(pic_ctx* c, bool save) {
 int pic_set_tiling(pic_ctx* c, int64_t tile_particles, int32_t block) {
     PIC_TRACE("pic_set_tiling");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (tile_particles < 1 || block < 64 || block > 1024 || (block & (block - 1)) != 0) {
         set_err(c, "pic_set_tiling: tile_particles must be >= 1 and block a power of two in [64, 1024]");
         return PIC_EINVAL;
@@ -1047,6 +1106,7 @@ static int time_trial(pic_ctx* c, int steps, double* ms_per_step) {
 int pic_autotune(pic_ctx* c, int32_t trial_steps, pic_tuning* out) {
     PIC_TRACE("pic_autotune");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (trial_steps < 1 || trial_steps > 64) {
         set_err(c, "pic_autotune: trial_steps must be in [1, 64]");
         return PIC_EINVAL;
@@ -1150,6 +1210,7 @@ int pic_deposit(pic_ctx* c, const double* x, const double* y, int64_t P, double 
                 double* rho) {
     PIC_TRACE("pic_deposit");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (!x || !rho || P < 1 || !(w >= 0.0) || !std::isfinite(w)) {
         set_err(c, "pic_deposit: null array, P < 1 or bad w");
         return PIC_EINVAL;
@@ -1171,6 +1232,7 @@ int pic_deposit(pic_ctx* c, const double* x, const double* y, int64_t P, double 
 int pic_fieldsolve(pic_ctx* c, const double* rho, double* E) {
     PIC_TRACE("pic_fieldsolve");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (!rho || !E) {
         set_err(c, "pic_fieldsolve: null array");
         return PIC_EINVAL;
@@ -1188,6 +1250,7 @@ int pic_fieldsolve(pic_ctx* c, const double* rho, double* E) {
 int pic_drift_half(pic_ctx* c, double* x, const double* v, int64_t P, double dt) {
     PIC_TRACE("pic_drift_half");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (!x || !v || P < 1) {
         set_err(c, "pic_drift_half: null array or P < 1");
         return PIC_EINVAL;
@@ -1204,6 +1267,7 @@ int pic_push_leapfrog(pic_ctx* c, double* x, double* v, const double* E, int64_t
                       int32_t shape) {
     PIC_TRACE("pic_push_leapfrog");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (!x || !v || !E || P < 1) {
         set_err(c, "pic_push_leapfrog: null array or P < 1");
         return PIC_EINVAL;
@@ -1226,6 +1290,7 @@ int pic_midpoint_position(pic_ctx* c, const double* X, const double* V, const do
                           int64_t P, double dt) {
     PIC_TRACE("pic_midpoint_position");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (!X || !V || !v || !x || P < 1) {
         set_err(c, "pic_midpoint_position: null array or P < 1");
         return PIC_EINVAL;
@@ -1245,6 +1310,7 @@ int pic_push_midpoint(pic_ctx* c, const double* X, const double* V, const double
                       const double* E, int64_t P, double dt, int32_t shape) {
     PIC_TRACE("pic_push_midpoint");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (!X || !V || !x || !v || !E || P < 1) {
         set_err(c, "pic_push_midpoint: null array or P < 1");
         return PIC_EINVAL;
@@ -1268,6 +1334,7 @@ int pic_simpson_kick(pic_ctx* c, const double* a, const double* b, const double*
                      const double* E, int64_t P, double coef, int32_t shape) {
     PIC_TRACE("pic_simpson_kick");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (!a || !b || !v_in || !v_out || !E || P < 1) {
         set_err(c, "pic_simpson_kick: null array or P < 1");
         return PIC_EINVAL;
@@ -1290,6 +1357,7 @@ int pic_simpson_kick(pic_ctx* c, const double* a, const double* b, const double*
 int pic_wrap(pic_ctx* c, double* x, int64_t P) {
     PIC_TRACE("pic_wrap");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (!x || P < 1) {
         set_err(c, "pic_wrap: null array or P < 1");
         return PIC_EINVAL;
@@ -1305,6 +1373,7 @@ int pic_isapprox(pic_ctx* c, const double* F, const double* E, int32_t n, double
                  int32_t* result) {
     PIC_TRACE("pic_isapprox");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (!F || !E || !result || n < 1 || n > kTreeChunk) {
         set_err(c, "pic_isapprox: null array or n outside [1, 4096]");
         return PIC_EINVAL;
@@ -1324,6 +1393,7 @@ int pic_isapprox(pic_ctx* c, const double* F, const double* E, int32_t n, double
 int pic_diagnostics(pic_ctx* c, const double* E, const double* v, int64_t P, double W, double* out4) {
     PIC_TRACE("pic_diagnostics");
     if (!c) return PIC_EINVAL;
+    if (c->busy.load()) return in_flight(c);
     if (!E || !v || !out4 || P < 1) {
         set_err(c, "pic_diagnostics: null array or P < 1");
         return PIC_EINVAL;

@@ -4,7 +4,7 @@
 // and their convergence test on the device and no host round trip or graph
 // node between steps.
 //
-// At N <= 512 and a few thousand particles a step is a chain of tiny
+// At N <= 256 and a few thousand particles a step is a chain of tiny
 // dependent phases (deposit, solve, test, gather) whose cost in the captured
 // graph is launch latency, not work.  Two kernels remove it:
 //
@@ -18,8 +18,8 @@
 //                 one erf of latency), plus one diagnostics workgroup.  Each
 //                 Picard iteration deposits into LDS limbs, adds them to a
 //                 device accumulator, meets at one grid barrier, and then
-//                 every workgroup solves the field redundantly (N <= 512 is a
-//                 few microseconds in LDS), runs the same convergence test on
+//                 every workgroup solves the field redundantly (N <= 256 is a
+//                 few microseconds in one wave), runs the same convergence test on
 //                 the same bits and gathers.  The launch is cooperative, so
 //                 all workgroups are resident; every barrier wait is bounded.
 //
@@ -83,66 +83,229 @@ __device__ __forceinline__ void stage_tables(const SmallLds& L, const double2* _
     __syncthreads();
 }
 
-// Radix-2 decimation-in-time stages on re/im (input in bit-reversed order,
-// output natural), one butterfly per thread and stage: the stage-by-stage
-// transform of the CPU restatement, butterfly for butterfly (twiddle of
-// stage s at table entry 2^(s-1) - 1 + j, conjugated for the inverse).
-__device__ __forceinline__ void lds_radix2(double* re, double* im, const double2* __restrict__ tw, int N, int log2N,
-                                           bool inverse) {
+// ---------------------------------------------------------------------------
+// The solve in one wave.  Element i of an N-point (N <= 256) transform lives in
+// lane i % 64, slot i / 64 of wave 0.  A radix-2 stage with half-size h < 64
+// pairs lanes l and l ^ h (lane shuffles), with h >= 64 slots q and q + h/64
+// of one lane; every butterfly is the stage-by-stage transform's (operands
+// b = the upper element, a = the lower, t = b w, a + t and a - t; twiddle of
+// stage s at table entry 2^(s-1) - 1 + j, conjugated for the inverse), so
+// the result is that of the CPU restatement bit for bit.
+// ---------------------------------------------------------------------------
+constexpr int kWaveSlots = kSmallMaxN / 64;
+
+__device__ __forceinline__ void wave_radix2(double (&re)[kWaveSlots], double (&im)[kWaveSlots],
+                                            const double2* __restrict__ tw, int N, int log2N, bool inverse) {
+    const int l = threadIdx.x & 63;
+    const int E = N >= 64 ? N >> 6 : 1;
     for (int st = 1; st <= log2N; ++st) {
         const int h = 1 << (st - 1);
-        for (int b = threadIdx.x; b < N / 2; b += blockDim.x) {
-            const int j = b & (h - 1);
-            const int i1 = ((b >> (st - 1)) << st) + j, i2 = i1 + h;
-            const double2 w = tw[h - 1 + j];
+        if (h < 64) {
+            const bool lower = (l & h) == 0;
+            const double2 w = tw[h - 1 + (l & (h - 1))];
             const double wr = w.x, wi = inverse ? -w.y : w.y;
-            const double ar = re[i2], ai = im[i2];
-            const double tr = ar * wr - ai * wi;
-            const double tq = ar * wi + ai * wr;
-            const double ur = re[i1], ui = im[i1];
-            re[i1] = ur + tr;
-            im[i1] = ui + tq;
-            re[i2] = ur - tr;
-            im[i2] = ui - tq;
+#pragma unroll
+            for (int q = 0; q < kWaveSlots; ++q) {
+                if (q >= E) break;
+                const double pr = __shfl_xor(re[q], h, 64), pi = __shfl_xor(im[q], h, 64);
+                const double br = lower ? pr : re[q], bi = lower ? pi : im[q];
+                const double ar = lower ? re[q] : pr, ai = lower ? im[q] : pi;
+                const double tr = br * wr - bi * wi;
+                const double tq = br * wi + bi * wr;
+                re[q] = lower ? ar + tr : ar - tr;
+                im[q] = lower ? ai + tq : ai - tq;
+            }
+        } else {
+            const int hs = h >> 6;
+#pragma unroll
+            for (int q = 0; q < kWaveSlots; ++q) {
+                if ((q & hs) || q + hs >= E) continue;
+                const double2 w = tw[h - 1 + ((q * 64 + l) & (h - 1))];
+                const double wr = w.x, wi = inverse ? -w.y : w.y;
+#pragma unroll
+                for (int qq = 0; qq < kWaveSlots; ++qq) {
+                    if (qq != q + hs) continue;
+                    const double br = re[qq], bi = im[qq];
+                    const double tr = br * wr - bi * wi;
+                    const double tq = br * wi + bi * wr;
+                    const double ar = re[q], ai = im[q];
+                    re[q] = ar + tr;
+                    im[q] = ai + tq;
+                    re[qq] = ar - tr;
+                    im[qq] = ai - tq;
+                }
+            }
         }
-        __syncthreads();
     }
 }
 
-// The solve of the charge staged in re/im (bit-reversed):
+// The canonical reduction tree (tree256 over lanes t < 256 that each fold the
+// elements t, t + 256, ... in order from +0.0) over the N values x of wave 0
+// (N <= 256 here; the loop also folds slot u + 4, which N = 512 would fill):
+// tree lane t = l + 64u folds slot u of lane l, the two
+// halving steps combine (u, u + 2) and then (0, 1) in lane l, and the last
+// six steps are tree256's own shuffles.  MAX: the NaN-propagating maximum.
+// Every value is >= 0 (squares, magnitudes, flags), so folding +0.0 for a
+// missing element leaves a sum or maximum unchanged.  Lane 0's result is
+// returned to every lane.
+template <bool MAX>
+__device__ __forceinline__ double wave_canon(const double (&x)[kWaveSlots], int N) {
+    auto op = [](double a, double b) { return MAX ? nanmax(a, b) : a + b; };
+    const int l = threadIdx.x & 63;
+    double P[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        P[u] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int q = u + 4 * k;
+            if (q < kWaveSlots && q * 64 + l < N) P[u] = op(P[u], x[q]);
+        }
+    }
+    double r = op(op(P[0], P[2]), op(P[1], P[3]));
+#pragma unroll
+    for (int h = 32; h > 0; h >>= 1) r = op(r, __shfl_down(r, h, 64));
+    return __shfl(r, 0, 64);
+}
+
+// The solve of the charge staged in re/im (bit-reversed), in wave 0:
 // E = real(ifft((xi = fft(rho) ./ ik; xi[1] *= 0; xi))) into Eld, with ik and
 // Julia's division by (0, 2 pi k') as in k_solve; with TEST also the
-// convergence test isapprox(Fl, E) of the midpoint loop (Fl then takes E).
-// Returns the test's result; emax gets max |E|.
+// convergence test isapprox(Fl, E) of the midpoint loop, with block_isapprox's
+// norms and decisions (Fl then takes E).  Returns the test's result; emax
+// gets max |E|.  Every thread of the block calls it.
 template <bool TEST>
 __device__ __forceinline__ bool small_solve(const SmallLds& L, int N, int log2N, double rtol, double& emax) {
-    lds_radix2(L.re, L.im, L.tw, N, log2N, false);
-    for (int m = threadIdx.x; m < N; m += blockDim.x) {
-        const double t = m == 0 ? L.xi[1] : (m <= N / 2 ? L.xi[m] : -L.xi[N - m]);
-        double a = L.im[m] * t;
-        double b = -(L.re[m] * t);
-        if (m == 0) {
-            a = a * 0.0;
-            b = b * 0.0;
+    if (threadIdx.x < 64) {
+        const int l = threadIdx.x;
+        double re[kWaveSlots], im[kWaveSlots];
+#pragma unroll
+        for (int q = 0; q < kWaveSlots; ++q) {
+            const int i = q * 64 + l;
+            re[q] = i < N ? L.re[i] : 0.0;
+            im[q] = i < N ? L.im[i] : 0.0;
         }
-        const int q = (int)brev((uint32_t)m, log2N);
-        L.bre[q] = a;
-        L.bim[q] = b;
+        wave_radix2(re, im, L.tw, N, log2N, false);
+#pragma unroll
+        for (int q = 0; q < kWaveSlots; ++q) {
+            const int m = q * 64 + l;
+            if (m >= N) continue;
+            const double t = m == 0 ? L.xi[1] : (m <= N / 2 ? L.xi[m] : -L.xi[N - m]);
+            double a = im[q] * t;
+            double b = -(re[q] * t);
+            if (m == 0) {
+                a = a * 0.0;
+                b = b * 0.0;
+            }
+            const int p = (int)brev((uint32_t)m, log2N);
+            L.bre[p] = a;
+            L.bim[p] = b;
+        }
+        // the permutation stays inside the wave: LDS operations of one wave
+        // complete in order; the fences keep the compiler from moving them
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int q = 0; q < kWaveSlots; ++q) {
+            const int i = q * 64 + l;
+            re[q] = i < N ? L.bre[i] : 0.0;
+            im[q] = i < N ? L.bim[i] : 0.0;
+        }
+        wave_radix2(re, im, L.tw, N, log2N, true);
+        const double scale = 1.0 / (double)N;
+        double e[kWaveSlots];
+#pragma unroll
+        for (int q = 0; q < kWaveSlots; ++q) {
+            const int i = q * 64 + l;
+            e[q] = re[q] * scale;
+            if (i < N) L.Eld[i] = e[q];
+        }
+        bool conv = false;
+        double me;
+        if (TEST) {
+            double f[kWaveSlots], x[kWaveSlots];
+#pragma unroll
+            for (int q = 0; q < kWaveSlots; ++q) {
+                const int i = q * 64 + l;
+                f[q] = i < N ? L.Fl[i] : 0.0;
+            }
+            // the three norms of block_norms3: d = F - E, F, E
+            double m3[3], s3[3];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+#pragma unroll
+                for (int q = 0; q < kWaveSlots; ++q) {
+                    const double xv = v == 0 ? f[q] - e[q] : (v == 1 ? f[q] : e[q]);
+                    x[q] = fabs(xv);
+                }
+                m3[v] = wave_canon<true>(x, N);
+#pragma unroll
+                for (int q = 0; q < kWaveSlots; ++q) {
+                    const double xv = v == 0 ? f[q] - e[q] : (v == 1 ? f[q] : e[q]);
+                    x[q] = xv * xv;
+                }
+                s3[v] = wave_canon<false>(x, N);
+            }
+            double nrm[3];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                const bool direct = m3[v] == 0.0 || isinf(m3[v]);
+                const bool scaled = !direct && !(isfinite((double)N * m3[v] * m3[v]) && m3[v] * m3[v] != 0.0);
+                if (scaled) {
+#pragma unroll
+                    for (int q = 0; q < kWaveSlots; ++q) {
+                        const double xv = v == 0 ? f[q] - e[q] : (v == 1 ? f[q] : e[q]);
+                        const double y = fabs(xv) / m3[v];
+                        x[q] = y * y;
+                    }
+                    s3[v] = wave_canon<false>(x, N);
+                }
+                nrm[v] = direct ? m3[v] : (scaled ? m3[v] * sqrt(s3[v]) : sqrt(s3[v]));
+            }
+            me = m3[2];
+            const double dn = nrm[0], nf = nrm[1], ne = nrm[2];
+            if (isfinite(dn)) {
+                double mx = nf > ne ? nf : ne;
+                if (nf != nf || ne != ne) mx = __longlong_as_double(0x7ff8000000000000LL);
+                const double tol = rtol * mx;
+                double lim = 0.0 > tol ? 0.0 : tol;
+                if (tol != tol) lim = tol;
+                conv = dn <= lim;
+            } else {
+#pragma unroll
+                for (int q = 0; q < kWaveSlots; ++q) {
+                    const double a = f[q], b = e[q];
+                    bool good = a == b;
+                    if (!good && isfinite(a) && isfinite(b)) {
+                        const double ma = fabs(a) > fabs(b) ? fabs(a) : fabs(b);
+                        const double t = rtol * ma;
+                        good = fabs(a - b) <= (0.0 > t ? 0.0 : t);
+                    }
+                    x[q] = good ? 0.0 : 1.0;
+                }
+                conv = wave_canon<true>(x, N) == 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < kWaveSlots; ++q) {
+                const int i = q * 64 + l;
+                if (i < N) L.Fl[i] = e[q];
+            }
+        } else {
+            double x[kWaveSlots];
+#pragma unroll
+            for (int q = 0; q < kWaveSlots; ++q) x[q] = fabs(e[q]);
+            me = wave_canon<true>(x, N);
+        }
+        if (l == 0) {
+            L.red[0] = me;
+            L.red[1] = conv ? 1.0 : 0.0;
+        }
     }
     __syncthreads();
-    lds_radix2(L.bre, L.bim, L.tw, N, log2N, true);
-    const double scale = 1.0 / (double)N;
-    for (int j = threadIdx.x; j < N; j += blockDim.x) L.Eld[j] = L.bre[j] * scale;
+    emax = L.red[0];
+    const bool conv = L.red[1] != 0.0;
     __syncthreads();
-    bool conv = false;
-    if (TEST) {
-        conv = block_isapprox(L.red, [&](int i) { return L.Fl[i]; }, [&](int i) { return L.Eld[i]; }, N, rtol, 0.0,
-                              emax);
-        for (int j = threadIdx.x; j < N; j += blockDim.x) L.Fl[j] = L.Eld[j];
-        __syncthreads();
-    } else {
-        emax = block_maxabs(L.red, N, [&](int64_t i) { return L.Eld[i]; });
-    }
     return conv;
 }
 

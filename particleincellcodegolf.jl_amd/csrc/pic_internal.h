@@ -139,7 +139,7 @@ int configure_kernels_mixed();
 
 // kernels_small.hip: the whole run of a small single FP64 instance in one
 // launch (the step loop, the Picard sweeps and their test on the device)
-constexpr int kSmallMaxN = 512;       // every workgroup holds the grid, its solve and its tables in LDS
+constexpr int kSmallMaxN = 256;       // the solve runs in one wave, 4 grid cells per lane
 constexpr int kSmallNgpSlots = 16;    // NGP: particles per thread of the one 512-thread workgroup (P <= 8192)
 constexpr int kSmallMaxSlots = 4;     // Gaussian: particles per 16-lane segment
 constexpr int kSmallMaxGroups = 128;  // Gaussian: particle workgroups (32 segments each; P <= 16384)

@@ -36,7 +36,7 @@ expression.
 module PICGolf
 
 export Context, PicConfig, deposit!, fieldsolve!, drift!, midpoint_position!, wrap!,
-       diagnostics, upload!, download, run!, quiet_start, autotune!, set_tiling!, tiling, PicTuning,
+       diagnostics, upload!, download, run!, run_async!, sync!, quiet_start, autotune!, set_tiling!, tiling, PicTuning,
        simpson_kick!, Pic1D2VConfig, Context1D2V, fields,
        Pic2DConfig, Context2D, gather, boris!,
        Pic2D3VSpecies, Pic2D3VConfig, Context2D3V, reset!, field, warmup!, loop!, lg_field, lg_field!, lg_bins,
@@ -78,6 +78,7 @@ PicConfig(; N, P, W, dt, w=W / P * N, shape=:gauss7, scheme=:midpoint, precision
 mutable struct Context
     ptr::Ptr{Cvoid}
     cfgs::Vector{PicConfig}
+    pending::Any        # (D, S) of a run_async! run, kept reachable until sync!
 end
 
 lasterror(p::Ptr{Cvoid}) = unsafe_string(ccall((:pic_last_error, libpicgolf), Cstring, (Ptr{Cvoid},), p))
@@ -97,7 +98,7 @@ function Context(cfgs::Vector{PicConfig})
     rc = ccall((:pic_create_batch, libpicgolf), Cint, (Ref{Ptr{Cvoid}}, Ptr{PicConfig}, Int32),
                h, cfgs, length(cfgs))
     rc == 0 || error("pic_create_batch: ", lasterror(C_NULL), " (code ", rc, ")")
-    ctx = Context(h[], cfgs)
+    ctx = Context(h[], cfgs, nothing)
     finalizer(destroy!, ctx)
     ctx
 end
@@ -287,6 +288,31 @@ function run!(ctx::Context, nsteps::Integer)
                                 ctx.ptr, nsteps, D, S)
     check(ctx, rc, "pic_run")
     (permutedims(D, (2, 1, 3)), S)
+end
+
+"""
+    run_async!(ctx, nsteps)
+
+Starts `nsteps` steps on a thread of the library (`pic_run_async`) and returns
+at once; `sync!(ctx)` waits and returns what `run!` would.  Other calls on
+the context fail until then.
+"""
+function run_async!(ctx::Context, nsteps::Integer)
+    n = length(ctx.cfgs)
+    D = zeros(4, nsteps, n)
+    S = zeros(Int32, nsteps, n)
+    rc = ccall((:pic_run_async, libpicgolf), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Int32}),
+               ctx.ptr, nsteps, D, S)
+    check(ctx, rc, "pic_run_async")
+    ctx.pending = (D, S)
+    nothing
+end
+
+function sync!(ctx::Context)
+    rc = ccall((:pic_sync, libpicgolf), Cint, (Ptr{Cvoid},), ctx.ptr)
+    p, ctx.pending = ctx.pending, nothing
+    check(ctx, rc, "pic_sync")
+    p === nothing ? nothing : (permutedims(p[1], (2, 1, 3)), p[2])
 end
 
 "Mirror of `struct pic_tuning` in include/pic.h."

@@ -30,7 +30,7 @@ EXPORTS = [
     "pic_deposit", "pic_fieldsolve", "pic_drift_half", "pic_push_leapfrog",
     "pic_midpoint_position", "pic_push_midpoint", "pic_simpson_kick", "pic_wrap", "pic_isapprox",
     "pic_diagnostics", "pic_set_tiling", "pic_get_tiling", "pic_autotune", "pic_set_run_path",
-    "pic_get_run_path", "pic_deposit_noalloc", "pic_noalloc_last_error",
+    "pic_get_run_path", "pic_deposit_noalloc", "pic_noalloc_last_error", "pic_run_async", "pic_sync",
 ]
 
 # every symbol include/pic2d.h declares
@@ -189,6 +189,8 @@ def load():
         "pic_diagnostics": [vp, vp, vp, i64, d, vp],
         "pic_set_tiling": [vp, i64, i32],
         "pic_set_run_path": [vp, i32, i32],
+        "pic_run_async": [vp, i64, vp, vp],
+        "pic_sync": [vp],
         "pic_deposit_noalloc": [vp, vp, i64, d, i32, i32, vp, vp],
         "pic_get_run_path": [vp, vp, vp],
         "pic_get_tiling": [vp, ctypes.POINTER(i64), ip, ip],

@@ -214,6 +214,26 @@ class Context:
         self._check(self._lib.pic_run(self._h, int(nsteps), L.ptr(diag), L.ptr(sweeps)), "pic_run")
         return diag, sweeps
 
+    def run_async(self, nsteps, want_diag=True):
+        """Starts nsteps steps and returns at once; sync() waits for them and
+        returns (diag, sweeps).  Other calls raise until then.  The arrays the
+        run writes into are held here until sync(), and only a call that
+        started a run replaces them."""
+        n = len(self.configs)
+        D = np.zeros((n, nsteps, 4)) if want_diag else None
+        S = np.zeros((n, nsteps), dtype=np.int32) if want_diag else None
+        self._check(self._lib.pic_run_async(self._h, int(nsteps), L.ptr(D), L.ptr(S)), "pic_run_async")
+        self._pending = (D, S)
+
+    def sync(self):
+        """Waits for the run_async run; (diag, sweeps), or (None, None) when
+        none was started."""
+        try:
+            self._check(self._lib.pic_sync(self._h), "pic_sync")
+        finally:
+            out, self._pending = getattr(self, "_pending", None) or (None, None), None
+        return out
+
     # -- run path ------------------------------------------------------------
     RUN_PATHS = {"auto": L.RUN_AUTO, "graph": L.RUN_GRAPH, "persistent": L.RUN_PERSISTENT}
 

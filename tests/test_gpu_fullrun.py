@@ -109,13 +109,13 @@ CASES = [
     # (N, P, shape, scheme, rtol, workgroups, steps): ragged P, extreme N,
     # one workgroup, several slots per lane group, the leapfrog Gaussian
     (8, 37, "gauss7", "midpoint", 1e-8, 0, 30),
-    (512, 3000, "gauss7", "midpoint", 1e-10, 0, 12),
+    (256, 3000, "gauss7", "midpoint", 1e-10, 0, 12),
     (64, 1000, "gauss6", "midpoint", 4 * EPS, 8, 10),
     (64, 16384, "gauss7", "midpoint", 1e-8, 0, 6),
     (32, 2047, "gauss7", "midpoint", 1e-8, 16, 20),
     (128, 8192, "gauss6", "leapfrog", 1e-8, 0, 40),
     (16, 100, "gauss7", "leapfrog", 1e-8, 1, 25),
-    (512, 8192, "ngp", "leapfrog", 1e-8, 0, 50),
+    (256, 8192, "ngp", "leapfrog", 1e-8, 0, 50),
     (8, 1, "ngp", "leapfrog", 1e-8, 0, 20),
     (64, 777, "ngp", "leapfrog", 1e-8, 0, 33),
 ]
@@ -163,6 +163,28 @@ def test_persistent_continues_across_calls_uploads_and_paths(pg, orc):
     ref = oracle_run(orc, cfg, x, v2, 12, E=E0)
     assert S4.max() > S1.max()
     assert_run_equal((*got, D4[0], S4[0]), ref)
+
+
+@pytest.mark.parametrize("path", ["graph", "persistent"])
+def test_async_run_matches_golden_and_refuses_calls_in_flight(pg, path):
+    """pic_run_async + pic_sync give the bits of pic_run; while the run is in
+    flight every other call returns PIC_ESTATE and leaves the run alone."""
+    from picgolf import configs
+    from picgolf._lib import PicError
+    cfg, x, v, T = configs.quiet_two_stream()
+    with pg.Context(cfg) as c:
+        c.set_run_path(path)
+        c.upload(x, v)
+        c.run_async(T)
+        # 8192 steps take far longer than these calls
+        for call in (c.download, lambda: c.run(1), lambda: c.upload(x, v), lambda: c.run_async(1)):
+            with pytest.raises(PicError) as e:
+                call()
+            assert e.value.code == -5
+        D, S = c.sync()
+        assert c.sync() == (None, None)
+        got = c.download()
+    check_golden("quiet_two_stream_8192.npz", (*got, D[0], S[0]))
 
 
 def test_run_path_selection(pg):
