@@ -28,7 +28,8 @@ namespace pic {
 constexpr int kHalo = kStencil32 + 1;
 constexpr int kWin = 2 * kStencil32 + 1;
 __host__ __device__ constexpr int pad_cells(int N) { return N + 2 * kHalo + 4; }
-// LDS of the push: u64 accumulator and FP32 E over the padded cells, then 32
+// LDS of the push: the charge accumulator (32-bit lo and hi words per padded
+// cell) and FP32 E over the padded cells, then 32
 // doubles for the block's diagnostics partials
 int push32_lds_bytes(int N) { return 12 * pad_cells(N) + 8 * 4 + 512; }
 
@@ -80,24 +81,36 @@ __device__ __forceinline__ uint32_t quanta(float w30) {
 }
 
 // Deposit of one particle's stencil: 11 exact integer adds into the block's
-// LDS accumulator.  Called by every lane of a wave (dep marks lanes with a
+// LDS accumulator, a slot's count being hi * 2^32 + lo.  Each add goes to the
+// 32-bit lo word and returns its old value; an add that wraps lo (old + q <
+// old, q < 2^32) adds one to hi.  A 32-bit LDS atomic costs fewer cycles than
+// a 64-bit one on random slots, and the rare carry adds run with few lanes.
+// A slot takes at most 2^30 quanta per particle of the tile, so hi counts at
+// most a quarter of the tile's particles.  Called by every lane of a wave (dep marks lanes with a
 // stencil to deposit).
-__device__ __forceinline__ void lane_deposit(u64* __restrict__ acc, const Stencil32& s, bool dep) {
+__device__ __forceinline__ void lane_deposit(uint32_t* __restrict__ lo, uint32_t* __restrict__ hi,
+                                             const Stencil32& s, bool dep) {
     if (!dep) return;
-    u64* a = acc + s.i0;
-#define PIC_DEP(J) atomicAdd(&a[J], (u64)quanta<J>(s.w[J]));
+    uint32_t q[kWin], old[kWin];
+#define PIC_DEP(J)                       \
+    q[J] = quanta<J>(s.w[J]);            \
+    old[J] = atomicAdd(&lo[s.i0 + J], q[J]);
     PIC_DEP(0) PIC_DEP(1) PIC_DEP(2) PIC_DEP(3) PIC_DEP(4) PIC_DEP(5) PIC_DEP(6) PIC_DEP(7) PIC_DEP(8) PIC_DEP(9)
     PIC_DEP(10)
 #undef PIC_DEP
+#pragma unroll
+    for (int j = 0; j < kWin; ++j)
+        if (old[j] + q[j] < old[j]) atomicAdd(&hi[s.i0 + j], 1u);
 }
 
 // Adds this block's halo-padded LDS charge into the instance's per-cell
 // accumulator (device-scope integer atomics; cell c collects every slot
 // k == c + kHalo (mod N)).
-__device__ __forceinline__ void flush_charge32(const u64* acc, int N, int NPAD, u64* __restrict__ racc) {
+__device__ __forceinline__ void flush_charge32(const uint32_t* lo, const uint32_t* hi, int N, int NPAD,
+                                               u64* __restrict__ racc) {
     for (int c = threadIdx.x; c < N; c += blockDim.x) {
         u64 v = 0;
-        for (int kk = (c + kHalo) % N; kk < NPAD; kk += N) v += acc[kk];
+        for (int kk = (c + kHalo) % N; kk < NPAD; kk += N) v += ((u64)hi[kk] << 32) + lo[kk];
         if (v) atomicAdd(racc + c, v);
     }
 }
@@ -125,7 +138,9 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     const bool last = (MODE == 1) && (S.last_sweep == k);
     const int32_t N = I.N, nmask = I.nmask;
     const int NPAD = pad_cells(N);
-    u64* acc = lds;
+    u64* acc = lds;  // the lo words of the NPAD slots, then their hi words
+    uint32_t* acc_lo = reinterpret_cast<uint32_t*>(acc);
+    uint32_t* acc_hi = acc_lo + NPAD;
     float* Eld = reinterpret_cast<float*>(lds + NPAD);
     if (MODE == 1)
         for (int i = threadIdx.x; i < NPAD; i += blockDim.x)
@@ -206,7 +221,7 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
             // once the Picard iterate settles the next centre repeats bit for bit
             if (dep && (MODE == 0 || cdep != cq)) stencil32(cdep, shift, s);
         }
-        if (deposit) lane_deposit(acc, s, dep);
+        if (deposit) lane_deposit(acc_lo, acc_hi, s, dep);
     }
     if (last) {
         // the step's kinetic diagnostics: one exact partial pair per tile
@@ -237,7 +252,7 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
         if (!prime) return;
     }
     __syncthreads();
-    flush_charge32(acc, N, NPAD, racc + I.red0);
+    flush_charge32(acc_lo, acc_hi, N, NPAD, racc + I.red0);
 }
 
 void launch_mid_push32(const RunBuffers& b, int mode, int k, bool last_slot, hipStream_t s) {

@@ -7,13 +7,17 @@ restatement (the oracle, oracle/picoracle.c), in the build container:
   ngp_two_stream_1024.npz    N=128, P=8192, NGP leapfrog, dt=1/4N, W=200,
                              seeded uniform x, v=-1/+1 by half, 1024 steps
                              (src/NGPFourier.jl:1-6)
+  es2d3v_256_500.npz         the electrostatic 2D3V workload, 256 x 256 cells,
+                             64 particles per cell, the script's shape pairing,
+                             FP64, 500 steps (src/Electrostatic2D3V.jl:24-57);
+                             about 5 minutes on 8 threads
 
 Each holds the sweep counts of every step, the D[t,1:4] trace every `stride`
 steps, the final E and x, v (every `pstride`-th particle), and SHA-256 digests of the whole D trace and of
 the final state, so a GPU run is compared with the oracle bit for bit over the
 full run without the oracle running on the GPU host.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [1d] [2d]
 """
 import hashlib
 import os
@@ -43,8 +47,26 @@ def write(name, cfg, shape, scheme, x, v, nsteps, stride, pstride=1):
     print(f"{path}: {os.path.getsize(path)} bytes, mean sweeps {So.mean():.3f}")
 
 
-def main():
+def write2d(name, cfg, st, nsteps, pstride):
+    import oracle
+    x, y, vx, vy, vz, Ex, Ey, K = oracle.run2d(cfg.NX, cfg.NY, cfg.dt, cfg.B0, cfg.w, *st, nsteps, ns=cfg.ns,
+                                               textbook=1 if cfg.pairing == "textbook" else 0, nthreads=8)
+    parts = {"x": x, "y": y, "vx": vx, "vy": vy, "vz": vz}
+    path = os.path.join(HERE, name)
+    np.savez_compressed(path, nsteps=nsteps, pstride=pstride, K=K, K_sha256=digest(K),
+                        Ex_sha256=digest(Ex), Ey_sha256=digest(Ey), Ex_row=Ex[::37].copy(),
+                        **{k: a[::pstride] for k, a in parts.items()},
+                        **{k + "_sha256": digest(a) for k, a in parts.items()})
+    print(f"{path}: {os.path.getsize(path)} bytes")
+
+
+def main(which=("1d", "2d")):
     from picgolf import configs
+    if "2d" in which:
+        cfg, st, T = configs.es2d3v(precision="fp64")
+        write2d("es2d3v_256_500.npz", cfg, st, T, pstride=4099)
+    if "1d" not in which:
+        return
     cfg, x, v, T = configs.quiet_two_stream()
     write("quiet_two_stream_8192.npz", cfg, 7, 1, x, v, T, 8)
     cfg, x, v, T = configs.ngp_two_stream()
@@ -52,4 +74,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]) or ("1d", "2d"))

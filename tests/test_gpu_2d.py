@@ -133,6 +133,34 @@ def test_fp64_run_bitwise_full_size(pg, orc, capsys):
     assert same(K, ref[7])
 
 
+def test_fp64_full_run_matches_golden(pg):
+    """The 256 x 256, 64 particles-per-cell workload over its whole 500 steps
+    bit for bit against the committed golden fixture of the CPU restatement
+    (tests/golden/make_golden.py): every diagnostics row, the final particles
+    (SHA-256 of each array, plus a sample) and the final fields."""
+    import hashlib
+    import os
+    from picgolf import configs
+
+    def digest(a):
+        return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "es2d3v_256_500.npz"))
+    cfg, st, T = configs.es2d3v(precision="fp64")
+    assert T == int(g["nsteps"])
+    with pg.Context2D(cfg) as c:
+        c.upload(*st)
+        K = c.run(T)
+        got = c.download()
+    assert same(K, g["K"]) and digest(K) == str(g["K_sha256"])
+    ps = int(g["pstride"])
+    for name, a in zip(("x", "y", "vx", "vy", "vz"), got[:5]):
+        assert same(a[::ps], g[name]), name
+        assert digest(a) == str(g[name + "_sha256"]), name
+    assert same(got[5][::37], g["Ex_row"])
+    assert digest(got[5]) == str(g["Ex_sha256"]) and digest(got[6]) == str(g["Ey_sha256"])
+
+
 # Field-energy tolerance of the mixed path against FP64 over 200 steps.  The
 # reference pairing's shape jumps by two cells when a particle crosses a cell
 # edge, so a position difference of ~1e-7 cell (FP32 velocity, 2^-20 weight

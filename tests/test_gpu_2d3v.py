@@ -162,10 +162,13 @@ def test_sort_bitwise(pg, orc):
 def test_sort_after_steps_follows_the_caller_order(pg, orc):
     """The context regroups its particles by tile while it steps; sort!
     afterwards permutes the caller's order (the downloaded one), and a second
-    run continues from it."""
+    run continues from the sorted order: its rows (whose particle sums follow
+    that order), gathered field and particles match the oracle's run from
+    the same state."""
     rng = np.random.default_rng(16)
     pl = plasma(rng, 20000, 2.0, 0.75)
-    with pg.Context2D3V(config(pl)) as c:
+    cfg = config(pl)
+    with pg.Context2D3V(cfg) as c:
         for s, (_, xyv) in enumerate(pl):
             c.upload(s, xyv)
         c.run(3)
@@ -174,8 +177,20 @@ def test_sort_after_steps_follows_the_caller_order(pg, orc):
         got = c.download(0)
         ref, operm = orc.sort23(before, 2.0 / 32, 0.75 / 16)
         assert np.array_equal(perm, operm) and same(got, ref)
-        r = c.run(2)
-        assert np.all(np.isfinite(r))
+        other = c.download(1)
+        Eacc = c.field()
+        r = c.run(4)
+        after = [c.download(s) for s in range(2)]
+        Eacc2 = c.field()
+    sp = oracle_species(pl)
+    sp[0]["xyv"], sp[1]["xyv"] = got, other
+    sp2, (Ex, Ey), orow, _, bad = orc.run23(32, 16, 2.0, 0.75, cfg.dt, cfg.B0, sp, 4, t0=3, buffer=3, ntskip=3,
+                                            ngskip=2, Eacc=Eacc, nthreads=8)
+    assert not bad
+    assert r.shape == orow.shape == (2, 8) and same(r, orow)
+    assert same(Eacc2[0], Ex) and same(Eacc2[1], Ey)
+    for s in range(2):
+        assert same(after[s], sp2[s]["xyv"])
 
 
 def test_upload_download_roundtrip_and_reset(pg, orc):

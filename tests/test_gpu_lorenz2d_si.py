@@ -93,6 +93,38 @@ def test_loop_without_diagnostics_and_thetas(pg, orc, theta, theta_source, dt):
         assert same(c.lg_bins(nbins=6)[:, 4:], p[5][:, 4:])
 
 
+def test_regroupings_over_many_steps_bitwise(pg, orc):
+    """The context regroups its particles (and rotates the buffers that hold
+    them) before the first step and every 16 steps: 20 diagnosed steps of
+    run(), and 17 steps of loop() followed by 3 of run(), cross a mid-run
+    regroup and match the oracle, which keeps the caller's order, in the
+    rows, state, grid, particles, bins and iteration counts."""
+    NX, NY, Lx, Ly, dt, b = 32, 32, 1.0, 1.0, 0.02, 3
+    B0 = (0.5, 0.0, 0.3)
+    sp = make_species(2048, Lx, Ly, seed=9)
+    sp1, F1, g1 = orc.lg_si_warmup(NX, NY, Lx, Ly, dt, sp, b)
+    with context(pg, NX, NY, Lx, Ly, dt, B0, sp, theta=0.8, theta_source=0.1, buffer=b) as c:
+        c.warmup()
+        rows = c.run(20)
+        o = orc.lg_si_run(NX, NY, Lx, Ly, dt, np.array(B0), sp1, F1, g1, np.zeros((6, NY, NX)), 20, theta=0.8,
+                          theta_source=0.1, buffer=b)
+        check(c, o[0], o[1], o[2], o[3])
+        assert same(rows, o[4]) and same(c.lg_bins(), o[5])
+        assert np.array_equal(c.lg_iterations(), o[6])
+    with context(pg, NX, NY, Lx, Ly, dt, B0, sp, theta=0.8, theta_source=0.1, buffer=b) as c:
+        c.warmup()
+        c.loop(17)
+        o = orc.lg_si_run(NX, NY, Lx, Ly, dt, np.array(B0), sp1, F1, g1, np.zeros((6, NY, NX)), 17, theta=0.8,
+                          theta_source=0.1, buffer=b, diagnose=False)
+        check(c, o[0], o[1], o[2], o[3])
+        assert np.array_equal(c.lg_iterations(), o[6])
+        rows = c.run(3)
+        p = orc.lg_si_run(NX, NY, Lx, Ly, dt, np.array(B0), o[0], o[1], o[2], o[3], 3, theta=0.8,
+                          theta_source=0.1, buffer=b, t0=17)
+        check(c, p[0], p[1], p[2], p[3])
+        assert same(rows, p[4]) and np.array_equal(c.lg_iterations(), p[6])
+
+
 def test_step_from_an_arbitrary_state(pg, orc):
     NX, NY, Lx, Ly, dt, b = 32, 32, 1.0, 1.0, 0.02, 4
     B0 = (0.3, -0.2, 0.7)
