@@ -32,6 +32,8 @@ def parse():
     ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"])
     ap.add_argument("--particles", type=int, default=2 ** 24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-full-run", action="store_true",
+                    help="skip the 2000-step mixed-vs-FP64 accuracy run of the headline workload")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the ensemble and 2D3V workloads reported beside the headline")
     ap.add_argument("--profile-steps", type=int, default=3)
@@ -229,31 +231,55 @@ def bench_ensemble(steps=10, warmup=10, n=512, autotune=True):
 
 def bench_2d(steps=500):
     """The electrostatic 2D3V workload (256 x 256, 64 particles per cell), the
-    whole 500-step run timed, mixed and FP64."""
+    whole 500-step run timed, mixed and FP64, with the script's area pairing
+    and with the textbook (continuous) one.  The script's pairing makes the
+    run chaotic in the particle positions; an FP64 run from positions
+    displaced at random by 1e-8 of a cell is reported beside it as the
+    control of what any inexact arithmetic can reach."""
     import numpy as np
     import picgolf
     from picgolf import configs
-    res = {}
-    for prec in ("mixed", "fp64"):
-        cfg, st, _ = configs.es2d3v(precision=prec)
-        with picgolf.Context2D(cfg) as c:
-            c.upload(*st)
-            c.run(2)                 # compile-free warm start: graphs captured, caches warm
-            c.upload(*st)
-            t0 = time.perf_counter()
-            K = c.run(steps)
-            secs = time.perf_counter() - t0
-        res[prec] = dict(pushes=cfg.P * steps / secs, ms=secs / steps * 1e3, K=K, P=cfg.P)
-    f, m = res["fp64"], res["mixed"]
-    rel = np.abs(m["K"][:, 0] - f["K"][:, 0]) / np.abs(f["K"][:, 0])
-    return {"workload": "electrostatic 2D3V 256x256, 64 particles per cell (4194304), B0 = pi/2 along x, "
-                        "dt=1/1024, reference area pairing",
-            "mixed_pushes_per_s": m["pushes"], "fp64_pushes_per_s": f["pushes"],
-            "mixed_over_fp64": m["pushes"] / f["pushes"], "mixed_ms_per_step": m["ms"],
-            "fp64_ms_per_step": f["ms"], "steps": steps,
-            "field_energy_rel_err_mixed_vs_fp64_max": float(rel.max()),
-            "field_energy_rel_err_mixed_vs_fp64_first_100_steps": float(rel[:50].max()),
-            "momentum_x_drift_mixed": float(np.abs(m["K"][:, 3] - m["K"][0, 3]).max())}
+
+    def one(pairing, dx_cells=0.0, precs=("mixed", "fp64")):
+        res = {}
+        for prec in precs:
+            cfg, st, _ = configs.es2d3v(precision=prec, pairing=pairing)
+            x, y = st[0], st[1]
+            if dx_cells:
+                rng = np.random.default_rng(77)
+                x = x + dx_cells * rng.uniform(-1, 1, x.size) / cfg.NX
+                y = y + dx_cells * rng.uniform(-1, 1, y.size) / cfg.NY
+                x = np.where(x > 1.0, x - 1.0, np.where(x <= 0.0, x + 1.0, x))
+                y = np.where(y > 1.0, y - 1.0, np.where(y <= 0.0, y + 1.0, y))
+            with picgolf.Context2D(cfg) as c:
+                c.upload(x, y, *st[2:])
+                c.run(2)                 # warm start: graphs captured, caches warm
+                c.upload(x, y, *st[2:])
+                t0 = time.perf_counter()
+                K = c.run(steps)
+                secs = time.perf_counter() - t0
+            res[prec] = dict(pushes=cfg.P * steps / secs, ms=secs / steps * 1e3, K=K, P=cfg.P)
+        return res
+
+    def summary(res):
+        f, m = res["fp64"], res["mixed"]
+        rel = np.abs(m["K"][:, 0] - f["K"][:, 0]) / np.abs(f["K"][:, 0])
+        return {"mixed_pushes_per_s": m["pushes"], "fp64_pushes_per_s": f["pushes"],
+                "mixed_over_fp64": m["pushes"] / f["pushes"], "mixed_ms_per_step": m["ms"],
+                "fp64_ms_per_step": f["ms"], "steps": steps,
+                "field_energy_rel_err_mixed_vs_fp64_max": float(rel.max()),
+                "field_energy_rel_err_mixed_vs_fp64_first_100_steps": float(rel[:50].max()),
+                "momentum_x_drift_mixed": float(np.abs(m["K"][:, 3] - m["K"][0, 3]).max())}
+
+    ref = one("reference")
+    out = {"workload": "electrostatic 2D3V 256x256, 64 particles per cell (4194304), B0 = pi/2 along x, "
+                       "dt=1/1024, reference area pairing", **summary(ref)}
+    ctl = one("reference", 1e-8, precs=("fp64",))["fp64"]["K"]
+    Kf = ref["fp64"]["K"]
+    out["fp64_control_1e-8_cell_field_energy_rel_err_max"] = float(
+        np.max(np.abs(ctl[:, 0] - Kf[:, 0]) / np.abs(Kf[:, 0])))
+    out["textbook_pairing"] = summary(one("textbook"))
+    return out
 
 
 def bench_pic2d3v(steps=20, warmup=2, N=1024, ppc=16, shape="bspline2"):
@@ -553,6 +579,38 @@ def bench_small():
     return out
 
 
+def full_run_accuracy(every=50):
+    """The headline workload over its whole configured run (2000 steps) in
+    both precisions: the mixed path's field-energy relative error against
+    FP64 at every step, its momentum drift, and the fitted rate of mode 2 of
+    the field (least squares on log|E^_2| sampled every `every` steps)."""
+    import numpy as np
+    import picgolf
+    from picgolf import configs
+    out, rates = {}, {}
+    for prec in ("mixed", "fp64"):
+        cfg, x, v, T = configs.scaled_two_stream(precision=prec)
+        Ds, a2 = [], []
+        t0 = time.perf_counter()
+        with picgolf.Context(cfg) as c:
+            c.upload(x, v)
+            for _ in range(T // every):
+                D, _ = c.run(every)
+                Ds.append(D[0])
+                a2.append(abs(np.fft.fft(c.download_field())[2]))
+        out[prec] = np.concatenate(Ds)
+        t = np.arange(1, T // every + 1) * every * cfg.dt
+        rates[prec] = float(np.polyfit(t, np.log(a2), 1)[0])
+        print(f"[bench] full run {prec}: {T} steps in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    m, f = out["mixed"], out["fp64"]
+    rel = np.abs(m[:, 0] - f[:, 0]) / np.abs(f[:, 0])
+    return {"steps": int(m.shape[0]), "field_energy_rel_err_max": float(rel.max()),
+            "field_energy_rel_err_final": float(rel[-1]),
+            "momentum_drift_mixed": float(np.max(np.abs(m[:, 3] - m[0, 3]))),
+            "mode2_rate_mixed": rates["mixed"], "mode2_rate_fp64": rates["fp64"],
+            "mode2_rate_ratio": rates["mixed"] / rates["fp64"]}
+
+
 def main():
     a = parse()
     if a.profile_steps < 1:
@@ -644,6 +702,8 @@ def main():
         d1f, d1m = f["D"][:, 0], m["D"][:, 0]
         out["field_energy_rel_err_mixed_vs_fp64"] = float(np.max(np.abs(d1m - d1f) / np.abs(d1f)))
         out["momentum_drift_mixed"] = float(np.max(np.abs(m["D"][:, 3] - m["D"][0, 3])))
+        if not a.no_full_run and a.particles == 2 ** 24:
+            out["full_run"] = full_run_accuracy()
     if not a.no_extra:
         out["workloads"] = {"small": bench_small(),
                             "ensemble_512": bench_ensemble(autotune=not a.no_autotune), "es2d3v_256": bench_2d(),

@@ -582,7 +582,7 @@ int pic_download(pic_ctx* c, int32_t inst, double* x, double* v, double* E) {
         if (x) HIPCHK(c, hipMemcpyAsync(x, c->rb.X[par] + d.poff, P * 8, hipMemcpyDeviceToHost, c->stream));
         if (v) HIPCHK(c, hipMemcpyAsync(v, c->rb.V[par] + d.poff, P * 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-    } else {
+    } else if (x || v) {
         std::vector<uint32_t> xq(P);
         std::vector<float> vh(P), vl(P);
         HIPCHK(c, hipMemcpyAsync(xq.data(), c->rb.Xq[par] + d.poff, P * 4, hipMemcpyDeviceToHost, c->stream));

@@ -32,6 +32,7 @@ struct pic2d_ctx {
     int64_t since_sort = 0;          // steps since the last tile sort
     uint32_t *xq[2] = {}, *yq[2] = {}, *pid[2] = {};
     float *vx[2] = {}, *vy[2] = {}, *vz[2] = {};
+    float *vxl[2] = {}, *vyl[2] = {}, *vzl[2] = {};
     double *X64[2] = {}, *Y64[2] = {}, *VX64[2] = {}, *VY64[2] = {}, *VZ64[2] = {};
     // operator scratch (grown on demand)
     void* op[8] = {};
@@ -125,12 +126,18 @@ static void set_par(pic2d_ctx* c, int par) {
     b.VXf = c->vx[par];
     b.VYf = c->vy[par];
     b.VZf = c->vz[par];
+    b.VXl = c->vxl[par];
+    b.VYl = c->vyl[par];
+    b.VZl = c->vzl[par];
     b.pid = c->pid[par];
     b.Xq_alt = c->xq[1 - par];
     b.Yq_alt = c->yq[1 - par];
     b.VXf_alt = c->vx[1 - par];
     b.VYf_alt = c->vy[1 - par];
     b.VZf_alt = c->vz[1 - par];
+    b.VXl_alt = c->vxl[1 - par];
+    b.VYl_alt = c->vyl[1 - par];
+    b.VZl_alt = c->vzl[1 - par];
     b.pid_alt = c->pid[1 - par];
 }
 
@@ -261,7 +268,8 @@ extern "C" int pic2d_create(pic2d_ctx** out, const pic2d_config* cfg) {
         if ((rc = dalloc(c, &c->pid[k], P))) return fail(rc);
         if (cfg->precision == PIC_PREC_MIXED) {
             if ((rc = dalloc(c, &c->xq[k], P)) || (rc = dalloc(c, &c->yq[k], P)) || (rc = dalloc(c, &c->vx[k], P)) ||
-                (rc = dalloc(c, &c->vy[k], P)) || (rc = dalloc(c, &c->vz[k], P)))
+                (rc = dalloc(c, &c->vy[k], P)) || (rc = dalloc(c, &c->vz[k], P)) || (rc = dalloc(c, &c->vxl[k], P)) ||
+                (rc = dalloc(c, &c->vyl[k], P)) || (rc = dalloc(c, &c->vzl[k], P)))
                 return fail(rc);
         } else {
             if ((rc = dalloc(c, &c->X64[k], P)) || (rc = dalloc(c, &c->Y64[k], P)) ||
@@ -345,11 +353,18 @@ extern "C" int pic2d_upload(pic2d_ctx* c, const double* x, const double* y, cons
         HIPCHK(c, hipMemcpy(b.Xq, q.data(), 4 * P, hipMemcpyHostToDevice));
         fx(y);
         HIPCHK(c, hipMemcpy(b.Yq, q.data(), 4 * P, hipMemcpyHostToDevice));
+        // v = hi + lo: hi = v rounded to FP32, lo = the rest rounded to FP32
         const double* vs[3] = {vx, vy, vz};
         float* ds[3] = {b.VXf, b.VYf, b.VZf};
+        float* dl[3] = {b.VXl, b.VYl, b.VZl};
+        std::vector<float> fl(P);
         for (int k = 0; k < 3; ++k) {
-            for (int64_t p = 0; p < P; ++p) f[p] = (float)vs[k][p];
+            for (int64_t p = 0; p < P; ++p) {
+                f[p] = (float)vs[k][p];
+                fl[p] = (float)(vs[k][p] - (double)f[p]);
+            }
             HIPCHK(c, hipMemcpy(ds[k], f.data(), 4 * P, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(dl[k], fl.data(), 4 * P, hipMemcpyHostToDevice));
         }
         for (int64_t p = 0; p < P; ++p) q[p] = (uint32_t)p;
         HIPCHK(c, hipMemcpy(b.pid, q.data(), 4 * P, hipMemcpyHostToDevice));
@@ -408,11 +423,14 @@ extern "C" int pic2d_download(pic2d_ctx* c, double* x, double* y, double* vx, do
         if (x && (rc = tx(b.Xq, x))) return rc;
         if (y && (rc = tx(b.Yq, y))) return rc;
         float* ss[3] = {b.VXf, b.VYf, b.VZf};
+        float* sl[3] = {b.VXl, b.VYl, b.VZl};
         double* ds[3] = {vx, vy, vz};
+        std::vector<float> fl(P);
         for (int k = 0; k < 3; ++k) {
             if (!ds[k]) continue;
             HIPCHK(c, hipMemcpy(f.data(), ss[k], 4 * P, hipMemcpyDeviceToHost));
-            for (int64_t p = 0; p < P; ++p) ds[k][id[p]] = (double)f[p];
+            HIPCHK(c, hipMemcpy(fl.data(), sl[k], 4 * P, hipMemcpyDeviceToHost));
+            for (int64_t p = 0; p < P; ++p) ds[k][id[p]] = (double)f[p] + (double)fl[p];
         }
     } else {
         // held by tile too: back to the caller's order through pid

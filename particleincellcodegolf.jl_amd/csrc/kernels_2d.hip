@@ -8,7 +8,9 @@
 // atomics), so every result is independent of the schedule.
 //
 // Mixed kernels keep positions as 32-bit fixed-point fractions of the box
-// (periodic wrap for free), velocities in FP32, and use exact integer area
+// (periodic wrap for free), velocities as FP32 hi + lo pairs (the step's
+// change, computed in FP32, is added to the pair exactly, so no FP32 rounding
+// of the velocity itself accumulates over a run), and use exact integer area
 // weights of 2^-20 of a cell per axis: a particle deposits exactly 2^40 quanta,
 // summed in u64 per cell.  The solve and the diagnostics stay FP64.
 //
@@ -196,8 +198,28 @@ k2d_boris64(double* __restrict__ VX, double* __restrict__ VY, double* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// Mixed particle loop: FP32 gather with exact quanta weights, FP32 Boris about
-// x, fixed-point move (x += rint(vx*dt*2^32) mod 2^32), exact quanta deposit.
+// v += d on a hi + lo pair: two-sum of hi and d, the error folded into lo,
+// renormalised (the error terms are exact for |d| well below |hi|, as here).
+__device__ __forceinline__ void add_hilo(float& hi, float& lo, float d) {
+    const float s = hi + d;
+    const float bb = s - hi;
+    const float err = (hi - (s - bb)) + (d - bb);
+    const float t = lo + err;
+    hi = s + t;
+    lo = t - (hi - s);
+}
+
+// rint((hi + lo) * step): the integer part of hi*step is taken first, so the
+// rounding sees both words (hi*step and lo*step are exact for a power-of-two
+// step, dt = 2^-k, and within half an FP32 ulp otherwise).
+__device__ __forceinline__ int32_t step_of(float hi, float lo, float step) {
+    const float a = hi * step, ai = rintf(a);
+    return (int32_t)ai + (int32_t)rintf((a - ai) + lo * step);
+}
+
+// Mixed particle loop: FP32 gather with exact quanta weights, the Boris update
+// about x as FP32 velocity changes added to the hi + lo pairs, fixed-point move
+// (x += rint(vx*dt*2^32) mod 2^32), exact quanta deposit.
 // Particles are held by tile.  A block handles part of one tile's
 // particles: it stages the field of the tile and kHalo2D cells around it in
 // LDS, gathers and deposits there, and adds the staged charge to the grid
@@ -221,8 +243,9 @@ __device__ __forceinline__ int32_t win_rel(int32_t c, int32_t o, int32_t n) {
 
 __global__ void __launch_bounds__(kTileThreads)
 k2d_push32t(Desc2D D, Tiles2D T, uint32_t* __restrict__ Xq, uint32_t* __restrict__ Yq, float* __restrict__ VX,
-            float* __restrict__ VY, float* __restrict__ VZ, const float2* __restrict__ E32, u64* __restrict__ racc,
-            float dt_2, float t1, float tscale, float xstep) {
+            float* __restrict__ VY, float* __restrict__ VZ, float* __restrict__ VXl, float* __restrict__ VYl,
+            float* __restrict__ VZl, const float2* __restrict__ E32, u64* __restrict__ racc, float dt_2, float t1,
+            float tscale, float xstep) {
     __shared__ float2 ew[(kTile2D + 2 * kHalo2D) * (kTile2D + 2 * kHalo2D)];
     __shared__ u64 aw[(kTile2D + 2 * kHalo2D) * (kTile2D + 2 * kHalo2D)];
     constexpr float kQ = 1.0f / (float)(1u << kQuanta2D);
@@ -242,6 +265,7 @@ k2d_push32t(Desc2D D, Tiles2D T, uint32_t* __restrict__ Xq, uint32_t* __restrict
     for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
         uint32_t xq = Xq[p], yq = Yq[p];
         float vx = VX[p], vy = VY[p], vz = VZ[p];
+        float lx = VXl[p], ly = VYl[p], lz = VZl[p];
         {
             const AreaQ ax = areaq(xq, D.NX, D.sx, D.pairing), ay = areaq(yq, D.NY, D.sy, D.pairing);
             const float wx0 = (float)ax.q0 * kQ, wx1 = (float)ax.q1 * kQ;
@@ -273,22 +297,24 @@ k2d_push32t(Desc2D D, Tiles2D T, uint32_t* __restrict__ Xq, uint32_t* __restrict
             wxy = wx1 * wy1;
             f1 = fmaf(e11.x, wxy, f1);
             f2 = fmaf(e11.y, wxy, f2);
+            // the Boris update as changes: half kick, rotation about x, half kick
             const float e1 = f1 * dt_2, e2 = f2 * dt_2;
-            const float m1 = vx + e1, m2 = vy + e2, m3 = vz;
+            const float m2 = vy + e2, m3 = vz;
             const float u2 = m2 + m3 * t1, u3 = m3 - m2 * t1;
-            const float p2 = m2 + (u3 * t1) * tscale;
-            const float p3 = m3 - (u2 * t1) * tscale;
-            vx = m1 + e1;
-            vy = p2 + e2;
-            vz = p3;
+            add_hilo(vx, lx, e1 + e1);
+            add_hilo(vy, ly, (e2 + e2) + (u3 * t1) * tscale);
+            add_hilo(vz, lz, -((u2 * t1) * tscale));
         }
-        xq += (uint32_t)__float2int_rn(vx * xstep);
-        yq += (uint32_t)__float2int_rn(vy * xstep);
+        xq += (uint32_t)step_of(vx, lx, xstep);
+        yq += (uint32_t)step_of(vy, ly, xstep);
         Xq[p] = xq;
         Yq[p] = yq;
         VX[p] = vx;
         VY[p] = vy;
         VZ[p] = vz;
+        VXl[p] = lx;
+        VYl[p] = ly;
+        VZl[p] = lz;
         const AreaQ bx = areaq(xq, D.NX, D.sx, D.pairing), by = areaq(yq, D.NY, D.sy, D.pairing);
         const u64 q00 = (u64)bx.q0 * by.q0, q10 = (u64)bx.q1 * by.q0, q01 = (u64)bx.q0 * by.q1,
                   q11 = (u64)bx.q1 * by.q1;
@@ -374,8 +400,10 @@ __global__ void __launch_bounds__(1024) k2d_tile_scan(Tiles2D T) {
 __global__ void __launch_bounds__(256)
 k2d_tile_scatter(Desc2D D, Tiles2D T, const uint32_t* __restrict__ Xq, const uint32_t* __restrict__ Yq,
                  const float* __restrict__ VX, const float* __restrict__ VY, const float* __restrict__ VZ,
+                 const float* __restrict__ VXl, const float* __restrict__ VYl, const float* __restrict__ VZl,
                  const uint32_t* __restrict__ pid, uint32_t* __restrict__ Xo, uint32_t* __restrict__ Yo,
                  float* __restrict__ VXo, float* __restrict__ VYo, float* __restrict__ VZo,
+                 float* __restrict__ VXlo, float* __restrict__ VYlo, float* __restrict__ VZlo,
                  uint32_t* __restrict__ pido, int64_t per_block) {
     extern __shared__ int32_t hist[];
     for (int i = threadIdx.x; i < T.ntiles; i += blockDim.x) hist[i] = 0;
@@ -395,6 +423,9 @@ k2d_tile_scatter(Desc2D D, Tiles2D T, const uint32_t* __restrict__ Xq, const uin
         VXo[d] = VX[p];
         VYo[d] = VY[p];
         VZo[d] = VZ[p];
+        VXlo[d] = VXl[p];
+        VYlo[d] = VYl[p];
+        VZlo[d] = VZl[p];
         pido[d] = pid[p];
     }
 }
@@ -562,13 +593,14 @@ k2d_push64t(Desc2D D, Tiles2D T, double* __restrict__ X, double* __restrict__ Y,
 // 4096 held particles the exact integer sums of (vx^2 + vy^2) on a 2^-50 grid
 // and of vx, vy on a 2^-40 grid.
 __global__ void __launch_bounds__(256)
-k2d_diag_parts32x(const float* __restrict__ VX, const float* __restrict__ VY, int64_t P, i128* __restrict__ ppart) {
+k2d_diag_parts32x(const float* __restrict__ VX, const float* __restrict__ VY, const float* __restrict__ VXl,
+                  const float* __restrict__ VYl, int64_t P, i128* __restrict__ ppart) {
     __shared__ i128 red[3 * 256];
     const int64_t s = (int64_t)blockIdx.x * kTreeChunk;
     const int64_t n = P - s < kTreeChunk ? P - s : kTreeChunk;
     i128 a = 0, b = 0, c = 0;
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const double vx = (double)VX[s + i], vy = (double)VY[s + i];
+        const double vx = (double)VX[s + i] + (double)VXl[s + i], vy = (double)VY[s + i] + (double)VYl[s + i];
         a += fix_round128(vx * vx + vy * vy, 50);
         b += fix_round128(vx, 40);
         c += fix_round128(vy, 40);
@@ -874,7 +906,7 @@ void launch2d_push(const Buffers2D& b, hipStream_t s) {
         const double t1 = b.B0 * b.dt / 2.0;
         const double tscale = 2.0 / (1.0 + t1 * t1);
         k2d_push32t<<<b.T.ntiles * b.T.splits, kTileThreads, 0, s>>>(
-            b.D, b.T, b.Xq, b.Yq, b.VXf, b.VYf, b.VZf, b.E32, b.racc, (float)(b.dt / 2.0), (float)t1,
+            b.D, b.T, b.Xq, b.Yq, b.VXf, b.VYf, b.VZf, b.VXl, b.VYl, b.VZl, b.E32, b.racc, (float)(b.dt / 2.0), (float)t1,
             (float)tscale, (float)(b.dt * 4294967296.0));
     } else {
         k2d_push64t<<<b.T.ntiles * b.T.splits, kTileThreads, 0, s>>>(b.D, b.T, b.X, b.Y, b.VX, b.VY, b.VZ, b.Ex, b.Ey,
@@ -889,8 +921,9 @@ void launch2d_tile_sort(const Buffers2D& b, hipStream_t s) {
     if (b.precision == 1) {
         k2d_tile_count<<<nb, 256, lds, s>>>(b.D, b.T, b.Xq, b.Yq);
         k2d_tile_scan<<<1, 1024, 0, s>>>(b.T);
-        k2d_tile_scatter<<<nb, 256, lds, s>>>(b.D, b.T, b.Xq, b.Yq, b.VXf, b.VYf, b.VZf, b.pid, b.Xq_alt, b.Yq_alt,
-                                              b.VXf_alt, b.VYf_alt, b.VZf_alt, b.pid_alt, per);
+        k2d_tile_scatter<<<nb, 256, lds, s>>>(b.D, b.T, b.Xq, b.Yq, b.VXf, b.VYf, b.VZf, b.VXl, b.VYl, b.VZl, b.pid,
+                                              b.Xq_alt, b.Yq_alt, b.VXf_alt, b.VYf_alt, b.VZf_alt, b.VXl_alt,
+                                              b.VYl_alt, b.VZl_alt, b.pid_alt, per);
     } else {
         k2d_tile_count64<<<nb, 256, lds, s>>>(b.D, b.T, b.X, b.Y);
         k2d_tile_scan<<<1, 1024, 0, s>>>(b.T);
@@ -932,7 +965,7 @@ void launch2d_diag(const Buffers2D& b, hipStream_t s) {
     if (b.precision == 1) {
         // exact sums: the held particle order changes with every tile sort
         i128* pp = reinterpret_cast<i128*>(b.ppart);
-        k2d_diag_parts32x<<<(unsigned)np, 256, 0, s>>>(b.VXf, b.VYf, P, pp);
+        k2d_diag_parts32x<<<(unsigned)np, 256, 0, s>>>(b.VXf, b.VYf, b.VXl, b.VYl, P, pp);
         k2d_diag_final32x<<<1, 256, 0, s>>>(b.gpart, ng, pp, np, G, P, b.D.w, b.diag, b.drow, b.diag_cap);
         return;
     }

@@ -50,12 +50,14 @@ struct Buffers2D {
     // particles (FP64 or mixed, by precision)
     double *X, *Y, *VX, *VY, *VZ;
     uint32_t *Xq, *Yq;
-    float *VXf, *VYf, *VZf;
+    float *VXf, *VYf, *VZf;  // mixed: velocity hi words ...
+    float *VXl, *VYl, *VZl;  // ... and lo words (v = hi + lo, |lo| <= ulp(hi)/2)
     uint32_t* pid;           // caller's index of each held particle
     // the other buffer of each particle array (tile sorts move the particles
     // there)
     uint32_t *Xq_alt, *Yq_alt, *pid_alt;
     float *VXf_alt, *VYf_alt, *VZf_alt;
+    float *VXl_alt, *VYl_alt, *VZl_alt;
     double *X_alt, *Y_alt, *VX_alt, *VY_alt, *VZ_alt;
     uint32_t* slot;          // FP64: the slot holding caller index i (diagnostics run in the caller's order)
     Tiles2D T;

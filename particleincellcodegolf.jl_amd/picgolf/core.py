@@ -207,6 +207,12 @@ class Context:
         self._check(self._lib.pic_download(self._h, inst, L.ptr(x), L.ptr(v), L.ptr(E)), "pic_download")
         return x, v, E
 
+    def download_field(self, inst=0):
+        """The field E carried between steps, without the particles."""
+        E = np.zeros(self.configs[inst].N)
+        self._check(self._lib.pic_download(self._h, inst, None, None, L.ptr(E)), "pic_download")
+        return E
+
     def run(self, nsteps, want_diag=True):
         n = len(self.configs)
         diag = np.zeros((n, nsteps, 4)) if want_diag else None

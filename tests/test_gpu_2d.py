@@ -161,35 +161,57 @@ def test_fp64_full_run_matches_golden(pg):
     assert digest(got[5]) == str(g["Ex_sha256"]) and digest(got[6]) == str(g["Ey_sha256"])
 
 
-# Field-energy tolerance of the mixed path against FP64 over 200 steps.  The
-# reference pairing's shape jumps by two cells when a particle crosses a cell
-# edge, so a position difference of ~1e-7 cell (FP32 velocity, 2^-20 weight
-# quanta) moves a whole particle's charge for the few particles that sit that
-# close to an edge; the textbook (continuous) shape has no such events.
-K1_TOL = {"reference": 1e-3, "textbook": 1e-5}
-
-
-@pytest.mark.parametrize("pairing", ["reference", "textbook"])
-def test_mixed_against_fp64(pg, pairing):
-    """Field energy, kinetic energy and momentum of the mixed path against the
-    FP64 path over 200 steps of the full 256 x 256 workload."""
+def run_es2d3v(pg, precision, pairing, T, dx_cells=0.0):
+    """K rows of the 256 x 256 workload; dx_cells: a random displacement of
+    every particle, uniform in +-dx_cells cells on each axis."""
     from picgolf import configs
-    T = 200
-    out = {}
-    for prec in ("fp64", "mixed"):
-        cfg, st, _ = configs.es2d3v(precision=prec, pairing=pairing)
-        with pg.Context2D(cfg) as c:
-            c.upload(*st)
-            out[prec] = (c.run(T), c.download())
-    Kf, Km = out["fp64"][0], out["mixed"][0]
-    rel = np.abs(Km - Kf) / np.abs(Kf)
-    print(f"2D3V {pairing} mixed vs FP64 over {T} steps: K1 max rel err {rel[:, 0].max():.3e}, "
-          f"K2 {rel[:, 1].max():.3e}, |dK4| {np.abs(Km[:, 3] - Kf[:, 3]).max():.3e}")
-    assert rel[:, 0].max() < K1_TOL[pairing]     # field energy
-    assert rel[:, 1].max() < 1e-7                # kinetic energy
-    # momentum along B: conserved by both to roundoff of the velocity format
-    assert np.abs(Km[:, 3] - Km[0, 3]).max() < 1e-7
+    cfg, st, _ = configs.es2d3v(precision=precision, pairing=pairing)
+    x, y, vx, vy, vz = st
+    if dx_cells:
+        rng = np.random.default_rng(77)
+        x = x + dx_cells * rng.uniform(-1, 1, x.size) / cfg.NX
+        y = y + dx_cells * rng.uniform(-1, 1, y.size) / cfg.NY
+        x = np.where(x > 1.0, x - 1.0, np.where(x <= 0.0, x + 1.0, x))
+        y = np.where(y > 1.0, y - 1.0, np.where(y <= 0.0, y + 1.0, y))
+    with pg.Context2D(cfg) as c:
+        c.upload(x, y, vx, vy, vz)
+        return c.run(T)
+
+
+def rel_k1(K, Kf):
+    return np.abs(K[:, 0] - Kf[:, 0]) / np.abs(Kf[:, 0])
+
+
+@pytest.mark.parametrize("pairing", ["textbook", "reference"])
+def test_mixed_against_fp64_over_the_run(pg, pairing):
+    """Field energy, kinetic energy and momentum of the mixed path against the
+    FP64 path over the whole 500-step run of the 256 x 256 workload.
+
+    Textbook (continuous) area weights: field energy within 1e-6.  The
+    script's pairing moves a particle's charge by two cells when it crosses a
+    cell edge, so the run is chaotic in the particle positions: FP64 runs
+    whose particles start displaced at random by 1e-8 or 1e-7 of a cell
+    (controls) drift apart in field energy by ~1e-5..1e-4, and no inexact
+    arithmetic can meet 1e-6 there.  The mixed path is held to the control
+    of 1e-7 cell (the size of its weight quanta), within a factor 5."""
+    T = 500
+    Kf = run_es2d3v(pg, "fp64", pairing, T)
+    Km = run_es2d3v(pg, "mixed", pairing, T)
+    rel = rel_k1(Km, Kf)
+    rel2 = np.abs(Km[:, 1] - Kf[:, 1]) / np.abs(Kf[:, 1])
+    print(f"2D3V {pairing} mixed vs FP64 over {T} steps: K1 max rel err {rel.max():.3e}, K2 {rel2.max():.3e}, "
+          f"|dK4| {np.abs(Km[:, 3] - Kf[:, 3]).max():.3e}")
+    assert rel2.max() < 1e-9                      # kinetic energy
+    assert np.abs(Km[:, 3] - Km[0, 3]).max() < 1e-12
     assert np.abs(Kf[:, 3] - Kf[0, 3]).max() < 1e-13
+    if pairing == "textbook":
+        assert rel.max() <= 1e-6
+        return
+    c8 = rel_k1(run_es2d3v(pg, "fp64", pairing, T, 1e-8), Kf).max()
+    c7 = rel_k1(run_es2d3v(pg, "fp64", pairing, T, 1e-7), Kf).max()
+    print(f"2D3V reference pairing FP64 controls over {T} steps: 1e-8 cell {c8:.3e}, 1e-7 cell {c7:.3e}")
+    assert c8 > 1e-5
+    assert rel.max() <= 5 * c7
 
 
 def test_mixed_plasma_frequencies(pg):

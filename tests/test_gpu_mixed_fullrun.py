@@ -35,7 +35,7 @@ def scaled():
             for _ in range(T // 50):
                 D, _ = c.run(50)
                 Ds.append(D[0])
-                Es.append(c.download()[2])
+                Es.append(c.download_field())
         out[prec] = (np.concatenate(Ds), np.array(Es))
         print(f"scaled {prec}: {T} steps done", flush=True)
     return cfg, T, out
