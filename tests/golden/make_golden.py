@@ -7,6 +7,11 @@ restatement (the oracle, oracle/picoracle.c), in the build container:
   ngp_two_stream_1024.npz    N=128, P=8192, NGP leapfrog, dt=1/4N, W=200,
                              seeded uniform x, v=-1/+1 by half, 1024 steps
                              (src/NGPFourier.jl:1-6)
+  scaled_two_stream_p65536_2000.npz
+                             the scaled two-stream workload (N=4096, Gaussian
+                             +-7 midpoint, rtol=1e-8, dt=1/6N, W=400, seeded
+                             x, +-1 beams + N(0, 0.05^2)) at P=2^16 over its
+                             full 2000 steps (src/GaussianFixedPoint.jl)
   es2d3v_256_500.npz         the electrostatic 2D3V workload, 256 x 256 cells,
                              64 particles per cell, the script's shape pairing,
                              FP64, 500 steps (src/Electrostatic2D3V.jl:24-57);
@@ -17,7 +22,7 @@ steps, the final E and x, v (every `pstride`-th particle), and SHA-256 digests o
 the final state, so a GPU run is compared with the oracle bit for bit over the
 full run without the oracle running on the GPU host.
 
-    python tests/golden/make_golden.py [1d] [2d]
+    python tests/golden/make_golden.py [1d] [scaled] [2d]
 """
 import hashlib
 import os
@@ -60,11 +65,14 @@ def write2d(name, cfg, st, nsteps, pstride):
     print(f"{path}: {os.path.getsize(path)} bytes")
 
 
-def main(which=("1d", "2d")):
+def main(which=("1d", "scaled", "2d")):
     from picgolf import configs
     if "2d" in which:
         cfg, st, T = configs.es2d3v(precision="fp64")
         write2d("es2d3v_256_500.npz", cfg, st, T, pstride=4099)
+    if "scaled" in which:
+        cfg, x, v, T = configs.scaled_two_stream(precision="fp64", P=2 ** 16)
+        write("scaled_two_stream_p65536_2000.npz", cfg, 7, 1, x, v, T, 20, pstride=64)
     if "1d" not in which:
         return
     cfg, x, v, T = configs.quiet_two_stream()
@@ -74,4 +82,4 @@ def main(which=("1d", "2d")):
 
 
 if __name__ == "__main__":
-    main(tuple(sys.argv[1:]) or ("1d", "2d"))
+    main(tuple(sys.argv[1:]) or ("1d", "scaled", "2d"))

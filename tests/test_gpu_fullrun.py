@@ -5,8 +5,9 @@ NGP leapfrog run (N=128, P=8192, 1024 steps) bit for bit against the committed
 golden fixtures of the CPU restatement (tests/golden/make_golden.py), through
 both run paths (captured step graphs and the persistent kernel); the quiet
 run's growth rate from the device trace against the analytic 2 pi/sqrt(3)
-(src/GaussianFixedPointQuiet.jl:19-20); the scaled workload at its full
-2^24 particles bit for bit against the OpenMP restatement for two steps.
+(src/GaussianFixedPointQuiet.jl:19-20); the scaled workload's whole
+2000-step run at P=2^16 against its golden fixture, and at its full 2^24
+particles bit for bit against the OpenMP restatement for two steps.
 
 The persistent kernels against the oracle on the edge cases of their
 layout: ragged particle counts, the smallest and largest grids, one
@@ -88,6 +89,20 @@ def test_ngp_two_stream_full_run_matches_golden(pg, path):
     from picgolf import configs
     cfg, x, v, T = configs.ngp_two_stream()
     check_golden("ngp_two_stream_1024.npz", run_path(pg, cfg, x, v, T, path))
+
+
+def test_scaled_config_full_run_at_reduced_particles_matches_golden(pg):
+    """The scaled workload's whole 2000-step FP64 run at P=2^16 (N=4096,
+    Gaussian +-7 midpoint, rtol=1e-8) bit for bit against the golden fixture:
+    every sweep count, the D trace, the final state and field."""
+    from picgolf import configs
+    cfg, x, v, T = configs.scaled_two_stream(precision="fp64", P=2 ** 16)
+    with pg.Context(cfg) as c:
+        assert c.run_path()[0] == "graph"
+        c.upload(x, v)
+        D, S = c.run(T)
+        xg, vg, Eg = c.download()
+    check_golden("scaled_two_stream_p65536_2000.npz", (xg, vg, Eg, D[0], S[0]))
 
 
 def oracle_run(orc, cfg, x, v, nsteps, E=None, nthreads=8):
