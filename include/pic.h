@@ -46,11 +46,12 @@ extern "C" {
  * (src/GaussianFixedPoint.jl:7-9, src/GaussianFixedPointQuiet.jl:8-10) */
 #define PIC_SCHEME_LEAPFROG 0
 #define PIC_SCHEME_MIDPOINT 1
-#define PIC_SCHEME_SIMPSON 2  /* Simpson-1/3 fixed point, src/GaussianFixedPointQuietSimpson13.jl:8-17 (FP64, N <= 1024) */
+#define PIC_SCHEME_SIMPSON 2  /* Simpson-1/3 fixed point, src/GaussianFixedPointQuietSimpson13.jl:8-17 (N <= 1024) */
 
 /* precision of pic_run: FP64 throughout (bitwise equal to the CPU
- * restatement), or MIXED: FP32 particle push and shape weights with exact
- * integer charge accumulation; FFT, convergence test and diagnostics FP64 */
+ * restatement), or MIXED (midpoint and Simpson schemes): FP32 particle push
+ * and shape weights with exact integer charge accumulation; FFT, convergence
+ * test and diagnostics FP64 */
 #define PIC_PREC_FP64 0
 #define PIC_PREC_MIXED 1
 

@@ -214,8 +214,8 @@ static const char* validate(const pic_config& k) {
     if (k.scheme == PIC_SCHEME_SIMPSON && k.N > 1024)
         return "the Simpson scheme is provided for N <= 1024 (its 3 x N convergence test is one reduction chunk)";
     if (k.precision != PIC_PREC_FP64 && k.precision != PIC_PREC_MIXED) return "unknown precision";
-    if (k.precision == PIC_PREC_MIXED && k.scheme != PIC_SCHEME_MIDPOINT)
-        return "mixed precision is provided for the midpoint scheme";
+    if (k.precision == PIC_PREC_MIXED && k.scheme == PIC_SCHEME_LEAPFROG)
+        return "mixed precision is provided for the midpoint and Simpson schemes";
     if (!(k.w >= 0.0) || !std::isfinite(k.w)) return "w must be finite and >= 0";
     if (!std::isfinite(k.dt) || !std::isfinite(k.W) || !(k.W > 0.0)) return "dt must be finite, W finite and > 0";
     if (k.scheme != PIC_SCHEME_LEAPFROG && (k.max_sweeps < 1 || k.max_sweeps > 64))
@@ -255,8 +255,9 @@ const char* pic_last_error(const pic_ctx* ctx) {
 // many tiles as fit at once (LDS-bound); mixed: two 1024-thread blocks per CU,
 // so the per-tile partials (N u64 each) stay small.
 static void default_tiling(const pic_ctx* c, int64_t* tileP, int* block) {
-    const int lds_push = c->scheme == PIC_SCHEME_SIMPSON ? simpson_push_lds_bytes(c->maxN)
-                                                          : push_lds_bytes(c->maxN, c->precision);
+    const int lds_push = c->scheme == PIC_SCHEME_SIMPSON
+                             ? (c->precision == 0 ? simpson_push_lds_bytes(c->maxN) : simpson32_push_lds_bytes(c->maxN))
+                             : push_lds_bytes(c->maxN, c->precision);
     const int wg_per_cu = std::max(1, std::min(4, (160 * 1024) / std::max(lds_push, 1)));
     const int64_t target = (int64_t)c->numCU * (c->precision == 0 ? wg_per_cu : 2);
     *tileP = (c->Ptot + target - 1) / target;
@@ -393,14 +394,24 @@ static int build_tables(pic_ctx* c) {
         b.prime_next = c->scheme == PIC_SCHEME_MIDPOINT ? 1 : 0;
     }
     if (c->scheme == PIC_SCHEME_SIMPSON) {
+        // FP64: three limbs per cell; mixed: one u64 of quanta
+        const size_t words = (c->precision == PIC_PREC_FP64 ? 3 : 1) * (size_t)red0;
         if ((rc = dalloc(c, &b.Es1, G))) return rc;
         if ((rc = dalloc(c, &b.Es2[0], G))) return rc;
         if ((rc = dalloc(c, &b.Es2[1], G))) return rc;
-        if ((rc = dalloc(c, &b.Vg, P))) return rc;
-        if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.racc2), 3 * (size_t)red0))) return rc;
-        if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.racc3), 3 * (size_t)red0))) return rc;
-        HIPCHK(c, hipMemset(b.racc2, 0, 3 * 8 * (size_t)red0));
-        HIPCHK(c, hipMemset(b.racc3, 0, 3 * 8 * (size_t)red0));
+        if (c->precision == PIC_PREC_FP64) {
+            if ((rc = dalloc(c, &b.Vg, P))) return rc;
+        } else {
+            if ((rc = dalloc(c, &b.G1s, P))) return rc;
+            if ((rc = dalloc(c, &b.E32s1, G))) return rc;
+            if ((rc = dalloc(c, &b.E32s2, G))) return rc;
+            HIPCHK(c, hipMemset(b.E32s1, 0, sizeof(float) * G));
+            HIPCHK(c, hipMemset(b.E32s2, 0, sizeof(float) * G));
+        }
+        if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.racc2), words))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<unsigned long long**>(&b.racc3), words))) return rc;
+        HIPCHK(c, hipMemset(b.racc2, 0, 8 * words));
+        HIPCHK(c, hipMemset(b.racc3, 0, 8 * words));
         HIPCHK(c, hipMemset(b.Es1, 0, sizeof(double) * G));
         HIPCHK(c, hipMemset(b.Es2[0], 0, sizeof(double) * G));
         HIPCHK(c, hipMemset(b.Es2[1], 0, sizeof(double) * G));
@@ -1032,9 +1043,15 @@ static std::vector<std::pair<void*, size_t>> run_state(pic_ctx* c) {
         v.push_back({b.Es1, G * 8});
         v.push_back({b.Es2[0], G * 8});
         v.push_back({b.Es2[1], G * 8});
-        v.push_back({b.Vg, P * 8});
-        v.push_back({b.racc2, 24 * cells});
-        v.push_back({b.racc3, 24 * cells});
+        if (c->precision == PIC_PREC_FP64) {
+            v.push_back({b.Vg, P * 8});
+        } else {
+            v.push_back({b.G1s, P * 4});
+            v.push_back({b.E32s1, G * 4});
+            v.push_back({b.E32s2, G * 4});
+        }
+        v.push_back({b.racc2, 8 * cells * (c->precision == 0 ? 3 : 1)});
+        v.push_back({b.racc3, 8 * cells * (c->precision == 0 ? 3 : 1)});
     }
     v.push_back({b.st, sizeof(InstState) * (size_t)c->ninst});
     v.push_back({b.halt, sizeof(int32_t)});

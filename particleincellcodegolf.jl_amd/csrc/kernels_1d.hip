@@ -404,10 +404,27 @@ k_simp_first(const InstDesc* __restrict__ insts, InstState* __restrict__ st, con
     flush_limbs(acc, N, racc + 3 * I.red0);
 }
 
-// E[1,:] = solve(rho(X, X)): one block per instance.
+// Reads and clears an instance's charge into re/im in bit-reversed order:
+// three limbs (FP64) or u64 quanta of wq each (mixed).
+__device__ __forceinline__ void take_charge(u64* __restrict__ racc, const InstDesc& I, int precision, double* re,
+                                            double* im) {
+    if (precision == 0) return take_charge64(racc + 3 * I.red0, re, im, I.N, I.log2N, I.q);
+    for (int j = threadIdx.x; j < I.N; j += blockDim.x) {
+        u64* p = racc + I.red0 + j;
+        const u64 a = *p;
+        *p = 0;
+        const int b = padi((int)brev((uint32_t)j, I.log2N));
+        re[b] = (double)a * I.wq;
+        im[b] = 0.0;
+    }
+}
+
+// E[1,:] = solve(rho(X, X)): one block per instance; mixed: also the FP32
+// copy for the gathers (E32s1, E * dt^2 * 4).
 __global__ void __launch_bounds__(kSolveThreads)
 k_simp_solve1(const InstDesc* __restrict__ insts, InstState* __restrict__ st, u64* __restrict__ racc,
-              double* __restrict__ Es1, const double2* __restrict__ twp, const int32_t* __restrict__ halt) {
+              int precision, double* __restrict__ Es1, float* __restrict__ E32s1, const double2* __restrict__ twp,
+              const int32_t* __restrict__ halt) {
     extern __shared__ double sm[];
     if (halted(halt, false)) return;
     const InstDesc I = insts[blockIdx.x];
@@ -417,10 +434,14 @@ k_simp_solve1(const InstDesc* __restrict__ insts, InstState* __restrict__ st, u6
     double* red = sm + 2 * NP + N;
     SolveRegs R;
     solve_prefetch(R, twp, N, log2N);
-    take_charge64(racc + 3 * I.red0, re, im, N, log2N, I.q);
+    take_charge(racc, I, precision, re, im);
     __syncthreads();
     solve_lds(re, im, R, twp, N, log2N);
-    for (int j = threadIdx.x; j < N; j += blockDim.x) Es1[I.goff + j] = solve_at(re, j);
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        const double e = solve_at(re, j);
+        Es1[I.goff + j] = e;
+        if (precision != 0) E32s1[I.goff + j] = (float)(e * I.escale2);
+    }
     const double emax = block_maxabs(red, N, [&](int64_t i) { return solve_at(re, (int)i); });
     if (threadIdx.x == 0 && !isfinite(emax)) st[blockIdx.x].err |= 1;
 }
@@ -470,8 +491,9 @@ k_simp_start(const InstDesc* __restrict__ insts, const InstState* __restrict__ s
 // instance; requires 3N <= 4096 (one canonical chunk).
 __global__ void __launch_bounds__(kSolveThreads)
 k_simp_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st, u64* __restrict__ racc2,
-             u64* __restrict__ racc3, const double* __restrict__ Es1, double* Es2a, double* Es2b, double* E3a,
-             double* E3b, const double2* __restrict__ twp, int k, int32_t* __restrict__ halt, int stall) {
+             u64* __restrict__ racc3, int precision, const double* __restrict__ Es1, double* Es2a, double* Es2b,
+             double* E3a, double* E3b, float* __restrict__ E32s2, float* __restrict__ E32s3,
+             const double2* __restrict__ twp, int k, int32_t* __restrict__ halt, int stall) {
     extern __shared__ double sm[];
     if (halted(halt, false)) return;
     const InstDesc I = insts[blockIdx.x];
@@ -490,20 +512,25 @@ k_simp_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st, u64
     double* E3o = (nxt ? E3b : E3a) + I.goff;
     SolveRegs R;
     solve_prefetch(R, twp, N, log2N);
-    take_charge64(racc2 + 3 * I.red0, re, im, N, log2N, I.q);
+    take_charge(racc2, I, precision, re, im);
     __syncthreads();
     solve_lds(re, im, R, twp, N, log2N);
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
         const double e = solve_at(re, j);
         E2o[j] = e;
         e2n[j] = e;
+        if (precision != 0) E32s2[I.goff + j] = (float)(e * I.escale2);
     }
     __syncthreads();
     solve_prefetch(R, twp, N, log2N);  // the first solve reloaded inverse twiddles
-    take_charge64(racc3 + 3 * I.red0, re, im, N, log2N, I.q);
+    take_charge(racc3, I, precision, re, im);
     __syncthreads();
     solve_lds(re, im, R, twp, N, log2N);
-    for (int j = threadIdx.x; j < N; j += blockDim.x) E3o[j] = solve_at(re, j);
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        const double e = solve_at(re, j);
+        E3o[j] = e;
+        if (precision != 0) E32s3[I.goff + j] = (float)(e * I.escale2);
+    }
     double emax;
     const bool conv = block_isapprox(
         red,
@@ -661,8 +688,9 @@ void launch_lf_second(const RunBuffers& b, hipStream_t s) {
 }
 
 void launch_diag(const RunBuffers& b, hipStream_t s) {
-    // the mixed midpoint push sums the kinetic terms in its final sweep
-    const bool fused = b.precision == 1 && b.scheme == 1;
+    // the mixed midpoint and Simpson pushes sum the kinetic terms in their
+    // final sweep
+    const bool fused = b.precision == 1 && (b.scheme == 1 || b.scheme == 2);
     if (b.precision == 0)
         k_diag_chunks64<<<b.nchunks, 256, 0, s>>>(b.insts, b.st, b.chunks, b.V[0], b.V[1], b.dpart, b.halt);
     k_diag_final<<<b.ninst, 256, 0, s>>>(b.insts, b.st, b.dpart, b.E[0], b.E[1], b.diag, b.sweeps,
@@ -677,10 +705,17 @@ void launch_simp_first(const RunBuffers& b, hipStream_t s) {
     u64* r1 = (u64*)b.racc;
     u64* r2 = (u64*)b.racc2;
     u64* r3 = (u64*)b.racc3;
+    if (b.precision == 1) {
+        launch_simp32_first(b, s);
+        k_simp_solve1<<<b.ninst, kSolveThreads, solve_lds_bytes(b.maxN), s>>>(b.insts, b.st, r1, 1, b.Es1, b.E32s1,
+                                                                             (const double2*)b.tw, b.halt);
+        launch_simp32_start(b, s);
+        return;
+    }
     if (b.R == 1) k_simp_first<1><<<g, t, la, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], r1, b.halt);
     else if (b.R == 6) k_simp_first<6><<<g, t, la, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], r1, b.halt);
     else k_simp_first<7><<<g, t, la, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], r1, b.halt);
-    k_simp_solve1<<<b.ninst, kSolveThreads, solve_lds_bytes(b.maxN), s>>>(b.insts, b.st, r1, b.Es1,
+    k_simp_solve1<<<b.ninst, kSolveThreads, solve_lds_bytes(b.maxN), s>>>(b.insts, b.st, r1, 0, b.Es1, nullptr,
                                                                          (const double2*)b.tw, b.halt);
     if (b.R == 1)
         k_simp_start<1><<<g, t, lb, s>>>(b.insts, b.st, b.tiles, b.X[0], b.X[1], b.V[0], b.V[1], b.Vg, b.Es1, r2,
@@ -695,11 +730,12 @@ void launch_simp_first(const RunBuffers& b, hipStream_t s) {
 
 void launch_simp_solve(const RunBuffers& b, int k, bool stall, hipStream_t s) {
     k_simp_solve<<<b.ninst, kSolveThreads, solve_lds_bytes(b.maxN), s>>>(
-        b.insts, b.st, (u64*)b.racc2, (u64*)b.racc3, b.Es1, b.Es2[0], b.Es2[1], b.E[0], b.E[1],
-        (const double2*)b.tw, k, b.halt, stall ? 1 : 0);
+        b.insts, b.st, (u64*)b.racc2, (u64*)b.racc3, b.precision, b.Es1, b.Es2[0], b.Es2[1], b.E[0], b.E[1],
+        b.E32s2, b.E32, (const double2*)b.tw, k, b.halt, stall ? 1 : 0);
 }
 
 void launch_simp_push(const RunBuffers& b, int k, bool last_slot, hipStream_t s) {
+    if (b.precision == 1) return launch_simp32_push(b, k, last_slot, s);
     const dim3 g(b.ntiles), t(b.block_push);
     const int lds = simpson_push_lds_bytes(b.maxN), ls = last_slot ? 1 : 0;
     u64* r2 = (u64*)b.racc2;

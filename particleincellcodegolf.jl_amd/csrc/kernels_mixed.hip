@@ -17,6 +17,7 @@
 // test and the diagnostics stay FP64 (kernels_1d.hip).
 #include "pic_device.h"
 #include "pic_internal.h"
+#include "stencil1d.h"
 #include "weights32.h"
 
 namespace pic {
@@ -112,6 +113,35 @@ __device__ __forceinline__ void flush_charge32(const uint32_t* lo, const uint32_
         u64 v = 0;
         for (int kk = (c + kHalo) % N; kk < NPAD; kk += N) v += ((u64)hi[kk] << 32) + lo[kk];
         if (v) atomicAdd(racc + c, v);
+    }
+}
+
+// The step's kinetic diagnostics of the final sweep: the block's exact sums
+// of v^2 and v (128-bit each) into the tile's partial pair of dpart (red:
+// 4 words of LDS per wave).  Called by every thread of the block.
+__device__ __forceinline__ void store_tile_kinetic(i128 q2, i128 q1, u64* red, double* __restrict__ dpart) {
+    q2 = wave_sum_i128(q2);
+    q1 = wave_sum_i128(q1);
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[4 * wv] = (u64)q2;
+        red[4 * wv + 1] = (u64)((u128)q2 >> 64);
+        red[4 * wv + 2] = (u64)q1;
+        red[4 * wv + 3] = (u64)((u128)q1 >> 64);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        i128 a = 0, b = 0;
+        for (int w = 0; w < nw; ++w) {
+            a += (i128)(((u128)red[4 * w + 1] << 64) | red[4 * w]);
+            b += (i128)(((u128)red[4 * w + 3] << 64) | red[4 * w + 2]);
+        }
+        u64* o = reinterpret_cast<u64*>(dpart) + 4 * (int64_t)blockIdx.x;
+        o[0] = (u64)a;
+        o[1] = (u64)((u128)a >> 64);
+        o[2] = (u64)b;
+        o[3] = (u64)((u128)b >> 64);
     }
 }
 
@@ -224,31 +254,7 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
         if (deposit) lane_deposit(acc_lo, acc_hi, s, dep);
     }
     if (last) {
-        // the step's kinetic diagnostics: one exact partial pair per tile
-        q2 = wave_sum_i128(q2);
-        q1 = wave_sum_i128(q1);
-        u64* red = lds + NPAD + (NPAD + 1) / 2;  // 4 words per wave
-        __syncthreads();
-        const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-        if ((threadIdx.x & 63) == 0) {
-            red[4 * wv] = (u64)q2;
-            red[4 * wv + 1] = (u64)((u128)q2 >> 64);
-            red[4 * wv + 2] = (u64)q1;
-            red[4 * wv + 3] = (u64)((u128)q1 >> 64);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            i128 a = 0, b = 0;
-            for (int w = 0; w < nw; ++w) {
-                a += (i128)(((u128)red[4 * w + 1] << 64) | red[4 * w]);
-                b += (i128)(((u128)red[4 * w + 3] << 64) | red[4 * w + 2]);
-            }
-            u64* o = reinterpret_cast<u64*>(dpart) + 4 * (int64_t)blockIdx.x;
-            o[0] = (u64)a;
-            o[1] = (u64)((u128)a >> 64);
-            o[2] = (u64)b;
-            o[3] = (u64)((u128)b >> 64);
-        }
+        store_tile_kinetic(q2, q1, lds + NPAD + (NPAD + 1) / 2, dpart);
         if (!prime) return;
     }
     __syncthreads();
@@ -267,6 +273,259 @@ void launch_mid_push32(const RunBuffers& b, int mode, int k, bool last_slot, hip
         k_mid_push32<1><<<g, t, b.lds_push, s>>>(b.insts, b.st, b.tiles, b.Xq[0], b.Xq[1], b.Vh[0], b.Vh[1],
                                                  b.Vl[0], b.Vl[1], b.vf[0], b.vf[1], b.E32, part, k, b.halt, ls,
                                                  b.dpart, b.prime_next);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed Simpson-1/3 (src/GaussianFixedPointQuietSimpson13.jl:8-17,
+// src/AreaFixedPointQuietSimpson13.jl:5-17) on the mixed particle state:
+// fixed-point positions, velocities as hi + lo pairs in steps of the position
+// grid, FP32 Picard iterate.  The three stage fields arrive as FP32 copies in
+// the same units as the midpoint's (E * dt^2 * 4, so that a gather with
+// weights in 2^-30 quanta is the kick in steps of the grid); charges are exact
+// integer quanta; the solves and the 3 x N convergence test stay FP64.
+// SH: 1 = the area weights of AreaFixedPointQuietSimpson13.jl:5 (exact
+// quanta from the fixed-point centre), 6 / 7 = the Gaussian shapes (FP32
+// weights over +-5 cells, as the midpoint).
+// ---------------------------------------------------------------------------
+
+// Area weights of a fixed-point centre c: i = ceil(c N) (Julia index, 0 for
+// c = 0), o = i - c N; 1 - o goes to array cell i - 1 and o to cell i - 2
+// (mod N), here in 2^-30 quanta (exact: o has 32 - log2 N fraction bits) and
+// as padded LDS slots (slot s0 holds cell i - 1, s0 - 1 cell i - 2).
+struct Area32 {
+    int32_t s0;
+    uint32_t q0, q1;
+};
+__device__ __forceinline__ Area32 area32(uint32_t cq, int shift) {
+    const uint32_t i = (uint32_t)(((uint64_t)cq + ((1ull << shift) - 1)) >> shift);
+    const uint32_t o = (uint32_t)(((uint64_t)i << shift) - cq);
+    const uint32_t o30 = o << (30 - shift);
+    return {(int32_t)i + kHalo - 1, (1u << 30) - o30, o30};
+}
+
+// One exact add into the block's LDS charge (lo word, hi word on a wrap).
+__device__ __forceinline__ void add_quanta(uint32_t* __restrict__ lo, uint32_t* __restrict__ hi, int slot,
+                                           uint32_t q) {
+    const uint32_t old = atomicAdd(&lo[slot], q);
+    if (old + q < old) atomicAdd(&hi[slot], 1u);
+}
+
+// The kick of a padded FP32 field at centre cq, in steps of the grid.
+template <int SH>
+__device__ __forceinline__ float simp_gather32(const float* __restrict__ Eld, uint32_t cq, int shift) {
+    if constexpr (SH == 1) {
+        const Area32 a = area32(cq, shift);
+        return fmaf(Eld[a.s0 - 1], (float)a.q1, Eld[a.s0] * (float)a.q0);
+    } else {
+        Stencil32 s;
+        stencil32(cq, shift, s);
+        const float* e = Eld + s.i0;
+        float kick = 0.0f;
+#pragma unroll
+        for (int j = 0; j < kWin; ++j) kick = fmaf(e[j], s.w[j], kick);
+        return kick;
+    }
+}
+
+template <int SH>
+__device__ __forceinline__ void simp_deposit32(uint32_t* __restrict__ lo, uint32_t* __restrict__ hi, uint32_t cq,
+                                               int shift) {
+    if constexpr (SH == 1) {
+        const Area32 a = area32(cq, shift);
+        add_quanta(lo, hi, a.s0, a.q0);
+        if (a.q1) add_quanta(lo, hi, a.s0 - 1, a.q1);
+    } else {
+        Stencil32 s;
+        stencil32(cq, shift, s);
+        lane_deposit(lo, hi, s, true);
+    }
+}
+
+// rho(X, X) of the step (centre X) into racc; the step's Picard loop starts.
+template <int SH>
+__global__ void __launch_bounds__(1024)
+k_simp_first32(const InstDesc* __restrict__ insts, InstState* __restrict__ st, const TileDesc* __restrict__ tiles,
+               const uint32_t* Xq0, const uint32_t* Xq1, u64* __restrict__ racc, const int32_t* __restrict__ halt) {
+    extern __shared__ u64 lds[];
+    if (halted(halt, false)) return;
+    const TileDesc T = tiles[blockIdx.x];
+    const InstDesc I = insts[T.inst];
+    const InstState S = st[T.inst];
+    if (T.local == 0 && threadIdx.x == 0) st[T.inst].last_sweep = -1;
+    const int NPAD = pad_cells(I.N);
+    uint32_t* lo = reinterpret_cast<uint32_t*>(lds);
+    uint32_t* hi = lo + NPAD;
+    zero_lds(lds, NPAD);
+    __syncthreads();
+    const uint32_t* Xq = (S.step & 1) ? Xq1 : Xq0;
+    const int shift = 32 - I.log2N;
+    const int64_t base = I.poff + T.start;
+    for (int64_t p = threadIdx.x; p < T.count; p += blockDim.x) simp_deposit32<SH>(lo, hi, Xq[base + p], shift);
+    __syncthreads();
+    flush_charge32(lo, hi, I.N, NPAD, racc + I.red0);
+}
+
+// Per particle: the first stage's kick g1/6 (the script's Vg - V, in steps of
+// the grid) and, with v_0 = V, the charges rho(X, x_0) and rho(x_0, x_0).
+template <int SH>
+__global__ void __launch_bounds__(1024)
+k_simp_start32(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
+               const TileDesc* __restrict__ tiles, const uint32_t* Xq0, const uint32_t* Xq1, const float* Vh0,
+               const float* Vh1, float* __restrict__ G1s, const float* __restrict__ E32s1, u64* __restrict__ racc2,
+               u64* __restrict__ racc3, const int32_t* __restrict__ halt) {
+    extern __shared__ u64 lds[];
+    if (halted(halt, false)) return;
+    const TileDesc T = tiles[blockIdx.x];
+    const InstDesc I = insts[T.inst];
+    const InstState S = st[T.inst];
+    const int NPAD = pad_cells(I.N);
+    uint32_t* lo2 = reinterpret_cast<uint32_t*>(lds);
+    uint32_t* hi2 = lo2 + NPAD;
+    uint32_t* lo3 = hi2 + NPAD;
+    uint32_t* hi3 = lo3 + NPAD;
+    float* E1ld = reinterpret_cast<float*>(lds + 2 * NPAD);
+    for (int i = threadIdx.x; i < NPAD; i += blockDim.x) E1ld[i] = E32s1[I.goff + ((i - kHalo) & I.nmask)];
+    zero_lds(lds, 2 * NPAD);
+    __syncthreads();
+    const int par = S.step & 1;
+    const uint32_t* Xq = par ? Xq1 : Xq0;
+    const float* Vh = par ? Vh1 : Vh0;
+    const int shift = 32 - I.log2N;
+    const int64_t base = I.poff + T.start;
+    for (int64_t p = threadIdx.x; p < T.count; p += blockDim.x) {
+        const int64_t g = base + p;
+        const uint32_t xq = Xq[g];
+        const float vh = Vh[g];
+        G1s[g] = simp_gather32<SH>(E1ld, xq, shift) / 6.0f;
+        simp_deposit32<SH>(lo2, hi2, advance(xq, vh + vh, 0.25f), shift);  // (X + x_0)/2
+        simp_deposit32<SH>(lo3, hi3, advance(xq, vh + vh, 0.5f), shift);   // x_0
+    }
+    __syncthreads();
+    flush_charge32(lo2, hi2, I.N, NPAD, racc2 + I.red0);
+    flush_charge32(lo3, hi3, I.N, NPAD, racc3 + I.red0);
+}
+
+// Sweep k: v_{k+1} = V + g1/6 + (4 g2)/6 + g3/6 with g2 at (X + x_k)/2 and g3 at
+// x_k; the final sweep adds the change to the hi + lo pair, writes x_k and
+// sums the exact kinetic terms; any other sweep writes the iterate and
+// deposits rho(X, x_{k+1}) and rho(x_{k+1}, x_{k+1}).
+template <int SH>
+__global__ void __launch_bounds__(1024)
+k_simp_push32(const InstDesc* __restrict__ insts, const InstState* __restrict__ st,
+              const TileDesc* __restrict__ tiles, uint32_t* Xq0, uint32_t* Xq1, float* Vh0, float* Vh1, float* Vl0,
+              float* Vl1, float* vf0, float* vf1, const float* __restrict__ G1s, const float* __restrict__ E32s2,
+              const float* __restrict__ E32s3, u64* __restrict__ racc2, u64* __restrict__ racc3, int k,
+              const int32_t* __restrict__ halt, int last_slot, double* __restrict__ dpart) {
+    extern __shared__ u64 lds[];
+    if (halted(halt, last_slot)) return;
+    const TileDesc T = tiles[blockIdx.x];
+    const InstDesc I = insts[T.inst];
+    const InstState S = st[T.inst];
+    if (sweep_over(S, k)) return;
+    const bool last = S.last_sweep == k;
+    const int NPAD = pad_cells(I.N);
+    uint32_t* lo2 = reinterpret_cast<uint32_t*>(lds);
+    uint32_t* hi2 = lo2 + NPAD;
+    uint32_t* lo3 = hi2 + NPAD;
+    uint32_t* hi3 = lo3 + NPAD;
+    float* E2ld = reinterpret_cast<float*>(lds + 2 * NPAD);
+    float* E3ld = E2ld + NPAD;
+    u64* red = lds + 3 * NPAD;  // 4 words per wave
+    for (int i = threadIdx.x; i < NPAD; i += blockDim.x) {
+        const int c = I.goff + ((i - kHalo) & I.nmask);
+        E2ld[i] = E32s2[c];
+        E3ld[i] = E32s3[c];
+    }
+    if (!last) zero_lds(lds, 2 * NPAD);
+    __syncthreads();
+    const int par = S.step & 1;
+    const uint32_t* Xq = par ? Xq1 : Xq0;
+    const float* Vh = par ? Vh1 : Vh0;
+    const float* Vl = par ? Vl1 : Vl0;
+    uint32_t* Xo = par ? Xq0 : Xq1;
+    float* Vho = par ? Vh0 : Vh1;
+    float* Vlo = par ? Vl0 : Vl1;
+    const float* vin = (k & 1) ? vf1 : vf0;
+    float* vout = ((k + 1) & 1) ? vf1 : vf0;
+    const int shift = 32 - I.log2N;
+    const int64_t base = I.poff + T.start;
+    i128 q2 = 0, q1 = 0;
+    for (int64_t p = threadIdx.x; p < T.count; p += blockDim.x) {
+        const int64_t g = base + p;
+        const uint32_t xq = Xq[g];
+        const float vh = Vh[g];
+        const float vk = (k == 0) ? vh : vin[g];
+        const uint32_t xk = advance(xq, vk + vh, 0.5f);
+        const float g2 = simp_gather32<SH>(E2ld, advance(xq, vk + vh, 0.25f), shift);
+        const float g3 = simp_gather32<SH>(E3ld, xk, shift);
+        const float d = (G1s[g] + (g2 * 4.0f) / 6.0f) + g3 / 6.0f;
+        if (last) {
+            // V_new = (V_hi + V_lo) + d, renormalised (two-sum)
+            const float sum = vh + d;
+            const float bb = sum - vh;
+            const float err = (vh - (sum - bb)) + (d - bb);
+            const float t = Vl[g] + err;
+            const float h = sum + t;
+            const float l = t - (h - sum);
+            Xo[g] = xk;
+            Vho[g] = h;
+            Vlo[g] = l;
+            const double vd = ((double)h + (double)l) * I.inv_vstep;
+            q2 += fix_round128(vd * vd, kV2Bits);
+            q1 += fix_round128(vd, kV1Bits);
+            continue;
+        }
+        const float vn = vh + d;
+        vout[g] = vn;
+        simp_deposit32<SH>(lo2, hi2, advance(xq, vn + vh, 0.25f), shift);  // (X + x_{k+1})/2
+        simp_deposit32<SH>(lo3, hi3, advance(xq, vn + vh, 0.5f), shift);   // x_{k+1}
+    }
+    if (last) {
+        store_tile_kinetic(q2, q1, red, dpart);
+        return;
+    }
+    __syncthreads();
+    flush_charge32(lo2, hi2, I.N, NPAD, racc2 + I.red0);
+    flush_charge32(lo3, hi3, I.N, NPAD, racc3 + I.red0);
+}
+
+// LDS: two padded charges (8 B per slot each), two padded FP32 fields, and
+// the final sweep's per-wave partials.
+int simpson32_push_lds_bytes(int N) { return 24 * pad_cells(N) + 32 * 16; }
+
+void launch_simp32_first(const RunBuffers& b, hipStream_t s) {
+    const dim3 g(b.ntiles), t(b.block_push);
+    const size_t lds = 8 * (size_t)pad_cells(b.maxN);
+    u64* r = (u64*)b.racc;
+    if (b.R == 1) k_simp_first32<1><<<g, t, lds, s>>>(b.insts, b.st, b.tiles, b.Xq[0], b.Xq[1], r, b.halt);
+    else k_simp_first32<7><<<g, t, lds, s>>>(b.insts, b.st, b.tiles, b.Xq[0], b.Xq[1], r, b.halt);
+}
+
+void launch_simp32_start(const RunBuffers& b, hipStream_t s) {
+    const dim3 g(b.ntiles), t(b.block_push);
+    const size_t lds = 20 * (size_t)pad_cells(b.maxN);
+    u64 *r2 = (u64*)b.racc2, *r3 = (u64*)b.racc3;
+    if (b.R == 1)
+        k_simp_start32<1><<<g, t, lds, s>>>(b.insts, b.st, b.tiles, b.Xq[0], b.Xq[1], b.Vh[0], b.Vh[1], b.G1s,
+                                            b.E32s1, r2, r3, b.halt);
+    else
+        k_simp_start32<7><<<g, t, lds, s>>>(b.insts, b.st, b.tiles, b.Xq[0], b.Xq[1], b.Vh[0], b.Vh[1], b.G1s,
+                                            b.E32s1, r2, r3, b.halt);
+}
+
+void launch_simp32_push(const RunBuffers& b, int k, bool last_slot, hipStream_t s) {
+    const dim3 g(b.ntiles), t(b.block_push);
+    const size_t lds = (size_t)simpson32_push_lds_bytes(b.maxN);
+    const int ls = last_slot ? 1 : 0;
+    u64 *r2 = (u64*)b.racc2, *r3 = (u64*)b.racc3;
+    if (b.R == 1)
+        k_simp_push32<1><<<g, t, lds, s>>>(b.insts, b.st, b.tiles, b.Xq[0], b.Xq[1], b.Vh[0], b.Vh[1], b.Vl[0],
+                                           b.Vl[1], b.vf[0], b.vf[1], b.G1s, b.E32s2, b.E32, r2, r3, k, b.halt, ls,
+                                           b.dpart);
+    else
+        k_simp_push32<7><<<g, t, lds, s>>>(b.insts, b.st, b.tiles, b.Xq[0], b.Xq[1], b.Vh[0], b.Vh[1], b.Vl[0],
+                                           b.Vl[1], b.vf[0], b.vf[1], b.G1s, b.E32s2, b.E32, r2, r3, k, b.halt, ls,
+                                           b.dpart);
 }
 
 int configure_kernels_mixed() {

@@ -111,6 +111,12 @@ struct RunBuffers {
     void* racc2;
     void* racc3;
     double* Vg;
+    // mixed Simpson: FP32 copies of E[1,:] and E[2,:] for the gathers (E32
+    // holds E[3,:]'s), and G1s[p] = gather(E[1,:], X)/6 in steps of the grid;
+    // racc2, racc3 hold u64 quanta
+    float* E32s1;
+    float* E32s2;
+    float* G1s;
 };
 
 // kernels_1d.hip
@@ -125,6 +131,11 @@ int simpson_push_lds_bytes(int N);
 void launch_simp_first(const RunBuffers& b, hipStream_t s);
 void launch_simp_solve(const RunBuffers& b, int k, bool stall, hipStream_t s);
 void launch_simp_push(const RunBuffers& b, int k, bool last_slot, hipStream_t s);
+// kernels_mixed.hip: the mixed forms (called by the launchers above)
+int simpson32_push_lds_bytes(int N);
+void launch_simp32_first(const RunBuffers& b, hipStream_t s);
+void launch_simp32_start(const RunBuffers& b, hipStream_t s);
+void launch_simp32_push(const RunBuffers& b, int k, bool last_slot, hipStream_t s);
 void launch_lf_first(const RunBuffers& b, hipStream_t s);
 void launch_lf_second(const RunBuffers& b, hipStream_t s);
 void launch_diag(const RunBuffers& b, hipStream_t s);

@@ -68,7 +68,7 @@ def quiet_two_stream(precision="fp64"):
     return cfg, x, v, 2 ** 13
 
 
-def quiet_simpson(shape="gauss7"):
+def quiet_simpson(shape="gauss7", precision="fp64"):
     """The Simpson-1/3 fixed-point scripts: N=64, P=2048, dt=1/6N, 8192 steps,
     W=32pi^2/3, bit-reversal quiet start; the Gaussian +-7 shape with
     rtol=4eps (src/GaussianFixedPointQuietSimpson13.jl:1-5) or the linear area
@@ -78,7 +78,7 @@ def quiet_simpson(shape="gauss7"):
     W = 32 * np.pi ** 2 / 3
     rtol = 1e-14 if shape == "area" else 4 * EPS
     cfg = Config(N=N, P=P, W=W, w=W / P * N, dt=1 / (6 * N), shape=shape, scheme="simpson",
-                 precision="fp64", rtol=rtol, max_sweeps=10)
+                 precision=precision, rtol=rtol, max_sweeps=10)
     x, v = quiet_start(P)
     return cfg, x, v, 2 ** 13
 

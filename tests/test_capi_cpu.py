@@ -51,7 +51,7 @@ def test_create_rejects_bad_config_with_message(field, value, msg):
 @pytest.mark.parametrize("kw,msg", [
     (dict(scheme="simpson", shape="ngp"), "Simpson scheme needs"),
     (dict(scheme="simpson", N=2048), "N <= 1024"),
-    (dict(scheme="simpson", precision="mixed"), "mixed precision"),
+    (dict(scheme="leapfrog", shape="ngp", precision="mixed"), "mixed precision"),
     (dict(scheme="midpoint", shape="area"), "Gaussian"),
     (dict(scheme="leapfrog", shape="area"), "Simpson"),
 ])
