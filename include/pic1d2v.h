@@ -13,10 +13,13 @@
  *   Es[:, ti] .+= E       time-binned field for omega-k spectra   :49, :57
  *   D[ti, 1:5]            diagnostics every bin steps             :52-56, :58-62
  *
- * FP64 only; every result is bit-identical to the CPU restatement (per-cell
+ * FP64: every result is bit-identical to the CPU restatement (per-cell
  * charge sums are exact signed 128-bit integer sums, every other sum uses the
- * canonical tree).  Return codes, ownership, threading and error reporting
- * follow pic.h.
+ * canonical tree).  MIXED (precision = PIC_PREC_MIXED) runs the loop with
+ * fixed-point positions, FP32 hi + lo velocities, FP32 Gaussian weights over
+ * +-5 cells and exact integer charge quanta per species, with the solve and
+ * the diagnostics in FP64; the operators stay FP64.  Return codes,
+ * ownership, threading and error reporting follow pic.h.
  */
 #ifndef PIC1D2V_H
 #define PIC1D2V_H
@@ -42,6 +45,7 @@ typedef struct pic1d2v_config {
     double B0;         /* magnetic field along z                                      */
     double dt;         /* time step                                                   */
     int32_t device;    /* HIP device ordinal                                          */
+    int32_t precision; /* PIC_PREC_FP64 (0, the default) or PIC_PREC_MIXED            */
 } pic1d2v_config;
 
 typedef struct pic1d2v_ctx pic1d2v_ctx;

@@ -111,6 +111,8 @@ static const char* validate(const pic1d2v_config& k) {
     if (!(k.w >= 0.0) || !std::isfinite(k.w)) return "w must be finite and >= 0";
     if (!(k.n0 > 0.0) || !std::isfinite(k.n0)) return "n0 must be finite and > 0";
     if (!std::isfinite(k.B0) || !std::isfinite(k.dt)) return "B0 and dt must be finite";
+    if (k.precision != PIC_PREC_FP64 && k.precision != PIC_PREC_MIXED) return "unknown precision";
+    if (k.precision == PIC_PREC_MIXED && !(k.dt > 0.0)) return "mixed precision needs dt > 0";
     return nullptr;
 }
 
@@ -190,10 +192,33 @@ int pic1d2v_create(pic1d2v_ctx** out, const pic1d2v_config* cfg) {
     b.qbits = frac_bits((int64_t)ns * cfg->P, cfg->w * qmax);
     const size_t P = (size_t)cfg->P, NP = (size_t)ns * P, N = (size_t)cfg->N;
     int rc;
-    if ((rc = dalloc(c, &b.x, NP)) || (rc = dalloc(c, &b.vx, NP)) || (rc = dalloc(c, &b.vy, NP))) return fail(rc);
+    b.precision = cfg->precision;
+    b.nchunks = ((int64_t)P + kTreeChunk - 1) / kTreeChunk;
+    if (b.precision == PIC_PREC_FP64) {
+        if ((rc = dalloc(c, &b.x, NP)) || (rc = dalloc(c, &b.vx, NP)) || (rc = dalloc(c, &b.vy, NP))) return fail(rc);
+    } else {
+        // the mixed state; velocities in steps of the 2^-32 position grid
+        b.vstep = cfg->dt * 4294967296.0;
+        b.inv_vstep = 1.0 / b.vstep;
+        b.escale2 = (cfg->dt * cfg->dt) * 4.0;
+        for (int s = 0; s < 2; ++s) {
+            b.kick_half[s] = (float)(b.q_m[s] / 2.0);
+            b.t3[s] = (float)(cfg->B0 * (cfg->dt / 2.0) * b.q_m[s]);
+        }
+        if ((rc = dalloc(c, &b.xq, NP)) || (rc = dalloc(c, &b.vxh, NP)) || (rc = dalloc(c, &b.vxl, NP)) ||
+            (rc = dalloc(c, &b.vyh, NP)) || (rc = dalloc(c, &b.vyl, NP)) || (rc = dalloc(c, &b.E32, N)) ||
+            (rc = dalloc(c, &b.racc32, 2 * N)))
+            return fail(rc);
+        unsigned char* part = nullptr;
+        if ((rc = dalloc(c, &part, 96 * (size_t)b.nchunks))) return fail(rc);  // 6 x 128 bits per chunk
+        b.dpart32 = part;
+        if (hipMemset(b.racc32, 0, 16 * N) != hipSuccess || hipMemset(b.E32, 0, 4 * N) != hipSuccess) {
+            set_err(c, "pic1d2v_create: hipMemset failed");
+            return fail(PIC_EHIP);
+        }
+    }
     if ((rc = dalloc(c, &b.racc, 6 * N))) return fail(rc);
     if ((rc = dalloc(c, &b.E, N))) return fail(rc);
-    b.nchunks = ((int64_t)P + kTreeChunk - 1) / kTreeChunk;
     if ((rc = dalloc(c, &b.dpart, 3 * (size_t)b.nchunks))) return fail(rc);
     if ((rc = dalloc(c, &b.st, 1))) return fail(rc);
     if (hipMemset(b.racc, 0, 48 * N) != hipSuccess || hipMemset(b.E, 0, 8 * N) != hipSuccess ||
@@ -243,9 +268,34 @@ int pic1d2v_upload(pic1d2v_ctx* c, int32_t species, const double* x, const doubl
     }
     HIPCHK(c, hipSetDevice(c->cfg.device));
     const size_t P = (size_t)c->cfg.P, off = (size_t)species * P;
-    HIPCHK(c, hipMemcpyAsync(c->b.x + off, x, 8 * P, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->b.vx + off, vx, 8 * P, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->b.vy + off, vy, 8 * P, hipMemcpyHostToDevice, c->stream));
+    Buffers12& b = c->b;
+    if (b.precision == PIC_PREC_FP64) {
+        HIPCHK(c, hipMemcpyAsync(b.x + off, x, 8 * P, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(b.vx + off, vx, 8 * P, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(b.vy + off, vy, 8 * P, hipMemcpyHostToDevice, c->stream));
+    } else {
+        // x mod 1 on the 2^-32 grid (2^32 wraps to 0); v * dt * 2^32 as hi + lo
+        std::vector<uint32_t> q(P);
+        std::vector<float> h(P), l(P);
+        for (size_t j = 0; j < P; ++j) {
+            const double f = x[j] - std::floor(x[j]);
+            q[j] = (uint32_t)(uint64_t)(int64_t)std::nearbyint(f * 4294967296.0);
+        }
+        HIPCHK(c, hipMemcpy(b.xq + off, q.data(), 4 * P, hipMemcpyHostToDevice));
+        const double* vs[2] = {vx, vy};
+        float* dh[2] = {b.vxh, b.vyh};
+        float* dl[2] = {b.vxl, b.vyl};
+        for (int k = 0; k < 2; ++k) {
+            for (size_t j = 0; j < P; ++j) {
+                const double u = vs[k][j] * b.vstep;
+                h[j] = (float)u;
+                l[j] = (float)(u - (double)h[j]);
+            }
+            HIPCHK(c, hipMemcpy(dh[k] + off, h.data(), 4 * P, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(dl[k] + off, l.data(), 4 * P, hipMemcpyHostToDevice));
+        }
+        HIPCHK(c, hipMemsetAsync(b.racc32, 0, 16 * (size_t)c->cfg.N, c->stream));
+    }
     // the run restarts: step 0, no field bins or diagnostics, charge to be deposited
     HIPCHK(c, hipMemsetAsync(c->b.st, 0, sizeof(State12), c->stream));
     HIPCHK(c, hipMemsetAsync(c->b.racc, 0, 48 * (size_t)c->cfg.N, c->stream));
@@ -267,10 +317,30 @@ int pic1d2v_download(pic1d2v_ctx* c, int32_t species, double* x, double* vx, dou
     }
     HIPCHK(c, hipSetDevice(c->cfg.device));
     const size_t P = (size_t)c->cfg.P, off = (size_t)species * P;
-    if (x) HIPCHK(c, hipMemcpyAsync(x, c->b.x + off, 8 * P, hipMemcpyDeviceToHost, c->stream));
-    if (vx) HIPCHK(c, hipMemcpyAsync(vx, c->b.vx + off, 8 * P, hipMemcpyDeviceToHost, c->stream));
-    if (vy) HIPCHK(c, hipMemcpyAsync(vy, c->b.vy + off, 8 * P, hipMemcpyDeviceToHost, c->stream));
+    const Buffers12& b = c->b;
+    if (b.precision == PIC_PREC_FP64) {
+        if (x) HIPCHK(c, hipMemcpyAsync(x, b.x + off, 8 * P, hipMemcpyDeviceToHost, c->stream));
+        if (vx) HIPCHK(c, hipMemcpyAsync(vx, b.vx + off, 8 * P, hipMemcpyDeviceToHost, c->stream));
+        if (vy) HIPCHK(c, hipMemcpyAsync(vy, b.vy + off, 8 * P, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return PIC_OK;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (x) {
+        std::vector<uint32_t> q(P);
+        HIPCHK(c, hipMemcpy(q.data(), b.xq + off, 4 * P, hipMemcpyDeviceToHost));
+        for (size_t j = 0; j < P; ++j) x[j] = (double)q[j] * 0x1p-32;
+    }
+    const float* sh[2] = {b.vxh, b.vyh};
+    const float* sl[2] = {b.vxl, b.vyl};
+    double* dst[2] = {vx, vy};
+    std::vector<float> h(P), l(P);
+    for (int k = 0; k < 2; ++k) {
+        if (!dst[k]) continue;
+        HIPCHK(c, hipMemcpy(h.data(), sh[k] + off, 4 * P, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(l.data(), sl[k] + off, 4 * P, hipMemcpyDeviceToHost));
+        for (size_t j = 0; j < P; ++j) dst[k][j] = ((double)h[j] + (double)l[j]) * b.inv_vstep;
+    }
     return PIC_OK;
 }
 
@@ -396,6 +466,7 @@ int pic1d2v_deposit(pic1d2v_ctx* c, const double* x, double* rho) {
     HIPCHK(c, hipMemcpyAsync(c->op[0], x, 8 * NP, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemsetAsync(c->op[1], 0, 48 * N, c->stream));
     Buffers12 o = c->b;
+    o.precision = PIC_PREC_FP64;  // the operators are FP64 in every context
     o.x = (double*)c->op[0];
     o.racc = (unsigned long long*)c->op[1];
     launch12_deposit(o, c->stream);

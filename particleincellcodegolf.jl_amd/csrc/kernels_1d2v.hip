@@ -69,9 +69,12 @@ k12_deposit(const Tile12* __restrict__ tiles, const double* __restrict__ x, u64*
 
 // E = real(ifft(fft(rho) ./ ik)) from the exact signed charge (which it
 // clears), Es[:, ceil(t/bin)] .+= E and sum(abs2, E).  One block.
+// Mixed (racc32 set): the charge is per species u64 quanta, cq[s] each
+// (q_s * w * 2^-30), summed in FP64; E32 gets E * escale2 in FP32.
 __global__ void __launch_bounds__(kSolveThreads)
 k12_solve(u64* __restrict__ racc, double* __restrict__ E, double* __restrict__ Es, int64_t cap_bins,
-          State12* __restrict__ st, const double2* __restrict__ twp, int N, int log2N, int qb, int bin) {
+          State12* __restrict__ st, const double2* __restrict__ twp, int N, int log2N, int qb, int bin,
+          u64* __restrict__ racc32, double cq0, double cq1, float* __restrict__ E32, double escale2) {
     extern __shared__ double sm[];
     const int NP = solve_np(N);
     double* re = sm;
@@ -80,16 +83,24 @@ k12_solve(u64* __restrict__ racc, double* __restrict__ E, double* __restrict__ E
     SolveRegs Rg;
     solve_prefetch(Rg, twp, N, log2N);
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
-        u64* a = racc + 3 * j;
-        u64* b = racc + 3 * N + 3 * j;
-        const u128 pos = limbs_to_u128(a[0], a[1], a[2]), neg = limbs_to_u128(b[0], b[1], b[2]);
-        a[0] = 0;
-        a[1] = 0;
-        a[2] = 0;
-        b[0] = 0;
-        b[1] = 0;
-        b[2] = 0;
-        const double r = pos >= neg ? fxp_to(pos - neg, qb) : -fxp_to(neg - pos, qb);
+        double r;
+        if (racc32) {
+            const u64 a0 = racc32[j], a1 = racc32[N + j];
+            racc32[j] = 0;
+            racc32[N + j] = 0;
+            r = cq0 * (double)a0 + cq1 * (double)a1;
+        } else {
+            u64* a = racc + 3 * j;
+            u64* b = racc + 3 * N + 3 * j;
+            const u128 pos = limbs_to_u128(a[0], a[1], a[2]), neg = limbs_to_u128(b[0], b[1], b[2]);
+            a[0] = 0;
+            a[1] = 0;
+            a[2] = 0;
+            b[0] = 0;
+            b[1] = 0;
+            b[2] = 0;
+            r = pos >= neg ? fxp_to(pos - neg, qb) : -fxp_to(neg - pos, qb);
+        }
         const int k = padi((int)brev((uint32_t)j, log2N));
         re[k] = r;
         im[k] = 0.0;
@@ -101,6 +112,7 @@ k12_solve(u64* __restrict__ racc, double* __restrict__ E, double* __restrict__ E
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
         const double e = solve_at(re, j);
         E[j] = e;
+        if (E32) E32[j] = (float)(e * escale2);
         if (bi < cap_bins) Es[bi * N + j] = Es[bi * N + j] + e;
     }
     const double se = block_chunk_sum(red, N, [&](int64_t i) {
@@ -264,6 +276,7 @@ __global__ void k12_op_boris(double* __restrict__ vx, double* __restrict__ vy, c
 int push12_lds_bytes(int N) { return 32 * N; }  // E + three limb arrays
 
 void launch12_deposit(const Buffers12& b, hipStream_t s) {
+    if (b.precision == 1) return launch12_deposit32(b, s);
     const int lds = 24 * b.N;
     u64* r = (u64*)b.racc;
     if (b.R == 6)
@@ -274,11 +287,15 @@ void launch12_deposit(const Buffers12& b, hipStream_t s) {
 
 void launch12_solve(const Buffers12& b, hipStream_t s) {
     const int lds = solve_lds_doubles(b.N) * 8;
+    const bool m = b.precision == 1;
+    const double cq0 = m ? b.q[0] * b.w * 0x1p-30 : 0.0, cq1 = m ? b.q[1] * b.w * 0x1p-30 : 0.0;
     k12_solve<<<1, kSolveThreads, lds, s>>>((u64*)b.racc, b.E, b.Es, b.cap_bins, b.st, b.tw, b.N, b.log2N,
-                                            b.qbits, b.bin);
+                                            b.qbits, b.bin, m ? (u64*)b.racc32 : nullptr, cq0, cq1,
+                                            m ? b.E32 : nullptr, b.escale2);
 }
 
 void launch12_push(const Buffers12& b, hipStream_t s) {
+    if (b.precision == 1) return launch12_push32(b, s);
     const int lds = push12_lds_bytes(b.N);
     u64* r = (u64*)b.racc;
     if (b.R == 6)
@@ -290,6 +307,7 @@ void launch12_push(const Buffers12& b, hipStream_t s) {
 }
 
 void launch12_diag(const Buffers12& b, hipStream_t s) {
+    if (b.precision == 1) return launch12_diag32(b, s);
     k12_diag_chunks<<<(unsigned)b.nchunks, 256, 0, s>>>(b.vx, b.vy, b.P, b.ns, b.mass[0], b.mass[1], b.bin,
                                                        b.st, b.dpart);
     k12_diag_final<<<1, 256, 0, s>>>(b.dpart, b.nchunks, b.P, b.N, b.n0, b.bin, b.st, b.D, b.cap_bins);

@@ -15,6 +15,7 @@
 // of 2^-30 of a particle's charge, so the per-cell total is exact and
 // independent of scheduling.  The FFT, the convergence
 // test and the diagnostics stay FP64 (kernels_1d.hip).
+#include "pic1d2v_internal.h"
 #include "pic_device.h"
 #include "pic_internal.h"
 #include "stencil1d.h"
@@ -526,6 +527,203 @@ void launch_simp32_push(const RunBuffers& b, int k, bool last_slot, hipStream_t 
         k_simp_push32<7><<<g, t, lds, s>>>(b.insts, b.st, b.tiles, b.Xq[0], b.Xq[1], b.Vh[0], b.Vh[1], b.Vl[0],
                                            b.Vl[1], b.vf[0], b.vf[1], b.G1s, b.E32s2, b.E32, r2, r3, k, b.halt, ls,
                                            b.dpart);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed magnetised 1D2V (src/NGP1D2V.jl, src/NGP1D2V2S.jl): the Gaussian
+// deposit and gather over +-5 cells with the FP32 weights, the Boris update
+// (src/NGP1D2V2S.jl:5-11, B along z) computed in FP32 as velocity changes and
+// added to hi + lo pairs, the move by the rounded velocity, exact charge
+// quanta per species.  The solve, the binned field and the diagnostics
+// stay FP64 (kernels_1d2v.hip).
+// ---------------------------------------------------------------------------
+
+// v += d on a hi + lo pair (two-sum, error folded into lo, renormalised).
+__device__ __forceinline__ void add_pair(float& hi, float& lo, float d) {
+    const float s = hi + d;
+    const float bb = s - hi;
+    const float err = (hi - (s - bb)) + (d - bb);
+    const float t = lo + err;
+    hi = s + t;
+    lo = t - (hi - s);
+}
+
+// rint(hi + lo): the integer part of hi first, so the rounding sees both words.
+__device__ __forceinline__ int32_t rint_pair(float hi, float lo) {
+    const float hi_i = rintf(hi);
+    return (int32_t)hi_i + (int32_t)rintf((hi - hi_i) + lo);
+}
+
+// The charge of every tile's particles at their current positions.
+__global__ void __launch_bounds__(1024)
+k12_deposit32(const Tile12* __restrict__ tiles, const uint32_t* __restrict__ xq, u64* __restrict__ racc32, int N,
+              int log2N) {
+    extern __shared__ u64 lds[];
+    const Tile12 T = tiles[blockIdx.x];
+    const int NPAD = pad_cells(N);
+    uint32_t* lo = reinterpret_cast<uint32_t*>(lds);
+    uint32_t* hi = lo + NPAD;
+    zero_lds(lds, NPAD);
+    __syncthreads();
+    const int shift = 32 - log2N;
+    for (int64_t p = threadIdx.x; p < T.count; p += blockDim.x) {
+        Stencil32 s;
+        stencil32(xq[T.start + p], shift, s);
+        lane_deposit(lo, hi, s, true);
+    }
+    __syncthreads();
+    flush_charge32(lo, hi, N, NPAD, racc32 + (int64_t)T.species * N);
+}
+
+// Ej at x, the Boris update, x += vx*dt (mod 1 by the fixed point), and the
+// charge at the new x for the next step.
+__global__ void __launch_bounds__(1024)
+k12_push32(const Tile12* __restrict__ tiles, uint32_t* __restrict__ xq, float* __restrict__ vxh,
+           float* __restrict__ vxl, float* __restrict__ vyh, float* __restrict__ vyl, const float* __restrict__ E32,
+           u64* __restrict__ racc32, int N, int log2N, float kh0, float kh1, float t30, float t31) {
+    extern __shared__ u64 lds[];
+    const Tile12 T = tiles[blockIdx.x];
+    const int NPAD = pad_cells(N);
+    const int32_t nmask = N - 1;
+    uint32_t* lo = reinterpret_cast<uint32_t*>(lds);
+    uint32_t* hi = lo + NPAD;
+    float* Eld = reinterpret_cast<float*>(lds + NPAD);
+    for (int i = threadIdx.x; i < NPAD; i += blockDim.x) Eld[i] = E32[(i - kHalo) & nmask];
+    zero_lds(lds, NPAD);
+    __syncthreads();
+    const float kh = T.species ? kh1 : kh0, t3 = T.species ? t31 : t30;
+    const float s3 = 1.0f + t3 * t3;
+    const int shift = 32 - log2N;
+    for (int64_t p = threadIdx.x; p < T.count; p += blockDim.x) {
+        const int64_t g = T.start + p;
+        uint32_t x = xq[g];
+        float ux = vxh[g], lx = vxl[g], uy = vyh[g], ly = vyl[g];
+        Stencil32 s;
+        stencil32(x, shift, s);
+        const float* e = Eld + s.i0;
+        float gsum = 0.0f;
+#pragma unroll
+        for (int j = 0; j < kWin; ++j) gsum = fmaf(e[j], s.w[j], gsum);
+        const float ek = gsum * kh;
+        // m = v + E h; c = m x t; a = m + c; d = a x t; v' = m + 2d/s + E h
+        const float m1 = ux + ek, m2 = uy;
+        const float a1 = m1 + m2 * t3, a2 = m2 - m1 * t3;
+        add_pair(ux, lx, (ek + ek) + (2.0f * (a2 * t3)) / s3);
+        add_pair(uy, ly, (2.0f * (-(a1 * t3))) / s3);
+        x += (uint32_t)rint_pair(ux, lx);
+        xq[g] = x;
+        vxh[g] = ux;
+        vxl[g] = lx;
+        vyh[g] = uy;
+        vyl[g] = ly;
+        stencil32(x, shift, s);
+        lane_deposit(lo, hi, s, true);
+    }
+    __syncthreads();
+    flush_charge32(lo, hi, N, NPAD, racc32 + (int64_t)T.species * N);
+}
+
+// Exact diagnostics partials when t % bin == 0, per chunk of 4096 particle
+// indices and per species: sums of vx^2 + vy^2 on a 2^-50 grid and of vx, vy
+// on a 2^-40 grid (velocities in natural units), so the totals do not depend
+// on the order of the particles or the chunks.
+__global__ void __launch_bounds__(256)
+k12_diag_chunks32(const float* __restrict__ vxh, const float* __restrict__ vxl, const float* __restrict__ vyh,
+                  const float* __restrict__ vyl, int64_t P, int ns, double inv_vstep, int bin,
+                  const State12* __restrict__ st, i128* __restrict__ part) {
+    __shared__ i128 red[3 * 256];
+    if ((st->step + 1) % bin != 0) return;
+    const int64_t c0 = (int64_t)blockIdx.x * kTreeChunk;
+    const int64_t n = P - c0 < kTreeChunk ? P - c0 : kTreeChunk;
+    for (int s = 0; s < ns; ++s) {
+        i128 a = 0, bx = 0, by = 0;
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const int64_t j = (int64_t)s * P + c0 + i;
+            const double vx = ((double)vxh[j] + (double)vxl[j]) * inv_vstep;
+            const double vy = ((double)vyh[j] + (double)vyl[j]) * inv_vstep;
+            a += fix_round128(vx * vx + vy * vy, kV2Bits);
+            bx += fix_round128(vx, kV1Bits);
+            by += fix_round128(vy, kV1Bits);
+        }
+        red[threadIdx.x] = a;
+        red[256 + threadIdx.x] = bx;
+        red[512 + threadIdx.x] = by;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) {
+                red[threadIdx.x] += red[threadIdx.x + h];
+                red[256 + threadIdx.x] += red[256 + threadIdx.x + h];
+                red[512 + threadIdx.x] += red[512 + threadIdx.x + h];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            i128* o = part + 6 * (int64_t)blockIdx.x + 3 * s;
+            o[0] = red[0];
+            o[1] = red[256];
+            o[2] = red[512];
+        }
+        __syncthreads();
+    }
+}
+
+// D[t/bin, 1:5] (src/NGP1D2V2S.jl:52-56) from the exact partials when
+// t % bin == 0; then step += 1.
+__global__ void __launch_bounds__(256)
+k12_diag_final32(const i128* __restrict__ part, int64_t nchunks, int64_t P, int ns, double m0, double m1, int N,
+                 double n0, int bin, State12* __restrict__ st, double* __restrict__ D, int64_t cap_bins) {
+    __shared__ i128 red[6 * 256];
+    const int64_t t = st->step + 1;
+    if (t % bin == 0) {
+        i128 v[6] = {0, 0, 0, 0, 0, 0};
+        for (int64_t c = threadIdx.x; c < nchunks; c += blockDim.x)
+            for (int q = 0; q < 3 * ns; ++q) v[q] += part[6 * c + q];
+        for (int q = 0; q < 6; ++q) red[q * 256 + threadIdx.x] = v[q];
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h)
+                for (int q = 0; q < 6; ++q) red[q * 256 + threadIdx.x] += red[q * 256 + threadIdx.x + h];
+            __syncthreads();
+        }
+        const int64_t bi = t / bin - 1;
+        if (threadIdx.x == 0 && bi < cap_bins) {
+            const double k2 = 0x1p-50, k1 = 0x1p-40, Pd = (double)P;
+            const double ms[2] = {m0, m1};
+            double sk = 0.0, sx = 0.0, sy = 0.0;
+            for (int s = 0; s < ns; ++s) {
+                sk = sk + ms[s] * ((double)red[(3 * s) * 256] * k2);
+                sx = sx + (ms[s] * ((double)red[(3 * s + 1) * 256] * k1)) / Pd;
+                sy = sy + (ms[s] * ((double)red[(3 * s + 2) * 256] * k1)) / Pd;
+            }
+            double* row = D + 5 * bi;
+            row[0] = row[0] + (st->se / (double)N) / 2.0;
+            row[1] = row[1] + ((sk * n0) / Pd) / 2.0;
+            row[3] = row[3] + sx;
+            row[4] = row[4] + sy;
+            row[2] = row[2] + (row[0] + row[1]);
+            for (int c = 0; c < 3; ++c) row[c] = row[c] * (2.0 / n0);
+        }
+    }
+    if (threadIdx.x == 0) st->step = t;
+}
+
+void launch12_deposit32(const Buffers12& b, hipStream_t s) {
+    const size_t lds = 8 * (size_t)pad_cells(b.N);
+    k12_deposit32<<<b.ntiles, b.block, lds, s>>>(b.tiles, b.xq, b.racc32, b.N, b.log2N);
+}
+
+void launch12_push32(const Buffers12& b, hipStream_t s) {
+    const size_t lds = 12 * (size_t)pad_cells(b.N);
+    k12_push32<<<b.ntiles, b.block, lds, s>>>(b.tiles, b.xq, b.vxh, b.vxl, b.vyh, b.vyl, b.E32, b.racc32, b.N,
+                                              b.log2N, b.kick_half[0], b.kick_half[1], b.t3[0], b.t3[1]);
+}
+
+void launch12_diag32(const Buffers12& b, hipStream_t s) {
+    i128* part = reinterpret_cast<i128*>(b.dpart32);
+    k12_diag_chunks32<<<(unsigned)b.nchunks, 256, 0, s>>>(b.vxh, b.vxl, b.vyh, b.vyl, b.P, b.ns, b.inv_vstep,
+                                                         b.bin, b.st, part);
+    k12_diag_final32<<<1, 256, 0, s>>>(part, b.nchunks, b.P, b.ns, b.mass[0], b.mass[1], b.N, b.n0, b.bin, b.st,
+                                       b.D, b.cap_bins);
 }
 
 int configure_kernels_mixed() {

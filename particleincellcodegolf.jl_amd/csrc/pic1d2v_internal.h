@@ -41,6 +41,20 @@ struct Buffers12 {
     const double2* tw;             // twiddle table (pass_twiddle_table)
     Tile12* tiles;
     int32_t ntiles, block;
+    // mixed (precision 1): positions as 32-bit fractions of the box,
+    // velocities as FP32 hi + lo pairs in steps of that grid per time step
+    // (v * dt * 2^32), FP32 copy of E in the gather's units (E * dt^2 * 4),
+    // per-species u64 charge quanta (2^-30 of a particle) and 6 exact 128-bit
+    // diagnostics partials per chunk
+    int32_t precision;
+    uint32_t* xq;
+    float *vxh, *vxl, *vyh, *vyl;
+    float* E32;
+    unsigned long long* racc32;
+    void* dpart32;
+    double vstep, inv_vstep, escale2;
+    float kick_half[2];  // q_m / 2: the half kick per gathered step-unit field
+    float t3[2];         // B0 * (dt / 2) * q_m
 };
 
 int push12_lds_bytes(int N);
@@ -54,5 +68,9 @@ void launch12_op_gather(const Buffers12& b, const double* E, const double* x, in
 void launch12_op_boris(double* vx, double* vy, const double* Ej, int64_t n, double B, double dt, double q_m,
                        hipStream_t s);
 int configure_kernels_1d2v();
+// kernels_mixed.hip: the mixed forms of the deposit, push and diagnostics
+void launch12_deposit32(const Buffers12& b, hipStream_t s);
+void launch12_push32(const Buffers12& b, hipStream_t s);
+void launch12_diag32(const Buffers12& b, hipStream_t s);
 
 }  // namespace pic

@@ -570,20 +570,24 @@ struct Pic1D2VConfig
     B0::Float64
     dt::Float64
     device::Int32
+    precision::Int32
 end
 
 """
-    Pic1D2VConfig(; N, P, bin, w, n0, B0, dt, q=(1.0,), q_m=(1.0,), mass=(1.0,), shape=:gauss7, device=0)
+    Pic1D2VConfig(; N, P, bin, w, n0, B0, dt, q=(1.0,), q_m=(1.0,), mass=(1.0,), shape=:gauss7, device=0,
+                  precision=:fp64)
 
 One species (src/NGP1D2V.jl: `q = q_m = mass = (1.0,)`, `w = n0/P`) or two
 (src/NGP1D2V2S.jl: `q = (-1.0, 1.0)`, `q_m = (-1.0, 1/M)`, `mass = (1.0, M)`,
-`w = n0/2P`); `bin = T÷TO`.
+`w = n0/2P`); `bin = T÷TO`.  `precision = :mixed` runs the loop with FP32
+particle arithmetic and exact integer charge sums.
 """
-function Pic1D2VConfig(; N, P, bin, w, n0, B0, dt, q=(1.0,), q_m=(1.0,), mass=(1.0,), shape=:gauss7, device=0)
+function Pic1D2VConfig(; N, P, bin, w, n0, B0, dt, q=(1.0,), q_m=(1.0,), mass=(1.0,), shape=:gauss7, device=0,
+                       precision=:fp64)
     ns = length(q)
     pad(t) = (Float64(t[1]), ns == 2 ? Float64(t[2]) : 0.0)
     Pic1D2VConfig(Int32(N), SHAPES[shape], Int32(ns), Int32(bin), Int64(P), pad(q), pad(q_m), pad(mass),
-                  Float64(w), Float64(n0), Float64(B0), Float64(dt), Int32(device))
+                  Float64(w), Float64(n0), Float64(B0), Float64(dt), Int32(device), PRECISIONS[precision])
 end
 
 mutable struct Context1D2V

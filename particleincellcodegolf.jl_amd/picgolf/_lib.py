@@ -100,7 +100,7 @@ class pic1d2v_config(ctypes.Structure):
         ("N", ctypes.c_int32), ("shape", ctypes.c_int32), ("nspecies", ctypes.c_int32), ("bin", ctypes.c_int32),
         ("P", ctypes.c_int64), ("q", ctypes.c_double * 2), ("q_m", ctypes.c_double * 2),
         ("mass", ctypes.c_double * 2), ("w", ctypes.c_double), ("n0", ctypes.c_double), ("B0", ctypes.c_double),
-        ("dt", ctypes.c_double), ("device", ctypes.c_int32),
+        ("dt", ctypes.c_double), ("device", ctypes.c_int32), ("precision", ctypes.c_int32),
     ]
 
 

@@ -24,7 +24,8 @@ SHAPES = {"gauss6": L.SHAPE_GAUSS6, "gauss7": L.SHAPE_GAUSS7}
 class Config1D2V:
     """N cells, P particles per species, time bins of `bin` steps (the
     scripts' T/TO), per-species charge q, charge-to-mass ratio q_m and kinetic
-    weight mass, particle weight w, n0, B0 along z and dt."""
+    weight mass, particle weight w, n0, B0 along z and dt; precision "fp64"
+    (bitwise equal to the CPU restatement) or "mixed"."""
     N: int
     P: int
     bin: int
@@ -37,6 +38,7 @@ class Config1D2V:
     mass: tuple = (1.0,)
     shape: str = "gauss7"
     device: int = 0
+    precision: str = "fp64"
 
     @property
     def nspecies(self):
@@ -51,6 +53,8 @@ class Config1D2V:
             c.q[s], c.q_m[s], c.mass[s] = float(self.q[s]), float(self.q_m[s]), float(self.mass[s])
         c.w, c.n0, c.B0, c.dt = float(self.w), float(self.n0), float(self.B0), float(self.dt)
         c.device = int(self.device)
+        c.precision = {"fp64": L.FP64, "mixed": L.MIXED}[self.precision] if isinstance(self.precision, str) \
+            else int(self.precision)
         return c
 
 

@@ -190,8 +190,10 @@ def test_2d3v_struct_layout_matches_the_header(tmp_path):
     import ctypes
     import subprocess
     from picgolf import _lib as L
-    structs = {"pic2d3v_config": L.pic2d3v_config, "pic2d3v_species": L.pic2d3v_species}
-    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pic2d3v.h"', "int main(void) {"]
+    structs = {"pic2d3v_config": L.pic2d3v_config, "pic2d3v_species": L.pic2d3v_species,
+               "pic1d2v_config": L.pic1d2v_config}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pic2d3v.h"', '#include "pic1d2v.h"',
+             "int main(void) {"]
     for name, cls in structs.items():
         lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
         for f, _ in cls._fields_:
