@@ -105,6 +105,29 @@ __device__ __forceinline__ void lane_deposit(uint32_t* __restrict__ lo, uint32_t
         if (old[j] + q[j] < old[j]) atomicAdd(&hi[s.i0 + j], 1u);
 }
 
+// One particle's stencil quanta added into (sign +1) or taken out of (-1)
+// the block's u64 LDS accumulator of charge changes (two's complement, so the
+// sums stay exact modulo 2^64 and the instance's totals, which are never
+// negative, come out exact).
+template <int SIGN>
+__device__ __forceinline__ void lane_change(u64* __restrict__ acc, const Stencil32& s) {
+    u64* a = acc + s.i0;
+#define PIC_CH(J) atomicAdd(&a[J], (u64)((int64_t)SIGN * (int64_t)quanta<J>(s.w[J])));
+    PIC_CH(0) PIC_CH(1) PIC_CH(2) PIC_CH(3) PIC_CH(4) PIC_CH(5) PIC_CH(6) PIC_CH(7) PIC_CH(8) PIC_CH(9)
+    PIC_CH(10)
+#undef PIC_CH
+}
+
+// Adds a block's u64 LDS charge changes into the instance's accumulator
+// (modulo 2^64; cell c collects every slot k == c + kHalo (mod N)).
+__device__ __forceinline__ void flush_delta(const u64* acc, int N, int NPAD, u64* __restrict__ racc) {
+    for (int c = threadIdx.x; c < N; c += blockDim.x) {
+        u64 v = 0;
+        for (int kk = (c + kHalo) % N; kk < NPAD; kk += N) v += acc[kk];
+        if (v) atomicAdd(racc + c, v);
+    }
+}
+
 // Adds this block's halo-padded LDS charge into the instance's per-cell
 // accumulator (device-scope integer atomics; cell c collects every slot
 // k == c + kHalo (mod N)).
@@ -180,6 +203,13 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     // (centre x + V*dt/2 with v_0 = V), which a first-deposit launch would
     // otherwise compute from the state it writes
     const bool deposit = !last || prime;
+    // a middle sweep (k >= 1, not the last) finds the charge of the centres
+    // it gathers at still in the accumulator (the solve keeps it) and adds,
+    // for each particle whose next centre differs, the exact quanta of its
+    // new stencil minus those of its old one; once the Picard iterate settles
+    // most centres repeat bit for bit and add nothing.  The result is the
+    // charge a full deposit gives, bit for bit.
+    const bool delta = (MODE == 1) && k >= 1 && !last;
     if (deposit)
         for (int i = threadIdx.x; i < NPAD; i += blockDim.x) acc[i] = 0;
     __syncthreads();
@@ -245,6 +275,14 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
                     const float vn = vh + kick;
                     vout[g] = vn;
                     cdep = advance(xq, vn + vh, cscale);
+                    if (delta) {
+                        if (cdep != cq) {
+                            lane_change<-1>(acc, s);
+                            stencil32(cdep, shift, s);
+                            lane_change<1>(acc, s);
+                        }
+                        dep = false;
+                    }
                 }
             } else {
                 cdep = advance(xq, vh + vh, cscale);
@@ -252,14 +290,15 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
             // once the Picard iterate settles the next centre repeats bit for bit
             if (dep && (MODE == 0 || cdep != cq)) stencil32(cdep, shift, s);
         }
-        if (deposit) lane_deposit(acc_lo, acc_hi, s, dep);
+        if (deposit && !delta) lane_deposit(acc_lo, acc_hi, s, dep);
     }
     if (last) {
         store_tile_kinetic(q2, q1, lds + NPAD + (NPAD + 1) / 2, dpart);
         if (!prime) return;
     }
     __syncthreads();
-    flush_charge32(acc_lo, acc_hi, N, NPAD, racc + I.red0);
+    if (delta) flush_delta(acc, N, NPAD, racc + I.red0);
+    else flush_charge32(acc_lo, acc_hi, N, NPAD, racc + I.red0);
 }
 
 void launch_mid_push32(const RunBuffers& b, int mode, int k, bool last_slot, hipStream_t s) {
