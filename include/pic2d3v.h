@@ -31,10 +31,19 @@
  *   diagnose!(LorenzGaugeDiagnostics, ...)  with its in-place inverse
  *                                 transforms of the state          :1332-1398
  *
- * FP64 throughout.  Results are bit-identical to the CPU restatement: per-cell
+ * FP64 by default.  Results are bit-identical to the CPU restatement: per-cell
  * charge is an exact signed fixed-point sum (independent of the particle
  * order and of the per-thread grids of :284-285), every other sum uses the
  * canonical reduction tree.
+ *
+ * precision = PIC_PREC_MIXED (electrostatic field only) holds positions as
+ * 32-bit fractions of the box and velocities as FP32 hi + lo pairs, gathers
+ * an FP32 copy of the field with FP32 shape weights, and deposits each
+ * species' charge as exact integer quanta (2^-22 of a cell per axis, summed
+ * in 128 bits per cell), so its results too are independent of the particle
+ * order.  The solve, the gathered field Exy and the diagnostics stay FP64;
+ * uploads and downloads convert.  The operator calls below are FP64 in
+ * either precision.
  *
  * Layouts follow Julia's.  A species' particles are the 5 x P column-major
  * matrix xyv: particle p is (x, y, vx, vy, vz) at xyv[5p .. 5p+4].  Grids are
@@ -173,6 +182,7 @@ typedef struct pic2d3v_config {
     double theta_source;   /* semi-implicit: sourceimex theta                          */
     double rtol;           /* semi-implicit: rtol of the convergence test (sqrt(eps))  */
     int32_t maxiters;      /* semi-implicit: at most maxiters + 1 iterations (10)      */
+    int32_t precision;     /* PIC_PREC_FP64, or PIC_PREC_MIXED (electrostatic field)  */
 } pic2d3v_config;
 
 typedef struct pic2d3v_ctx pic2d3v_ctx;

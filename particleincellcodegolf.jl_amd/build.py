@@ -18,7 +18,7 @@ LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libpicgolf.so")
 ARCH = os.environ.get("PIC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels_1d.hip", "kernels_mixed.hip", "kernels_small.hip", "noalloc.hip", "kernels_2d.hip", "kernels_1d2v.hip", "kernels_2d3v.hip",
+SOURCES = ["kernels_1d.hip", "kernels_mixed.hip", "kernels_small.hip", "noalloc.hip", "kernels_2d.hip", "kernels_1d2v.hip", "kernels_2d3v.hip", "kernels_2d3v_mixed.hip",
            "kernels_2d3v_lg.hip", "kernels_em1d.hip", "capi.cpp", "capi2d.cpp", "capi1d2v.cpp", "capi2d3v.cpp",
            "capi_em1d.cpp"]
 

@@ -39,6 +39,11 @@ struct pic2d3v_ctx {
     Diag23 d{};
     Buffers2D b{};           // the 2D solve's descriptor and buffers
     double2* E = nullptr;    // the gathered field Exy (interior, (Ex, Ey) per cell)
+    // mixed precision: the FP32 copy of Exy the particles gather, the charge
+    // density the solve reads
+    bool mixed = false;
+    float2* E32 = nullptr;
+    double* rho = nullptr;
     std::vector<void*> allocs;
     double* tw = nullptr;
     int64_t step = 0;        // t of the next step
@@ -222,6 +227,14 @@ static int validate(const pic2d3v_config* g, std::string& why) {
         why = "the species hold more than 2^26 particles together";
         return PIC_EINVAL;
     }
+    if (g->precision != PIC_PREC_FP64 && g->precision != PIC_PREC_MIXED) {
+        why = "precision must be PIC_PREC_FP64 or PIC_PREC_MIXED";
+        return PIC_EINVAL;
+    }
+    if (g->precision == PIC_PREC_MIXED && g->field != PIC2D3V_ELECTROSTATIC) {
+        why = "mixed precision is provided for the electrostatic field";
+        return PIC_EINVAL;
+    }
     if (g->field != PIC2D3V_ELECTROSTATIC && g->field != PIC2D3V_LORENZ_STAGGERED && g->field != PIC2D3V_LORENZ &&
         g->field != PIC2D3V_LORENZ_SEMI_IMPLICIT) {
         why = "field must be PIC2D3V_ELECTROSTATIC, PIC2D3V_LORENZ_STAGGERED, PIC2D3V_LORENZ or "
@@ -260,6 +273,11 @@ static int zero_state(pic2d3v_ctx* c) {
     HIPCHK(c, hipMemsetAsync(c->b.racc, 0, 3 * sizeof(unsigned long long) * (size_t)G, c->stream));
     HIPCHK(c, hipMemsetAsync(c->b.flag, 0, sizeof(int32_t), c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d.t, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+    if (c->mixed) {
+        HIPCHK(c, hipMemsetAsync(c->E32, 0, sizeof(float2) * (size_t)G, c->stream));
+        for (int s = 0; s < c->cfg.nspecies; ++s)
+            HIPCHK(c, hipMemsetAsync(c->sp[s].cnt, 0, 2 * sizeof(unsigned long long) * (size_t)G, c->stream));
+    }
     if (c->lorenz) {
         LG23& L = c->lg;
         HIPCHK(c, hipMemsetAsync(L.F, 0, sizeof(double2) * (c->semi ? kLGSlotsSI : kLGSlots) * (size_t)G, c->stream));
@@ -294,6 +312,59 @@ static int zero_state(pic2d3v_ctx* c) {
     return PIC_OK;
 }
 
+// v as a float hi + lo pair
+static void split_f(double v, float* hl) {
+    hl[0] = (float)v;
+    hl[1] = (float)(v - (double)hl[0]);
+}
+
+// The mixed state of a species (and of its regrouping buffers), its Boris
+// constants and its charge counters.  Boris as a change, v' = v + 2e + d*bs
+// with e = E*eh, m = v + e, d = (m + m x t) x t:
+//   ElectrostaticBoris (:39-54): t = B dt/2, eh = q_m dt/2,
+//     bs = 2 q_m^2 / (1 + q_m^2 |t|^2);
+//   ElectromagneticBoris (:56-69): t = B q_m dt/2, eh = q_m dt/2,
+//     bs = 2 / (1 + |t|^2).
+static int alloc_mixed(pic2d3v_ctx* c, Species23& sp, Species23& al, int64_t G) {
+    const int64_t P = sp.P;
+    int rc;
+    for (Species23* q : {&sp, &al}) {
+        q->mixed = 1;
+        if ((rc = dalloc(c, &q->xq, P)) || (rc = dalloc(c, &q->yq, P)) || (rc = dalloc(c, &q->vxh, P)) ||
+            (rc = dalloc(c, &q->vyh, P)) || (rc = dalloc(c, &q->vzh, P)) || (rc = dalloc(c, &q->vxl, P)) ||
+            (rc = dalloc(c, &q->vyl, P)) || (rc = dalloc(c, &q->vzl, P)))
+            return rc;
+    }
+    if ((rc = dalloc(c, &sp.cnt, 2 * G))) return rc;
+    al.cnt = sp.cnt;
+    const double dt_2 = c->cfg.dt / 2.0, qm = sp.q_m;
+    double t[3], tt = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        t[k] = c->cfg.B0[k] * (c->cfg.boris == PIC2D3V_BORIS_TEXTBOOK ? dt_2 * qm : dt_2);
+        tt += t[k] * t[k];
+    }
+    const double bs = c->cfg.boris == PIC2D3V_BORIS_TEXTBOOK ? 2.0 / (1.0 + tt) : 2.0 * qm * qm / (1.0 + qm * qm * tt);
+    for (Species23* q : {&sp, &al}) {
+        for (int k = 0; k < 3; ++k) q->bt[k] = (float)t[k];
+        q->bs = (float)bs;
+        q->eh = (float)(dt_2 * qm);
+        q->qscale = sp.qw * std::ldexp(1.0, -2 * kQ23);
+    }
+    return PIC_OK;
+}
+
+// exchanges the mixed state arrays of a and b
+static void swap_mixed(Species23& a, Species23& b) {
+    std::swap(a.xq, b.xq);
+    std::swap(a.yq, b.yq);
+    std::swap(a.vxh, b.vxh);
+    std::swap(a.vyh, b.vyh);
+    std::swap(a.vzh, b.vzh);
+    std::swap(a.vxl, b.vxl);
+    std::swap(a.vyl, b.vyl);
+    std::swap(a.vzl, b.vzl);
+}
+
 extern "C" int pic2d3v_create(pic2d3v_ctx** out, const pic2d3v_config* cfg) {
     PIC_TRACE("pic2d3v_create");
     if (!out || !cfg) {
@@ -323,6 +394,7 @@ extern "C" int pic2d3v_create(pic2d3v_ctx** out, const pic2d3v_config* cfg) {
     configure_kernels_2d();
     const int NX = cfg->NX, NY = cfg->NY, ns = cfg->nspecies;
     const int64_t G = (int64_t)NX * NY;
+    c->mixed = cfg->precision == PIC_PREC_MIXED;
     Grid23& g = c->g;
     g.NX = NX;
     g.NY = NY;
@@ -335,6 +407,10 @@ extern "C" int pic2d3v_create(pic2d3v_ctx** out, const pic2d3v_config* cfg) {
     g.dt = cfg->dt;
     for (int k = 0; k < 3; ++k) g.B[k] = cfg->B0[k];
     g.boris = cfg->boris;
+    g.sx = 32 - g.lx;
+    g.sy = 32 - g.ly;
+    split_f(cfg->dt * 4294967296.0 / cfg->Lx, g.stepx);
+    split_f(cfg->dt * 4294967296.0 / cfg->Ly, g.stepy);
     // qw = (charge*weight)/dV with dV = (Lx/NX)*(Ly/NY) (loop!, :533, :537)
     const double dV = (cfg->Lx / (double)NX) * (cfg->Ly / (double)NY);
     int64_t Ps[kMaxSpecies23];
@@ -365,6 +441,7 @@ extern "C" int pic2d3v_create(pic2d3v_ctx** out, const pic2d3v_config* cfg) {
         al.slot = sp.slot;
         launch23_identity(c->pid[s], sp.slot, q.P, c->stream);
         c->held_identity[s] = true;
+        if (cfg->precision == PIC_PREC_MIXED && (rc = alloc_mixed(c, sp, al, G))) return fail(rc);
     }
     {
         Tiles23& T = c->tiles;
@@ -394,6 +471,7 @@ extern "C" int pic2d3v_create(pic2d3v_ctx** out, const pic2d3v_config* cfg) {
         (rc = dalloc(c, &b.Ey, G)) || (rc = dalloc(c, &b.phi, G)) || (rc = dalloc(c, &b.flag, 1)) ||
         (rc = dalloc(c, &c->E, G)))
         return fail(rc);
+    if (c->mixed && ((rc = dalloc(c, &c->E32, G)) || (rc = dalloc(c, &c->rho, G)))) return fail(rc);
     b.E32 = nullptr;
     const std::vector<double> twp = pass_twiddle_table();
     if ((rc = dalloc(c, &c->tw, twp.size()))) return fail(rc);
@@ -529,7 +607,8 @@ extern "C" int pic2d3v_upload(pic2d3v_ctx* c, int32_t s, const double* xyv) {
     int rc;
     if ((rc = scratch(c, 0, sizeof(double) * 5 * (size_t)sp.P, &st))) return rc;
     HIPCHK(c, hipMemcpyAsync(st, xyv, sizeof(double) * 5 * (size_t)sp.P, hipMemcpyHostToDevice, c->stream));
-    launch23_unpack((const double*)st, sp, c->stream);
+    if (c->mixed) launch23m_unpack(c->g, (const double*)st, sp, c->stream);
+    else launch23_unpack((const double*)st, sp, c->stream);
     launch23_identity(c->pid[s], sp.slot, sp.P, c->stream);
     if ((rc = finish(c))) return rc;
     c->uploaded[s] = true;
@@ -551,7 +630,8 @@ extern "C" int pic2d3v_download(pic2d3v_ctx* c, int32_t s, double* xyv) {
     void* st = nullptr;
     int rc;
     if ((rc = scratch(c, 0, sizeof(double) * 5 * (size_t)sp.P, &st))) return rc;
-    launch23_pack(sp, (double*)st, c->stream);
+    if (c->mixed) launch23m_pack(c->g, sp, (double*)st, c->stream);
+    else launch23_pack(sp, (double*)st, c->stream);
     HIPCHK(c, hipMemcpyAsync(xyv, st, sizeof(double) * 5 * (size_t)sp.P, hipMemcpyDeviceToHost, c->stream));
     return finish(c);
 }
@@ -592,6 +672,13 @@ static int regroup(pic2d3v_ctx* c) {
     for (int s = 0; s < c->cfg.nspecies; ++s) {
         Species23& sp = c->sp[s];
         Species23& al = c->alt[s];
+        if (c->mixed) {
+            launch23m_regroup(c->g, c->tiles, sp, al, c->pid[s], c->pid_alt[s], c->stream);
+            swap_mixed(sp, al);
+            std::swap(c->pid[s], c->pid_alt[s]);
+            c->held_identity[s] = false;
+            continue;
+        }
         launch23_regroup(c->g, c->tiles, sp, al, c->pid[s], c->pid_alt[s], c->stream);
         std::swap(sp.x, al.x);
         std::swap(sp.y, al.y);
@@ -615,12 +702,17 @@ static int to_caller(pic2d3v_ctx* c, int32_t s) {
     if (c->held_identity[s]) return PIC_OK;
     Species23& sp = c->sp[s];
     Species23& al = c->alt[s];
-    launch23_to_caller(sp, al, c->stream);
-    std::swap(sp.x, al.x);
-    std::swap(sp.y, al.y);
-    std::swap(sp.vx, al.vx);
-    std::swap(sp.vy, al.vy);
-    std::swap(sp.vz, al.vz);
+    if (c->mixed) {
+        launch23m_to_caller(sp, al, c->stream);
+        swap_mixed(sp, al);
+    } else {
+        launch23_to_caller(sp, al, c->stream);
+        std::swap(sp.x, al.x);
+        std::swap(sp.y, al.y);
+        std::swap(sp.vx, al.vx);
+        std::swap(sp.vy, al.vy);
+        std::swap(sp.vz, al.vz);
+    }
     launch23_identity(c->pid[s], sp.slot, sp.P, c->stream);
     HIPCHK(c, hipGetLastError());
     c->held_identity[s] = true;
@@ -645,6 +737,9 @@ extern "C" int pic2d3v_sort(pic2d3v_ctx* c, int32_t s, double dx, double dy, int
     int rc;
     // sort!(species) permutes the caller's order: from that order
     if ((rc = to_caller(c, s))) return rc;
+    // mixed precision: the keys from FP64 copies of the state, the
+    // permutation applied to the mixed arrays
+    if (c->mixed) launch23m_to64(c->g, c->sp[s], c->stream);
     const Species23& sp = c->sp[s];
     const int64_t P = sp.P;
     Sort23 w{};
@@ -689,13 +784,22 @@ extern "C" int pic2d3v_sort(pic2d3v_ctx* c, int32_t s, double dx, double dy, int
         return PIC_EINVAL;
     }
     launch23_sort_order(sp, nb, w, c->stream);
-    if ((rc = scratch(c, 0, sizeof(double) * 5 * per, &p))) return rc;
-    double* dst[5];
-    for (int k = 0; k < 5; ++k) dst[k] = (double*)p + k * per;
-    launch23_sort_apply(sp, w.perm, dst, c->stream);
-    double* cols[5] = {sp.x, sp.y, sp.vx, sp.vy, sp.vz};
-    for (int k = 0; k < 5; ++k)
-        HIPCHK(c, hipMemcpyAsync(cols[k], dst[k], sizeof(double) * per, hipMemcpyDeviceToDevice, c->stream));
+    if (c->mixed) {
+        launch23m_permute(c->sp[s], c->alt[s], w.perm, c->stream);
+        swap_mixed(c->sp[s], c->alt[s]);
+        // the captured steps hold the old arrays: drop them once nothing runs
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        free_graphs(c);
+    } else {
+        if ((rc = scratch(c, 0, sizeof(double) * 5 * per, &p))) return rc;
+        double* dst[5];
+        for (int k = 0; k < 5; ++k) dst[k] = (double*)p + k * per;
+        launch23_sort_apply(sp, w.perm, dst, c->stream);
+        double* cols[5] = {sp.x, sp.y, sp.vx, sp.vy, sp.vz};
+        for (int k = 0; k < 5; ++k)
+            HIPCHK(c, hipMemcpyAsync(cols[k], dst[k], sizeof(double) * per, hipMemcpyDeviceToDevice, c->stream));
+    }
     std::vector<uint32_t> hp;
     if (perm) {
         hp.resize(per);
@@ -805,9 +909,17 @@ static void enqueue_step_lg(pic2d3v_ctx* c, int variant, hipStream_t s) {
 }
 
 static void enqueue_step(pic2d3v_ctx* c, bool diag, hipStream_t s) {
-    for (int k = 0; k < c->cfg.nspecies; ++k) launch23_push(c->g, c->sp[k], c->E, c->b.racc, s);
-    launch2d_solve(c->b, 3, nullptr, true, s);
-    launch23_update(c->g, c->d, c->b, c->E, c->cfg.field_update == PIC2D3V_FIELD_ACCUMULATE ? 1 : 0, s);
+    const int acc = c->cfg.field_update == PIC2D3V_FIELD_ACCUMULATE ? 1 : 0;
+    if (c->mixed) {
+        for (int k = 0; k < c->cfg.nspecies; ++k) launch23m_push(c->g, c->sp[k], c->E32, s);
+        launch23m_rho(c->g, c->sp, c->cfg.nspecies, c->rho, s);
+        launch2d_solve(c->b, 2, c->rho, true, s);
+        launch23_update32(c->g, c->d, c->b, c->E, c->E32, acc, s);
+    } else {
+        for (int k = 0; k < c->cfg.nspecies; ++k) launch23_push(c->g, c->sp[k], c->E, c->b.racc, s);
+        launch2d_solve(c->b, 3, nullptr, true, s);
+        launch23_update(c->g, c->d, c->b, c->E, acc, s);
+    }
     if (diag) launch23_diag(c->g, c->d, c->E, c->sp, c->cfg.nspecies, s);
     launch23_tick(c->d.t, s);
 }

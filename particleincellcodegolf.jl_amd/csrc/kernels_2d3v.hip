@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(256) k23_limbs_to_rho(int64_t G, int q, u64* _
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k23_update(Grid23 g, Diag23 d, const double* __restrict__ Ex, const double* __restrict__ Ey,
-           const double* __restrict__ phi, double2* __restrict__ E, int accumulate) {
+           const double* __restrict__ phi, double2* __restrict__ E, float2* __restrict__ E32, int accumulate) {
     const int64_t G = (int64_t)g.NX * g.NY;
     const int64_t b = *d.t / d.ntskip;
     const bool bin = b < d.cap;
@@ -258,12 +258,10 @@ k23_update(Grid23 g, Diag23 d, const double* __restrict__ Ex, const double* __re
     const int lg = __ffs(d.ngskip) - 1;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < G; k += (int64_t)gridDim.x * blockDim.x) {
         const double ex = Ex[k], ey = Ey[k];
-        if (accumulate) {
-            const double2 e = E[k];
-            E[k] = make_double2(e.x + ex, e.y + ey);
-        } else {
-            E[k] = make_double2(0.0 + ex, 0.0 + ey);
-        }
+        const double2 e = accumulate ? E[k] : make_double2(0.0, 0.0);
+        const double2 en = make_double2(e.x + ex, e.y + ey);
+        E[k] = en;
+        if (E32) E32[k] = make_float2((float)en.x, (float)en.y);  // the mixed gathers' copy
         const int32_t i = (int32_t)(k & (g.NX - 1)), j = (int32_t)(k >> g.lx);
         if (bin && ((i | j) & gm) == 0) {
             const int64_t o = b * (int64_t)d.NXd * d.NYd + (i >> lg) + (int64_t)d.NXd * (j >> lg);
@@ -318,7 +316,14 @@ k23_diag_kinetic(Species23 s, double* __restrict__ kpart) {
         const uint32_t c = q - 3u * dp;
         const int64_t p = p0 + dp;
         const int64_t h = s.slot ? (int64_t)s.slot[p] : p;  // caller index p's slot
-        const double v = c == 0 ? s.vx[h] : (c == 1 ? s.vy[h] : s.vz[h]);
+        double v;
+        if (s.mixed) {
+            const float* hi = c == 0 ? s.vxh : (c == 1 ? s.vyh : s.vzh);
+            const float* lo = c == 0 ? s.vxl : (c == 1 ? s.vyl : s.vzl);
+            v = (double)hi[h] + (double)lo[h];
+        } else {
+            v = c == 0 ? s.vx[h] : (c == 1 ? s.vy[h] : s.vz[h]);
+        }
         return v * v;
     });
     if (threadIdx.x == 0) kpart[blockIdx.x] = r;
@@ -334,7 +339,16 @@ k23_diag_momentum(Species23 s, double* __restrict__ mpart) {
     if (threadIdx.x < kTreeLanes)
         for (int64_t i = threadIdx.x; i < n; i += kTreeLanes) {
             const int64_t hs = s.slot ? (int64_t)s.slot[b0 + i] : b0 + i;  // caller index b0 + i's slot
-            const double vx = s.vx[hs], vy = s.vy[hs], vz = s.vz[hs];
+            double vx, vy, vz;
+            if (s.mixed) {
+                vx = (double)s.vxh[hs] + (double)s.vxl[hs];
+                vy = (double)s.vyh[hs] + (double)s.vyl[hs];
+                vz = (double)s.vzh[hs] + (double)s.vzl[hs];
+            } else {
+                vx = s.vx[hs];
+                vy = s.vy[hs];
+                vz = s.vz[hs];
+            }
             a = a + vx;
             b = b + vy;
             c = c + vz;
@@ -773,8 +787,13 @@ void launch23_boris_op(const Grid23& g, double q_m, double* vx, double* vy, doub
 
 void launch23_update(const Grid23& g, const Diag23& d, const Buffers2D& b, double2* E, int accumulate,
                      hipStream_t st) {
+    launch23_update32(g, d, b, E, nullptr, accumulate, st);
+}
+
+void launch23_update32(const Grid23& g, const Diag23& d, const Buffers2D& b, double2* E, float2* E32,
+                       int accumulate, hipStream_t st) {
     const int64_t G = (int64_t)g.NX * g.NY;
-    k23_update<<<blocks_for(G, 256), 256, 0, st>>>(g, d, b.Ex, b.Ey, b.phi, E, accumulate);
+    k23_update<<<blocks_for(G, 256), 256, 0, st>>>(g, d, b.Ex, b.Ey, b.phi, E, E32, accumulate);
 }
 
 void launch23_tick(int64_t* t, hipStream_t st) { k23_tick<<<1, 64, 0, st>>>(t); }
@@ -822,9 +841,11 @@ void launch23_regroup(const Grid23& g, const Tiles23& T, const Species23& s, con
     const int64_t per = 16384;
     const unsigned nb = (unsigned)((s.P + per - 1) / per);
     k23_regroup_count<<<nb, 256, lds, st>>>(g, T, s, per);
-    k23_regroup_scan<<<1, 1024, 0, st>>>(T);
+    launch23_regroup_scan(T, st);
     k23_regroup_scatter<<<nb, 256, lds, st>>>(g, T, s, o, pid, pido, per);
 }
+
+void launch23_regroup_scan(const Tiles23& T, hipStream_t st) { k23_regroup_scan<<<1, 1024, 0, st>>>(T); }
 
 void launch23_to_caller(const Species23& s, const Species23& o, hipStream_t st) {
     if (s.P > 0) k23_to_caller<<<blocks_for(s.P, 256), 256, 0, st>>>(s, o);

@@ -1,6 +1,6 @@
 // pic2d3v_internal.h -- descriptors and launchers of the electrostatic path
 // of the structured 2D3V module, shared by the host runtime (capi2d3v.cpp)
-// and the kernels (kernels_2d3v.hip).
+// and the kernels (kernels_2d3v.hip, kernels_2d3v_mixed.hip).
 #pragma once
 #include "pic2d_internal.h"
 
@@ -21,7 +21,16 @@ struct Grid23 {
     double B[3];
     int32_t q;            // fraction bits of the signed exact charge sums
     int32_t boris;        // 0: ElectrostaticBoris, 1: ElectromagneticBoris form
+    // mixed precision: x = xq*Lx/2^32 (xq = 0 meaning x = Lx), so the cell of
+    // xq is (xq - 1) >> sx; a move of v*dt is v*step quanta of 2^-32 of the
+    // box, step = dt*2^32/L held as a float hi + lo pair
+    int32_t sx, sy;
+    float stepx[2], stepy[2];
 };
+
+// mixed precision: shape weights in quanta of 2^-kQ23 of a cell per axis
+// (products of two: 2^-44 of a particle's charge)
+constexpr int kQ23 = 22;
 
 // One species on the device (SoA).
 struct Species23 {
@@ -34,6 +43,15 @@ struct Species23 {
     // the slot holding caller index i (the particles are regrouped by tile
     // every few steps; diagnostics and downloads run in the caller's order)
     uint32_t* slot;
+    // mixed precision: the state (positions in fixed point, velocities as
+    // FP32 hi + lo), the Boris constants (v' = v + 2e + ((v+e) + (v+e) x t) x t * bs
+    // with e = E*eh), and the species' charge quanta (lo words, then hi words)
+    int32_t mixed;
+    uint32_t *xq, *yq;
+    float *vxh, *vyh, *vzh, *vxl, *vyl, *vzl;
+    float bt[3], bs, eh;
+    unsigned long long* cnt;
+    double qscale;        // qw * 2^(-2 kQ23): the charge of one quantum
 };
 
 // Regrouping of held particles by kRegroupTile23 x kRegroupTile23-cell tile.
@@ -76,6 +94,8 @@ void launch23_pack(const Species23& s, double* xyv, hipStream_t st);
 void launch23_regroup(const Grid23& g, const Tiles23& T, const Species23& s, const Species23& o, const uint32_t* pid,
                       uint32_t* pido, hipStream_t st);
 void launch23_to_caller(const Species23& s, const Species23& o, hipStream_t st);
+// the exclusive scan of the tile counts into start and the cursors
+void launch23_regroup_scan(const Tiles23& T, hipStream_t st);
 void launch23_identity(uint32_t* pid, uint32_t* slot, int64_t P, hipStream_t st);
 // operators
 void launch23_deposit_op(const Grid23& g, int shape, const double* x, const double* y, int64_t n, double w,
@@ -87,6 +107,24 @@ void launch23_boris_op(const Grid23& g, double q_m, double* vx, double* vy, doub
                        const double* ey, int64_t n, hipStream_t st);
 void launch23_pair_field(const double* Ex, const double* Ey, double2* E, int64_t G, hipStream_t st);
 void launch23_split_field(const double2* E, double* Ex, double* Ey, int64_t G, hipStream_t st);
+
+// mixed precision (kernels_2d3v_mixed.hip): the particle loop (gather from
+// the FP32 field copy E32, Boris, fixed-point move, exact quanta deposit into
+// s.cnt); the species' quanta into rho (read and cleared); xyv <-> the mixed
+// state; the mixed state by tile, back to the caller's order, permuted; FP64
+// copies of the held state (for the sort's keys)
+void launch23m_push(const Grid23& g, const Species23& s, const float2* E32, hipStream_t st);
+void launch23m_rho(const Grid23& g, const Species23* sp, int ns, double* rho, hipStream_t st);
+void launch23m_unpack(const Grid23& g, const double* xyv, const Species23& s, hipStream_t st);
+void launch23m_pack(const Grid23& g, const Species23& s, double* xyv, hipStream_t st);
+void launch23m_regroup(const Grid23& g, const Tiles23& T, const Species23& s, const Species23& o,
+                       const uint32_t* pid, uint32_t* pido, hipStream_t st);
+void launch23m_to_caller(const Species23& s, const Species23& o, hipStream_t st);
+void launch23m_permute(const Species23& s, const Species23& o, const uint32_t* perm, hipStream_t st);
+void launch23m_to64(const Grid23& g, const Species23& s, hipStream_t st);
+// update! writing the FP32 copy of Exy as well
+void launch23_update32(const Grid23& g, const Diag23& d, const Buffers2D& b, double2* E, float2* E32,
+                       int accumulate, hipStream_t st);
 
 // sort!(species, dx, dy)
 struct Sort23 {

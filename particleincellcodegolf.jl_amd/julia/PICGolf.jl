@@ -769,12 +769,13 @@ struct Pic2D3VConfig
     theta_source::Float64
     rtol::Float64
     maxiters::Int32
+    precision::Int32
 end
 
 """
     Pic2D3VConfig(species; NX, NY=NX, Lx=1, Ly=1, dt, B0x=0, B0y=0, B0z=0, buffer=3, ntskip=1, ngskip=1,
                   field_update=:accumulate, boris=:reference, device=0, field=:electrostatic, theta=1,
-                  theta_source=0, rtol=sqrt(eps()), maxiters=10)
+                  theta_source=0, rtol=sqrt(eps()), maxiters=10, precision=:fp64)
 
 The `ElectrostaticField(NX, NY, Lx, Ly; dt, B0x, B0y, B0z, buffer)` and
 `ElectrostaticDiagnostics(..., ntskip, ngskip)` of a plasma of up to four
@@ -787,11 +788,14 @@ B0x, B0y, B0z, imex=ImEx(theta), buffer)` and `LorenzGaugeDiagnostics` instead
 (`theta=0` is `Explicit()`, `1` is `Implicit()`; src/2D3V.jl uses `ImEx(1)`);
 `field=:lorenz` builds a `LorenzGaugeField`, which runs with `loop!` only;
 `field=:lorenz_semi_implicit` a `LorenzGaugeSemiImplicitField(...; fieldimex=ImEx(theta),
-sourceimex=ImEx(theta_source), buffer, rtol, maxiters)`.
+sourceimex=ImEx(theta_source), buffer, rtol, maxiters)`.  `precision=:mixed`
+(electrostatic field only) runs the particles in mixed precision: fixed-point
+positions, FP32 hi + lo velocities, FP32 gathers and exact integer deposits.
 """
 function Pic2D3VConfig(species::Vector{Pic2D3VSpecies}; NX, NY=NX, Lx=1.0, Ly=1.0, dt, B0x=0.0, B0y=0.0, B0z=0.0,
                        buffer=3, ntskip=1, ngskip=1, field_update=:accumulate, boris=:reference, device=0,
-                       field=:electrostatic, theta=1.0, theta_source=0.0, rtol=sqrt(eps()), maxiters=10)
+                       field=:electrostatic, theta=1.0, theta_source=0.0, rtol=sqrt(eps()), maxiters=10,
+                       precision=:fp64)
     ns = length(species)
     1 <= ns <= MAX_SPECIES_2D3V || error("Pic2D3VConfig: 1 to $MAX_SPECIES_2D3V species")
     none = Pic2D3VSpecies(0, 0, 0.0, 0.0, 0.0)
@@ -799,7 +803,8 @@ function Pic2D3VConfig(species::Vector{Pic2D3VSpecies}; NX, NY=NX, Lx=1.0, Ly=1.
     Pic2D3VConfig(Int32(NX), Int32(NY), Float64(Lx), Float64(Ly), Float64(dt),
                   (Float64(B0x), Float64(B0y), Float64(B0z)), Int32(buffer), Int32(ntskip), Int32(ngskip),
                   FIELD_UPDATES[field_update], BORIS_VARIANTS[boris], Int32(ns), sp, Int32(device),
-                  FIELDS2D3V[field], Float64(theta), Float64(theta_source), Float64(rtol), Int32(maxiters))
+                  FIELDS2D3V[field], Float64(theta), Float64(theta_source), Float64(rtol), Int32(maxiters),
+                  PRECISIONS[precision])
 end
 
 mutable struct Context2D3V

@@ -119,7 +119,7 @@ class pic2d3v_config(ctypes.Structure):
         ("boris", ctypes.c_int32), ("nspecies", ctypes.c_int32),
         ("species", pic2d3v_species * MAX_SPECIES_2D3V), ("device", ctypes.c_int32), ("field", ctypes.c_int32),
         ("theta", ctypes.c_double), ("theta_source", ctypes.c_double), ("rtol", ctypes.c_double),
-        ("maxiters", ctypes.c_int32),
+        ("maxiters", ctypes.c_int32), ("precision", ctypes.c_int32),
     ]
 
 

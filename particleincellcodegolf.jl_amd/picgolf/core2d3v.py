@@ -70,7 +70,10 @@ class Config2D3V:
     "lorenz" a LorenzGaugeField, run with loop() only; "lorenz_semi_implicit"
     a LorenzGaugeSemiImplicitField with fieldimex theta, sourceimex
     theta_source and the convergence test's rtol and maxiters (the
-    reference's defaults sqrt(eps()) and 10)."""
+    reference's defaults sqrt(eps()) and 10).  precision "mixed"
+    (electrostatic field only) runs the particles with fixed-point positions,
+    FP32 hi + lo velocities, FP32 gathers and exact integer deposits; the
+    field, the solve and the diagnostics stay FP64."""
     NX: int
     NY: int
     dt: float
@@ -89,6 +92,7 @@ class Config2D3V:
     theta_source: float = 0.0
     rtol: float = float(np.sqrt(np.finfo(float).eps))
     maxiters: int = 10
+    precision: str = "fp64"
 
     def to_c(self):
         c = L.pic2d3v_config()
@@ -111,6 +115,8 @@ class Config2D3V:
         c.theta_source = float(self.theta_source)
         c.rtol = float(self.rtol)
         c.maxiters = int(self.maxiters)
+        c.precision = {"fp64": L.FP64, "mixed": L.MIXED}[self.precision] if isinstance(self.precision, str) \
+            else int(self.precision)
         return c
 
 

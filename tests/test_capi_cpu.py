@@ -231,6 +231,8 @@ def test_2d3v_struct_layout_matches_the_header(tmp_path):
     # the buffered deposit grid must hold every stencil cell (the reference's BoundsError)
     (dict(field="lorenz_semi_implicit", buffer=1, species=[dict(shape=4)]), "buffer >= 2"),
     (dict(field="lorenz_semi_implicit", buffer=2, species=[dict(shape=1), dict(shape=7)]), "buffer >= 3"),
+    (dict(precision=7), "precision"),
+    (dict(precision="mixed", field="lorenz_staggered"), "mixed precision"),
 ])
 def test_create2d3v_rejects_bad_config_with_message(change, msg):
     import picgolf
