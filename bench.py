@@ -282,12 +282,17 @@ def bench_2d(steps=500):
     return out
 
 
-def bench_pic2d3v(steps=20, warmup=2, N=1024, ppc=16, shape="bspline2"):
+def bench_pic2d3v(steps=20, warmup=2, N=1024, ppc=16, shape="bspline2", precision="both", cpu=True):
     """The structured 2D3V module's electrostatic loop at the grid and
     particle count of its driver script (src/2D3V.jl: 1024 x 1024 cells, 16
     particles per cell per species, electrons and ions of mass 32, quadratic
-    B-splines, both species sorted per cell once), FP64, the field gathered
-    as the reference's update! leaves it."""
+    B-splines, both species sorted per cell once), the field gathered as the
+    reference's update! leaves it; FP64, then mixed from the same state, and
+    an FP64 run from positions displaced by up to four 2^-32 quanta of the
+    box as the control of what the mixed position resolution can reach (the
+    reference's shapes move a particle's charge by two cells when it crosses
+    a cell edge).  precision "mixed" or "fp64" times that run alone (for
+    profiles); cpu=False leaves out the CPU restatement's sample."""
     import numpy as np
     import picgolf
     from picgolf.core2d3v import Config2D3V, SpeciesConfig
@@ -298,17 +303,48 @@ def bench_pic2d3v(steps=20, warmup=2, N=1024, ppc=16, shape="bspline2"):
     for q, m, vth in ((-1.0, 1.0, 0.05), (1.0, 32.0, 0.05 / np.sqrt(32))):
         sps.append(SpeciesConfig(P=P, shape=shape, charge=q, mass=m, weight=n0 / P))
         xyvs.append(np.vstack([rng.random(P), rng.random(P), vth * rng.standard_normal((3, P))]))
-    cfg = Config2D3V(NX=N, NY=N, dt=1.0 / (8 * N), species=sps, B0=(np.sqrt(n0) / 4, 0.0, 0.0), ntskip=4,
-                     ngskip=4)
-    with picgolf.Context2D3V(cfg) as c:
-        for k, xyv in enumerate(xyvs):
-            c.upload(k, xyv)
-            c.sort(k, 1.0 / N, 1.0 / N)
-        del xyvs
-        c.run(warmup)
-        t0 = time.perf_counter()
-        rows = c.run(steps)
-        secs = time.perf_counter() - t0
+    def one(prec, states):
+        cfg = Config2D3V(NX=N, NY=N, dt=1.0 / (8 * N), species=sps, B0=(np.sqrt(n0) / 4, 0.0, 0.0), ntskip=4,
+                         ngskip=4, precision=prec)
+        with picgolf.Context2D3V(cfg) as c:
+            for k, xyv in enumerate(states):
+                c.upload(k, xyv)
+                c.sort(k, 1.0 / N, 1.0 / N)
+            c.run(warmup)
+            t0 = time.perf_counter()
+            rows = c.run(steps)
+            secs = time.perf_counter() - t0
+        return cfg, rows, secs
+
+    if precision != "both":
+        cfg, rows, secs = one(precision, xyvs)
+        return {"workload": f"structured 2D3V electrostatic loop: {N}x{N} cells, 2 species x {P} particles, "
+                            f"{shape}, {precision}", "pushes_per_s": 2 * P * steps / secs,
+                "ms_per_step": secs / steps * 1e3, "steps": steps, "warmup": warmup}
+    cfg, rows, secs = one("fp64", xyvs)
+    _, rows_m, secs_m = one("mixed", xyvs)
+    for xyv in xyvs:
+        for k in (0, 1):
+            z = xyv[k] + rng.uniform(-4, 4, P) * 2.0 ** -32
+            xyv[k] = np.where(z > 1.0, z - 1.0, np.where(z > 0.0, z, z + 1.0))
+    _, rows_c, _ = one("fp64", xyvs)
+    del xyvs
+
+    def rel(a, col):
+        return float(np.max(np.abs(a[:, col] - rows[:, col]) / np.abs(rows[:, col])))
+    out = {"workload": f"structured 2D3V electrostatic loop: {N}x{N} cells, 2 species x {P} particles "
+                       f"({ppc} per cell each), {shape}, ions of mass 32, B0 along x, dt=1/(8N), "
+                       f"diagnostics every 4 steps, FP64 (mixed under 'mixed'), particles sorted per cell once "
+                       f"before the timed steps",
+           "pushes_per_s": 2 * P * steps / secs, "ms_per_step": secs / steps * 1e3, "steps": steps,
+           "warmup": warmup, "kinetic_energy_first_last": [float(rows[0, 0]), float(rows[-1, 0])],
+           "mixed": {"pushes_per_s": 2 * P * steps / secs_m, "ms_per_step": secs_m / steps * 1e3,
+                     "mixed_over_fp64": secs / secs_m,
+                     "field_energy_rel_err_vs_fp64": rel(rows_m, 1), "kinetic_rel_err_vs_fp64": rel(rows_m, 0),
+                     "fp64_displaced_control_field_energy_rel_err": rel(rows_c, 1),
+                     "fp64_displaced_control_kinetic_rel_err": rel(rows_c, 0)}}
+    if not cpu:
+        return out
     # the CPU restatement on a sample: the same grid, 1/32 of each species (sorted), two steps
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -322,15 +358,11 @@ def bench_pic2d3v(steps=20, warmup=2, N=1024, ppc=16, shape="bspline2"):
     t0 = time.perf_counter()
     oracle.run23(N, N, 1.0, 1.0, cfg.dt, cfg.B0, sample, 2, ntskip=4, ngskip=4, nthreads=threads)
     cpu_secs = time.perf_counter() - t0
-    return {"workload": f"structured 2D3V electrostatic loop: {N}x{N} cells, 2 species x {P} particles "
-                        f"({ppc} per cell each), {shape}, ions of mass 32, B0 along x, dt=1/(8N), "
-                        f"diagnostics every 4 steps, FP64, particles sorted per cell once before the timed steps",
-            "pushes_per_s": 2 * P * steps / secs, "ms_per_step": secs / steps * 1e3, "steps": steps,
-            "warmup": warmup, "kinetic_energy_first_last": [float(rows[0, 0]), float(rows[-1, 0])],
-            "cpu_baseline": {"value": 2 * n * 2 / cpu_secs, "unit": "particle-pushes/s", "cores": threads,
-                             "kind": "port", "sample": f"FP64 CPU restatement, the same {N}x{N} grid and solve, "
-                             f"2 species x {n} particles (1/32), 2 steps; push on {threads} threads, deposit "
-                             f"and solve on one"}}
+    out["cpu_baseline"] = {"value": 2 * n * 2 / cpu_secs, "unit": "particle-pushes/s", "cores": threads,
+                           "kind": "port", "sample": f"FP64 CPU restatement, the same {N}x{N} grid and solve, "
+                           f"2 species x {n} particles (1/32), 2 steps; push on {threads} threads, deposit "
+                           f"and solve on one"}
+    return out
 
 
 def driver_2d3v_parameters(N=1024):
@@ -638,7 +670,8 @@ def main():
         print(json.dumps(bench_2d(steps=a.steps)))
         return
     if a.workload == "pic2d3v":
-        print(json.dumps(bench_pic2d3v(steps=a.steps, warmup=a.warmup)))
+        print(json.dumps(bench_pic2d3v(steps=a.steps, warmup=a.warmup, precision=a.precision,
+                                       cpu=not a.no_cpu_baseline)))
         return
     if a.workload in ("lorenz2d3v", "lorenz2d3v_si"):
         f = "lorenz_semi_implicit" if a.workload == "lorenz2d3v_si" else "lorenz_staggered"

@@ -36,6 +36,7 @@ struct pic2d3v_ctx {
     bool held_identity[kMaxSpecies23] = {};
     Tiles23 tiles{};
     int64_t since_regroup = 0;
+    bool regroup_due = true;  // after an upload: regroup before the next step
     Diag23 d{};
     Buffers2D b{};           // the 2D solve's descriptor and buffers
     double2* E = nullptr;    // the gathered field Exy (interior, (Ex, Ey) per cell)
@@ -272,6 +273,7 @@ static int zero_state(pic2d3v_ctx* c) {
     HIPCHK(c, hipMemsetAsync(c->E, 0, sizeof(double2) * (size_t)G, c->stream));
     HIPCHK(c, hipMemsetAsync(c->b.racc, 0, 3 * sizeof(unsigned long long) * (size_t)G, c->stream));
     HIPCHK(c, hipMemsetAsync(c->b.flag, 0, sizeof(int32_t), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->g.fallback, 0, sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d.t, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
     if (c->mixed) {
         HIPCHK(c, hipMemsetAsync(c->E32, 0, sizeof(float2) * (size_t)G, c->stream));
@@ -472,6 +474,7 @@ extern "C" int pic2d3v_create(pic2d3v_ctx** out, const pic2d3v_config* cfg) {
         (rc = dalloc(c, &c->E, G)))
         return fail(rc);
     if (c->mixed && ((rc = dalloc(c, &c->E32, G)) || (rc = dalloc(c, &c->rho, G)))) return fail(rc);
+    if ((rc = dalloc(c, &g.fallback, 1))) return fail(rc);
     b.E32 = nullptr;
     const std::vector<double> twp = pass_twiddle_table();
     if ((rc = dalloc(c, &c->tw, twp.size()))) return fail(rc);
@@ -613,7 +616,7 @@ extern "C" int pic2d3v_upload(pic2d3v_ctx* c, int32_t s, const double* xyv) {
     if ((rc = finish(c))) return rc;
     c->uploaded[s] = true;
     c->held_identity[s] = true;
-    c->since_regroup = kRegroupEvery23;  // regroup before the next step
+    c->regroup_due = true;
     return PIC_OK;
 }
 
@@ -694,7 +697,32 @@ static int regroup(pic2d3v_ctx* c) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     free_graphs(c);
     c->since_regroup = 0;
+    c->regroup_due = false;
+    HIPCHK(c, hipMemsetAsync(c->g.fallback, 0, sizeof(unsigned long long), c->stream));
     return PIC_OK;
+}
+
+// Before a step: regroup when an upload asked for it; otherwise every
+// kRegroupEvery23 steps, on the Lorenz-gauge paths always, on the
+// electrostatic path only when more than kRegroupFallback of the stencils
+// pushed since the last check fell back to global memory (the pushes count
+// them).
+static int maybe_regroup(pic2d3v_ctx* c) {
+    if (c->regroup_due) return regroup(c);
+    if (c->since_regroup < kRegroupEvery23) return PIC_OK;
+    if (!c->lorenz) {
+        unsigned long long fb = 0;
+        HIPCHK(c, hipMemcpyAsync(&fb, c->g.fallback, sizeof fb, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        int64_t P = 0;
+        for (int s = 0; s < c->cfg.nspecies; ++s) P += c->sp[s].P;
+        if ((double)fb <= kRegroupFallback * 2.0 * (double)P * (double)c->since_regroup) {
+            HIPCHK(c, hipMemsetAsync(c->g.fallback, 0, sizeof fb, c->stream));
+            c->since_regroup = 0;
+            return PIC_OK;
+        }
+    }
+    return regroup(c);
 }
 
 // The held particles of species s back in the caller's order.
@@ -1011,14 +1039,14 @@ extern "C" int pic2d3v_run(pic2d3v_ctx* c, int64_t nsteps, double* rows) {
     if (c->semi) {
         c->iters.clear();
         for (int64_t t = t0; t < t1; ++t) {
-            if (c->since_regroup >= kRegroupEvery23 && (rc = regroup(c))) return rc;
+            if ((rc = maybe_regroup(c))) return rc;
             if ((rc = step_si(c, t % nt == 0 ? 2 : 1))) return rc;
             c->step = t + 1;
             ++c->since_regroup;
         }
     } else {
         for (int64_t t = t0; t < t1; ++t) {
-            if (c->since_regroup >= kRegroupEvery23 && (rc = regroup(c))) return rc;
+            if ((rc = maybe_regroup(c))) return rc;
             ++c->since_regroup;
             const bool diag = t % nt == 0;
             hipGraphExec_t& ge = c->lorenz ? c->g_lg[diag ? 2 : 1] : (diag ? c->g_diag : c->g_plain);
@@ -1161,7 +1189,7 @@ extern "C" int pic2d3v_loop(pic2d3v_ctx* c, int64_t nsteps) {
     if (c->semi) {
         c->iters.clear();
         for (int64_t t = 0; t < nsteps; ++t) {
-            if (c->since_regroup >= kRegroupEvery23 && (rc = regroup(c))) return rc;
+            if ((rc = maybe_regroup(c))) return rc;
             if ((rc = step_si(c, 0))) return rc;
             c->step += 1;
             ++c->since_regroup;
@@ -1169,7 +1197,7 @@ extern "C" int pic2d3v_loop(pic2d3v_ctx* c, int64_t nsteps) {
         return finish(c);
     }
     for (int64_t t = 0; t < nsteps; ++t) {
-        if (c->since_regroup >= kRegroupEvery23 && (rc = regroup(c))) return rc;
+        if ((rc = maybe_regroup(c))) return rc;
         ++c->since_regroup;
         if (!c->g_lg[0] && (rc = capture(c, 0, &c->g_lg[0]))) return rc;
         HIPCHK(c, hipGraphLaunch(c->g_lg[0], c->stream));

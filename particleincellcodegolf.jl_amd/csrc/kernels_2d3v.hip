@@ -125,6 +125,7 @@ k23_push(Grid23 g, Species23 s, const double2* __restrict__ E, u64* __restrict__
         }
     }
     __syncthreads();
+    int fb = 0;  // particles that read or added through global memory
 #pragma unroll 1
     for (int k = 0; k < kTile23PPT; ++k) {
         const int64_t p = c0 + t + (int64_t)k * kTile23Threads;
@@ -153,6 +154,7 @@ k23_push(Grid23 g, Species23 s, const double2* __restrict__ E, u64* __restrict__
                 a = fma(e.x, wxy, a);
                 b = fma(e.y, wxy, b);
             }
+        fb += in ? 0 : 1;
         const double vx0 = vx, vy0 = vy;
         boris23<BV>(g, s.q_m, a, b, vx, vy, vz);
         x = unimod23(x + (vx0 + vx) / 2.0 * g.dt, g.Lx);
@@ -172,6 +174,7 @@ k23_push(Grid23 g, Species23 s, const double2* __restrict__ E, u64* __restrict__
             ly[q] = (cy[q] - oy) & (NY - 1);
             in = in && lx[q] < wx && ly[q] < wy;
         }
+        fb += in ? 0 : 1;
 #pragma unroll
         for (int jj = 0; jj < n; ++jj)
 #pragma unroll
@@ -181,6 +184,7 @@ k23_push(Grid23 g, Species23 s, const double2* __restrict__ E, u64* __restrict__
                 else add_limbs_s(racc, G, cx[ii] + (int64_t)NX * cy[jj], v);
             }
     }
+    count_fallback(g.fallback, fb);
     __syncthreads();
     if (tile) {
         for (int e = t; e < wx * wy; e += kTile23Threads) {
@@ -316,14 +320,7 @@ k23_diag_kinetic(Species23 s, double* __restrict__ kpart) {
         const uint32_t c = q - 3u * dp;
         const int64_t p = p0 + dp;
         const int64_t h = s.slot ? (int64_t)s.slot[p] : p;  // caller index p's slot
-        double v;
-        if (s.mixed) {
-            const float* hi = c == 0 ? s.vxh : (c == 1 ? s.vyh : s.vzh);
-            const float* lo = c == 0 ? s.vxl : (c == 1 ? s.vyl : s.vzl);
-            v = (double)hi[h] + (double)lo[h];
-        } else {
-            v = c == 0 ? s.vx[h] : (c == 1 ? s.vy[h] : s.vz[h]);
-        }
+        const double v = c == 0 ? s.vx[h] : (c == 1 ? s.vy[h] : s.vz[h]);
         return v * v;
     });
     if (threadIdx.x == 0) kpart[blockIdx.x] = r;
@@ -339,16 +336,7 @@ k23_diag_momentum(Species23 s, double* __restrict__ mpart) {
     if (threadIdx.x < kTreeLanes)
         for (int64_t i = threadIdx.x; i < n; i += kTreeLanes) {
             const int64_t hs = s.slot ? (int64_t)s.slot[b0 + i] : b0 + i;  // caller index b0 + i's slot
-            double vx, vy, vz;
-            if (s.mixed) {
-                vx = (double)s.vxh[hs] + (double)s.vxl[hs];
-                vy = (double)s.vyh[hs] + (double)s.vyl[hs];
-                vz = (double)s.vzh[hs] + (double)s.vzl[hs];
-            } else {
-                vx = s.vx[hs];
-                vy = s.vy[hs];
-                vz = s.vz[hs];
-            }
+            const double vx = s.vx[hs], vy = s.vy[hs], vz = s.vz[hs];
             a = a + vx;
             b = b + vy;
             c = c + vz;
@@ -816,6 +804,10 @@ void launch23_diag(const Grid23& g, const Diag23& d, const double2* E, const Spe
         f.s[s].nm = (sp[s].P + kTreeChunk - 1) / kTreeChunk;
         f.s[s].mass = sp[s].mass;
         f.s[s].weight = sp[s].weight;
+        if (sp[s].mixed) {
+            f.s[s].nk = f.s[s].nm = launch23m_diag_species(sp[s], d.kpart[s], d.mpart[s], st);
+            continue;
+        }
         k23_diag_kinetic<<<(unsigned)f.s[s].nk, 256, 0, st>>>(sp[s], d.kpart[s]);
         k23_diag_momentum<<<(unsigned)f.s[s].nm, 256, 0, st>>>(sp[s], d.mpart[s]);
     }

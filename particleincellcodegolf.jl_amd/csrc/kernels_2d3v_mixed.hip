@@ -66,6 +66,12 @@ __device__ __forceinline__ void quantize23(const float* w, uint32_t* q) {
     q[N - 1] = (uint32_t)(one - prev);
 }
 
+// the product of two numbers below 2^24 (the masks let the compiler use the
+// full-rate 24-bit multiplies)
+__device__ __forceinline__ u64 quanta_product(uint32_t a, uint32_t b) {
+    return (u64)(a & 0xFFFFFFu) * (b & 0xFFFFFFu);
+}
+
 // depositindicesfractions + the shape (:1106-1187) on zq: zN = zq/2^sh, the
 // base cell c = ceil(zN) - 1 = (zq - 1) >> sh and r = ceil(zN) - zN =
 // ((c + 1) << sh) - zq in units of 2^-sh (both exact in 32-bit wrapping
@@ -84,6 +90,24 @@ __device__ __forceinline__ int32_t shape_q23(uint32_t zq, int sh, uint32_t* q) {
         q[0] = (1u << kQ23) - rq;
         q[1] = rq;
         return (int32_t)c;
+    } else if constexpr (SH == 4) {
+        // the quadratic B-spline in integers: x = (up + 1/2) - r on 24 bits
+        // (u), weights x^2/2, 1 - x^2/2 - (1-x)^2/2, (1-x)^2/2 rounded to
+        // quanta from the exact 48-bit squares (1 - x = (v + 1)/2^24 with
+        // v = 2^24 - 1 - u, so that v fits the 24-bit multiplier)
+        const uint32_t up = r > (1u << (sh - 1)) ? 1u : 0u;
+        const uint32_t xi = ((1u + 2u * up) << (sh - 1)) - r;  // x in units of 2^-sh, [0, 2^sh)
+        const uint32_t u = sh >= 24 ? xi >> (sh - 24) : xi << (24 - sh);
+        const uint32_t v = 0xFFFFFFu - u;
+        const u64 uu = quanta_product(u, u);
+        const u64 vv = quanta_product(v, v) + 2u * (u64)v + 1u;
+        constexpr u64 half = 1ull << (48 + 1 - kQ23 - 1);
+        const uint32_t q0 = (uint32_t)((uu + half) >> (48 + 1 - kQ23));
+        const uint32_t q2 = (uint32_t)((vv + half) >> (48 + 1 - kQ23));
+        q[0] = q0;
+        q[1] = (1u << kQ23) - q0 - q2;
+        q[2] = q2;
+        return (int32_t)c + (int32_t)up - 1;
     } else {
         constexpr int n = SH - 2;
         const float sc = __int_as_float((127 - sh) << 23);  // 2^-sh
@@ -140,8 +164,10 @@ __device__ __forceinline__ void add_q128(u64* __restrict__ cnt, int64_t G, int64
 // leaves the window (every particle, when the window would not fit) reads and
 // adds through global memory instead, with the same result.
 // ---------------------------------------------------------------------------
+// Occupancy: the quadratic B-spline's loop holds 84 registers by default (5
+// waves per SIMD); capped at 76 it runs 6 waves without spilling.
 template <int SH>
-__global__ void __launch_bounds__(kTile23Threads)
+__global__ void __launch_bounds__(kTile23Threads) __attribute__((amdgpu_waves_per_eu(SH == 4 ? 6 : 1)))
 k23m_push(Grid23 g, Species23 s, const float2* __restrict__ E32) {
     constexpr int n = ShapeCount<SH>::value;
     constexpr int LO = ShapeExtent<SH>::lo, HI = ShapeExtent<SH>::hi;
@@ -156,18 +182,14 @@ k23m_push(Grid23 g, Species23 s, const float2* __restrict__ E32) {
     u64* __restrict__ cnt = s.cnt;
     const int64_t c0 = (int64_t)blockIdx.x * kTile23Chunk;
     const int ax = (int)((s.xq[c0] - 1u) >> g.sx), ay = (int)((s.yq[c0] - 1u) >> g.sy);
-    uint32_t xs[kTile23PPT], ys[kTile23PPT];
     int mnx = INT_MAX, mxx = INT_MIN, mny = INT_MAX, mxy = INT_MIN;
 #pragma unroll
     for (int k = 0; k < kTile23PPT; ++k) {
         const int64_t p = c0 + t + (int64_t)k * kTile23Threads;
-        xs[k] = 0u;
-        ys[k] = 0u;
         if (p < s.P) {
-            xs[k] = s.xq[p];
-            ys[k] = s.yq[p];
-            const int dx = (((int)((xs[k] - 1u) >> g.sx) - ax + NX / 2) & (NX - 1)) - NX / 2;
-            const int dy = (((int)((ys[k] - 1u) >> g.sy) - ay + NY / 2) & (NY - 1)) - NY / 2;
+            const uint32_t xq = s.xq[p], yq = s.yq[p];
+            const int dx = (((int)((xq - 1u) >> g.sx) - ax + NX / 2) & (NX - 1)) - NX / 2;
+            const int dy = (((int)((yq - 1u) >> g.sy) - ay + NY / 2) & (NY - 1)) - NY / 2;
             mnx = min(mnx, dx);
             mxx = max(mxx, dx);
             mny = min(mny, dy);
@@ -221,36 +243,52 @@ k23m_push(Grid23 g, Species23 s, const float2* __restrict__ E32) {
     }
     __syncthreads();
     const float t1 = s.bt[0], t2 = s.bt[1], t3 = s.bt[2];
+    int fb = 0;  // stencils read or added through global memory
 #pragma unroll 1
     for (int k = 0; k < kTile23PPT; ++k) {
         const int64_t p = c0 + t + (int64_t)k * kTile23Threads;
         if (p >= s.P) break;
-        uint32_t xq = xs[k], yq = ys[k];
+        uint32_t xq = s.xq[p], yq = s.yq[p];  // read again (L2): fewer registers held over the staging
         float vx = s.vxh[p], vy = s.vyh[p], vz = s.vzh[p];
         float lx = s.vxl[p], ly = s.vyl[p], lz = s.vzl[p];
         uint32_t qx[n], qy[n];
-        int32_t cx[n], cy[n], rx[n], ry[n];
-        // gather at the old position
+        // gather at the old position.  The stencil's cells are consecutive
+        // (wrapped), so it lies in the window when its first cell's window
+        // offset leaves room for the rest; the window and the fallback read
+        // through separate loads (LDS, global), not one generic pointer.
         int32_t fx = shape_q23<SH>(xq, g.sx, qx), fy = shape_q23<SH>(yq, g.sy, qy);
-        bool in = tile;
+        int32_t rx = (fx - ox) & (NX - 1), ry = (fy - oy) & (NY - 1);
+        bool in = tile && rx + n <= wx && ry + n <= wy;
+        float wgx[n], wgy[n];
 #pragma unroll
         for (int q = 0; q < n; ++q) {
-            cx[q] = (fx + q) & (NX - 1);
-            cy[q] = (fy + q) & (NY - 1);
-            rx[q] = (cx[q] - ox) & (NX - 1);
-            ry[q] = (cy[q] - oy) & (NY - 1);
-            in = in && rx[q] < wx && ry[q] < wy;
+            wgx[q] = (float)qx[q] * kW;
+            wgy[q] = (float)qy[q] * kW;
         }
         float f1 = 0.0f, f2 = 0.0f;
+        fb += in ? 0 : 1;
+        if (in) {
+            const float2* ew = Ew + rx + __umul24(wx, ry);
 #pragma unroll
-        for (int jj = 0; jj < n; ++jj) {
-            const float wyj = (float)qy[jj] * kW;
+            for (int jj = 0; jj < n; ++jj)
 #pragma unroll
-            for (int ii = 0; ii < n; ++ii) {
-                const float w = ((float)qx[ii] * kW) * wyj;
-                const float2 e = in ? Ew[rx[ii] + wx * ry[jj]] : E32[cx[ii] + (int64_t)NX * cy[jj]];
-                f1 = fmaf(e.x, w, f1);
-                f2 = fmaf(e.y, w, f2);
+                for (int ii = 0; ii < n; ++ii) {
+                    const float w = wgx[ii] * wgy[jj];
+                    const float2 e = ew[ii + wx * jj];
+                    f1 = fmaf(e.x, w, f1);
+                    f2 = fmaf(e.y, w, f2);
+                }
+        } else {
+#pragma unroll
+            for (int jj = 0; jj < n; ++jj) {
+                const int64_t row = (int64_t)NX * ((fy + jj) & (NY - 1));
+#pragma unroll
+                for (int ii = 0; ii < n; ++ii) {
+                    const float w = wgx[ii] * wgy[jj];
+                    const float2 e = E32[row + ((fx + ii) & (NX - 1))];
+                    f1 = fmaf(e.x, w, f1);
+                    f2 = fmaf(e.y, w, f2);
+                }
             }
         }
         // Boris as a change: half kick e, rotation of m = v + e, half kick
@@ -277,25 +315,32 @@ k23m_push(Grid23 g, Species23 s, const float2* __restrict__ E32) {
         // deposit at the new position
         fx = shape_q23<SH>(xq, g.sx, qx);
         fy = shape_q23<SH>(yq, g.sy, qy);
-        in = tile;
+        rx = (fx - ox) & (NX - 1);
+        ry = (fy - oy) & (NY - 1);
+        in = tile && rx + n <= wx && ry + n <= wy;
+        fb += in ? 0 : 1;
+        if (in) {
+            u64* rw = Rw + rx + __umul24(wx, ry);
 #pragma unroll
-        for (int q = 0; q < n; ++q) {
-            cx[q] = (fx + q) & (NX - 1);
-            cy[q] = (fy + q) & (NY - 1);
-            rx[q] = (cx[q] - ox) & (NX - 1);
-            ry[q] = (cy[q] - oy) & (NY - 1);
-            in = in && rx[q] < wx && ry[q] < wy;
-        }
+            for (int jj = 0; jj < n; ++jj)
 #pragma unroll
-        for (int jj = 0; jj < n; ++jj)
+                for (int ii = 0; ii < n; ++ii) {
+                    const u64 v = quanta_product(qx[ii], qy[jj]);
+                    if (v) atomicAdd(rw + ii + wx * jj, v);
+                }
+        } else {
 #pragma unroll
-            for (int ii = 0; ii < n; ++ii) {
-                const u64 v = (u64)qx[ii] * qy[jj];
-                if (!v) continue;
-                if (in) atomicAdd(Rw + rx[ii] + wx * ry[jj], v);
-                else add_q128(cnt, G, cx[ii] + (int64_t)NX * cy[jj], v);
+            for (int jj = 0; jj < n; ++jj) {
+                const int64_t row = (int64_t)NX * ((fy + jj) & (NY - 1));
+#pragma unroll
+                for (int ii = 0; ii < n; ++ii) {
+                    const u64 v = quanta_product(qx[ii], qy[jj]);
+                    if (v) add_q128(cnt, G, row + ((fx + ii) & (NX - 1)), v);
+                }
             }
+        }
     }
+    count_fallback(g.fallback, fb);
     __syncthreads();
     if (tile) {
         for (int e = t; e < wx * wy; e += kTile23Threads) {
@@ -445,6 +490,48 @@ __global__ void __launch_bounds__(256) k23m_permute(Species23 s, Species23 o, co
 }
 
 // ---------------------------------------------------------------------------
+// diagnostics: the kinetic sum sum(|v|^2), the momentum sums (vx, vy, vz)
+// and the characteristic ones (|vx|, |vy|, |vz|) of a species in one pass
+// over the caller's order (each particle read once, through slot): a block
+// takes kTreeChunk particles, its lanes sum them strided and the fixed tree
+// folds the lanes, so the partials do not depend on the held order.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k23m_diag_species(Species23 s, double* __restrict__ kpart, double* __restrict__ mpart) {
+    __shared__ double red[3 * kTreeLanes + 8];
+    const int64_t b0 = (int64_t)blockIdx.x * kTreeChunk;
+    const int64_t n = s.P - b0 < kTreeChunk ? s.P - b0 : kTreeChunk;
+    double ke = 0.0, a = 0.0, b = 0.0, c = 0.0, e = 0.0, f = 0.0, h = 0.0;
+    if (threadIdx.x < kTreeLanes)
+        for (int64_t i = threadIdx.x; i < n; i += kTreeLanes) {
+            const int64_t p = s.slot[b0 + i];
+            const double vx = (double)s.vxh[p] + (double)s.vxl[p];
+            const double vy = (double)s.vyh[p] + (double)s.vyl[p];
+            const double vz = (double)s.vzh[p] + (double)s.vzl[p];
+            ke = ke + ((vx * vx + vy * vy) + vz * vz);
+            a = a + vx;
+            b = b + vy;
+            c = c + vz;
+            e = e + fabs(vx);
+            f = f + fabs(vy);
+            h = h + fabs(vz);
+        }
+    tree256x3<false>(red, a, b, c);
+    tree256x3<false>(red, e, f, h);
+    ke = tree256<false>(red, ke);
+    if (threadIdx.x == 0) {
+        kpart[blockIdx.x] = ke;
+        double* o = mpart + 6 * blockIdx.x;
+        o[0] = a;
+        o[1] = b;
+        o[2] = c;
+        o[3] = e;
+        o[4] = f;
+        o[5] = h;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 static unsigned blocks23m(int64_t n, int threads) {
@@ -505,6 +592,12 @@ void launch23m_regroup(const Grid23& g, const Tiles23& T, const Species23& s, co
 
 void launch23m_to_caller(const Species23& s, const Species23& o, hipStream_t st) {
     if (s.P > 0) k23m_to_caller<<<blocks23m(s.P, 256), 256, 0, st>>>(s, o);
+}
+
+int64_t launch23m_diag_species(const Species23& s, double* kpart, double* mpart, hipStream_t st) {
+    const int64_t nb = (s.P + kTreeChunk - 1) / kTreeChunk;
+    k23m_diag_species<<<(unsigned)nb, 256, 0, st>>>(s, kpart, mpart);
+    return nb;
 }
 
 void launch23m_permute(const Species23& s, const Species23& o, const uint32_t* perm, hipStream_t st) {

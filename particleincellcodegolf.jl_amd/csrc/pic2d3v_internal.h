@@ -26,6 +26,10 @@ struct Grid23 {
     // box, step = dt*2^32/L held as a float hi + lo pair
     int32_t sx, sy;
     float stepx[2], stepy[2];
+    // the electrostatic pushes add the number of stencils (a gather and a
+    // deposit per particle) that left their chunk's window and read or added
+    // through global memory
+    unsigned long long* fallback;
 };
 
 // mixed precision: shape weights in quanta of 2^-kQ23 of a cell per axis
@@ -56,7 +60,8 @@ struct Species23 {
 
 // Regrouping of held particles by kRegroupTile23 x kRegroupTile23-cell tile.
 constexpr int kRegroupTile23 = 8;
-constexpr int kRegroupEvery23 = 16;  // steps between regroupings
+constexpr int kRegroupEvery23 = 16;  // steps between regroupings (the electrostatic path: between checks)
+constexpr double kRegroupFallback = 0.01;  // electrostatic path: regroup above this fraction of stencils falling back
 struct Tiles23 {
     int32_t TX, TY, ntx, nty, ntiles;
     int32_t *count, *start, *cursor;
@@ -122,6 +127,9 @@ void launch23m_regroup(const Grid23& g, const Tiles23& T, const Species23& s, co
 void launch23m_to_caller(const Species23& s, const Species23& o, hipStream_t st);
 void launch23m_permute(const Species23& s, const Species23& o, const uint32_t* perm, hipStream_t st);
 void launch23m_to64(const Grid23& g, const Species23& s, hipStream_t st);
+// the kinetic and momentum partials of a mixed species (one of each kind per
+// kTreeChunk particles); returns the number of chunks
+int64_t launch23m_diag_species(const Species23& s, double* kpart, double* mpart, hipStream_t st);
 // update! writing the FP32 copy of Exy as well
 void launch23_update32(const Grid23& g, const Diag23& d, const Buffers2D& b, double2* E, float2* E32,
                        int accumulate, hipStream_t st);

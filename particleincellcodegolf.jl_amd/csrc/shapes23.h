@@ -245,6 +245,14 @@ __device__ __forceinline__ int wave_max_i(int v) {
     return v;
 }
 
+// adds the wave's count of fallback particles to *fallback (every lane of
+// the wave calls it)
+__device__ __forceinline__ void count_fallback(u64* fallback, int fb) {
+#pragma unroll
+    for (int h = 32; h > 0; h >>= 1) fb += __shfl_xor(fb, h, 64);
+    if ((threadIdx.x & 63) == 0 && fb && fallback) atomicAdd(fallback, (u64)fb);
+}
+
 __device__ __forceinline__ void add_limbs_lds(u64* Rw, int idx, i128 v) {
     const u128 u = (u128)v;
     constexpr u64 M = (1ull << kLimbS) - 1;
