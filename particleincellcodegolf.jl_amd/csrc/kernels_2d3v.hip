@@ -38,8 +38,10 @@ namespace pic {
 // the field, so the result does not depend on the window.
 // ---------------------------------------------------------------------------
 
+// Occupancy: the quadratic B-spline's loop is capped at 128 registers, four
+// waves per SIMD (the LDS window allows four blocks per CU).
 template <int SH, int BV>
-__global__ void __launch_bounds__(kTile23Threads)
+__global__ void __launch_bounds__(kTile23Threads) __attribute__((amdgpu_waves_per_eu(SH == 4 ? 4 : 1)))
 k23_push(Grid23 g, Species23 s, const double2* __restrict__ E, u64* __restrict__ racc) {
     constexpr int n = ShapeCount<SH>::value;
     constexpr int LO = ShapeExtent<SH>::lo, HI = ShapeExtent<SH>::hi;
@@ -53,22 +55,16 @@ k23_push(Grid23 g, Species23 s, const double2* __restrict__ E, u64* __restrict__
     const int64_t c0 = (int64_t)blockIdx.x * kTile23Chunk;
     // the window: the chunk's base cells relative to its first particle's
     const int ax = base_cell(s.x[c0], g.NX_Lx, NX), ay = base_cell(s.y[c0], g.NY_Ly, NY);
-    double xs[kTile23PPT], ys[kTile23PPT], us[kTile23PPT], vs[kTile23PPT], ws[kTile23PPT];
+    double xs[kTile23PPT], ys[kTile23PPT];
     int mnx = INT_MAX, mxx = INT_MIN, mny = INT_MAX, mxy = INT_MIN;
 #pragma unroll
     for (int k = 0; k < kTile23PPT; ++k) {
         const int64_t p = c0 + t + (int64_t)k * kTile23Threads;
         xs[k] = 0.0;
         ys[k] = 0.0;
-        us[k] = 0.0;
-        vs[k] = 0.0;
-        ws[k] = 0.0;
         if (p < s.P) {
             xs[k] = s.x[p];
             ys[k] = s.y[p];
-            us[k] = s.vx[p];
-            vs[k] = s.vy[p];
-            ws[k] = s.vz[p];
             const int dx = ((base_cell(xs[k], g.NX_Lx, NX) - ax + NX / 2) & (NX - 1)) - NX / 2;
             const int dy = ((base_cell(ys[k], g.NY_Ly, NY) - ay + NY / 2) & (NY - 1)) - NY / 2;
             mnx = min(mnx, dx);
@@ -130,11 +126,12 @@ k23_push(Grid23 g, Species23 s, const double2* __restrict__ E, u64* __restrict__
     for (int k = 0; k < kTile23PPT; ++k) {
         const int64_t p = c0 + t + (int64_t)k * kTile23Threads;
         if (p >= s.P) break;
-        double x = xs[k], y = ys[k], vx = us[k], vy = vs[k], vz = ws[k];
+        double x = xs[k], y = ys[k], vx = s.vx[p], vy = s.vy[p], vz = s.vz[p];
         int32_t cx[n], cy[n];
         double wxs[n], wys[n];
         int lx[n], ly[n];
-        // gather at the old position
+        // gather at the old position (the window and the fallback read
+        // through separate loads, LDS and global, not one generic pointer)
         shape_w<SH>(x, g.NX_Lx, NX, cx, wxs);
         shape_w<SH>(y, g.NY_Ly, NY, cy, wys);
         bool in = tile;
@@ -145,15 +142,27 @@ k23_push(Grid23 g, Species23 s, const double2* __restrict__ E, u64* __restrict__
             in = in && lx[q] < wx && ly[q] < wy;
         }
         double a = 0.0, b = 0.0;
+        if (in) {
 #pragma unroll
-        for (int jj = 0; jj < n; ++jj)
+            for (int jj = 0; jj < n; ++jj)
 #pragma unroll
-            for (int ii = 0; ii < n; ++ii) {
-                const double wxy = wxs[ii] * wys[jj];
-                const double2 e = in ? Ew[lx[ii] + wx * ly[jj]] : E[cx[ii] + (int64_t)NX * cy[jj]];
-                a = fma(e.x, wxy, a);
-                b = fma(e.y, wxy, b);
-            }
+                for (int ii = 0; ii < n; ++ii) {
+                    const double wxy = wxs[ii] * wys[jj];
+                    const double2 e = Ew[lx[ii] + wx * ly[jj]];
+                    a = fma(e.x, wxy, a);
+                    b = fma(e.y, wxy, b);
+                }
+        } else {
+#pragma unroll
+            for (int jj = 0; jj < n; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < n; ++ii) {
+                    const double wxy = wxs[ii] * wys[jj];
+                    const double2 e = E[cx[ii] + (int64_t)NX * cy[jj]];
+                    a = fma(e.x, wxy, a);
+                    b = fma(e.y, wxy, b);
+                }
+        }
         fb += in ? 0 : 1;
         const double vx0 = vx, vy0 = vy;
         boris23<BV>(g, s.q_m, a, b, vx, vy, vz);
@@ -175,14 +184,19 @@ k23_push(Grid23 g, Species23 s, const double2* __restrict__ E, u64* __restrict__
             in = in && lx[q] < wx && ly[q] < wy;
         }
         fb += in ? 0 : 1;
+        if (in) {
 #pragma unroll
-        for (int jj = 0; jj < n; ++jj)
+            for (int jj = 0; jj < n; ++jj)
 #pragma unroll
-            for (int ii = 0; ii < n; ++ii) {
-                const i128 v = fix_round128(wxs[ii] * wys[jj] * s.qw, g.q);
-                if (in) add_limbs_lds(Rw, lx[ii] + wx * ly[jj], v);
-                else add_limbs_s(racc, G, cx[ii] + (int64_t)NX * cy[jj], v);
-            }
+                for (int ii = 0; ii < n; ++ii)
+                    add_limbs_lds(Rw, lx[ii] + wx * ly[jj], fix_round128(wxs[ii] * wys[jj] * s.qw, g.q));
+        } else {
+#pragma unroll
+            for (int jj = 0; jj < n; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < n; ++ii)
+                    add_limbs_s(racc, G, cx[ii] + (int64_t)NX * cy[jj], fix_round128(wxs[ii] * wys[jj] * s.qw, g.q));
+        }
     }
     count_fallback(g.fallback, fb);
     __syncthreads();

@@ -174,6 +174,7 @@ k23m_push(Grid23 g, Species23 s, const float2* __restrict__ E32) {
     constexpr float kW = 1.0f / (float)(1u << kQ23);
     __shared__ float2 Ew[kTile23Cells];
     __shared__ u64 Rw[kTile23Cells];
+    __shared__ uint32_t Xs[kTile23Chunk], Ys[kTile23Chunk];  // the chunk's positions, read once
     __shared__ int red[kTile23Threads / 64][4];
     __shared__ int win[5];
     const int t = threadIdx.x;
@@ -188,6 +189,8 @@ k23m_push(Grid23 g, Species23 s, const float2* __restrict__ E32) {
         const int64_t p = c0 + t + (int64_t)k * kTile23Threads;
         if (p < s.P) {
             const uint32_t xq = s.xq[p], yq = s.yq[p];
+            Xs[t + k * kTile23Threads] = xq;
+            Ys[t + k * kTile23Threads] = yq;
             const int dx = (((int)((xq - 1u) >> g.sx) - ax + NX / 2) & (NX - 1)) - NX / 2;
             const int dy = (((int)((yq - 1u) >> g.sy) - ay + NY / 2) & (NY - 1)) - NY / 2;
             mnx = min(mnx, dx);
@@ -248,7 +251,7 @@ k23m_push(Grid23 g, Species23 s, const float2* __restrict__ E32) {
     for (int k = 0; k < kTile23PPT; ++k) {
         const int64_t p = c0 + t + (int64_t)k * kTile23Threads;
         if (p >= s.P) break;
-        uint32_t xq = s.xq[p], yq = s.yq[p];  // read again (L2): fewer registers held over the staging
+        uint32_t xq = Xs[t + k * kTile23Threads], yq = Ys[t + k * kTile23Threads];
         float vx = s.vxh[p], vy = s.vyh[p], vz = s.vzh[p];
         float lx = s.vxl[p], ly = s.vyl[p], lz = s.vzl[p];
         uint32_t qx[n], qy[n];

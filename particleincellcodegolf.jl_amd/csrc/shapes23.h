@@ -217,7 +217,9 @@ __device__ __forceinline__ void deposit23(const Grid23& g, u64* __restrict__ rac
 constexpr int kTile23Threads = 256;
 constexpr int kTile23PPT = 4;
 constexpr int kTile23Chunk = kTile23Threads * kTile23PPT;
-constexpr int kTile23Cells = 1024;  // window cells: 16 B of field + 3 x 8 B of limbs each (40 KB)
+// window cells: 16 B of field + 3 x 8 B of limbs each in the FP64 push
+// (39.4 KB with its bookkeeping: four blocks share a CU's 160 KB)
+constexpr int kTile23Cells = 1008;
 constexpr int kTile23Halo = 1;
 
 // stencil extent about the base cell i = ceil(zN): cells i - lo .. i + hi
