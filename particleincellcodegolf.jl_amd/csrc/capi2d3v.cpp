@@ -48,7 +48,11 @@ struct pic2d3v_ctx {
     std::vector<void*> allocs;
     double* tw = nullptr;
     int64_t step = 0;        // t of the next step
-    hipGraphExec_t g_plain = nullptr, g_diag = nullptr;
+    // step graphs per buffer parity: a regroup exchanges every species' state
+    // with its other buffer set and flips the parity, so the graphs of both
+    // sets stay valid and are captured once each
+    int parity = 0;
+    hipGraphExec_t g_plain[2] = {}, g_diag[2] = {};
     // Lorenz gauge: the state, the half-step positions, and the step graphs
     // (loop! alone, with diagnose! without and with the row)
     bool lorenz = false;
@@ -60,7 +64,7 @@ struct pic2d3v_ctx {
     std::vector<int64_t> iters;  // semi-implicit: iterations of the last call's steps
     double* xh[kMaxSpecies23] = {};
     double* yh[kMaxSpecies23] = {};
-    hipGraphExec_t g_lg[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t g_lg[2][3] = {};
     void* op[8] = {};        // scratch (grown on demand)
     size_t op_bytes[8] = {};
     std::string err;
@@ -128,12 +132,11 @@ static int scratch(pic2d3v_ctx* c, int slot, size_t bytes, void** out) {
 }
 
 static void free_graphs(pic2d3v_ctx* c) {
-    if (c->g_plain) (void)hipGraphExecDestroy(c->g_plain);
-    if (c->g_diag) (void)hipGraphExecDestroy(c->g_diag);
-    c->g_plain = c->g_diag = nullptr;
-    for (auto& g : c->g_lg) {
-        if (g) (void)hipGraphExecDestroy(g);
-        g = nullptr;
+    for (int q = 0; q < 2; ++q) {
+        for (hipGraphExec_t* g : {&c->g_plain[q], &c->g_diag[q], &c->g_lg[q][0], &c->g_lg[q][1], &c->g_lg[q][2]}) {
+            if (*g) (void)hipGraphExecDestroy(*g);
+            *g = nullptr;
+        }
     }
 }
 
@@ -669,8 +672,24 @@ extern "C" int pic2d3v_field(pic2d3v_ctx* c, double* Ex, double* Ey) {
 
 // ---- sort!(species, dx, dy) -------------------------------------------------
 
-// Regroups every species' held particles by tile (see launch23_regroup); the
-// captured step graphs hold the old arrays and are dropped.
+// The state arrays (FP64 or mixed) of `from` copied into those of `to`, on
+// the context's stream.
+static int copy_state(pic2d3v_ctx* c, const Species23& from, const Species23& to) {
+    const size_t P = (size_t)from.P;
+    if (c->mixed) {
+        const void* src[8] = {from.xq, from.yq, from.vxh, from.vyh, from.vzh, from.vxl, from.vyl, from.vzl};
+        void* dst[8] = {to.xq, to.yq, to.vxh, to.vyh, to.vzh, to.vxl, to.vyl, to.vzl};
+        for (int k = 0; k < 8; ++k) HIPCHK(c, hipMemcpyAsync(dst[k], src[k], 4 * P, hipMemcpyDeviceToDevice, c->stream));
+        return PIC_OK;
+    }
+    const void* src[5] = {from.x, from.y, from.vx, from.vy, from.vz};
+    void* dst[5] = {to.x, to.y, to.vx, to.vy, to.vz};
+    for (int k = 0; k < 5; ++k) HIPCHK(c, hipMemcpyAsync(dst[k], src[k], 8 * P, hipMemcpyDeviceToDevice, c->stream));
+    return PIC_OK;
+}
+
+// Regroups every species' held particles by tile (see launch23_regroup) into
+// the other buffer set and flips the parity of the step graphs.
 static int regroup(pic2d3v_ctx* c) {
     for (int s = 0; s < c->cfg.nspecies; ++s) {
         Species23& sp = c->sp[s];
@@ -692,10 +711,7 @@ static int regroup(pic2d3v_ctx* c) {
         c->held_identity[s] = false;
     }
     HIPCHK(c, hipGetLastError());
-    // a graph launched before the regroup may still run on the stream: wait
-    // for it before its executable is destroyed
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    free_graphs(c);
+    c->parity ^= 1;
     c->since_regroup = 0;
     c->regroup_due = false;
     HIPCHK(c, hipMemsetAsync(c->g.fallback, 0, sizeof(unsigned long long), c->stream));
@@ -725,27 +741,19 @@ static int maybe_regroup(pic2d3v_ctx* c) {
     return regroup(c);
 }
 
-// The held particles of species s back in the caller's order.
+// The held particles of species s back in the caller's order (through the
+// other buffer set and copied back, so the arrays the graphs hold stay).
 static int to_caller(pic2d3v_ctx* c, int32_t s) {
     if (c->held_identity[s]) return PIC_OK;
     Species23& sp = c->sp[s];
     Species23& al = c->alt[s];
-    if (c->mixed) {
-        launch23m_to_caller(sp, al, c->stream);
-        swap_mixed(sp, al);
-    } else {
-        launch23_to_caller(sp, al, c->stream);
-        std::swap(sp.x, al.x);
-        std::swap(sp.y, al.y);
-        std::swap(sp.vx, al.vx);
-        std::swap(sp.vy, al.vy);
-        std::swap(sp.vz, al.vz);
-    }
+    if (c->mixed) launch23m_to_caller(sp, al, c->stream);
+    else launch23_to_caller(sp, al, c->stream);
+    int rc;
+    if ((rc = copy_state(c, al, sp))) return rc;
     launch23_identity(c->pid[s], sp.slot, sp.P, c->stream);
     HIPCHK(c, hipGetLastError());
     c->held_identity[s] = true;
-    HIPCHK(c, hipStreamSynchronize(c->stream));  // no graph in flight when it is destroyed
-    free_graphs(c);
     return PIC_OK;
 }
 
@@ -814,11 +822,7 @@ extern "C" int pic2d3v_sort(pic2d3v_ctx* c, int32_t s, double dx, double dy, int
     launch23_sort_order(sp, nb, w, c->stream);
     if (c->mixed) {
         launch23m_permute(c->sp[s], c->alt[s], w.perm, c->stream);
-        swap_mixed(c->sp[s], c->alt[s]);
-        // the captured steps hold the old arrays: drop them once nothing runs
-        HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        free_graphs(c);
+        if ((rc = copy_state(c, c->alt[s], c->sp[s]))) return rc;
     } else {
         if ((rc = scratch(c, 0, sizeof(double) * 5 * per, &p))) return rc;
         double* dst[5];
@@ -1049,7 +1053,8 @@ extern "C" int pic2d3v_run(pic2d3v_ctx* c, int64_t nsteps, double* rows) {
             if ((rc = maybe_regroup(c))) return rc;
             ++c->since_regroup;
             const bool diag = t % nt == 0;
-            hipGraphExec_t& ge = c->lorenz ? c->g_lg[diag ? 2 : 1] : (diag ? c->g_diag : c->g_plain);
+            const int q = c->parity;
+            hipGraphExec_t& ge = c->lorenz ? c->g_lg[q][diag ? 2 : 1] : (diag ? c->g_diag[q] : c->g_plain[q]);
             if (!ge && (rc = capture(c, c->lorenz ? (diag ? 2 : 1) : (diag ? 1 : 0), &ge))) return rc;
             HIPCHK(c, hipGraphLaunch(ge, c->stream));
         }
@@ -1199,8 +1204,9 @@ extern "C" int pic2d3v_loop(pic2d3v_ctx* c, int64_t nsteps) {
     for (int64_t t = 0; t < nsteps; ++t) {
         if ((rc = maybe_regroup(c))) return rc;
         ++c->since_regroup;
-        if (!c->g_lg[0] && (rc = capture(c, 0, &c->g_lg[0]))) return rc;
-        HIPCHK(c, hipGraphLaunch(c->g_lg[0], c->stream));
+        hipGraphExec_t& ge = c->g_lg[c->parity][0];
+        if (!ge && (rc = capture(c, 0, &ge))) return rc;
+        HIPCHK(c, hipGraphLaunch(ge, c->stream));
     }
     if ((rc = finish(c))) return rc;
     c->step += nsteps;
