@@ -317,26 +317,29 @@ k23_diag_field(Grid23 g, Diag23 d, const double2* __restrict__ E, int64_t nf) {
     if (threadIdx.x == 0) d.fpart[blockIdx.x] = r;
 }
 
-// sum(abs2, velocities(s)) over the 3P entries in xyv's column order
+// sum(abs2, velocities(s)) over the 3P entries in xyv's column order.  A
+// block's chunk of entries covers about a third as many particles: they are
+// read once each (caller index -> slot -> vx, vy, vz) into LDS as the chunk's
+// squares, then summed by the canonical tree in entry order.
 __global__ void __launch_bounds__(256)
 k23_diag_kinetic(Species23 s, double* __restrict__ kpart) {
     __shared__ double red[kTreeLanes + 8];
+    __shared__ double sq[kTreeChunk];
     const int64_t n3 = 3 * s.P;
     const int64_t b0 = (int64_t)blockIdx.x * kTreeChunk;
     const int64_t n = n3 - b0 < kTreeChunk ? n3 - b0 : kTreeChunk;
-    // entry b0 + i is component c of particle p: one 64-bit division per
-    // block, 32-bit ones per entry
-    const int64_t p0 = b0 / 3;
-    const uint32_t r0 = (uint32_t)(b0 - 3 * p0);
-    const double r = block_chunk_sum(red, n, [&](int64_t i) {
-        const uint32_t q = r0 + (uint32_t)i;
-        const uint32_t dp = q / 3u;
-        const uint32_t c = q - 3u * dp;
-        const int64_t p = p0 + dp;
+    const int64_t p0 = b0 / 3, p1 = (b0 + n - 1) / 3;  // the particles of entries b0 .. b0 + n - 1
+    for (int64_t p = p0 + threadIdx.x; p <= p1; p += blockDim.x) {
         const int64_t h = s.slot ? (int64_t)s.slot[p] : p;  // caller index p's slot
-        const double v = c == 0 ? s.vx[h] : (c == 1 ? s.vy[h] : s.vz[h]);
-        return v * v;
-    });
+        const double v[3] = {s.vx[h], s.vy[h], s.vz[h]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int64_t e = 3 * p + c - b0;
+            if (e >= 0 && e < n) sq[e] = v[c] * v[c];
+        }
+    }
+    __syncthreads();
+    const double r = block_chunk_sum(red, n, [&](int64_t i) { return sq[i]; });
     if (threadIdx.x == 0) kpart[blockIdx.x] = r;
 }
 
