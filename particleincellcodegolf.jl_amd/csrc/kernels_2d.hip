@@ -735,7 +735,12 @@ k2d_cols(Desc2D D, const double2* __restrict__ S, double2* __restrict__ T1, doub
     // re/im planes: line c of array a (0: spectrum, 1: Ex^, 2: Ey^, 3: phi^) at (a*C + c)*st
     double* re = lds2;
     double* im = lds2 + 4 * C * st;
-    const int i0 = blockIdx.x * C;
+    // blocks go round-robin to the 8 XCDs, each with its own L2: give every
+    // XCD a contiguous range of columns, so the neighbouring columns that
+    // share a cache line are read through one L2
+    const int nb = gridDim.x;
+    const int bi = (nb & 7) == 0 ? (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+    const int i0 = bi * C;
     for (int e = threadIdx.x; e < C * NY; e += blockDim.x) {
         const int c = e & (C - 1), j = e >> lc;
         const double2 v = S[(int64_t)j * NX + i0 + c];

@@ -176,7 +176,11 @@ __global__ void __launch_bounds__(256) k_lg_cols(LGList lst, int NX, int NY, int
     double* re = lds_lg;
     double* im = lds_lg + C * st;
     double2* Z = lst.a[blockIdx.y];
-    const int i0 = blockIdx.x * C;
+    // a contiguous range of columns per XCD (blocks go round-robin to the 8
+    // XCDs): neighbouring columns share cache lines, read through one L2
+    const int nb = gridDim.x;
+    const int bi = (nb & 7) == 0 ? (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+    const int i0 = bi * C;
     for (int e = threadIdx.x; e < C * NY; e += blockDim.x) {
         const int c = e & (C - 1), j = e >> lc;
         const double2 v = Z[(int64_t)j * NX + i0 + c];
