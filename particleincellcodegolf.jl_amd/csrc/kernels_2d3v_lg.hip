@@ -43,8 +43,12 @@
 
 namespace pic {
 
-constexpr int kLGWinCells = 1024;  // push window: six doubles per cell (48 KB)
-constexpr int kLGDepCells = 768;   // deposit window: NG grids x three limbs per cell
+// Window budgets, sized so that four blocks share a CU's 160 KB of LDS: the
+// push stages six doubles per cell (39 KB), the current deposit three grids
+// of three limbs (39 KB).  A chunk of 1024 particles sorted per cell spans
+// about 6 x 69 cells with the quadratic B-spline, a regrouped one 13 x 13.
+constexpr int kLGWinCells = 832;
+constexpr int kLGDepCells = 552;
 
 __host__ __device__ inline int lg_cols(int NX, int NY) {
     const int c = kLGLinePoints / NY < 1 ? 1 : kLGLinePoints / NY;
@@ -364,8 +368,10 @@ __device__ __forceinline__ void lg_window(const Grid23& g, const double* xs, con
 // 1 (LorenzGaugeField, :683-690): x = unimod(x + (vx0 + vx)/2*dt); 2
 // (LorenzGaugeSemiImplicitField, :1036-1040): xw = unimod(x + vx0*dt) and vw
 // into the work arrays o, s unchanged (modes 0 and 1 write s, o unused).
+// (the quadratic B-spline's loop is capped at 128 registers: four waves per
+// SIMD, which the LDS window allows)
 template <int SH, int MODE>
-__global__ void __launch_bounds__(kTile23Threads)
+__global__ void __launch_bounds__(kTile23Threads) __attribute__((amdgpu_waves_per_eu(SH == 4 ? 4 : 1)))
 k_lg_push(LG23 L, Grid23 g, Species23 s, Species23 o, int sidx, double* __restrict__ xh, double* __restrict__ yh) {
     constexpr bool CENTRED = MODE == 1;
     constexpr int n = ShapeCount<SH>::value;
@@ -422,21 +428,29 @@ k_lg_push(LG23 L, Grid23 g, Species23 s, Species23 o, int sidx, double* __restri
             in = in && lx[q] < wx && ly[q] < wy;
         }
         double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        // the window and the fallback in blocks of their own (one branch per
+        // particle, not one per stencil cell)
+        if (in) {
 #pragma unroll
-        for (int jj = 0; jj < n; ++jj)
+            for (int jj = 0; jj < n; ++jj)
 #pragma unroll
-            for (int ii = 0; ii < n; ++ii) {
-                const double wxy = wxs[ii] * wys[jj];
-                if (in) {
+                for (int ii = 0; ii < n; ++ii) {
+                    const double wxy = wxs[ii] * wys[jj];
                     const int e = lx[ii] + wx * ly[jj];
 #pragma unroll
                     for (int c = 0; c < 6; ++c) f[c] = fma(Ew[c * kLGWinCells + e], wxy, f[c]);
-                } else {
+                }
+        } else {
+#pragma unroll
+            for (int jj = 0; jj < n; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < n; ++ii) {
+                    const double wxy = wxs[ii] * wys[jj];
                     const int64_t kk = cx[ii] + (int64_t)NX * cy[jj];
 #pragma unroll
                     for (int c = 0; c < 6; ++c) f[c] = fma(L.EB[c * G + kk], wxy, f[c]);
                 }
-            }
+        }
         const double vx0 = vx, vy0 = vy;
         lg_boris(dt, s.q_m, f, vx, vy, vz);
         if (MODE == 2) {
