@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(kEMThreads) k_em_move(EM1D s, int sweep) {
 }
 
 // d((X+x)/2), d((X+x)/2, (V2+v2)/2), d((X+x)/2, (V3+v3)/2) (:20, :22, :23)
-__global__ void __launch_bounds__(kEMThreads) k_em_deposit3(EM1D s, int sweep) {
+__global__ void __launch_bounds__(kEMDepThreads) k_em_deposit3(EM1D s, int sweep) {
     extern __shared__ u64 ldsu[];
     if (s.ctl[sweep] == 0) return;
     const int N = s.N;
@@ -1130,7 +1130,7 @@ void em_launch_sweep(const EM1D& s, int sweep, hipStream_t st) {
     const unsigned nt = (unsigned)((s.P + s.tile - 1) / s.tile);
     if (sweep == 0) k_em_move<true><<<nb, kEMThreads, 0, st>>>(s, sweep);
     else k_em_move<false><<<nb, kEMThreads, 0, st>>>(s, sweep);
-    k_em_deposit3<<<nt, kEMThreads, sizeof(u64) * 9 * (size_t)s.N, st>>>(s, sweep);
+    k_em_deposit3<<<nt, kEMDepThreads, sizeof(u64) * 9 * (size_t)s.N, st>>>(s, sweep);
     k_em_solve<0, true><<<1, kEMSolveThreads, em_solve_lds_bytes(s.N), st>>>(s, sweep, nullptr, nullptr, nullptr);
     k_em_push<<<nb, kEMThreads, sizeof(double) * 5 * (size_t)s.N, st>>>(s, sweep);
 }

@@ -9,6 +9,11 @@ namespace pic {
 constexpr int kMaxNEM = 1024;
 constexpr int64_t kMaxPEM = int64_t(1) << 26;
 constexpr int kEMThreads = 256;
+// the charge-and-currents deposit holds a whole grid of limbs in LDS per
+// block (9N u64, 72 KB at N = 1024: two blocks per CU) and 146 registers per
+// lane (three waves per SIMD): 768-thread blocks run three waves per SIMD
+// where two 256-thread blocks ran two
+constexpr int kEMDepThreads = 768;
 constexpr int kEMSolveThreads = 256;
 constexpr int kEMMomentGroups = 4;  // moment grids per group: 2, 2, 2, 1
 constexpr int kEMScanTile = 4096;
