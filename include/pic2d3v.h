@@ -211,7 +211,9 @@ int pic2d3v_reset(pic2d3v_ctx* ctx);
  * Fails with PIC_EINVAL if the cells span more than 2^24 or one cell holds
  * more than 16384 particles.  The context itself keeps its particles grouped
  * by 8 x 8-cell tile for the LDS windows of its deposits (before the first
- * step after an upload and every 16 steps); that order is internal:
+ * step after an upload; then, checked every 16 steps, when more than 1% of
+ * the stencils fell outside their windows, or always for the Lorenz-gauge
+ * fields); that order is internal:
  * pic2d3v_download, the diagnostics and this sort see the caller's order,
  * and no result depends on it. */
 int pic2d3v_sort(pic2d3v_ctx* ctx, int32_t species, double dx, double dy, int64_t* perm);
