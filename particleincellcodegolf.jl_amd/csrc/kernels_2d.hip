@@ -723,7 +723,9 @@ k2d_rows_fwd(Desc2D D, u64* __restrict__ racc, const double* __restrict__ rho, d
 }
 
 // Forward transform along y of C columns, the spectral field (:142-152), and
-// the inverse transforms along y of Ex^, Ey^ (and phi^ when want_phi).
+// the inverse transforms along y of Ex^, Ey^ (and phi^ when want_phi), one
+// after the other through one work group of lines, so that the block holds
+// two groups of C lines in LDS (the spectrum and the work lines).
 __global__ void __launch_bounds__(kFFT2DThreads)
 k2d_cols(Desc2D D, const double2* __restrict__ S, double2* __restrict__ T1, double2* __restrict__ T2,
          double2* __restrict__ T3, const double2* __restrict__ tw, int32_t* __restrict__ flag, int want_phi) {
@@ -732,9 +734,9 @@ k2d_cols(Desc2D D, const double2* __restrict__ S, double2* __restrict__ T1, doub
     const int C = cols_per_block2d(NX, NY);
     const int st = line_stride(NY);
     const int lc = __ffs(C) - 1;
-    // re/im planes: line c of array a (0: spectrum, 1: Ex^, 2: Ey^, 3: phi^) at (a*C + c)*st
+    // re/im planes: line c of group a (0: spectrum, 1: work) at (a*C + c)*st
     double* re = lds2;
-    double* im = lds2 + 4 * C * st;
+    double* im = lds2 + 2 * C * st;
     // blocks go round-robin to the 8 XCDs, each with its own L2: give every
     // XCD a contiguous range of columns, so the neighbouring columns that
     // share a cache line are read through one L2
@@ -750,6 +752,8 @@ k2d_cols(Desc2D D, const double2* __restrict__ S, double2* __restrict__ T1, doub
     }
     __syncthreads();
     fft_lines(re, im, tw, ly, C, st, false);
+    // the spectral field: phi^ = rho^ * minvkk in place in the spectrum
+    // group (bit-reversed for the inverse), phi[1, 1] = 0
     bool bad = false;
     for (int e = threadIdx.x; e < C * NY; e += blockDim.x) {
         const int c = e & (C - 1), j = e >> lc;
@@ -769,26 +773,39 @@ k2d_cols(Desc2D D, const double2* __restrict__ S, double2* __restrict__ T1, doub
         const double tr = a * cc - b * d;
         const double ti = a * d + b * cc;
         bad |= !(isfinite(tr) && isfinite(ti));
+        // the work group takes the pre-product spectrum (a, b) in bit-reversed
+        // order for phi^; (tr, ti) replace the spectrum in natural order
         const int pb = padi((int)brev((uint32_t)j, ly));
-        re[(C + c) * st + pb] = tr * kx;
-        im[(C + c) * st + pb] = ti * kx;
-        re[(2 * C + c) * st + pb] = tr * ky;
-        im[(2 * C + c) * st + pb] = ti * ky;
-        if (want_phi) {
-            re[(3 * C + c) * st + pb] = a;
-            im[(3 * C + c) * st + pb] = b;
-        }
+        re[(C + c) * st + pb] = a;
+        im[(C + c) * st + pb] = b;
+        re[c * st + padi(j)] = tr;
+        im[c * st + padi(j)] = ti;
     }
     if (bad) atomicOr(flag, 1);
     __syncthreads();
-    fft_lines(re + C * st, im + C * st, tw, ly, (want_phi ? 3 : 2) * C, st, true);
-    for (int e = threadIdx.x; e < C * NY; e += blockDim.x) {
-        const int c = e & (C - 1), j = e >> lc;
-        const int64_t k = (int64_t)j * NX + i0 + c;
-        const int pj = padi(j);
-        T1[k] = make_double2(re[(C + c) * st + pj], im[(C + c) * st + pj]);
-        T2[k] = make_double2(re[(2 * C + c) * st + pj], im[(2 * C + c) * st + pj]);
-        if (want_phi) T3[k] = make_double2(re[(3 * C + c) * st + pj], im[(3 * C + c) * st + pj]);
+    // phi^ first (its lines are in place), then Ex^ = tr*kx and Ey^ = tr*ky
+    // from the products kept in the spectrum group
+    for (int a = want_phi ? 0 : 1; a < 3; ++a) {
+        double2* T = a == 0 ? T3 : (a == 1 ? T1 : T2);
+        if (a > 0) {
+            for (int e = threadIdx.x; e < C * NY; e += blockDim.x) {
+                const int c = e & (C - 1), j = e >> lc;
+                const int i = i0 + c;
+                const double k = a == 1 ? D.kxs * (double)(i < NX / 2 ? i : i - NX)
+                                        : D.kys * (double)(j < NY / 2 ? j : j - NY);
+                const int pb = padi((int)brev((uint32_t)j, ly));
+                re[(C + c) * st + pb] = re[c * st + padi(j)] * k;
+                im[(C + c) * st + pb] = im[c * st + padi(j)] * k;
+            }
+            __syncthreads();
+        }
+        fft_lines(re + C * st, im + C * st, tw, ly, C, st, true);
+        for (int e = threadIdx.x; e < C * NY; e += blockDim.x) {
+            const int c = e & (C - 1), j = e >> lc;
+            const int pj = padi(j);
+            T[(int64_t)j * NX + i0 + c] = make_double2(re[(C + c) * st + pj], im[(C + c) * st + pj]);
+        }
+        __syncthreads();
     }
 }
 

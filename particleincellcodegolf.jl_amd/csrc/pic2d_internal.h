@@ -85,13 +85,13 @@ __host__ __device__ inline int cols_per_block2d(int NX, int NY) {
     const int c = kLine2D / NY < 1 ? 1 : kLine2D / NY;
     return c < NX ? c : NX;
 }
-// re and im planes of `arrays` groups of R rows / of the column kernel's four
-// groups (spectrum, Ex^, Ey^, phi^) of C columns
+// re and im planes of `arrays` groups of R rows / of the column kernel's two
+// groups (spectrum, work) of C columns
 inline size_t rows_lds_bytes(int NX, int NY, int arrays) {
     return sizeof(double) * 2 * (size_t)arrays * rows_per_block2d(NX, NY) * line_stride(NX);
 }
 inline size_t cols_lds_bytes(int NX, int NY) {
-    return sizeof(double) * 2 * 4 * (size_t)cols_per_block2d(NX, NY) * line_stride(NY);
+    return sizeof(double) * 2 * 2 * (size_t)cols_per_block2d(NX, NY) * line_stride(NY);
 }
 
 // one time step: push + deposit, then the field solve
