@@ -17,7 +17,7 @@ enum LGSlot {
 enum { kLGRhoM = kLGSlots, kLGJxM, kLGJyM, kLGJzM, kLGSlotsSI };
 constexpr int kLGBins = 14;      // binned fields of LorenzGaugeDiagnostics
 constexpr int kLGRow = 11;       // doubles per diagnostics row
-constexpr int kLGLinePoints = 4096;  // transform points per block (whole lines)
+constexpr int kLGLinePoints = 2048;  // transform points per block (whole lines; 35 KB of LDS at 1024)
 
 struct LG23 {
     int32_t NX, NY, lx, ly;
