@@ -308,13 +308,17 @@ __device__ __forceinline__ void radix2_group(double* ar, double* ai, const doubl
 // ---------------------------------------------------------------------------
 typedef __int128 i128;
 
-// rint(a * 2^q) as a 128-bit integer; exact below 2^126 in magnitude, which
-// saturates.  Non-finite values give 0 (a non-finite field is reported by the
-// solve's check).
-__device__ __forceinline__ i128 fix_round128(double a, int q) {
+// rint(a * 2^q) (ties to even) as a 128-bit integer; exact below 2^126 in
+// magnitude, which saturates.  Non-finite values, and a * 2^q beyond the
+// double range, give 0 (a non-finite field is reported by the solve's check).
+// Below 2^63 the integer t converts directly; beyond, it is split into
+// hi * 2^64 + lo by a floor.  lo = t - hi * 2^64 is exact there for either
+// sign: t's lowest bit is at least 2^11, so lo < 2^64 needs at most 53 bits
+// (below 2^63 a negative t's lo could need 54).
+__host__ __device__ __forceinline__ i128 fix_round128(double a, int q) {
     const double t = rint(ldexp(a, q));
-    if (fabs(t) < 0x1p62) return (i128)(long long)t;
-    if (t != t || isinf(t)) return 0;
+    if (fabs(t) < 0x1p63) return (i128)(long long)t;
+    if (t != t || __builtin_isinf(t)) return 0;
     if (!(fabs(t) < 0x1p126)) return t > 0 ? ((i128)1 << 126) : -((i128)1 << 126);
     const double hi = floor(t * 0x1p-64);  // exact: t is an integer beyond 2^53
     const double lo = t - hi * 0x1p64;     // exact, in [0, 2^64)
