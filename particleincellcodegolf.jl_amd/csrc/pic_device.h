@@ -348,10 +348,14 @@ __device__ __forceinline__ i128 wave_sum_i128(i128 v) {
 // t+h: the same operands as the LDS form).  Requires blockDim.x >= 256; all
 // threads of the block must call.  `red` is LDS of >= 256 doubles.
 // ---------------------------------------------------------------------------
+// NaN-propagating maximum: a if a is NaN, else b if b is NaN, else
+// (a > b ? a : b).  Written as three selects so that the compiler emits
+// compares and v_cndmask instead of a branch per call (the norm and tree
+// loops call it thousands of times per launch).
 __device__ __forceinline__ double nanmax(double a, double b) {
-    if (a != a) return a;
-    if (b != b) return b;
-    return a > b ? a : b;
+    const double m = a > b ? a : b;
+    const double r = b != b ? b : m;
+    return a != a ? a : r;
 }
 
 template <bool MAX>

@@ -308,9 +308,10 @@ __device__ void block_norms3(double* red, int n, FF&& F, FE&& E, double& nd, dou
                 xf[k] = i < n ? F(i) : 0.0;
                 xe[k] = i < n ? E(i) : 0.0;
             }
+            // a value past n is +0.0, which leaves every maximum (>= +0 or
+            // NaN) and every sum of squares (>= +0, Inf or NaN) as it is
 #pragma unroll
-            for (int k = 0; k < K; ++k)
-                if (t + k * kTreeLanes < n) add(xf[k], xe[k]);
+            for (int k = 0; k < K; ++k) add(xf[k], xe[k]);
         } else {
             for (int i = t; i < n; i += kTreeLanes) add(F(i), E(i));
         }
