@@ -89,10 +89,7 @@ __device__ __forceinline__ uint32_t quanta(float w30) {
 // a 64-bit one on random slots, and the rare carry adds run with few lanes.
 // A slot takes at most 2^30 quanta per particle of the tile, so hi counts at
 // most a quarter of the tile's particles.  Called by every lane of a wave (dep marks lanes with a
-// stencil to deposit).  The wraps of a lane's 11 adds are collected as a bit
-// mask and added in a loop over its set bits: a wave then issues one hi add
-// per wrap of its busiest lane (mostly one or two) instead of one per window
-// slot that wrapped in any lane.
+// stencil to deposit).
 __device__ __forceinline__ void lane_deposit(uint32_t* __restrict__ lo, uint32_t* __restrict__ hi,
                                              const Stencil32& s, bool dep) {
     if (!dep) return;
@@ -103,13 +100,9 @@ __device__ __forceinline__ void lane_deposit(uint32_t* __restrict__ lo, uint32_t
     PIC_DEP(0) PIC_DEP(1) PIC_DEP(2) PIC_DEP(3) PIC_DEP(4) PIC_DEP(5) PIC_DEP(6) PIC_DEP(7) PIC_DEP(8) PIC_DEP(9)
     PIC_DEP(10)
 #undef PIC_DEP
-    uint32_t wraps = 0;
 #pragma unroll
-    for (int j = 0; j < kWin; ++j) wraps |= (old[j] + q[j] < old[j]) ? (1u << j) : 0u;
-    while (wraps) {
-        atomicAdd(&hi[s.i0 + __builtin_ctz(wraps)], 1u);
-        wraps &= wraps - 1;
-    }
+    for (int j = 0; j < kWin; ++j)
+        if (old[j] + q[j] < old[j]) atomicAdd(&hi[s.i0 + j], 1u);
 }
 
 // One particle's stencil quanta added into (sign +1) or taken out of (-1)
