@@ -384,8 +384,25 @@ template <typename F>
 __device__ __forceinline__ double block_chunk_sum(double* red, int64_t n, F&& elem) {
     const int t = threadIdx.x;
     double s = 0.0;
-    if (t < kTreeLanes)
-        for (int64_t i = t; i < n; i += kTreeLanes) s = s + elem(i);
+    if (t < kTreeLanes) {
+        if (n > 0 && n <= kTreeChunk) {
+            // a lane's (at most 16) values are all loaded before its sum, so
+            // their latencies overlap: past n the element of index n-1 is
+            // read (no branch around the load) and -0.0 taken in its place;
+            // x + (-0.0) is x for every x, signed zeros and NaN included
+            constexpr int K = kTreeChunk / kTreeLanes;
+            double x[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int64_t i = t + (int64_t)k * kTreeLanes;
+                x[k] = elem(i < n ? i : n - 1);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) s = s + (t + (int64_t)k * kTreeLanes < n ? x[k] : -0.0);
+        } else {
+            for (int64_t i = t; i < n; i += kTreeLanes) s = s + elem(i);
+        }
+    }
     return tree256<false>(red, s);
 }
 
@@ -409,8 +426,23 @@ template <typename F>
 __device__ __forceinline__ double block_maxabs(double* red, int64_t n, F&& elem) {
     const int t = threadIdx.x;
     double m = 0.0;
-    if (t < kTreeLanes)
-        for (int64_t i = t; i < n; i += kTreeLanes) m = nanmax(m, fabs(elem(i)));
+    if (t < kTreeLanes) {
+        if (n > 0 && n <= kTreeChunk) {
+            // loads first, as in block_chunk_sum; +0.0 past n leaves the
+            // maximum (>= +0 or NaN) as it is
+            constexpr int K = kTreeChunk / kTreeLanes;
+            double x[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int64_t i = t + (int64_t)k * kTreeLanes;
+                x[k] = elem(i < n ? i : n - 1);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) m = nanmax(m, t + (int64_t)k * kTreeLanes < n ? fabs(x[k]) : 0.0);
+        } else {
+            for (int64_t i = t; i < n; i += kTreeLanes) m = nanmax(m, fabs(elem(i)));
+        }
+    }
     return tree256<true>(red, m);
 }
 
