@@ -183,10 +183,13 @@ k_solve(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
         void* __restrict__ racc, int precision, double* E0, double* E1, float* E32,
         const double2* __restrict__ twp, int k, int32_t* __restrict__ halt, int stall) {
     extern __shared__ double sm[];
-    if (halted(halt, false)) return;
+    // halt, the instance and its state are loaded together before one test
+    // of every early exit (| evaluates each operand, so no load waits behind
+    // a branch of its own; I.N is never 0)
+    const int32_t h = *halt;
     const InstDesc I = insts[blockIdx.x];
     InstState S = st[blockIdx.x];
-    if (TEST && sweep_over(S, k)) return;
+    if ((h >= 1) | (I.N <= 0) | (TEST && sweep_over(S, k))) return;
     const int N = I.N, log2N = I.log2N;
     const int NP = solve_np(N);
     double* re = sm;
@@ -302,13 +305,17 @@ k_diag_final(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
              double* __restrict__ diag, int32_t* __restrict__ sweeps, int64_t cap,
              int scheme, int32_t* __restrict__ halt, int tile_partials) {
     __shared__ double red[4 * kTreeLanes + 8];
-    if (*halt) {
-        // the step did not finish: no diagnostics; from here on the graph is halted
-        if (blockIdx.x == 0 && threadIdx.x == 0) *halt = 2;
-        return;
-    }
+    // halt, the instance and its state load together ahead of the test
+    // (I.N is never 0 and e_cur is 0 or 1: those operands only keep the
+    // loads ahead of the branch)
+    const int32_t h = *halt;
     const InstDesc I = insts[blockIdx.x];
     InstState S = st[blockIdx.x];
+    if ((h != 0) | (I.N <= 0) | (S.e_cur < 0)) {
+        // the step did not finish: no diagnostics; from here on the graph is halted
+        if (h != 0 && blockIdx.x == 0 && threadIdx.x == 0) *halt = 2;
+        return;
+    }
     const double* E = (S.e_cur ? E1 : E0) + I.goff;
     double se, sv, sp;
     if (tile_partials) {

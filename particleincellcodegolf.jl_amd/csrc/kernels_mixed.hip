@@ -177,18 +177,22 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
              const float* __restrict__ E32, u64* __restrict__ racc, int k, const int32_t* __restrict__ halt,
              int last_slot, double* __restrict__ dpart, int prime) {
     extern __shared__ u64 lds[];
-    {
-        const int32_t h = *halt;  // see RunBuffers::halt
-        if (h >= ((MODE == 1 && last_slot) ? 2 : 1)) return;
-    }
+    // the loads the early exits need go out before one test of them all:
+    // halt and the tile together, then the instance and its state (each
+    // scalar-cache round trip costs a launch part of a microsecond; I.N,
+    // never 0, keeps the instance's load ahead of the test)
+    const int32_t h = *halt;  // see RunBuffers::halt
     const TileDesc T = tiles[blockIdx.x];
     const InstDesc I = insts[T.inst];
     InstState S = st[T.inst];
+    // (| rather than ||: every operand is evaluated, so no load waits
+    // behind a branch of its own)
+    const bool over = MODE == 1 && S.last_sweep >= 0 && S.last_sweep < k;
+    if ((h >= ((MODE == 1 && last_slot) ? 2 : 1)) | (I.N <= 0) | over) return;
     if (MODE == 0) {
         if (T.local == 0 && threadIdx.x == 0) st[T.inst].last_sweep = -1;
         S.last_sweep = -1;
     }
-    if (MODE == 1 && S.last_sweep >= 0 && S.last_sweep < k) return;
     const bool last = (MODE == 1) && (S.last_sweep == k);
     const int32_t N = I.N, nmask = I.nmask;
     const int NPAD = pad_cells(N);
