@@ -170,7 +170,7 @@ __device__ __forceinline__ void store_tile_kinetic(i128 q2, i128 q1, u64* red, d
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
 k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
              const TileDesc* __restrict__ tiles, uint32_t* Xq0, uint32_t* Xq1, float* Vh0,
              float* Vh1, float* Vl0, float* Vl1, float* vf0, float* vf1,
@@ -235,67 +235,79 @@ k_mid_push32(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     // over this thread's particles
     i128 q2 = 0, q1 = 0;
     // every lane of a wave runs the same iterations (the deposit reduces
-    // across the wave); lanes past the tile's end carry no particle
+    // across the wave); lanes past the tile's end carry no particle.  The
+    // loop is compiled once per kind of launch, chosen once per block:
+    // 0 the first deposit, 1 sweep 0 (full deposit of the next centres),
+    // 2 a middle sweep (charge changes only), 3 the final sweep.
     const int64_t nit = (T.count + blockDim.x - 1) / blockDim.x;
-    for (int64_t it = 0; it < nit; ++it) {
-        const int64_t p = threadIdx.x + it * blockDim.x;
-        const bool valid = p < T.count;
-        const int64_t g = base + (valid ? p : 0);
-        Stencil32 s;
-        s.i0 = 0;
-        bool dep = false;
-        if (valid) {
-            const uint32_t xq = Xq[g];
-            const float vh = Vh[g];
-            uint32_t cq = 0;
-            uint32_t cdep;  // centre of this particle's deposit
-            dep = true;
-            if (MODE == 1) {
-                const float vk = (k == 0) ? vh : vin[g];
-                cq = advance(xq, vk + vh, cscale);
-                stencil32(cq, shift, s);
-                const float* e = Eld + s.i0;  // slot of array cell i0-6
-                float kick = 0.0f;
-#pragma unroll
-                for (int j = 0; j < kWin; ++j) kick = fmaf(e[j], s.w[j], kick);
-                if (last) {
-                    // V_new = (V_hi + V_lo) + kick, renormalised (two-sum)
-                    const float sum = vh + kick;
-                    const float bb = sum - vh;
-                    const float err = (vh - (sum - bb)) + (kick - bb);
-                    const float t = Vl[g] + err;
-                    const float hi = sum + t;
-                    const float lo = t - (hi - sum);
-                    const uint32_t xn = advance(xq, vk + vh, xscale);
-                    Xo[g] = xn;
-                    Vho[g] = hi;
-                    Vlo[g] = lo;
-                    const double vd = ((double)hi + (double)lo) * I.inv_vstep;  // the sum is exact
-                    q2 += fix_round128(vd * vd, kV2Bits);
-                    q1 += fix_round128(vd, kV1Bits);
-                    dep = prime != 0;
-                    cdep = advance(xn, hi + hi, cscale);  // next step's first centre, v_0 = V
+    auto sweep = [&](auto kind) {
+        constexpr int K = decltype(kind)::value;
+        for (int64_t it = 0; it < nit; ++it) {
+            const int64_t p = threadIdx.x + it * blockDim.x;
+            const bool valid = p < T.count;
+            const int64_t g = base + (valid ? p : 0);
+            Stencil32 s;
+            s.i0 = 0;
+            bool dep = false;
+            if (valid) {
+                const uint32_t xq = Xq[g];
+                const float vh = Vh[g];
+                uint32_t cq = 0;
+                uint32_t cdep;  // centre of this particle's deposit
+                dep = true;
+                if constexpr (K == 0) {
+                    cdep = advance(xq, vh + vh, cscale);
                 } else {
-                    const float vn = vh + kick;
-                    vout[g] = vn;
-                    cdep = advance(xq, vn + vh, cscale);
-                    if (delta) {
-                        if (cdep != cq) {
-                            lane_change<-1>(acc, s);
-                            stencil32(cdep, shift, s);
-                            lane_change<1>(acc, s);
+                    const float vk = (K == 1) ? vh : (K == 2) ? vin[g] : ((k == 0) ? vh : vin[g]);
+                    cq = advance(xq, vk + vh, cscale);
+                    stencil32(cq, shift, s);
+                    const float* e = Eld + s.i0;  // slot of array cell i0-6
+                    float kick = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < kWin; ++j) kick = fmaf(e[j], s.w[j], kick);
+                    if constexpr (K == 3) {
+                        // V_new = (V_hi + V_lo) + kick, renormalised (two-sum)
+                        const float sum = vh + kick;
+                        const float bb = sum - vh;
+                        const float err = (vh - (sum - bb)) + (kick - bb);
+                        const float t = Vl[g] + err;
+                        const float hi = sum + t;
+                        const float lo = t - (hi - sum);
+                        const uint32_t xn = advance(xq, vk + vh, xscale);
+                        Xo[g] = xn;
+                        Vho[g] = hi;
+                        Vlo[g] = lo;
+                        const double vd = ((double)hi + (double)lo) * I.inv_vstep;  // the sum is exact
+                        q2 += fix_round128(vd * vd, kV2Bits);
+                        q1 += fix_round128(vd, kV1Bits);
+                        dep = prime != 0;
+                        cdep = advance(xn, hi + hi, cscale);  // next step's first centre, v_0 = V
+                    } else {
+                        const float vn = vh + kick;
+                        vout[g] = vn;
+                        cdep = advance(xq, vn + vh, cscale);
+                        if constexpr (K == 2) {
+                            if (cdep != cq) {
+                                lane_change<-1>(acc, s);
+                                stencil32(cdep, shift, s);
+                                lane_change<1>(acc, s);
+                            }
+                            dep = false;
                         }
-                        dep = false;
                     }
                 }
-            } else {
-                cdep = advance(xq, vh + vh, cscale);
+                // once the Picard iterate settles the next centre repeats bit for bit
+                if (dep && (K == 0 || cdep != cq)) stencil32(cdep, shift, s);
             }
-            // once the Picard iterate settles the next centre repeats bit for bit
-            if (dep && (MODE == 0 || cdep != cq)) stencil32(cdep, shift, s);
+            if constexpr (K != 2) {
+                if (K != 3 || prime) lane_deposit(acc_lo, acc_hi, s, dep);
+            }
         }
-        if (deposit && !delta) lane_deposit(acc_lo, acc_hi, s, dep);
-    }
+    };
+    if (MODE == 0) sweep(std::integral_constant<int, 0>{});
+    else if (last) sweep(std::integral_constant<int, 3>{});
+    else if (delta) sweep(std::integral_constant<int, 2>{});
+    else sweep(std::integral_constant<int, 1>{});
     if (last) {
         store_tile_kinetic(q2, q1, lds + NPAD + (NPAD + 1) / 2, dpart);
         if (!prime) return;
