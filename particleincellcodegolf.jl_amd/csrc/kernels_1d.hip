@@ -70,33 +70,40 @@ k_mid_push64(const InstDesc* __restrict__ insts, InstState* __restrict__ st,
     const int32_t nmask = I.nmask;
     const int q = I.q;
     const int64_t base = I.poff + T.start;
-    for (int64_t p = threadIdx.x; p < T.count; p += blockDim.x) {
-        const int64_t g = base + p;
-        const double Xp = X[g], Vp = V[g];
-        double vn = Vp;
-        if (MODE == 1) {
-            const double vk = (k == 0) ? Vp : vin[g];
-            const double x = Xp + ((vk + Vp) / 2.0) * dt;
-            const double c = (x + Xp) / 2.0;
-            const double Sg = gather_gauss<R>(Eld, c, Nd, nmask);
-            vn = Vp + Sg * dt;
-            if (last) {
-                const double Xn = jl_mod1f(x);
-                Xo[g] = Xn;
-                Vo[g] = vn;
-                if (prime) {
-                    // next step: X = Xn, V = vn, x_0 = X + ((V + V)/2)*dt
-                    const double x0 = Xn + ((vn + vn) / 2.0) * dt;
-                    deposit_gauss<R>(acc, N, (x0 + Xn) / 2.0, Nd, nmask, w, q);
+    // the particle loop is compiled once for the final sweep and once for
+    // the others, and the block picks one
+    auto sweep = [&](auto final_sweep) {
+        constexpr bool LAST = decltype(final_sweep)::value;
+        for (int64_t p = threadIdx.x; p < T.count; p += blockDim.x) {
+            const int64_t g = base + p;
+            const double Xp = X[g], Vp = V[g];
+            double vn = Vp;
+            if (MODE == 1) {
+                const double vk = (k == 0) ? Vp : vin[g];
+                const double x = Xp + ((vk + Vp) / 2.0) * dt;
+                const double c = (x + Xp) / 2.0;
+                const double Sg = gather_gauss<R>(Eld, c, Nd, nmask);
+                vn = Vp + Sg * dt;
+                if constexpr (LAST) {
+                    const double Xn = jl_mod1f(x);
+                    Xo[g] = Xn;
+                    Vo[g] = vn;
+                    if (prime) {
+                        // next step: X = Xn, V = vn, x_0 = X + ((V + V)/2)*dt
+                        const double x0 = Xn + ((vn + vn) / 2.0) * dt;
+                        deposit_gauss<R>(acc, N, (x0 + Xn) / 2.0, Nd, nmask, w, q);
+                    }
+                    continue;
                 }
-                continue;
+                vout[g] = vn;
             }
-            vout[g] = vn;
+            const double x2 = Xp + ((vn + Vp) / 2.0) * dt;
+            const double c2 = (x2 + Xp) / 2.0;
+            deposit_gauss<R>(acc, N, c2, Nd, nmask, w, q);
         }
-        const double x2 = Xp + ((vn + Vp) / 2.0) * dt;
-        const double c2 = (x2 + Xp) / 2.0;
-        deposit_gauss<R>(acc, N, c2, Nd, nmask, w, q);
-    }
+    };
+    if (last) sweep(std::true_type{});
+    else sweep(std::false_type{});
     if (!deposit) return;
     __syncthreads();
     flush_limbs(acc, N, racc + 3 * I.red0);
