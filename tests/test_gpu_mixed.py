@@ -1,7 +1,7 @@
 """Mixed precision (FP32 push and shape weights, exact integer charge sums,
 FP64 field solve / convergence test / diagnostics) against the FP64 path.
 
-Bars (from the north-star target): field-energy relative error <= 1e-6 over
+Bars (the accuracy the mixed form is held to): field-energy relative error <= 1e-6 over
 the run and the fitted growth rate within 1% of FP64; results reproducible
 bit for bit from run to run."""
 import numpy as np
@@ -45,7 +45,7 @@ def test_mixed_is_deterministic(pg):
 
 
 def test_mixed_field_energy_and_momentum_vs_fp64_full_size(pg):
-    """The BASELINE workload at full size: N=4096, P=2^24, 300 steps."""
+    """The scaled two-stream workload at full size: N=4096, P=2^24, 300 steps."""
     from picgolf import configs
     n = 300
     cm, x, v, _ = configs.scaled_two_stream(precision="mixed")

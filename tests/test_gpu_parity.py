@@ -188,7 +188,7 @@ def test_batch_equals_independent_runs(pg, orc):
 
 
 def test_full_size_scaled_fp64_properties(pg):
-    """BASELINE size (N=4096, P=2^24): momentum conserved, run-to-run bitwise
+    """The scaled two-stream size (N=4096, P=2^24): momentum conserved, run-to-run bitwise
     reproducible (no float atomics anywhere)."""
     from picgolf import configs
     cfg, x, v, _ = configs.scaled_two_stream(precision="fp64")
