@@ -23,6 +23,13 @@ SOURCES = ["kernels_1d.hip", "kernels_mixed.hip", "kernels_small.hip", "noalloc.
            "capi_em1d.cpp"]
 
 
+# Per-unit compiler options.  The mixed-precision kernels schedule better
+# with LLVM's iterative ILP machine scheduler: the 1D mixed push averaged
+# 93.5 against 95.2 us over five alternating runs (instruction order only;
+# the results are the same bit for bit).
+EXTRA_FLAGS = {"kernels_mixed.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]}
+
+
 def hipcc():
     for cand in (os.environ.get("HIPCC"), shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
         if cand and os.path.exists(cand):
@@ -50,6 +57,7 @@ def build(verbose=False):
         cmd = [hipcc(), *common, "-c", path, "-o", obj]
         # hipcc compiles .cpp as HIP too: every unit gets the same target flags
         cmd[1:1] = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+        cmd[1:1] = EXTRA_FLAGS.get(src, [])
         cmds.append(cmd)
         objs.append(obj)
     # the units are independent: compile them side by side
