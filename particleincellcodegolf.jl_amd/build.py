@@ -18,15 +18,16 @@ LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libpicgolf.so")
 ARCH = os.environ.get("PIC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels_1d.hip", "kernels_mixed.hip", "kernels_small.hip", "noalloc.hip", "kernels_2d.hip", "kernels_1d2v.hip", "kernels_2d3v.hip", "kernels_2d3v_mixed.hip",
+SOURCES = ["kernels_1d.hip", "kernels_mixed.hip", "kernels_mixed_aux.hip", "kernels_small.hip", "noalloc.hip", "kernels_2d.hip", "kernels_1d2v.hip", "kernels_2d3v.hip", "kernels_2d3v_mixed.hip",
            "kernels_2d3v_lg.hip", "kernels_em1d.hip", "capi.cpp", "capi2d.cpp", "capi1d2v.cpp", "capi2d3v.cpp",
            "capi_em1d.cpp"]
 
 
-# Per-unit compiler options.  The mixed-precision kernels schedule better
-# with LLVM's iterative ILP machine scheduler: the 1D mixed push averaged
-# 93.5 against 95.2 us over five alternating runs (instruction order only;
-# the results are the same bit for bit).
+# Per-unit compiler options.  The mixed midpoint push schedules better with
+# LLVM's iterative ILP machine scheduler: it averaged 93.5 against 95.2 us
+# over five alternating runs (instruction order only; the results are the
+# same bit for bit).  The mixed Simpson push ran about 5% slower under it,
+# so it lives in a unit of its own (kernels_mixed_aux.hip) without it.
 EXTRA_FLAGS = {"kernels_mixed.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]}
 
 
