@@ -68,7 +68,7 @@ void launch12_op_gather(const Buffers12& b, const double* E, const double* x, in
 void launch12_op_boris(double* vx, double* vy, const double* Ej, int64_t n, double B, double dt, double q_m,
                        hipStream_t s);
 int configure_kernels_1d2v();
-// kernels_mixed.hip: the mixed forms of the deposit, push and diagnostics
+// kernels_mixed_aux.hip: the mixed forms of the deposit, push and diagnostics
 void launch12_deposit32(const Buffers12& b, hipStream_t s);
 void launch12_push32(const Buffers12& b, hipStream_t s);
 void launch12_diag32(const Buffers12& b, hipStream_t s);

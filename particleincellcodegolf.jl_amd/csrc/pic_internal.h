@@ -131,7 +131,7 @@ int simpson_push_lds_bytes(int N);
 void launch_simp_first(const RunBuffers& b, hipStream_t s);
 void launch_simp_solve(const RunBuffers& b, int k, bool stall, hipStream_t s);
 void launch_simp_push(const RunBuffers& b, int k, bool last_slot, hipStream_t s);
-// kernels_mixed.hip: the mixed forms (called by the launchers above)
+// kernels_mixed_aux.hip: the mixed forms (called by the launchers above)
 int simpson32_push_lds_bytes(int N);
 void launch_simp32_first(const RunBuffers& b, hipStream_t s);
 void launch_simp32_start(const RunBuffers& b, hipStream_t s);
